@@ -1,0 +1,137 @@
+/*
+ * usct.h -- C ABI of the MI355X acoustic FD solver and its exact discrete-adjoint
+ * FWI gradient (libusct.so, built for gfx950).
+ *
+ * All array arguments marked "device" are device pointers in HBM (any
+ * allocator: torch, hipMalloc, ...).  Index lists and dims are host arrays.
+ * Fields are FP32; models, wavelets, gradients and misfits are FP64 at the
+ * boundary, like the reference.  Every entry point returns 0 on success or a
+ * usct_status code, with a message in usct_last_error() (thread-local).
+ * Work is ordered after prior work on `stream` (a hipStream_t, NULL = the null
+ * stream) and later work on `stream` sees the results.
+ *
+ * Reference interfaces these replace (paths relative to the reference root):
+ *   usct_forward_fmc_direct         software/core/libuap.hpp:109-116 (uap::forward_fmc)
+ *                                   software/python/_uap.cpp:25-37 (_uap.forward_fmc)
+ *   usct_misfit_and_gradient_direct software/core/libuap.hpp:169-224
+ *                                   (uap::misfit_and_gradient),
+ *                                   software/python/_uap.cpp:41-55
+ *   usct_plan_* / usct_forward_fmc / usct_misfit / usct_misfit_and_gradient
+ *                                   the same three operations (plus misfit-only,
+ *                                   software/python/uap.py:56-67) split into a
+ *                                   reusable plan, as the reference's CuPy backend
+ *                                   re-uses its kernels across calls
+ *                                   (software/python/uap_gpu.py:166-233).
+ */
+#ifndef USCT_H
+#define USCT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    USCT_OK = 0,
+    USCT_EINVAL = 1,   /* bad argument: shape, index out of range, unknown misfit */
+    USCT_ENOMEM = 2,   /* workspace does not fit the memory budget / HBM */
+    USCT_EHIP = 3,     /* HIP runtime error */
+    USCT_ENOTSUP = 4   /* valid request this build does not implement */
+} usct_status;
+
+/* Execution method of a plan. */
+enum {
+    USCT_METHOD_AUTO = 0,    /* batched */
+    USCT_METHOD_PLAIN = 1,   /* one launch per shot per time step, full history */
+    USCT_METHOD_BATCHED = 2  /* all shots per launch, checkpointed, hipGraph-replayed */
+};
+
+/* Plan flags. */
+enum {
+    USCT_FLAG_SPONGE = 1,    /* calls pass a multiplicative sponge mask (2D only) */
+    USCT_FLAG_AUTOTUNE = 2,  /* time candidate tile shapes at creation, keep the best */
+    USCT_FLAG_NO_GRAPH = 4   /* batched: launch eagerly instead of replaying a hipGraph */
+};
+
+/* Misfit codes (software/python/uap.py:41). */
+enum { USCT_MISFIT_L2 = 0, USCT_MISFIT_GCN = 1 };
+
+typedef struct usct_plan usct_plan;
+
+typedef struct {
+    int method;            /* USCT_METHOD_PLAIN or USCT_METHOD_BATCHED */
+    int lanes_per_group;   /* shots per wavefront lane group (16, 32, 64); 1 for plain */
+    int tile_x, tile_y;    /* cells per lane along x (and y in 3D) */
+    int rows_per_wave;     /* y rows (2D) or z planes (3D) marched by one wavefront */
+    int shots_padded;      /* shot count rounded up to a lane-group multiple */
+    int shot_passes;       /* batched: passes over shot subsets that fit the budget */
+    int shots_per_pass;    /* batched: padded shots per pass */
+    int ckpt_every;        /* checkpoint interval K (>= nt-1: whole history stored) */
+    int n_windows;         /* recompute windows per pass */
+    int64_t workspace_bytes;  /* device memory held by the plan */
+} usct_plan_info;
+
+/*
+ * Create a plan for a grid, a time axis, transmit cells and receiver cells.
+ *   dims      ndim entries, row-major (2D: ny, nx; 3D: nz, ny, nx)
+ *   tx_lin    n_tx linear source cells (one shot each; uap.py:25-30)
+ *   rec_lin   n_rec linear receiver cells (duplicates allowed and accumulate)
+ *   ckpt_every  0 = choose from the memory budget
+ *   mem_budget  bytes of HBM the plan may hold; 0 = 85% of free memory
+ */
+int usct_plan_create(usct_plan **plan, int ndim, const int *dims, double h, double dt,
+                     int nt, const int64_t *tx_lin, int n_tx, const int64_t *rec_lin,
+                     int n_rec, int method, int ckpt_every, int64_t mem_budget, int flags);
+int usct_plan_destroy(usct_plan *plan);
+int usct_plan_get_info(const usct_plan *plan, usct_plan_info *info);
+
+/*
+ * Full-matrix-capture forward (fwi.py:102-119).
+ *   m        device, N doubles, squared slowness
+ *   wavelet  device, nt doubles
+ *   sponge   device, N doubles or NULL (plan must have USCT_FLAG_SPONGE if not NULL)
+ *   data     device, n_tx * nt * n_rec floats (row 0 of each shot is zero)
+ */
+int usct_forward_fmc(usct_plan *plan, const double *m, const double *wavelet,
+                     const double *sponge, float *data, void *stream);
+
+/* Misfit only (fwi.py:246-263).  dobs: device n_tx*nt*n_rec floats; J: device double. */
+int usct_misfit(usct_plan *plan, const double *m, const double *wavelet,
+                const double *sponge, const float *dobs, int misfit_type, double *J,
+                void *stream);
+
+/* Misfit and exact discrete-adjoint gradient w.r.t. m (fwi.py:266-332).
+ * grad: device N doubles; J: device double. */
+int usct_misfit_and_gradient(usct_plan *plan, const double *m, const double *wavelet,
+                             const double *sponge, const float *dobs, int misfit_type,
+                             double *J, double *grad, void *stream);
+
+/*
+ * Time one gradient launch by launch with HIP events on the plan's stream.
+ * fwd_ms / adj_ms: mean duration of one forward-step / adjoint-step kernel;
+ * n_fwd / n_adj: launches timed (recompute launches count as forward steps).
+ */
+int usct_profile_gradient(usct_plan *plan, const double *m, const double *wavelet,
+                          const double *sponge, const float *dobs, double *fwd_ms,
+                          double *adj_ms, int *n_fwd, int *n_adj, void *stream);
+
+/* Stateless forms with the argument order of uap::forward_fmc and
+ * uap::misfit_and_gradient (device pointers as above, J returned on the host). */
+int usct_forward_fmc_direct(const double *m, const int *dims, int ndim, double h, double dt,
+                            int nt, const int64_t *tx_lin, int n_tx, const int64_t *rec_lin,
+                            int n_rec, const double *wavelet, float *data, int method,
+                            void *stream);
+int usct_misfit_and_gradient_direct(const double *m, const int *dims, int ndim, double h,
+                                    double dt, int nt, const int64_t *tx_lin, int n_tx,
+                                    const int64_t *rec_lin, int n_rec, const double *wavelet,
+                                    const float *dobs, double *grad, double *J_host,
+                                    int misfit_type, int method, void *stream);
+
+const char *usct_last_error(void);
+int usct_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* USCT_H */

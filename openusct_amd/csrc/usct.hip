@@ -1,0 +1,1050 @@
+// usct.hip -- plans, orchestration and C ABI of libusct.so (gfx950).
+//
+// A plan fixes the grid, time axis, transmit and receiver cells and allocates
+// the device workspace.  Two execution methods:
+//
+//  plain    one shot at a time, one kernel launch per time step, the full
+//           forward history kept in HBM (what a direct port of the reference's
+//           per-shot loop does: software/python/uap_gpu.py:120-140).
+//  batched  every shot of a pass in each launch (field layout cell-major,
+//           shot-minor).  Forward levels are checkpointed every K steps and
+//           re-computed window by window during the adjoint sweep; when the
+//           whole history fits the memory budget K covers the whole time axis
+//           and nothing is re-computed.  The launch sequence of a pass is
+//           captured once into a hipGraph and replayed on every call.
+//
+// Level bookkeeping of the batched gradient (levels n = -1..nt-1, p^-1 = p^0 = 0):
+//   window s (s = 0..NS-1) serves adjoint steps k in [sK+1, min(sK+K, nt-1)]
+//   and holds levels sK-1 .. sK+K in slots W[0..K+1] (W[i] = level sK-1+i).
+//   Checkpoint pair of window s (1 <= s <= NS-2): levels (sK-1, sK).
+//   The forward pass writes the checkpoint levels straight into their slots
+//   and the last window's levels straight into W, so checkpointing adds no
+//   copies; window s < NS-1 is re-computed from its pair before its adjoint.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/usct.h"
+#include "usct_kernels.hpp"
+
+using namespace usct;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPX(expr)                                                                       \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(USCT_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+#define TRY(expr)              \
+    do {                       \
+        int rc_ = (expr);      \
+        if (rc_) return rc_;   \
+    } while (0)
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2 };
+
+struct Profile {
+    std::vector<hipEvent_t> fwd_ev, adj_ev;  // start/stop pairs
+};
+
+}  // namespace
+
+struct usct_plan {
+    // problem
+    int ndim = 2, nz = 1, ny = 0, nx = 0;
+    int64_t N = 0;
+    double h = 0, dt = 0;
+    int nt = 0, n_tx = 0, n_rec = 0;
+    std::vector<int64_t> tx, rec;
+    int method = USCT_METHOD_BATCHED, flags = 0;
+    bool sponge = false;
+    // kernel configuration
+    int LB = 64, CX = 8, CY = 1, RY = 32;
+    int Bp = 1;        // padded shots per pass
+    int passes = 1;
+    int K = 0, NS = 1;  // checkpoint interval and number of windows
+    int64_t budget = 0;
+    // device memory
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    float2 *sp2 = nullptr;
+    float *spg = nullptr;  // sponge (FP32)
+    int *rec_head = nullptr, *rec_next = nullptr;
+    int *src_all = nullptr;  // [passes][Bp] source cells (-1 = padding)
+    int *src_cell = nullptr; // [Bp] cells of the current pass
+    float *sinc = nullptr;   // [nt][Bp]
+    float *Z = nullptr;      // zero field
+    float *F[3] = {nullptr, nullptr, nullptr};
+    std::vector<float *> Wslot, CK;
+    float *recb = nullptr, *dobsT = nullptr, *res = nullptr;
+    float2 *part = nullptr;
+    int npart = 1024;
+    float *g_hi = nullptr, *g_lo = nullptr;
+    double *J_acc = nullptr;
+    double *gstats = nullptr;
+    unsigned char *tmask = nullptr;  // [passes][n_rec*Bp]
+    unsigned char *tmask_cur = nullptr;
+    // execution
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    std::map<int, hipGraphExec_t> graphs;  // key: mode*4 + misfit
+
+    int64_t field_elems() const { return N * (int64_t)Bp; }
+    int ng() const { return Bp / LB; }
+};
+
+namespace {
+
+int dalloc(usct_plan *P, void **p, int64_t bytes) {
+    if (bytes <= 0) bytes = 16;
+    hipError_t e = hipMalloc(p, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(USCT_ENOMEM, "hipMalloc(%lld bytes) failed: %s", (long long)bytes,
+                    hipGetErrorString(e));
+    }
+    P->allocs.push_back(*p);
+    P->bytes += bytes;
+    return 0;
+}
+
+template <class T>
+int dalloc_t(usct_plan *P, T **p, int64_t count) {
+    return dalloc(P, (void **)p, count * (int64_t)sizeof(T));
+}
+
+void free_plan_memory(usct_plan *P) {
+    for (auto &kv : P->graphs) (void)hipGraphExecDestroy(kv.second);
+    P->graphs.clear();
+    for (void *p : P->allocs) (void)hipFree(p);
+    P->allocs.clear();
+    P->Wslot.clear();
+    P->CK.clear();
+    P->bytes = 0;
+}
+
+inline unsigned grid_for(int64_t n, int block = 256) {
+    return (unsigned)std::max<int64_t>(1, cdiv(n, block));
+}
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch (template instantiation tables)
+// ---------------------------------------------------------------------------
+struct Tile {
+    int nxs, nys, nyc;  // 2D: nxs, nyc; 3D: nxs, nys, nyc = z chunks
+    int64_t waves;
+};
+
+Tile tile_of(const usct_plan *P) {
+    Tile t{};
+    const int W = kWave / P->LB;
+    t.nxs = (int)cdiv(P->nx, (int64_t)W * P->CX);
+    if (P->ndim == 2) {
+        t.nys = 1;
+        t.nyc = (int)cdiv(P->ny, P->RY);
+        t.waves = (int64_t)P->ng() * t.nxs * t.nyc;
+    } else {
+        t.nys = (int)cdiv(P->ny, P->CY);
+        t.nyc = (int)cdiv(P->nz, P->RY);
+        t.waves = (int64_t)P->ng() * t.nxs * t.nys * t.nyc;
+    }
+    return t;
+}
+
+template <int LB, bool SP, bool REC>
+void fwd2d_lb(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
+              const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
+    hipLaunchKernelGGL((k_fwd2d<LB, 8, SP, REC>), g, dim3(kBlock), 0, s, pc, pp, pn, P->sp2,
+                       P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, rec, P->ny, P->nx,
+                       P->Bp, P->RY, t.nxs, t.nyc, P->ng());
+}
+
+template <bool SP, bool REC>
+void fwd2d_sel(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
+               const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
+    switch (P->LB) {
+        case 16: fwd2d_lb<16, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+        case 32: fwd2d_lb<32, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+        default: fwd2d_lb<64, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+    }
+}
+
+template <int LB, int CX, int CY, bool REC>
+void fwd3d_t(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
+             const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
+    hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC>), g, dim3(kBlock), 0, s, pc, pp, pn, P->sp2,
+                       P->src_cell, sinc, P->rec_head, P->rec_next, rec, P->nz, P->ny, P->nx,
+                       P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
+}
+
+template <int LB, bool REC>
+void fwd3d_lb(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
+              const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
+    if (P->CX == 4 && P->CY == 4) fwd3d_t<LB, 4, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, t);
+    else if (P->CX == 8 && P->CY == 2) fwd3d_t<LB, 8, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, t);
+    else fwd3d_t<LB, 4, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, t);
+}
+
+template <bool REC>
+void fwd3d_sel(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
+               const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
+    switch (P->LB) {
+        case 16: fwd3d_lb<16, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+        case 32: fwd3d_lb<32, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+        default: fwd3d_lb<64, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+    }
+}
+
+void launch_fwd_batched(const usct_plan *P, hipStream_t s, const float *pc, const float *pp,
+                        float *pn, int n, bool record) {
+    const Tile t = tile_of(P);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+    const float *sinc = P->sinc + (int64_t)n * P->Bp;
+    float *rec = P->recb + (int64_t)n * P->n_rec * P->Bp;
+    if (P->ndim == 2) {
+        if (P->sponge) {
+            if (record) fwd2d_sel<true, true>(g, s, pc, pp, pn, P, sinc, rec, t);
+            else fwd2d_sel<true, false>(g, s, pc, pp, pn, P, sinc, rec, t);
+        } else {
+            if (record) fwd2d_sel<false, true>(g, s, pc, pp, pn, P, sinc, rec, t);
+            else fwd2d_sel<false, false>(g, s, pc, pp, pn, P, sinc, rec, t);
+        }
+    } else {
+        if (record) fwd3d_sel<true>(g, s, pc, pp, pn, P, sinc, rec, t);
+        else fwd3d_sel<false>(g, s, pc, pp, pn, P, sinc, rec, t);
+    }
+}
+
+template <int LB, bool SP>
+void adj2d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
+              const float *pa, const float *pb, const float *pc2, const usct_plan *P,
+              const float *sinc, const float *res, const Tile &t) {
+    hipLaunchKernelGGL((k_adj2d<LB, 8, SP>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, pb, pc2,
+                       P->sp2, P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, res, P->g_hi,
+                       P->g_lo, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nyc, P->ng());
+}
+
+template <bool SP>
+void adj2d_sel(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
+               const float *pa, const float *pb, const float *pc2, const usct_plan *P,
+               const float *sinc, const float *res, const Tile &t) {
+    switch (P->LB) {
+        case 16: adj2d_lb<16, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
+        case 32: adj2d_lb<32, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
+        default: adj2d_lb<64, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
+    }
+}
+
+template <int LB, int CX, int CY>
+void adj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
+             const float *pa, const usct_plan *P, const float *sinc, const float *res,
+             const Tile &t) {
+    hipLaunchKernelGGL((k_adj3d<LB, CX, CY>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, P->sp2,
+                       P->src_cell, sinc, P->rec_head, P->rec_next, res, P->g_hi, P->g_lo, P->nz,
+                       P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
+}
+
+template <int LB>
+void adj3d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
+              const float *pa, const usct_plan *P, const float *sinc, const float *res,
+              const Tile &t) {
+    if (P->CX == 4 && P->CY == 4) adj3d_t<LB, 4, 4>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else if (P->CX == 8 && P->CY == 2) adj3d_t<LB, 8, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else adj3d_t<LB, 4, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+}
+
+void launch_adj_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
+                        float *lk, const float *pa, const float *pb, const float *pc2, int k) {
+    const Tile t = tile_of(P);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+    const float *sinc = P->sinc + (int64_t)k * P->Bp;
+    const float *res = P->res + (int64_t)k * P->n_rec * P->Bp;
+    if (P->ndim == 2) {
+        if (P->sponge) adj2d_sel<true>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
+        else adj2d_sel<false>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
+    } else {
+        switch (P->LB) {
+            case 16: adj3d_lb<16>(g, s, l1, l2, lk, pa, P, sinc, res, t); break;
+            case 32: adj3d_lb<32>(g, s, l1, l2, lk, pa, P, sinc, res, t); break;
+            default: adj3d_lb<64>(g, s, l1, l2, lk, pa, P, sinc, res, t); break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Level pointers
+// ---------------------------------------------------------------------------
+float *fwd_level(const usct_plan *P, int n, bool store) {
+    if (n <= 0) return P->Z;
+    if (store) {
+        const int base_last = (P->NS - 1) * P->K;
+        if (n >= base_last - 1) return P->Wslot[n - base_last + 1];
+        for (int s = 1; s <= P->NS - 2; ++s) {
+            if (n == s * P->K - 1) return P->CK[2 * (s - 1)];
+            if (n == s * P->K) return P->CK[2 * (s - 1) + 1];
+        }
+    }
+    return P->F[n % 3];
+}
+
+float *win_level(const usct_plan *P, int s, int level) {
+    if (level <= 0) return P->Z;
+    const int base = s * P->K;
+    const int i = level - base + 1;
+    if (s == P->NS - 1 || i >= 2) return P->Wslot[i];
+    return P->CK[2 * (s - 1) + i];
+}
+
+float *lam_level(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->F[k % 3]; }
+
+void ev_start(Profile *pr, std::vector<hipEvent_t> &v, hipStream_t s) {
+    if (!pr) return;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, s);
+    v.push_back(a);
+    v.push_back(b);
+}
+void ev_stop(Profile *pr, std::vector<hipEvent_t> &v, hipStream_t s) {
+    if (!pr) return;
+    (void)hipEventRecord(v.back(), s);
+}
+
+// Residual and misfit of the traces in recb against dobsT for the current pass.
+void launch_residual(usct_plan *P, hipStream_t s, int misfit, bool want_res) {
+    const int64_t ntr = (int64_t)P->n_rec * P->Bp;
+    const int64_t count = (int64_t)P->nt * ntr;
+    if (misfit == USCT_MISFIT_L2) {
+        const int nb = (int)std::min<int64_t>(P->npart, grid_for(count));
+        hipLaunchKernelGGL(k_residual_l2, dim3(nb), dim3(kBlock), 0, s, P->recb, P->dobsT,
+                           want_res ? P->res : nullptr, count, P->part);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, P->part, nb, 0.5,
+                           P->J_acc, 1);
+    } else {
+        hipLaunchKernelGGL(k_gcn_stats, dim3(grid_for(ntr)), dim3(256), 0, s, P->recb, P->dobsT,
+                           P->gstats, P->nt, ntr);
+        hipLaunchKernelGGL(k_gcn_misfit, dim3(1), dim3(64), 0, s, P->gstats, P->tmask_cur, ntr,
+                           P->J_acc, 1);
+        if (want_res)
+            hipLaunchKernelGGL(k_gcn_residual, dim3(grid_for(count)), dim3(256), 0, s, P->recb,
+                               P->dobsT, P->gstats, P->tmask_cur, P->res, P->nt, ntr);
+    }
+}
+
+// The launch sequence of one batched pass (captured into a hipGraph).
+void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile *pr) {
+    const bool store = mode == MODE_GRAD;
+    const int nt = P->nt;
+    // rec row 0 = 0 (fwi.py:58: rec starts as zeros, level 0 is never written)
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
+                       P->recb, (int64_t)P->n_rec * P->Bp, 0.0f);
+    for (int n = 1; n < nt; ++n) {
+        if (pr) ev_start(pr, pr->fwd_ev, s);
+        launch_fwd_batched(P, s, fwd_level(P, n - 1, store), fwd_level(P, n - 2, store),
+                           fwd_level(P, n, store), n, true);
+        if (pr) ev_stop(pr, pr->fwd_ev, s);
+    }
+    if (mode == MODE_FWD) return;
+    launch_residual(P, s, misfit, store);
+    if (!store) return;
+    for (int w = P->NS - 1; w >= 0; --w) {
+        const int base = w * P->K;
+        const int khi = std::min(base + P->K, nt - 1);
+        if (w < P->NS - 1) {
+            for (int n = base + 1; n <= khi; ++n) {
+                if (pr) ev_start(pr, pr->fwd_ev, s);
+                launch_fwd_batched(P, s, win_level(P, w, n - 1), win_level(P, w, n - 2),
+                                   win_level(P, w, n), n, false);
+                if (pr) ev_stop(pr, pr->fwd_ev, s);
+            }
+        }
+        for (int k = khi; k >= base + 1; --k) {
+            const float *pa, *pb = nullptr, *pc2 = nullptr;
+            if (P->sponge) {
+                pa = win_level(P, w, k);
+                pb = win_level(P, w, k - 1);
+                pc2 = win_level(P, w, k - 2);
+            } else {
+                pa = win_level(P, w, k - 1);
+            }
+            if (pr) ev_start(pr, pr->adj_ev, s);
+            launch_adj_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2), lam_level(P, k),
+                               pa, pb, pc2, k);
+            if (pr) ev_stop(pr, pr->adj_ev, s);
+        }
+    }
+}
+
+int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(USCT_EHIP, "kernel launch failed: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int run_pass_sequence(usct_plan *P, int mode, int misfit, Profile *pr) {
+    const bool use_graph = !(P->flags & USCT_FLAG_NO_GRAPH) && pr == nullptr;
+    if (!use_graph) {
+        batched_sequence(P, P->stream, mode, misfit, pr);
+        return check_launch();
+    }
+    const int key = mode * 4 + misfit;
+    auto it = P->graphs.find(key);
+    if (it == P->graphs.end()) {
+        hipGraph_t graph;
+        HIPX(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+        batched_sequence(P, P->stream, mode, misfit, nullptr);
+        hipError_t le = hipGetLastError();
+        HIPX(hipStreamEndCapture(P->stream, &graph));
+        if (le != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            return fail(USCT_EHIP, "launch during capture failed: %s", hipGetErrorString(le));
+        }
+        hipGraphExec_t exec;
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) return fail(USCT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        it = P->graphs.emplace(key, exec).first;
+    }
+    HIPX(hipGraphLaunch(it->second, P->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched driver
+// ---------------------------------------------------------------------------
+int run_batched(usct_plan *P, int mode, int misfit, const double *m, const double *w,
+                const double *sponge, const float *dobs, float *data, double *J, double *grad,
+                Profile *pr) {
+    hipStream_t s = P->stream;
+    const int64_t N = P->N;
+    hipLaunchKernelGGL(k_setup_s, dim3(grid_for(N)), dim3(256), 0, s, m, P->sp2, N,
+                       P->dt * P->dt, 1.0 / (P->h * P->h));
+    if (P->sponge)  // FP64 mask -> FP32 working copy
+        hipLaunchKernelGGL(k_d2f, dim3(grid_for(N)), dim3(256), 0, s, sponge, P->spg, N);
+    if (J) HIPX(hipMemsetAsync(P->J_acc, 0, sizeof(double), s));
+    if (grad) {
+        HIPX(hipMemsetAsync(P->g_hi, 0, sizeof(float) * N * P->ng(), s));
+        HIPX(hipMemsetAsync(P->g_lo, 0, sizeof(float) * N * P->ng(), s));
+    }
+    const int64_t ntr = (int64_t)P->n_rec * P->Bp;
+    for (int p = 0; p < P->passes; ++p) {
+        const int b0 = p * P->Bp;
+        const int nb = std::min(P->Bp, P->n_tx - b0);
+        HIPX(hipMemcpyAsync(P->src_cell, P->src_all + (int64_t)p * P->Bp, sizeof(int) * P->Bp,
+                            hipMemcpyDeviceToDevice, s));
+        if (misfit == USCT_MISFIT_GCN)
+            HIPX(hipMemcpyAsync(P->tmask_cur, P->tmask + p * ntr, ntr, hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_setup_sinc, dim3(grid_for((int64_t)P->nt * P->Bp)), dim3(256), 0, s,
+                           m, w, P->src_cell, P->sinc, P->nt, P->Bp, P->dt * P->dt);
+        if (dobs)
+            hipLaunchKernelGGL(k_dobs_to_batched, dim3(grid_for((int64_t)P->nt * ntr)), dim3(256),
+                               0, s, dobs, P->dobsT, P->nt, P->n_rec, P->Bp, b0, nb);
+        TRY(check_launch());
+        TRY(run_pass_sequence(P, mode, misfit, pr));
+        if (data)
+            hipLaunchKernelGGL(k_batched_to_data, dim3(grid_for((int64_t)nb * P->nt * P->n_rec)),
+                               dim3(256), 0, s, P->recb, data, P->nt, P->n_rec, P->Bp, b0, nb);
+    }
+    if (grad)
+        hipLaunchKernelGGL(k_finalize_grad, dim3(grid_for(N)), dim3(256), 0, s, P->g_hi, P->g_lo,
+                           P->ng(), N, m, grad, 0);
+    if (J) HIPX(hipMemcpyAsync(J, P->J_acc, sizeof(double), hipMemcpyDeviceToDevice, s));
+    return check_launch();
+}
+
+// ---------------------------------------------------------------------------
+// Plain driver: one shot at a time, one launch per time step, full history.
+// ---------------------------------------------------------------------------
+int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double *w,
+              const double *sponge, const float *dobs, float *data, double *J, double *grad,
+              Profile *pr) {
+    hipStream_t s = P->stream;
+    const int64_t N = P->N;
+    const unsigned gN = grid_for(N);
+    hipLaunchKernelGGL(k_setup_s, dim3(gN), dim3(256), 0, s, m, P->sp2, N, P->dt * P->dt,
+                       1.0 / (P->h * P->h));
+    if (P->sponge)
+        hipLaunchKernelGGL(k_d2f, dim3(gN), dim3(256), 0, s, sponge, P->spg, N);
+    if (J) HIPX(hipMemsetAsync(P->J_acc, 0, sizeof(double), s));
+    if (grad) {
+        HIPX(hipMemsetAsync(P->g_hi, 0, sizeof(float) * N, s));
+        HIPX(hipMemsetAsync(P->g_lo, 0, sizeof(float) * N, s));
+    }
+    const bool store = mode == MODE_GRAD;
+    const int nt = P->nt, nr = P->n_rec;
+    const int nd = P->ndim;
+    // History slots: level n in Wslot[n] (n >= 1), levels <= 0 are Z.
+    auto lvl = [&](int n) -> float * {
+        if (n <= 0) return P->Z;
+        return store ? P->Wslot[n] : P->F[n % 3];
+    };
+    for (int b = 0; b < P->n_tx; ++b) {
+        const int src = (int)P->tx[b];
+        // S_src * w[n-1] per level, same FP64 product as the batched path.
+        hipLaunchKernelGGL(k_setup_sinc, dim3(grid_for(nt)), dim3(256), 0, s, m, w,
+                           P->src_all + b, P->sinc, nt, 1, P->dt * P->dt);
+        HIPX(hipMemsetAsync(P->recb, 0, sizeof(float) * nr, s));
+        for (int n = 1; n < nt; ++n) {
+            if (pr) ev_start(pr, pr->fwd_ev, s);
+            if (P->sponge)
+                hipLaunchKernelGGL((k_fwd_plain<true, true>), dim3(gN), dim3(256), 0, s, lvl(n - 1),
+                                   lvl(n - 2), lvl(n), P->sp2, P->spg, src, P->sinc + n, P->rec_head,
+                                   P->rec_next, P->recb + (int64_t)n * nr, P->nz, P->ny, P->nx, nd);
+            else
+                hipLaunchKernelGGL((k_fwd_plain<false, true>), dim3(gN), dim3(256), 0, s,
+                                   lvl(n - 1), lvl(n - 2), lvl(n), P->sp2, P->spg, src, P->sinc + n,
+                                   P->rec_head, P->rec_next, P->recb + (int64_t)n * nr, P->nz,
+                                   P->ny, P->nx, nd);
+            if (pr) ev_stop(pr, pr->fwd_ev, s);
+        }
+        if (data)
+            HIPX(hipMemcpyAsync(data + (int64_t)b * nt * nr, P->recb, sizeof(float) * nt * nr,
+                                hipMemcpyDeviceToDevice, s));
+        if (mode == MODE_FWD) continue;
+        // residual of this shot against dobs[b] (layout [n][j] already)
+        HIPX(hipMemcpyAsync(P->dobsT, dobs + (int64_t)b * nt * nr, sizeof(float) * nt * nr,
+                            hipMemcpyDeviceToDevice, s));
+        if (misfit == USCT_MISFIT_GCN)
+            HIPX(hipMemcpyAsync(P->tmask_cur, P->tmask, nr, hipMemcpyDeviceToDevice, s));
+        launch_residual(P, s, misfit, store);
+        if (!store) continue;
+        for (int k = nt - 1; k >= 1; --k) {
+            const float *pa, *pb = nullptr, *pc2 = nullptr;
+            if (P->sponge) { pa = lvl(k); pb = lvl(k - 1); pc2 = lvl(k - 2); }
+            else pa = lvl(k - 1);
+            float *lk = lam_level(P, k);
+            if (pr) ev_start(pr, pr->adj_ev, s);
+            if (P->sponge)
+                hipLaunchKernelGGL((k_adj_plain<true>), dim3(gN), dim3(256), 0, s, lam_level(P, k + 1),
+                                   lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2, P->spg, src, P->sinc + k,
+                                   P->rec_head, P->rec_next, P->res + (int64_t)k * nr, P->g_hi,
+                                   P->g_lo, P->nz, P->ny, P->nx, nd);
+            else
+                hipLaunchKernelGGL((k_adj_plain<false>), dim3(gN), dim3(256), 0, s,
+                                   lam_level(P, k + 1), lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2,
+                                   P->spg, src, P->sinc + k, P->rec_head, P->rec_next,
+                                   P->res + (int64_t)k * nr, P->g_hi, P->g_lo, P->nz, P->ny, P->nx,
+                                   nd);
+            if (pr) ev_stop(pr, pr->adj_ev, s);
+        }
+        TRY(check_launch());
+    }
+    if (grad)
+        hipLaunchKernelGGL(k_finalize_grad, dim3(gN), dim3(256), 0, s, P->g_hi, P->g_lo, 1, N, m,
+                           grad, 0);
+    if (J) HIPX(hipMemcpyAsync(J, P->J_acc, sizeof(double), hipMemcpyDeviceToDevice, s));
+    return check_launch();
+}
+
+// ---------------------------------------------------------------------------
+// Planning
+// ---------------------------------------------------------------------------
+struct MemPlan {
+    int Bp, passes, K, NS;
+    int64_t bytes;
+    double cost;  // relative time estimate
+};
+
+int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
+    const int64_t field = P->N * (int64_t)Bp * 4;
+    const int64_t traces = (int64_t)P->nt * P->n_rec * Bp * 4;
+    int64_t fields = 4;  // Z + F[3]
+    if (grad) fields += (K + 2) + 2 * std::max(NS - 2, 0);
+    int64_t b = fields * field + 3 * traces;
+    b += P->N * 8 /*sp2*/ + P->N * 4 /*sponge*/ + 2 * P->N * 4 * (Bp / 16) /*g*/;
+    b += (int64_t)P->nt * Bp * 4 + (int64_t)P->n_rec * Bp * 24 + (int64_t)P->N * 4 + (1 << 20);
+    return b;
+}
+
+int choose_batched(usct_plan *P, int ckpt_every) {
+    const int B = P->n_tx;
+    // lanes per shot group: the widest group with at most 1/8 padding waste
+    int lb = 64;
+    for (int c : {64, 32, 16}) {
+        lb = c;
+        const int64_t pad = cdiv(B, c) * c;
+        if ((pad - B) * 8 <= pad) break;
+    }
+    P->LB = lb;
+    const int nt = P->nt;
+    const int64_t total_groups = cdiv(B, lb);
+    MemPlan best{0, 0, 0, 0, 0, 1e300};
+    // Try shot-group counts per pass from all groups down to one.
+    for (int64_t gpp = total_groups; gpp >= 1; --gpp) {
+        const int Bp = (int)(gpp * lb);
+        const int passes = (int)cdiv(total_groups, gpp);
+        // candidate K values: whole history, requested, and the memory minimiser
+        std::vector<int> Ks;
+        if (ckpt_every > 0) Ks.push_back(std::min(std::max(ckpt_every, 2), std::max(nt - 1, 2)));
+        else {
+            Ks.push_back(std::max(nt - 1, 2));
+            int kmin = std::max(2, (int)std::lround(std::sqrt(2.0 * std::max(nt - 1, 1))));
+            Ks.push_back(kmin);
+        }
+        for (int K : Ks) {
+            const int NS = (int)std::max<int64_t>(1, cdiv(nt - 1, K));
+            const int64_t bytes = batched_bytes(P, Bp, K, NS, true);
+            if (bytes > P->budget) continue;
+            // cost: bytes streamed per shot-level (fwd 12, adj 17, recompute 12 if NS > 1),
+            // padded shots count, narrower lane groups stream a little less efficiently.
+            const double per = 12.0 + 17.0 + (NS > 1 ? 12.0 * (1.0 - 1.0 / NS) : 0.0);
+            const double eff = lb == 64 ? 1.0 : (lb == 32 ? 0.95 : 0.88);
+            // launch overhead of ~3 us per step launch, in streamed-byte units at 5 TB/s
+            const double launches = (double)passes * nt * (NS > 1 ? 3.0 : 2.0);
+            const double cost = (double)passes * Bp * per * (double)P->N * nt / eff +
+                                launches * 15e6;
+            if (cost < best.cost) best = MemPlan{Bp, passes, K, NS, bytes, cost};
+        }
+    }
+    if (best.Bp == 0)
+        return fail(USCT_ENOMEM, "batched plan does not fit the memory budget of %lld bytes",
+                    (long long)P->budget);
+    P->Bp = best.Bp;
+    P->passes = best.passes;
+    P->K = best.K;
+    P->NS = best.NS;
+    return 0;
+}
+
+int allocate_batched(usct_plan *P) {
+    const int64_t FE = P->field_elems();
+    TRY(dalloc_t(P, &P->Z, FE));
+    HIPX(hipMemset(P->Z, 0, sizeof(float) * FE));
+    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->F[i], FE));
+    const int nW = P->K + 2;
+    P->Wslot.resize(nW);
+    for (int i = 0; i < nW; ++i) TRY(dalloc_t(P, &P->Wslot[i], FE));
+    const int nC = 2 * std::max(P->NS - 2, 0);
+    P->CK.resize(nC);
+    for (int i = 0; i < nC; ++i) TRY(dalloc_t(P, &P->CK[i], FE));
+    const int64_t ntr = (int64_t)P->n_rec * P->Bp;
+    TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * ntr));
+    TRY(dalloc_t(P, &P->dobsT, (int64_t)P->nt * ntr));
+    TRY(dalloc_t(P, &P->res, (int64_t)P->nt * ntr));
+    // sized for the narrowest lane group the autotuner may pick (16 lanes)
+    TRY(dalloc_t(P, &P->g_hi, P->N * std::max(P->Bp / 16, 1)));
+    TRY(dalloc_t(P, &P->g_lo, P->N * std::max(P->Bp / 16, 1)));
+    TRY(dalloc_t(P, &P->sinc, (int64_t)P->nt * P->Bp));
+    TRY(dalloc_t(P, &P->src_cell, P->Bp));
+    TRY(dalloc_t(P, &P->gstats, 3 * ntr));
+    TRY(dalloc_t(P, &P->tmask_cur, ntr));
+    // per-pass source cells and trace masks
+    std::vector<int> src((size_t)P->passes * P->Bp, -1);
+    std::vector<unsigned char> tm((size_t)P->passes * ntr, 0);
+    for (int i = 0; i < P->n_tx; ++i) {
+        src[i] = (int)P->tx[i];
+        const int p = i / P->Bp, b = i % P->Bp;
+        for (int j = 0; j < P->n_rec; ++j) tm[(size_t)p * ntr + (size_t)j * P->Bp + b] = 1;
+    }
+    TRY(dalloc_t(P, &P->src_all, (int64_t)src.size()));
+    HIPX(hipMemcpy(P->src_all, src.data(), sizeof(int) * src.size(), hipMemcpyHostToDevice));
+    TRY(dalloc_t(P, &P->tmask, (int64_t)tm.size()));
+    HIPX(hipMemcpy(P->tmask, tm.data(), tm.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int allocate_plain(usct_plan *P) {
+    const int64_t N = P->N;
+    TRY(dalloc_t(P, &P->Z, N));
+    HIPX(hipMemset(P->Z, 0, sizeof(float) * N));
+    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->F[i], N));
+    // full history, levels 1..nt-1 (slot 0 unused)
+    const int64_t hist = (int64_t)P->nt * N * 4;
+    if (hist + P->bytes > P->budget)
+        return fail(USCT_ENOMEM, "plain plan: full history (%lld bytes) exceeds the budget",
+                    (long long)hist);
+    float *H = nullptr;
+    TRY(dalloc_t(P, &H, (int64_t)P->nt * N));
+    P->Wslot.resize(P->nt);
+    for (int n = 0; n < P->nt; ++n) P->Wslot[n] = H + (int64_t)n * N;
+    const int nr = P->n_rec;
+    TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * nr));
+    TRY(dalloc_t(P, &P->dobsT, (int64_t)P->nt * nr));
+    TRY(dalloc_t(P, &P->res, (int64_t)P->nt * nr));
+    TRY(dalloc_t(P, &P->g_hi, N));
+    TRY(dalloc_t(P, &P->g_lo, N));
+    TRY(dalloc_t(P, &P->sinc, P->nt));
+    TRY(dalloc_t(P, &P->gstats, 3 * (int64_t)nr));
+    TRY(dalloc_t(P, &P->tmask, nr));
+    TRY(dalloc_t(P, &P->tmask_cur, nr));
+    std::vector<unsigned char> tm(nr, 1);
+    HIPX(hipMemcpy(P->tmask, tm.data(), nr, hipMemcpyHostToDevice));
+    std::vector<int> src(P->n_tx);
+    for (int i = 0; i < P->n_tx; ++i) src[i] = (int)P->tx[i];
+    TRY(dalloc_t(P, &P->src_all, std::max(P->n_tx, 1)));
+    if (P->n_tx)
+        HIPX(hipMemcpy(P->src_all, src.data(), sizeof(int) * P->n_tx, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Time a few forward and adjoint launches of the current tile choice.
+double time_tile(usct_plan *P) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    const int reps = 4;
+    // warm-up
+    launch_fwd_batched(P, P->stream, P->F[0], P->F[1], P->F[2], 1, false);
+    launch_adj_batched(P, P->stream, P->F[0], P->F[1], P->F[2], P->Wslot[0], P->Wslot[0],
+                       P->Wslot[0], 1);
+    (void)hipEventRecord(a, P->stream);
+    for (int r = 0; r < reps; ++r) {
+        launch_fwd_batched(P, P->stream, P->F[r % 3], P->F[(r + 1) % 3], P->F[(r + 2) % 3], 1, false);
+        launch_adj_batched(P, P->stream, P->F[r % 3], P->F[(r + 1) % 3], P->F[(r + 2) % 3],
+                           P->Wslot[0], P->Wslot[0], P->Wslot[0], 1);
+    }
+    (void)hipEventRecord(b, P->stream);
+    (void)hipEventSynchronize(b);
+    float ms = 1e30f;
+    if (hipGetLastError() != hipSuccess) ms = 1e30f;
+    else (void)hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return ms;
+}
+
+// Autotuner: tile shape (rows per wavefront; 3D patch) and lane-group width
+// among those that keep the same padded shot count.
+int autotune(usct_plan *P) {
+    // zero the fields used by the timing launches
+    for (int i = 0; i < 3; ++i) HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
+    HIPX(hipMemset(P->Wslot[0], 0, sizeof(float) * P->field_elems()));
+    HIPX(hipMemset(P->sinc, 0, sizeof(float) * P->nt * P->Bp));
+    HIPX(hipMemset(P->sp2, 0, sizeof(float2) * P->N));
+    HIPX(hipMemset(P->g_hi, 0, sizeof(float) * P->N * std::max(P->Bp / 16, 1)));
+    HIPX(hipMemset(P->g_lo, 0, sizeof(float) * P->N * std::max(P->Bp / 16, 1)));
+    HIPX(hipMemset(P->res, 0, sizeof(float) * P->nt * P->n_rec * P->Bp));
+    HIPX(hipMemcpy(P->src_cell, P->src_all, sizeof(int) * P->Bp, hipMemcpyDeviceToDevice));
+    std::vector<int> lbs;
+    for (int c : {64, 32, 16})
+        if (P->Bp % c == 0) lbs.push_back(c);
+    struct Cand { int lb, cx, cy, ry; };
+    std::vector<Cand> cands;
+    if (P->ndim == 2) {
+        for (int lb : lbs)
+            for (int ry : {16, 32, 64, 128}) cands.push_back({lb, 8, 1, ry});
+    } else {
+        for (int lb : lbs)
+            for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}})
+                for (int rz : {16, 32, 64}) {
+                    if (pc.first * pc.second > lb) continue;
+                    cands.push_back({lb, pc.first, pc.second, rz});
+                }
+    }
+    double best = 1e300;
+    Cand bc{P->LB, P->CX, P->CY, P->RY};
+    for (const Cand &c : cands) {
+        P->LB = c.lb; P->CX = c.cx; P->CY = c.cy; P->RY = c.ry;
+        const double t = time_tile(P);
+        if (t < best) { best = t; bc = c; }
+    }
+    P->LB = bc.lb; P->CX = bc.cx; P->CY = bc.cy; P->RY = bc.ry;
+    return 0;
+}
+
+int build_receivers(usct_plan *P) {
+    std::vector<int> head(P->N, -1), next(std::max(P->n_rec, 1), -1);
+    // chains in ascending receiver order (fwi.py:321-322 subtracts j = 0, 1, ...)
+    for (int j = P->n_rec - 1; j >= 0; --j) {
+        const int64_t c = P->rec[j];
+        next[j] = head[c];
+        head[c] = j;
+    }
+    TRY(dalloc_t(P, &P->rec_head, P->N));
+    TRY(dalloc_t(P, &P->rec_next, (int64_t)next.size()));
+    HIPX(hipMemcpy(P->rec_head, head.data(), sizeof(int) * P->N, hipMemcpyHostToDevice));
+    HIPX(hipMemcpy(P->rec_next, next.data(), sizeof(int) * next.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int validate(int ndim, const int *dims, double h, double dt, int nt, const int64_t *tx, int n_tx,
+             const int64_t *rec, int n_rec, int64_t *N) {
+    if (ndim != 2 && ndim != 3) return fail(USCT_EINVAL, "ndim must be 2 or 3 (got %d)", ndim);
+    int64_t n = 1;
+    for (int d = 0; d < ndim; ++d) {
+        if (dims[d] < 1) return fail(USCT_EINVAL, "dims[%d] = %d must be positive", d, dims[d]);
+        n *= dims[d];
+    }
+    if (n >= (int64_t)1 << 31) return fail(USCT_EINVAL, "grid of %lld cells exceeds 2^31", (long long)n);
+    if (!(h > 0) || !(dt > 0)) return fail(USCT_EINVAL, "h and dt must be positive");
+    if (nt < 1) return fail(USCT_EINVAL, "nt must be >= 1 (got %d)", nt);
+    if (n_tx < 0 || n_rec < 0) return fail(USCT_EINVAL, "negative shot or receiver count");
+    if (n_rec < 1) return fail(USCT_EINVAL, "at least one receiver is required");
+    for (int i = 0; i < n_tx; ++i)
+        if (tx[i] < 0 || tx[i] >= n)
+            return fail(USCT_EINVAL, "tx_lin[%d] = %lld outside the grid of %lld cells", i,
+                        (long long)tx[i], (long long)n);
+    for (int j = 0; j < n_rec; ++j)
+        if (rec[j] < 0 || rec[j] >= n)
+            return fail(USCT_EINVAL, "rec_lin[%d] = %lld outside the grid of %lld cells", j,
+                        (long long)rec[j], (long long)n);
+    *N = n;
+    return 0;
+}
+
+int enter(usct_plan *P, void *stream) {
+    hipStream_t user = (hipStream_t)stream;
+    HIPX(hipEventRecord(P->ev_in, user));
+    HIPX(hipStreamWaitEvent(P->stream, P->ev_in, 0));
+    return 0;
+}
+
+int leave(usct_plan *P, void *stream) {
+    hipStream_t user = (hipStream_t)stream;
+    HIPX(hipEventRecord(P->ev_out, P->stream));
+    HIPX(hipStreamWaitEvent(user, P->ev_out, 0));
+    return 0;
+}
+
+int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
+        const double *sponge, const float *dobs, float *data, double *J, double *grad,
+        void *stream, Profile *pr) {
+    if (!P) return fail(USCT_EINVAL, "null plan");
+    if (!m || !w) return fail(USCT_EINVAL, "m and wavelet are required");
+    if (misfit != USCT_MISFIT_L2 && misfit != USCT_MISFIT_GCN)
+        return fail(USCT_EINVAL, "unknown misfit type %d", misfit);
+    if ((sponge != nullptr) != P->sponge)
+        return fail(USCT_EINVAL, sponge ? "sponge passed to a plan created without USCT_FLAG_SPONGE"
+                                        : "plan created with USCT_FLAG_SPONGE needs a sponge");
+    if (mode != MODE_FWD && !dobs) return fail(USCT_EINVAL, "dobs is required");
+    if (P->n_tx == 0) {
+        TRY(enter(P, stream));
+        if (J) HIPX(hipMemsetAsync(J, 0, sizeof(double), P->stream));
+        if (grad) HIPX(hipMemsetAsync(grad, 0, sizeof(double) * P->N, P->stream));
+        return leave(P, stream);
+    }
+    TRY(enter(P, stream));
+    int rc = P->method == USCT_METHOD_PLAIN
+                 ? run_plain(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr)
+                 : run_batched(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr);
+    int rc2 = leave(P, stream);
+    return rc ? rc : rc2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, double dt, int nt,
+                     const int64_t *tx_lin, int n_tx, const int64_t *rec_lin, int n_rec,
+                     int method, int ckpt_every, int64_t mem_budget, int flags) {
+    if (!out || !dims) return fail(USCT_EINVAL, "null argument");
+    *out = nullptr;
+    int64_t N = 0;
+    TRY(validate(ndim, dims, h, dt, nt, tx_lin, n_tx, rec_lin, n_rec, &N));
+    if ((flags & USCT_FLAG_SPONGE) && ndim != 2)
+        return fail(USCT_ENOTSUP, "the sponge is defined for 2D grids only (solver.py:30)");
+    if (method != USCT_METHOD_AUTO && method != USCT_METHOD_PLAIN && method != USCT_METHOD_BATCHED)
+        return fail(USCT_EINVAL, "unknown method %d", method);
+    usct_plan *P = new usct_plan();
+    P->ndim = ndim;
+    P->nz = ndim == 3 ? dims[0] : 1;
+    P->ny = dims[ndim - 2];
+    P->nx = dims[ndim - 1];
+    P->N = N;
+    P->h = h;
+    P->dt = dt;
+    P->nt = nt;
+    P->n_tx = n_tx;
+    P->n_rec = n_rec;
+    P->tx.assign(tx_lin, tx_lin + n_tx);
+    P->rec.assign(rec_lin, rec_lin + n_rec);
+    P->method = method == USCT_METHOD_PLAIN ? USCT_METHOD_PLAIN : USCT_METHOD_BATCHED;
+    P->flags = flags;
+    P->sponge = (flags & USCT_FLAG_SPONGE) != 0;
+    if (ndim == 3) { P->CX = 4; P->CY = 4; P->RY = 32; }
+    int rc = 0;
+    auto bail = [&](int code) {
+        free_plan_memory(P);
+        if (P->stream) (void)hipStreamDestroy(P->stream);
+        if (P->ev_in) (void)hipEventDestroy(P->ev_in);
+        if (P->ev_out) (void)hipEventDestroy(P->ev_out);
+        delete P;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_in, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_out, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(USCT_EHIP, "stream/event creation failed"));
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
+        return bail(fail(USCT_EHIP, "hipMemGetInfo failed"));
+    P->budget = mem_budget > 0 ? mem_budget : (int64_t)(0.85 * (double)free_b);
+    if ((rc = build_receivers(P))) return bail(rc);
+    if ((rc = dalloc_t(P, &P->sp2, N))) return bail(rc);
+    if (P->sponge && (rc = dalloc_t(P, &P->spg, N))) return bail(rc);
+    if ((rc = dalloc_t(P, &P->J_acc, 1))) return bail(rc);
+    if ((rc = dalloc_t(P, &P->part, P->npart))) return bail(rc);
+    if (P->method == USCT_METHOD_PLAIN) {
+        P->LB = 1; P->CX = 1; P->CY = 1; P->RY = 1;
+        P->Bp = 1;
+        P->passes = n_tx;
+        P->K = nt;
+        P->NS = 1;
+        if ((rc = allocate_plain(P))) return bail(rc);
+    } else {
+        if (n_tx == 0) {
+            P->Bp = 64; P->passes = 0; P->K = std::max(nt - 1, 2); P->NS = 1;
+        } else {
+            P->budget -= P->bytes;
+            if ((rc = choose_batched(P, ckpt_every))) return bail(rc);
+            P->budget += P->bytes;
+            if ((rc = allocate_batched(P))) return bail(rc);
+            if ((flags & USCT_FLAG_AUTOTUNE) && (rc = autotune(P))) return bail(rc);
+        }
+    }
+    *out = P;
+    return 0;
+}
+
+int usct_plan_destroy(usct_plan *P) {
+    if (!P) return 0;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    free_plan_memory(P);
+    if (P->stream) (void)hipStreamDestroy(P->stream);
+    if (P->ev_in) (void)hipEventDestroy(P->ev_in);
+    if (P->ev_out) (void)hipEventDestroy(P->ev_out);
+    delete P;
+    return 0;
+}
+
+int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
+    if (!P || !info) return fail(USCT_EINVAL, "null argument");
+    info->method = P->method;
+    info->lanes_per_group = P->LB;
+    info->tile_x = P->CX;
+    info->tile_y = P->CY;
+    info->rows_per_wave = P->RY;
+    info->shots_padded = P->method == USCT_METHOD_PLAIN ? P->n_tx : P->Bp * P->passes;
+    info->shot_passes = P->passes;
+    info->shots_per_pass = P->Bp;
+    info->ckpt_every = P->K;
+    info->n_windows = P->NS;
+    info->workspace_bytes = P->bytes;
+    return 0;
+}
+
+int usct_forward_fmc(usct_plan *P, const double *m, const double *wavelet, const double *sponge,
+                     float *data, void *stream) {
+    if (!data) return fail(USCT_EINVAL, "data is required");
+    return run(P, MODE_FWD, USCT_MISFIT_L2, m, wavelet, sponge, nullptr, data, nullptr, nullptr,
+               stream, nullptr);
+}
+
+int usct_misfit(usct_plan *P, const double *m, const double *wavelet, const double *sponge,
+                const float *dobs, int misfit_type, double *J, void *stream) {
+    if (!J) return fail(USCT_EINVAL, "J is required");
+    return run(P, MODE_MISFIT, misfit_type, m, wavelet, sponge, dobs, nullptr, J, nullptr, stream,
+               nullptr);
+}
+
+int usct_misfit_and_gradient(usct_plan *P, const double *m, const double *wavelet,
+                             const double *sponge, const float *dobs, int misfit_type, double *J,
+                             double *grad, void *stream) {
+    if (!J || !grad) return fail(USCT_EINVAL, "J and grad are required");
+    return run(P, MODE_GRAD, misfit_type, m, wavelet, sponge, dobs, nullptr, J, grad, stream,
+               nullptr);
+}
+
+int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,
+                          const double *sponge, const float *dobs, double *fwd_ms, double *adj_ms,
+                          int *n_fwd, int *n_adj, void *stream) {
+    if (!P) return fail(USCT_EINVAL, "null plan");
+    double *Jd = nullptr, *gd = nullptr;
+    HIPX(hipMalloc(&Jd, sizeof(double)));
+    if (hipMalloc(&gd, sizeof(double) * P->N) != hipSuccess) {
+        (void)hipFree(Jd);
+        return fail(USCT_ENOMEM, "profile: gradient buffer");
+    }
+    Profile pr;
+    int rc = run(P, MODE_GRAD, USCT_MISFIT_L2, m, wavelet, sponge, dobs, nullptr, Jd, gd, stream, &pr);
+    if (!rc) rc = hipStreamSynchronize(P->stream) == hipSuccess ? 0 : fail(USCT_EHIP, "sync");
+    auto sum = [](std::vector<hipEvent_t> &v, double *ms, int *cnt) {
+        double s = 0;
+        int c = 0;
+        for (size_t i = 0; i + 1 < v.size(); i += 2) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, v[i], v[i + 1]) == hipSuccess) { s += t; ++c; }
+        }
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        if (ms) *ms = c ? s / c : 0.0;
+        if (cnt) *cnt = c;
+    };
+    sum(pr.fwd_ev, fwd_ms, n_fwd);
+    sum(pr.adj_ev, adj_ms, n_adj);
+    (void)hipFree(Jd);
+    (void)hipFree(gd);
+    return rc;
+}
+
+int usct_forward_fmc_direct(const double *m, const int *dims, int ndim, double h, double dt, int nt,
+                            const int64_t *tx_lin, int n_tx, const int64_t *rec_lin, int n_rec,
+                            const double *wavelet, float *data, int method, void *stream) {
+    usct_plan *P = nullptr;
+    TRY(usct_plan_create(&P, ndim, dims, h, dt, nt, tx_lin, n_tx, rec_lin, n_rec, method, 0, 0,
+                         USCT_FLAG_NO_GRAPH));
+    int rc = usct_forward_fmc(P, m, wavelet, nullptr, data, stream);
+    if (!rc && hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+        rc = fail(USCT_EHIP, "stream synchronize failed");
+    usct_plan_destroy(P);
+    return rc;
+}
+
+int usct_misfit_and_gradient_direct(const double *m, const int *dims, int ndim, double h,
+                                    double dt, int nt, const int64_t *tx_lin, int n_tx,
+                                    const int64_t *rec_lin, int n_rec, const double *wavelet,
+                                    const float *dobs, double *grad, double *J_host,
+                                    int misfit_type, int method, void *stream) {
+    if (!J_host) return fail(USCT_EINVAL, "J_host is required");
+    usct_plan *P = nullptr;
+    TRY(usct_plan_create(&P, ndim, dims, h, dt, nt, tx_lin, n_tx, rec_lin, n_rec, method, 0, 0,
+                         USCT_FLAG_NO_GRAPH));
+    double *Jd = nullptr;
+    if (hipMalloc(&Jd, sizeof(double)) != hipSuccess) {
+        usct_plan_destroy(P);
+        return fail(USCT_ENOMEM, "J buffer");
+    }
+    int rc = usct_misfit_and_gradient(P, m, wavelet, nullptr, dobs, misfit_type, Jd, grad, stream);
+    if (!rc && hipMemcpyAsync(J_host, Jd, sizeof(double), hipMemcpyDeviceToHost,
+                              (hipStream_t)stream) != hipSuccess)
+        rc = fail(USCT_EHIP, "J copy failed");
+    if (!rc && hipStreamSynchronize((hipStream_t)stream) != hipSuccess)
+        rc = fail(USCT_EHIP, "stream synchronize failed");
+    (void)hipFree(Jd);
+    usct_plan_destroy(P);
+    return rc;
+}
+
+const char *usct_last_error(void) { return g_err.c_str(); }
+
+int usct_version(void) { return 1; }
+
+}  // extern "C"

@@ -1,0 +1,812 @@
+// usct_kernels.hpp -- CDNA4 (gfx950) kernels of the acoustic leapfrog solver and
+// its exact discrete adjoint.
+//
+// Two field layouts:
+//   plain    p[cell]                 one shot, one thread per cell
+//   batched  p[cell * Bp + shot]     all shots of a pass, shot index fastest.
+//            A wavefront's lanes hold LB consecutive shots of 64/LB adjacent
+//            cell strips, so every load/store is a contiguous run of LB floats,
+//            and per-cell data (S', sponge, receiver map) is shared by the LB
+//            lanes.  Each lane marches a strip of CX cells (2D: along y,
+//            3D: a CX x CY patch along z) and keeps the three stencil rows or
+//            planes in registers; only the x (and y) halo is re-read.
+//
+// Arithmetic (per cell, FP32 fields, identical in both layouts):
+//   lap_u   = interior-only 5/7-point sum, centre term first, then axes
+//             0..ndim-1, "+1" before "-1" (simulation/ringfwi/solver.py:45-62),
+//             without the 1/h^2 factor;
+//   S'      = dt^2 / (m h^2) computed in FP64 and carried as an FP32 hi/lo pair,
+//             so the FP32 update sees S' to ~2^-48 (the FP32 rounding of S is
+//             a systematic velocity bias otherwise);
+//   forward p^n = (2 p^{n-1} - p^{n-2}) + S' lap_u(p^{n-1}) + S_src w[n-1]
+//             (fwi.py:63-67, lap[src] += w before the S product);
+//   adjoint lam^k = (2 lam^{k+1} - lam^{k+2}) + lap_u(S' lam^{k+1}) - R^T r^k
+//             (fwi.py:310-322); sponge multiplies p^n and lam^k (fwi.py:68,313);
+//   imaging d2p^k = p^k - 2p^{k-1} + p^{k-2}; without a sponge the recursion
+//             makes this exactly S' lap_u(p^{k-1}) + S_src w[k-1], which reads one
+//             stored level instead of three and avoids a cancellation
+//             (fwi.py:325-326);  g += lam^k d2p^k with Neumaier compensation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace usct {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;  // 4 wavefronts
+
+// Blocks are dispatched round-robin over the 8 XCDs (block i -> XCD i % 8).
+// Give each XCD a contiguous range of logical blocks so neighbouring strips,
+// which share halo rows, share an L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+    const int per = nblk >> 3, rem = nblk & 7;
+    const int xcd = bid & 7, slot = bid >> 3;
+    return xcd * per + (xcd < rem ? xcd : rem) + slot;
+}
+
+__device__ __forceinline__ float lap5(float c, float yp, float ym, float xp, float xm) {
+    float a = -4.0f * c;
+    a = a + yp;
+    a = a + ym;
+    a = a + xp;
+    a = a + xm;
+    return a;
+}
+
+__device__ __forceinline__ float lap7(float c, float zp, float zm, float yp, float ym,
+                                      float xp, float xm) {
+    float a = -6.0f * c;
+    a = a + zp;
+    a = a + zm;
+    a = a + yp;
+    a = a + ym;
+    a = a + xp;
+    a = a + xm;
+    return a;
+}
+
+// S' * v with S' = hi + lo.
+__device__ __forceinline__ float smul(float2 s, float v) { return fmaf(s.x, v, s.y * v); }
+
+__device__ __forceinline__ float leap(float cur, float prev, float inc) {
+    return (2.0f * cur - prev) + inc;
+}
+
+__device__ __forceinline__ void neumaier(float &s, float &c, float x) {
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x)) c = c + ((s - t) + x);
+    else c = c + ((x - t) + s);
+    s = t;
+}
+
+// Reduce CX per-lane values over each aligned group of LB lanes and scatter the
+// CX sums: afterwards lane l holds the sum for cell index sum_cell<LB,CX>(l)
+// (recursive halving: CX/2 + CX/4 + ... + log2(LB/CX) shuffles instead of
+// CX*log2(LB)).  Order is fixed, so results do not depend on scheduling.
+template <int LB, int CX>
+__device__ __forceinline__ float reduce_scatter(float (&v)[CX], int lane) {
+    static_assert((LB & (LB - 1)) == 0 && (CX & (CX - 1)) == 0 && CX <= LB, "pow2");
+#pragma unroll
+    for (int half = CX / 2, off = LB / 2; half >= 1; half >>= 1, off >>= 1) {
+        const bool up = (lane & off) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const float send = up ? v[i] : v[i + half];
+            const float keep = up ? v[i + half] : v[i];
+            v[i] = keep + __shfl_xor(send, off, kWave);
+        }
+    }
+    float s = v[0];
+#pragma unroll
+    for (int off = LB / (2 * CX); off >= 1; off >>= 1) s = s + __shfl_xor(s, off, kWave);
+    return s;
+}
+
+template <int LB, int CX>
+__device__ __forceinline__ int sum_cell(int lane) {
+    int ci = 0;
+#pragma unroll
+    for (int half = CX / 2, off = LB / 2; half >= 1; half >>= 1, off >>= 1)
+        if (lane & off) ci += half;
+    return ci;
+}
+
+// ---------------------------------------------------------------------------
+// Batched 2D forward step: level n from levels n-1 (pc) and n-2 (pp) into pn.
+// sinc[b] = S_src(b) * w[n-1] (FP64 product rounded once); rec: [n_rec][Bp] row
+// of level n.  Grid: ng * nxs * nyc wavefronts, 4 per block.
+// ---------------------------------------------------------------------------
+template <int LB, int CX, bool SPONGE, bool REC>
+__global__ void __launch_bounds__(kBlock)
+k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
+        const float2 *__restrict__ sp2, const float *__restrict__ sponge,
+        const int *__restrict__ src_cell, const float *__restrict__ sinc,
+        const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+        float *__restrict__ rec, int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+    constexpr int W = kWave / LB;
+    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    const int t = wv / ng;
+    const int xs = t % nxs, yc = t / nxs;
+    if (yc >= nyc) return;
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = yc * RY;
+    const int y1 = min(y0 + RY, ny);
+    if (x0 >= nx) return;
+    const size_t BP = (size_t)Bp;
+    const int my_src = src_cell[b];
+    const float my_inc = sinc[b];
+    auto at = [&](int y, int x) -> size_t {
+        x = min(max(x, 0), nx - 1);
+        return ((size_t)y * nx + x) * BP + b;
+    };
+    float rm[CX], rc[CX + 2], rp[CX];
+    {
+        const int ym = max(y0 - 1, 0);
+#pragma unroll
+        for (int i = 0; i < CX; ++i) rm[i] = pc[at(ym, x0 + i)];
+#pragma unroll
+        for (int i = 0; i < CX + 2; ++i) rc[i] = pc[at(y0, x0 - 1 + i)];
+    }
+    for (int y = y0; y < y1; ++y) {
+        const int yp = min(y + 1, ny - 1);
+#pragma unroll
+        for (int i = 0; i < CX; ++i) rp[i] = pc[at(yp, x0 + i)];
+        const bool yint = (y >= 1) && (y <= ny - 2);
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            const int x = x0 + i;
+            if (x < nx) {
+                const int c = y * nx + x;
+                float lu = 0.0f;
+                if (yint && x >= 1 && x <= nx - 2) lu = lap5(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
+                float inc = smul(sp2[c], lu);
+                if (c == my_src) inc = inc + my_inc;
+                const size_t a = (size_t)c * BP + b;
+                float v = leap(rc[i + 1], pp[a], inc);
+                if (SPONGE) v = v * sponge[c];
+                pn[a] = v;
+                if (REC) {
+                    for (int j = rec_head[c]; j >= 0; j = rec_next[j]) rec[(size_t)j * BP + b] = v;
+                }
+            }
+        }
+        if (y + 1 < y1) {
+#pragma unroll
+            for (int i = 0; i < CX; ++i) { rm[i] = rc[i + 1]; rc[i + 1] = rp[i]; }
+            rc[0] = pc[at(y + 1, x0 - 1)];
+            rc[CX + 1] = pc[at(y + 1, x0 + CX)];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched 2D adjoint step with the imaging condition.
+//   l1 = lam^{k+1}, l2 = lam^{k+2}, lk = lam^k (output)
+//   no sponge: pa = p^{k-1};  sponge: pa = p^k, pb = p^{k-1}, pc2 = p^{k-2}
+//   sinc[b] = S_src(b) w[k-1]; res = [n_rec][Bp] residual row of level k
+//   g_hi/g_lo: [ng][N] Neumaier accumulators of sum_shots sum_k lam^k d2p^k
+// ---------------------------------------------------------------------------
+template <int LB, int CX, bool SPONGE>
+__global__ void __launch_bounds__(kBlock)
+k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
+        const float *__restrict__ pa, const float *__restrict__ pb, const float *__restrict__ pc2,
+        const float2 *__restrict__ sp2, const float *__restrict__ sponge,
+        const int *__restrict__ src_cell, const float *__restrict__ sinc,
+        const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+        const float *__restrict__ res, float *__restrict__ g_hi, float *__restrict__ g_lo,
+        int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+    constexpr int W = kWave / LB;
+    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    const int t = wv / ng;
+    const int xs = t % nxs, yc = t / nxs;
+    if (yc >= nyc) return;  // uniform over the wavefront
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;  // may be >= nx: lane still joins the reductions
+    const int y0 = yc * RY;
+    const int y1 = min(y0 + RY, ny);
+    const size_t BP = (size_t)Bp;
+    const size_t N = (size_t)ny * nx;
+    const int my_src = src_cell[b];
+    const float my_inc = sinc[b];
+    const int xl = min(x0, nx - 1);
+    auto at = [&](int y, int x) -> size_t {
+        x = min(max(x, 0), nx - 1);
+        return ((size_t)y * nx + x) * BP + b;
+    };
+    auto cl = [&](int y, int x) -> int { return y * nx + min(max(x, 0), nx - 1); };
+    // q = S' lam^{k+1} rows (qm, qc with halo, qp); raw centre row of lam^{k+1}.
+    float qm[CX], qc[CX + 2], qp[CX], lc[CX], lpn[CX];
+    float fm[CX], fc[CX + 2], fp[CX];  // p^{k-1} rows (no sponge)
+    (void)xl;
+    {
+        const int ym = max(y0 - 1, 0);
+#pragma unroll
+        for (int i = 0; i < CX; ++i) qm[i] = smul(sp2[cl(ym, x0 + i)], l1[at(ym, x0 + i)]);
+#pragma unroll
+        for (int i = 0; i < CX + 2; ++i) {
+            const float v = l1[at(y0, x0 - 1 + i)];
+            qc[i] = smul(sp2[cl(y0, x0 - 1 + i)], v);
+            if (i >= 1 && i <= CX) lc[i - 1] = v;
+        }
+        if (!SPONGE) {
+#pragma unroll
+            for (int i = 0; i < CX; ++i) fm[i] = pa[at(ym, x0 + i)];
+#pragma unroll
+            for (int i = 0; i < CX + 2; ++i) fc[i] = pa[at(y0, x0 - 1 + i)];
+        }
+    }
+    for (int y = y0; y < y1; ++y) {
+        const int yp = min(y + 1, ny - 1);
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            const float v = l1[at(yp, x0 + i)];
+            lpn[i] = v;
+            qp[i] = smul(sp2[cl(yp, x0 + i)], v);
+            if (!SPONGE) fp[i] = pa[at(yp, x0 + i)];
+        }
+        const bool yint = (y >= 1) && (y <= ny - 2);
+        float tv[CX];
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            const int x = x0 + i;
+            tv[i] = 0.0f;
+            if (x < nx) {
+                const int c = y * nx + x;
+                const bool in = yint && x >= 1 && x <= nx - 2;
+                const float lq = in ? lap5(qc[i + 1], qp[i], qm[i], qc[i + 2], qc[i]) : 0.0f;
+                const size_t a = (size_t)c * BP + b;
+                float v = leap(lc[i], l2[a], lq);
+                if (SPONGE) v = v * sponge[c];
+                for (int j = rec_head[c]; j >= 0; j = rec_next[j]) v = v - res[(size_t)j * BP + b];
+                lk[a] = v;
+                float d2;
+                if (SPONGE) {
+                    d2 = (pa[a] - 2.0f * pb[a]) + pc2[a];
+                } else {
+                    const float lp = in ? lap5(fc[i + 1], fp[i], fm[i], fc[i + 2], fc[i]) : 0.0f;
+                    d2 = smul(sp2[c], lp);
+                    if (c == my_src) d2 = d2 + my_inc;
+                }
+                tv[i] = v * d2;
+            }
+        }
+        const float s = reduce_scatter<LB, CX>(tv, lane);
+        if ((sb & (LB / CX - 1)) == 0) {
+            const int x = x0 + sum_cell<LB, CX>(lane);
+            if (x < nx) {
+                const size_t gi = (size_t)sg * N + (size_t)y * nx + x;
+                float hi = g_hi[gi], lo = g_lo[gi];
+                neumaier(hi, lo, s);
+                g_hi[gi] = hi;
+                g_lo[gi] = lo;
+            }
+        }
+        if (y + 1 < y1) {
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                qm[i] = qc[i + 1];
+                qc[i + 1] = qp[i];
+                lc[i] = lpn[i];
+            }
+            qc[0] = smul(sp2[cl(y + 1, x0 - 1)], l1[at(y + 1, x0 - 1)]);
+            qc[CX + 1] = smul(sp2[cl(y + 1, x0 + CX)], l1[at(y + 1, x0 + CX)]);
+            if (!SPONGE) {
+#pragma unroll
+                for (int i = 0; i < CX; ++i) { fm[i] = fc[i + 1]; fc[i + 1] = fp[i]; }
+                fc[0] = pa[at(y + 1, x0 - 1)];
+                fc[CX + 1] = pa[at(y + 1, x0 + CX)];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched 3D forward step: each lane owns a CX x CY patch and marches RZ planes.
+// ---------------------------------------------------------------------------
+template <int LB, int CX, int CY, bool REC>
+__global__ void __launch_bounds__(kBlock)
+k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
+        const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
+        const float *__restrict__ sinc, const int *__restrict__ rec_head,
+        const int *__restrict__ rec_next, float *__restrict__ rec, int nz, int ny, int nx,
+        int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+    constexpr int W = kWave / LB;
+    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    int t = wv / ng;
+    const int xs = t % nxs;
+    t /= nxs;
+    const int ys = t % nys, zc = t / nys;
+    if (zc >= nzc) return;
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = ys * CY;
+    const int z0 = zc * RZ;
+    const int z1 = min(z0 + RZ, nz);
+    if (x0 >= nx) return;
+    const size_t BP = (size_t)Bp;
+    const int my_src = src_cell[b];
+    const float my_inc = sinc[b];
+    auto at = [&](int z, int y, int x) -> size_t {
+        x = min(max(x, 0), nx - 1);
+        y = min(max(y, 0), ny - 1);
+        return (((size_t)z * ny + y) * nx + x) * BP + b;
+    };
+    float zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
+    auto load_halo = [&](int z) {
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            zcu[0][i + 1] = pc[at(z, y0 - 1, x0 + i)];
+            zcu[CY + 1][i + 1] = pc[at(z, y0 + CY, x0 + i)];
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            zcu[j + 1][0] = pc[at(z, y0 + j, x0 - 1)];
+            zcu[j + 1][CX + 1] = pc[at(z, y0 + j, x0 + CX)];
+        }
+    };
+    {
+        const int zmi = max(z0 - 1, 0);
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                zm[j][i] = pc[at(zmi, y0 + j, x0 + i)];
+                zcu[j + 1][i + 1] = pc[at(z0, y0 + j, x0 + i)];
+            }
+        load_halo(z0);
+    }
+    for (int z = z0; z < z1; ++z) {
+        const int zpi = min(z + 1, nz - 1);
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) zp[j][i] = pc[at(zpi, y0 + j, x0 + i)];
+        const bool zint = (z >= 1) && (z <= nz - 2);
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            const int y = y0 + j;
+            const bool yint = zint && (y >= 1) && (y <= ny - 2);
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                if (x < nx && y < ny) {
+                    const int c = (z * ny + y) * nx + x;
+                    float lu = 0.0f;
+                    if (yint && x >= 1 && x <= nx - 2)
+                        lu = lap7(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
+                                  zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
+                    float inc = smul(sp2[c], lu);
+                    if (c == my_src) inc = inc + my_inc;
+                    const size_t a = (size_t)c * BP + b;
+                    const float v = leap(zcu[j + 1][i + 1], pp[a], inc);
+                    pn[a] = v;
+                    if (REC) {
+                        for (int r = rec_head[c]; r >= 0; r = rec_next[r]) rec[(size_t)r * BP + b] = v;
+                    }
+                }
+            }
+        }
+        if (z + 1 < z1) {
+#pragma unroll
+            for (int j = 0; j < CY; ++j)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    zm[j][i] = zcu[j + 1][i + 1];
+                    zcu[j + 1][i + 1] = zp[j][i];
+                }
+            load_halo(z + 1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched 3D adjoint step with the imaging condition (no sponge in 3D:
+// the reference sponge is 2D only, solver.py:30).
+// ---------------------------------------------------------------------------
+template <int LB, int CX, int CY>
+__global__ void __launch_bounds__(kBlock)
+k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
+        const float *__restrict__ pa, const float2 *__restrict__ sp2,
+        const int *__restrict__ src_cell, const float *__restrict__ sinc,
+        const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+        const float *__restrict__ res, float *__restrict__ g_hi, float *__restrict__ g_lo,
+        int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+    constexpr int W = kWave / LB;
+    constexpr int CC = CX * CY;
+    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    int t = wv / ng;
+    const int xs = t % nxs;
+    t /= nxs;
+    const int ys = t % nys, zc = t / nys;
+    if (zc >= nzc) return;  // uniform over the wavefront
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = ys * CY;
+    const int z0 = zc * RZ;
+    const int z1 = min(z0 + RZ, nz);
+    const size_t BP = (size_t)Bp;
+    const size_t N = (size_t)nz * ny * nx;
+    const int my_src = src_cell[b];
+    const float my_inc = sinc[b];
+    auto cl = [&](int z, int y, int x) -> int {
+        x = min(max(x, 0), nx - 1);
+        y = min(max(y, 0), ny - 1);
+        return (z * ny + y) * nx + x;
+    };
+    auto ldq = [&](int z, int y, int x) -> float {
+        const int c = cl(z, y, x);
+        return smul(sp2[c], l1[(size_t)c * BP + b]);
+    };
+    auto ldp = [&](int z, int y, int x) -> float { return pa[(size_t)cl(z, y, x) * BP + b]; };
+    float qm[CY][CX], qc[CY + 2][CX + 2], qp[CY][CX], lc[CY][CX], lpn[CY][CX];
+    float fm[CY][CX], fc[CY + 2][CX + 2], fp[CY][CX];
+    auto load_halo = [&](int z) {
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            qc[0][i + 1] = ldq(z, y0 - 1, x0 + i);
+            qc[CY + 1][i + 1] = ldq(z, y0 + CY, x0 + i);
+            fc[0][i + 1] = ldp(z, y0 - 1, x0 + i);
+            fc[CY + 1][i + 1] = ldp(z, y0 + CY, x0 + i);
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            qc[j + 1][0] = ldq(z, y0 + j, x0 - 1);
+            qc[j + 1][CX + 1] = ldq(z, y0 + j, x0 + CX);
+            fc[j + 1][0] = ldp(z, y0 + j, x0 - 1);
+            fc[j + 1][CX + 1] = ldp(z, y0 + j, x0 + CX);
+        }
+    };
+    {
+        const int zmi = max(z0 - 1, 0);
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                qm[j][i] = ldq(zmi, y0 + j, x0 + i);
+                fm[j][i] = ldp(zmi, y0 + j, x0 + i);
+                const int c = cl(z0, y0 + j, x0 + i);
+                const float v = l1[(size_t)c * BP + b];
+                lc[j][i] = v;
+                qc[j + 1][i + 1] = smul(sp2[c], v);
+                fc[j + 1][i + 1] = pa[(size_t)c * BP + b];
+            }
+        load_halo(z0);
+    }
+    for (int z = z0; z < z1; ++z) {
+        const int zpi = min(z + 1, nz - 1);
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int c = cl(zpi, y0 + j, x0 + i);
+                const float v = l1[(size_t)c * BP + b];
+                lpn[j][i] = v;
+                qp[j][i] = smul(sp2[c], v);
+                fp[j][i] = pa[(size_t)c * BP + b];
+            }
+        const bool zint = (z >= 1) && (z <= nz - 2);
+        float tv[CC];
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            const int y = y0 + j;
+            const bool yint = zint && (y >= 1) && (y <= ny - 2);
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                tv[j * CX + i] = 0.0f;
+                if (x < nx && y < ny) {
+                    const int c = (z * ny + y) * nx + x;
+                    const bool in = yint && x >= 1 && x <= nx - 2;
+                    const float lq = in ? lap7(qc[j + 1][i + 1], qp[j][i], qm[j][i], qc[j + 2][i + 1],
+                                               qc[j][i + 1], qc[j + 1][i + 2], qc[j + 1][i])
+                                        : 0.0f;
+                    const size_t a = (size_t)c * BP + b;
+                    float v = leap(lc[j][i], l2[a], lq);
+                    for (int r = rec_head[c]; r >= 0; r = rec_next[r]) v = v - res[(size_t)r * BP + b];
+                    lk[a] = v;
+                    const float lp = in ? lap7(fc[j + 1][i + 1], fp[j][i], fm[j][i], fc[j + 2][i + 1],
+                                               fc[j][i + 1], fc[j + 1][i + 2], fc[j + 1][i])
+                                        : 0.0f;
+                    float d2 = smul(sp2[c], lp);
+                    if (c == my_src) d2 = d2 + my_inc;
+                    tv[j * CX + i] = v * d2;
+                }
+            }
+        }
+        const float s = reduce_scatter<LB, CC>(tv, lane);
+        if ((sb & (LB / CC - 1)) == 0) {
+            const int ci = sum_cell<LB, CC>(lane);
+            const int x = x0 + ci % CX, y = y0 + ci / CX;
+            if (x < nx && y < ny) {
+                const size_t gi = (size_t)sg * N + ((size_t)z * ny + y) * nx + x;
+                float hi = g_hi[gi], lo = g_lo[gi];
+                neumaier(hi, lo, s);
+                g_hi[gi] = hi;
+                g_lo[gi] = lo;
+            }
+        }
+        if (z + 1 < z1) {
+#pragma unroll
+            for (int j = 0; j < CY; ++j)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    qm[j][i] = qc[j + 1][i + 1];
+                    qc[j + 1][i + 1] = qp[j][i];
+                    lc[j][i] = lpn[j][i];
+                    fm[j][i] = fc[j + 1][i + 1];
+                    fc[j + 1][i + 1] = fp[j][i];
+                }
+            load_halo(z + 1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain (one shot) kernels: one thread per cell, dimension-general (2D: nz = 1).
+// ---------------------------------------------------------------------------
+template <bool SPONGE, bool REC>
+__global__ void __launch_bounds__(kBlock)
+k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
+            const float2 *__restrict__ sp2, const float *__restrict__ sponge, int src,
+            const float *__restrict__ sinc, const int *__restrict__ rec_head,
+            const int *__restrict__ rec_next, float *__restrict__ rec, int nz, int ny, int nx,
+            int ndim) {
+    const int64_t N = (int64_t)nz * ny * nx;
+    const int64_t c64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c64 >= N) return;
+    const int c = (int)c64;
+    const int x = c % nx;
+    const int y = (c / nx) % ny;
+    const int z = c / (nx * ny);
+    float lu = 0.0f;
+    const float pcv = pc[c];
+    if (ndim == 2) {
+        if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+            lu = lap5(pcv, pc[c + nx], pc[c - nx], pc[c + 1], pc[c - 1]);
+    } else {
+        const int sz = nx * ny;
+        if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+            lu = lap7(pcv, pc[c + sz], pc[c - sz], pc[c + nx], pc[c - nx], pc[c + 1], pc[c - 1]);
+    }
+    float inc = smul(sp2[c], lu);
+    if (c == src) inc = inc + *sinc;
+    float v = leap(pcv, pp[c], inc);
+    if (SPONGE) v = v * sponge[c];
+    pn[c] = v;
+    if (REC) {
+        for (int j = rec_head[c]; j >= 0; j = rec_next[j]) rec[j] = v;
+    }
+}
+
+template <bool SPONGE>
+__global__ void __launch_bounds__(kBlock)
+k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
+            const float *__restrict__ pa, const float *__restrict__ pb,
+            const float *__restrict__ pc2, const float2 *__restrict__ sp2,
+            const float *__restrict__ sponge, int src, const float *__restrict__ sinc,
+            const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+            const float *__restrict__ res, float *__restrict__ g_hi, float *__restrict__ g_lo,
+            int nz, int ny, int nx, int ndim) {
+    const int64_t N = (int64_t)nz * ny * nx;
+    const int64_t c64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c64 >= N) return;
+    const int c = (int)c64;
+    const int x = c % nx;
+    const int y = (c / nx) % ny;
+    const int z = c / (nx * ny);
+    bool in;
+    float lq = 0.0f, lp = 0.0f;
+    auto q = [&](int cc) { return smul(sp2[cc], l1[cc]); };
+    if (ndim == 2) {
+        in = (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2);
+        if (in) {
+            lq = lap5(q(c), q(c + nx), q(c - nx), q(c + 1), q(c - 1));
+            if (!SPONGE) lp = lap5(pa[c], pa[c + nx], pa[c - nx], pa[c + 1], pa[c - 1]);
+        }
+    } else {
+        const int sz = nx * ny;
+        in = (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2);
+        if (in) {
+            lq = lap7(q(c), q(c + sz), q(c - sz), q(c + nx), q(c - nx), q(c + 1), q(c - 1));
+            if (!SPONGE)
+                lp = lap7(pa[c], pa[c + sz], pa[c - sz], pa[c + nx], pa[c - nx], pa[c + 1], pa[c - 1]);
+        }
+    }
+    float v = leap(l1[c], l2[c], lq);
+    if (SPONGE) v = v * sponge[c];
+    for (int j = rec_head[c]; j >= 0; j = rec_next[j]) v = v - res[j];
+    lk[c] = v;
+    float d2;
+    if (SPONGE) {
+        d2 = (pa[c] - 2.0f * pb[c]) + pc2[c];
+    } else {
+        d2 = smul(sp2[c], lp);
+        if (c == src) d2 = d2 + *sinc;
+    }
+    float hi = g_hi[c], lo = g_lo[c];
+    neumaier(hi, lo, v * d2);
+    g_hi[c] = hi;
+    g_lo[c] = lo;
+}
+
+// ---------------------------------------------------------------------------
+// Set-up, residual, reductions and layout kernels.
+// ---------------------------------------------------------------------------
+
+// S = dt^2/m (fwi.py:52), S' = S / h^2 as an FP32 hi/lo pair.
+__global__ void k_setup_s(const double *__restrict__ m, float2 *__restrict__ sp2, int64_t N,
+                          double dt2, double inv_h2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double s = (dt2 / m[i]) * inv_h2;
+    const float hi = (float)s;
+    sp2[i] = make_float2(hi, (float)(s - (double)hi));
+}
+
+// sinc[n][b] = S(src_b) * w[n-1] for n >= 1 (0 for padded shots and n = 0).
+__global__ void k_setup_sinc(const double *__restrict__ m, const double *__restrict__ w,
+                             const int *__restrict__ src_cell, float *__restrict__ sinc, int nt,
+                             int Bp, double dt2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nt * Bp) return;
+    const int n = (int)(i / Bp), b = (int)(i % Bp);
+    const int s = src_cell[b];
+    float v = 0.0f;
+    if (n >= 1 && s >= 0) v = (float)((dt2 / m[s]) * w[n - 1]);
+    sinc[i] = v;
+}
+
+__global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// dobsT[n][j][b] = dobs[b0 + b][n][j] for b < nb, 0 for padded shots.
+__global__ void k_dobs_to_batched(const float *__restrict__ dobs, float *__restrict__ dT, int nt,
+                                  int n_rec, int Bp, int b0, int nb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = (int64_t)nt * n_rec * Bp;
+    if (i >= tot) return;
+    const int b = (int)(i % Bp);
+    const int64_t nj = i / Bp;
+    dT[i] = b < nb ? dobs[((int64_t)(b0 + b) * nt) * n_rec + nj] : 0.0f;
+}
+
+// data[b0 + b][n][j] = rec[n][j][b] for b < nb.
+__global__ void k_batched_to_data(const float *__restrict__ rec, float *__restrict__ data, int nt,
+                                  int n_rec, int Bp, int b0, int nb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t tot = (int64_t)nb * nt * n_rec;
+    if (i >= tot) return;
+    const int64_t nj = i % ((int64_t)nt * n_rec);
+    const int b = (int)(i / ((int64_t)nt * n_rec));
+    data[(int64_t)(b0 + b) * nt * n_rec + nj] = rec[nj * Bp + b];
+}
+
+// L2 residual r = dsyn - dobs over `count` entries (padded entries are 0 - 0),
+// and per-block Neumaier partial sums of r^2 (fwi.py:155-157).
+__global__ void __launch_bounds__(kBlock)
+k_residual_l2(const float *__restrict__ dsyn, const float *__restrict__ dobs,
+              float *__restrict__ res, int64_t count, float2 *__restrict__ part) {
+    float s = 0.0f, c = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float r = dsyn[i] - dobs[i];
+        if (res) res[i] = r;
+        neumaier(s, c, r * r);
+    }
+    __shared__ float sh[kBlock], ch[kBlock];
+    sh[threadIdx.x] = s;
+    ch[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = kBlock / 2; off >= 1; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            float a = sh[threadIdx.x], ac = ch[threadIdx.x] + ch[threadIdx.x + off];
+            neumaier(a, ac, sh[threadIdx.x + off]);
+            sh[threadIdx.x] = a;
+            ch[threadIdx.x] = ac;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = make_float2(sh[0], ch[0]);
+}
+
+// J = scale * sum of the partials (fixed order, FP64 at the end).
+__global__ void k_reduce_partials(const float2 *__restrict__ part, int n, double scale,
+                                  double *__restrict__ J, int accumulate) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float s = 0.0f, c = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        neumaier(s, c, part[i].x);
+        c = c + part[i].y;
+    }
+    const double v = scale * ((double)s + (double)c);
+    if (accumulate) *J += v;
+    else *J = v;
+}
+
+// GCN per-trace statistics over time (fwi.py:160-164): for trace t = (b, j)
+// stats[t] = (|s|^2, |o|^2, <s,o>) in FP64; layout of traces [n][j][b] with
+// stride n_rec*Bp between time samples.
+__global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restrict__ dobs,
+                            double *__restrict__ stats, int nt, int64_t ntr) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntr) return;
+    double ss = 0.0, oo = 0.0, so = 0.0;
+    for (int n = 0; n < nt; ++n) {
+        const double s = dsyn[(int64_t)n * ntr + t], o = dobs[(int64_t)n * ntr + t];
+        ss += s * s;
+        oo += o * o;
+        so += s * o;
+    }
+    stats[3 * t] = ss;
+    stats[3 * t + 1] = oo;
+    stats[3 * t + 2] = so;
+}
+
+// GCN adjoint source res = -(o/no - c s/ns)/ns and per-trace misfit 1 - c
+// (fwi.py:164-167); tmask[t] = 0 for padded traces.
+__global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__restrict__ dobs,
+                               const double *__restrict__ stats, const unsigned char *__restrict__ tmask,
+                               float *__restrict__ res, int nt, int64_t ntr) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nt * ntr) return;
+    const int64_t t = i % ntr;
+    if (!tmask[t]) { res[i] = 0.0f; return; }
+    const double eps = 1e-12;
+    const double ns = sqrt(stats[3 * t]) + eps, no = sqrt(stats[3 * t + 1]) + eps;
+    const double c = stats[3 * t + 2] / (ns * no);
+    res[i] = (float)(-((double)dobs[i] / no - c * ((double)dsyn[i] / ns)) / ns);
+}
+
+__global__ void k_gcn_misfit(const double *__restrict__ stats, const unsigned char *__restrict__ tmask,
+                             int64_t ntr, double *__restrict__ J, int accumulate) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double eps = 1e-12;
+    double s = 0.0, comp = 0.0;
+    for (int64_t t = 0; t < ntr; ++t) {
+        if (!tmask[t]) continue;
+        const double ns = sqrt(stats[3 * t]) + eps, no = sqrt(stats[3 * t + 1]) + eps;
+        const double x = 1.0 - stats[3 * t + 2] / (ns * no);
+        const double y = s + x;
+        comp += (fabs(s) >= fabs(x)) ? (s - y) + x : (x - y) + s;
+        s = y;
+    }
+    if (accumulate) *J += s + comp;
+    else *J = s + comp;
+}
+
+// grad[c] = (sum_g hi + lo) / m[c]   (fwi.py:331), FP64.
+__global__ void k_finalize_grad(const float *__restrict__ g_hi, const float *__restrict__ g_lo,
+                                int ng, int64_t N, const double *__restrict__ m,
+                                double *__restrict__ grad, int accumulate) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    double s = 0.0;
+    for (int g = 0; g < ng; ++g) s += (double)g_hi[(int64_t)g * N + i] + (double)g_lo[(int64_t)g * N + i];
+    const double v = s / m[i];
+    grad[i] = accumulate ? grad[i] + v : v;
+}
+
+}  // namespace usct

@@ -1,0 +1,279 @@
+"""Drop-in solver module with the reference's function signatures.
+
+``import openusct_amd.fwi as fwi`` replaces ``from ringfwi import fwi`` (and the
+C++ ``uap`` module, software/python/uap.py:1-67) for the acoustic path; the
+module itself is also a ``backend=`` for the reference's ``invert``
+(simulation/ringfwi/fwi.py:594-603), which calls
+``backend.misfit_and_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge,
+src_list, misfit_type=...)`` and ``backend.misfit(...)`` positionally.
+
+Inputs are NumPy arrays (float64 models, as in the reference) or torch
+tensors; results come back as NumPy float64 like ``ringfwi.fwi``.  Every
+call runs on the HIP device through libusct; there is no CPU path.
+
+Semantics (fwi.py:34-119, 246-332): the source sample used to advance to level
+n is ``wavelet[n-1]``; receivers record ``p^n`` for n >= 1 with row 0 = 0;
+``src_list`` indexes elements and every element receives; the sponge is
+applied (unlike the reference's C++ and CuPy backends, which ignore it,
+uap.py:32-33); duplicate receivers accumulate in the adjoint; ``g`` is the
+gradient with respect to m (squared slowness).
+"""
+
+from __future__ import annotations
+
+from collections import OrderedDict
+
+import numpy as np
+import torch
+
+from .plan import Plan, linear_indices
+
+# Backend selection for the drop-in functions: "batched" (default) or "plain".
+METHOD = "batched"
+_CACHE_SIZE = 2
+_plans: "OrderedDict[tuple, Plan]" = OrderedDict()
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("openusct_amd needs a HIP device; none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def check_cfl(m, dt, h):
+    """Reject time steps beyond the leapfrog stability limit
+    c_max dt / h <= 1/sqrt(ndim) (simulation/docs/theory.md:24)."""
+    m = m.detach().cpu().numpy() if isinstance(m, torch.Tensor) else np.asarray(m)
+    if m.size == 0 or not np.all(np.isfinite(m)) or np.min(m) <= 0:
+        raise ValueError("m must be finite and positive (squared slowness)")
+    c_max = float(1.0 / np.sqrt(np.min(m)))
+    lim = 1.0 / np.sqrt(m.ndim)
+    if c_max * dt / h > lim * (1.0 + 1e-12):
+        raise ValueError(f"CFL violated: c_max*dt/h = {c_max * dt / h:.4f} > {lim:.4f}")
+
+
+def _tx_rec(geom, shape, src_list):
+    if not hasattr(geom, "idx"):
+        raise TypeError("geom must provide .idx, .n_elements and .element_index(k)")
+    rec = linear_indices(geom.idx, shape)
+    src = list(range(geom.n_elements)) if src_list is None else list(src_list)
+    for s in src:
+        if not 0 <= int(s) < geom.n_elements:
+            raise ValueError(f"src_list entry {s} is not an element of the array")
+    tx = np.array([linear_indices([geom.element_index(int(s))], shape)[0] for s in src],
+                  dtype=np.int64)
+    return tx, rec
+
+
+def get_plan(shape, h, dt, nt, tx, rec, sponge, method=None):
+    """Plan for this problem, reused across calls (line searches, iterations)."""
+    method = method or METHOD
+    key = (tuple(shape), float(h), float(dt), int(nt), tx.tobytes(), rec.tobytes(), bool(sponge),
+           method, torch.cuda.current_device())
+    plan = _plans.get(key)
+    if plan is not None:
+        _plans.move_to_end(key)
+        return plan
+    while len(_plans) >= _CACHE_SIZE:
+        _plans.popitem(last=False)[1].close()
+    plan = Plan(shape, h, dt, nt, tx, rec, method=method, sponge=sponge)
+    _plans[key] = plan
+    return plan
+
+
+def clear_plans():
+    while _plans:
+        _plans.popitem()[1].close()
+
+
+def _to_dev(a, dtype):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=_device(), dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=_device())
+
+
+def _shape_of(m):
+    return tuple(m.shape)
+
+
+def _setup(m, geom, wavelet, dt, h, nt, sponge, src_list, method=None):
+    shape = _shape_of(m)
+    if len(shape) not in (2, 3):
+        raise ValueError(f"m must be 2D or 3D, got shape {shape}")
+    nt = int(nt)
+    check_cfl(m, dt, h)
+    if len(wavelet) < nt:
+        raise ValueError(f"wavelet has {len(wavelet)} samples, nt = {nt}")
+    if sponge is not None and len(shape) != 2:
+        raise NotImplementedError("the sponge is defined for 2D grids only")
+    tx, rec = _tx_rec(geom, shape, src_list)
+    plan = get_plan(shape, h, dt, nt, tx, rec, sponge is not None, method)
+    md = _to_dev(m, torch.float64)
+    wd = _to_dev(np.asarray(wavelet, dtype=np.float64)[:nt], torch.float64)
+    sd = None if sponge is None else _to_dev(sponge, torch.float64)
+    return plan, md, wd, sd
+
+
+def forward_fmc(m, geom, wavelet, dt, h, nt, sponge=None, src_list=None, footprints=None):
+    """Full-matrix capture (fwi.py:102-119): data (n_src, nt, n_elements) float64."""
+    if footprints is not None:
+        raise NotImplementedError("finite-aperture footprints are not implemented on the device")
+    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list)
+    d = plan.forward_fmc(md, wd, sd)
+    return d.double().cpu().numpy()
+
+
+def _dobs_dev(dobs, plan):
+    want = (plan.n_tx, plan.nt, plan.n_rec)
+    if tuple(np.shape(dobs)) != want:
+        raise ValueError(f"dobs has shape {tuple(np.shape(dobs))}, expected {want}")
+    return _to_dev(dobs, torch.float32)
+
+
+def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, misfit_type="l2",
+           footprints=None, time_weights=None):
+    """Misfit only (fwi.py:246-263)."""
+    if footprints is not None or time_weights is not None:
+        raise NotImplementedError("footprints / time_weights are not implemented on the device")
+    if misfit_type not in ("l2", "gcn"):
+        raise NotImplementedError(f"misfit_type {misfit_type!r} is not implemented on the device")
+    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list)
+    J = plan.misfit(md, wd, _dobs_dev(dobs, plan), sd, misfit_type)
+    return float(J.item())
+
+
+def misfit_and_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None,
+                        misfit_type="l2", footprints=None, time_weights=None):
+    """Misfit and exact discrete-adjoint gradient w.r.t. m (fwi.py:266-332)."""
+    if footprints is not None or time_weights is not None:
+        raise NotImplementedError("footprints / time_weights are not implemented on the device")
+    if misfit_type not in ("l2", "gcn"):
+        raise NotImplementedError(f"misfit_type {misfit_type!r} is not implemented on the device")
+    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list)
+    J, g = plan.misfit_and_gradient(md, wd, _dobs_dev(dobs, plan), sd, misfit_type)
+    return float(J.item()), g.cpu().numpy()
+
+
+def simulate_wavefield(m, geom, src_element, wavelet, dt, h, nt, sponge=None):
+    """Full pressure history of one transmit (fwi.py:92-99), computed level by
+    level on the device with one-level forward runs of growing length."""
+    shape = _shape_of(m)
+    nt = int(nt)
+    check_cfl(m, dt, h)
+    cells = linear_indices([geom.element_index(int(src_element))], shape)
+    # every grid cell is a receiver: the traces are the history
+    rec = np.arange(int(np.prod(shape)), dtype=np.int64)
+    plan = Plan(shape, h, dt, nt, cells, rec, method="plain", sponge=sponge is not None)
+    try:
+        d = plan.forward_fmc(_to_dev(m, torch.float64),
+                             _to_dev(np.asarray(wavelet, np.float64)[:nt], torch.float64),
+                             None if sponge is None else _to_dev(sponge, torch.float64))
+    finally:
+        plan.close()
+    return d[0].double().cpu().numpy().reshape((nt,) + shape)
+
+
+def invert(m0, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, n_iter=15,
+           step_frac=0.02, update_mask=None, m_bounds=None, smooth_sigma=1.0, n_linesearch=8,
+           verbose=False, backend=None, misfit_type="l2", footprints=None, progress=None,
+           time_weights=None, optimizer="gd", n_cg=8, gn_damp=1e-2):
+    """Preconditioned steepest descent with backtracking line search
+    (fwi.py:535-667), every misfit and gradient evaluated on the device.
+
+    Steps per iteration: mask, Gaussian-smooth (scipy, as the reference), step
+    = step_frac * mean(m) / max|g|, halve up to n_linesearch times until the
+    misfit drops, clip to the effective bounds inside the update mask.
+    """
+    from scipy.ndimage import gaussian_filter
+
+    if optimizer not in ("gd",):
+        raise NotImplementedError("only optimizer='gd' runs on the device")
+    be = backend if backend is not None else _THIS
+    mg = lambda mm: be.misfit_and_gradient(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,
+                                           misfit_type=misfit_type)
+    mf = lambda mm: be.misfit(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,
+                              misfit_type=misfit_type)
+    m = np.array(m0, dtype=np.float64, copy=True)
+    lo = hi = None
+    if m_bounds is not None:
+        lo = np.minimum(m_bounds[0], m0)
+        hi = np.maximum(m_bounds[1], m0)
+    history = []
+    J, g = mg(m)
+    for it in range(n_iter):
+        history.append(J)
+        if verbose:
+            print(f"  iter {it:2d}  misfit = {J:.6e}")
+        d = g * update_mask if update_mask is not None else g.copy()
+        if smooth_sigma:
+            d = gaussian_filter(d, smooth_sigma)
+            if update_mask is not None:
+                d = d * update_mask
+        dmax = float(np.max(np.abs(d)))
+        if dmax == 0.0:
+            break
+        step = step_frac * float(np.mean(m)) / dmax
+        accepted = False
+        for _ in range(n_linesearch):
+            trial = m - step * d
+            if m_bounds is not None:
+                clipped = np.clip(trial, lo, hi)
+                trial = np.where(update_mask > 0, clipped, trial) if update_mask is not None \
+                    else clipped
+            Jt = mf(trial)
+            if Jt < J:
+                accepted = True
+                break
+            step *= 0.5
+        if not accepted:
+            if verbose:
+                print("  line search failed to reduce misfit; stopping")
+            break
+        m = trial
+        J, g = mg(m)
+        if progress is not None:
+            progress(it + 1, n_iter)
+    history.append(J)
+    return m, history
+
+
+def fwi_from_dataset(dataset, start_c=None, update_frac=0.95, m_bounds=None, n_iter=12,
+                     step_frac=0.03, verbose=False, **invert_kw):
+    """FWI on a Dataset-like object (fwi.py:702-774): grid from its ground truth
+    (else 5 points per wavelength), element positions re-mapped to grid indices,
+    update region a disc (2D) or cylinder (3D) of update_frac x array radius."""
+    from .geometry import GridArray, remap_positions
+    from .phantom import m_to_velocity, velocity_to_m
+
+    ds = dataset
+    dim = ds.geometry.dim
+    if ds.ground_truth is not None:
+        h = ds.ground_truth["h_m"]
+        shape = ds.ground_truth["c"].shape
+    else:
+        h = ds.nominal_speed_m_s / (ds.tx_centre_freq_hz * 5.0)
+        npix = int(round(2.2 * ds.geometry.radius_m / h)) + 1
+        shape = (npix,) * dim
+    n = shape[0]
+    domain = (n - 1) * h
+    idx = remap_positions(ds.geometry.element_pos, n, h)
+    geom = GridArray(idx=idx, n=n, h=h, radius_m=ds.geometry.radius_m, domain_m=domain)
+    if start_c is None:
+        start_c = np.full(shape, ds.nominal_speed_m_s)
+    m0 = velocity_to_m(start_c)
+    centre = domain / 2.0
+    coords = np.mgrid[tuple(slice(0, n) for _ in range(dim))].astype(float) * h
+    axes = range(dim) if dim == 2 else range(1, dim)
+    r = np.sqrt(sum((coords[a] - centre) ** 2 for a in axes))
+    update_mask = (r <= ds.geometry.radius_m * update_frac).astype(float)
+    m_rec, history = invert(m0, geom, ds.tx_wavelet, ds.dt, h, ds.n_samples, ds.data,
+                            src_list=list(ds.tx_elements), n_iter=n_iter, step_frac=step_frac,
+                            update_mask=update_mask, m_bounds=m_bounds, verbose=verbose,
+                            **invert_kw)
+    return {"c": m_to_velocity(m_rec), "history": history, "extent": (-centre, domain - centre),
+            "h": h, "geom": geom}
+
+
+import sys as _sys  # noqa: E402
+
+_THIS = _sys.modules[__name__]

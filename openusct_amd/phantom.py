@@ -1,0 +1,108 @@
+"""Velocity phantoms and the c <-> m conversion (simulation/ringfwi/phantom.py).
+
+``velocity_to_m`` / ``m_to_velocity`` (phantom.py:13-18), the disc and cylinder
+backgrounds (phantom.py:21-33, 166-177) and circular / spherical inclusions
+(phantom.py:36-48, 191-204) follow the reference's float64 expressions.  The
+ellipsoid inclusion and the seeded inclusion sets are this build's workload
+generators (the reference has spheres only).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def velocity_to_m(c):
+    return 1.0 / (c * c)
+
+
+def m_to_velocity(m):
+    return 1.0 / np.sqrt(m)
+
+
+def coupling_background(shape, c_specimen, c_couplant, radius_m, h):
+    n = shape[0]
+    centre = (n - 1) * h / 2.0
+    y, x = np.mgrid[0:n, 0:n] * h
+    r = np.hypot(x - centre, y - centre)
+    return np.where(r <= radius_m, c_specimen, c_couplant).astype(float)
+
+
+def add_inclusion(c, centre_frac, radius_m, c_inclusion, h):
+    n = c.shape[0]
+    domain = (n - 1) * h
+    cx, cy = centre_frac[0] * domain, centre_frac[1] * domain
+    y, x = np.mgrid[0:n, 0:n] * h
+    r = np.hypot(x - cx, y - cy)
+    out = c.copy()
+    out[r <= radius_m] = c_inclusion
+    return out
+
+
+def cylinder_background(shape, c_specimen, c_couplant, radius_m, h):
+    nz, ny, nx = shape
+    cx = (nx - 1) * h / 2.0
+    cy = (ny - 1) * h / 2.0
+    z, y, x = np.mgrid[0:nz, 0:ny, 0:nx].astype(float) * h
+    r = np.hypot(x - cx, y - cy)
+    return np.where(r <= radius_m, c_specimen, c_couplant)
+
+
+def add_sphere(c, centre_frac, radius_m, c_inclusion, h):
+    nz, ny, nx = c.shape
+    cx = centre_frac[0] * (nx - 1) * h
+    cy = centre_frac[1] * (ny - 1) * h
+    cz = centre_frac[2] * (nz - 1) * h
+    z, y, x = np.mgrid[0:nz, 0:ny, 0:nx].astype(float) * h
+    r = np.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2)
+    out = c.copy()
+    out[r <= radius_m] = c_inclusion
+    return out
+
+
+def add_ellipsoid(c, centre_m, semi_axes_m, c_inclusion, h):
+    """Axis-aligned ellipsoid inclusion; centre (x, y, z) and semi-axes in metres."""
+    nz, ny, nx = c.shape
+    z, y, x = np.mgrid[0:nz, 0:ny, 0:nx].astype(float) * h
+    q = sum(((v - c0) / a) ** 2 for v, c0, a in zip((x, y, z), centre_m, semi_axes_m))
+    out = c.copy()
+    out[q <= 1.0] = c_inclusion
+    return out
+
+
+def seeded_discs(n, h, seed, count, c_bg=1500.0, r_mm=(3.0, 8.0), c_range=(1400.0, 1800.0),
+                 inside_mm=16.0):
+    """Background c_bg with ``count`` circular inclusions drawn from default_rng(seed):
+    radius ~ U(r_mm) mm, c ~ U(c_range), centre uniform in the disc of radius
+    (inside_mm - radius) about the grid centre; applied in order (later wins)."""
+    rng = np.random.default_rng(seed)
+    c = np.full((n, n), float(c_bg))
+    domain = (n - 1) * h
+    for _ in range(count):
+        r = rng.uniform(*r_mm) * 1e-3
+        cv = rng.uniform(*c_range)
+        rho = (inside_mm * 1e-3 - r) * np.sqrt(rng.uniform())
+        th = rng.uniform(0.0, 2.0 * np.pi)
+        fx = (domain / 2.0 + rho * np.cos(th)) / domain
+        fy = (domain / 2.0 + rho * np.sin(th)) / domain
+        c = add_inclusion(c, (fx, fy), r, cv, h)
+    return c
+
+
+def seeded_ellipsoids(n, h, seed, count, c_bg=1500.0, a_mm=(3.0, 8.0),
+                      c_range=(1400.0, 1800.0), inside_mm=20.0):
+    """Cubic background with ``count`` ellipsoids from default_rng(seed): semi-axes
+    ~ U(a_mm) mm, c ~ U(c_range), centred inside a cylinder of radius
+    (inside_mm - max semi-axis) and the same half-height about the grid centre."""
+    rng = np.random.default_rng(seed)
+    c = np.full((n, n, n), float(c_bg))
+    mid = (n - 1) * h / 2.0
+    for _ in range(count):
+        ax = rng.uniform(*a_mm, size=3) * 1e-3
+        cv = rng.uniform(*c_range)
+        lim = inside_mm * 1e-3 - ax.max()
+        rho = lim * np.sqrt(rng.uniform())
+        th = rng.uniform(0.0, 2.0 * np.pi)
+        zc = rng.uniform(-lim, lim)
+        c = add_ellipsoid(c, (mid + rho * np.cos(th), mid + rho * np.sin(th), mid + zc), ax, cv, h)
+    return c

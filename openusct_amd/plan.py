@@ -1,0 +1,191 @@
+"""Torch-tensor front end of a libusct plan.
+
+A :class:`Plan` fixes the grid, time axis, transmit cells and receiver cells of
+a full-matrix capture and holds the device workspace (fields, checkpoints,
+traces, the captured hipGraph).  Models, wavelets and observed data vary per
+call.  Tensors are device tensors; torch is used only for memory and streams.
+
+The three operations are the reference's drop-in trio
+(simulation/ringfwi/fwi.py:102-119 forward_fmc, fwi.py:246-263 misfit,
+fwi.py:266-332 misfit_and_gradient), with the same index semantics as the C++
+binding (software/python/uap.py:19-30: row-major linear cells, one shot per
+transmit cell, every receiver cell records).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_METHODS = {"auto": _lib.METHOD_AUTO, "plain": _lib.METHOD_PLAIN, "batched": _lib.METHOD_BATCHED}
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def linear_indices(idx, shape):
+    """Row-major linear cell of each grid index tuple (software/python/uap.py:19-22)."""
+    idx = np.asarray(idx, dtype=np.int64).reshape(-1, len(shape))
+    if idx.size and ((idx < 0).any() or (idx >= np.asarray(shape)).any()):
+        raise ValueError(f"element index outside the grid {tuple(shape)}")
+    return np.ravel_multi_index(tuple(idx.T), shape).astype(np.int64)
+
+
+class Plan:
+    """A reusable solver plan on the current HIP device."""
+
+    def __init__(self, shape, h, dt, nt, tx_lin, rec_lin, method="batched", ckpt_every=0,
+                 mem_budget=0, sponge=False, autotune=False, graph=True):
+        lib = _lib.load()
+        self.shape = tuple(int(s) for s in shape)
+        self.ndim = len(self.shape)
+        self.h, self.dt, self.nt = float(h), float(dt), int(nt)
+        self.tx = np.ascontiguousarray(tx_lin, dtype=np.int64).reshape(-1)
+        self.rec = np.ascontiguousarray(rec_lin, dtype=np.int64).reshape(-1)
+        self.n_tx, self.n_rec = len(self.tx), len(self.rec)
+        self.N = int(np.prod(self.shape))
+        self.sponge = bool(sponge)
+        if method not in _METHODS:
+            raise ValueError(f"unknown method {method!r}; expected one of {sorted(_METHODS)}")
+        flags = (_lib.FLAG_SPONGE if sponge else 0) | (_lib.FLAG_AUTOTUNE if autotune else 0) | \
+            (0 if graph else _lib.FLAG_NO_GRAPH)
+        dims = (ctypes.c_int * self.ndim)(*self.shape)
+        handle = ctypes.c_void_p()
+        torch.cuda.init()
+        rc = lib.usct_plan_create(
+            ctypes.byref(handle), self.ndim, dims, self.h, self.dt, self.nt,
+            self.tx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), self.n_tx,
+            self.rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), self.n_rec,
+            _METHODS[method], int(ckpt_every), int(mem_budget), flags)
+        _lib.check(rc)
+        self._h = handle
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    # -- lifetime ---------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().usct_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def info(self):
+        inf = _lib.PlanInfo()
+        _lib.check(_lib.load().usct_plan_get_info(self._h, ctypes.byref(inf)))
+        return inf.as_dict()
+
+    # -- argument checks --------------------------------------------------------
+    def _model(self, m):
+        if not isinstance(m, torch.Tensor) or m.device.type != "cuda":
+            raise TypeError("m must be a CUDA (HIP) torch tensor")
+        if tuple(m.shape) != self.shape:
+            raise ValueError(f"m has shape {tuple(m.shape)}, plan expects {self.shape}")
+        return m.to(torch.float64).contiguous()
+
+    def _wavelet(self, w):
+        w = w.to(device=self.device, dtype=torch.float64).contiguous().reshape(-1)
+        if w.numel() < self.nt:
+            raise ValueError(f"wavelet has {w.numel()} samples, nt = {self.nt}")
+        return w
+
+    def _sponge(self, sponge):
+        if self.sponge and sponge is None:
+            raise ValueError("plan was created for a sponge; pass the sponge mask")
+        if not self.sponge and sponge is not None:
+            raise ValueError("plan was created without a sponge")
+        if sponge is None:
+            return None
+        sponge = sponge.to(device=self.device, dtype=torch.float64).contiguous()
+        if tuple(sponge.shape) != self.shape:
+            raise ValueError(f"sponge has shape {tuple(sponge.shape)}, expected {self.shape}")
+        return sponge
+
+    def _dobs(self, dobs):
+        want = (self.n_tx, self.nt, self.n_rec)
+        if tuple(dobs.shape) != want:
+            raise ValueError(f"dobs has shape {tuple(dobs.shape)}, expected {want}")
+        return dobs.to(device=self.device, dtype=torch.float32).contiguous()
+
+    # -- operations -------------------------------------------------------------
+    def forward_fmc(self, m, wavelet, sponge=None, out=None, stream=None):
+        """Traces (n_tx, nt, n_rec) float32 on the device (row 0 is zero)."""
+        m, w, sp = self._model(m), self._wavelet(wavelet), self._sponge(sponge)
+        if out is None:
+            out = torch.empty((self.n_tx, self.nt, self.n_rec), dtype=torch.float32,
+                              device=self.device)
+        _lib.check(_lib.load().usct_forward_fmc(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(out),
+                                                _stream_handle(stream)))
+        return out
+
+    def misfit(self, m, wavelet, dobs, sponge=None, misfit_type="l2", stream=None):
+        """Misfit as a 1-element float64 device tensor."""
+        if misfit_type not in _lib.MISFIT_CODES:
+            raise ValueError(f"unknown misfit_type {misfit_type!r}")
+        m, w, sp, d = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(dobs)
+        J = torch.empty(1, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().usct_misfit(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d),
+                                           _lib.MISFIT_CODES[misfit_type], _ptr(J),
+                                           _stream_handle(stream)))
+        return J
+
+    def misfit_and_gradient(self, m, wavelet, dobs, sponge=None, misfit_type="l2", stream=None):
+        """(J, g): J a 1-element float64 tensor, g float64 with the model's shape."""
+        if misfit_type not in _lib.MISFIT_CODES:
+            raise ValueError(f"unknown misfit_type {misfit_type!r}")
+        m, w, sp, d = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(dobs)
+        J = torch.empty(1, dtype=torch.float64, device=self.device)
+        g = torch.empty(self.shape, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().usct_misfit_and_gradient(
+            self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d), _lib.MISFIT_CODES[misfit_type],
+            _ptr(J), _ptr(g), _stream_handle(stream)))
+        return J, g
+
+    def profile_gradient(self, m, wavelet, dobs, sponge=None, stream=None):
+        """Mean per-launch kernel time (ms) of forward and adjoint steps of one gradient."""
+        m, w, sp, d = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(dobs)
+        fwd, adj = ctypes.c_double(), ctypes.c_double()
+        nf, na = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().usct_profile_gradient(
+            self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d), ctypes.byref(fwd), ctypes.byref(adj),
+            ctypes.byref(nf), ctypes.byref(na), _stream_handle(stream)))
+        return {"fwd_ms": fwd.value, "adj_ms": adj.value, "n_fwd": nf.value, "n_adj": na.value}
+
+
+def footprint_bytes(shape, nt, n_tx, n_rec, method="batched", ckpt_every=0):
+    """Device bytes a plan would hold (same model as the C planner, whole history
+    when it fits, else K ~ sqrt(2 nt)); used by the serving loop's admission."""
+    N = int(np.prod(shape))
+    if method == "plain":
+        return 4 * N * (nt + 4) + 12 * nt * n_rec + 20 * N
+    lb = 64
+    for c in (64, 32, 16):
+        lb = c
+        pad = -(-n_tx // c) * c
+        if (pad - n_tx) * 8 <= pad:
+            break
+    Bp = -(-n_tx // lb) * lb
+    K = ckpt_every if ckpt_every > 0 else max(2, int(round((2.0 * max(nt - 1, 1)) ** 0.5)))
+    NS = max(1, -(-(nt - 1) // K))
+    fields = 4 + (K + 2) + 2 * max(NS - 2, 0)
+    return 4 * N * Bp * fields + 12 * nt * n_rec * Bp + 20 * N + 8 * N * (Bp // 16)

@@ -1,0 +1,160 @@
+"""Write the golden fixtures under tests/golden/ from the reference's Python path.
+
+Run in a container that holds the reference checkout (never on a GPU host):
+
+    python tests/golden/make_golden.py [/path/to/reference]
+
+It imports ``ringfwi`` from ``<reference>/simulation`` and records inputs and
+outputs of the functions whose behaviour only the reference itself pins:
+
+* the 2D sponge path (simulation/ringfwi/solver.py:22-42, fwi.py:68-69,313-314),
+  whose gradient is the reference's inexact formula;
+* element-index rounding of RingArray / CylinderArray (geometry.py:47-50,118-122)
+  and the dataset re-mapping of fwi_from_dataset (fwi.py:744-746);
+* phantom builders (phantom.py:13-48,166-204) and the Ricker wavelet
+  (sources.py:8-26);
+* forward / gradient on the reference's own test problems
+  (simulation/tests/test_gradient.py:20-40, test_gradient_3d.py:18-37), with
+  duplicate receivers (fwi.py:321-322 accumulates them) and the GCN misfit.
+
+The fixtures are data only (numpy .npz, no pickles).
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def main(ref_root):
+    sys.path.insert(0, os.path.join(ref_root, "simulation"))
+    from ringfwi import fwi, phantom  # noqa: E402
+    from ringfwi.geometry import CylinderArray, GridArray, RingArray  # noqa: E402
+    from ringfwi.solver import make_sponge  # noqa: E402
+    from ringfwi.sources import ricker  # noqa: E402
+    from ringfwi.acquire import simulate_dataset  # noqa: E402
+
+    # -- geometry: element indices (tie rounding) -----------------------------
+    rings = [(8, 0.024, 0.06, 1e-3), (16, 0.024, 0.060, 6e-4), (12, 0.016, 0.045, 1.25e-3),
+             (64, 0.022, 0.051, 2e-4), (128, 0.022, 0.1022, 2e-4),
+             (256, 0.044, 0.1023, 1e-4), (128, 0.044, 0.1023, 1e-4)]
+    cyls = [(2, 6, 0.010, 0.012, 0.03, 1.5e-3), (3, 12, 0.015, 0.014, 0.036, 1.25e-3),
+            (8, 64, 0.028, 0.028, 0.06375, 2.5e-4), (4, 32, 0.014, 0.014, 0.032, 2.5e-4)]
+    geo = {}
+    for i, (ne, r, d, h) in enumerate(rings):
+        g = RingArray(n_elements=ne, radius_m=r, domain_m=d, h=h)
+        geo[f"ring{i}_params"] = np.array([ne, r, d, h])
+        geo[f"ring{i}_n"] = np.array(g.n)
+        geo[f"ring{i}_idx"] = np.array(g.idx, dtype=np.int64)
+        geo[f"ring{i}_pos"] = g.element_positions
+    for i, (nr, pr, r, ht, d, h) in enumerate(cyls):
+        g = CylinderArray(n_rings=nr, per_ring=pr, radius_m=r, height_m=ht, domain_m=d, h=h)
+        geo[f"cyl{i}_params"] = np.array([nr, pr, r, ht, d, h])
+        geo[f"cyl{i}_n"] = np.array(g.n)
+        geo[f"cyl{i}_idx"] = np.array(g.idx, dtype=np.int64)
+        geo[f"cyl{i}_pos"] = g.element_positions
+    np.savez_compressed(os.path.join(HERE, "geometry.npz"), **geo)
+
+    # -- sources --------------------------------------------------------------
+    src = {}
+    for i, (nt, dt, f0, t0) in enumerate([(350, 1.2e-7, 0.4e6, None), (1000, 5e-8, 5e5, None),
+                                          (4000, 2.5e-8, 5e5, None), (300, 1e-7, 0.3e6, 2e-6)]):
+        src[f"ricker{i}_params"] = np.array([nt, dt, f0, np.nan if t0 is None else t0])
+        src[f"ricker{i}"] = ricker(nt, dt, f0, t0)
+    np.savez_compressed(os.path.join(HERE, "sources.npz"), **src)
+
+    # -- phantoms and sponge --------------------------------------------------
+    ph = {}
+    n, h = 61, 1e-3
+    c = phantom.coupling_background((n, n), 3000.0, 1480.0, 0.02, h)
+    ph["coupling"] = c
+    ph["inclusion"] = phantom.add_inclusion(c, (0.58, 0.46), 0.006, 2650.0, h)
+    n3, h3 = 21, 1.5e-3
+    c3 = phantom.cylinder_background((n3, n3, n3), 3000.0, 1480.0, 0.008, h3)
+    ph["cylinder"] = c3
+    ph["sphere"] = phantom.add_sphere(c3, (0.56, 0.5, 0.45), 0.004, 2700.0, h3)
+    ph["m_of_c"] = phantom.velocity_to_m(ph["inclusion"])
+    ph["c_of_m"] = phantom.m_to_velocity(ph["m_of_c"])
+    ph["sponge_61"] = make_sponge((61, 61))
+    ph["sponge_40x70"] = make_sponge((40, 70), width=10, strength=0.1)
+    np.savez_compressed(os.path.join(HERE, "phantom.npz"), **ph)
+
+    # -- 2D forward / gradient (test_gradient.py:20-40), with and without sponge
+    h = 1.0e-3
+    geom = RingArray(n_elements=8, radius_m=0.024, domain_m=0.06, h=h)
+    n = geom.n
+    dt, nt, f0 = 1.2e-7, 350, 0.4e6
+    wav = ricker(nt, dt, f0)
+    c_true = phantom.coupling_background((n, n), 3000.0, 1480.0, 0.02, h)
+    c_true = phantom.add_inclusion(c_true, (0.5, 0.5), 0.006, 3400.0, h)
+    m_true = phantom.velocity_to_m(c_true)
+    m0 = phantom.velocity_to_m(phantom.coupling_background((n, n), 3000.0, 1480.0, 0.02, h))
+    src_list = [0, 4]
+    out = dict(idx=np.array(geom.idx, dtype=np.int64), h=h, dt=dt, nt=nt, wavelet=wav,
+               m_true=m_true, m0=m0, src_list=np.array(src_list))
+    for tag, sp in (("nosponge", None), ("sponge", make_sponge((n, n)))):
+        dobs = fwi.forward_fmc(m_true, geom, wav, dt, h, nt, sp, src_list)
+        J, g = fwi.misfit_and_gradient(m0, geom, wav, dt, h, nt, dobs, sp, src_list)
+        out[f"{tag}_dobs"] = dobs
+        out[f"{tag}_J"] = np.array(J)
+        out[f"{tag}_g"] = g
+        out[f"{tag}_misfit"] = np.array(fwi.misfit(m0, geom, wav, dt, h, nt, dobs, sp, src_list))
+    Jg, gg = fwi.misfit_and_gradient(m0, geom, wav, dt, h, nt, out["nosponge_dobs"], None,
+                                     src_list, misfit_type="gcn")
+    out["gcn_J"] = np.array(Jg)
+    out["gcn_g"] = gg
+    np.savez_compressed(os.path.join(HERE, "fmc_2d.npz"), **out)
+
+    # -- duplicate receivers: two elements on one grid cell --------------------
+    idx = list(geom.idx) + [geom.idx[2], geom.idx[2]]
+    dup = GridArray(idx=idx, n=n, h=h, radius_m=0.024, domain_m=0.06)
+    dobs = fwi.forward_fmc(m_true, dup, wav, dt, h, nt, None, [2, 9])
+    J, g = fwi.misfit_and_gradient(m0, dup, wav, dt, h, nt, dobs, None, [2, 9])
+    np.savez_compressed(os.path.join(HERE, "dup_2d.npz"), idx=np.array(idx, dtype=np.int64),
+                        src_list=np.array([2, 9]), dobs=dobs, J=np.array(J), g=g)
+
+    # -- 3D forward / gradient (test_gradient_3d.py:18-37) --------------------
+    h = 1.5e-3
+    cyl = CylinderArray(n_rings=2, per_ring=6, radius_m=0.010, height_m=0.012,
+                        domain_m=0.03, h=h)
+    n = cyl.n
+    dt, nt = 1.5e-7, 140
+    wav = ricker(nt, dt, 0.35e6)
+    c_true = phantom.cylinder_background((n, n, n), 3000.0, 1480.0, 0.008, h)
+    c_true = phantom.add_sphere(c_true, (0.5, 0.5, 0.5), 0.004, 2700.0, h)
+    m_true = phantom.velocity_to_m(c_true)
+    m0 = phantom.velocity_to_m(phantom.cylinder_background((n, n, n), 3000.0, 1480.0, 0.008, h))
+    src_list = [0, 6]
+    dobs = fwi.forward_fmc(m_true, cyl, wav, dt, h, nt, src_list=src_list)
+    J, g = fwi.misfit_and_gradient(m0, cyl, wav, dt, h, nt, dobs, src_list=src_list)
+    np.savez_compressed(os.path.join(HERE, "fmc_3d.npz"), idx=np.array(cyl.idx, dtype=np.int64),
+                        h=h, dt=dt, nt=nt, wavelet=wav, m_true=m_true, m0=m0,
+                        src_list=np.array(src_list), dobs=dobs, J=np.array(J), g=g)
+
+    # -- dataset re-mapping in fwi_from_dataset (fwi.py:732-746) --------------
+    rm = {}
+    for i, (ne, r, d, hh) in enumerate([(12, 0.016, 0.045, 1.25e-3), (64, 0.022, 0.051, 2e-4)]):
+        ring = RingArray(n_elements=ne, radius_m=r, domain_m=d, h=hh)
+        nn = ring.n
+        wv = ricker(60, 2e-7, 0.4e6)
+        ds = simulate_dataset(np.full((nn, nn), 1500.0), ring, wv[:2], 2e-7, 1500.0,
+                              src_list=[0]) if nn < 100 else None
+        # Re-map positions exactly as fwi_from_dataset does, from stored positions.
+        domain = (nn - 1) * hh
+        abspos = ring.element_positions + domain / 2.0
+        rm[f"remap{i}_params"] = np.array([ne, r, d, hh])
+        rm[f"remap{i}_idx"] = np.array([tuple(int(round(p[a] / hh)) for a in reversed(range(2)))
+                                        for p in abspos], dtype=np.int64)
+        if ds is not None:
+            res = fwi.fwi_from_dataset(ds, n_iter=0)
+            rm[f"remap{i}_fwi_idx"] = np.array(res["geom"].idx, dtype=np.int64)
+    np.savez_compressed(os.path.join(HERE, "remap.npz"), **rm)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,219 @@
+"""GPU parity: the HIP path against the reference's golden vectors and the FP64 oracle.
+
+Tolerance (the north-star bar for FP32 fields vs the FP64 reference): relative
+L2 <= 1e-5 for traces, misfits and gradients.  Between the device's own
+methods (plain / batched, graph / eager, checkpointed / whole history) forward
+traces must agree bit for bit.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import IdxGeom, golden, rel_l2
+
+TOL = 1e-5
+
+pytestmark = pytest.mark.gpu
+
+
+def _fmc2d():
+    f = golden("fmc_2d.npz")
+    return f, IdxGeom(f["idx"]), float(f["h"]), float(f["dt"]), int(f["nt"]), f["wavelet"], \
+        [int(s) for s in f["src_list"]]
+
+
+@pytest.fixture(autouse=True)
+def _fresh_plans():
+    from openusct_amd import fwi
+    yield
+    fwi.clear_plans()
+    fwi.METHOD = "batched"
+
+
+@pytest.mark.parametrize("method", ["batched", "plain"])
+def test_forward_and_gradient_2d_golden(method):
+    from openusct_amd import fwi
+    fwi.METHOD = method
+    f, geom, h, dt, nt, w, src = _fmc2d()
+    print(f"[case] 2D golden 61x61 nt={nt} shots={src} method={method}", flush=True)
+    d = fwi.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src)
+    assert d.shape == f["nosponge_dobs"].shape
+    assert np.all(d[:, 0, :] == 0.0)
+    e_d = rel_l2(d, f["nosponge_dobs"])
+    J, g = fwi.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f["nosponge_dobs"], None, src)
+    e_J = abs(J - float(f["nosponge_J"])) / float(f["nosponge_J"])
+    e_g = rel_l2(g, f["nosponge_g"])
+    Jm = fwi.misfit(f["m0"], geom, w, dt, h, nt, f["nosponge_dobs"], None, src)
+    e_m = abs(Jm - float(f["nosponge_misfit"])) / float(f["nosponge_misfit"])
+    print(f"traces {e_d:.2e}  J {e_J:.2e}  misfit {e_m:.2e}  gradient {e_g:.2e}")
+    assert e_d < TOL and e_J < TOL and e_m < TOL and e_g < TOL
+
+
+@pytest.mark.parametrize("method", ["batched", "plain"])
+def test_sponge_2d_golden(method):
+    from openusct_amd import fwi
+    fwi.METHOD = method
+    f, geom, h, dt, nt, w, src = _fmc2d()
+    sp = golden("phantom.npz")["sponge_61"]
+    print(f"[case] 2D sponge golden method={method}", flush=True)
+    d = fwi.forward_fmc(f["m_true"], geom, w, dt, h, nt, sp, src)
+    e_d = rel_l2(d, f["sponge_dobs"])
+    J, g = fwi.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f["sponge_dobs"], sp, src)
+    e_J = abs(J - float(f["sponge_J"])) / float(f["sponge_J"])
+    e_g = rel_l2(g, f["sponge_g"])
+    print(f"traces {e_d:.2e}  J {e_J:.2e}  gradient {e_g:.2e}")
+    assert e_d < TOL and e_J < TOL and e_g < TOL
+
+
+@pytest.mark.parametrize("method", ["batched", "plain"])
+def test_gcn_2d_golden(method):
+    from openusct_amd import fwi
+    fwi.METHOD = method
+    f, geom, h, dt, nt, w, src = _fmc2d()
+    print(f"[case] 2D GCN golden method={method}", flush=True)
+    J, g = fwi.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f["nosponge_dobs"], None, src,
+                                   misfit_type="gcn")
+    e_J = abs(J - float(f["gcn_J"])) / float(f["gcn_J"])
+    e_g = rel_l2(g, f["gcn_g"])
+    print(f"GCN J {e_J:.2e}  gradient {e_g:.2e}")
+    assert e_J < TOL and e_g < TOL
+
+
+@pytest.mark.parametrize("method", ["batched", "plain"])
+def test_forward_and_gradient_3d_golden(method):
+    from openusct_amd import fwi
+    fwi.METHOD = method
+    f = golden("fmc_3d.npz")
+    geom = IdxGeom(f["idx"])
+    h, dt, nt, w = float(f["h"]), float(f["dt"]), int(f["nt"]), f["wavelet"]
+    src = [int(s) for s in f["src_list"]]
+    print(f"[case] 3D golden 21^3 nt={nt} method={method}", flush=True)
+    d = fwi.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src)
+    e_d = rel_l2(d, f["dobs"])
+    J, g = fwi.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f["dobs"], None, src)
+    e_J = abs(J - float(f["J"])) / float(f["J"])
+    e_g = rel_l2(g, f["g"])
+    print(f"traces {e_d:.2e}  J {e_J:.2e}  gradient {e_g:.2e}")
+    assert e_d < TOL and e_J < TOL and e_g < TOL
+
+
+@pytest.mark.parametrize("method", ["batched", "plain"])
+def test_duplicate_receivers_accumulate(method):
+    from openusct_amd import fwi
+    fwi.METHOD = method
+    f, _, h, dt, nt, w, _ = _fmc2d()
+    du = golden("dup_2d.npz")
+    geom = IdxGeom(du["idx"])
+    src = [int(s) for s in du["src_list"]]
+    print(f"[case] duplicate receivers method={method}", flush=True)
+    d = fwi.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src)
+    assert np.array_equal(d[:, :, 2], d[:, :, 8]) and np.array_equal(d[:, :, 8], d[:, :, 9])
+    J, g = fwi.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, du["dobs"], None, src)
+    e_J = abs(J - float(du["J"])) / float(du["J"])
+    e_g = rel_l2(g, du["g"])
+    print(f"J {e_J:.2e}  gradient {e_g:.2e}")
+    assert e_J < TOL and e_g < TOL
+
+
+def test_plain_and_batched_traces_bit_identical():
+    from openusct_amd import fwi
+    f, geom, h, dt, nt, w, _ = _fmc2d()
+    src = list(range(8))
+    fwi.METHOD = "plain"
+    a = fwi.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src)
+    fwi.METHOD = "batched"
+    b = fwi.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src)
+    assert np.array_equal(a, b)
+
+
+def _plan_problem(shape_n=61, n_src=5, nt=350):
+    from openusct_amd.plan import linear_indices
+    f, geom, h, dt, _, w, _ = _fmc2d()
+    rec = linear_indices(geom.idx, f["m0"].shape)
+    tx = rec[:n_src]
+    dev = torch.device("cuda")
+    m0 = torch.tensor(f["m0"], device=dev)
+    mt = torch.tensor(f["m_true"], device=dev)
+    wd = torch.tensor(w[:nt], device=dev)
+    return f, h, dt, nt, tx, rec, m0, mt, wd
+
+
+def test_checkpointed_gradient_equals_whole_history():
+    from openusct_amd.plan import Plan
+    f, h, dt, nt, tx, rec, m0, mt, wd = _plan_problem()
+    with Plan(m0.shape, h, dt, nt, tx, rec) as full:
+        dobs = full.forward_fmc(mt, wd)
+        J0, g0 = full.misfit_and_gradient(m0, wd, dobs)
+        assert full.info["n_windows"] == 1
+    for K in (2, 7, 50, 349):
+        with Plan(m0.shape, h, dt, nt, tx, rec, ckpt_every=K) as ck:
+            print(f"[case] checkpoint K={K} info={ck.info}", flush=True)
+            J1, g1 = ck.misfit_and_gradient(m0, wd, dobs)
+            assert torch.equal(J0, J1)
+            assert torch.equal(g0, g1), K
+
+
+def test_graph_replays_match_eager():
+    from openusct_amd.plan import Plan
+    f, h, dt, nt, tx, rec, m0, mt, wd = _plan_problem(n_src=8)
+    with Plan(m0.shape, h, dt, nt, tx, rec, graph=False) as eager:
+        dobs = eager.forward_fmc(mt, wd)
+        Je, ge = eager.misfit_and_gradient(m0, wd, dobs)
+    with Plan(m0.shape, h, dt, nt, tx, rec, ckpt_every=40) as gp:
+        for rep in range(3):
+            d2 = gp.forward_fmc(mt, wd)
+            torch.cuda.synchronize()
+            assert torch.equal(d2, dobs), rep
+            Jg, gg = gp.misfit_and_gradient(m0, wd, dobs)
+            torch.cuda.synchronize()
+            # eager work between replays
+            _ = (gg * 2).sum().item()
+            assert torch.equal(Je, Jg) and torch.equal(ge, gg), rep
+
+
+def test_multi_pass_matches_single_pass():
+    from openusct_amd.plan import Plan
+    from openusct_amd.plan import linear_indices
+    f, geom, h, dt, _, w, _ = _fmc2d()
+    n = f["m0"].shape[0]
+    # 40 shots on all 61x61 cells along a diagonal band
+    rec = linear_indices(geom.idx, (n, n))
+    tx = np.array([(5 + i) * n + 7 + i for i in range(40)], dtype=np.int64)
+    nt = 200
+    dev = torch.device("cuda")
+    m0 = torch.tensor(f["m0"], device=dev)
+    mt = torch.tensor(f["m_true"], device=dev)
+    wd = torch.tensor(w[:nt], device=dev)
+    with Plan((n, n), h, dt, nt, tx, rec) as one:
+        dobs = one.forward_fmc(mt, wd)
+        J1, g1 = one.misfit_and_gradient(m0, wd, dobs)
+        info1 = one.info
+    field = n * n * 4
+    budget = 60 * field * 16 + (8 << 20)  # room for about one 16-shot pass
+    with Plan((n, n), h, dt, nt, tx, rec, mem_budget=budget) as many:
+        info = many.info
+        print("[case] multi-pass", info1, info, flush=True)
+        assert info["shot_passes"] > 1
+        d2 = many.forward_fmc(mt, wd)
+        assert torch.equal(d2, dobs)
+        J2, g2 = many.misfit_and_gradient(m0, wd, dobs)
+    assert abs(J1.item() - J2.item()) <= 1e-6 * abs(J1.item())
+    assert rel_l2(g2.cpu().numpy(), g1.cpu().numpy()) < 1e-6
+
+
+def test_autotuned_plan_matches_default():
+    from openusct_amd.plan import Plan
+    f, h, dt, nt, tx, rec, m0, mt, wd = _plan_problem(n_src=33, nt=200)
+    with Plan(m0.shape, h, dt, nt, tx, rec) as a:
+        dobs = a.forward_fmc(mt, wd)
+        Ja, ga = a.misfit_and_gradient(m0, wd, dobs)
+    with Plan(m0.shape, h, dt, nt, tx, rec, autotune=True) as b:
+        print("[case] autotuned", b.info, flush=True)
+        db = b.forward_fmc(mt, wd)
+        assert torch.equal(db, dobs)
+        Jb, gb = b.misfit_and_gradient(m0, wd, dobs)
+    assert abs(Ja.item() - Jb.item()) <= 1e-6 * abs(Ja.item())
+    assert rel_l2(gb.cpu().numpy(), ga.cpu().numpy()) < 1e-6
