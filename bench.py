@@ -1,0 +1,238 @@
+"""Benchmark: FWI gradient throughput (Gpt-updates/s, forward + adjoint).
+
+Workload (one MI355X): 2D ring full-matrix capture on a 512 x 512 grid,
+dx = 0.2 mm, dt = 50 ns, nt = 1000, 128-element ring of radius 22 mm, all 128
+elements transmit, Ricker 500 kHz, true model = 1500 m/s water with three
+seeded inclusions (c in [1400, 1800] m/s), initial model 1500 m/s.  Observed
+data is synthesised on the device.  A step is one misfit-and-gradient
+evaluation over all 128 shots: 2 * N * (nt - 1) * 128 = 6.70e10 grid-point
+updates.
+
+Prints one JSON line with the batched throughput as ``value``, the plain
+per-shot path (one launch per shot per time step) beside it, the roofline of
+the dominant kernel (per-launch HIP-event timing) and the reference CPU core
+timed on a bounded sample of the same kind of workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_FWD = 16.0   # algorithmic bytes of one forward update (FP32)
+BYTES_ADJ = 28.0   # adjoint + imaging update
+BYTES_PER_UPDATE = 22.0
+
+
+def workload(n_el=128, seed=2):
+    import numpy as np
+
+    from openusct_amd import phantom
+    from openusct_amd.geometry import RingArray
+    from openusct_amd.sources import ricker
+
+    h, dt, nt = 2e-4, 5e-8, 1000
+    geom = RingArray(n_elements=n_el, radius_m=0.022, domain_m=0.1022, h=h)
+    n = geom.n
+    w = ricker(nt, dt, 5e5)
+    c_true = phantom.seeded_discs(n, h, seed, 3)
+    m_true = phantom.velocity_to_m(c_true)
+    m0 = np.full((n, n), 1.0 / 1500.0 ** 2)
+    return geom, h, dt, nt, w, m_true, m0
+
+
+def cpu_baseline(budget_s=20.0):
+    """Time the reference's own C++ core (oracle/_ref, built from its sources) on
+    the 256^2 / 64-element / nt = 1000 problem, FP64 with OpenMP over all cores
+    of this process's affinity mask; falls back to the oracle restatement."""
+    import numpy as np
+
+    from openusct_amd import phantom
+    from openusct_amd.geometry import RingArray
+    from openusct_amd.plan import linear_indices
+    from openusct_amd.sources import ricker
+
+    cores = len(os.sched_getaffinity(0))
+    h, dt, nt = 2e-4, 5e-8, 1000
+    geom = RingArray(n_elements=64, radius_m=0.022, domain_m=0.051, h=h)
+    n = geom.n
+    w = ricker(nt, dt, 5e5)
+    m_true = phantom.velocity_to_m(phantom.seeded_discs(n, h, 1, 3))
+    m0 = np.full((n, n), 1.0 / 1500.0 ** 2)
+    rec = linear_indices(geom.idx, (n, n)).astype(np.int32)
+    kind = "reference"
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle", "_ref"))
+        import _uap  # the reference's pybind11 module, compiled from its own sources
+
+        def fwd(m, els):
+            return _uap.forward_fmc(m, h, dt, nt, rec[els].copy(), rec, w)
+
+        def grad(m, els, d):
+            return _uap.misfit_and_gradient(m, h, dt, nt, rec[els].copy(), rec, w, d, 0)
+    except ImportError:
+        from oracle import oracle as O
+        kind = "port"
+
+        def fwd(m, els):
+            return O.forward_fmc(m, geom, w, dt, h, nt, None, els)
+
+        def grad(m, els, d):
+            return O.misfit_and_gradient(m, geom, w, dt, h, nt, d, None, els)
+    # one shot first to size the sample to the time budget
+    d1 = fwd(m_true, [0])
+    t0 = time.perf_counter()
+    grad(m0, [0], d1)
+    t1 = time.perf_counter() - t0
+    shots = int(max(1, min(8, budget_s // max(t1, 1e-3))))
+    els = list(range(0, 64, 8))[:shots]
+    d = fwd(m_true, els)
+    t0 = time.perf_counter()
+    grad(m0, els, d)
+    t = time.perf_counter() - t0
+    upd = 2.0 * n * n * (nt - 1) * shots
+    return {"value": upd / t / 1e9, "unit": "Gpt-updates/s", "cores": cores, "kind": kind,
+            "sample": f"256x256 grid, nt=1000, {shots} of 8 shots (every 8th of 64 ring elements), "
+                      f"FP64 misfit+gradient, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=128)
+    ap.add_argument("--plain-shots", type=int, default=128)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ckpt-every", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(shots would be sharded across GPUs with one all-reduce of g; not built).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from openusct_amd.plan import Plan, linear_indices
+
+    dev = torch.device("cuda", 0)
+    geom, h, dt, nt, w, m_true, m0 = workload()
+    n = geom.n
+    N = n * n
+    rec = linear_indices(geom.idx, (n, n))
+    tx = rec[: args.shots].copy()
+    wd = torch.tensor(w, device=dev)
+    mt = torch.tensor(m_true, device=dev)
+    md = torch.tensor(m0, device=dev)
+
+    print(f"[bench] building batched plan ({len(tx)} shots, {n}x{n}, nt={nt})", file=sys.stderr,
+          flush=True)
+    plan = Plan((n, n), h, dt, nt, tx, rec, method="batched", ckpt_every=args.ckpt_every,
+                autotune=args.autotune)
+    info = plan.info
+    dobs = plan.forward_fmc(mt, wd)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        J, g = plan.misfit_and_gradient(md, wd, dobs)
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        J, g = plan.misfit_and_gradient(md, wd, dobs)
+    torch.cuda.synchronize()
+    dt_run = time.perf_counter() - t0
+    ms = dt_run / args.steps * 1e3
+    upd = 2.0 * N * (nt - 1) * len(tx)
+    value = upd / (ms / 1e3) / 1e9
+    print(f"[bench] batched {ms:.1f} ms/gradient, {value:.1f} Gpt-updates/s", file=sys.stderr,
+          flush=True)
+
+    prof = plan.profile_gradient(md, wd, dobs)
+    adj_bytes = BYTES_ADJ * N * len(tx)
+    fwd_bytes = BYTES_FWD * N * len(tx)
+    adj_gbs = adj_bytes / (prof["adj_ms"] / 1e3) / 1e9
+    fwd_gbs = fwd_bytes / (prof["fwd_ms"] / 1e3) / 1e9
+    print(f"[bench] per launch: fwd {prof['fwd_ms']:.3f} ms ({fwd_gbs:.0f} GB/s alg), "
+          f"adj {prof['adj_ms']:.3f} ms ({adj_gbs:.0f} GB/s alg)", file=sys.stderr, flush=True)
+    Jv = float(J.item())
+    plan.close()
+    del dobs
+    torch.cuda.empty_cache()
+
+    plain = None
+    if not args.no_plain:
+        ps = min(args.plain_shots, len(tx))
+        pplan = Plan((n, n), h, dt, nt, tx[:ps], rec, method="plain")
+        pd = pplan.forward_fmc(mt, wd)
+        pplan.misfit_and_gradient(md, wd, pd)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pplan.misfit_and_gradient(md, wd, pd)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        plain = 2.0 * N * (nt - 1) * ps / tp / 1e9
+        print(f"[bench] plain {tp * 1e3:.1f} ms/gradient over {ps} shots, {plain:.1f} "
+              f"Gpt-updates/s", file=sys.stderr, flush=True)
+        pplan.close()
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] timing the reference CPU core", file=sys.stderr, flush=True)
+        cpu = cpu_baseline()
+
+    line = {
+        "metric": "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency",
+        "value": round(value, 3),
+        "unit": "Gpt-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded inclusion phantom, observed data simulated on the device)",
+        "config": {
+            "workload": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 128-element ring "
+                        "R=22mm, 128 shots, Ricker 500kHz, FP32 fields; one step = one "
+                        "misfit+gradient over all shots",
+            "grid": [n, n], "nt": nt, "shots": len(tx), "receivers": len(rec),
+            "method": "batched", "plan": info,
+        },
+        "plain_value": None if plain is None else round(plain, 3),
+        "batched_vs_plain": None if plain is None else round(value / plain, 3),
+        "misfit": Jv,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_adj2d (adjoint step + imaging, all shots)",
+            "achieved": round(adj_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(adj_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "per_launch_ms": round(prof["adj_ms"], 4),
+            "alg_bytes_per_launch": adj_bytes,
+            "fwd_kernel": {"achieved": round(fwd_gbs, 1), "per_launch_ms": round(prof["fwd_ms"], 4),
+                           "alg_bytes_per_launch": fwd_bytes},
+            "gradient_level": round(value * BYTES_PER_UPDATE, 1),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -100,10 +100,15 @@ struct usct_plan {
     int *src_all = nullptr;  // [passes][Bp] source cells (-1 = padding)
     int *src_cell = nullptr; // [Bp] cells of the current pass
     float *sinc = nullptr;   // [nt][Bp]
+    float *srcd2 = nullptr;  // [nt][Bp] recorded increment at each shot's source cell
     float *Z = nullptr;      // zero field
     float *F[3] = {nullptr, nullptr, nullptr};
     std::vector<float *> Wslot, CK;
-    float *recb = nullptr, *dobsT = nullptr, *res = nullptr;
+    // bfloat16 low parts of the forward state: rotation, zero level, checkpoints
+    uint16_t *Lo[3] = {nullptr, nullptr, nullptr};
+    uint16_t *Zlo = nullptr;
+    std::vector<uint16_t *> CKlo;
+    float *recb = nullptr, *recb_lo = nullptr, *dobsT = nullptr, *res = nullptr;
     float2 *part = nullptr;
     int npart = 1024;
     float *g_hi = nullptr, *g_lo = nullptr;
@@ -147,6 +152,7 @@ void free_plan_memory(usct_plan *P) {
     P->allocs.clear();
     P->Wslot.clear();
     P->CK.clear();
+    P->CKlo.clear();
     P->bytes = 0;
 }
 
@@ -178,67 +184,79 @@ Tile tile_of(const usct_plan *P) {
     return t;
 }
 
+// A forward level: FP32 high part and bfloat16 low part.
+struct Lvl {
+    float *hi;
+    uint16_t *lo;
+};
+
 template <int LB, bool SP, bool REC>
-void fwd2d_lb(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
-              const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
-    hipLaunchKernelGGL((k_fwd2d<LB, 8, SP, REC>), g, dim3(kBlock), 0, s, pc, pp, pn, P->sp2,
-                       P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, rec, P->ny, P->nx,
-                       P->Bp, P->RY, t.nxs, t.nyc, P->ng());
+void fwd2d_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
+              const float *sinc, float *rec, float *rec_lo, const Tile &t) {
+    const float *sinc0 = P->sinc;
+    float *d2 = P->srcd2 + (sinc - sinc0);
+    hipLaunchKernelGGL((k_fwd2d<LB, 8, SP, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pn.hi,
+                       pc.lo, pp.lo, pn.lo, P->sp2, P->spg, P->src_cell, sinc, P->rec_head,
+                       P->rec_next, rec, rec_lo, d2, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nyc,
+                       P->ng());
 }
 
 template <bool SP, bool REC>
-void fwd2d_sel(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
-               const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
+void fwd2d_sel(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
+               const float *sinc, float *rec, float *rl, const Tile &t) {
     switch (P->LB) {
-        case 16: fwd2d_lb<16, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
-        case 32: fwd2d_lb<32, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
-        default: fwd2d_lb<64, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+        case 16: fwd2d_lb<16, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
+        case 32: fwd2d_lb<32, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
+        default: fwd2d_lb<64, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
     }
 }
 
 template <int LB, int CX, int CY, bool REC>
-void fwd3d_t(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
-             const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
-    hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC>), g, dim3(kBlock), 0, s, pc, pp, pn, P->sp2,
-                       P->src_cell, sinc, P->rec_head, P->rec_next, rec, P->nz, P->ny, P->nx,
-                       P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
+void fwd3d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
+             const float *sinc, float *rec, float *rl, const Tile &t) {
+    float *d2 = P->srcd2 + (sinc - P->sinc);
+    hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pn.hi,
+                       pc.lo, pp.lo, pn.lo, P->sp2, P->src_cell, sinc, P->rec_head, P->rec_next,
+                       rec, rl, d2, P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc,
+                       P->ng());
 }
 
 template <int LB, bool REC>
-void fwd3d_lb(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
-              const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
-    if (P->CX == 4 && P->CY == 4) fwd3d_t<LB, 4, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, t);
-    else if (P->CX == 8 && P->CY == 2) fwd3d_t<LB, 8, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, t);
-    else fwd3d_t<LB, 4, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, t);
+void fwd3d_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
+              const float *sinc, float *rec, float *rl, const Tile &t) {
+    if (P->CX == 4 && P->CY == 4) fwd3d_t<LB, 4, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else if (P->CX == 8 && P->CY == 2) fwd3d_t<LB, 8, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else fwd3d_t<LB, 4, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
 }
 
 template <bool REC>
-void fwd3d_sel(dim3 g, hipStream_t s, const float *pc, const float *pp, float *pn,
-               const usct_plan *P, const float *sinc, float *rec, const Tile &t) {
+void fwd3d_sel(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
+               const float *sinc, float *rec, float *rl, const Tile &t) {
     switch (P->LB) {
-        case 16: fwd3d_lb<16, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
-        case 32: fwd3d_lb<32, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
-        default: fwd3d_lb<64, REC>(g, s, pc, pp, pn, P, sinc, rec, t); break;
+        case 16: fwd3d_lb<16, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
+        case 32: fwd3d_lb<32, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
+        default: fwd3d_lb<64, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
     }
 }
 
-void launch_fwd_batched(const usct_plan *P, hipStream_t s, const float *pc, const float *pp,
-                        float *pn, int n, bool record) {
+void launch_fwd_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, int n,
+                        bool record) {
     const Tile t = tile_of(P);
     const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
     const float *sinc = P->sinc + (int64_t)n * P->Bp;
     float *rec = P->recb + (int64_t)n * P->n_rec * P->Bp;
+    float *rl = P->recb_lo + (int64_t)n * P->n_rec * P->Bp;
     if (P->ndim == 2) {
         if (P->sponge) {
-            if (record) fwd2d_sel<true, true>(g, s, pc, pp, pn, P, sinc, rec, t);
-            else fwd2d_sel<true, false>(g, s, pc, pp, pn, P, sinc, rec, t);
+            if (record) fwd2d_sel<true, true>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+            else fwd2d_sel<true, false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
         } else {
-            if (record) fwd2d_sel<false, true>(g, s, pc, pp, pn, P, sinc, rec, t);
-            else fwd2d_sel<false, false>(g, s, pc, pp, pn, P, sinc, rec, t);
+            if (record) fwd2d_sel<false, true>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+            else fwd2d_sel<false, false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
         }
     } else {
-        if (record) fwd3d_sel<true>(g, s, pc, pp, pn, P, sinc, rec, t);
-        else fwd3d_sel<false>(g, s, pc, pp, pn, P, sinc, rec, t);
+        if (record) fwd3d_sel<true>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+        else fwd3d_sel<false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     }
 }
 
@@ -246,9 +264,10 @@ template <int LB, bool SP>
 void adj2d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
               const float *pa, const float *pb, const float *pc2, const usct_plan *P,
               const float *sinc, const float *res, const Tile &t) {
+    const float *d2 = P->srcd2 + (sinc - P->sinc);
     hipLaunchKernelGGL((k_adj2d<LB, 8, SP>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, pb, pc2,
-                       P->sp2, P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, res, P->g_hi,
-                       P->g_lo, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nyc, P->ng());
+                       P->sp2, P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, res, d2,
+                       P->g_hi, P->g_lo, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nyc, P->ng());
 }
 
 template <bool SP>
@@ -266,9 +285,10 @@ template <int LB, int CX, int CY>
 void adj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
              const float *pa, const usct_plan *P, const float *sinc, const float *res,
              const Tile &t) {
+    const float *d2 = P->srcd2 + (sinc - P->sinc);
     hipLaunchKernelGGL((k_adj3d<LB, CX, CY>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, P->sp2,
-                       P->src_cell, sinc, P->rec_head, P->rec_next, res, P->g_hi, P->g_lo, P->nz,
-                       P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
+                       P->src_cell, sinc, P->rec_head, P->rec_next, res, d2, P->g_hi, P->g_lo,
+                       P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
 }
 
 template <int LB>
@@ -301,25 +321,38 @@ void launch_adj_batched(const usct_plan *P, hipStream_t s, const float *l1, cons
 // ---------------------------------------------------------------------------
 // Level pointers
 // ---------------------------------------------------------------------------
-float *fwd_level(const usct_plan *P, int n, bool store) {
-    if (n <= 0) return P->Z;
-    if (store) {
-        const int base_last = (P->NS - 1) * P->K;
-        if (n >= base_last - 1) return P->Wslot[n - base_last + 1];
-        for (int s = 1; s <= P->NS - 2; ++s) {
-            if (n == s * P->K - 1) return P->CK[2 * (s - 1)];
-            if (n == s * P->K) return P->CK[2 * (s - 1) + 1];
-        }
+// Checkpoint slot (0..) holding level n, or -1.
+int ck_slot(const usct_plan *P, int n) {
+    if (P->K <= 0 || n < 1) return -1;
+    const int s = (n + 1) / P->K;
+    if (s >= 1 && s <= P->NS - 2) {
+        if (n == s * P->K - 1) return 2 * (s - 1);
+        if (n == s * P->K) return 2 * (s - 1) + 1;
     }
-    return P->F[n % 3];
+    return -1;
 }
 
-float *win_level(const usct_plan *P, int s, int level) {
-    if (level <= 0) return P->Z;
+// Where level n lives during the forward pass.  Low parts rotate over three
+// buffers except at checkpoint levels, whose low parts are kept.
+Lvl fwd_level(const usct_plan *P, int n, bool store) {
+    if (n <= 0) return {P->Z, P->Zlo};
+    if (store) {
+        const int c = ck_slot(P, n);
+        if (c >= 0) return {P->CK[c], P->CKlo[c]};
+        const int base_last = (P->NS - 1) * P->K;
+        if (n >= base_last - 1) return {P->Wslot[n - base_last + 1], P->Lo[n % 3]};
+    }
+    return {P->F[n % 3], P->Lo[n % 3]};
+}
+
+// Where level `level` lives while window s is re-computed / consumed.
+Lvl win_level(const usct_plan *P, int s, int level) {
+    if (level <= 0) return {P->Z, P->Zlo};
     const int base = s * P->K;
     const int i = level - base + 1;
-    if (s == P->NS - 1 || i >= 2) return P->Wslot[i];
-    return P->CK[2 * (s - 1) + i];
+    if (s == P->NS - 1 || i >= 2) return {P->Wslot[i], P->Lo[level % 3]};
+    const int c = 2 * (s - 1) + i;
+    return {P->CK[c], P->CKlo[c]};
 }
 
 float *lam_level(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->F[k % 3]; }
@@ -344,18 +377,18 @@ void launch_residual(usct_plan *P, hipStream_t s, int misfit, bool want_res) {
     const int64_t count = (int64_t)P->nt * ntr;
     if (misfit == USCT_MISFIT_L2) {
         const int nb = (int)std::min<int64_t>(P->npart, grid_for(count));
-        hipLaunchKernelGGL(k_residual_l2, dim3(nb), dim3(kBlock), 0, s, P->recb, P->dobsT,
-                           want_res ? P->res : nullptr, count, P->part);
+        hipLaunchKernelGGL(k_residual_l2, dim3(nb), dim3(kBlock), 0, s, P->recb, P->recb_lo,
+                           P->dobsT, want_res ? P->res : nullptr, count, P->part);
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, P->part, nb, 0.5,
                            P->J_acc, 1);
     } else {
-        hipLaunchKernelGGL(k_gcn_stats, dim3(grid_for(ntr)), dim3(256), 0, s, P->recb, P->dobsT,
-                           P->gstats, P->nt, ntr);
+        hipLaunchKernelGGL(k_gcn_stats, dim3(grid_for(ntr)), dim3(256), 0, s, P->recb, P->recb_lo,
+                           P->dobsT, P->gstats, P->nt, ntr);
         hipLaunchKernelGGL(k_gcn_misfit, dim3(1), dim3(64), 0, s, P->gstats, P->tmask_cur, ntr,
                            P->J_acc, 1);
         if (want_res)
             hipLaunchKernelGGL(k_gcn_residual, dim3(grid_for(count)), dim3(256), 0, s, P->recb,
-                               P->dobsT, P->gstats, P->tmask_cur, P->res, P->nt, ntr);
+                               P->recb_lo, P->dobsT, P->gstats, P->tmask_cur, P->res, P->nt, ntr);
     }
 }
 
@@ -366,6 +399,8 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
     // rec row 0 = 0 (fwi.py:58: rec starts as zeros, level 0 is never written)
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
                        P->recb, (int64_t)P->n_rec * P->Bp, 0.0f);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
+                       P->recb_lo, (int64_t)P->n_rec * P->Bp, 0.0f);
     for (int n = 1; n < nt; ++n) {
         if (pr) ev_start(pr, pr->fwd_ev, s);
         launch_fwd_batched(P, s, fwd_level(P, n - 1, store), fwd_level(P, n - 2, store),
@@ -389,11 +424,11 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
         for (int k = khi; k >= base + 1; --k) {
             const float *pa, *pb = nullptr, *pc2 = nullptr;
             if (P->sponge) {
-                pa = win_level(P, w, k);
-                pb = win_level(P, w, k - 1);
-                pc2 = win_level(P, w, k - 2);
+                pa = win_level(P, w, k).hi;
+                pb = win_level(P, w, k - 1).hi;
+                pc2 = win_level(P, w, k - 2).hi;
             } else {
-                pa = win_level(P, w, k - 1);
+                pa = win_level(P, w, k - 1).hi;
             }
             if (pr) ev_start(pr, pr->adj_ev, s);
             launch_adj_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2), lam_level(P, k),
@@ -471,7 +506,8 @@ int run_batched(usct_plan *P, int mode, int misfit, const double *m, const doubl
         TRY(run_pass_sequence(P, mode, misfit, pr));
         if (data)
             hipLaunchKernelGGL(k_batched_to_data, dim3(grid_for((int64_t)nb * P->nt * P->n_rec)),
-                               dim3(256), 0, s, P->recb, data, P->nt, P->n_rec, P->Bp, b0, nb);
+                               dim3(256), 0, s, P->recb, P->recb_lo, data, P->nt, P->n_rec, P->Bp,
+                               b0, nb);
     }
     if (grad)
         hipLaunchKernelGGL(k_finalize_grad, dim3(grid_for(N)), dim3(256), 0, s, P->g_hi, P->g_lo,
@@ -506,28 +542,33 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
         if (n <= 0) return P->Z;
         return store ? P->Wslot[n] : P->F[n % 3];
     };
+    auto llo = [&](int n) -> uint16_t * { return n <= 0 ? P->Zlo : P->Lo[n % 3]; };
     for (int b = 0; b < P->n_tx; ++b) {
         const int src = (int)P->tx[b];
         // S_src * w[n-1] per level, same FP64 product as the batched path.
         hipLaunchKernelGGL(k_setup_sinc, dim3(grid_for(nt)), dim3(256), 0, s, m, w,
                            P->src_all + b, P->sinc, nt, 1, P->dt * P->dt);
         HIPX(hipMemsetAsync(P->recb, 0, sizeof(float) * nr, s));
+        HIPX(hipMemsetAsync(P->recb_lo, 0, sizeof(float) * nr, s));
         for (int n = 1; n < nt; ++n) {
             if (pr) ev_start(pr, pr->fwd_ev, s);
             if (P->sponge)
                 hipLaunchKernelGGL((k_fwd_plain<true, true>), dim3(gN), dim3(256), 0, s, lvl(n - 1),
-                                   lvl(n - 2), lvl(n), P->sp2, P->spg, src, P->sinc + n, P->rec_head,
-                                   P->rec_next, P->recb + (int64_t)n * nr, P->nz, P->ny, P->nx, nd);
+                                   lvl(n - 2), lvl(n), llo(n - 1), llo(n - 2), llo(n), P->sp2,
+                                   P->spg, src, P->sinc + n, P->rec_head, P->rec_next,
+                                   P->recb + (int64_t)n * nr, P->recb_lo + (int64_t)n * nr,
+                                   P->srcd2 + n, P->nz, P->ny, P->nx, nd);
             else
                 hipLaunchKernelGGL((k_fwd_plain<false, true>), dim3(gN), dim3(256), 0, s,
-                                   lvl(n - 1), lvl(n - 2), lvl(n), P->sp2, P->spg, src, P->sinc + n,
-                                   P->rec_head, P->rec_next, P->recb + (int64_t)n * nr, P->nz,
-                                   P->ny, P->nx, nd);
+                                   lvl(n - 1), lvl(n - 2), lvl(n), llo(n - 1), llo(n - 2), llo(n),
+                                   P->sp2, P->spg, src, P->sinc + n, P->rec_head, P->rec_next,
+                                   P->recb + (int64_t)n * nr, P->recb_lo + (int64_t)n * nr,
+                                   P->srcd2 + n, P->nz, P->ny, P->nx, nd);
             if (pr) ev_stop(pr, pr->fwd_ev, s);
         }
         if (data)
-            HIPX(hipMemcpyAsync(data + (int64_t)b * nt * nr, P->recb, sizeof(float) * nt * nr,
-                                hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_batched_to_data, dim3(grid_for((int64_t)nt * nr)), dim3(256), 0, s,
+                               P->recb, P->recb_lo, data, nt, nr, 1, b, 1);
         if (mode == MODE_FWD) continue;
         // residual of this shot against dobs[b] (layout [n][j] already)
         HIPX(hipMemcpyAsync(P->dobsT, dobs + (int64_t)b * nt * nr, sizeof(float) * nt * nr,
@@ -544,13 +585,13 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
             if (pr) ev_start(pr, pr->adj_ev, s);
             if (P->sponge)
                 hipLaunchKernelGGL((k_adj_plain<true>), dim3(gN), dim3(256), 0, s, lam_level(P, k + 1),
-                                   lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2, P->spg, src, P->sinc + k,
+                                   lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2, P->spg, src, P->srcd2 + k,
                                    P->rec_head, P->rec_next, P->res + (int64_t)k * nr, P->g_hi,
                                    P->g_lo, P->nz, P->ny, P->nx, nd);
             else
                 hipLaunchKernelGGL((k_adj_plain<false>), dim3(gN), dim3(256), 0, s,
                                    lam_level(P, k + 1), lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2,
-                                   P->spg, src, P->sinc + k, P->rec_head, P->rec_next,
+                                   P->spg, src, P->srcd2 + k, P->rec_head, P->rec_next,
                                    P->res + (int64_t)k * nr, P->g_hi, P->g_lo, P->nz, P->ny, P->nx,
                                    nd);
             if (pr) ev_stop(pr, pr->adj_ev, s);
@@ -577,8 +618,12 @@ int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
     const int64_t field = P->N * (int64_t)Bp * 4;
     const int64_t traces = (int64_t)P->nt * P->n_rec * Bp * 4;
     int64_t fields = 4;  // Z + F[3]
-    if (grad) fields += (K + 2) + 2 * std::max(NS - 2, 0);
-    int64_t b = fields * field + 3 * traces;
+    int64_t halves = 4;  // Zlo + Lo[3]
+    if (grad) {
+        fields += (K + 2) + 2 * std::max(NS - 2, 0);
+        halves += 2 * std::max(NS - 2, 0);
+    }
+    int64_t b = fields * field + halves * (field / 2) + 4 * traces;
     b += P->N * 8 /*sp2*/ + P->N * 4 /*sponge*/ + 2 * P->N * 4 * (Bp / 16) /*g*/;
     b += (int64_t)P->nt * Bp * 4 + (int64_t)P->n_rec * Bp * 24 + (int64_t)P->N * 4 + (1 << 20);
     return b;
@@ -615,7 +660,7 @@ int choose_batched(usct_plan *P, int ckpt_every) {
             if (bytes > P->budget) continue;
             // cost: bytes streamed per shot-level (fwd 12, adj 17, recompute 12 if NS > 1),
             // padded shots count, narrower lane groups stream a little less efficiently.
-            const double per = 12.0 + 17.0 + (NS > 1 ? 12.0 * (1.0 - 1.0 / NS) : 0.0);
+            const double per = 19.0 + 17.0 + (NS > 1 ? 19.0 * (1.0 - 1.0 / NS) : 0.0);
             const double eff = lb == 64 ? 1.0 : (lb == 32 ? 0.95 : 0.88);
             // launch overhead of ~3 us per step launch, in streamed-byte units at 5 TB/s
             const double launches = (double)passes * nt * (NS > 1 ? 3.0 : 2.0);
@@ -639,20 +684,27 @@ int allocate_batched(usct_plan *P) {
     TRY(dalloc_t(P, &P->Z, FE));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * FE));
     for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->F[i], FE));
+    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->Lo[i], FE));
+    TRY(dalloc_t(P, &P->Zlo, FE));
+    HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * FE));
     const int nW = P->K + 2;
     P->Wslot.resize(nW);
     for (int i = 0; i < nW; ++i) TRY(dalloc_t(P, &P->Wslot[i], FE));
     const int nC = 2 * std::max(P->NS - 2, 0);
     P->CK.resize(nC);
     for (int i = 0; i < nC; ++i) TRY(dalloc_t(P, &P->CK[i], FE));
+    P->CKlo.resize(nC);
+    for (int i = 0; i < nC; ++i) TRY(dalloc_t(P, &P->CKlo[i], FE));
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
     TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * ntr));
+    TRY(dalloc_t(P, &P->recb_lo, (int64_t)P->nt * ntr));
     TRY(dalloc_t(P, &P->dobsT, (int64_t)P->nt * ntr));
     TRY(dalloc_t(P, &P->res, (int64_t)P->nt * ntr));
     // sized for the narrowest lane group the autotuner may pick (16 lanes)
     TRY(dalloc_t(P, &P->g_hi, P->N * std::max(P->Bp / 16, 1)));
     TRY(dalloc_t(P, &P->g_lo, P->N * std::max(P->Bp / 16, 1)));
     TRY(dalloc_t(P, &P->sinc, (int64_t)P->nt * P->Bp));
+    TRY(dalloc_t(P, &P->srcd2, (int64_t)P->nt * P->Bp));
     TRY(dalloc_t(P, &P->src_cell, P->Bp));
     TRY(dalloc_t(P, &P->gstats, 3 * ntr));
     TRY(dalloc_t(P, &P->tmask_cur, ntr));
@@ -676,6 +728,9 @@ int allocate_plain(usct_plan *P) {
     TRY(dalloc_t(P, &P->Z, N));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * N));
     for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->F[i], N));
+    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->Lo[i], N));
+    TRY(dalloc_t(P, &P->Zlo, N));
+    HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * N));
     // full history, levels 1..nt-1 (slot 0 unused)
     const int64_t hist = (int64_t)P->nt * N * 4;
     if (hist + P->bytes > P->budget)
@@ -687,11 +742,13 @@ int allocate_plain(usct_plan *P) {
     for (int n = 0; n < P->nt; ++n) P->Wslot[n] = H + (int64_t)n * N;
     const int nr = P->n_rec;
     TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * nr));
+    TRY(dalloc_t(P, &P->recb_lo, (int64_t)P->nt * nr));
     TRY(dalloc_t(P, &P->dobsT, (int64_t)P->nt * nr));
     TRY(dalloc_t(P, &P->res, (int64_t)P->nt * nr));
     TRY(dalloc_t(P, &P->g_hi, N));
     TRY(dalloc_t(P, &P->g_lo, N));
     TRY(dalloc_t(P, &P->sinc, P->nt));
+    TRY(dalloc_t(P, &P->srcd2, P->nt));
     TRY(dalloc_t(P, &P->gstats, 3 * (int64_t)nr));
     TRY(dalloc_t(P, &P->tmask, nr));
     TRY(dalloc_t(P, &P->tmask_cur, nr));
@@ -712,12 +769,15 @@ double time_tile(usct_plan *P) {
     (void)hipEventCreate(&b);
     const int reps = 4;
     // warm-up
-    launch_fwd_batched(P, P->stream, P->F[0], P->F[1], P->F[2], 1, false);
+    launch_fwd_batched(P, P->stream, {P->F[0], P->Lo[0]}, {P->F[1], P->Lo[1]},
+                       {P->F[2], P->Lo[2]}, 1, false);
     launch_adj_batched(P, P->stream, P->F[0], P->F[1], P->F[2], P->Wslot[0], P->Wslot[0],
                        P->Wslot[0], 1);
     (void)hipEventRecord(a, P->stream);
     for (int r = 0; r < reps; ++r) {
-        launch_fwd_batched(P, P->stream, P->F[r % 3], P->F[(r + 1) % 3], P->F[(r + 2) % 3], 1, false);
+        launch_fwd_batched(P, P->stream, {P->F[r % 3], P->Lo[r % 3]},
+                           {P->F[(r + 1) % 3], P->Lo[(r + 1) % 3]},
+                           {P->F[(r + 2) % 3], P->Lo[(r + 2) % 3]}, 1, false);
         launch_adj_batched(P, P->stream, P->F[r % 3], P->F[(r + 1) % 3], P->F[(r + 2) % 3],
                            P->Wslot[0], P->Wslot[0], P->Wslot[0], 1);
     }
@@ -735,7 +795,10 @@ double time_tile(usct_plan *P) {
 // among those that keep the same padded shot count.
 int autotune(usct_plan *P) {
     // zero the fields used by the timing launches
-    for (int i = 0; i < 3; ++i) HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
+    for (int i = 0; i < 3; ++i) {
+        HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
+        HIPX(hipMemset(P->Lo[i], 0, sizeof(uint16_t) * P->field_elems()));
+    }
     HIPX(hipMemset(P->Wslot[0], 0, sizeof(float) * P->field_elems()));
     HIPX(hipMemset(P->sinc, 0, sizeof(float) * P->nt * P->Bp));
     HIPX(hipMemset(P->sp2, 0, sizeof(float2) * P->N));

@@ -74,6 +74,47 @@ __device__ __forceinline__ float leap(float cur, float prev, float inc) {
     return (2.0f * cur - prev) + inc;
 }
 
+// Forward state p = hi + lo with lo kept in bfloat16 (round to nearest even):
+// the FP32 rounding of p at every step, integrated over thousands of steps, is
+// what separates FP32 traces from FP64 ones; carrying it in 16 extra bits
+// removes most of that drift.
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// Knuth TwoSum: s + e == a + b exactly.
+__device__ __forceinline__ float two_sum(float a, float b, float &e) {
+    const float s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+    return s;
+}
+
+// New level from (ph + pl) at n-1, (qh + ql) at n-2 and the FP32 increment:
+// hi + lo = 2 p^{n-1} - p^{n-2} + inc, optionally times the sponge.
+__device__ __forceinline__ float leap_hilo(float ph, float pl, float qh, float ql, float inc,
+                                           float &lo) {
+    float e;
+    const float s = two_sum(2.0f * ph, -qh, e);
+    const float t = (e + (2.0f * pl - ql)) + inc;
+    return two_sum(s, t, lo);
+}
+
+__device__ __forceinline__ float scale_hilo(float hi, float lo, float w, float &lo_out) {
+    const float h2 = hi * w;
+    const float err = fmaf(hi, w, -h2);
+    return two_sum(h2, fmaf(lo, w, err), lo_out);
+}
+
+// At a source cell the field is a sharp peak: its FP32 Laplacian and the
+// source term nearly cancel.  There the increment is formed in FP64 from the
+// full hi + lo state and recorded, so the adjoint images with the increment
+// the forward actually applied.
+__device__ __forceinline__ double hl(float h, uint16_t l) { return (double)h + (double)bf2f(l); }
+
 __device__ __forceinline__ void neumaier(float &s, float &c, float x) {
     const float t = s + x;
     if (fabsf(s) >= fabsf(x)) c = c + ((s - t) + x);
@@ -121,10 +162,12 @@ __device__ __forceinline__ int sum_cell(int lane) {
 template <int LB, int CX, bool SPONGE, bool REC>
 __global__ void __launch_bounds__(kBlock)
 k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
-        const float2 *__restrict__ sp2, const float *__restrict__ sponge,
-        const int *__restrict__ src_cell, const float *__restrict__ sinc,
-        const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-        float *__restrict__ rec, int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+        const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
+        uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+        const float *__restrict__ sponge, const int *__restrict__ src_cell,
+        const float *__restrict__ sinc, const int *__restrict__ rec_head,
+        const int *__restrict__ rec_next, float *__restrict__ rec, float *__restrict__ rec_lo,
+        float *__restrict__ srcd2, int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
     constexpr int W = kWave / LB;
     const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -165,14 +208,37 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                 const int c = y * nx + x;
                 float lu = 0.0f;
                 if (yint && x >= 1 && x <= nx - 2) lu = lap5(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
-                float inc = smul(sp2[c], lu);
-                if (c == my_src) inc = inc + my_inc;
+                const float2 s2 = sp2[c];
+                float inc = smul(s2, lu);
                 const size_t a = (size_t)c * BP + b;
-                float v = leap(rc[i + 1], pp[a], inc);
-                if (SPONGE) v = v * sponge[c];
+                const uint16_t cl_ = pcl[a];
+                double lapd = 0.0;
+                if (c == my_src) {
+                    if (yint && x >= 1 && x <= nx - 2) {
+                        const size_t sx = BP, sy = (size_t)nx * BP;
+                        lapd = -4.0 * hl(rc[i + 1], cl_);
+                        lapd += hl(rp[i], pcl[a + sy]);
+                        lapd += hl(rm[i], pcl[a - sy]);
+                        lapd += hl(rc[i + 2], pcl[a + sx]);
+                        lapd += hl(rc[i], pcl[a - sx]);
+                    }
+                    inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)my_inc);
+                }
+                float lo;
+                const float qh = pp[a];
+                const uint16_t ql = ppl[a];
+                float v = leap_hilo(rc[i + 1], bf2f(cl_), qh, bf2f(ql), inc, lo);
+                if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
+                const uint16_t lb = f2bf(lo);
                 pn[a] = v;
+                pnl[a] = lb;
+                if (c == my_src)
+                    srcd2[b] = (float)((hl(v, lb) - 2.0 * hl(rc[i + 1], cl_)) + hl(qh, ql));
                 if (REC) {
-                    for (int j = rec_head[c]; j >= 0; j = rec_next[j]) rec[(size_t)j * BP + b] = v;
+                    for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
+                        rec[(size_t)j * BP + b] = v;
+                        rec_lo[(size_t)j * BP + b] = bf2f(lb);
+                    }
                 }
             }
         }
@@ -199,8 +265,8 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         const float2 *__restrict__ sp2, const float *__restrict__ sponge,
         const int *__restrict__ src_cell, const float *__restrict__ sinc,
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-        const float *__restrict__ res, float *__restrict__ g_hi, float *__restrict__ g_lo,
-        int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+        const float *__restrict__ res, const float *__restrict__ srcd2, float *__restrict__ g_hi,
+        float *__restrict__ g_lo, int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
     constexpr int W = kWave / LB;
     const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -216,7 +282,7 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     const size_t BP = (size_t)Bp;
     const size_t N = (size_t)ny * nx;
     const int my_src = src_cell[b];
-    const float my_inc = sinc[b];
+    const float my_d2 = srcd2[b];
     const int xl = min(x0, nx - 1);
     auto at = [&](int y, int x) -> size_t {
         x = min(max(x, 0), nx - 1);
@@ -274,8 +340,8 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                 } else {
                     const float lp = in ? lap5(fc[i + 1], fp[i], fm[i], fc[i + 2], fc[i]) : 0.0f;
                     d2 = smul(sp2[c], lp);
-                    if (c == my_src) d2 = d2 + my_inc;
                 }
+                if (c == my_src) d2 = my_d2;
                 tv[i] = v * d2;
             }
         }
@@ -315,10 +381,12 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 template <int LB, int CX, int CY, bool REC>
 __global__ void __launch_bounds__(kBlock)
 k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
-        const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
-        const float *__restrict__ sinc, const int *__restrict__ rec_head,
-        const int *__restrict__ rec_next, float *__restrict__ rec, int nz, int ny, int nx,
-        int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+        const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
+        uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+        const int *__restrict__ src_cell, const float *__restrict__ sinc,
+        const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+        float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int nz,
+        int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
     constexpr int W = kWave / LB;
     const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -387,13 +455,38 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     if (yint && x >= 1 && x <= nx - 2)
                         lu = lap7(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
-                    float inc = smul(sp2[c], lu);
-                    if (c == my_src) inc = inc + my_inc;
+                    const float2 s2 = sp2[c];
+                    float inc = smul(s2, lu);
                     const size_t a = (size_t)c * BP + b;
-                    const float v = leap(zcu[j + 1][i + 1], pp[a], inc);
+                    const uint16_t cl_ = pcl[a];
+                    if (c == my_src) {
+                        double lapd = 0.0;
+                        if (yint && x >= 1 && x <= nx - 2) {
+                            const size_t sx = BP, sy = (size_t)nx * BP, sz = (size_t)ny * nx * BP;
+                            lapd = -6.0 * hl(zcu[j + 1][i + 1], cl_);
+                            lapd += hl(zp[j][i], pcl[a + sz]);
+                            lapd += hl(zm[j][i], pcl[a - sz]);
+                            lapd += hl(zcu[j + 2][i + 1], pcl[a + sy]);
+                            lapd += hl(zcu[j][i + 1], pcl[a - sy]);
+                            lapd += hl(zcu[j + 1][i + 2], pcl[a + sx]);
+                            lapd += hl(zcu[j + 1][i], pcl[a - sx]);
+                        }
+                        inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)my_inc);
+                    }
+                    float lo;
+                    const float qh = pp[a];
+                    const uint16_t ql = ppl[a];
+                    const float v = leap_hilo(zcu[j + 1][i + 1], bf2f(cl_), qh, bf2f(ql), inc, lo);
+                    const uint16_t lb = f2bf(lo);
                     pn[a] = v;
+                    pnl[a] = lb;
+                    if (c == my_src)
+                        srcd2[b] = (float)((hl(v, lb) - 2.0 * hl(zcu[j + 1][i + 1], cl_)) + hl(qh, ql));
                     if (REC) {
-                        for (int r = rec_head[c]; r >= 0; r = rec_next[r]) rec[(size_t)r * BP + b] = v;
+                        for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
+                            rec[(size_t)r * BP + b] = v;
+                            rec_lo[(size_t)r * BP + b] = bf2f(lb);
+                        }
                     }
                 }
             }
@@ -421,8 +514,9 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         const float *__restrict__ pa, const float2 *__restrict__ sp2,
         const int *__restrict__ src_cell, const float *__restrict__ sinc,
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-        const float *__restrict__ res, float *__restrict__ g_hi, float *__restrict__ g_lo,
-        int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+        const float *__restrict__ res, const float *__restrict__ srcd2, float *__restrict__ g_hi,
+        float *__restrict__ g_lo, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
+        int nzc, int ng) {
     constexpr int W = kWave / LB;
     constexpr int CC = CX * CY;
     const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
@@ -442,7 +536,7 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     const size_t BP = (size_t)Bp;
     const size_t N = (size_t)nz * ny * nx;
     const int my_src = src_cell[b];
-    const float my_inc = sinc[b];
+    const float my_d2 = srcd2[b];
     auto cl = [&](int z, int y, int x) -> int {
         x = min(max(x, 0), nx - 1);
         y = min(max(y, 0), ny - 1);
@@ -523,7 +617,7 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                                                fc[j][i + 1], fc[j + 1][i + 2], fc[j + 1][i])
                                         : 0.0f;
                     float d2 = smul(sp2[c], lp);
-                    if (c == my_src) d2 = d2 + my_inc;
+                    if (c == my_src) d2 = my_d2;
                     tv[j * CX + i] = v * d2;
                 }
             }
@@ -562,10 +656,12 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 template <bool SPONGE, bool REC>
 __global__ void __launch_bounds__(kBlock)
 k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
-            const float2 *__restrict__ sp2, const float *__restrict__ sponge, int src,
-            const float *__restrict__ sinc, const int *__restrict__ rec_head,
-            const int *__restrict__ rec_next, float *__restrict__ rec, int nz, int ny, int nx,
-            int ndim) {
+            const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
+            uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+            const float *__restrict__ sponge, int src, const float *__restrict__ sinc,
+            const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+            float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int nz,
+            int ny, int nx, int ndim) {
     const int64_t N = (int64_t)nz * ny * nx;
     const int64_t c64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c64 >= N) return;
@@ -583,13 +679,46 @@ k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *_
         if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
             lu = lap7(pcv, pc[c + sz], pc[c - sz], pc[c + nx], pc[c - nx], pc[c + 1], pc[c - 1]);
     }
-    float inc = smul(sp2[c], lu);
-    if (c == src) inc = inc + *sinc;
-    float v = leap(pcv, pp[c], inc);
-    if (SPONGE) v = v * sponge[c];
+    const float2 s2 = sp2[c];
+    float inc = smul(s2, lu);
+    if (c == src) {
+        double lapd = 0.0;
+        if (ndim == 2) {
+            if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2) {
+                lapd = -4.0 * hl(pcv, pcl[c]);
+                lapd += hl(pc[c + nx], pcl[c + nx]);
+                lapd += hl(pc[c - nx], pcl[c - nx]);
+                lapd += hl(pc[c + 1], pcl[c + 1]);
+                lapd += hl(pc[c - 1], pcl[c - 1]);
+            }
+        } else {
+            const int sz = nx * ny;
+            if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2) {
+                lapd = -6.0 * hl(pcv, pcl[c]);
+                lapd += hl(pc[c + sz], pcl[c + sz]);
+                lapd += hl(pc[c - sz], pcl[c - sz]);
+                lapd += hl(pc[c + nx], pcl[c + nx]);
+                lapd += hl(pc[c - nx], pcl[c - nx]);
+                lapd += hl(pc[c + 1], pcl[c + 1]);
+                lapd += hl(pc[c - 1], pcl[c - 1]);
+            }
+        }
+        inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)*sinc);
+    }
+    float lo;
+    const float qh = pp[c];
+    const uint16_t ql = ppl[c];
+    float v = leap_hilo(pcv, bf2f(pcl[c]), qh, bf2f(ql), inc, lo);
+    if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
+    const uint16_t lb = f2bf(lo);
     pn[c] = v;
+    pnl[c] = lb;
+    if (c == src) *srcd2 = (float)((hl(v, lb) - 2.0 * hl(pcv, pcl[c])) + hl(qh, ql));
     if (REC) {
-        for (int j = rec_head[c]; j >= 0; j = rec_next[j]) rec[j] = v;
+        for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
+            rec[j] = v;
+            rec_lo[j] = bf2f(lb);
+        }
     }
 }
 
@@ -598,7 +727,7 @@ __global__ void __launch_bounds__(kBlock)
 k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
             const float *__restrict__ pa, const float *__restrict__ pb,
             const float *__restrict__ pc2, const float2 *__restrict__ sp2,
-            const float *__restrict__ sponge, int src, const float *__restrict__ sinc,
+            const float *__restrict__ sponge, int src, const float *__restrict__ srcd2,
             const int *__restrict__ rec_head, const int *__restrict__ rec_next,
             const float *__restrict__ res, float *__restrict__ g_hi, float *__restrict__ g_lo,
             int nz, int ny, int nx, int ndim) {
@@ -636,8 +765,8 @@ k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *_
         d2 = (pa[c] - 2.0f * pb[c]) + pc2[c];
     } else {
         d2 = smul(sp2[c], lp);
-        if (c == src) d2 = d2 + *sinc;
     }
+    if (c == src) d2 = *srcd2;
     float hi = g_hi[c], lo = g_lo[c];
     neumaier(hi, lo, v * d2);
     g_hi[c] = hi;
@@ -694,25 +823,27 @@ __global__ void k_dobs_to_batched(const float *__restrict__ dobs, float *__restr
 }
 
 // data[b0 + b][n][j] = rec[n][j][b] for b < nb.
-__global__ void k_batched_to_data(const float *__restrict__ rec, float *__restrict__ data, int nt,
-                                  int n_rec, int Bp, int b0, int nb) {
+__global__ void k_batched_to_data(const float *__restrict__ rec, const float *__restrict__ rec_lo,
+                                  float *__restrict__ data, int nt, int n_rec, int Bp, int b0,
+                                  int nb) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t tot = (int64_t)nb * nt * n_rec;
     if (i >= tot) return;
     const int64_t nj = i % ((int64_t)nt * n_rec);
     const int b = (int)(i / ((int64_t)nt * n_rec));
-    data[(int64_t)(b0 + b) * nt * n_rec + nj] = rec[nj * Bp + b];
+    data[(int64_t)(b0 + b) * nt * n_rec + nj] = rec[nj * Bp + b] + rec_lo[nj * Bp + b];
 }
 
 // L2 residual r = dsyn - dobs over `count` entries (padded entries are 0 - 0),
 // and per-block Neumaier partial sums of r^2 (fwi.py:155-157).
 __global__ void __launch_bounds__(kBlock)
-k_residual_l2(const float *__restrict__ dsyn, const float *__restrict__ dobs,
-              float *__restrict__ res, int64_t count, float2 *__restrict__ part) {
+k_residual_l2(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
+              const float *__restrict__ dobs, float *__restrict__ res, int64_t count,
+              float2 *__restrict__ part) {
     float s = 0.0f, c = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const float r = dsyn[i] - dobs[i];
+        const float r = (dsyn[i] - dobs[i]) + dsyn_lo[i];
         if (res) res[i] = r;
         neumaier(s, c, r * r);
     }
@@ -749,13 +880,15 @@ __global__ void k_reduce_partials(const float2 *__restrict__ part, int n, double
 // GCN per-trace statistics over time (fwi.py:160-164): for trace t = (b, j)
 // stats[t] = (|s|^2, |o|^2, <s,o>) in FP64; layout of traces [n][j][b] with
 // stride n_rec*Bp between time samples.
-__global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restrict__ dobs,
-                            double *__restrict__ stats, int nt, int64_t ntr) {
+__global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
+                            const float *__restrict__ dobs, double *__restrict__ stats, int nt,
+                            int64_t ntr) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntr) return;
     double ss = 0.0, oo = 0.0, so = 0.0;
     for (int n = 0; n < nt; ++n) {
-        const double s = dsyn[(int64_t)n * ntr + t], o = dobs[(int64_t)n * ntr + t];
+        const int64_t q = (int64_t)n * ntr + t;
+        const double s = (double)dsyn[q] + (double)dsyn_lo[q], o = dobs[q];
         ss += s * s;
         oo += o * o;
         so += s * o;
@@ -767,9 +900,10 @@ __global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restr
 
 // GCN adjoint source res = -(o/no - c s/ns)/ns and per-trace misfit 1 - c
 // (fwi.py:164-167); tmask[t] = 0 for padded traces.
-__global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__restrict__ dobs,
-                               const double *__restrict__ stats, const unsigned char *__restrict__ tmask,
-                               float *__restrict__ res, int nt, int64_t ntr) {
+__global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
+                               const float *__restrict__ dobs, const double *__restrict__ stats,
+                               const unsigned char *__restrict__ tmask, float *__restrict__ res,
+                               int nt, int64_t ntr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nt * ntr) return;
     const int64_t t = i % ntr;
@@ -777,7 +911,8 @@ __global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__re
     const double eps = 1e-12;
     const double ns = sqrt(stats[3 * t]) + eps, no = sqrt(stats[3 * t + 1]) + eps;
     const double c = stats[3 * t + 2] / (ns * no);
-    res[i] = (float)(-((double)dobs[i] / no - c * ((double)dsyn[i] / ns)) / ns);
+    const double s = (double)dsyn[i] + (double)dsyn_lo[i];
+    res[i] = (float)(-((double)dobs[i] / no - c * (s / ns)) / ns);
 }
 
 __global__ void k_gcn_misfit(const double *__restrict__ stats, const unsigned char *__restrict__ tmask,

@@ -100,7 +100,7 @@ def seeded_ellipsoids(n, h, seed, count, c_bg=1500.0, a_mm=(3.0, 8.0),
     for _ in range(count):
         ax = rng.uniform(*a_mm, size=3) * 1e-3
         cv = rng.uniform(*c_range)
-        lim = inside_mm * 1e-3 - ax.max()
+        lim = max(inside_mm * 1e-3 - ax.max(), 0.0)
         rho = lim * np.sqrt(rng.uniform())
         th = rng.uniform(0.0, 2.0 * np.pi)
         zc = rng.uniform(-lim, lim)

@@ -8,14 +8,17 @@ It imports ``ringfwi`` from ``<reference>/simulation`` and records inputs and
 outputs of the functions whose behaviour only the reference itself pins:
 
 * the 2D sponge path (simulation/ringfwi/solver.py:22-42, fwi.py:68-69,313-314),
-  whose gradient is the reference's inexact formula;
+  whose gradient is the reference's inexact formula (fmc_2d.npz holds it on the
+  test_gradient.py grid, where the sponge covers almost every cell, and
+  sponge_2d.npz on a grid where it is a border layer);
 * element-index rounding of RingArray / CylinderArray (geometry.py:47-50,118-122)
   and the dataset re-mapping of fwi_from_dataset (fwi.py:744-746);
 * phantom builders (phantom.py:13-48,166-204) and the Ricker wavelet
   (sources.py:8-26);
 * forward / gradient on the reference's own test problems
   (simulation/tests/test_gradient.py:20-40, test_gradient_3d.py:18-37), with
-  duplicate receivers (fwi.py:321-322 accumulates them) and the GCN misfit.
+  duplicate receivers (fwi.py:321-322 accumulates them) and the GCN misfit,
+  plus a better-resolved 3D cylinder problem (fmc_3d_fine.npz).
 
 The fixtures are data only (numpy .npz, no pickles).
 """
@@ -132,6 +135,39 @@ def main(ref_root):
     dobs = fwi.forward_fmc(m_true, cyl, wav, dt, h, nt, src_list=src_list)
     J, g = fwi.misfit_and_gradient(m0, cyl, wav, dt, h, nt, dobs, src_list=src_list)
     np.savez_compressed(os.path.join(HERE, "fmc_3d.npz"), idx=np.array(cyl.idx, dtype=np.int64),
+                        h=h, dt=dt, nt=nt, wavelet=wav, m_true=m_true, m0=m0,
+                        src_list=np.array(src_list), dobs=dobs, J=np.array(J), g=g)
+
+    # -- 2D sponge on a grid where the sponge is a border layer (width 25 of 121)
+    h = 5e-4
+    geom = RingArray(n_elements=8, radius_m=0.012, domain_m=0.06, h=h)
+    n = geom.n
+    dt, nt = 1e-7, 500
+    wav = ricker(nt, dt, 0.25e6)
+    c0 = phantom.coupling_background((n, n), 2000.0, 1480.0, 0.008, h)
+    m0 = phantom.velocity_to_m(c0)
+    m_true = phantom.velocity_to_m(phantom.add_inclusion(c0, (0.52, 0.5), 0.003, 2300.0, h))
+    sp = make_sponge((n, n))
+    src_list = [0, 3]
+    dobs = fwi.forward_fmc(m_true, geom, wav, dt, h, nt, sp, src_list)
+    J, g = fwi.misfit_and_gradient(m0, geom, wav, dt, h, nt, dobs, sp, src_list)
+    np.savez_compressed(os.path.join(HERE, "sponge_2d.npz"), idx=np.array(geom.idx, dtype=np.int64),
+                        h=h, dt=dt, nt=nt, wavelet=wav, m_true=m_true, m0=m0, sponge=sp,
+                        src_list=np.array(src_list), dobs=dobs, J=np.array(J), g=g)
+
+    # -- 3D at about 10 points per wavelength (test_gradient_3d.py's problem is ~3)
+    h = 5e-4
+    cyl = CylinderArray(n_rings=2, per_ring=8, radius_m=0.007, height_m=0.006, domain_m=0.02, h=h)
+    n = cyl.n
+    dt, nt = 1e-7, 200
+    wav = ricker(nt, dt, 0.3e6)
+    c0 = phantom.cylinder_background((n, n, n), 1500.0, 1480.0, 0.006, h)
+    m0 = phantom.velocity_to_m(c0)
+    m_true = phantom.velocity_to_m(phantom.add_sphere(c0, (0.55, 0.5, 0.5), 0.002, 1650.0, h))
+    src_list = [0, 9]
+    dobs = fwi.forward_fmc(m_true, cyl, wav, dt, h, nt, src_list=src_list)
+    J, g = fwi.misfit_and_gradient(m0, cyl, wav, dt, h, nt, dobs, src_list=src_list)
+    np.savez_compressed(os.path.join(HERE, "fmc_3d_fine.npz"), idx=np.array(cyl.idx, dtype=np.int64),
                         h=h, dt=dt, nt=nt, wavelet=wav, m_true=m_true, m0=m0,
                         src_list=np.array(src_list), dobs=dobs, J=np.array(J), g=g)
 

@@ -1,0 +1,190 @@
+"""Host-side logic (CPU only): input builders against the reference's golden
+vectors, index handling, argument checks, and the C-ABI library's exports."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import IdxGeom, golden
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_ring_indices_match_reference_including_ties():
+    from openusct_amd.geometry import RingArray
+    g = golden("geometry.npz")
+    i = 0
+    while f"ring{i}_params" in g:
+        ne, r, d, h = g[f"ring{i}_params"]
+        ring = RingArray(n_elements=int(ne), radius_m=float(r), domain_m=float(d), h=float(h))
+        assert ring.n == int(g[f"ring{i}_n"])
+        assert np.array_equal(np.array(ring.idx), g[f"ring{i}_idx"]), i
+        assert np.array_equal(ring.element_positions, g[f"ring{i}_pos"])
+        i += 1
+    assert i >= 7
+
+
+def test_cylinder_indices_match_reference():
+    from openusct_amd.geometry import CylinderArray
+    g = golden("geometry.npz")
+    i = 0
+    while f"cyl{i}_params" in g:
+        nr, pr, r, ht, d, h = g[f"cyl{i}_params"]
+        cyl = CylinderArray(n_rings=int(nr), per_ring=int(pr), radius_m=float(r),
+                            height_m=float(ht), domain_m=float(d), h=float(h))
+        assert cyl.n == int(g[f"cyl{i}_n"])
+        assert np.array_equal(np.array(cyl.idx), g[f"cyl{i}_idx"]), i
+        assert np.array_equal(cyl.element_positions, g[f"cyl{i}_pos"])
+        i += 1
+    assert i >= 4
+
+
+def test_c1_has_half_integer_ties_resolved_half_even():
+    """C1 element 0 sits at x/h = 237.5: Python's round goes to the even 238."""
+    from openusct_amd.geometry import RingArray
+    ring = RingArray(64, 0.022, 0.051, 2e-4)
+    assert ring.xy[0][0] / 2e-4 == pytest.approx(237.5)
+    assert ring.idx[0][1] == 238
+
+
+def test_remap_matches_fwi_from_dataset():
+    from openusct_amd.geometry import RingArray, remap_positions
+    rm = golden("remap.npz")
+    i = 0
+    while f"remap{i}_params" in rm:
+        ne, r, d, h = rm[f"remap{i}_params"]
+        ring = RingArray(int(ne), float(r), float(d), float(h))
+        idx = remap_positions(ring.element_positions, ring.n, float(h))
+        assert np.array_equal(np.array(idx), rm[f"remap{i}_idx"])
+        if f"remap{i}_fwi_idx" in rm:
+            assert np.array_equal(np.array(idx), rm[f"remap{i}_fwi_idx"])
+        i += 1
+
+
+def test_ricker_matches_reference():
+    from openusct_amd.sources import ricker
+    s = golden("sources.npz")
+    i = 0
+    while f"ricker{i}_params" in s:
+        nt, dt, f0, t0 = s[f"ricker{i}_params"]
+        w = ricker(int(nt), float(dt), float(f0), None if np.isnan(t0) else float(t0))
+        assert np.array_equal(w, s[f"ricker{i}"])
+        i += 1
+
+
+def test_phantoms_and_sponge_match_reference():
+    from openusct_amd import phantom
+    from openusct_amd.solver import make_sponge
+    p = golden("phantom.npz")
+    c = phantom.coupling_background((61, 61), 3000.0, 1480.0, 0.02, 1e-3)
+    assert np.array_equal(c, p["coupling"])
+    assert np.array_equal(phantom.add_inclusion(c, (0.58, 0.46), 0.006, 2650.0, 1e-3),
+                          p["inclusion"])
+    c3 = phantom.cylinder_background((21, 21, 21), 3000.0, 1480.0, 0.008, 1.5e-3)
+    assert np.array_equal(c3, p["cylinder"])
+    assert np.array_equal(phantom.add_sphere(c3, (0.56, 0.5, 0.45), 0.004, 2700.0, 1.5e-3),
+                          p["sphere"])
+    assert np.array_equal(phantom.velocity_to_m(p["inclusion"]), p["m_of_c"])
+    assert np.array_equal(phantom.m_to_velocity(p["m_of_c"]), p["c_of_m"])
+    assert np.array_equal(make_sponge((61, 61)), p["sponge_61"])
+    assert np.array_equal(make_sponge((40, 70), width=10, strength=0.1), p["sponge_40x70"])
+
+
+def test_seeded_phantoms_are_reproducible_and_in_range():
+    from openusct_amd import configs
+    a, b = configs.c1(nt_scale=0.01), configs.c1(nt_scale=0.01)
+    assert np.array_equal(a.c_true, b.c_true)
+    vals = np.unique(a.c_true)
+    assert vals.min() >= 1400.0 and vals.max() <= 1800.0 and 1500.0 in vals
+    assert len(vals) >= 3
+    c4 = configs.c4(n=64, nt_scale=0.01)
+    assert 1400.0 <= c4.c_true.min() and c4.c_true.max() <= 1800.0
+
+
+def test_configs_satisfy_cfl_and_shapes():
+    from openusct_amd import configs
+    for name, shape, nshots in [("C1", (256, 256), 8), ("C2", (512, 512), 128),
+                                ("C3", (1024, 1024), 256)]:
+        p = configs.BUILDERS[name](nt_scale=0.001)
+        assert p.shape == shape and len(p.src_list) == nshots
+        assert 1800.0 * p.dt / p.h <= 1 / np.sqrt(2)
+    p4 = configs.c4(nt_scale=0.001)
+    assert p4.shape == (256, 256, 256) and len(p4.src_list) == 64 and p4.geom.n_elements == 512
+    assert 1800.0 * p4.dt / p4.h <= 1 / np.sqrt(3)
+
+
+def test_linear_indices_row_major_and_bounds():
+    from openusct_amd.plan import linear_indices
+    assert list(linear_indices([(1, 2), (0, 0), (3, 4)], (4, 5))) == [7, 0, 19]
+    assert list(linear_indices([(1, 2, 3)], (4, 5, 6))) == [1 * 30 + 2 * 6 + 3]
+    with pytest.raises(ValueError):
+        linear_indices([(4, 0)], (4, 5))
+
+
+def test_cfl_check_rejects_unstable_dt():
+    from openusct_amd.fwi import check_cfl
+    m = np.full((8, 8), 1.0 / 1500.0 ** 2)
+    check_cfl(m, 5e-8, 2e-4)  # Courant 0.375
+    with pytest.raises(ValueError):
+        check_cfl(m, 1e-7, 2e-4)  # Courant 0.75 > 0.707
+    with pytest.raises(ValueError):
+        check_cfl(np.zeros((4, 4)), 1e-8, 1e-3)
+
+
+def test_fwi_rejects_unsupported_options_before_touching_the_device():
+    from openusct_amd import fwi
+    g = IdxGeom([(1, 1)])
+    m = np.full((8, 8), 1.0 / 1500.0 ** 2)
+    with pytest.raises(NotImplementedError):
+        fwi.forward_fmc(m, g, np.zeros(4), 1e-8, 1e-3, 4, footprints=[])
+    with pytest.raises(NotImplementedError):
+        fwi.misfit(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)), misfit_type="envelope")
+    with pytest.raises(NotImplementedError):
+        fwi.misfit_and_gradient(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)),
+                                time_weights=[np.ones((4, 1))])
+
+
+def test_plugin_registry():
+    from openusct_amd import plugins
+    assert "fwi" in plugins.available()
+    assert plugins.info("fwi")["output"] == "model"
+    with pytest.raises(KeyError):
+        plugins.run("nope", None)
+
+    @plugins.register("echo", description="test")
+    def _echo(ds, **kw):
+        return ds, kw
+
+    assert plugins.run("echo", 3, a=1) == (3, {"a": 1})
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from openusct_amd import _lib
+    lib = _lib.load()
+    header = open(os.path.join(ROOT, "include", "usct.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*(usct_\w+)\(", header, re.M))
+    assert declared == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.usct_version() == 1
+    # an error path that needs no device: bad dims are rejected before any HIP call
+    plan = ctypes.c_void_p()
+    dims = (ctypes.c_int * 2)(0, 5)
+    rc = lib.usct_plan_create(ctypes.byref(plan), 2, dims, 1e-3, 1e-7, 10, None, 0, None, 0,
+                              2, 0, 0, 0)
+    assert rc == _lib.USCT_EINVAL
+    assert b"dims" in lib.usct_last_error()
+
+
+def test_footprint_model_orders():
+    from openusct_amd.plan import footprint_bytes
+    small = footprint_bytes((256, 256), 1000, 8, 64)
+    big = footprint_bytes((512, 512), 1000, 128, 128)
+    assert 0 < small < big
+    assert footprint_bytes((512, 512), 1000, 128, 128, ckpt_every=50) < \
+        footprint_bytes((512, 512), 1000, 128, 128, ckpt_every=999)
