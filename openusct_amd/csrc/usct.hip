@@ -258,6 +258,20 @@ void launch_fwd_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl p
         if (record) fwd3d_sel<true>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
         else fwd3d_sel<false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     }
+    // FP64 update of each shot's source cell, overwriting what the stream kernel wrote
+    float *d2 = P->srcd2 + (sinc - P->sinc);
+    const dim3 gs((unsigned)cdiv(P->Bp, 64)), bs(64);
+#define USCT_FIX(SP, ND, RC)                                                                   \
+    hipLaunchKernelGGL((k_src_fix<SP, ND, RC>), gs, bs, 0, s, pc.hi, pc.lo, pp.hi, pp.lo, pn.hi, \
+                       pn.lo, P->sp2, P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, rec, \
+                       rl, d2, P->nz, P->ny, P->nx, P->Bp)
+    if (P->ndim == 2) {
+        if (P->sponge) { if (record) USCT_FIX(true, 2, true); else USCT_FIX(true, 2, false); }
+        else { if (record) USCT_FIX(false, 2, true); else USCT_FIX(false, 2, false); }
+    } else {
+        if (record) USCT_FIX(false, 3, true); else USCT_FIX(false, 3, false);
+    }
+#undef USCT_FIX
 }
 
 template <int LB, bool SP>
@@ -679,6 +693,21 @@ int choose_batched(usct_plan *P, int ckpt_every) {
     return 0;
 }
 
+// Rows (2D) or planes (3D) per wavefront: the largest power of two in [4, 64]
+// that still gives about 8 wavefronts per SIMD (256 CUs x 4 SIMDs), so the
+// streaming loads of enough wavefronts are in flight to cover HBM latency.
+void default_rows(usct_plan *P) {
+    const int W = kWave / P->LB;
+    const int64_t nxs = cdiv(P->nx, (int64_t)W * P->CX);
+    const int64_t cols = P->ndim == 2 ? (int64_t)P->ng() * nxs
+                                      : (int64_t)P->ng() * nxs * cdiv(P->ny, P->CY);
+    const int64_t depth = P->ndim == 2 ? P->ny : P->nz;
+    const int64_t target = 256 * 4 * 8;
+    int r = 64;
+    while (r > 4 && cols * cdiv(depth, r) < target) r >>= 1;
+    P->RY = r;
+}
+
 int allocate_batched(usct_plan *P) {
     const int64_t FE = P->field_elems();
     TRY(dalloc_t(P, &P->Z, FE));
@@ -813,11 +842,11 @@ int autotune(usct_plan *P) {
     std::vector<Cand> cands;
     if (P->ndim == 2) {
         for (int lb : lbs)
-            for (int ry : {16, 32, 64, 128}) cands.push_back({lb, 8, 1, ry});
+            for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, 8, 1, ry});
     } else {
         for (int lb : lbs)
             for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}})
-                for (int rz : {16, 32, 64}) {
+                for (int rz : {4, 8, 16, 32, 64}) {
                     if (pc.first * pc.second > lb) continue;
                     cands.push_back({lb, pc.first, pc.second, rz});
                 }
@@ -981,6 +1010,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
             if ((rc = choose_batched(P, ckpt_every))) return bail(rc);
             P->budget += P->bytes;
             if ((rc = allocate_batched(P))) return bail(rc);
+            default_rows(P);
             if ((flags & USCT_FLAG_AUTOTUNE) && (rc = autotune(P))) return bail(rc);
         }
     }

@@ -115,6 +115,51 @@ __device__ __forceinline__ float scale_hilo(float hi, float lo, float w, float &
 // the forward actually applied.
 __device__ __forceinline__ double hl(float h, uint16_t l) { return (double)h + (double)bf2f(l); }
 
+// Re-do the update of one source cell (address a, cell c) in FP64 from the
+// hi + lo state, overwrite its new level and trace samples, and record the
+// increment.  Kept out of line so the streaming loops stay unchanged.
+// st[0..2] are the x, y, z strides in elements (z unused in 2D).
+template <bool SPONGE, int NDIM, bool REC>
+__device__ __forceinline__ void source_cell_update(
+    const float *__restrict__ pc, const uint16_t *__restrict__ pcl, const float *__restrict__ pp,
+    const uint16_t *__restrict__ ppl, float *__restrict__ pn, uint16_t *__restrict__ pnl,
+    const float2 *__restrict__ sp2, const float *__restrict__ sponge, float inc_src, size_t a,
+    int c, size_t sx, size_t sy, size_t sz, bool interior, const int *__restrict__ rec_head,
+    const int *__restrict__ rec_next, float *__restrict__ rec, float *__restrict__ rec_lo,
+    size_t BP, int b, float *__restrict__ srcd2) {
+    const float ph = pc[a];
+    const uint16_t pl = pcl[a];
+    double lapd = 0.0;
+    if (interior) {
+        lapd = -2.0 * NDIM * hl(ph, pl);
+        if (NDIM == 3) {
+            lapd += hl(pc[a + sz], pcl[a + sz]);
+            lapd += hl(pc[a - sz], pcl[a - sz]);
+        }
+        lapd += hl(pc[a + sy], pcl[a + sy]);
+        lapd += hl(pc[a - sy], pcl[a - sy]);
+        lapd += hl(pc[a + sx], pcl[a + sx]);
+        lapd += hl(pc[a - sx], pcl[a - sx]);
+    }
+    const float2 s2 = sp2[c];
+    const float inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)inc_src);
+    const float qh = pp[a];
+    const uint16_t ql = ppl[a];
+    float lo;
+    float v = leap_hilo(ph, bf2f(pl), qh, bf2f(ql), inc, lo);
+    if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
+    const uint16_t lb = f2bf(lo);
+    pn[a] = v;
+    pnl[a] = lb;
+    srcd2[b] = (float)((hl(v, lb) - 2.0 * hl(ph, pl)) + hl(qh, ql));
+    if (REC) {
+        for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
+            rec[(size_t)j * BP + b] = v;
+            rec_lo[(size_t)j * BP + b] = bf2f(lb);
+        }
+    }
+}
+
 __device__ __forceinline__ void neumaier(float &s, float &c, float x) {
     const float t = s + x;
     if (fabsf(s) >= fabsf(x)) c = c + ((s - t) + x);
@@ -208,32 +253,14 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                 const int c = y * nx + x;
                 float lu = 0.0f;
                 if (yint && x >= 1 && x <= nx - 2) lu = lap5(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
-                const float2 s2 = sp2[c];
-                float inc = smul(s2, lu);
+                const float inc = smul(sp2[c], lu);
                 const size_t a = (size_t)c * BP + b;
-                const uint16_t cl_ = pcl[a];
-                double lapd = 0.0;
-                if (c == my_src) {
-                    if (yint && x >= 1 && x <= nx - 2) {
-                        const size_t sx = BP, sy = (size_t)nx * BP;
-                        lapd = -4.0 * hl(rc[i + 1], cl_);
-                        lapd += hl(rp[i], pcl[a + sy]);
-                        lapd += hl(rm[i], pcl[a - sy]);
-                        lapd += hl(rc[i + 2], pcl[a + sx]);
-                        lapd += hl(rc[i], pcl[a - sx]);
-                    }
-                    inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)my_inc);
-                }
                 float lo;
-                const float qh = pp[a];
-                const uint16_t ql = ppl[a];
-                float v = leap_hilo(rc[i + 1], bf2f(cl_), qh, bf2f(ql), inc, lo);
+                float v = leap_hilo(rc[i + 1], bf2f(pcl[a]), pp[a], bf2f(ppl[a]), inc, lo);
                 if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
                 const uint16_t lb = f2bf(lo);
                 pn[a] = v;
                 pnl[a] = lb;
-                if (c == my_src)
-                    srcd2[b] = (float)((hl(v, lb) - 2.0 * hl(rc[i + 1], cl_)) + hl(qh, ql));
                 if (REC) {
                     for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
                         rec[(size_t)j * BP + b] = v;
@@ -455,33 +482,14 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     if (yint && x >= 1 && x <= nx - 2)
                         lu = lap7(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
-                    const float2 s2 = sp2[c];
-                    float inc = smul(s2, lu);
+                    const float inc = smul(sp2[c], lu);
                     const size_t a = (size_t)c * BP + b;
-                    const uint16_t cl_ = pcl[a];
-                    if (c == my_src) {
-                        double lapd = 0.0;
-                        if (yint && x >= 1 && x <= nx - 2) {
-                            const size_t sx = BP, sy = (size_t)nx * BP, sz = (size_t)ny * nx * BP;
-                            lapd = -6.0 * hl(zcu[j + 1][i + 1], cl_);
-                            lapd += hl(zp[j][i], pcl[a + sz]);
-                            lapd += hl(zm[j][i], pcl[a - sz]);
-                            lapd += hl(zcu[j + 2][i + 1], pcl[a + sy]);
-                            lapd += hl(zcu[j][i + 1], pcl[a - sy]);
-                            lapd += hl(zcu[j + 1][i + 2], pcl[a + sx]);
-                            lapd += hl(zcu[j + 1][i], pcl[a - sx]);
-                        }
-                        inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)my_inc);
-                    }
                     float lo;
-                    const float qh = pp[a];
-                    const uint16_t ql = ppl[a];
-                    const float v = leap_hilo(zcu[j + 1][i + 1], bf2f(cl_), qh, bf2f(ql), inc, lo);
+                    const float v = leap_hilo(zcu[j + 1][i + 1], bf2f(pcl[a]), pp[a], bf2f(ppl[a]),
+                                              inc, lo);
                     const uint16_t lb = f2bf(lo);
                     pn[a] = v;
                     pnl[a] = lb;
-                    if (c == my_src)
-                        srcd2[b] = (float)((hl(v, lb) - 2.0 * hl(zcu[j + 1][i + 1], cl_)) + hl(qh, ql));
                     if (REC) {
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
                             rec[(size_t)r * BP + b] = v;
@@ -679,41 +687,27 @@ k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *_
         if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
             lu = lap7(pcv, pc[c + sz], pc[c - sz], pc[c + nx], pc[c - nx], pc[c + 1], pc[c - 1]);
     }
-    const float2 s2 = sp2[c];
-    float inc = smul(s2, lu);
     if (c == src) {
-        double lapd = 0.0;
-        if (ndim == 2) {
-            if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2) {
-                lapd = -4.0 * hl(pcv, pcl[c]);
-                lapd += hl(pc[c + nx], pcl[c + nx]);
-                lapd += hl(pc[c - nx], pcl[c - nx]);
-                lapd += hl(pc[c + 1], pcl[c + 1]);
-                lapd += hl(pc[c - 1], pcl[c - 1]);
-            }
-        } else {
-            const int sz = nx * ny;
-            if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2) {
-                lapd = -6.0 * hl(pcv, pcl[c]);
-                lapd += hl(pc[c + sz], pcl[c + sz]);
-                lapd += hl(pc[c - sz], pcl[c - sz]);
-                lapd += hl(pc[c + nx], pcl[c + nx]);
-                lapd += hl(pc[c - nx], pcl[c - nx]);
-                lapd += hl(pc[c + 1], pcl[c + 1]);
-                lapd += hl(pc[c - 1], pcl[c - 1]);
-            }
-        }
-        inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)*sinc);
+        if (ndim == 2)
+            source_cell_update<SPONGE, 2, REC>(
+                pc, pcl, pp, ppl, pn, pnl, sp2, sponge, *sinc, (size_t)c, c, 1, (size_t)nx, 0,
+                y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2, rec_head, rec_next, rec, rec_lo, 1,
+                0, srcd2);
+        else
+            source_cell_update<SPONGE, 3, REC>(
+                pc, pcl, pp, ppl, pn, pnl, sp2, sponge, *sinc, (size_t)c, c, 1, (size_t)nx,
+                (size_t)nx * ny,
+                z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2, rec_head,
+                rec_next, rec, rec_lo, 1, 0, srcd2);
+        return;
     }
+    const float inc = smul(sp2[c], lu);
     float lo;
-    const float qh = pp[c];
-    const uint16_t ql = ppl[c];
-    float v = leap_hilo(pcv, bf2f(pcl[c]), qh, bf2f(ql), inc, lo);
+    float v = leap_hilo(pcv, bf2f(pcl[c]), pp[c], bf2f(ppl[c]), inc, lo);
     if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
     const uint16_t lb = f2bf(lo);
     pn[c] = v;
     pnl[c] = lb;
-    if (c == src) *srcd2 = (float)((hl(v, lb) - 2.0 * hl(pcv, pcl[c])) + hl(qh, ql));
     if (REC) {
         for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
             rec[j] = v;
@@ -771,6 +765,33 @@ k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *_
     neumaier(hi, lo, v * d2);
     g_hi[c] = hi;
     g_lo[c] = lo;
+}
+
+// ---------------------------------------------------------------------------
+// Batched source-cell update: one thread per shot, launched after each
+// forward step of the batched path (layout cell * Bp + shot).
+// ---------------------------------------------------------------------------
+template <bool SPONGE, int NDIM, bool REC>
+__global__ void __launch_bounds__(64)
+k_src_fix(const float *__restrict__ pc, const uint16_t *__restrict__ pcl,
+          const float *__restrict__ pp, const uint16_t *__restrict__ ppl, float *__restrict__ pn,
+          uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+          const float *__restrict__ sponge, const int *__restrict__ src_cell,
+          const float *__restrict__ sinc, const int *__restrict__ rec_head,
+          const int *__restrict__ rec_next, float *__restrict__ rec, float *__restrict__ rec_lo,
+          float *__restrict__ srcd2, int nz, int ny, int nx, int Bp) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= Bp) return;
+    const int c = src_cell[b];
+    if (c < 0) return;
+    const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+    bool in = y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2;
+    if (NDIM == 3) in = in && z >= 1 && z <= nz - 2;
+    const size_t BP = (size_t)Bp;
+    source_cell_update<SPONGE, NDIM, REC>(pc, pcl, pp, ppl, pn, pnl, sp2, sponge, sinc[b],
+                                          (size_t)c * BP + b, c, BP, (size_t)nx * BP,
+                                          (size_t)nx * ny * BP, in, rec_head, rec_next, rec,
+                                          rec_lo, BP, b, srcd2);
 }
 
 // ---------------------------------------------------------------------------
