@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ckpt-every", type=int, default=0)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -140,7 +140,7 @@ def main():
     print(f"[bench] building batched plan ({len(tx)} shots, {n}x{n}, nt={nt})", file=sys.stderr,
           flush=True)
     plan = Plan((n, n), h, dt, nt, tx, rec, method="batched", ckpt_every=args.ckpt_every,
-                autotune=args.autotune)
+                autotune=not args.no_autotune)
     info = plan.info
     dobs = plan.forward_fmc(mt, wd)
     torch.cuda.synchronize()

@@ -46,6 +46,13 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     return xcd * per + (xcd < rem ? xcd : rem) + slot;
 }
 
+// Global wavefront index (XCD-remapped block, then wave within the block),
+// marked wave-uniform so everything derived from it lives in scalar registers.
+__device__ __forceinline__ int wave_id() {
+    const int w = (int)(threadIdx.x >> 6);
+    return xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + __builtin_amdgcn_readfirstlane(w);
+}
+
 __device__ __forceinline__ float lap5(float c, float yp, float ym, float xp, float xm) {
     float a = -4.0f * c;
     a = a + yp;
@@ -199,10 +206,27 @@ __device__ __forceinline__ int sum_cell(int lane) {
     return ci;
 }
 
+// Element at a byte offset from a wave-uniform base: lets the compiler use the
+// "scalar base + 32-bit vector offset" form of global loads and stores.
+template <class T>
+__device__ __forceinline__ T ldb(const T *base, unsigned boff) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+template <class T>
+__device__ __forceinline__ void stb(T *base, unsigned boff, T v) {
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff) = v;
+}
+
 // ---------------------------------------------------------------------------
 // Batched 2D forward step: level n from levels n-1 (pc) and n-2 (pp) into pn.
 // sinc[b] = S_src(b) * w[n-1] (FP64 product rounded once); rec: [n_rec][Bp] row
-// of level n.  Grid: ng * nxs * nyc wavefronts, 4 per block.
+// of level n; row_rec[y] = receivers on grid row y.  Grid: ng * nxs * nyc
+// wavefronts, 4 per block.
+//
+// Addressing: a field row starts at a wave-uniform pointer (scalar registers)
+// and each lane keeps 32-bit element offsets of its CX + 2 columns (x clamped
+// into the grid), so every load is "uniform base + 32-bit lane offset" with no
+// per-access 64-bit arithmetic.
 // ---------------------------------------------------------------------------
 template <int LB, int CX, bool SPONGE, bool REC>
 __global__ void __launch_bounds__(kBlock)
@@ -211,10 +235,11 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
         const float *__restrict__ sponge, const int *__restrict__ src_cell,
         const float *__restrict__ sinc, const int *__restrict__ rec_head,
-        const int *__restrict__ rec_next, float *__restrict__ rec, float *__restrict__ rec_lo,
-        float *__restrict__ srcd2, int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+        const int *__restrict__ rec_next, const int *__restrict__ row_rec,
+        float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int ny,
+        int nx, int Bp, int RY, int nxs, int nyc, int ng) {
     constexpr int W = kWave / LB;
-    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     const int t = wv / ng;
@@ -223,48 +248,59 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
+    if (x0 >= nx) return;
     const int y0 = yc * RY;
     const int y1 = min(y0 + RY, ny);
-    if (x0 >= nx) return;
-    const size_t BP = (size_t)Bp;
-    const int my_src = src_cell[b];
-    const float my_inc = sinc[b];
-    auto at = [&](int y, int x) -> size_t {
-        x = min(max(x, 0), nx - 1);
-        return ((size_t)y * nx + x) * BP + b;
-    };
+    const size_t rowE = (size_t)nx * Bp;
+    (void)src_cell;
+    (void)sinc;
+    (void)srcd2;
+    unsigned xc[CX + 2];  // clamped column of each stencil position
+    unsigned eo[CX + 2];  // its element offset within a field row
+#pragma unroll
+    for (int i = 0; i < CX + 2; ++i) {
+        xc[i] = (unsigned)min(max(x0 - 1 + i, 0), nx - 1);
+        eo[i] = xc[i] * (unsigned)Bp + (unsigned)b;
+    }
     float rm[CX], rc[CX + 2], rp[CX];
     {
-        const int ym = max(y0 - 1, 0);
+        const float *r0 = pc + (size_t)max(y0 - 1, 0) * rowE;
+        const float *r1 = pc + (size_t)y0 * rowE;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) rm[i] = pc[at(ym, x0 + i)];
+        for (int i = 0; i < CX; ++i) rm[i] = ldb(r0, 4u * eo[i + 1]);
 #pragma unroll
-        for (int i = 0; i < CX + 2; ++i) rc[i] = pc[at(y0, x0 - 1 + i)];
+        for (int i = 0; i < CX + 2; ++i) rc[i] = ldb(r1, 4u * eo[i]);
     }
     for (int y = y0; y < y1; ++y) {
-        const int yp = min(y + 1, ny - 1);
+        const float *rpn = pc + (size_t)min(y + 1, ny - 1) * rowE;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) rp[i] = pc[at(yp, x0 + i)];
+        for (int i = 0; i < CX; ++i) rp[i] = ldb(rpn, 4u * eo[i + 1]);
         const bool yint = (y >= 1) && (y <= ny - 2);
+        const float *ppr = pp + (size_t)y * rowE;
+        const uint16_t *pclr = pcl + (size_t)y * rowE;
+        const uint16_t *pplr = ppl + (size_t)y * rowE;
+        float *pnr = pn + (size_t)y * rowE;
+        uint16_t *pnlr = pnl + (size_t)y * rowE;
+        const float2 *spr = sp2 + (size_t)y * nx;
+        const bool has_rec = REC && row_rec[y] > 0;
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
             const int x = x0 + i;
             if (x < nx) {
-                const int c = y * nx + x;
                 float lu = 0.0f;
                 if (yint && x >= 1 && x <= nx - 2) lu = lap5(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
-                const float inc = smul(sp2[c], lu);
-                const size_t a = (size_t)c * BP + b;
+                const float inc = smul(ldb(spr, 8u * xc[i + 1]), lu);
+                const unsigned e = eo[i + 1];
                 float lo;
-                float v = leap_hilo(rc[i + 1], bf2f(pcl[a]), pp[a], bf2f(ppl[a]), inc, lo);
-                if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
+                float v = leap_hilo(rc[i + 1], bf2f(ldb(pclr, 2u * e)), ldb(ppr, 4u * e), bf2f(ldb(pplr, 2u * e)), inc, lo);
+                if (SPONGE) v = scale_hilo(v, lo, sponge[(size_t)y * nx + x], lo);
                 const uint16_t lb = f2bf(lo);
-                pn[a] = v;
-                pnl[a] = lb;
-                if (REC) {
-                    for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
-                        rec[(size_t)j * BP + b] = v;
-                        rec_lo[(size_t)j * BP + b] = bf2f(lb);
+                stb(pnr, 4u * e, v);
+                stb(pnlr, 2u * e, lb);
+                if (has_rec) {
+                    for (int j = rec_head[y * nx + x]; j >= 0; j = rec_next[j]) {
+                        rec[(size_t)j * Bp + b] = v;
+                        rec_lo[(size_t)j * Bp + b] = bf2f(lb);
                     }
                 }
             }
@@ -272,8 +308,8 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         if (y + 1 < y1) {
 #pragma unroll
             for (int i = 0; i < CX; ++i) { rm[i] = rc[i + 1]; rc[i + 1] = rp[i]; }
-            rc[0] = pc[at(y + 1, x0 - 1)];
-            rc[CX + 1] = pc[at(y + 1, x0 + CX)];
+            rc[0] = ldb(rpn, 4u * eo[0]);
+            rc[CX + 1] = ldb(rpn, 4u * eo[CX + 1]);
         }
     }
 }
@@ -282,7 +318,8 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 // Batched 2D adjoint step with the imaging condition.
 //   l1 = lam^{k+1}, l2 = lam^{k+2}, lk = lam^k (output)
 //   no sponge: pa = p^{k-1};  sponge: pa = p^k, pb = p^{k-1}, pc2 = p^{k-2}
-//   sinc[b] = S_src(b) w[k-1]; res = [n_rec][Bp] residual row of level k
+//   res = [n_rec][Bp] residual row of level k; srcd2[b] = recorded increment
+//   of level k at shot b's source cell
 //   g_hi/g_lo: [ng][N] Neumaier accumulators of sum_shots sum_k lam^k d2p^k
 // ---------------------------------------------------------------------------
 template <int LB, int CX, bool SPONGE>
@@ -292,10 +329,11 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         const float2 *__restrict__ sp2, const float *__restrict__ sponge,
         const int *__restrict__ src_cell, const float *__restrict__ sinc,
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-        const float *__restrict__ res, const float *__restrict__ srcd2, float *__restrict__ g_hi,
-        float *__restrict__ g_lo, int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+        const int *__restrict__ row_rec, const float *__restrict__ res,
+        const float *__restrict__ srcd2, float *__restrict__ g_hi, float *__restrict__ g_lo,
+        int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
     constexpr int W = kWave / LB;
-    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     const int t = wv / ng;
@@ -303,72 +341,89 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     if (yc >= nyc) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
-    const int x0 = (xs * W + st) * CX;  // may be >= nx: lane still joins the reductions
+    const int x0 = (xs * W + st) * CX;  // may be >= nx: the lane still joins the reductions
     const int y0 = yc * RY;
     const int y1 = min(y0 + RY, ny);
-    const size_t BP = (size_t)Bp;
+    const size_t rowE = (size_t)nx * Bp;
     const size_t N = (size_t)ny * nx;
+    (void)sinc;
     const int my_src = src_cell[b];
     const float my_d2 = srcd2[b];
-    const int xl = min(x0, nx - 1);
-    auto at = [&](int y, int x) -> size_t {
-        x = min(max(x, 0), nx - 1);
-        return ((size_t)y * nx + x) * BP + b;
-    };
-    auto cl = [&](int y, int x) -> int { return y * nx + min(max(x, 0), nx - 1); };
-    // q = S' lam^{k+1} rows (qm, qc with halo, qp); raw centre row of lam^{k+1}.
+    unsigned xc[CX + 2];  // clamped column of each stencil position
+    unsigned eo[CX + 2];  // its element offset within a field row
+#pragma unroll
+    for (int i = 0; i < CX + 2; ++i) {
+        xc[i] = (unsigned)min(max(x0 - 1 + i, 0), nx - 1);
+        eo[i] = xc[i] * (unsigned)Bp + (unsigned)b;
+    }
+    // q = S' lam^{k+1} rows (qm, qc with halo, qp); lc: raw centre row of lam^{k+1}.
     float qm[CX], qc[CX + 2], qp[CX], lc[CX], lpn[CX];
     float fm[CX], fc[CX + 2], fp[CX];  // p^{k-1} rows (no sponge)
-    (void)xl;
     {
         const int ym = max(y0 - 1, 0);
+        const float *a0 = l1 + (size_t)ym * rowE;
+        const float *a1 = l1 + (size_t)y0 * rowE;
+        const float2 *s0 = sp2 + (size_t)ym * nx;
+        const float2 *s1 = sp2 + (size_t)y0 * nx;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) qm[i] = smul(sp2[cl(ym, x0 + i)], l1[at(ym, x0 + i)]);
+        for (int i = 0; i < CX; ++i) qm[i] = smul(ldb(s0, 8u * xc[i + 1]), ldb(a0, 4u * eo[i + 1]));
 #pragma unroll
         for (int i = 0; i < CX + 2; ++i) {
-            const float v = l1[at(y0, x0 - 1 + i)];
-            qc[i] = smul(sp2[cl(y0, x0 - 1 + i)], v);
+            const float v = ldb(a1, 4u * eo[i]);
+            qc[i] = smul(ldb(s1, 8u * xc[i]), v);
             if (i >= 1 && i <= CX) lc[i - 1] = v;
         }
         if (!SPONGE) {
+            const float *f0 = pa + (size_t)ym * rowE;
+            const float *f1 = pa + (size_t)y0 * rowE;
 #pragma unroll
-            for (int i = 0; i < CX; ++i) fm[i] = pa[at(ym, x0 + i)];
+            for (int i = 0; i < CX; ++i) fm[i] = ldb(f0, 4u * eo[i + 1]);
 #pragma unroll
-            for (int i = 0; i < CX + 2; ++i) fc[i] = pa[at(y0, x0 - 1 + i)];
+            for (int i = 0; i < CX + 2; ++i) fc[i] = ldb(f1, 4u * eo[i]);
         }
     }
     for (int y = y0; y < y1; ++y) {
         const int yp = min(y + 1, ny - 1);
+        const float *lnr = l1 + (size_t)yp * rowE;
+        const float2 *snr = sp2 + (size_t)yp * nx;
+        const float *fnr = pa + (size_t)yp * rowE;
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
-            const float v = l1[at(yp, x0 + i)];
+            const float v = ldb(lnr, 4u * eo[i + 1]);
             lpn[i] = v;
-            qp[i] = smul(sp2[cl(yp, x0 + i)], v);
-            if (!SPONGE) fp[i] = pa[at(yp, x0 + i)];
+            qp[i] = smul(ldb(snr, 8u * xc[i + 1]), v);
+            if (!SPONGE) fp[i] = ldb(fnr, 4u * eo[i + 1]);
         }
         const bool yint = (y >= 1) && (y <= ny - 2);
+        const float *l2r = l2 + (size_t)y * rowE;
+        float *lkr = lk + (size_t)y * rowE;
+        const float2 *spr = sp2 + (size_t)y * nx;
+        const bool has_rec = row_rec[y] > 0;
         float tv[CX];
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
             const int x = x0 + i;
             tv[i] = 0.0f;
             if (x < nx) {
-                const int c = y * nx + x;
                 const bool in = yint && x >= 1 && x <= nx - 2;
                 const float lq = in ? lap5(qc[i + 1], qp[i], qm[i], qc[i + 2], qc[i]) : 0.0f;
-                const size_t a = (size_t)c * BP + b;
-                float v = leap(lc[i], l2[a], lq);
-                if (SPONGE) v = v * sponge[c];
-                for (int j = rec_head[c]; j >= 0; j = rec_next[j]) v = v - res[(size_t)j * BP + b];
-                lk[a] = v;
+                const unsigned e = eo[i + 1];
+                float v = leap(lc[i], ldb(l2r, 4u * e), lq);
+                if (SPONGE) v = v * sponge[(size_t)y * nx + x];
+                if (has_rec) {
+                    for (int j = rec_head[y * nx + x]; j >= 0; j = rec_next[j])
+                        v = v - res[(size_t)j * Bp + b];
+                }
+                stb(lkr, 4u * e, v);
                 float d2;
                 if (SPONGE) {
+                    const size_t a = (size_t)y * rowE + e;
                     d2 = (pa[a] - 2.0f * pb[a]) + pc2[a];
                 } else {
                     const float lp = in ? lap5(fc[i + 1], fp[i], fm[i], fc[i + 2], fc[i]) : 0.0f;
-                    d2 = smul(sp2[c], lp);
+                    d2 = smul(ldb(spr, 8u * xc[i + 1]), lp);
                 }
-                if (c == my_src) d2 = my_d2;
+                if (y * nx + x == my_src) d2 = my_d2;
                 tv[i] = v * d2;
             }
         }
@@ -376,11 +431,12 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         if ((sb & (LB / CX - 1)) == 0) {
             const int x = x0 + sum_cell<LB, CX>(lane);
             if (x < nx) {
-                const size_t gi = (size_t)sg * N + (size_t)y * nx + x;
-                float hi = g_hi[gi], lo = g_lo[gi];
+                float *gh = g_hi + (size_t)sg * N + (size_t)y * nx;
+                float *gl = g_lo + (size_t)sg * N + (size_t)y * nx;
+                float hi = gh[x], lo = gl[x];
                 neumaier(hi, lo, s);
-                g_hi[gi] = hi;
-                g_lo[gi] = lo;
+                gh[x] = hi;
+                gl[x] = lo;
             }
         }
         if (y + 1 < y1) {
@@ -390,13 +446,13 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                 qc[i + 1] = qp[i];
                 lc[i] = lpn[i];
             }
-            qc[0] = smul(sp2[cl(y + 1, x0 - 1)], l1[at(y + 1, x0 - 1)]);
-            qc[CX + 1] = smul(sp2[cl(y + 1, x0 + CX)], l1[at(y + 1, x0 + CX)]);
+            qc[0] = smul(ldb(snr, 8u * xc[0]), ldb(lnr, 4u * eo[0]));
+            qc[CX + 1] = smul(ldb(snr, 8u * xc[CX + 1]), ldb(lnr, 4u * eo[CX + 1]));
             if (!SPONGE) {
 #pragma unroll
                 for (int i = 0; i < CX; ++i) { fm[i] = fc[i + 1]; fc[i + 1] = fp[i]; }
-                fc[0] = pa[at(y + 1, x0 - 1)];
-                fc[CX + 1] = pa[at(y + 1, x0 + CX)];
+                fc[0] = ldb(fnr, 4u * eo[0]);
+                fc[CX + 1] = ldb(fnr, 4u * eo[CX + 1]);
             }
         }
     }
@@ -415,7 +471,7 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int nz,
         int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
     constexpr int W = kWave / LB;
-    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     int t = wv / ng;
@@ -527,7 +583,7 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         int nzc, int ng) {
     constexpr int W = kWave / LB;
     constexpr int CC = CX * CY;
-    const int wv = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (threadIdx.x >> 6);
+    const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     int t = wv / ng;

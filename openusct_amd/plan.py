@@ -45,7 +45,7 @@ class Plan:
     """A reusable solver plan on the current HIP device."""
 
     def __init__(self, shape, h, dt, nt, tx_lin, rec_lin, method="batched", ckpt_every=0,
-                 mem_budget=0, sponge=False, autotune=False, graph=True):
+                 mem_budget=0, sponge=False, autotune=True, graph=True):
         lib = _lib.load()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
