@@ -32,21 +32,22 @@ BYTES_ADJ = 28.0   # adjoint + imaging update
 BYTES_PER_UPDATE = 22.0
 
 
-def workload(n_el=128, seed=2):
-    import numpy as np
+WORKLOADS = {
+    "C2": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 128-element ring R=22mm, "
+          "128 shots, Ricker 500kHz, FP32 fields; one step = one misfit+gradient over all shots",
+    "C2-256": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 256-element ring R=22mm, "
+              "256 shots, Ricker 500kHz, FP32 fields; one step = one misfit+gradient",
+    "C3": "2D ring FMC 1024x1024, dx=0.1mm, dt=25ns, nt=4000, 256-element ring R=44mm, "
+          "256 shots, Ricker 500kHz, FP32 fields, checkpointed; one step = one misfit+gradient",
+    "C4": "3D cylinder 256^3, dx=0.25mm, dt=30ns, nt=2000, 8 rings x 64 elements R=28mm, "
+          "64 shots, Ricker 500kHz, FP32 fields, checkpoint every 50 steps; one step = one "
+          "misfit+gradient",
+}
 
-    from openusct_amd import phantom
-    from openusct_amd.geometry import RingArray
-    from openusct_amd.sources import ricker
 
-    h, dt, nt = 2e-4, 5e-8, 1000
-    geom = RingArray(n_elements=n_el, radius_m=0.022, domain_m=0.1022, h=h)
-    n = geom.n
-    w = ricker(nt, dt, 5e5)
-    c_true = phantom.seeded_discs(n, h, seed, 3)
-    m_true = phantom.velocity_to_m(c_true)
-    m0 = np.full((n, n), 1.0 / 1500.0 ** 2)
-    return geom, h, dt, nt, w, m_true, m0
+def workload(name="C2"):
+    from openusct_amd import configs
+    return configs.BUILDERS[name]()
 
 
 def cpu_baseline(budget_s=20.0):
@@ -109,7 +110,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--shots", type=int, default=128)
+    ap.add_argument("--workload", default="C2", choices=sorted(WORKLOADS))
+    ap.add_argument("--shots", type=int, default=0, help="use only the first N shots")
     ap.add_argument("--plain-shots", type=int, default=128)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -128,19 +130,22 @@ def main():
     from openusct_amd.plan import Plan, linear_indices
 
     dev = torch.device("cuda", 0)
-    geom, h, dt, nt, w, m_true, m0 = workload()
-    n = geom.n
-    N = n * n
-    rec = linear_indices(geom.idx, (n, n))
-    tx = rec[: args.shots].copy()
-    wd = torch.tensor(w, device=dev)
-    mt = torch.tensor(m_true, device=dev)
-    md = torch.tensor(m0, device=dev)
+    prob = workload(args.workload)
+    shape, h, dt, nt = prob.shape, prob.h, prob.dt, prob.nt
+    N = int(np.prod(shape))
+    rec = linear_indices(prob.geom.idx, shape)
+    src = prob.src_list[: args.shots] if args.shots else prob.src_list
+    tx = rec[src].copy()
+    wd = torch.tensor(prob.wavelet, device=dev)
+    mt = torch.tensor(prob.m_true, device=dev)
+    md = torch.tensor(prob.m0, device=dev)
+    ckpt = args.ckpt_every or prob.ckpt_every
 
-    print(f"[bench] building batched plan ({len(tx)} shots, {n}x{n}, nt={nt})", file=sys.stderr,
-          flush=True)
-    plan = Plan((n, n), h, dt, nt, tx, rec, method="batched", ckpt_every=args.ckpt_every,
+    print(f"[bench] {args.workload}: building batched plan ({len(tx)} shots, {shape}, nt={nt})",
+          file=sys.stderr, flush=True)
+    plan = Plan(shape, h, dt, nt, tx, rec, method="batched", ckpt_every=ckpt,
                 autotune=not args.no_autotune)
+    print(f"[bench] plan {plan.info}", file=sys.stderr, flush=True)
     info = plan.info
     dobs = plan.forward_fmc(mt, wd)
     torch.cuda.synchronize()
@@ -160,8 +165,9 @@ def main():
           flush=True)
 
     prof = plan.profile_gradient(md, wd, dobs)
-    adj_bytes = BYTES_ADJ * N * len(tx)
-    fwd_bytes = BYTES_FWD * N * len(tx)
+    per_pass = len(tx) / max(info["shot_passes"], 1)
+    adj_bytes = BYTES_ADJ * N * per_pass
+    fwd_bytes = BYTES_FWD * N * per_pass
     adj_gbs = adj_bytes / (prof["adj_ms"] / 1e3) / 1e9
     fwd_gbs = fwd_bytes / (prof["fwd_ms"] / 1e3) / 1e9
     print(f"[bench] per launch: fwd {prof['fwd_ms']:.3f} ms ({fwd_gbs:.0f} GB/s alg), "
@@ -172,9 +178,9 @@ def main():
     torch.cuda.empty_cache()
 
     plain = None
-    if not args.no_plain:
+    if not args.no_plain and args.workload in ("C2", "C2-256"):
         ps = min(args.plain_shots, len(tx))
-        pplan = Plan((n, n), h, dt, nt, tx[:ps], rec, method="plain")
+        pplan = Plan(shape, h, dt, nt, tx[:ps], rec, method="plain")
         pd = pplan.forward_fmc(mt, wd)
         pplan.misfit_and_gradient(md, wd, pd)  # warm-up
         torch.cuda.synchronize()
@@ -206,18 +212,18 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded inclusion phantom, observed data simulated on the device)",
         "config": {
-            "workload": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 128-element ring "
-                        "R=22mm, 128 shots, Ricker 500kHz, FP32 fields; one step = one "
-                        "misfit+gradient over all shots",
-            "grid": [n, n], "nt": nt, "shots": len(tx), "receivers": len(rec),
-            "method": "batched", "plan": info,
+            "workload": WORKLOADS[args.workload] if not args.shots else
+            WORKLOADS[args.workload] + f" (first {len(tx)} shots)",
+            "grid": list(shape), "nt": nt, "shots": len(tx), "receivers": len(rec),
+            "method": "batched", "plan": info, "peak_hbm_bytes": torch.cuda.max_memory_allocated() + info["workspace_bytes"],
+            "recompute_launch_fraction": round(max(prof["n_fwd"] - (nt - 1) * info["shot_passes"], 0) / max(prof["n_fwd"] + prof["n_adj"], 1), 4),
         },
         "plain_value": None if plain is None else round(plain, 3),
         "batched_vs_plain": None if plain is None else round(value / plain, 3),
         "misfit": Jv,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_adj2d (adjoint step + imaging, all shots)",
+            "kernel": ("k_adj2d" if len(shape) == 2 else "k_adj3d") + " (adjoint step + imaging, all shots of a pass)",
             "achieved": round(adj_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -51,7 +51,8 @@ enum {
 enum {
     USCT_FLAG_SPONGE = 1,    /* calls pass a multiplicative sponge mask (2D only) */
     USCT_FLAG_AUTOTUNE = 2,  /* time candidate tile shapes at creation, keep the best */
-    USCT_FLAG_NO_GRAPH = 4   /* batched: launch eagerly instead of replaying a hipGraph */
+    USCT_FLAG_NO_GRAPH = 4,  /* batched: launch eagerly instead of replaying a hipGraph */
+    USCT_FLAG_FORWARD_ONLY = 8  /* no gradient workspace (forward_fmc / misfit only) */
 };
 
 /* Misfit codes (software/python/uap.py:41). */

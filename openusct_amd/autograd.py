@@ -1,0 +1,74 @@
+"""Sound speed -> misfit as a differentiable torch function.
+
+``misfit_from_velocity(c, plan, wavelet, dobs)`` returns the waveform misfit J
+of the model ``c`` (m/s) as a scalar tensor whose backward is the exact
+discrete-adjoint gradient computed by the HIP solver: the forward pass runs
+misfit_and_gradient once (forward sweep, checkpoints, re-computation and
+adjoint sweep all on the device) and keeps g = dJ/dm; backward applies the
+chain rule through m = 1/c^2 (simulation/ringfwi/phantom.py:13-14):
+
+    dJ/dc = g * dm/dc = -2 g / c^3.
+
+The gradient is that of the solver's discrete scheme, so it matches a finite
+difference of the same solver's misfit (not the continuous PDE).
+"""
+
+from __future__ import annotations
+
+import torch
+
+from .plan import Plan
+
+
+class SoundSpeedMisfit(torch.autograd.Function):
+    """J(c) with the adjoint-state gradient; non-tensor arguments get no gradient."""
+
+    @staticmethod
+    def forward(ctx, c, plan: Plan, wavelet, dobs, misfit_type="l2", sponge=None):
+        c64 = c.detach().to(torch.float64)
+        m = 1.0 / (c64 * c64)
+        J, g = plan.misfit_and_gradient(m, wavelet, dobs, sponge, misfit_type)
+        ctx.save_for_backward(c64, g)
+        ctx.in_dtype = c.dtype
+        return J[0].to(c.dtype) if c.dtype != torch.float64 else J[0]
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        c64, g = ctx.saved_tensors
+        dc = (-2.0 * g / (c64 * c64 * c64)) * grad_out.to(torch.float64)
+        return dc.to(ctx.in_dtype), None, None, None, None, None
+
+
+def misfit_from_velocity(c, plan, wavelet, dobs, misfit_type="l2", sponge=None):
+    """Differentiable misfit of the velocity model ``c`` (a HIP tensor)."""
+    return SoundSpeedMisfit.apply(c, plan, wavelet, dobs, misfit_type, sponge)
+
+
+def gradient_descent(c0, plan, wavelet, dobs, steps=5, step_frac=0.02, c_bounds=None,
+                     mask=None, misfit_type="l2"):
+    """Plain steepest descent on c through :class:`SoundSpeedMisfit`.
+
+    Each step: J = misfit(c); J.backward(); c -= step_frac * mean(c) * grad / max|grad|
+    (the step rule of simulation/ringfwi/fwi.py:642, applied to c), optionally
+    restricted to ``mask`` and clipped to ``c_bounds``.  Returns (c, history).
+    """
+    c = c0.detach().clone().to(torch.float64).requires_grad_(True)
+    history = []
+    for _ in range(steps):
+        if c.grad is not None:
+            c.grad = None
+        J = misfit_from_velocity(c, plan, wavelet, dobs, misfit_type)
+        J.backward()
+        history.append(float(J.item()))
+        with torch.no_grad():
+            d = c.grad if mask is None else c.grad * mask
+            dmax = d.abs().max()
+            if float(dmax) == 0.0:
+                break
+            c -= step_frac * c.mean() * d / dmax
+            if c_bounds is not None:
+                c.clamp_(c_bounds[0], c_bounds[1])
+    with torch.no_grad():
+        J = misfit_from_velocity(c, plan, wavelet, dobs, misfit_type)
+    history.append(float(J.item()))
+    return c.detach(), history

@@ -687,7 +687,7 @@ int choose_batched(usct_plan *P, int ckpt_every) {
         }
         for (int K : Ks) {
             const int NS = (int)std::max<int64_t>(1, cdiv(nt - 1, K));
-            const int64_t bytes = batched_bytes(P, Bp, K, NS, true);
+            const int64_t bytes = batched_bytes(P, Bp, K, NS, !(P->flags & USCT_FLAG_FORWARD_ONLY));
             if (bytes > P->budget) continue;
             // cost: bytes streamed per shot-level (fwd 12, adj 17, recompute 12 if NS > 1),
             // padded shots count, narrower lane groups stream a little less efficiently.
@@ -733,10 +733,11 @@ int allocate_batched(usct_plan *P) {
     for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->Lo[i], FE));
     TRY(dalloc_t(P, &P->Zlo, FE));
     HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * FE));
-    const int nW = P->K + 2;
+    const bool fwd_only = (P->flags & USCT_FLAG_FORWARD_ONLY) != 0;
+    const int nW = fwd_only ? 0 : P->K + 2;
     P->Wslot.resize(nW);
     for (int i = 0; i < nW; ++i) TRY(dalloc_t(P, &P->Wslot[i], FE));
-    const int nC = 2 * std::max(P->NS - 2, 0);
+    const int nC = fwd_only ? 0 : 2 * std::max(P->NS - 2, 0);
     P->CK.resize(nC);
     for (int i = 0; i < nC; ++i) TRY(dalloc_t(P, &P->CK[i], FE));
     P->CKlo.resize(nC);
@@ -817,15 +818,18 @@ double time_tile(usct_plan *P) {
     // warm-up
     launch_fwd_batched(P, P->stream, {P->F[0], P->Lo[0]}, {P->F[1], P->Lo[1]},
                        {P->F[2], P->Lo[2]}, 1, false);
-    launch_adj_batched(P, P->stream, P->F[0], P->F[1], P->F[2], P->Wslot[0], P->Wslot[0],
-                       P->Wslot[0], 1);
+    const bool adj = !P->Wslot.empty();
+    if (adj)
+        launch_adj_batched(P, P->stream, P->F[0], P->F[1], P->F[2], P->Wslot[0], P->Wslot[0],
+                           P->Wslot[0], 1);
     (void)hipEventRecord(a, P->stream);
     for (int r = 0; r < reps; ++r) {
         launch_fwd_batched(P, P->stream, {P->F[r % 3], P->Lo[r % 3]},
                            {P->F[(r + 1) % 3], P->Lo[(r + 1) % 3]},
                            {P->F[(r + 2) % 3], P->Lo[(r + 2) % 3]}, 1, false);
-        launch_adj_batched(P, P->stream, P->F[r % 3], P->F[(r + 1) % 3], P->F[(r + 2) % 3],
-                           P->Wslot[0], P->Wslot[0], P->Wslot[0], 1);
+        if (adj)
+            launch_adj_batched(P, P->stream, P->F[r % 3], P->F[(r + 1) % 3], P->F[(r + 2) % 3],
+                               P->Wslot[0], P->Wslot[0], P->Wslot[0], 1);
     }
     (void)hipEventRecord(b, P->stream);
     (void)hipEventSynchronize(b);
@@ -861,7 +865,7 @@ int autotune(usct_plan *P) {
         HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
         HIPX(hipMemset(P->Lo[i], 0, sizeof(uint16_t) * P->field_elems()));
     }
-    HIPX(hipMemset(P->Wslot[0], 0, sizeof(float) * P->field_elems()));
+    if (!P->Wslot.empty()) HIPX(hipMemset(P->Wslot[0], 0, sizeof(float) * P->field_elems()));
     HIPX(hipMemset(P->sinc, 0, sizeof(float) * P->nt * P->Bp));
     HIPX(hipMemset(P->sp2, 0, sizeof(float2) * P->N));
     HIPX(hipMemset(P->g_hi, 0, sizeof(float) * P->N * std::max(P->Bp / 16, 1)));
@@ -967,6 +971,8 @@ int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
         return fail(USCT_EINVAL, sponge ? "sponge passed to a plan created without USCT_FLAG_SPONGE"
                                         : "plan created with USCT_FLAG_SPONGE needs a sponge");
     if (mode != MODE_FWD && !dobs) return fail(USCT_EINVAL, "dobs is required");
+    if (mode == MODE_GRAD && (P->flags & USCT_FLAG_FORWARD_ONLY))
+        return fail(USCT_ENOTSUP, "plan was created with USCT_FLAG_FORWARD_ONLY");
     if (P->n_tx == 0) {
         TRY(enter(P, stream));
         if (J) HIPX(hipMemsetAsync(J, 0, sizeof(double), P->stream));

@@ -61,12 +61,19 @@ def add_sphere(c, centre_frac, radius_m, c_inclusion, h):
 
 
 def add_ellipsoid(c, centre_m, semi_axes_m, c_inclusion, h):
-    """Axis-aligned ellipsoid inclusion; centre (x, y, z) and semi-axes in metres."""
+    """Axis-aligned ellipsoid inclusion; centre (x, y, z) and semi-axes in metres.
+    Only the bounding box of the ellipsoid is visited."""
     nz, ny, nx = c.shape
-    z, y, x = np.mgrid[0:nz, 0:ny, 0:nx].astype(float) * h
-    q = sum(((v - c0) / a) ** 2 for v, c0, a in zip((x, y, z), centre_m, semi_axes_m))
     out = c.copy()
-    out[q <= 1.0] = c_inclusion
+    lo = [max(0, int(np.floor((c0 - a) / h)) - 1) for c0, a in zip(centre_m, semi_axes_m)]
+    hi = [min(n, int(np.ceil((c0 + a) / h)) + 2) for c0, a, n in zip(centre_m, semi_axes_m,
+                                                                       (nx, ny, nz))]
+    if any(l >= u for l, u in zip(lo, hi)):
+        return out
+    z, y, x = np.mgrid[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]].astype(float) * h
+    q = sum(((v - c0) / a) ** 2 for v, c0, a in zip((x, y, z), centre_m, semi_axes_m))
+    box = out[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]]
+    box[q <= 1.0] = c_inclusion
     return out
 
 

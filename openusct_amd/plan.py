@@ -45,7 +45,7 @@ class Plan:
     """A reusable solver plan on the current HIP device."""
 
     def __init__(self, shape, h, dt, nt, tx_lin, rec_lin, method="batched", ckpt_every=0,
-                 mem_budget=0, sponge=False, autotune=True, graph=True):
+                 mem_budget=0, sponge=False, autotune=True, graph=True, forward_only=False):
         lib = _lib.load()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -58,7 +58,7 @@ class Plan:
         if method not in _METHODS:
             raise ValueError(f"unknown method {method!r}; expected one of {sorted(_METHODS)}")
         flags = (_lib.FLAG_SPONGE if sponge else 0) | (_lib.FLAG_AUTOTUNE if autotune else 0) | \
-            (0 if graph else _lib.FLAG_NO_GRAPH)
+            (0 if graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_FORWARD_ONLY if forward_only else 0)
         dims = (ctypes.c_int * self.ndim)(*self.shape)
         handle = ctypes.c_void_p()
         torch.cuda.init()
@@ -172,12 +172,11 @@ class Plan:
         return {"fwd_ms": fwd.value, "adj_ms": adj.value, "n_fwd": nf.value, "n_adj": na.value}
 
 
-def footprint_bytes(shape, nt, n_tx, n_rec, method="batched", ckpt_every=0):
-    """Device bytes a plan would hold (same model as the C planner, whole history
-    when it fits, else K ~ sqrt(2 nt)); used by the serving loop's admission."""
+def footprint_bytes(shape, nt, n_tx, n_rec, ckpt_every=0, gradient=True):
+    """Device bytes of a batched plan in one pass, the same model as the C planner:
+    every field is N * Bp floats (bf16 low parts half that); checkpoint interval
+    ``ckpt_every`` or, by default, the memory minimiser K ~ sqrt(2 nt)."""
     N = int(np.prod(shape))
-    if method == "plain":
-        return 4 * N * (nt + 4) + 12 * nt * n_rec + 20 * N
     lb = 64
     for c in (64, 32, 16):
         lb = c
@@ -186,6 +185,12 @@ def footprint_bytes(shape, nt, n_tx, n_rec, method="batched", ckpt_every=0):
             break
     Bp = -(-n_tx // lb) * lb
     K = ckpt_every if ckpt_every > 0 else max(2, int(round((2.0 * max(nt - 1, 1)) ** 0.5)))
+    K = min(K, max(nt - 1, 2))
     NS = max(1, -(-(nt - 1) // K))
-    fields = 4 + (K + 2) + 2 * max(NS - 2, 0)
-    return 4 * N * Bp * fields + 12 * nt * n_rec * Bp + 20 * N + 8 * N * (Bp // 16)
+    field = 4 * N * Bp
+    fields = 4 + ((K + 2) + 2 * max(NS - 2, 0) if gradient else 0)
+    halves = 4 + (2 * max(NS - 2, 0) if gradient else 0)
+    traces = 4 * nt * n_rec * Bp
+    b = fields * field + halves * (field // 2) + 4 * traces
+    b += 8 * N + 4 * N + 8 * N * max(Bp // 16, 1) + 8 * nt * Bp + 24 * n_rec * Bp + 8 * N
+    return int(b + (1 << 20))

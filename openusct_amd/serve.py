@@ -1,0 +1,287 @@
+"""Continuous-batching request loop for FWI-gradient and forward-acquisition requests.
+
+Requests of any grid size (2D or 3D) and shot count arrive over time.  The
+loop admits a waiting request as soon as its device footprint fits the memory
+budget and an execution slot is free (first fit in arrival order, with a
+starvation guard that stops overtaking once the oldest request has waited
+``max_wait_s``), so new work starts while earlier work is still running.
+Every admitted request gets its own plan (device-resident model, wavefields,
+checkpoints and traces) and its own HIP stream, so the device overlaps the
+kernels of all requests in flight.  Each request is batched across its own
+shots.
+
+Results do not depend on what else is in flight: a request's arithmetic is
+fixed by its own inputs (tile shapes are chosen by rule, not by timing), and
+its reductions run in a fixed order inside its own buffers.
+
+Plans of finished requests are released lazily (``hipFree`` waits for the
+whole device): only when a waiting request needs their memory, or when the
+device is idle.
+"""
+
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .plan import Plan, footprint_bytes
+
+
+@dataclass
+class Request:
+    """One solve.  ``kind`` is "gradient" (needs ``dobs``) or "forward"."""
+    rid: int
+    kind: str
+    shape: tuple
+    h: float
+    dt: float
+    nt: int
+    tx: np.ndarray            # linear source cells, one per shot
+    rec: np.ndarray           # linear receiver cells
+    m: np.ndarray             # squared slowness (float64)
+    wavelet: np.ndarray
+    dobs: np.ndarray | None = None
+    arrival: float = 0.0      # seconds after the loop starts
+
+    @property
+    def updates(self):
+        n = int(np.prod(self.shape)) * (self.nt - 1) * len(self.tx)
+        return 2 * n if self.kind == "gradient" else n
+
+    def footprint(self, cap=None):
+        """Device bytes to reserve: the one-pass plan, or ``cap`` (the plan then runs
+        its shots in several passes within it)."""
+        b = footprint_bytes(self.shape, self.nt, len(self.tx), len(self.rec),
+                            gradient=self.kind == "gradient")
+        return b if cap is None else min(b, cap)
+
+
+@dataclass
+class Result:
+    rid: int
+    kind: str
+    J: float | None
+    grad: torch.Tensor | None
+    data: torch.Tensor | None
+    arrival: float
+    admitted: float
+    done: float
+    updates: int
+    workspace_bytes: int = 0
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+@dataclass
+class _Flight:
+    req: Request
+    plan: Plan
+    stream: torch.cuda.Stream
+    event: torch.cuda.Event
+    admitted: float
+    J: torch.Tensor | None = None
+    grad: torch.Tensor | None = None
+    data: torch.Tensor | None = None
+    tensors: list = field(default_factory=list)
+
+
+class Server:
+    """Single-device continuous-batching loop (one process, one GPU)."""
+
+    def __init__(self, mem_budget=None, max_inflight=8, max_wait_s=5.0, use_graph=False):
+        torch.cuda.init()
+        free, total = torch.cuda.mem_get_info()
+        self.budget = int(mem_budget if mem_budget is not None else 0.8 * free)
+        self.max_inflight = int(max_inflight)
+        self.max_wait_s = float(max_wait_s)
+        self.use_graph = use_graph
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        self.cap = self.budget // 2  # largest reservation of one request
+        self._held = 0            # bytes of plans in flight or awaiting release
+        self._retired = []        # finished plans not yet freed
+
+    # -- memory -----------------------------------------------------------------
+    def _release(self):
+        if not self._retired:
+            return
+        for p, nbytes in self._retired:
+            p.close()
+            self._held -= nbytes
+        self._retired.clear()
+
+    def _fits(self, req, in_flight):
+        need = req.footprint(self.cap) + (64 << 20)
+        if self._held + need <= self.budget:
+            return True
+        retired = sum(b for _, b in self._retired)
+        if retired and self._held - retired + need <= self.budget:
+            self._release()
+            return True
+        if need > self.budget:
+            raise MemoryError(f"request {req.rid} needs {need} bytes, budget {self.budget}")
+        return False
+
+    # -- execution ----------------------------------------------------------------
+    def _launch(self, req, now):
+        stream = torch.cuda.Stream(device=self.device)
+        est = req.footprint(self.cap)
+        # the plan's own planner must stay within what admission accounted for
+        plan = Plan(req.shape, req.h, req.dt, req.nt, req.tx, req.rec, method="batched",
+                    mem_budget=int(est * 1.05) + (64 << 20), autotune=False,
+                    graph=self.use_graph, forward_only=req.kind == "forward")
+        nbytes = max(int(plan.info["workspace_bytes"]), est)
+        self._held += nbytes
+        fl = _Flight(req, plan, stream, torch.cuda.Event(), now)
+        fl.tensors.append(nbytes)
+        with torch.cuda.stream(stream):
+            m = torch.as_tensor(req.m, dtype=torch.float64).to(self.device, non_blocking=False)
+            w = torch.as_tensor(req.wavelet, dtype=torch.float64).to(self.device)
+            if req.kind == "gradient":
+                d = torch.as_tensor(req.dobs, dtype=torch.float32).to(self.device)
+                fl.J, fl.grad = plan.misfit_and_gradient(m, w, d, stream=stream)
+                fl.tensors += [m, w, d]
+            elif req.kind == "forward":
+                fl.data = plan.forward_fmc(m, w, stream=stream)
+                fl.tensors += [m, w]
+            else:
+                raise ValueError(f"unknown request kind {req.kind!r}")
+            fl.event.record(stream)
+        return fl
+
+    def run(self, requests, on_result=None, progress=None):
+        """Serve ``requests`` (arrival times relative to now); returns results by rid."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queue, flights, results = [], [], {}
+        t0 = time.perf_counter()
+        n_total = len(pending)
+        while pending or queue or flights:
+            now = time.perf_counter() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            # admission: first fit in arrival order; no overtaking a request that has
+            # waited longer than max_wait_s
+            i = 0
+            while i < len(queue) and len(flights) < self.max_inflight:
+                r = queue[i]
+                if self._fits(r, flights):
+                    queue.pop(i)
+                    flights.append(self._launch(r, time.perf_counter() - t0))
+                    continue
+                if now - queue[0].arrival > self.max_wait_s:
+                    break
+                i += 1
+            # completion
+            still = []
+            for fl in flights:
+                if fl.event.query():
+                    done = time.perf_counter() - t0
+                    r = fl.req
+                    res = Result(r.rid, r.kind, None if fl.J is None else float(fl.J.item()),
+                                 fl.grad, fl.data, r.arrival, fl.admitted, done, r.updates,
+                                 fl.tensors[0])
+                    results[r.rid] = res
+                    self._retired.append((fl.plan, fl.tensors[0]))
+                    if on_result:
+                        on_result(res)
+                    if progress:
+                        progress(len(results), n_total)
+                else:
+                    still.append(fl)
+            flights = still
+            if not flights and not queue:
+                self._release()
+                if pending:
+                    time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
+            elif not any(f.event.query() for f in flights):
+                time.sleep(2e-4)
+        self._release()
+        return results
+
+
+def summarize(results):
+    """p50 / p99 latency and aggregate Gpt-updates/s of a finished run."""
+    res = list(results.values())
+    lat = np.array([r.latency for r in res])
+    t_first = min(r.arrival for r in res)
+    t_last = max(r.done for r in res)
+    upd = sum(r.updates for r in res)
+    return {"requests": len(res), "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
+            "makespan_s": t_last - t_first, "gpt_updates_per_s": upd / (t_last - t_first) / 1e9,
+            "updates": upd}
+
+
+def make_mix(n_requests=500, seed=5, nt_scale=1.0, shots=(16, 256), grids=None, rate=None,
+             with_3d=True):
+    """The serving mix: Poisson arrivals from default_rng(seed), gradient or forward with
+    equal odds, grids from the C1/C3/C4 families, shots ~ U{shots}, phantoms seeded by
+    request id.  Observed data of gradient requests is a forward model of the seeded
+    phantom computed later by the caller (``fill_observed``)."""
+    from . import phantom
+    from .geometry import CylinderArray, RingArray
+    from .plan import linear_indices
+    from .sources import ricker
+
+    rng = np.random.default_rng(seed)
+    if grids is None:
+        grids = [("2d", 256), ("2d", 512), ("2d", 1024)]
+        if with_3d:
+            grids += [("3d", 128), ("3d", 192), ("3d", 256)]
+    reqs = []
+    t = 0.0
+    for rid in range(n_requests):
+        kind = "gradient" if rng.random() < 0.5 else "forward"
+        dim, n = grids[int(rng.integers(len(grids)))]
+        ns = int(rng.integers(shots[0], shots[1] + 1))
+        if dim == "2d":
+            h, dt, nt = (2e-4, 5e-8, 1000) if n <= 512 else (1e-4, 2.5e-8, 4000)
+            domain = (n - 1) * h
+            geom = RingArray(max(ns, 16), 0.43 * domain, domain, h)
+            shape = (n, n)
+            c = phantom.seeded_discs(n, h, 1000 + rid, 3, inside_mm=0.31 * domain * 1e3)
+        else:
+            h, dt, nt = 2.5e-4, 3e-8, 2000
+            domain = (n - 1) * h
+            per = max(8, (ns + 7) // 8)
+            geom = CylinderArray(8, per, 0.44 * domain, 0.44 * domain, domain, h)
+            shape = (n, n, n)
+            f = domain / 0.06375
+            c = phantom.seeded_ellipsoids(n, h, 1000 + rid, 8, a_mm=(3 * f, 8 * f),
+                                          inside_mm=20 * f)
+        nt = max(8, int(round(nt * nt_scale)))
+        rec = linear_indices(geom.idx, shape)
+        tx = rec[np.linspace(0, len(rec) - 1, ns).astype(int)]
+        m_true = phantom.velocity_to_m(c)
+        m0 = np.full(shape, 1.0 / 1500.0 ** 2)
+        req = Request(rid, kind, shape, h, dt, nt, tx, rec, m0 if kind == "gradient" else m_true,
+                      ricker(nt, dt, 5e5), None, 0.0)
+        req.m_true = m_true
+        reqs.append(req)
+    # arrival rate: about 80 % utilisation at `rate` Gpt-updates/s, if given
+    if rate:
+        mean_gap = np.mean([r.updates for r in reqs]) / (rate * 1e9) / 0.8
+    else:
+        mean_gap = 0.0
+    for r in reqs:
+        t += rng.exponential(mean_gap) if mean_gap > 0 else 0.0
+        r.arrival = t
+    return reqs
+
+
+def fill_observed(reqs):
+    """Synthesise observed data (device forward of each request's true model)."""
+    for r in reqs:
+        if r.kind != "gradient":
+            continue
+        with Plan(r.shape, r.h, r.dt, r.nt, r.tx, r.rec, autotune=False, graph=False,
+                  forward_only=True) as p:
+            d = p.forward_fmc(torch.as_tensor(r.m_true, device="cuda"),
+                              torch.as_tensor(r.wavelet, device="cuda"))
+            r.dobs = d.cpu().numpy()
+    return reqs

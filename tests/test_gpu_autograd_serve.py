@@ -1,0 +1,138 @@
+"""GPU: the sound-speed autograd function and the continuous-batching server."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _c1_small(shots=4, nt_scale=0.4):
+    from openusct_amd import configs
+    return configs.c1(nt_scale=nt_scale, shots=shots)
+
+
+def _plan_for(prob, **kw):
+    from openusct_amd.plan import Plan, linear_indices
+    rec = linear_indices(prob.geom.idx, prob.shape)
+    return Plan(prob.shape, prob.h, prob.dt, prob.nt, rec[prob.src_list], rec, **kw)
+
+
+def test_autograd_matches_chain_rule_and_oracle():
+    from oracle import oracle as O
+    from openusct_amd.autograd import misfit_from_velocity
+    prob = _c1_small()
+    dev = torch.device("cuda")
+    dobs = O.forward_fmc(prob.m_true, prob.geom, prob.wavelet, prob.dt, prob.h, prob.nt, None,
+                         prob.src_list)
+    c0 = np.sqrt(1.0 / prob.m0)
+    J_ref, g_ref = O.misfit_and_gradient(prob.m0, prob.geom, prob.wavelet, prob.dt, prob.h,
+                                         prob.nt, dobs, None, prob.src_list)
+    dc_ref = -2.0 * g_ref / c0 ** 3
+    with _plan_for(prob) as plan:
+        w = torch.tensor(prob.wavelet, device=dev)
+        d = torch.tensor(dobs, dtype=torch.float32, device=dev)
+        c = torch.tensor(c0, device=dev, requires_grad=True)
+        J = misfit_from_velocity(c, plan, w, d)
+        J.backward()
+        Jp, gp = plan.misfit_and_gradient(torch.tensor(prob.m0, device=dev), w, d)
+        assert J.item() == Jp.item()
+        assert torch.equal(c.grad, -2.0 * gp / c.detach() ** 3)
+    e_J = abs(J.item() - J_ref) / J_ref
+    e_g = rel_l2(c.grad.cpu().numpy(), dc_ref)
+    print(f"[case] autograd C1 4 shots: J {e_J:.2e} dJ/dc {e_g:.2e}")
+    assert e_J < 1e-5 and e_g < 1e-5
+
+
+def test_autograd_directional_finite_difference():
+    """As simulation/tests/test_gradient_misfits.py:38-59: a random direction with a
+    zero 5-cell border; the FP64 oracle's misfit gives the central difference."""
+    from oracle import oracle as O
+    from openusct_amd.autograd import misfit_from_velocity
+    prob = _c1_small(shots=2, nt_scale=0.3)
+    dev = torch.device("cuda")
+    dobs = O.forward_fmc(prob.m_true, prob.geom, prob.wavelet, prob.dt, prob.h, prob.nt, None,
+                         prob.src_list)
+    c0 = np.sqrt(1.0 / prob.m0)
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal(prob.shape)
+    v[:5, :] = v[-5:, :] = v[:, :5] = v[:, -5:] = 0.0
+    v *= 1.0 / np.abs(v).max()
+    with _plan_for(prob) as plan:
+        c = torch.tensor(c0, device=dev, requires_grad=True)
+        J = misfit_from_velocity(c, plan, torch.tensor(prob.wavelet, device=dev),
+                                 torch.tensor(dobs, dtype=torch.float32, device=dev))
+        J.backward()
+        ad = float((c.grad.cpu().numpy() * v).sum())
+    eps = 0.05  # m/s
+    Jp = O.misfit(1.0 / (c0 + eps * v) ** 2, prob.geom, prob.wavelet, prob.dt, prob.h, prob.nt,
+                  dobs, None, prob.src_list)
+    Jm = O.misfit(1.0 / (c0 - eps * v) ** 2, prob.geom, prob.wavelet, prob.dt, prob.h, prob.nt,
+                  dobs, None, prob.src_list)
+    fd = (Jp - Jm) / (2 * eps)
+    print(f"[case] directional FD: fd {fd:.6e} ad {ad:.6e} rel {abs(fd - ad) / abs(fd):.2e}")
+    assert abs(fd - ad) <= 1e-3 * abs(fd)
+
+
+def test_gradient_descent_reduces_misfit():
+    from oracle import oracle as O
+    from openusct_amd.autograd import gradient_descent
+    prob = _c1_small(shots=4, nt_scale=0.5)
+    dev = torch.device("cuda")
+    dobs = O.forward_fmc(prob.m_true, prob.geom, prob.wavelet, prob.dt, prob.h, prob.nt, None,
+                         prob.src_list)
+    with _plan_for(prob) as plan:
+        c, hist = gradient_descent(torch.tensor(np.sqrt(1 / prob.m0), device=dev), plan,
+                                   torch.tensor(prob.wavelet, device=dev),
+                                   torch.tensor(dobs, dtype=torch.float32, device=dev),
+                                   steps=5, step_frac=0.005, c_bounds=(1400.0, 1800.0))
+    print("[case] GD misfit history", hist)
+    assert hist[-1] < hist[0]
+
+
+def _small_mix(n=10, seed=11):
+    from openusct_amd.serve import fill_observed, make_mix
+    reqs = make_mix(n, seed=seed, nt_scale=0.05, shots=(16, 80),
+                    grids=[("2d", 256), ("2d", 512), ("3d", 64)])
+    for i, r in enumerate(reqs):
+        r.arrival = 0.01 * i
+    return fill_observed(reqs)
+
+
+def test_server_results_same_alone_and_in_mix():
+    from openusct_amd.serve import Server, summarize
+    reqs = _small_mix()
+    mixed = Server(max_inflight=6).run(reqs)
+    assert sorted(mixed) == [r.rid for r in reqs]
+    print("[case] mix", summarize(mixed))
+    for r in reqs[:6]:
+        r0 = r.arrival
+        r.arrival = 0.0
+        alone = Server(max_inflight=1).run([r])[r.rid]
+        r.arrival = r0
+        a, b = alone, mixed[r.rid]
+        if r.kind == "gradient":
+            assert a.J == b.J
+            assert torch.equal(a.grad, b.grad)
+        else:
+            assert torch.equal(a.data, b.data)
+
+
+def test_server_admission_under_tight_budget():
+    """A budget that holds about one request at a time still completes every
+    request, with the same results."""
+    from openusct_amd.serve import Server
+    reqs = _small_mix(n=6, seed=12)
+    big = max(r.footprint() for r in reqs)
+    res_tight = Server(mem_budget=2 * big + (256 << 20), max_inflight=8).run(reqs)
+    res_free = Server(max_inflight=8).run(reqs)
+    for r in reqs:
+        a, b = res_tight[r.rid], res_free[r.rid]
+        if r.kind == "gradient":
+            assert a.J == b.J and torch.equal(a.grad, b.grad)
+        else:
+            assert torch.equal(a.data, b.data)
