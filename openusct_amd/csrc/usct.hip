@@ -147,6 +147,23 @@ int dalloc_t(usct_plan *P, T **p, int64_t count) {
     return dalloc(P, (void **)p, count * (int64_t)sizeof(T));
 }
 
+// Stencil operands of the batched kernels are read with unclamped columns: the
+// x halo of a strip may fall up to kGuardCells cells (times the shot stride)
+// before the first or after the last element.  Such buffers carry a zeroed
+// guard zone on both sides; the returned pointer is the first element.
+constexpr int64_t kGuardCells = 64;
+
+template <class T>
+int dalloc_guarded(usct_plan *P, T **p, int64_t count, int64_t stride) {
+    const int64_t g = kGuardCells * std::max<int64_t>(stride, 1);
+    T *base = nullptr;
+    TRY(dalloc_t(P, &base, count + 2 * g));
+    HIPX(hipMemset(base, 0, sizeof(T) * g));
+    HIPX(hipMemset(base + g + count, 0, sizeof(T) * g));
+    *p = base + g;
+    return 0;
+}
+
 void free_plan_memory(usct_plan *P) {
     for (auto &kv : P->graphs) (void)hipGraphExecDestroy(kv.second);
     P->graphs.clear();
@@ -225,8 +242,8 @@ void fwd3d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
     float *d2 = P->srcd2 + (sinc - P->sinc);
     hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pn.hi,
                        pc.lo, pp.lo, pn.lo, P->sp2, P->src_cell, sinc, P->rec_head, P->rec_next,
-                       rec, rl, d2, P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc,
-                       P->ng());
+                       P->row_rec, rec, rl, d2, P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys,
+                       t.nyc, P->ng());
 }
 
 template <int LB, bool REC>
@@ -318,8 +335,8 @@ void adj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
              const Tile &t) {
     const float *d2 = P->srcd2 + (sinc - P->sinc);
     hipLaunchKernelGGL((k_adj3d<LB, CX, CY>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, P->sp2,
-                       P->src_cell, sinc, P->rec_head, P->rec_next, res, d2, P->g_hi, P->g_lo,
-                       P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
+                       P->src_cell, sinc, P->rec_head, P->rec_next, P->row_rec, res, d2, P->g_hi,
+                       P->g_lo, P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
 }
 
 template <int LB>
@@ -655,6 +672,7 @@ int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
         halves += 2 * std::max(NS - 2, 0);
     }
     int64_t b = fields * field + halves * (field / 2) + 4 * traces;
+    b += fields * 2 * kGuardCells * Bp * 4;
     b += P->N * 8 /*sp2*/ + P->N * 4 /*sponge*/ + 2 * P->N * 4 * (Bp / 16) /*g*/;
     b += (int64_t)P->nt * Bp * 4 + (int64_t)P->n_rec * Bp * 24 + (int64_t)P->N * 4 + (1 << 20);
     return b;
@@ -727,19 +745,19 @@ void default_rows(usct_plan *P) {
 
 int allocate_batched(usct_plan *P) {
     const int64_t FE = P->field_elems();
-    TRY(dalloc_t(P, &P->Z, FE));
+    TRY(dalloc_guarded(P, &P->Z, FE, P->Bp));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * FE));
-    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->F[i], FE));
+    for (int i = 0; i < 3; ++i) TRY(dalloc_guarded(P, &P->F[i], FE, P->Bp));
     for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->Lo[i], FE));
     TRY(dalloc_t(P, &P->Zlo, FE));
     HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * FE));
     const bool fwd_only = (P->flags & USCT_FLAG_FORWARD_ONLY) != 0;
     const int nW = fwd_only ? 0 : P->K + 2;
     P->Wslot.resize(nW);
-    for (int i = 0; i < nW; ++i) TRY(dalloc_t(P, &P->Wslot[i], FE));
+    for (int i = 0; i < nW; ++i) TRY(dalloc_guarded(P, &P->Wslot[i], FE, P->Bp));
     const int nC = fwd_only ? 0 : 2 * std::max(P->NS - 2, 0);
     P->CK.resize(nC);
-    for (int i = 0; i < nC; ++i) TRY(dalloc_t(P, &P->CK[i], FE));
+    for (int i = 0; i < nC; ++i) TRY(dalloc_guarded(P, &P->CK[i], FE, P->Bp));
     P->CKlo.resize(nC);
     for (int i = 0; i < nC; ++i) TRY(dalloc_t(P, &P->CKlo[i], FE));
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
@@ -930,6 +948,11 @@ int validate(int ndim, const int *dims, double h, double dt, int nt, const int64
         n *= dims[d];
     }
     if (n >= (int64_t)1 << 31) return fail(USCT_EINVAL, "grid of %lld cells exceeds 2^31", (long long)n);
+    // kernels address a row (2D) or plane (3D) of all shots with 32-bit byte offsets
+    const int64_t plane = ndim == 2 ? dims[1] : (int64_t)dims[1] * dims[2];
+    if (plane * 4 * (int64_t)((n_tx + 63) / 64 * 64) >= ((int64_t)1 << 32))
+        return fail(USCT_EINVAL, "a grid %s of %lld cells times %d shots exceeds 32-bit offsets",
+                    ndim == 2 ? "row" : "plane", (long long)plane, n_tx);
     if (!(h > 0) || !(dt > 0)) return fail(USCT_EINVAL, "h and dt must be positive");
     if (nt < 1) return fail(USCT_EINVAL, "nt must be >= 1 (got %d)", nt);
     if (n_tx < 0 || n_rec < 0) return fail(USCT_EINVAL, "negative shot or receiver count");
@@ -1037,7 +1060,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
         return bail(fail(USCT_EHIP, "hipMemGetInfo failed"));
     P->budget = mem_budget > 0 ? mem_budget : (int64_t)(0.85 * (double)free_b);
     if ((rc = build_receivers(P))) return bail(rc);
-    if ((rc = dalloc_t(P, &P->sp2, N))) return bail(rc);
+    if ((rc = dalloc_guarded(P, &P->sp2, N, 1))) return bail(rc);
     if (P->sponge && (rc = dalloc_t(P, &P->spg, N))) return bail(rc);
     if ((rc = dalloc_t(P, &P->J_acc, 1))) return bail(rc);
     if ((rc = dalloc_t(P, &P->part, P->npart))) return bail(rc);

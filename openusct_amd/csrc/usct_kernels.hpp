@@ -223,10 +223,12 @@ __device__ __forceinline__ void stb(T *base, unsigned boff, T v) {
 // of level n; row_rec[y] = receivers on grid row y.  Grid: ng * nxs * nyc
 // wavefronts, 4 per block.
 //
-// Addressing: a field row starts at a wave-uniform pointer (scalar registers)
-// and each lane keeps 32-bit element offsets of its CX + 2 columns (x clamped
-// into the grid), so every load is "uniform base + 32-bit lane offset" with no
-// per-access 64-bit arithmetic.
+// Addressing: each lane keeps one 32-bit byte offset, that of its strip's first
+// cell and shot; row starts (clamped to the grid, wave-uniform) and the column
+// steps +-i*Bp go into the scalar base address.  Columns are not clamped: the
+// x halo of the first and last column reads the neighbouring row (or the
+// guard zone every field carries before and after it), values that only feed
+// border cells, whose Laplacian is zero.
 // ---------------------------------------------------------------------------
 template <int LB, int CX, bool SPONGE, bool REC>
 __global__ void __launch_bounds__(kBlock)
@@ -251,36 +253,28 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     if (x0 >= nx) return;
     const int y0 = yc * RY;
     const int y1 = min(y0 + RY, ny);
-    const size_t rowE = (size_t)nx * Bp;
     (void)src_cell;
     (void)sinc;
     (void)srcd2;
-    unsigned xc[CX + 2];  // clamped column of each stencil position
-    unsigned eo[CX + 2];  // its element offset within a field row
-#pragma unroll
-    for (int i = 0; i < CX + 2; ++i) {
-        xc[i] = (unsigned)min(max(x0 - 1 + i, 0), nx - 1);
-        eo[i] = xc[i] * (unsigned)Bp + (unsigned)b;
-    }
+    const size_t rowE = (size_t)nx * Bp;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;  // lane byte offset, float fields
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;  // bf16 fields
+    const unsigned oc = (unsigned)x0 * 8u;             // S' (float2 per cell)
     float rm[CX], rc[CX + 2], rp[CX];
     {
         const float *r0 = pc + (size_t)max(y0 - 1, 0) * rowE;
         const float *r1 = pc + (size_t)y0 * rowE;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) rm[i] = ldb(r0, 4u * eo[i + 1]);
+        for (int i = 0; i < CX; ++i) rm[i] = ldb(r0 + (ptrdiff_t)i * Bp, o4);
 #pragma unroll
-        for (int i = 0; i < CX + 2; ++i) rc[i] = ldb(r1, 4u * eo[i]);
+        for (int i = 0; i < CX + 2; ++i) rc[i] = ldb(r1 + (ptrdiff_t)(i - 1) * Bp, o4);
     }
     for (int y = y0; y < y1; ++y) {
         const float *rpn = pc + (size_t)min(y + 1, ny - 1) * rowE;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) rp[i] = ldb(rpn, 4u * eo[i + 1]);
+        for (int i = 0; i < CX; ++i) rp[i] = ldb(rpn + (ptrdiff_t)i * Bp, o4);
         const bool yint = (y >= 1) && (y <= ny - 2);
-        const float *ppr = pp + (size_t)y * rowE;
-        const uint16_t *pclr = pcl + (size_t)y * rowE;
-        const uint16_t *pplr = ppl + (size_t)y * rowE;
-        float *pnr = pn + (size_t)y * rowE;
-        uint16_t *pnlr = pnl + (size_t)y * rowE;
+        const size_t ro = (size_t)y * rowE;
         const float2 *spr = sp2 + (size_t)y * nx;
         const bool has_rec = REC && row_rec[y] > 0;
 #pragma unroll
@@ -289,14 +283,15 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             if (x < nx) {
                 float lu = 0.0f;
                 if (yint && x >= 1 && x <= nx - 2) lu = lap5(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
-                const float inc = smul(ldb(spr, 8u * xc[i + 1]), lu);
-                const unsigned e = eo[i + 1];
+                const float inc = smul(ldb(spr + i, oc), lu);
+                const size_t ei = ro + (size_t)i * Bp;
                 float lo;
-                float v = leap_hilo(rc[i + 1], bf2f(ldb(pclr, 2u * e)), ldb(ppr, 4u * e), bf2f(ldb(pplr, 2u * e)), inc, lo);
+                float v = leap_hilo(rc[i + 1], bf2f(ldb(pcl + ei, o2)), ldb(pp + ei, o4),
+                                    bf2f(ldb(ppl + ei, o2)), inc, lo);
                 if (SPONGE) v = scale_hilo(v, lo, sponge[(size_t)y * nx + x], lo);
                 const uint16_t lb = f2bf(lo);
-                stb(pnr, 4u * e, v);
-                stb(pnlr, 2u * e, lb);
+                stb(pn + ei, o4, v);
+                stb(pnl + ei, o2, lb);
                 if (has_rec) {
                     for (int j = rec_head[y * nx + x]; j >= 0; j = rec_next[j]) {
                         rec[(size_t)j * Bp + b] = v;
@@ -308,8 +303,8 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         if (y + 1 < y1) {
 #pragma unroll
             for (int i = 0; i < CX; ++i) { rm[i] = rc[i + 1]; rc[i + 1] = rp[i]; }
-            rc[0] = ldb(rpn, 4u * eo[0]);
-            rc[CX + 1] = ldb(rpn, 4u * eo[CX + 1]);
+            rc[0] = ldb(rpn - Bp, o4);
+            rc[CX + 1] = ldb(rpn + (ptrdiff_t)CX * Bp, o4);
         }
     }
 }
@@ -321,6 +316,8 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 //   res = [n_rec][Bp] residual row of level k; srcd2[b] = recorded increment
 //   of level k at shot b's source cell
 //   g_hi/g_lo: [ng][N] Neumaier accumulators of sum_shots sum_k lam^k d2p^k
+// Addressing as k_fwd2d (lanes past the last column still load, from the
+// guard zone, and join the reductions with zero contributions).
 // ---------------------------------------------------------------------------
 template <int LB, int CX, bool SPONGE>
 __global__ void __launch_bounds__(kBlock)
@@ -341,7 +338,7 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     if (yc >= nyc) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
-    const int x0 = (xs * W + st) * CX;  // may be >= nx: the lane still joins the reductions
+    const int x0 = (xs * W + st) * CX;
     const int y0 = yc * RY;
     const int y1 = min(y0 + RY, ny);
     const size_t rowE = (size_t)nx * Bp;
@@ -349,13 +346,8 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     (void)sinc;
     const int my_src = src_cell[b];
     const float my_d2 = srcd2[b];
-    unsigned xc[CX + 2];  // clamped column of each stencil position
-    unsigned eo[CX + 2];  // its element offset within a field row
-#pragma unroll
-    for (int i = 0; i < CX + 2; ++i) {
-        xc[i] = (unsigned)min(max(x0 - 1 + i, 0), nx - 1);
-        eo[i] = xc[i] * (unsigned)Bp + (unsigned)b;
-    }
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned oc = (unsigned)x0 * 8u;
     // q = S' lam^{k+1} rows (qm, qc with halo, qp); lc: raw centre row of lam^{k+1}.
     float qm[CX], qc[CX + 2], qp[CX], lc[CX], lpn[CX];
     float fm[CX], fc[CX + 2], fp[CX];  // p^{k-1} rows (no sponge)
@@ -366,20 +358,20 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         const float2 *s0 = sp2 + (size_t)ym * nx;
         const float2 *s1 = sp2 + (size_t)y0 * nx;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) qm[i] = smul(ldb(s0, 8u * xc[i + 1]), ldb(a0, 4u * eo[i + 1]));
+        for (int i = 0; i < CX; ++i) qm[i] = smul(ldb(s0 + i, oc), ldb(a0 + (ptrdiff_t)i * Bp, o4));
 #pragma unroll
         for (int i = 0; i < CX + 2; ++i) {
-            const float v = ldb(a1, 4u * eo[i]);
-            qc[i] = smul(ldb(s1, 8u * xc[i]), v);
+            const float v = ldb(a1 + (ptrdiff_t)(i - 1) * Bp, o4);
+            qc[i] = smul(ldb(s1 + (i - 1), oc), v);
             if (i >= 1 && i <= CX) lc[i - 1] = v;
         }
         if (!SPONGE) {
             const float *f0 = pa + (size_t)ym * rowE;
             const float *f1 = pa + (size_t)y0 * rowE;
 #pragma unroll
-            for (int i = 0; i < CX; ++i) fm[i] = ldb(f0, 4u * eo[i + 1]);
+            for (int i = 0; i < CX; ++i) fm[i] = ldb(f0 + (ptrdiff_t)i * Bp, o4);
 #pragma unroll
-            for (int i = 0; i < CX + 2; ++i) fc[i] = ldb(f1, 4u * eo[i]);
+            for (int i = 0; i < CX + 2; ++i) fc[i] = ldb(f1 + (ptrdiff_t)(i - 1) * Bp, o4);
         }
     }
     for (int y = y0; y < y1; ++y) {
@@ -389,14 +381,13 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         const float *fnr = pa + (size_t)yp * rowE;
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
-            const float v = ldb(lnr, 4u * eo[i + 1]);
+            const float v = ldb(lnr + (ptrdiff_t)i * Bp, o4);
             lpn[i] = v;
-            qp[i] = smul(ldb(snr, 8u * xc[i + 1]), v);
-            if (!SPONGE) fp[i] = ldb(fnr, 4u * eo[i + 1]);
+            qp[i] = smul(ldb(snr + i, oc), v);
+            if (!SPONGE) fp[i] = ldb(fnr + (ptrdiff_t)i * Bp, o4);
         }
         const bool yint = (y >= 1) && (y <= ny - 2);
-        const float *l2r = l2 + (size_t)y * rowE;
-        float *lkr = lk + (size_t)y * rowE;
+        const size_t ro = (size_t)y * rowE;
         const float2 *spr = sp2 + (size_t)y * nx;
         const bool has_rec = row_rec[y] > 0;
         float tv[CX];
@@ -407,21 +398,20 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
             if (x < nx) {
                 const bool in = yint && x >= 1 && x <= nx - 2;
                 const float lq = in ? lap5(qc[i + 1], qp[i], qm[i], qc[i + 2], qc[i]) : 0.0f;
-                const unsigned e = eo[i + 1];
-                float v = leap(lc[i], ldb(l2r, 4u * e), lq);
+                const size_t ei = ro + (size_t)i * Bp;
+                float v = leap(lc[i], ldb(l2 + ei, o4), lq);
                 if (SPONGE) v = v * sponge[(size_t)y * nx + x];
                 if (has_rec) {
                     for (int j = rec_head[y * nx + x]; j >= 0; j = rec_next[j])
                         v = v - res[(size_t)j * Bp + b];
                 }
-                stb(lkr, 4u * e, v);
+                stb(lk + ei, o4, v);
                 float d2;
                 if (SPONGE) {
-                    const size_t a = (size_t)y * rowE + e;
-                    d2 = (pa[a] - 2.0f * pb[a]) + pc2[a];
+                    d2 = (ldb(pa + ei, o4) - 2.0f * ldb(pb + ei, o4)) + ldb(pc2 + ei, o4);
                 } else {
                     const float lp = in ? lap5(fc[i + 1], fp[i], fm[i], fc[i + 2], fc[i]) : 0.0f;
-                    d2 = smul(ldb(spr, 8u * xc[i + 1]), lp);
+                    d2 = smul(ldb(spr + i, oc), lp);
                 }
                 if (y * nx + x == my_src) d2 = my_d2;
                 tv[i] = v * d2;
@@ -446,20 +436,22 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                 qc[i + 1] = qp[i];
                 lc[i] = lpn[i];
             }
-            qc[0] = smul(ldb(snr, 8u * xc[0]), ldb(lnr, 4u * eo[0]));
-            qc[CX + 1] = smul(ldb(snr, 8u * xc[CX + 1]), ldb(lnr, 4u * eo[CX + 1]));
+            qc[0] = smul(ldb(snr - 1, oc), ldb(lnr - Bp, o4));
+            qc[CX + 1] = smul(ldb(snr + CX, oc), ldb(lnr + (ptrdiff_t)CX * Bp, o4));
             if (!SPONGE) {
 #pragma unroll
                 for (int i = 0; i < CX; ++i) { fm[i] = fc[i + 1]; fc[i + 1] = fp[i]; }
-                fc[0] = ldb(fnr, 4u * eo[0]);
-                fc[CX + 1] = ldb(fnr, 4u * eo[CX + 1]);
+                fc[0] = ldb(fnr - Bp, o4);
+                fc[CX + 1] = ldb(fnr + (ptrdiff_t)CX * Bp, o4);
             }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Batched 3D forward step: each lane owns a CX x CY patch and marches RZ planes.
+// Batched 3D forward step: each lane owns a CX x CY patch and marches RZ
+// planes along z.  Row starts (y clamped, wave-uniform) and column steps go
+// into the scalar base; a lane keeps one byte offset (addressing as k_fwd2d).
 // ---------------------------------------------------------------------------
 template <int LB, int CX, int CY, bool REC>
 __global__ void __launch_bounds__(kBlock)
@@ -468,8 +460,9 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
         const int *__restrict__ src_cell, const float *__restrict__ sinc,
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-        float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int nz,
-        int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+        const int *__restrict__ row_rec, float *__restrict__ rec, float *__restrict__ rec_lo,
+        float *__restrict__ srcd2, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
+        int nzc, int ng) {
     constexpr int W = kWave / LB;
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
@@ -486,70 +479,80 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     const int z0 = zc * RZ;
     const int z1 = min(z0 + RZ, nz);
     if (x0 >= nx) return;
-    const size_t BP = (size_t)Bp;
-    const int my_src = src_cell[b];
-    const float my_inc = sinc[b];
-    auto at = [&](int z, int y, int x) -> size_t {
-        x = min(max(x, 0), nx - 1);
-        y = min(max(y, 0), ny - 1);
-        return (((size_t)z * ny + y) * nx + x) * BP + b;
+    (void)src_cell;
+    (void)sinc;
+    (void)srcd2;
+    const size_t rowE = (size_t)nx * Bp;
+    const size_t planeE = rowE * ny;
+    const size_t planeC = (size_t)nx * ny;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    // row j of the patch (j = -1..CY), clamped into the grid; wave-uniform
+    auto row = [&](int j) -> size_t { return (size_t)min(max(y0 + j, 0), ny - 1); };
+    auto F = [&](const float *plane, int j, int i) -> float {
+        return ldb(plane + row(j) * rowE + (ptrdiff_t)i * Bp, o4);
     };
     float zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
-    auto load_halo = [&](int z) {
+    auto load_halo = [&](const float *pl) {
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
-            zcu[0][i + 1] = pc[at(z, y0 - 1, x0 + i)];
-            zcu[CY + 1][i + 1] = pc[at(z, y0 + CY, x0 + i)];
+            zcu[0][i + 1] = F(pl, -1, i);
+            zcu[CY + 1][i + 1] = F(pl, CY, i);
         }
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
-            zcu[j + 1][0] = pc[at(z, y0 + j, x0 - 1)];
-            zcu[j + 1][CX + 1] = pc[at(z, y0 + j, x0 + CX)];
+            zcu[j + 1][0] = F(pl, j, -1);
+            zcu[j + 1][CX + 1] = F(pl, j, CX);
         }
     };
     {
-        const int zmi = max(z0 - 1, 0);
+        const float *pm = pc + (size_t)max(z0 - 1, 0) * planeE;
+        const float *p0 = pc + (size_t)z0 * planeE;
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
-                zm[j][i] = pc[at(zmi, y0 + j, x0 + i)];
-                zcu[j + 1][i + 1] = pc[at(z0, y0 + j, x0 + i)];
+                zm[j][i] = F(pm, j, i);
+                zcu[j + 1][i + 1] = F(p0, j, i);
             }
-        load_halo(z0);
+        load_halo(p0);
     }
     for (int z = z0; z < z1; ++z) {
-        const int zpi = min(z + 1, nz - 1);
+        const float *pnz = pc + (size_t)min(z + 1, nz - 1) * planeE;
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
-            for (int i = 0; i < CX; ++i) zp[j][i] = pc[at(zpi, y0 + j, x0 + i)];
+            for (int i = 0; i < CX; ++i) zp[j][i] = F(pnz, j, i);
         const bool zint = (z >= 1) && (z <= nz - 2);
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
             const int y = y0 + j;
             const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const bool has_rec = REC && y < ny && row_rec[z * ny + y] > 0;
+            const size_t ro = (size_t)z * planeE + (size_t)y * rowE;
+            const float2 *spr = sp2 + (size_t)z * planeC + (size_t)y * nx;
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
                 const int x = x0 + i;
                 if (x < nx && y < ny) {
-                    const int c = (z * ny + y) * nx + x;
                     float lu = 0.0f;
                     if (yint && x >= 1 && x <= nx - 2)
                         lu = lap7(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
-                    const float inc = smul(sp2[c], lu);
-                    const size_t a = (size_t)c * BP + b;
+                    const float inc = smul(ldb(spr + i, oc), lu);
+                    const size_t ei = ro + (size_t)i * Bp;
                     float lo;
-                    const float v = leap_hilo(zcu[j + 1][i + 1], bf2f(pcl[a]), pp[a], bf2f(ppl[a]),
-                                              inc, lo);
+                    const float v = leap_hilo(zcu[j + 1][i + 1], bf2f(ldb(pcl + ei, o2)),
+                                              ldb(pp + ei, o4), bf2f(ldb(ppl + ei, o2)), inc, lo);
                     const uint16_t lb = f2bf(lo);
-                    pn[a] = v;
-                    pnl[a] = lb;
-                    if (REC) {
+                    stb(pn + ei, o4, v);
+                    stb(pnl + ei, o2, lb);
+                    if (has_rec) {
+                        const int c = (z * ny + y) * nx + x;
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
-                            rec[(size_t)r * BP + b] = v;
-                            rec_lo[(size_t)r * BP + b] = bf2f(lb);
+                            rec[(size_t)r * Bp + b] = v;
+                            rec_lo[(size_t)r * Bp + b] = bf2f(lb);
                         }
                     }
                 }
@@ -563,14 +566,14 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     zm[j][i] = zcu[j + 1][i + 1];
                     zcu[j + 1][i + 1] = zp[j][i];
                 }
-            load_halo(z + 1);
+            load_halo(pnz);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Batched 3D adjoint step with the imaging condition (no sponge in 3D:
-// the reference sponge is 2D only, solver.py:30).
+// the reference sponge is 2D only, solver.py:30).  Addressing as k_fwd3d.
 // ---------------------------------------------------------------------------
 template <int LB, int CX, int CY>
 __global__ void __launch_bounds__(kBlock)
@@ -578,9 +581,9 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         const float *__restrict__ pa, const float2 *__restrict__ sp2,
         const int *__restrict__ src_cell, const float *__restrict__ sinc,
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-        const float *__restrict__ res, const float *__restrict__ srcd2, float *__restrict__ g_hi,
-        float *__restrict__ g_lo, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
-        int nzc, int ng) {
+        const int *__restrict__ row_rec, const float *__restrict__ res,
+        const float *__restrict__ srcd2, float *__restrict__ g_hi, float *__restrict__ g_lo,
+        int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
     constexpr int W = kWave / LB;
     constexpr int CC = CX * CY;
     const int wv = wave_id();
@@ -597,65 +600,71 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     const int y0 = ys * CY;
     const int z0 = zc * RZ;
     const int z1 = min(z0 + RZ, nz);
-    const size_t BP = (size_t)Bp;
-    const size_t N = (size_t)nz * ny * nx;
+    (void)sinc;
+    const size_t rowE = (size_t)nx * Bp;
+    const size_t planeE = rowE * ny;
+    const size_t planeC = (size_t)nx * ny;
+    const size_t N = planeC * nz;
     const int my_src = src_cell[b];
     const float my_d2 = srcd2[b];
-    auto cl = [&](int z, int y, int x) -> int {
-        x = min(max(x, 0), nx - 1);
-        y = min(max(y, 0), ny - 1);
-        return (z * ny + y) * nx + x;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    auto row = [&](int j) -> size_t { return (size_t)min(max(y0 + j, 0), ny - 1); };
+    auto F = [&](const float *plane, int j, int i) -> float {
+        return ldb(plane + row(j) * rowE + (ptrdiff_t)i * Bp, o4);
     };
-    auto ldq = [&](int z, int y, int x) -> float {
-        const int c = cl(z, y, x);
-        return smul(sp2[c], l1[(size_t)c * BP + b]);
+    auto Q = [&](const float *plane, const float2 *splane, int j, int i) -> float {
+        return smul(ldb(splane + row(j) * nx + i, oc), F(plane, j, i));
     };
-    auto ldp = [&](int z, int y, int x) -> float { return pa[(size_t)cl(z, y, x) * BP + b]; };
     float qm[CY][CX], qc[CY + 2][CX + 2], qp[CY][CX], lc[CY][CX], lpn[CY][CX];
     float fm[CY][CX], fc[CY + 2][CX + 2], fp[CY][CX];
-    auto load_halo = [&](int z) {
+    auto load_halo = [&](const float *lz, const float2 *sz, const float *fz) {
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
-            qc[0][i + 1] = ldq(z, y0 - 1, x0 + i);
-            qc[CY + 1][i + 1] = ldq(z, y0 + CY, x0 + i);
-            fc[0][i + 1] = ldp(z, y0 - 1, x0 + i);
-            fc[CY + 1][i + 1] = ldp(z, y0 + CY, x0 + i);
+            qc[0][i + 1] = Q(lz, sz, -1, i);
+            qc[CY + 1][i + 1] = Q(lz, sz, CY, i);
+            fc[0][i + 1] = F(fz, -1, i);
+            fc[CY + 1][i + 1] = F(fz, CY, i);
         }
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
-            qc[j + 1][0] = ldq(z, y0 + j, x0 - 1);
-            qc[j + 1][CX + 1] = ldq(z, y0 + j, x0 + CX);
-            fc[j + 1][0] = ldp(z, y0 + j, x0 - 1);
-            fc[j + 1][CX + 1] = ldp(z, y0 + j, x0 + CX);
+            qc[j + 1][0] = Q(lz, sz, j, -1);
+            qc[j + 1][CX + 1] = Q(lz, sz, j, CX);
+            fc[j + 1][0] = F(fz, j, -1);
+            fc[j + 1][CX + 1] = F(fz, j, CX);
         }
     };
     {
         const int zmi = max(z0 - 1, 0);
+        const float *lm = l1 + (size_t)zmi * planeE, *l0 = l1 + (size_t)z0 * planeE;
+        const float2 *sm = sp2 + (size_t)zmi * planeC, *s0 = sp2 + (size_t)z0 * planeC;
+        const float *fmz = pa + (size_t)zmi * planeE, *f0 = pa + (size_t)z0 * planeE;
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
-                qm[j][i] = ldq(zmi, y0 + j, x0 + i);
-                fm[j][i] = ldp(zmi, y0 + j, x0 + i);
-                const int c = cl(z0, y0 + j, x0 + i);
-                const float v = l1[(size_t)c * BP + b];
+                qm[j][i] = Q(lm, sm, j, i);
+                fm[j][i] = F(fmz, j, i);
+                const float v = F(l0, j, i);
                 lc[j][i] = v;
-                qc[j + 1][i + 1] = smul(sp2[c], v);
-                fc[j + 1][i + 1] = pa[(size_t)c * BP + b];
+                qc[j + 1][i + 1] = smul(ldb(s0 + row(j) * nx + i, oc), v);
+                fc[j + 1][i + 1] = F(f0, j, i);
             }
-        load_halo(z0);
+        load_halo(l0, s0, f0);
     }
     for (int z = z0; z < z1; ++z) {
         const int zpi = min(z + 1, nz - 1);
+        const float *lnz = l1 + (size_t)zpi * planeE;
+        const float2 *snz = sp2 + (size_t)zpi * planeC;
+        const float *fnz = pa + (size_t)zpi * planeE;
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
-                const int c = cl(zpi, y0 + j, x0 + i);
-                const float v = l1[(size_t)c * BP + b];
+                const float v = F(lnz, j, i);
                 lpn[j][i] = v;
-                qp[j][i] = smul(sp2[c], v);
-                fp[j][i] = pa[(size_t)c * BP + b];
+                qp[j][i] = smul(ldb(snz + row(j) * nx + i, oc), v);
+                fp[j][i] = F(fnz, j, i);
             }
         const bool zint = (z >= 1) && (z <= nz - 2);
         float tv[CC];
@@ -663,24 +672,30 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         for (int j = 0; j < CY; ++j) {
             const int y = y0 + j;
             const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const bool has_rec = y < ny && row_rec[z * ny + y] > 0;
+            const size_t ro = (size_t)z * planeE + (size_t)y * rowE;
+            const float2 *spr = sp2 + (size_t)z * planeC + (size_t)y * nx;
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
                 const int x = x0 + i;
                 tv[j * CX + i] = 0.0f;
                 if (x < nx && y < ny) {
-                    const int c = (z * ny + y) * nx + x;
                     const bool in = yint && x >= 1 && x <= nx - 2;
                     const float lq = in ? lap7(qc[j + 1][i + 1], qp[j][i], qm[j][i], qc[j + 2][i + 1],
                                                qc[j][i + 1], qc[j + 1][i + 2], qc[j + 1][i])
                                         : 0.0f;
-                    const size_t a = (size_t)c * BP + b;
-                    float v = leap(lc[j][i], l2[a], lq);
-                    for (int r = rec_head[c]; r >= 0; r = rec_next[r]) v = v - res[(size_t)r * BP + b];
-                    lk[a] = v;
+                    const size_t ei = ro + (size_t)i * Bp;
+                    float v = leap(lc[j][i], ldb(l2 + ei, o4), lq);
+                    const int c = (z * ny + y) * nx + x;
+                    if (has_rec) {
+                        for (int r = rec_head[c]; r >= 0; r = rec_next[r])
+                            v = v - res[(size_t)r * Bp + b];
+                    }
+                    stb(lk + ei, o4, v);
                     const float lp = in ? lap7(fc[j + 1][i + 1], fp[j][i], fm[j][i], fc[j + 2][i + 1],
                                                fc[j][i + 1], fc[j + 1][i + 2], fc[j + 1][i])
                                         : 0.0f;
-                    float d2 = smul(sp2[c], lp);
+                    float d2 = smul(ldb(spr + i, oc), lp);
                     if (c == my_src) d2 = my_d2;
                     tv[j * CX + i] = v * d2;
                 }
@@ -709,7 +724,7 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                     fm[j][i] = fc[j + 1][i + 1];
                     fc[j + 1][i + 1] = fp[j][i];
                 }
-            load_halo(z + 1);
+            load_halo(lnz, snz, fnz);
         }
     }
 }
