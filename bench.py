@@ -117,6 +117,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ckpt-every", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="skip the per-launch event timing pass (for counter runs)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first run an elementwise copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -130,6 +134,13 @@ def main():
     from openusct_amd.plan import Plan, linear_indices
 
     dev = torch.device("cuda", 0)
+    if args.calib_copy_mb:
+        # y = x * 1: reads and writes exactly calib_copy_mb MiB each
+        x = torch.ones(args.calib_copy_mb << 18, dtype=torch.float32, device=dev)
+        for _ in range(3):
+            y = x * 1.0
+        torch.cuda.synchronize()
+        del x, y
     prob = workload(args.workload)
     shape, h, dt, nt = prob.shape, prob.h, prob.dt, prob.nt
     N = int(np.prod(shape))
@@ -164,7 +175,10 @@ def main():
     print(f"[bench] batched {ms:.1f} ms/gradient, {value:.1f} Gpt-updates/s", file=sys.stderr,
           flush=True)
 
-    prof = plan.profile_gradient(md, wd, dobs)
+    if args.no_profile:
+        prof = {"fwd_ms": float("nan"), "adj_ms": float("nan"), "n_fwd": 0, "n_adj": 0}
+    else:
+        prof = plan.profile_gradient(md, wd, dobs)
     per_pass = len(tx) / max(info["shot_passes"], 1)
     adj_bytes = BYTES_ADJ * N * per_pass
     fwd_bytes = BYTES_FWD * N * per_pass
@@ -198,6 +212,15 @@ def main():
         print("[bench] timing the reference CPU core", file=sys.stderr, flush=True)
         cpu = cpu_baseline()
 
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "pmc_traffic_ring512_128shots.json")
+    if args.workload == "C2" and len(tx) == 128 and os.path.exists(tfile):
+        # HBM bytes per launch of the adjoint kernel from the committed rocprofv3 --pmc
+        # FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy of known size)
+        with open(tfile) as fh:
+            k = json.load(fh)["kernels"]["k_adj2d"]
+        traffic = round(k["read_bytes"] + k["write_bytes"])
+
     line = {
         "metric": "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency",
         "value": round(value, 3),
@@ -228,7 +251,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(adj_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "per_launch_ms": round(prof["adj_ms"], 4),
             "alg_bytes_per_launch": adj_bytes,
             "fwd_kernel": {"achieved": round(fwd_gbs, 1), "per_launch_ms": round(prof["fwd_ms"], 4),
