@@ -180,12 +180,17 @@ def main():
     else:
         prof = plan.profile_gradient(md, wd, dobs)
     per_pass = len(tx) / max(info["shot_passes"], 1)
-    adj_bytes = BYTES_ADJ * N * per_pass
-    fwd_bytes = BYTES_FWD * N * per_pass
+    lv = max(int(prof.get("levels_per_launch", 1)), 1)  # time levels per launch
+    adj_bytes = BYTES_ADJ * N * per_pass * lv
+    fwd_bytes = BYTES_FWD * N * per_pass * lv
     adj_gbs = adj_bytes / (prof["adj_ms"] / 1e3) / 1e9
     fwd_gbs = fwd_bytes / (prof["fwd_ms"] / 1e3) / 1e9
-    print(f"[bench] per launch: fwd {prof['fwd_ms']:.3f} ms ({fwd_gbs:.0f} GB/s alg), "
-          f"adj {prof['adj_ms']:.3f} ms ({adj_gbs:.0f} GB/s alg)", file=sys.stderr, flush=True)
+    print(f"[bench] per launch ({lv} level(s)): fwd {prof['fwd_ms']:.3f} ms ({fwd_gbs:.0f} GB/s "
+          f"alg), adj {prof['adj_ms']:.3f} ms ({adj_gbs:.0f} GB/s alg)", file=sys.stderr,
+          flush=True)
+    suffix = "_x2" if lv == 2 else ""
+    adj_kernel = ("k_adj2d" if len(shape) == 2 else "k_adj3d") + suffix
+    fwd_kernel = ("k_fwd2d" if len(shape) == 2 else "k_fwd3d") + suffix
     Jv = float(J.item())
     plan.close()
     del dobs
@@ -218,8 +223,9 @@ def main():
         # HBM bytes per launch of the adjoint kernel from the committed rocprofv3 --pmc
         # FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy of known size)
         with open(tfile) as fh:
-            k = json.load(fh)["kernels"]["k_adj2d"]
-        traffic = round(k["read_bytes"] + k["write_bytes"])
+            k = json.load(fh)["kernels"].get(adj_kernel)
+        if k is not None:
+            traffic = round(k["read_bytes"] + k["write_bytes"])
 
     line = {
         "metric": "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency",
@@ -239,14 +245,15 @@ def main():
             WORKLOADS[args.workload] + f" (first {len(tx)} shots)",
             "grid": list(shape), "nt": nt, "shots": len(tx), "receivers": len(rec),
             "method": "batched", "plan": info, "peak_hbm_bytes": torch.cuda.max_memory_allocated() + info["workspace_bytes"],
-            "recompute_launch_fraction": round(max(prof["n_fwd"] - (nt - 1) * info["shot_passes"], 0) / max(prof["n_fwd"] + prof["n_adj"], 1), 4),
+            "recompute_level_fraction": round(max(prof["n_fwd"] - (nt - 1) * info["shot_passes"], 0) / max(prof["n_fwd"] + prof["n_adj"], 1), 4),
         },
         "plain_value": None if plain is None else round(plain, 3),
         "batched_vs_plain": None if plain is None else round(value / plain, 3),
         "misfit": Jv,
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_adj2d" if len(shape) == 2 else "k_adj3d") + " (adjoint step + imaging, all shots of a pass)",
+            "kernel": adj_kernel + f" (adjoint + imaging, {lv} time level(s) per launch, all "
+                                   "shots of a pass)",
             "achieved": round(adj_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -254,7 +261,9 @@ def main():
             "traffic": traffic,
             "per_launch_ms": round(prof["adj_ms"], 4),
             "alg_bytes_per_launch": adj_bytes,
-            "fwd_kernel": {"achieved": round(fwd_gbs, 1), "per_launch_ms": round(prof["fwd_ms"], 4),
+            "levels_per_launch": lv,
+            "fwd_kernel": {"name": fwd_kernel, "achieved": round(fwd_gbs, 1),
+                           "per_launch_ms": round(prof["fwd_ms"], 4),
                            "alg_bytes_per_launch": fwd_bytes},
             "gradient_level": round(value * BYTES_PER_UPDATE, 1),
         },

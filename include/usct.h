@@ -52,7 +52,8 @@ enum {
     USCT_FLAG_SPONGE = 1,    /* calls pass a multiplicative sponge mask (2D only) */
     USCT_FLAG_AUTOTUNE = 2,  /* time candidate tile shapes at creation, keep the best */
     USCT_FLAG_NO_GRAPH = 4,  /* batched: launch eagerly instead of replaying a hipGraph */
-    USCT_FLAG_FORWARD_ONLY = 8  /* no gradient workspace (forward_fmc / misfit only) */
+    USCT_FLAG_FORWARD_ONLY = 8, /* no gradient workspace (forward_fmc / misfit only) */
+    USCT_FLAG_NO_FUSE = 16      /* batched 2D: one time level per launch instead of two */
 };
 
 /* Misfit codes (software/python/uap.py:41). */
@@ -110,12 +111,16 @@ int usct_misfit_and_gradient(usct_plan *plan, const double *m, const double *wav
 
 /*
  * Time one gradient launch by launch with HIP events on the plan's stream.
- * fwd_ms / adj_ms: mean duration of one forward-step / adjoint-step kernel;
- * n_fwd / n_adj: launches timed (recompute launches count as forward steps).
+ * levels_per_launch: time levels one launch advances (2 when the plan runs
+ *   two-level kernels, else 1);
+ * fwd_ms / adj_ms: mean duration of the forward / adjoint launches that advance
+ *   levels_per_launch levels (a trailing single-level launch is not averaged in);
+ * n_fwd / n_adj: time levels advanced in all (re-computed levels count as forward).
  */
 int usct_profile_gradient(usct_plan *plan, const double *m, const double *wavelet,
                           const double *sponge, const float *dobs, double *fwd_ms,
-                          double *adj_ms, int *n_fwd, int *n_adj, void *stream);
+                          double *adj_ms, int *n_fwd, int *n_adj, int *levels_per_launch,
+                          void *stream);
 
 /* Stateless forms with the argument order of uap::forward_fmc and
  * uap::misfit_and_gradient (device pointers as above, J returned on the host). */

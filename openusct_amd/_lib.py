@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libusct.so")
 
 USCT_OK, USCT_EINVAL, USCT_ENOMEM, USCT_EHIP, USCT_ENOTSUP = range(5)
 METHOD_AUTO, METHOD_PLAIN, METHOD_BATCHED = 0, 1, 2
-FLAG_SPONGE, FLAG_AUTOTUNE, FLAG_NO_GRAPH, FLAG_FORWARD_ONLY = 1, 2, 4, 8
+FLAG_SPONGE, FLAG_AUTOTUNE, FLAG_NO_GRAPH, FLAG_FORWARD_ONLY, FLAG_NO_FUSE = 1, 2, 4, 8, 16
 MISFIT_CODES = {"l2": 0, "gcn": 1}
 
 # Every symbol declared in include/usct.h.
@@ -74,7 +74,7 @@ def load(path=LIB_PATH):
     lib.usct_misfit.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
     lib.usct_misfit_and_gradient.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp, vp]
     lib.usct_profile_gradient.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double),
-                                          ctypes.POINTER(ctypes.c_double), intp, intp, vp]
+                                          ctypes.POINTER(ctypes.c_double), intp, intp, intp, vp]
     lib.usct_forward_fmc_direct.argtypes = [vp, intp, ip, dp, dp, ip, i64p, ip, i64p, ip, vp, fp,
                                             ip, vp]
     lib.usct_misfit_and_gradient_direct.argtypes = [vp, intp, ip, dp, dp, ip, i64p, ip, i64p, ip,

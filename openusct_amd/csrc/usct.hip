@@ -73,6 +73,7 @@ enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2 };
 
 struct Profile {
     std::vector<hipEvent_t> fwd_ev, adj_ev;  // start/stop pairs
+    std::vector<int> fwd_lv, adj_lv;         // time levels advanced by each timed launch
 };
 
 }  // namespace
@@ -104,10 +105,10 @@ struct usct_plan {
     float *sinc = nullptr;   // [nt][Bp]
     float *srcd2 = nullptr;  // [nt][Bp] recorded increment at each shot's source cell
     float *Z = nullptr;      // zero field
-    float *F[3] = {nullptr, nullptr, nullptr};
+    float *F[4] = {nullptr, nullptr, nullptr, nullptr};  // rotation (4: fused two-level kernels)
     std::vector<float *> Wslot, CK;
     // bfloat16 low parts of the forward state: rotation, zero level, checkpoints
-    uint16_t *Lo[3] = {nullptr, nullptr, nullptr};
+    uint16_t *Lo[4] = {nullptr, nullptr, nullptr, nullptr};
     uint16_t *Zlo = nullptr;
     std::vector<uint16_t *> CKlo;
     float *recb = nullptr, *recb_lo = nullptr, *dobsT = nullptr, *res = nullptr;
@@ -290,13 +291,51 @@ void launch_fwd_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl p
     hipLaunchKernelGGL((k_src_fix<SP, ND, RC>), gs, bs, 0, s, pc.hi, pc.lo, pp.hi, pp.lo, pn.hi, \
                        pn.lo, P->sp2, P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, rec, \
                        rl, d2, P->nz, P->ny, P->nx, P->Bp)
-    if (P->ndim == 2) {
-        if (P->sponge) { if (record) USCT_FIX(true, 2, true); else USCT_FIX(true, 2, false); }
-        else { if (record) USCT_FIX(false, 2, true); else USCT_FIX(false, 2, false); }
-    } else {
+    if (P->ndim == 3) {  // 2D adds the source in FP32 inside the stencil kernel
         if (record) USCT_FIX(false, 3, true); else USCT_FIX(false, 3, false);
     }
 #undef USCT_FIX
+}
+
+// Two forward levels per launch (2D, no sponge): levels n and n+1.
+template <int LB, int CX, bool REC>
+void fwd2d_x2_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_plan *P,
+                int n, const Tile &t) {
+    const float *sa = P->sinc + (int64_t)n * P->Bp;
+    const int64_t tr = (int64_t)P->n_rec * P->Bp;
+    float *ra = P->recb + (int64_t)n * tr, *rla = P->recb_lo + (int64_t)n * tr;
+    hipLaunchKernelGGL((k_fwd2d_x2<LB, CX, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pa.hi,
+                       pb.hi, pc.lo, pp.lo, pa.lo, pb.lo, P->sp2, P->src_cell, sa, sa + P->Bp,
+                       P->rec_head, P->rec_next, P->row_rec, ra, rla, ra + tr, rla + tr, P->ny,
+                       P->nx, P->Bp, P->RY, t.nxs, t.nyc, P->ng());
+}
+
+template <int LB, bool REC>
+void fwd2d_x2_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_plan *P,
+                 int n, const Tile &t) {
+    if (P->CX == 4) fwd2d_x2_t<LB, 4, REC>(g, s, pc, pp, pa, pb, P, n, t);
+    else fwd2d_x2_t<LB, 8, REC>(g, s, pc, pp, pa, pb, P, n, t);
+}
+
+void launch_fwd2_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb,
+                         int n, bool record) {
+    const Tile t = tile_of(P);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+#define USCT_X2(LB_)                                                          \
+    if (record) fwd2d_x2_lb<LB_, true>(g, s, pc, pp, pa, pb, P, n, t);        \
+    else fwd2d_x2_lb<LB_, false>(g, s, pc, pp, pa, pb, P, n, t);
+    switch (P->LB) {
+        case 16: USCT_X2(16) break;
+        case 32: USCT_X2(32) break;
+        default: USCT_X2(64) break;
+    }
+#undef USCT_X2
+}
+
+// Whether the plan advances two time levels per launch.
+bool fused(const usct_plan *P) {
+    return P->method == USCT_METHOD_BATCHED && P->ndim == 2 && !P->sponge &&
+           !(P->flags & USCT_FLAG_NO_FUSE);
 }
 
 template <int LB, int CX, bool SP>
@@ -327,6 +366,34 @@ void adj2d_sel(dim3 g, hipStream_t s, const float *l1, const float *l2, float *l
         case 32: adj2d_lb<32, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
         default: adj2d_lb<64, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
     }
+}
+
+// Two adjoint levels per launch (2D, no sponge): lam^k and lam^{k-1}.
+template <int LB, int CX>
+void adj2d_x2_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lu, float *lv,
+                const float *pa1, const float *pa2, const usct_plan *P, int k, const Tile &t) {
+    const float *su = P->sinc + (int64_t)k * P->Bp;
+    const int64_t tr = (int64_t)P->n_rec * P->Bp;
+    const float *ru = P->res + (int64_t)k * tr;
+    hipLaunchKernelGGL((k_adj2d_x2<LB, CX>), g, dim3(kBlock), 0, s, l1, l2, lu, lv, pa1, pa2,
+                       P->sp2, P->src_cell, su, su - P->Bp, P->rec_head, P->rec_next, P->row_rec,
+                       ru, ru - tr, P->g_hi, P->g_lo, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nyc,
+                       P->ng());
+}
+
+void launch_adj2_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
+                         float *lu, float *lv, const float *pa1, const float *pa2, int k) {
+    const Tile t = tile_of(P);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+#define USCT_A2(LB_)                                                                        \
+    if (P->CX == 4) adj2d_x2_t<LB_, 4>(g, s, l1, l2, lu, lv, pa1, pa2, P, k, t);            \
+    else adj2d_x2_t<LB_, 8>(g, s, l1, l2, lu, lv, pa1, pa2, P, k, t);
+    switch (P->LB) {
+        case 16: USCT_A2(16) break;
+        case 32: USCT_A2(32) break;
+        default: USCT_A2(64) break;
+    }
+#undef USCT_A2
 }
 
 template <int LB, int CX, int CY>
@@ -380,17 +447,18 @@ int ck_slot(const usct_plan *P, int n) {
     return -1;
 }
 
-// Where level n lives during the forward pass.  Low parts rotate over three
-// buffers except at checkpoint levels, whose low parts are kept.
+// Where level n lives during the forward pass.  Low parts rotate over four
+// buffers (a two-level launch touches levels n-2 .. n+1) except at checkpoint
+// levels, whose low parts are kept.
 Lvl fwd_level(const usct_plan *P, int n, bool store) {
     if (n <= 0) return {P->Z, P->Zlo};
     if (store) {
         const int c = ck_slot(P, n);
         if (c >= 0) return {P->CK[c], P->CKlo[c]};
         const int base_last = (P->NS - 1) * P->K;
-        if (n >= base_last - 1) return {P->Wslot[n - base_last + 1], P->Lo[n % 3]};
+        if (n >= base_last - 1) return {P->Wslot[n - base_last + 1], P->Lo[n % 4]};
     }
-    return {P->F[n % 3], P->Lo[n % 3]};
+    return {P->F[n % 4], P->Lo[n % 4]};
 }
 
 // Where level `level` lives while window s is re-computed / consumed.
@@ -398,12 +466,12 @@ Lvl win_level(const usct_plan *P, int s, int level) {
     if (level <= 0) return {P->Z, P->Zlo};
     const int base = s * P->K;
     const int i = level - base + 1;
-    if (s == P->NS - 1 || i >= 2) return {P->Wslot[i], P->Lo[level % 3]};
+    if (s == P->NS - 1 || i >= 2) return {P->Wslot[i], P->Lo[level % 4]};
     const int c = 2 * (s - 1) + i;
     return {P->CK[c], P->CKlo[c]};
 }
 
-float *lam_level(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->F[k % 3]; }
+float *lam_level(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->F[k % 4]; }
 
 void ev_start(Profile *pr, std::vector<hipEvent_t> &v, hipStream_t s) {
     if (!pr) return;
@@ -417,6 +485,19 @@ void ev_start(Profile *pr, std::vector<hipEvent_t> &v, hipStream_t s) {
 void ev_stop(Profile *pr, std::vector<hipEvent_t> &v, hipStream_t s) {
     if (!pr) return;
     (void)hipEventRecord(v.back(), s);
+}
+
+// Forward levels n, n+1 (two-level launch) or n alone; returns levels advanced.
+int fwd_step(usct_plan *P, hipStream_t s, Lvl l2, Lvl l1, Lvl a, Lvl b, int n, bool two,
+             bool record, Profile *pr) {
+    if (pr) ev_start(pr, pr->fwd_ev, s);
+    if (two) launch_fwd2_batched(P, s, l1, l2, a, b, n, record);
+    else launch_fwd_batched(P, s, l1, l2, a, n, record);
+    if (pr) {
+        ev_stop(pr, pr->fwd_ev, s);
+        pr->fwd_lv.push_back(two ? 2 : 1);
+    }
+    return two ? 2 : 1;
 }
 
 // Residual and misfit of the traces in recb against dobsT for the current pass.
@@ -449,11 +530,12 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
                        P->recb, (int64_t)P->n_rec * P->Bp, 0.0f);
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
                        P->recb_lo, (int64_t)P->n_rec * P->Bp, 0.0f);
-    for (int n = 1; n < nt; ++n) {
-        if (pr) ev_start(pr, pr->fwd_ev, s);
-        launch_fwd_batched(P, s, fwd_level(P, n - 1, store), fwd_level(P, n - 2, store),
-                           fwd_level(P, n, store), n, true);
-        if (pr) ev_stop(pr, pr->fwd_ev, s);
+    const bool x2 = fused(P);
+    for (int n = 1; n < nt;) {
+        const bool two = x2 && n + 1 < nt;
+        n += fwd_step(P, s, fwd_level(P, n - 2, store), fwd_level(P, n - 1, store),
+                      fwd_level(P, n, store), two ? fwd_level(P, n + 1, store) : Lvl{},
+                      n, two, true, pr);
     }
     if (mode == MODE_FWD) return;
     launch_residual(P, s, misfit, store);
@@ -462,26 +544,37 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
         const int base = w * P->K;
         const int khi = std::min(base + P->K, nt - 1);
         if (w < P->NS - 1) {
-            for (int n = base + 1; n <= khi; ++n) {
-                if (pr) ev_start(pr, pr->fwd_ev, s);
-                launch_fwd_batched(P, s, win_level(P, w, n - 1), win_level(P, w, n - 2),
-                                   win_level(P, w, n), n, false);
-                if (pr) ev_stop(pr, pr->fwd_ev, s);
+            for (int n = base + 1; n <= khi;) {
+                const bool two = x2 && n + 1 <= khi;
+                n += fwd_step(P, s, win_level(P, w, n - 2), win_level(P, w, n - 1),
+                              win_level(P, w, n), two ? win_level(P, w, n + 1) : Lvl{}, n, two,
+                              false, pr);
             }
         }
-        for (int k = khi; k >= base + 1; --k) {
-            const float *pa, *pb = nullptr, *pc2 = nullptr;
-            if (P->sponge) {
-                pa = win_level(P, w, k).hi;
-                pb = win_level(P, w, k - 1).hi;
-                pc2 = win_level(P, w, k - 2).hi;
-            } else {
-                pa = win_level(P, w, k - 1).hi;
-            }
+        for (int k = khi; k >= base + 1;) {
+            const bool two = x2 && k - 1 >= base + 1;
             if (pr) ev_start(pr, pr->adj_ev, s);
-            launch_adj_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2), lam_level(P, k),
-                               pa, pb, pc2, k);
-            if (pr) ev_stop(pr, pr->adj_ev, s);
+            if (two) {
+                launch_adj2_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2),
+                                    lam_level(P, k), lam_level(P, k - 1),
+                                    win_level(P, w, k - 1).hi, win_level(P, w, k - 2).hi, k);
+            } else {
+                const float *pa, *pb = nullptr, *pc2 = nullptr;
+                if (P->sponge) {
+                    pa = win_level(P, w, k).hi;
+                    pb = win_level(P, w, k - 1).hi;
+                    pc2 = win_level(P, w, k - 2).hi;
+                } else {
+                    pa = win_level(P, w, k - 1).hi;
+                }
+                launch_adj_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2),
+                                   lam_level(P, k), pa, pb, pc2, k);
+            }
+            if (pr) {
+                ev_stop(pr, pr->adj_ev, s);
+                pr->adj_lv.push_back(two ? 2 : 1);
+            }
+            k -= two ? 2 : 1;
         }
     }
 }
@@ -588,9 +681,9 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
     // History slots: level n in Wslot[n] (n >= 1), levels <= 0 are Z.
     auto lvl = [&](int n) -> float * {
         if (n <= 0) return P->Z;
-        return store ? P->Wslot[n] : P->F[n % 3];
+        return store ? P->Wslot[n] : P->F[n % 4];
     };
-    auto llo = [&](int n) -> uint16_t * { return n <= 0 ? P->Zlo : P->Lo[n % 3]; };
+    auto llo = [&](int n) -> uint16_t * { return n <= 0 ? P->Zlo : P->Lo[n % 4]; };
     for (int b = 0; b < P->n_tx; ++b) {
         const int src = (int)P->tx[b];
         // S_src * w[n-1] per level, same FP64 product as the batched path.
@@ -612,7 +705,10 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
                                    P->sp2, P->spg, src, P->sinc + n, P->rec_head, P->rec_next,
                                    P->recb + (int64_t)n * nr, P->recb_lo + (int64_t)n * nr,
                                    P->srcd2 + n, P->nz, P->ny, P->nx, nd);
-            if (pr) ev_stop(pr, pr->fwd_ev, s);
+            if (pr) {
+                ev_stop(pr, pr->fwd_ev, s);
+                pr->fwd_lv.push_back(1);
+            }
         }
         if (data)
             hipLaunchKernelGGL(k_batched_to_data, dim3(grid_for((int64_t)nt * nr)), dim3(256), 0, s,
@@ -633,16 +729,19 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
             if (pr) ev_start(pr, pr->adj_ev, s);
             if (P->sponge)
                 hipLaunchKernelGGL((k_adj_plain<true>), dim3(gN), dim3(256), 0, s, lam_level(P, k + 1),
-                                   lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2, P->spg, src, P->srcd2 + k,
+                                   lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2, P->spg, src, P->sinc + k, P->srcd2 + k,
                                    P->rec_head, P->rec_next, P->res + (int64_t)k * nr, P->g_hi,
                                    P->g_lo, P->nz, P->ny, P->nx, nd);
             else
                 hipLaunchKernelGGL((k_adj_plain<false>), dim3(gN), dim3(256), 0, s,
                                    lam_level(P, k + 1), lam_level(P, k + 2), lk, pa, pb, pc2, P->sp2,
-                                   P->spg, src, P->srcd2 + k, P->rec_head, P->rec_next,
+                                   P->spg, src, P->sinc + k, P->srcd2 + k, P->rec_head, P->rec_next,
                                    P->res + (int64_t)k * nr, P->g_hi, P->g_lo, P->nz, P->ny, P->nx,
                                    nd);
-            if (pr) ev_stop(pr, pr->adj_ev, s);
+            if (pr) {
+                ev_stop(pr, pr->adj_ev, s);
+                pr->adj_lv.push_back(1);
+            }
         }
         TRY(check_launch());
     }
@@ -665,14 +764,14 @@ struct MemPlan {
 int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
     const int64_t field = P->N * (int64_t)Bp * 4;
     const int64_t traces = (int64_t)P->nt * P->n_rec * Bp * 4;
-    int64_t fields = 4;  // Z + F[3]
-    int64_t halves = 4;  // Zlo + Lo[3]
+    int64_t fields = 5;  // Z + F[4]
+    int64_t halves = 5;  // Zlo + Lo[4]
     if (grad) {
         fields += (K + 2) + 2 * std::max(NS - 2, 0);
         halves += 2 * std::max(NS - 2, 0);
     }
     int64_t b = fields * field + halves * (field / 2) + 4 * traces;
-    b += fields * 2 * kGuardCells * Bp * 4;
+    b += (fields * 4 + halves * 2) * 2 * kGuardCells * Bp;
     b += P->N * 8 /*sp2*/ + P->N * 4 /*sponge*/ + 2 * P->N * 4 * (Bp / 16) /*g*/;
     b += (int64_t)P->nt * Bp * 4 + (int64_t)P->n_rec * Bp * 24 + (int64_t)P->N * 4 + (1 << 20);
     return b;
@@ -747,9 +846,9 @@ int allocate_batched(usct_plan *P) {
     const int64_t FE = P->field_elems();
     TRY(dalloc_guarded(P, &P->Z, FE, P->Bp));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * FE));
-    for (int i = 0; i < 3; ++i) TRY(dalloc_guarded(P, &P->F[i], FE, P->Bp));
-    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->Lo[i], FE));
-    TRY(dalloc_t(P, &P->Zlo, FE));
+    for (int i = 0; i < 4; ++i) TRY(dalloc_guarded(P, &P->F[i], FE, P->Bp));
+    for (int i = 0; i < 4; ++i) TRY(dalloc_guarded(P, &P->Lo[i], FE, P->Bp));
+    TRY(dalloc_guarded(P, &P->Zlo, FE, P->Bp));
     HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * FE));
     const bool fwd_only = (P->flags & USCT_FLAG_FORWARD_ONLY) != 0;
     const int nW = fwd_only ? 0 : P->K + 2;
@@ -759,7 +858,7 @@ int allocate_batched(usct_plan *P) {
     P->CK.resize(nC);
     for (int i = 0; i < nC; ++i) TRY(dalloc_guarded(P, &P->CK[i], FE, P->Bp));
     P->CKlo.resize(nC);
-    for (int i = 0; i < nC; ++i) TRY(dalloc_t(P, &P->CKlo[i], FE));
+    for (int i = 0; i < nC; ++i) TRY(dalloc_guarded(P, &P->CKlo[i], FE, P->Bp));
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
     TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * ntr));
     TRY(dalloc_t(P, &P->recb_lo, (int64_t)P->nt * ntr));
@@ -792,8 +891,8 @@ int allocate_plain(usct_plan *P) {
     const int64_t N = P->N;
     TRY(dalloc_t(P, &P->Z, N));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * N));
-    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->F[i], N));
-    for (int i = 0; i < 3; ++i) TRY(dalloc_t(P, &P->Lo[i], N));
+    for (int i = 0; i < 4; ++i) TRY(dalloc_t(P, &P->F[i], N));
+    for (int i = 0; i < 4; ++i) TRY(dalloc_t(P, &P->Lo[i], N));
     TRY(dalloc_t(P, &P->Zlo, N));
     HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * N));
     // full history, levels 1..nt-1 (slot 0 unused)
@@ -833,22 +932,24 @@ double time_tile(usct_plan *P) {
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     const int reps = 4;
-    // warm-up
-    launch_fwd_batched(P, P->stream, {P->F[0], P->Lo[0]}, {P->F[1], P->Lo[1]},
-                       {P->F[2], P->Lo[2]}, 1, false);
     const bool adj = !P->Wslot.empty();
-    if (adj)
-        launch_adj_batched(P, P->stream, P->F[0], P->F[1], P->F[2], P->Wslot[0], P->Wslot[0],
-                           P->Wslot[0], 1);
+    const bool x2 = fused(P) && P->nt >= 3;  // levels 1 and 2 exist
+    // one forward launch (one or two levels) and one adjoint launch, rotating buffers
+    auto step = [&](int r) {
+        Lvl L[4];
+        for (int i = 0; i < 4; ++i) L[i] = {P->F[(r + i) % 4], P->Lo[(r + i) % 4]};
+        if (x2) launch_fwd2_batched(P, P->stream, L[1], L[0], L[2], L[3], 1, false);
+        else launch_fwd_batched(P, P->stream, L[1], L[0], L[2], 1, false);
+        if (adj && x2)
+            launch_adj2_batched(P, P->stream, L[0].hi, L[1].hi, L[2].hi, L[3].hi, P->Wslot[0],
+                                P->Wslot[0], 2);
+        else if (adj)
+            launch_adj_batched(P, P->stream, L[0].hi, L[1].hi, L[2].hi, P->Wslot[0],
+                               P->Wslot[0], P->Wslot[0], 1);
+    };
+    step(0);  // warm-up
     (void)hipEventRecord(a, P->stream);
-    for (int r = 0; r < reps; ++r) {
-        launch_fwd_batched(P, P->stream, {P->F[r % 3], P->Lo[r % 3]},
-                           {P->F[(r + 1) % 3], P->Lo[(r + 1) % 3]},
-                           {P->F[(r + 2) % 3], P->Lo[(r + 2) % 3]}, 1, false);
-        if (adj)
-            launch_adj_batched(P, P->stream, P->F[r % 3], P->F[(r + 1) % 3], P->F[(r + 2) % 3],
-                               P->Wslot[0], P->Wslot[0], P->Wslot[0], 1);
-    }
+    for (int r = 0; r < reps; ++r) step(r);
     (void)hipEventRecord(b, P->stream);
     (void)hipEventSynchronize(b);
     float ms = 1e30f;
@@ -879,7 +980,7 @@ int autotune(usct_plan *P) {
         return 0;
     }
     // zero the fields used by the timing launches
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
         HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
         HIPX(hipMemset(P->Lo[i], 0, sizeof(uint16_t) * P->field_elems()));
     }
@@ -909,9 +1010,13 @@ int autotune(usct_plan *P) {
     }
     double best = 1e300;
     Cand bc{P->LB, P->CX, P->CY, P->RY};
+    const char *log = std::getenv("USCT_TUNE_LOG");  // print every candidate's time
     for (const Cand &c : cands) {
         P->LB = c.lb; P->CX = c.cx; P->CY = c.cy; P->RY = c.ry;
         const double t = time_tile(P);
+        if (log && *log == '1')
+            std::fprintf(stderr, "[usct tune] lanes %d tile %dx%d rows %d: %.3f ms\n", c.lb, c.cx,
+                         c.cy, c.ry, t);
         if (t < best) { best = t; bc = c; }
     }
     P->LB = bc.lb; P->CX = bc.cx; P->CY = bc.cy; P->RY = bc.ry;
@@ -1138,7 +1243,7 @@ int usct_misfit_and_gradient(usct_plan *P, const double *m, const double *wavele
 
 int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,
                           const double *sponge, const float *dobs, double *fwd_ms, double *adj_ms,
-                          int *n_fwd, int *n_adj, void *stream) {
+                          int *n_fwd, int *n_adj, int *levels_per_launch, void *stream) {
     if (!P) return fail(USCT_EINVAL, "null plan");
     double *Jd = nullptr, *gd = nullptr;
     HIPX(hipMalloc(&Jd, sizeof(double)));
@@ -1149,19 +1254,30 @@ int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,
     Profile pr;
     int rc = run(P, MODE_GRAD, USCT_MISFIT_L2, m, wavelet, sponge, dobs, nullptr, Jd, gd, stream, &pr);
     if (!rc) rc = hipStreamSynchronize(P->stream) == hipSuccess ? 0 : fail(USCT_EHIP, "sync");
-    auto sum = [](std::vector<hipEvent_t> &v, double *ms, int *cnt) {
-        double s = 0;
-        int c = 0;
+    int lv = 1;
+    for (int l : pr.fwd_lv) lv = std::max(lv, l);
+    for (int l : pr.adj_lv) lv = std::max(lv, l);
+    // mean over the launches that advance lv levels; levels counted over all
+    auto sum = [lv](std::vector<hipEvent_t> &v, const std::vector<int> &lvs, double *ms,
+                    int *levels) {
+        double s = 0, s_any = 0;
+        int c = 0, c_any = 0, total = 0;
         for (size_t i = 0; i + 1 < v.size(); i += 2) {
             float t = 0;
-            if (hipEventElapsedTime(&t, v[i], v[i + 1]) == hipSuccess) { s += t; ++c; }
+            const int l = i / 2 < lvs.size() ? lvs[i / 2] : 1;
+            total += l;
+            if (hipEventElapsedTime(&t, v[i], v[i + 1]) != hipSuccess) continue;
+            s_any += t;
+            ++c_any;
+            if (l == lv) { s += t; ++c; }
         }
         for (hipEvent_t e : v) (void)hipEventDestroy(e);
-        if (ms) *ms = c ? s / c : 0.0;
-        if (cnt) *cnt = c;
+        if (ms) *ms = c ? s / c : (c_any ? s_any / c_any : 0.0);
+        if (levels) *levels = total;
     };
-    sum(pr.fwd_ev, fwd_ms, n_fwd);
-    sum(pr.adj_ev, adj_ms, n_adj);
+    sum(pr.fwd_ev, pr.fwd_lv, fwd_ms, n_fwd);
+    sum(pr.adj_ev, pr.adj_lv, adj_ms, n_adj);
+    if (levels_per_launch) *levels_per_launch = lv;
     (void)hipFree(Jd);
     (void)hipFree(gd);
     return rc;

@@ -253,9 +253,9 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     if (x0 >= nx) return;
     const int y0 = yc * RY;
     const int y1 = min(y0 + RY, ny);
-    (void)src_cell;
-    (void)sinc;
     (void)srcd2;
+    const int my_src = src_cell[b];
+    const float my_inc = sinc[b];
     const size_t rowE = (size_t)nx * Bp;
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;  // lane byte offset, float fields
     const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;  // bf16 fields
@@ -283,7 +283,8 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             if (x < nx) {
                 float lu = 0.0f;
                 if (yint && x >= 1 && x <= nx - 2) lu = lap5(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
-                const float inc = smul(ldb(spr + i, oc), lu);
+                float inc = smul(ldb(spr + i, oc), lu);
+                if (y * nx + x == my_src) inc = inc + my_inc;
                 const size_t ei = ro + (size_t)i * Bp;
                 float lo;
                 float v = leap_hilo(rc[i + 1], bf2f(ldb(pcl + ei, o2)), ldb(pp + ei, o4),
@@ -306,6 +307,187 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             rc[0] = ldb(rpn - Bp, o4);
             rc[CX + 1] = ldb(rpn + (ptrdiff_t)CX * Bp, o4);
         }
+    }
+}
+
+// Row index clamped to the grid (halo rows of a wavefront read the edge row).
+__device__ __forceinline__ size_t clamp_row(int y, int ny) { return (size_t)min(max(y, 0), ny - 1); }
+
+// ---------------------------------------------------------------------------
+// Batched 2D forward, two levels per launch (temporal blocking, no sponge):
+// level n (A) from levels n-1 (pc) and n-2 (pp), then level n+1 (B) from A and
+// pc.  A wavefront marches its rows with B one row behind A, so the three A
+// rows around B's row sit in registers.  Each lane computes A on its CX cells
+// plus one halo column on either side, and on one halo row above and below
+// its row range: the same operations on the same operands as the owner of
+// those cells, hence the same values.  Both levels and their traces are
+// bit-identical to two single-level launches; per shot-cell the pair reads
+// 12 bytes and writes 12 instead of reading 24 and writing 12.
+//   sinc_a[b] = S_src w[n-1], sinc_b[b] = S_src w[n];
+//   rec_a / rec_b: trace rows of levels n and n+1.
+// Halo columns read one or two cells beyond the strip; the guard zones of the
+// operands cover the reads before the first and after the last row.
+// ---------------------------------------------------------------------------
+template <int LB, int CX, bool REC>
+__global__ void __launch_bounds__(kBlock)
+k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
+           float *__restrict__ pb, const uint16_t *__restrict__ pcl,
+           const uint16_t *__restrict__ ppl, uint16_t *__restrict__ pal,
+           uint16_t *__restrict__ pbl, const float2 *__restrict__ sp2,
+           const int *__restrict__ src_cell, const float *__restrict__ sinc_a,
+           const float *__restrict__ sinc_b, const int *__restrict__ rec_head,
+           const int *__restrict__ rec_next, const int *__restrict__ row_rec,
+           float *__restrict__ rec_a, float *__restrict__ rec_lo_a, float *__restrict__ rec_b,
+           float *__restrict__ rec_lo_b, int ny, int nx, int Bp, int RY, int nxs, int nyc,
+           int ng) {
+    constexpr int W = kWave / LB;
+    constexpr int CA = CX + 2;  // A columns x0-1 .. x0+CX
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    const int t = wv / ng;
+    const int xs = t % nxs, yc = t / nxs;
+    if (yc >= nyc) return;
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    if (x0 >= nx) return;
+    const int y0 = yc * RY;
+    const int y1 = min(y0 + RY, ny);
+    const int my_src = src_cell[b];
+    const float inc_a = sinc_a[b], inc_b = sinc_b[b];
+    const size_t rowE = (size_t)nx * Bp;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    // pc rows around the front row yf (index j <-> column x0-2+j for uc/un,
+    // x0-1+j for um): um = yf-1, uc = yf, un = yf+1.  un and the low parts /
+    // level n-2 of row yf (cl, ph, pl) are loaded one iteration ahead, so the
+    // loads of the next row are in flight while this row is computed.
+    float um[CA], uc[CA + 2], un[CA + 2];
+    float cl[CA], ph[CA], pl[CA];
+    // A rows yf-2, yf-1, yf (index j <-> column x0-1+j)
+    float am[CA], ac[CA], an[CA];
+    float alo[CX], clo[CX];  // row yf-1, own columns: low parts of A and of pc
+    {
+        const float *r0 = pc + clamp_row(y0 - 2, ny) * rowE;
+        const float *r1 = pc + clamp_row(y0 - 1, ny) * rowE;
+        const float *r2 = pc + clamp_row(y0, ny) * rowE;
+        const size_t e1 = clamp_row(y0 - 1, ny) * rowE;
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            um[j] = ldb(r0 + (ptrdiff_t)(j - 1) * Bp, o4);
+            const ptrdiff_t e = (ptrdiff_t)e1 + (ptrdiff_t)(j - 1) * Bp;
+            cl[j] = bf2f(ldb(pcl + e, o2));
+            ph[j] = ldb(pp + e, o4);
+            pl[j] = bf2f(ldb(ppl + e, o2));
+        }
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) {
+            uc[j] = ldb(r1 + (ptrdiff_t)(j - 2) * Bp, o4);
+            un[j] = ldb(r2 + (ptrdiff_t)(j - 2) * Bp, o4);
+        }
+#pragma unroll
+        for (int j = 0; j < CA; ++j) { am[j] = 0.0f; ac[j] = 0.0f; an[j] = 0.0f; }
+#pragma unroll
+        for (int i = 0; i < CX; ++i) { alo[i] = 0.0f; clo[i] = 0.0f; }
+    }
+    for (int yf = y0 - 1; yf <= y1; ++yf) {
+        const int yr = (int)clamp_row(yf, ny);  // row actually read for halo rows
+        // ---- loads for the next iteration: pc row yf+2, pcl/pp/ppl row yf+1
+        float nn[CA + 2], ncl[CA], nph[CA], npl[CA];
+        {
+            const float *rnn = pc + clamp_row(yf + 2, ny) * rowE;
+            const size_t en = clamp_row(yf + 1, ny) * rowE;
+#pragma unroll
+            for (int j = 0; j < CA + 2; ++j) nn[j] = ldb(rnn + (ptrdiff_t)(j - 2) * Bp, o4);
+#pragma unroll
+            for (int j = 0; j < CA; ++j) {
+                const ptrdiff_t e = (ptrdiff_t)en + (ptrdiff_t)(j - 1) * Bp;
+                ncl[j] = bf2f(ldb(pcl + e, o2));
+                nph[j] = ldb(pp + e, o4);
+                npl[j] = bf2f(ldb(ppl + e, o2));
+            }
+        }
+        // ---- level n on row yf, columns x0-1 .. x0+CX
+        const bool own_a = yf >= y0 && yf < y1;
+        const bool yint = yf >= 1 && yf <= ny - 2;
+        const size_t ro = (size_t)yr * rowE;
+        const float2 *spr = sp2 + (size_t)yr * nx;
+        const bool rec_a_row = REC && own_a && row_rec[yr] > 0;
+        float nlo[CX], ncx[CX];
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            const int x = x0 - 1 + j;
+            float lu = 0.0f;
+            if (yint && x >= 1 && x <= nx - 2) lu = lap5(uc[j + 1], un[j + 1], um[j], uc[j + 2], uc[j]);
+            float inc = smul(ldb(spr + (j - 1), oc), lu);
+            if (yf * nx + x == my_src) inc = inc + inc_a;
+            float lo;
+            const float v = leap_hilo(uc[j + 1], cl[j], ph[j], pl[j], inc, lo);
+            const uint16_t lb = f2bf(lo);
+            am[j] = ac[j];
+            ac[j] = an[j];
+            an[j] = v;
+            if (j >= 1 && j <= CX) {
+                nlo[j - 1] = bf2f(lb);
+                ncx[j - 1] = cl[j];
+                if (own_a && x < nx) {
+                    const ptrdiff_t e = (ptrdiff_t)ro + (ptrdiff_t)(j - 1) * Bp;
+                    stb(pa + e, o4, v);
+                    stb(pal + e, o2, lb);
+                    if (rec_a_row) {
+                        for (int r = rec_head[yr * nx + x]; r >= 0; r = rec_next[r]) {
+                            rec_a[(size_t)r * Bp + b] = v;
+                            rec_lo_a[(size_t)r * Bp + b] = bf2f(lb);
+                        }
+                    }
+                }
+            }
+        }
+        // ---- level n+1 on row y = yf-1, own columns
+        const int y = yf - 1;
+        if (y >= y0 && y < y1) {
+            const bool yint_b = y >= 1 && y <= ny - 2;
+            const size_t rob = (size_t)y * rowE;
+            const float2 *spb = sp2 + (size_t)y * nx;
+            const bool rec_b_row = REC && row_rec[y] > 0;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                if (x < nx) {
+                    float lu = 0.0f;
+                    if (yint_b && x >= 1 && x <= nx - 2)
+                        lu = lap5(ac[i + 1], an[i + 1], am[i + 1], ac[i + 2], ac[i]);
+                    float inc = smul(ldb(spb + i, oc), lu);
+                    if (y * nx + x == my_src) inc = inc + inc_b;
+                    float lo;
+                    const float v = leap_hilo(ac[i + 1], alo[i], um[i + 1], clo[i], inc, lo);
+                    const uint16_t lb = f2bf(lo);
+                    const size_t e = rob + (size_t)i * Bp;
+                    stb(pb + e, o4, v);
+                    stb(pbl + e, o2, lb);
+                    if (rec_b_row) {
+                        for (int r = rec_head[y * nx + x]; r >= 0; r = rec_next[r]) {
+                            rec_b[(size_t)r * Bp + b] = v;
+                            rec_lo_b[(size_t)r * Bp + b] = bf2f(lb);
+                        }
+                    }
+                }
+            }
+        }
+        // ---- advance
+#pragma unroll
+        for (int i = 0; i < CX; ++i) { alo[i] = nlo[i]; clo[i] = ncx[i]; }
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            um[j] = uc[j + 1];
+            cl[j] = ncl[j];
+            ph[j] = nph[j];
+            pl[j] = npl[j];
+        }
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) { uc[j] = un[j]; un[j] = nn[j]; }
     }
 }
 
@@ -343,9 +525,9 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     const int y1 = min(y0 + RY, ny);
     const size_t rowE = (size_t)nx * Bp;
     const size_t N = (size_t)ny * nx;
-    (void)sinc;
+    (void)srcd2;
     const int my_src = src_cell[b];
-    const float my_d2 = srcd2[b];
+    const float my_inc = sinc[b];
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
     const unsigned oc = (unsigned)x0 * 8u;
     // q = S' lam^{k+1} rows (qm, qc with halo, qp); lc: raw centre row of lam^{k+1}.
@@ -412,8 +594,8 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                 } else {
                     const float lp = in ? lap5(fc[i + 1], fp[i], fm[i], fc[i + 2], fc[i]) : 0.0f;
                     d2 = smul(ldb(spr + i, oc), lp);
+                    if (y * nx + x == my_src) d2 = d2 + my_inc;
                 }
-                if (y * nx + x == my_src) d2 = my_d2;
                 tv[i] = v * d2;
             }
         }
@@ -444,6 +626,230 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                 fc[0] = ldb(fnr - Bp, o4);
                 fc[CX + 1] = ldb(fnr + (ptrdiff_t)CX * Bp, o4);
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched 2D adjoint, two levels per launch (temporal blocking, no sponge):
+// lam^k (U) from lam^{k+1} (l1) and lam^{k+2} (l2), then lam^{k-1} (V) from U
+// and l1, each imaged against its own stored forward level:
+//   g += U d2p^k     with d2p^k     = S' lap_u(p^{k-1}) + S_src w[k-1]  (pa1)
+//   g += V d2p^{k-1} with d2p^{k-1} = S' lap_u(p^{k-2}) + S_src w[k-2]  (pa2)
+// V trails U by one row; U is computed on one halo column either side and one
+// halo row above and below the wavefront's rows, with the receiver residuals
+// of those cells, so it equals the owners' values.  Each accumulator cell gets
+// the two Neumaier updates in the order two single-level launches apply them,
+// so fields and gradient are bit-identical to those launches.  Per shot-cell
+// the pair reads 16 bytes and writes 8 instead of reading 24 and writing 8.
+//   sinc_u/sinc_v: S_src w[k-1], S_src w[k-2]; res_u/res_v: residual rows k, k-1.
+// ---------------------------------------------------------------------------
+template <int LB, int CX>
+__global__ void __launch_bounds__(kBlock)
+k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lu,
+           float *__restrict__ lv, const float *__restrict__ pa1, const float *__restrict__ pa2,
+           const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
+           const float *__restrict__ sinc_u, const float *__restrict__ sinc_v,
+           const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+           const int *__restrict__ row_rec, const float *__restrict__ res_u,
+           const float *__restrict__ res_v, float *__restrict__ g_hi, float *__restrict__ g_lo,
+           int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+    constexpr int W = kWave / LB;
+    constexpr int CA = CX + 2;  // U columns x0-1 .. x0+CX
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    const int t = wv / ng;
+    const int xs = t % nxs, yc = t / nxs;
+    if (yc >= nyc) return;  // uniform over the wavefront
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = yc * RY;
+    const int y1 = min(y0 + RY, ny);
+    const size_t rowE = (size_t)nx * Bp;
+    const size_t N = (size_t)ny * nx;
+    const int my_src = src_cell[b];
+    const float inc_u = sinc_u[b], inc_v = sinc_v[b];
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    // Registers at the top of the iteration for front row yf (index j <-> column
+    // x0-2+j for the CA+2-wide rows, x0-1+j for CA-wide, x0+j for CX-wide):
+    //   qm, qc       S' lam^{k+1} rows yf-1 (CA), yf (CA+2)
+    //   lc           lam^{k+1} row yf (CA)
+    //   lr           lam^{k+1} row yf+1 (CA+2), loaded one iteration ahead
+    //   l2c          lam^{k+2} row yf (CA), loaded one iteration ahead
+    //   fm, fc, fr   p^{k-1} rows yf-1 (CX), yf (CX+2), yf+1 (CX+2, ahead)
+    //   hm, hc, hr   p^{k-2} rows yf-2 (CX), yf-1 (CX+2), yf (CX+2, ahead)
+    //   wm, wc, wn   S' U rows yf-2, yf-1, yf-... (CA; shifted as U is made)
+    //   l1p, up      row yf-1, own columns: lam^{k+1} and U
+    float qm[CA], qc[CA + 2], lc[CA], lr[CA + 2], l2c[CA];
+    float fm[CX], fc[CX + 2], fr[CX + 2];
+    float hm[CX], hc[CX + 2], hr[CX + 2];
+    float wm[CA], wc[CA], wn[CA];
+    float l1p[CX], up[CX];
+    float s_prev = 0.0f;  // U imaging sum of row yf-1 (this lane's cell)
+    {
+        const size_t r0 = clamp_row(y0 - 2, ny), r1 = clamp_row(y0 - 1, ny);
+        const size_t r2 = clamp_row(y0, ny);
+        const float *a0 = l1 + r0 * rowE, *a1 = l1 + r1 * rowE, *a2 = l1 + r2 * rowE;
+        const float2 *s0 = sp2 + r0 * nx, *s1 = sp2 + r1 * nx;
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            qm[j] = smul(ldb(s0 + (j - 1), oc), ldb(a0 + (ptrdiff_t)(j - 1) * Bp, o4));
+            l2c[j] = ldb(l2 + r1 * rowE + (ptrdiff_t)(j - 1) * Bp, o4);
+        }
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) {
+            const float v = ldb(a1 + (ptrdiff_t)(j - 2) * Bp, o4);
+            qc[j] = smul(ldb(s1 + (j - 2), oc), v);
+            if (j >= 1 && j <= CA) lc[j - 1] = v;
+            lr[j] = ldb(a2 + (ptrdiff_t)(j - 2) * Bp, o4);
+        }
+        const float *h0 = pa2 + clamp_row(y0 - 3, ny) * rowE;
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            fm[i] = ldb(pa1 + r0 * rowE + (ptrdiff_t)i * Bp, o4);
+            hm[i] = ldb(h0 + (ptrdiff_t)i * Bp, o4);
+        }
+#pragma unroll
+        for (int i = 0; i < CX + 2; ++i) {
+            fc[i] = ldb(pa1 + r1 * rowE + (ptrdiff_t)(i - 1) * Bp, o4);
+            fr[i] = ldb(pa1 + r2 * rowE + (ptrdiff_t)(i - 1) * Bp, o4);
+            hc[i] = ldb(pa2 + r0 * rowE + (ptrdiff_t)(i - 1) * Bp, o4);
+            hr[i] = ldb(pa2 + r1 * rowE + (ptrdiff_t)(i - 1) * Bp, o4);
+        }
+#pragma unroll
+        for (int j = 0; j < CA; ++j) { wm[j] = 0.0f; wc[j] = 0.0f; wn[j] = 0.0f; }
+#pragma unroll
+        for (int i = 0; i < CX; ++i) { l1p[i] = 0.0f; up[i] = 0.0f; }
+    }
+    for (int yf = y0 - 1; yf <= y1; ++yf) {
+        const size_t yr = clamp_row(yf, ny), yn = clamp_row(yf + 1, ny);
+        // ---- loads for the next iteration
+        float ln2[CA + 2], l2n[CA], fn2[CX + 2], hn[CX + 2];
+        {
+            const float *a = l1 + clamp_row(yf + 2, ny) * rowE;
+            const float *c = l2 + yn * rowE;
+            const float *f = pa1 + clamp_row(yf + 2, ny) * rowE;
+            const float *h = pa2 + yn * rowE;
+#pragma unroll
+            for (int j = 0; j < CA + 2; ++j) ln2[j] = ldb(a + (ptrdiff_t)(j - 2) * Bp, o4);
+#pragma unroll
+            for (int j = 0; j < CA; ++j) l2n[j] = ldb(c + (ptrdiff_t)(j - 1) * Bp, o4);
+#pragma unroll
+            for (int i = 0; i < CX + 2; ++i) {
+                fn2[i] = ldb(f + (ptrdiff_t)(i - 1) * Bp, o4);
+                hn[i] = ldb(h + (ptrdiff_t)(i - 1) * Bp, o4);
+            }
+        }
+        // ---- S' lam^{k+1} on row yf+1
+        float qn[CA + 2];
+        {
+            const float2 *snr = sp2 + yn * nx;
+#pragma unroll
+            for (int j = 0; j < CA + 2; ++j) qn[j] = smul(ldb(snr + (j - 2), oc), lr[j]);
+        }
+        // ---- U = lam^k on row yf, columns x0-1 .. x0+CX
+        const bool own_u = yf >= y0 && yf < y1;
+        const bool yint = yf >= 1 && yf <= ny - 2;
+        const bool rec_u = yf >= 0 && yf < ny && row_rec[yr] > 0;
+        const float2 *spr = sp2 + yr * nx;
+        float uo[CX], tv[CX];
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            const int x = x0 - 1 + j;
+            const bool in = yint && x >= 1 && x <= nx - 2;
+            const float lq = in ? lap5(qc[j + 1], qn[j + 1], qm[j], qc[j + 2], qc[j]) : 0.0f;
+            float v = leap(lc[j], l2c[j], lq);
+            if (rec_u && x >= 0 && x < nx) {
+                for (int r = rec_head[(int)yr * nx + x]; r >= 0; r = rec_next[r])
+                    v = v - res_u[(size_t)r * Bp + b];
+            }
+            wm[j] = wc[j];
+            wc[j] = wn[j];
+            wn[j] = smul(ldb(spr + (j - 1), oc), v);
+            if (j >= 1 && j <= CX) {
+                uo[j - 1] = v;
+                if (own_u && x < nx) stb(lu + yr * rowE + (size_t)(j - 1) * Bp, o4, v);
+            }
+        }
+        float s_u = 0.0f;
+        if (own_u) {
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                tv[i] = 0.0f;
+                if (x < nx) {
+                    const bool in = yint && x >= 1 && x <= nx - 2;
+                    const float lp = in ? lap5(fc[i + 1], fr[i + 1], fm[i], fc[i + 2], fc[i]) : 0.0f;
+                    float d2 = smul(ldb(spr + i, oc), lp);
+                    if ((int)yr * nx + x == my_src) d2 = d2 + inc_u;
+                    tv[i] = uo[i] * d2;
+                }
+            }
+            s_u = reduce_scatter<LB, CX>(tv, lane);
+        }
+        // ---- V = lam^{k-1} on row y = yf-1, own columns
+        const int y = yf - 1;
+        if (y >= y0 && y < y1) {
+            const bool yint_v = y >= 1 && y <= ny - 2;
+            const size_t rov = (size_t)y * rowE;
+            const float2 *spv = sp2 + (size_t)y * nx;
+            const bool rec_v = row_rec[y] > 0;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                tv[i] = 0.0f;
+                if (x < nx) {
+                    const bool in = yint_v && x >= 1 && x <= nx - 2;
+                    const float lw = in ? lap5(wc[i + 1], wn[i + 1], wm[i + 1], wc[i + 2], wc[i]) : 0.0f;
+                    float v = leap(up[i], l1p[i], lw);
+                    if (rec_v) {
+                        for (int r = rec_head[y * nx + x]; r >= 0; r = rec_next[r])
+                            v = v - res_v[(size_t)r * Bp + b];
+                    }
+                    stb(lv + rov + (size_t)i * Bp, o4, v);
+                    const float lp = in ? lap5(hc[i + 1], hr[i + 1], hm[i], hc[i + 2], hc[i]) : 0.0f;
+                    float d2 = smul(ldb(spv + i, oc), lp);
+                    if (y * nx + x == my_src) d2 = d2 + inc_v;
+                    tv[i] = v * d2;
+                }
+            }
+            const float s_v = reduce_scatter<LB, CX>(tv, lane);
+            if ((sb & (LB / CX - 1)) == 0) {
+                const int x = x0 + sum_cell<LB, CX>(lane);
+                if (x < nx) {
+                    float *gh = g_hi + (size_t)sg * N + (size_t)y * nx;
+                    float *gl = g_lo + (size_t)sg * N + (size_t)y * nx;
+                    float hi = gh[x], lo = gl[x];
+                    neumaier(hi, lo, s_prev);
+                    neumaier(hi, lo, s_v);
+                    gh[x] = hi;
+                    gl[x] = lo;
+                }
+            }
+        }
+        // ---- advance
+#pragma unroll
+        for (int i = 0; i < CX; ++i) { l1p[i] = lc[i + 1]; up[i] = uo[i]; }
+        s_prev = s_u;
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            qm[j] = qc[j + 1];
+            lc[j] = lr[j + 1];
+            l2c[j] = l2n[j];
+        }
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) { qc[j] = qn[j]; lr[j] = ln2[j]; }
+#pragma unroll
+        for (int i = 0; i < CX; ++i) { fm[i] = fc[i + 1]; hm[i] = hc[i + 1]; }
+#pragma unroll
+        for (int i = 0; i < CX + 2; ++i) {
+            fc[i] = fr[i];
+            fr[i] = fn2[i];
+            hc[i] = hr[i];
+            hr[i] = hn[i];
         }
     }
 }
@@ -758,21 +1164,16 @@ k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *_
         if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
             lu = lap7(pcv, pc[c + sz], pc[c - sz], pc[c + nx], pc[c - nx], pc[c + 1], pc[c - 1]);
     }
-    if (c == src) {
-        if (ndim == 2)
-            source_cell_update<SPONGE, 2, REC>(
-                pc, pcl, pp, ppl, pn, pnl, sp2, sponge, *sinc, (size_t)c, c, 1, (size_t)nx, 0,
-                y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2, rec_head, rec_next, rec, rec_lo, 1,
-                0, srcd2);
-        else
-            source_cell_update<SPONGE, 3, REC>(
+    if (c == src && ndim == 3) {
+        source_cell_update<SPONGE, 3, REC>(
                 pc, pcl, pp, ppl, pn, pnl, sp2, sponge, *sinc, (size_t)c, c, 1, (size_t)nx,
                 (size_t)nx * ny,
                 z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2, rec_head,
                 rec_next, rec, rec_lo, 1, 0, srcd2);
         return;
     }
-    const float inc = smul(sp2[c], lu);
+    float inc = smul(sp2[c], lu);
+    if (c == src) inc = inc + *sinc;  // 2D: source added in FP32 as the batched path does
     float lo;
     float v = leap_hilo(pcv, bf2f(pcl[c]), pp[c], bf2f(ppl[c]), inc, lo);
     if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
@@ -792,10 +1193,10 @@ __global__ void __launch_bounds__(kBlock)
 k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
             const float *__restrict__ pa, const float *__restrict__ pb,
             const float *__restrict__ pc2, const float2 *__restrict__ sp2,
-            const float *__restrict__ sponge, int src, const float *__restrict__ srcd2,
-            const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-            const float *__restrict__ res, float *__restrict__ g_hi, float *__restrict__ g_lo,
-            int nz, int ny, int nx, int ndim) {
+            const float *__restrict__ sponge, int src, const float *__restrict__ sinc,
+            const float *__restrict__ srcd2, const int *__restrict__ rec_head,
+            const int *__restrict__ rec_next, const float *__restrict__ res,
+            float *__restrict__ g_hi, float *__restrict__ g_lo, int nz, int ny, int nx, int ndim) {
     const int64_t N = (int64_t)nz * ny * nx;
     const int64_t c64 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c64 >= N) return;
@@ -827,11 +1228,11 @@ k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *_
     lk[c] = v;
     float d2;
     if (SPONGE) {
-        d2 = (pa[c] - 2.0f * pb[c]) + pc2[c];
+        d2 = (pa[c] - 2.0f * pb[c]) + pc2[c];  // stored levels: the source is in them
     } else {
         d2 = smul(sp2[c], lp);
+        if (c == src) d2 = ndim == 3 ? *srcd2 : d2 + *sinc;
     }
-    if (c == src) d2 = *srcd2;
     float hi = g_hi[c], lo = g_lo[c];
     neumaier(hi, lo, v * d2);
     g_hi[c] = hi;

@@ -45,7 +45,8 @@ class Plan:
     """A reusable solver plan on the current HIP device."""
 
     def __init__(self, shape, h, dt, nt, tx_lin, rec_lin, method="batched", ckpt_every=0,
-                 mem_budget=0, sponge=False, autotune=True, graph=True, forward_only=False):
+                 mem_budget=0, sponge=False, autotune=True, graph=True, forward_only=False,
+                 fuse=True):
         lib = _lib.load()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -58,7 +59,8 @@ class Plan:
         if method not in _METHODS:
             raise ValueError(f"unknown method {method!r}; expected one of {sorted(_METHODS)}")
         flags = (_lib.FLAG_SPONGE if sponge else 0) | (_lib.FLAG_AUTOTUNE if autotune else 0) | \
-            (0 if graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_FORWARD_ONLY if forward_only else 0)
+            (0 if graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_FORWARD_ONLY if forward_only else 0) | \
+            (0 if fuse else _lib.FLAG_NO_FUSE)
         dims = (ctypes.c_int * self.ndim)(*self.shape)
         handle = ctypes.c_void_p()
         torch.cuda.init()
@@ -162,14 +164,17 @@ class Plan:
         return J, g
 
     def profile_gradient(self, m, wavelet, dobs, sponge=None, stream=None):
-        """Mean per-launch kernel time (ms) of forward and adjoint steps of one gradient."""
+        """Kernel timing of one gradient: mean duration (ms) of the forward and of the
+        adjoint launches that advance ``levels_per_launch`` time levels each, and the
+        number of time levels advanced in all (re-computed levels count as forward)."""
         m, w, sp, d = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(dobs)
         fwd, adj = ctypes.c_double(), ctypes.c_double()
-        nf, na = ctypes.c_int(), ctypes.c_int()
+        nf, na, lv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _lib.check(_lib.load().usct_profile_gradient(
             self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d), ctypes.byref(fwd), ctypes.byref(adj),
-            ctypes.byref(nf), ctypes.byref(na), _stream_handle(stream)))
-        return {"fwd_ms": fwd.value, "adj_ms": adj.value, "n_fwd": nf.value, "n_adj": na.value}
+            ctypes.byref(nf), ctypes.byref(na), ctypes.byref(lv), _stream_handle(stream)))
+        return {"fwd_ms": fwd.value, "adj_ms": adj.value, "n_fwd": nf.value, "n_adj": na.value,
+                "levels_per_launch": lv.value}
 
 
 def footprint_bytes(shape, nt, n_tx, n_rec, ckpt_every=0, gradient=True):
@@ -188,9 +193,10 @@ def footprint_bytes(shape, nt, n_tx, n_rec, ckpt_every=0, gradient=True):
     K = min(K, max(nt - 1, 2))
     NS = max(1, -(-(nt - 1) // K))
     field = 4 * N * Bp
-    fields = 4 + ((K + 2) + 2 * max(NS - 2, 0) if gradient else 0)
-    halves = 4 + (2 * max(NS - 2, 0) if gradient else 0)
+    fields = 5 + ((K + 2) + 2 * max(NS - 2, 0) if gradient else 0)
+    halves = 5 + (2 * max(NS - 2, 0) if gradient else 0)
     traces = 4 * nt * n_rec * Bp
     b = fields * field + halves * (field // 2) + 4 * traces
+    b += (4 * fields + 2 * halves) * 2 * 64 * Bp  # guard zones
     b += 8 * N + 4 * N + 8 * N * max(Bp // 16, 1) + 8 * nt * Bp + 24 * n_rec * Bp + 8 * N
     return int(b + (1 << 20))

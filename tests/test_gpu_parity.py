@@ -188,6 +188,29 @@ def test_checkpointed_gradient_equals_whole_history():
             assert torch.equal(g0, g1), K
 
 
+@pytest.mark.parametrize("nt,K", [(350, 0), (349, 0), (300, 7), (301, 8), (3, 0), (2, 0)])
+def test_two_level_launches_bit_identical(nt, K):
+    """Two time levels per launch (forward and adjoint) against one level per
+    launch with the same tiles: traces, misfit and gradient equal bit for bit,
+    with odd remainders in the forward pass, the checkpoint windows and the
+    adjoint sweep, and the shortest time axes."""
+    from openusct_amd.plan import Plan
+    f, h, dt, _, tx, rec, m0, mt, wd = _plan_problem(n_src=24, nt=nt)
+    kw = dict(ckpt_every=K, autotune=False)
+    with Plan(m0.shape, h, dt, nt, tx, rec, fuse=False, **kw) as one:
+        info = one.info
+        print(f"[case] two-level vs one-level nt={nt} K={K} {info}", flush=True)
+        dobs = one.forward_fmc(mt, wd)
+        J1, g1 = one.misfit_and_gradient(m0, wd, dobs)
+    with Plan(m0.shape, h, dt, nt, tx, rec, fuse=True, **kw) as two:
+        assert two.info == info
+        d2 = two.forward_fmc(mt, wd)
+        J2, g2 = two.misfit_and_gradient(m0, wd, dobs)
+    assert torch.equal(d2, dobs)
+    assert torch.equal(J1, J2)
+    assert torch.equal(g1, g2)
+
+
 def test_graph_replays_match_eager():
     from openusct_amd.plan import Plan
     f, h, dt, nt, tx, rec, m0, mt, wd = _plan_problem(n_src=8)
