@@ -4,8 +4,9 @@
  *
  * All array arguments marked "device" are device pointers in HBM (any
  * allocator: torch, hipMalloc, ...).  Index lists and dims are host arrays.
- * Fields are FP32; models, wavelets, gradients and misfits are FP64 at the
- * boundary, like the reference.  Every entry point returns 0 on success or a
+ * Fields are FP32 on the device; models, wavelets, traces (synthetic and
+ * observed), gradients and misfits are FP64 at the boundary, like the
+ * reference's core (software/core/libuap.hpp:109-116, 169-173).  Every entry point returns 0 on success or a
  * usct_status code, with a message in usct_last_error() (thread-local).
  * Work is ordered after prior work on `stream` (a hipStream_t, NULL = the null
  * stream) and later work on `stream` sees the results.
@@ -63,15 +64,19 @@ typedef struct usct_plan usct_plan;
 
 typedef struct {
     int method;            /* USCT_METHOD_PLAIN or USCT_METHOD_BATCHED */
-    int lanes_per_group;   /* shots per wavefront lane group (16, 32, 64); 1 for plain */
-    int tile_x, tile_y;    /* cells per lane along x (and y in 3D) */
-    int rows_per_wave;     /* y rows (2D) or z planes (3D) marched by one wavefront */
+    int lanes_per_group;   /* adjoint kernel: shots per wavefront lane group (16, 32, 64);
+                              1 for plain */
+    int tile_x, tile_y;    /* adjoint kernel: cells per lane along x (and y in 3D) */
+    int rows_per_wave;     /* adjoint kernel: y rows (2D) or z planes (3D) per wavefront */
     int shots_padded;      /* shot count rounded up to a lane-group multiple */
     int shot_passes;       /* batched: passes over shot subsets that fit the budget */
     int shots_per_pass;    /* batched: padded shots per pass */
     int ckpt_every;        /* checkpoint interval K (>= nt-1: whole history stored) */
     int n_windows;         /* recompute windows per pass */
     int64_t workspace_bytes;  /* device memory held by the plan */
+    int fwd_lanes_per_group;  /* the same four for the forward kernel */
+    int fwd_tile_x, fwd_tile_y;
+    int fwd_rows_per_wave;
 } usct_plan_info;
 
 /*
@@ -93,20 +98,20 @@ int usct_plan_get_info(const usct_plan *plan, usct_plan_info *info);
  *   m        device, N doubles, squared slowness
  *   wavelet  device, nt doubles
  *   sponge   device, N doubles or NULL (plan must have USCT_FLAG_SPONGE if not NULL)
- *   data     device, n_tx * nt * n_rec floats (row 0 of each shot is zero)
+ *   data     device, n_tx * nt * n_rec doubles (row 0 of each shot is zero)
  */
 int usct_forward_fmc(usct_plan *plan, const double *m, const double *wavelet,
-                     const double *sponge, float *data, void *stream);
+                     const double *sponge, double *data, void *stream);
 
-/* Misfit only (fwi.py:246-263).  dobs: device n_tx*nt*n_rec floats; J: device double. */
+/* Misfit only (fwi.py:246-263).  dobs: device n_tx*nt*n_rec doubles; J: device double. */
 int usct_misfit(usct_plan *plan, const double *m, const double *wavelet,
-                const double *sponge, const float *dobs, int misfit_type, double *J,
+                const double *sponge, const double *dobs, int misfit_type, double *J,
                 void *stream);
 
 /* Misfit and exact discrete-adjoint gradient w.r.t. m (fwi.py:266-332).
  * grad: device N doubles; J: device double. */
 int usct_misfit_and_gradient(usct_plan *plan, const double *m, const double *wavelet,
-                             const double *sponge, const float *dobs, int misfit_type,
+                             const double *sponge, const double *dobs, int misfit_type,
                              double *J, double *grad, void *stream);
 
 /*
@@ -118,7 +123,7 @@ int usct_misfit_and_gradient(usct_plan *plan, const double *m, const double *wav
  * n_fwd / n_adj: time levels advanced in all (re-computed levels count as forward).
  */
 int usct_profile_gradient(usct_plan *plan, const double *m, const double *wavelet,
-                          const double *sponge, const float *dobs, double *fwd_ms,
+                          const double *sponge, const double *dobs, double *fwd_ms,
                           double *adj_ms, int *n_fwd, int *n_adj, int *levels_per_launch,
                           void *stream);
 
@@ -126,12 +131,12 @@ int usct_profile_gradient(usct_plan *plan, const double *m, const double *wavele
  * uap::misfit_and_gradient (device pointers as above, J returned on the host). */
 int usct_forward_fmc_direct(const double *m, const int *dims, int ndim, double h, double dt,
                             int nt, const int64_t *tx_lin, int n_tx, const int64_t *rec_lin,
-                            int n_rec, const double *wavelet, float *data, int method,
+                            int n_rec, const double *wavelet, double *data, int method,
                             void *stream);
 int usct_misfit_and_gradient_direct(const double *m, const int *dims, int ndim, double h,
                                     double dt, int nt, const int64_t *tx_lin, int n_tx,
                                     const int64_t *rec_lin, int n_rec, const double *wavelet,
-                                    const float *dobs, double *grad, double *J_host,
+                                    const double *dobs, double *grad, double *J_host,
                                     int misfit_type, int method, void *stream);
 
 const char *usct_last_error(void);

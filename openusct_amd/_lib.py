@@ -40,6 +40,10 @@ class PlanInfo(ctypes.Structure):
         ("ckpt_every", ctypes.c_int),
         ("n_windows", ctypes.c_int),
         ("workspace_bytes", ctypes.c_int64),
+        ("fwd_lanes_per_group", ctypes.c_int),
+        ("fwd_tile_x", ctypes.c_int),
+        ("fwd_tile_y", ctypes.c_int),
+        ("fwd_rows_per_wave", ctypes.c_int),
     ]
 
     def as_dict(self):

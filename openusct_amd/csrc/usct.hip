@@ -71,6 +71,10 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2 };
 
+struct Tiling {
+    int LB, CX, CY, RY;  // lanes per shot group, cells per lane (x, y), rows per wavefront
+};
+
 struct Profile {
     std::vector<hipEvent_t> fwd_ev, adj_ev;  // start/stop pairs
     std::vector<int> fwd_lv, adj_lv;         // time levels advanced by each timed launch
@@ -88,7 +92,10 @@ struct usct_plan {
     int method = USCT_METHOD_BATCHED, flags = 0;
     bool sponge = false;
     // kernel configuration
-    int LB = 64, CX = 8, CY = 1, RY = 32;
+    // Tiles of the forward and of the adjoint kernels (chosen independently: the
+    // field layout does not depend on them; the gradient accumulators use the
+    // adjoint's lane groups).
+    Tiling tf{64, 8, 1, 32}, ta{64, 8, 1, 32};
     int Bp = 1;        // padded shots per pass
     int passes = 1;
     int K = 0, NS = 1;  // checkpoint interval and number of windows
@@ -102,7 +109,8 @@ struct usct_plan {
     int *row_rec = nullptr;  // receivers per grid row (2D: ny rows, 3D: nz*ny rows)
     int *src_all = nullptr;  // [passes][Bp] source cells (-1 = padding)
     int *src_cell = nullptr; // [Bp] cells of the current pass
-    float *sinc = nullptr;   // [nt][Bp]
+    float *sinc = nullptr;   // [nt][Bp] S_src w[n-1], FP32 (2D kernels)
+    double *sincd = nullptr; // [nt][Bp] the same in FP64 (3D kernels)
     float *srcd2 = nullptr;  // [nt][Bp] recorded increment at each shot's source cell
     float *Z = nullptr;      // zero field
     float *F[4] = {nullptr, nullptr, nullptr, nullptr};  // rotation (4: fused two-level kernels)
@@ -111,7 +119,8 @@ struct usct_plan {
     uint16_t *Lo[4] = {nullptr, nullptr, nullptr, nullptr};
     uint16_t *Zlo = nullptr;
     std::vector<uint16_t *> CKlo;
-    float *recb = nullptr, *recb_lo = nullptr, *dobsT = nullptr, *res = nullptr;
+    float *recb = nullptr, *recb_lo = nullptr, *res = nullptr;
+    float *dobsT = nullptr, *dobsT_lo = nullptr;  // observed traces, FP32 hi + lo
     float2 *part = nullptr;
     int npart = 1024;
     float *g_hi = nullptr, *g_lo = nullptr;
@@ -125,7 +134,7 @@ struct usct_plan {
     std::map<int, hipGraphExec_t> graphs;  // key: mode*4 + misfit
 
     int64_t field_elems() const { return N * (int64_t)Bp; }
-    int ng() const { return Bp / LB; }
+    int ng_adj() const { return Bp / ta.LB; }  // shot groups of the gradient accumulators
 };
 
 namespace {
@@ -188,18 +197,19 @@ struct Tile {
     int64_t waves;
 };
 
-Tile tile_of(const usct_plan *P) {
+Tile tile_of(const usct_plan *P, const Tiling &T) {
     Tile t{};
-    const int W = kWave / P->LB;
-    t.nxs = (int)cdiv(P->nx, (int64_t)W * P->CX);
+    const int W = kWave / T.LB;
+    const int64_t ng = P->Bp / T.LB;
+    t.nxs = (int)cdiv(P->nx, (int64_t)W * T.CX);
     if (P->ndim == 2) {
         t.nys = 1;
-        t.nyc = (int)cdiv(P->ny, P->RY);
-        t.waves = (int64_t)P->ng() * t.nxs * t.nyc;
+        t.nyc = (int)cdiv(P->ny, T.RY);
+        t.waves = ng * t.nxs * t.nyc;
     } else {
-        t.nys = (int)cdiv(P->ny, P->CY);
-        t.nyc = (int)cdiv(P->nz, P->RY);
-        t.waves = (int64_t)P->ng() * t.nxs * t.nys * t.nyc;
+        t.nys = (int)cdiv(P->ny, T.CY);
+        t.nyc = (int)cdiv(P->nz, T.RY);
+        t.waves = ng * t.nxs * t.nys * t.nyc;
     }
     return t;
 }
@@ -213,24 +223,24 @@ struct Lvl {
 template <int LB, int CX, bool SP, bool REC>
 void fwd2d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
              const float *sinc, float *rec, float *rec_lo, const Tile &t) {
-    float *d2 = P->srcd2 + (sinc - P->sinc);
+    const double *sd = P->sincd + (sinc - P->sinc);
     hipLaunchKernelGGL((k_fwd2d<LB, CX, SP, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pn.hi,
-                       pc.lo, pp.lo, pn.lo, P->sp2, P->spg, P->src_cell, sinc, P->rec_head,
-                       P->rec_next, P->row_rec, rec, rec_lo, d2, P->ny, P->nx, P->Bp, P->RY,
-                       t.nxs, t.nyc, P->ng());
+                       pc.lo, pp.lo, pn.lo, P->sp2, P->spg, P->src_cell, sd, P->rec_head,
+                       P->rec_next, P->row_rec, rec, rec_lo, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs,
+                       t.nyc, P->Bp / P->tf.LB);
 }
 
 template <int LB, bool SP, bool REC>
 void fwd2d_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
               const float *sinc, float *rec, float *rec_lo, const Tile &t) {
-    if (P->CX == 4) fwd2d_t<LB, 4, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rec_lo, t);
+    if (P->tf.CX == 4) fwd2d_t<LB, 4, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rec_lo, t);
     else fwd2d_t<LB, 8, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rec_lo, t);
 }
 
 template <bool SP, bool REC>
 void fwd2d_sel(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
                const float *sinc, float *rec, float *rl, const Tile &t) {
-    switch (P->LB) {
+    switch (P->tf.LB) {
         case 16: fwd2d_lb<16, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
         case 32: fwd2d_lb<32, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
         default: fwd2d_lb<64, SP, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
@@ -241,24 +251,25 @@ template <int LB, int CX, int CY, bool REC>
 void fwd3d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
              const float *sinc, float *rec, float *rl, const Tile &t) {
     float *d2 = P->srcd2 + (sinc - P->sinc);
+    const double *sd = P->sincd + (sinc - P->sinc);
     hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pn.hi,
-                       pc.lo, pp.lo, pn.lo, P->sp2, P->src_cell, sinc, P->rec_head, P->rec_next,
-                       P->row_rec, rec, rl, d2, P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys,
-                       t.nyc, P->ng());
+                       pc.lo, pp.lo, pn.lo, P->sp2, P->src_cell, sd, P->rec_head, P->rec_next,
+                       P->row_rec, rec, rl, d2, P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys,
+                       t.nyc, P->Bp / P->tf.LB);
 }
 
 template <int LB, bool REC>
 void fwd3d_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
               const float *sinc, float *rec, float *rl, const Tile &t) {
-    if (P->CX == 4 && P->CY == 4) fwd3d_t<LB, 4, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
-    else if (P->CX == 8 && P->CY == 2) fwd3d_t<LB, 8, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    if (P->tf.CX == 4 && P->tf.CY == 4) fwd3d_t<LB, 4, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else if (P->tf.CX == 8 && P->tf.CY == 2) fwd3d_t<LB, 8, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     else fwd3d_t<LB, 4, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
 }
 
 template <bool REC>
 void fwd3d_sel(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
                const float *sinc, float *rec, float *rl, const Tile &t) {
-    switch (P->LB) {
+    switch (P->tf.LB) {
         case 16: fwd3d_lb<16, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
         case 32: fwd3d_lb<32, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
         default: fwd3d_lb<64, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t); break;
@@ -267,7 +278,7 @@ void fwd3d_sel(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P
 
 void launch_fwd_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, int n,
                         bool record) {
-    const Tile t = tile_of(P);
+    const Tile t = tile_of(P, P->tf);
     const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
     const float *sinc = P->sinc + (int64_t)n * P->Bp;
     float *rec = P->recb + (int64_t)n * P->n_rec * P->Bp;
@@ -284,47 +295,36 @@ void launch_fwd_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl p
         if (record) fwd3d_sel<true>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
         else fwd3d_sel<false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     }
-    // FP64 update of each shot's source cell, overwriting what the stream kernel wrote
-    float *d2 = P->srcd2 + (sinc - P->sinc);
-    const dim3 gs((unsigned)cdiv(P->Bp, 64)), bs(64);
-#define USCT_FIX(SP, ND, RC)                                                                   \
-    hipLaunchKernelGGL((k_src_fix<SP, ND, RC>), gs, bs, 0, s, pc.hi, pc.lo, pp.hi, pp.lo, pn.hi, \
-                       pn.lo, P->sp2, P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, rec, \
-                       rl, d2, P->nz, P->ny, P->nx, P->Bp)
-    if (P->ndim == 3) {  // 2D adds the source in FP32 inside the stencil kernel
-        if (record) USCT_FIX(false, 3, true); else USCT_FIX(false, 3, false);
-    }
-#undef USCT_FIX
 }
 
 // Two forward levels per launch (2D, no sponge): levels n and n+1.
 template <int LB, int CX, bool REC>
 void fwd2d_x2_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_plan *P,
                 int n, const Tile &t) {
-    const float *sa = P->sinc + (int64_t)n * P->Bp;
+    const double *sa = P->sincd + (int64_t)n * P->Bp;
     const int64_t tr = (int64_t)P->n_rec * P->Bp;
     float *ra = P->recb + (int64_t)n * tr, *rla = P->recb_lo + (int64_t)n * tr;
     hipLaunchKernelGGL((k_fwd2d_x2<LB, CX, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pa.hi,
                        pb.hi, pc.lo, pp.lo, pa.lo, pb.lo, P->sp2, P->src_cell, sa, sa + P->Bp,
                        P->rec_head, P->rec_next, P->row_rec, ra, rla, ra + tr, rla + tr, P->ny,
-                       P->nx, P->Bp, P->RY, t.nxs, t.nyc, P->ng());
+                       P->nx, P->Bp, P->tf.RY, t.nxs, t.nyc, P->Bp / P->tf.LB);
 }
 
 template <int LB, bool REC>
 void fwd2d_x2_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_plan *P,
                  int n, const Tile &t) {
-    if (P->CX == 4) fwd2d_x2_t<LB, 4, REC>(g, s, pc, pp, pa, pb, P, n, t);
+    if (P->tf.CX == 4) fwd2d_x2_t<LB, 4, REC>(g, s, pc, pp, pa, pb, P, n, t);
     else fwd2d_x2_t<LB, 8, REC>(g, s, pc, pp, pa, pb, P, n, t);
 }
 
 void launch_fwd2_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb,
                          int n, bool record) {
-    const Tile t = tile_of(P);
+    const Tile t = tile_of(P, P->tf);
     const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
 #define USCT_X2(LB_)                                                          \
     if (record) fwd2d_x2_lb<LB_, true>(g, s, pc, pp, pa, pb, P, n, t);        \
     else fwd2d_x2_lb<LB_, false>(g, s, pc, pp, pa, pb, P, n, t);
-    switch (P->LB) {
+    switch (P->tf.LB) {
         case 16: USCT_X2(16) break;
         case 32: USCT_X2(32) break;
         default: USCT_X2(64) break;
@@ -345,15 +345,15 @@ void adj2d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
     const float *d2 = P->srcd2 + (sinc - P->sinc);
     hipLaunchKernelGGL((k_adj2d<LB, CX, SP>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, pb, pc2,
                        P->sp2, P->spg, P->src_cell, sinc, P->rec_head, P->rec_next, P->row_rec,
-                       res, d2, P->g_hi, P->g_lo, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nyc,
-                       P->ng());
+                       res, d2, P->g_hi, P->g_lo, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nyc,
+                       P->ng_adj());
 }
 
 template <int LB, bool SP>
 void adj2d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
               const float *pa, const float *pb, const float *pc2, const usct_plan *P,
               const float *sinc, const float *res, const Tile &t) {
-    if (P->CX == 4) adj2d_t<LB, 4, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
+    if (P->ta.CX == 4) adj2d_t<LB, 4, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
     else adj2d_t<LB, 8, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
 }
 
@@ -361,7 +361,7 @@ template <bool SP>
 void adj2d_sel(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
                const float *pa, const float *pb, const float *pc2, const usct_plan *P,
                const float *sinc, const float *res, const Tile &t) {
-    switch (P->LB) {
+    switch (P->ta.LB) {
         case 16: adj2d_lb<16, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
         case 32: adj2d_lb<32, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
         default: adj2d_lb<64, SP>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t); break;
@@ -377,18 +377,18 @@ void adj2d_x2_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *
     const float *ru = P->res + (int64_t)k * tr;
     hipLaunchKernelGGL((k_adj2d_x2<LB, CX>), g, dim3(kBlock), 0, s, l1, l2, lu, lv, pa1, pa2,
                        P->sp2, P->src_cell, su, su - P->Bp, P->rec_head, P->rec_next, P->row_rec,
-                       ru, ru - tr, P->g_hi, P->g_lo, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nyc,
-                       P->ng());
+                       ru, ru - tr, P->g_hi, P->g_lo, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nyc,
+                       P->ng_adj());
 }
 
 void launch_adj2_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
                          float *lu, float *lv, const float *pa1, const float *pa2, int k) {
-    const Tile t = tile_of(P);
+    const Tile t = tile_of(P, P->ta);
     const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
 #define USCT_A2(LB_)                                                                        \
-    if (P->CX == 4) adj2d_x2_t<LB_, 4>(g, s, l1, l2, lu, lv, pa1, pa2, P, k, t);            \
+    if (P->ta.CX == 4) adj2d_x2_t<LB_, 4>(g, s, l1, l2, lu, lv, pa1, pa2, P, k, t);            \
     else adj2d_x2_t<LB_, 8>(g, s, l1, l2, lu, lv, pa1, pa2, P, k, t);
-    switch (P->LB) {
+    switch (P->ta.LB) {
         case 16: USCT_A2(16) break;
         case 32: USCT_A2(32) break;
         default: USCT_A2(64) break;
@@ -403,21 +403,21 @@ void adj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
     const float *d2 = P->srcd2 + (sinc - P->sinc);
     hipLaunchKernelGGL((k_adj3d<LB, CX, CY>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, P->sp2,
                        P->src_cell, sinc, P->rec_head, P->rec_next, P->row_rec, res, d2, P->g_hi,
-                       P->g_lo, P->nz, P->ny, P->nx, P->Bp, P->RY, t.nxs, t.nys, t.nyc, P->ng());
+                       P->g_lo, P->nz, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nys, t.nyc, P->ng_adj());
 }
 
 template <int LB>
 void adj3d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
               const float *pa, const usct_plan *P, const float *sinc, const float *res,
               const Tile &t) {
-    if (P->CX == 4 && P->CY == 4) adj3d_t<LB, 4, 4>(g, s, l1, l2, lk, pa, P, sinc, res, t);
-    else if (P->CX == 8 && P->CY == 2) adj3d_t<LB, 8, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    if (P->ta.CX == 4 && P->ta.CY == 4) adj3d_t<LB, 4, 4>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else if (P->ta.CX == 8 && P->ta.CY == 2) adj3d_t<LB, 8, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
     else adj3d_t<LB, 4, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
 }
 
 void launch_adj_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
                         float *lk, const float *pa, const float *pb, const float *pc2, int k) {
-    const Tile t = tile_of(P);
+    const Tile t = tile_of(P, P->ta);
     const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
     const float *sinc = P->sinc + (int64_t)k * P->Bp;
     const float *res = P->res + (int64_t)k * P->n_rec * P->Bp;
@@ -425,7 +425,7 @@ void launch_adj_batched(const usct_plan *P, hipStream_t s, const float *l1, cons
         if (P->sponge) adj2d_sel<true>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
         else adj2d_sel<false>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
     } else {
-        switch (P->LB) {
+        switch (P->ta.LB) {
             case 16: adj3d_lb<16>(g, s, l1, l2, lk, pa, P, sinc, res, t); break;
             case 32: adj3d_lb<32>(g, s, l1, l2, lk, pa, P, sinc, res, t); break;
             default: adj3d_lb<64>(g, s, l1, l2, lk, pa, P, sinc, res, t); break;
@@ -507,17 +507,18 @@ void launch_residual(usct_plan *P, hipStream_t s, int misfit, bool want_res) {
     if (misfit == USCT_MISFIT_L2) {
         const int nb = (int)std::min<int64_t>(P->npart, grid_for(count));
         hipLaunchKernelGGL(k_residual_l2, dim3(nb), dim3(kBlock), 0, s, P->recb, P->recb_lo,
-                           P->dobsT, want_res ? P->res : nullptr, count, P->part);
+                           P->dobsT, P->dobsT_lo, want_res ? P->res : nullptr, count, P->part);
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, P->part, nb, 0.5,
                            P->J_acc, 1);
     } else {
         hipLaunchKernelGGL(k_gcn_stats, dim3(grid_for(ntr)), dim3(256), 0, s, P->recb, P->recb_lo,
-                           P->dobsT, P->gstats, P->nt, ntr);
+                           P->dobsT, P->dobsT_lo, P->gstats, P->nt, ntr);
         hipLaunchKernelGGL(k_gcn_misfit, dim3(1), dim3(64), 0, s, P->gstats, P->tmask_cur, ntr,
                            P->J_acc, 1);
         if (want_res)
             hipLaunchKernelGGL(k_gcn_residual, dim3(grid_for(count)), dim3(256), 0, s, P->recb,
-                               P->recb_lo, P->dobsT, P->gstats, P->tmask_cur, P->res, P->nt, ntr);
+                               P->recb_lo, P->dobsT, P->dobsT_lo, P->gstats, P->tmask_cur,
+                               P->res, P->nt, ntr);
     }
 }
 
@@ -617,7 +618,7 @@ int run_pass_sequence(usct_plan *P, int mode, int misfit, Profile *pr) {
 // Batched driver
 // ---------------------------------------------------------------------------
 int run_batched(usct_plan *P, int mode, int misfit, const double *m, const double *w,
-                const double *sponge, const float *dobs, float *data, double *J, double *grad,
+                const double *sponge, const double *dobs, double *data, double *J, double *grad,
                 Profile *pr) {
     hipStream_t s = P->stream;
     const int64_t N = P->N;
@@ -627,8 +628,8 @@ int run_batched(usct_plan *P, int mode, int misfit, const double *m, const doubl
         hipLaunchKernelGGL(k_d2f, dim3(grid_for(N)), dim3(256), 0, s, sponge, P->spg, N);
     if (J) HIPX(hipMemsetAsync(P->J_acc, 0, sizeof(double), s));
     if (grad) {
-        HIPX(hipMemsetAsync(P->g_hi, 0, sizeof(float) * N * P->ng(), s));
-        HIPX(hipMemsetAsync(P->g_lo, 0, sizeof(float) * N * P->ng(), s));
+        HIPX(hipMemsetAsync(P->g_hi, 0, sizeof(float) * N * P->ng_adj(), s));
+        HIPX(hipMemsetAsync(P->g_lo, 0, sizeof(float) * N * P->ng_adj(), s));
     }
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
     for (int p = 0; p < P->passes; ++p) {
@@ -639,20 +640,21 @@ int run_batched(usct_plan *P, int mode, int misfit, const double *m, const doubl
         if (misfit == USCT_MISFIT_GCN)
             HIPX(hipMemcpyAsync(P->tmask_cur, P->tmask + p * ntr, ntr, hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL(k_setup_sinc, dim3(grid_for((int64_t)P->nt * P->Bp)), dim3(256), 0, s,
-                           m, w, P->src_cell, P->sinc, P->nt, P->Bp, P->dt * P->dt);
+                           m, w, P->src_cell, P->sinc, P->sincd, P->nt, P->Bp, P->dt * P->dt);
+        const int64_t NJ = (int64_t)P->nt * P->n_rec;
+        const dim3 gt((unsigned)cdiv(NJ, kTT), (unsigned)cdiv(P->Bp, kTT));
         if (dobs)
-            hipLaunchKernelGGL(k_dobs_to_batched, dim3(grid_for((int64_t)P->nt * ntr)), dim3(256),
-                               0, s, dobs, P->dobsT, P->nt, P->n_rec, P->Bp, b0, nb);
+            hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT,
+                               P->dobsT_lo, NJ, P->Bp, b0, nb);
         TRY(check_launch());
         TRY(run_pass_sequence(P, mode, misfit, pr));
         if (data)
-            hipLaunchKernelGGL(k_batched_to_data, dim3(grid_for((int64_t)nb * P->nt * P->n_rec)),
-                               dim3(256), 0, s, P->recb, P->recb_lo, data, P->nt, P->n_rec, P->Bp,
-                               b0, nb);
+            hipLaunchKernelGGL(k_batched_to_data, gt, dim3(256), 0, s, P->recb, P->recb_lo, data,
+                               NJ, P->Bp, b0, nb);
     }
     if (grad)
         hipLaunchKernelGGL(k_finalize_grad, dim3(grid_for(N)), dim3(256), 0, s, P->g_hi, P->g_lo,
-                           P->ng(), N, m, grad, 0);
+                           P->ng_adj(), N, m, grad, 0);
     if (J) HIPX(hipMemcpyAsync(J, P->J_acc, sizeof(double), hipMemcpyDeviceToDevice, s));
     return check_launch();
 }
@@ -661,7 +663,7 @@ int run_batched(usct_plan *P, int mode, int misfit, const double *m, const doubl
 // Plain driver: one shot at a time, one launch per time step, full history.
 // ---------------------------------------------------------------------------
 int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double *w,
-              const double *sponge, const float *dobs, float *data, double *J, double *grad,
+              const double *sponge, const double *dobs, double *data, double *J, double *grad,
               Profile *pr) {
     hipStream_t s = P->stream;
     const int64_t N = P->N;
@@ -688,7 +690,7 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
         const int src = (int)P->tx[b];
         // S_src * w[n-1] per level, same FP64 product as the batched path.
         hipLaunchKernelGGL(k_setup_sinc, dim3(grid_for(nt)), dim3(256), 0, s, m, w,
-                           P->src_all + b, P->sinc, nt, 1, P->dt * P->dt);
+                           P->src_all + b, P->sinc, P->sincd, nt, 1, P->dt * P->dt);
         HIPX(hipMemsetAsync(P->recb, 0, sizeof(float) * nr, s));
         HIPX(hipMemsetAsync(P->recb_lo, 0, sizeof(float) * nr, s));
         for (int n = 1; n < nt; ++n) {
@@ -696,13 +698,13 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
             if (P->sponge)
                 hipLaunchKernelGGL((k_fwd_plain<true, true>), dim3(gN), dim3(256), 0, s, lvl(n - 1),
                                    lvl(n - 2), lvl(n), llo(n - 1), llo(n - 2), llo(n), P->sp2,
-                                   P->spg, src, P->sinc + n, P->rec_head, P->rec_next,
+                                   P->spg, src, P->sincd + n, P->rec_head, P->rec_next,
                                    P->recb + (int64_t)n * nr, P->recb_lo + (int64_t)n * nr,
                                    P->srcd2 + n, P->nz, P->ny, P->nx, nd);
             else
                 hipLaunchKernelGGL((k_fwd_plain<false, true>), dim3(gN), dim3(256), 0, s,
                                    lvl(n - 1), lvl(n - 2), lvl(n), llo(n - 1), llo(n - 2), llo(n),
-                                   P->sp2, P->spg, src, P->sinc + n, P->rec_head, P->rec_next,
+                                   P->sp2, P->spg, src, P->sincd + n, P->rec_head, P->rec_next,
                                    P->recb + (int64_t)n * nr, P->recb_lo + (int64_t)n * nr,
                                    P->srcd2 + n, P->nz, P->ny, P->nx, nd);
             if (pr) {
@@ -710,13 +712,15 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
                 pr->fwd_lv.push_back(1);
             }
         }
+        const int64_t NJ = (int64_t)nt * nr;
+        const dim3 gt((unsigned)cdiv(NJ, kTT), 1);
         if (data)
-            hipLaunchKernelGGL(k_batched_to_data, dim3(grid_for((int64_t)nt * nr)), dim3(256), 0, s,
-                               P->recb, P->recb_lo, data, nt, nr, 1, b, 1);
+            hipLaunchKernelGGL(k_batched_to_data, gt, dim3(256), 0, s, P->recb, P->recb_lo, data,
+                               NJ, 1, b, 1);
         if (mode == MODE_FWD) continue;
-        // residual of this shot against dobs[b] (layout [n][j] already)
-        HIPX(hipMemcpyAsync(P->dobsT, dobs + (int64_t)b * nt * nr, sizeof(float) * nt * nr,
-                            hipMemcpyDeviceToDevice, s));
+        // residual of this shot against dobs[b] (FP64 -> FP32 hi + lo)
+        hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT, P->dobsT_lo,
+                           NJ, 1, b, 1);
         if (misfit == USCT_MISFIT_GCN)
             HIPX(hipMemcpyAsync(P->tmask_cur, P->tmask, nr, hipMemcpyDeviceToDevice, s));
         launch_residual(P, s, misfit, store);
@@ -770,7 +774,7 @@ int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
         fields += (K + 2) + 2 * std::max(NS - 2, 0);
         halves += 2 * std::max(NS - 2, 0);
     }
-    int64_t b = fields * field + halves * (field / 2) + 4 * traces;
+    int64_t b = fields * field + halves * (field / 2) + 5 * traces;  // rec hi/lo, dobs hi/lo, res
     b += (fields * 4 + halves * 2) * 2 * kGuardCells * Bp;
     b += P->N * 8 /*sp2*/ + P->N * 4 /*sponge*/ + 2 * P->N * 4 * (Bp / 16) /*g*/;
     b += (int64_t)P->nt * Bp * 4 + (int64_t)P->n_rec * Bp * 24 + (int64_t)P->N * 4 + (1 << 20);
@@ -786,7 +790,7 @@ int choose_batched(usct_plan *P, int ckpt_every) {
         const int64_t pad = cdiv(B, c) * c;
         if ((pad - B) * 8 <= pad) break;
     }
-    P->LB = lb;
+    P->tf.LB = P->ta.LB = lb;
     const int nt = P->nt;
     const int64_t total_groups = cdiv(B, lb);
     MemPlan best{0, 0, 0, 0, 0, 1e300};
@@ -830,16 +834,16 @@ int choose_batched(usct_plan *P, int ckpt_every) {
 // Rows (2D) or planes (3D) per wavefront: the largest power of two in [4, 64]
 // that still gives about 8 wavefronts per SIMD (256 CUs x 4 SIMDs), so the
 // streaming loads of enough wavefronts are in flight to cover HBM latency.
-void default_rows(usct_plan *P) {
-    const int W = kWave / P->LB;
-    const int64_t nxs = cdiv(P->nx, (int64_t)W * P->CX);
-    const int64_t cols = P->ndim == 2 ? (int64_t)P->ng() * nxs
-                                      : (int64_t)P->ng() * nxs * cdiv(P->ny, P->CY);
+void default_rows(const usct_plan *P, Tiling &T) {
+    const int W = kWave / T.LB;
+    const int64_t ng = P->Bp / T.LB;
+    const int64_t nxs = cdiv(P->nx, (int64_t)W * T.CX);
+    const int64_t cols = P->ndim == 2 ? ng * nxs : ng * nxs * cdiv(P->ny, T.CY);
     const int64_t depth = P->ndim == 2 ? P->ny : P->nz;
     const int64_t target = 256 * 4 * 8;
     int r = 64;
     while (r > 4 && cols * cdiv(depth, r) < target) r >>= 1;
-    P->RY = r;
+    T.RY = r;
 }
 
 int allocate_batched(usct_plan *P) {
@@ -863,11 +867,13 @@ int allocate_batched(usct_plan *P) {
     TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * ntr));
     TRY(dalloc_t(P, &P->recb_lo, (int64_t)P->nt * ntr));
     TRY(dalloc_t(P, &P->dobsT, (int64_t)P->nt * ntr));
+    TRY(dalloc_t(P, &P->dobsT_lo, (int64_t)P->nt * ntr));
     TRY(dalloc_t(P, &P->res, (int64_t)P->nt * ntr));
     // sized for the narrowest lane group the autotuner may pick (16 lanes)
     TRY(dalloc_t(P, &P->g_hi, P->N * std::max(P->Bp / 16, 1)));
     TRY(dalloc_t(P, &P->g_lo, P->N * std::max(P->Bp / 16, 1)));
     TRY(dalloc_t(P, &P->sinc, (int64_t)P->nt * P->Bp));
+    TRY(dalloc_t(P, &P->sincd, (int64_t)P->nt * P->Bp));
     TRY(dalloc_t(P, &P->srcd2, (int64_t)P->nt * P->Bp));
     TRY(dalloc_t(P, &P->src_cell, P->Bp));
     TRY(dalloc_t(P, &P->gstats, 3 * ntr));
@@ -908,10 +914,12 @@ int allocate_plain(usct_plan *P) {
     TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * nr));
     TRY(dalloc_t(P, &P->recb_lo, (int64_t)P->nt * nr));
     TRY(dalloc_t(P, &P->dobsT, (int64_t)P->nt * nr));
+    TRY(dalloc_t(P, &P->dobsT_lo, (int64_t)P->nt * nr));
     TRY(dalloc_t(P, &P->res, (int64_t)P->nt * nr));
     TRY(dalloc_t(P, &P->g_hi, N));
     TRY(dalloc_t(P, &P->g_lo, N));
     TRY(dalloc_t(P, &P->sinc, P->nt));
+    TRY(dalloc_t(P, &P->sincd, P->nt));
     TRY(dalloc_t(P, &P->srcd2, P->nt));
     TRY(dalloc_t(P, &P->gstats, 3 * (int64_t)nr));
     TRY(dalloc_t(P, &P->tmask, nr));
@@ -926,26 +934,27 @@ int allocate_plain(usct_plan *P) {
     return 0;
 }
 
-// Time a few forward and adjoint launches of the current tile choice.
-double time_tile(usct_plan *P) {
+// Time a few forward (adjoint = false) or adjoint launches of the current
+// tile choice of that kernel, rotating over the plan's buffers.
+double time_tile(usct_plan *P, bool adjoint) {
     hipEvent_t a, b;
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     const int reps = 4;
-    const bool adj = !P->Wslot.empty();
     const bool x2 = fused(P) && P->nt >= 3;  // levels 1 and 2 exist
-    // one forward launch (one or two levels) and one adjoint launch, rotating buffers
     auto step = [&](int r) {
         Lvl L[4];
         for (int i = 0; i < 4; ++i) L[i] = {P->F[(r + i) % 4], P->Lo[(r + i) % 4]};
-        if (x2) launch_fwd2_batched(P, P->stream, L[1], L[0], L[2], L[3], 1, false);
-        else launch_fwd_batched(P, P->stream, L[1], L[0], L[2], 1, false);
-        if (adj && x2)
+        if (!adjoint) {
+            if (x2) launch_fwd2_batched(P, P->stream, L[1], L[0], L[2], L[3], 1, false);
+            else launch_fwd_batched(P, P->stream, L[1], L[0], L[2], 1, false);
+        } else if (x2) {
             launch_adj2_batched(P, P->stream, L[0].hi, L[1].hi, L[2].hi, L[3].hi, P->Wslot[0],
                                 P->Wslot[0], 2);
-        else if (adj)
+        } else {
             launch_adj_batched(P, P->stream, L[0].hi, L[1].hi, L[2].hi, P->Wslot[0],
                                P->Wslot[0], P->Wslot[0], 1);
+        }
     };
     step(0);  // warm-up
     (void)hipEventRecord(a, P->stream);
@@ -960,23 +969,25 @@ double time_tile(usct_plan *P) {
     return ms;
 }
 
-// Autotuner: tile shape (rows per wavefront; 3D patch) and lane-group width
-// among those that keep the same padded shot count.  Results are kept per
-// (ndim, dims, padded shots, sponge) for the life of the process.
+// Autotuner: lane-group width (among those that divide the padded shot count),
+// cells per lane and rows per wavefront, for the forward and the adjoint kernel
+// separately.  Results are kept per (ndim, dims, padded shots, sponge) for the
+// life of the process.
 struct TuneKey {
     int ndim, nz, ny, nx, Bp, sponge;
     bool operator<(const TuneKey &o) const {
         return std::tie(ndim, nz, ny, nx, Bp, sponge) < std::tie(o.ndim, o.nz, o.ny, o.nx, o.Bp, o.sponge);
     }
 };
-struct TuneVal { int lb, cx, cy, ry; };
+struct TuneVal { Tiling tf, ta; };
 std::map<TuneKey, TuneVal> g_tuned;
 
 int autotune(usct_plan *P) {
     const TuneKey key{P->ndim, P->nz, P->ny, P->nx, P->Bp, P->sponge ? 1 : 0};
     auto hit = g_tuned.find(key);
     if (hit != g_tuned.end()) {
-        P->LB = hit->second.lb; P->CX = hit->second.cx; P->CY = hit->second.cy; P->RY = hit->second.ry;
+        P->tf = hit->second.tf;
+        P->ta = hit->second.ta;
         return 0;
     }
     // zero the fields used by the timing launches
@@ -986,6 +997,7 @@ int autotune(usct_plan *P) {
     }
     if (!P->Wslot.empty()) HIPX(hipMemset(P->Wslot[0], 0, sizeof(float) * P->field_elems()));
     HIPX(hipMemset(P->sinc, 0, sizeof(float) * P->nt * P->Bp));
+    HIPX(hipMemset(P->sincd, 0, sizeof(double) * P->nt * P->Bp));
     HIPX(hipMemset(P->sp2, 0, sizeof(float2) * P->N));
     HIPX(hipMemset(P->g_hi, 0, sizeof(float) * P->N * std::max(P->Bp / 16, 1)));
     HIPX(hipMemset(P->g_lo, 0, sizeof(float) * P->N * std::max(P->Bp / 16, 1)));
@@ -994,33 +1006,37 @@ int autotune(usct_plan *P) {
     std::vector<int> lbs;
     for (int c : {64, 32, 16})
         if (P->Bp % c == 0) lbs.push_back(c);
-    struct Cand { int lb, cx, cy, ry; };
-    std::vector<Cand> cands;
-    if (P->ndim == 2) {
-        for (int lb : lbs)
-            for (int cx : {4, 8})
-                for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry});
-    } else {
-        for (int lb : lbs)
-            for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}})
-                for (int rz : {4, 8, 16, 32, 64}) {
-                    if (pc.first * pc.second > lb) continue;
-                    cands.push_back({lb, pc.first, pc.second, rz});
-                }
-    }
-    double best = 1e300;
-    Cand bc{P->LB, P->CX, P->CY, P->RY};
     const char *log = std::getenv("USCT_TUNE_LOG");  // print every candidate's time
-    for (const Cand &c : cands) {
-        P->LB = c.lb; P->CX = c.cx; P->CY = c.cy; P->RY = c.ry;
-        const double t = time_tile(P);
-        if (log && *log == '1')
-            std::fprintf(stderr, "[usct tune] lanes %d tile %dx%d rows %d: %.3f ms\n", c.lb, c.cx,
-                         c.cy, c.ry, t);
-        if (t < best) { best = t; bc = c; }
+    for (int adjoint = 0; adjoint < 2; ++adjoint) {
+        if (adjoint && P->Wslot.empty()) break;  // forward-only plan
+        std::vector<Tiling> cands;
+        if (P->ndim == 2) {
+            for (int lb : lbs)
+                for (int cx : {4, 8})
+                    for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry});
+        } else {
+            for (int lb : lbs)
+                for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}})
+                    for (int rz : {4, 8, 16, 32, 64}) {
+                        // the adjoint reduces a lane's CX*CY cells over its lane group
+                        if (adjoint && pc.first * pc.second > lb) continue;
+                        cands.push_back({lb, pc.first, pc.second, rz});
+                    }
+        }
+        Tiling &T = adjoint ? P->ta : P->tf;
+        double best = 1e300;
+        Tiling bt = T;
+        for (const Tiling &c : cands) {
+            T = c;
+            const double t = time_tile(P, adjoint != 0);
+            if (log && *log == '1')
+                std::fprintf(stderr, "[usct tune] %s lanes %d tile %dx%d rows %d: %.3f ms\n",
+                             adjoint ? "adjoint" : "forward", c.LB, c.CX, c.CY, c.RY, t);
+            if (t < best) { best = t; bt = c; }
+        }
+        T = bt;
     }
-    P->LB = bc.lb; P->CX = bc.cx; P->CY = bc.cy; P->RY = bc.ry;
-    g_tuned[key] = TuneVal{bc.lb, bc.cx, bc.cy, bc.ry};
+    g_tuned[key] = TuneVal{P->tf, P->ta};
     return 0;
 }
 
@@ -1089,7 +1105,7 @@ int leave(usct_plan *P, void *stream) {
 }
 
 int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
-        const double *sponge, const float *dobs, float *data, double *J, double *grad,
+        const double *sponge, const double *dobs, double *data, double *J, double *grad,
         void *stream, Profile *pr) {
     if (!P) return fail(USCT_EINVAL, "null plan");
     if (!m || !w) return fail(USCT_EINVAL, "m and wavelet are required");
@@ -1146,7 +1162,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     P->method = method == USCT_METHOD_PLAIN ? USCT_METHOD_PLAIN : USCT_METHOD_BATCHED;
     P->flags = flags;
     P->sponge = (flags & USCT_FLAG_SPONGE) != 0;
-    if (ndim == 3) { P->CX = 4; P->CY = 4; P->RY = 32; }
+    if (ndim == 3) P->tf = P->ta = Tiling{64, 4, 4, 32};
     int rc = 0;
     auto bail = [&](int code) {
         free_plan_memory(P);
@@ -1170,7 +1186,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     if ((rc = dalloc_t(P, &P->J_acc, 1))) return bail(rc);
     if ((rc = dalloc_t(P, &P->part, P->npart))) return bail(rc);
     if (P->method == USCT_METHOD_PLAIN) {
-        P->LB = 1; P->CX = 1; P->CY = 1; P->RY = 1;
+        P->tf = P->ta = Tiling{1, 1, 1, 1};
         P->Bp = 1;
         P->passes = n_tx;
         P->K = nt;
@@ -1184,7 +1200,8 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
             if ((rc = choose_batched(P, ckpt_every))) return bail(rc);
             P->budget += P->bytes;
             if ((rc = allocate_batched(P))) return bail(rc);
-            default_rows(P);
+            default_rows(P, P->tf);
+            default_rows(P, P->ta);
             if ((flags & USCT_FLAG_AUTOTUNE) && (rc = autotune(P))) return bail(rc);
         }
     }
@@ -1206,10 +1223,14 @@ int usct_plan_destroy(usct_plan *P) {
 int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
     if (!P || !info) return fail(USCT_EINVAL, "null argument");
     info->method = P->method;
-    info->lanes_per_group = P->LB;
-    info->tile_x = P->CX;
-    info->tile_y = P->CY;
-    info->rows_per_wave = P->RY;
+    info->lanes_per_group = P->ta.LB;
+    info->tile_x = P->ta.CX;
+    info->tile_y = P->ta.CY;
+    info->rows_per_wave = P->ta.RY;
+    info->fwd_lanes_per_group = P->tf.LB;
+    info->fwd_tile_x = P->tf.CX;
+    info->fwd_tile_y = P->tf.CY;
+    info->fwd_rows_per_wave = P->tf.RY;
     info->shots_padded = P->method == USCT_METHOD_PLAIN ? P->n_tx : P->Bp * P->passes;
     info->shot_passes = P->passes;
     info->shots_per_pass = P->Bp;
@@ -1220,21 +1241,21 @@ int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
 }
 
 int usct_forward_fmc(usct_plan *P, const double *m, const double *wavelet, const double *sponge,
-                     float *data, void *stream) {
+                     double *data, void *stream) {
     if (!data) return fail(USCT_EINVAL, "data is required");
     return run(P, MODE_FWD, USCT_MISFIT_L2, m, wavelet, sponge, nullptr, data, nullptr, nullptr,
                stream, nullptr);
 }
 
 int usct_misfit(usct_plan *P, const double *m, const double *wavelet, const double *sponge,
-                const float *dobs, int misfit_type, double *J, void *stream) {
+                const double *dobs, int misfit_type, double *J, void *stream) {
     if (!J) return fail(USCT_EINVAL, "J is required");
     return run(P, MODE_MISFIT, misfit_type, m, wavelet, sponge, dobs, nullptr, J, nullptr, stream,
                nullptr);
 }
 
 int usct_misfit_and_gradient(usct_plan *P, const double *m, const double *wavelet,
-                             const double *sponge, const float *dobs, int misfit_type, double *J,
+                             const double *sponge, const double *dobs, int misfit_type, double *J,
                              double *grad, void *stream) {
     if (!J || !grad) return fail(USCT_EINVAL, "J and grad are required");
     return run(P, MODE_GRAD, misfit_type, m, wavelet, sponge, dobs, nullptr, J, grad, stream,
@@ -1242,7 +1263,7 @@ int usct_misfit_and_gradient(usct_plan *P, const double *m, const double *wavele
 }
 
 int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,
-                          const double *sponge, const float *dobs, double *fwd_ms, double *adj_ms,
+                          const double *sponge, const double *dobs, double *fwd_ms, double *adj_ms,
                           int *n_fwd, int *n_adj, int *levels_per_launch, void *stream) {
     if (!P) return fail(USCT_EINVAL, "null plan");
     double *Jd = nullptr, *gd = nullptr;
@@ -1285,7 +1306,7 @@ int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,
 
 int usct_forward_fmc_direct(const double *m, const int *dims, int ndim, double h, double dt, int nt,
                             const int64_t *tx_lin, int n_tx, const int64_t *rec_lin, int n_rec,
-                            const double *wavelet, float *data, int method, void *stream) {
+                            const double *wavelet, double *data, int method, void *stream) {
     usct_plan *P = nullptr;
     TRY(usct_plan_create(&P, ndim, dims, h, dt, nt, tx_lin, n_tx, rec_lin, n_rec, method, 0, 0,
                          USCT_FLAG_NO_GRAPH));
@@ -1299,7 +1320,7 @@ int usct_forward_fmc_direct(const double *m, const int *dims, int ndim, double h
 int usct_misfit_and_gradient_direct(const double *m, const int *dims, int ndim, double h,
                                     double dt, int nt, const int64_t *tx_lin, int n_tx,
                                     const int64_t *rec_lin, int n_rec, const double *wavelet,
-                                    const float *dobs, double *grad, double *J_host,
+                                    const double *dobs, double *grad, double *J_host,
                                     int misfit_type, int method, void *stream) {
     if (!J_host) return fail(USCT_EINVAL, "J_host is required");
     usct_plan *P = nullptr;

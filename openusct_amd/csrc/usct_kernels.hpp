@@ -26,6 +26,9 @@
 //             makes this exactly S' lap_u(p^{k-1}) + S_src w[k-1], which reads one
 //             stored level instead of three and avoids a cancellation
 //             (fwi.py:325-326);  g += lam^k d2p^k with Neumaier compensation.
+// The forward state is stored as FP32 hi + bfloat16 lo.  2D updates it with
+// FP32 TwoSum arithmetic; 3D forms the whole update in FP64 (see k_fwd3d).
+// Adjoint fields are FP32.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -84,88 +87,13 @@ __device__ __forceinline__ float leap(float cur, float prev, float inc) {
 // Forward state p = hi + lo with lo kept in bfloat16 (round to nearest even):
 // the FP32 rounding of p at every step, integrated over thousands of steps, is
 // what separates FP32 traces from FP64 ones; carrying it in 16 extra bits
-// removes most of that drift.
+// removes most of that drift (the update itself is FP64, see lap5d below).
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
-
-// Knuth TwoSum: s + e == a + b exactly.
-__device__ __forceinline__ float two_sum(float a, float b, float &e) {
-    const float s = a + b;
-    const float bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-    return s;
-}
-
-// New level from (ph + pl) at n-1, (qh + ql) at n-2 and the FP32 increment:
-// hi + lo = 2 p^{n-1} - p^{n-2} + inc, optionally times the sponge.
-__device__ __forceinline__ float leap_hilo(float ph, float pl, float qh, float ql, float inc,
-                                           float &lo) {
-    float e;
-    const float s = two_sum(2.0f * ph, -qh, e);
-    const float t = (e + (2.0f * pl - ql)) + inc;
-    return two_sum(s, t, lo);
-}
-
-__device__ __forceinline__ float scale_hilo(float hi, float lo, float w, float &lo_out) {
-    const float h2 = hi * w;
-    const float err = fmaf(hi, w, -h2);
-    return two_sum(h2, fmaf(lo, w, err), lo_out);
-}
-
-// At a source cell the field is a sharp peak: its FP32 Laplacian and the
-// source term nearly cancel.  There the increment is formed in FP64 from the
-// full hi + lo state and recorded, so the adjoint images with the increment
-// the forward actually applied.
-__device__ __forceinline__ double hl(float h, uint16_t l) { return (double)h + (double)bf2f(l); }
-
-// Re-do the update of one source cell (address a, cell c) in FP64 from the
-// hi + lo state, overwrite its new level and trace samples, and record the
-// increment.  Kept out of line so the streaming loops stay unchanged.
-// st[0..2] are the x, y, z strides in elements (z unused in 2D).
-template <bool SPONGE, int NDIM, bool REC>
-__device__ __forceinline__ void source_cell_update(
-    const float *__restrict__ pc, const uint16_t *__restrict__ pcl, const float *__restrict__ pp,
-    const uint16_t *__restrict__ ppl, float *__restrict__ pn, uint16_t *__restrict__ pnl,
-    const float2 *__restrict__ sp2, const float *__restrict__ sponge, float inc_src, size_t a,
-    int c, size_t sx, size_t sy, size_t sz, bool interior, const int *__restrict__ rec_head,
-    const int *__restrict__ rec_next, float *__restrict__ rec, float *__restrict__ rec_lo,
-    size_t BP, int b, float *__restrict__ srcd2) {
-    const float ph = pc[a];
-    const uint16_t pl = pcl[a];
-    double lapd = 0.0;
-    if (interior) {
-        lapd = -2.0 * NDIM * hl(ph, pl);
-        if (NDIM == 3) {
-            lapd += hl(pc[a + sz], pcl[a + sz]);
-            lapd += hl(pc[a - sz], pcl[a - sz]);
-        }
-        lapd += hl(pc[a + sy], pcl[a + sy]);
-        lapd += hl(pc[a - sy], pcl[a - sy]);
-        lapd += hl(pc[a + sx], pcl[a + sx]);
-        lapd += hl(pc[a - sx], pcl[a - sx]);
-    }
-    const float2 s2 = sp2[c];
-    const float inc = (float)(((double)s2.x + (double)s2.y) * lapd + (double)inc_src);
-    const float qh = pp[a];
-    const uint16_t ql = ppl[a];
-    float lo;
-    float v = leap_hilo(ph, bf2f(pl), qh, bf2f(ql), inc, lo);
-    if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
-    const uint16_t lb = f2bf(lo);
-    pn[a] = v;
-    pnl[a] = lb;
-    srcd2[b] = (float)((hl(v, lb) - 2.0 * hl(ph, pl)) + hl(qh, ql));
-    if (REC) {
-        for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
-            rec[(size_t)j * BP + b] = v;
-            rec_lo[(size_t)j * BP + b] = bf2f(lb);
-        }
-    }
-}
 
 __device__ __forceinline__ void neumaier(float &s, float &c, float x) {
     const float t = s + x;
@@ -218,10 +146,51 @@ __device__ __forceinline__ void stb(T *base, unsigned boff, T v) {
 }
 
 // ---------------------------------------------------------------------------
+// Forward update in FP64 (2D and 3D).  The state is stored as FP32 hi +
+// bfloat16 lo (6 bytes); the update reads the full P = hi + lo of every
+// stencil point, forms lap_u(P), S' lap_u(P), the source term S_src w[n-1]
+// and 2P^{n-1} - P^{n-2} + increment in FP64, and splits the result back into
+// hi + lo.  The gradient sums lam^k d2p^k over many levels with strong
+// cancellation: an increment rounded to FP32 at every step (or a Laplacian of
+// the high parts only, or an FP32 source term) leaves a wavefield error of a
+// few 1e-7 that becomes a gradient error of 1e-5 .. 1e-4 on coarse or
+// high-contrast grids; with the FP64 update the gradient is within ~1e-6 of
+// the FP64 reference.  CDNA FP64 vector arithmetic runs at the rate of
+// (unpacked) FP32, and the storage, hence the HBM traffic, is unchanged.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lap5d(double c, double yp, double ym, double xp, double xm) {
+    double a = -4.0 * c;
+    a = a + yp;
+    a = a + ym;
+    a = a + xp;
+    a = a + xm;
+    return a;
+}
+__device__ __forceinline__ double lap7d(double c, double zp, double zm, double yp, double ym,
+                                        double xp, double xm) {
+    double a = -6.0 * c;
+    a = a + zp;
+    a = a + zm;
+    a = a + yp;
+    a = a + ym;
+    a = a + xp;
+    a = a + xm;
+    return a;
+}
+__device__ __forceinline__ double hl(float h, uint16_t l) { return (double)h + (double)bf2f(l); }
+__device__ __forceinline__ double s2d(float2 s) { return (double)s.x + (double)s.y; }
+// Split an FP64 value into the stored FP32 hi and bfloat16 lo.
+__device__ __forceinline__ float split_hl(double v, uint16_t &lo) {
+    const float h = (float)v;
+    lo = f2bf((float)(v - (double)h));
+    return h;
+}
+
+// ---------------------------------------------------------------------------
 // Batched 2D forward step: level n from levels n-1 (pc) and n-2 (pp) into pn.
-// sinc[b] = S_src(b) * w[n-1] (FP64 product rounded once); rec: [n_rec][Bp] row
-// of level n; row_rec[y] = receivers on grid row y.  Grid: ng * nxs * nyc
-// wavefronts, 4 per block.
+// sincd[b] = S_src(b) * w[n-1] in FP64; rec: [n_rec][Bp] row of level n;
+// row_rec[y] = receivers on grid row y.  Grid: ng * nxs * nyc wavefronts, 4
+// per block.  Sponge: the new level is multiplied by the mask (fwi.py:68).
 //
 // Addressing: each lane keeps one 32-bit byte offset, that of its strip's first
 // cell and shot; row starts (clamped to the grid, wave-uniform) and the column
@@ -236,10 +205,10 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
         uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
         const float *__restrict__ sponge, const int *__restrict__ src_cell,
-        const float *__restrict__ sinc, const int *__restrict__ rec_head,
+        const double *__restrict__ sincd, const int *__restrict__ rec_head,
         const int *__restrict__ rec_next, const int *__restrict__ row_rec,
-        float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int ny,
-        int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+        float *__restrict__ rec, float *__restrict__ rec_lo, int ny, int nx, int Bp, int RY,
+        int nxs, int nyc, int ng) {
     constexpr int W = kWave / LB;
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
@@ -253,26 +222,25 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     if (x0 >= nx) return;
     const int y0 = yc * RY;
     const int y1 = min(y0 + RY, ny);
-    (void)srcd2;
     const int my_src = src_cell[b];
-    const float my_inc = sinc[b];
+    const double my_inc = sincd[b];
     const size_t rowE = (size_t)nx * Bp;
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;  // lane byte offset, float fields
     const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;  // bf16 fields
     const unsigned oc = (unsigned)x0 * 8u;             // S' (float2 per cell)
-    float rm[CX], rc[CX + 2], rp[CX];
+    auto P = [&](size_t e) -> double { return hl(ldb(pc + e, o4), ldb(pcl + e, o2)); };
+    double rm[CX], rc[CX + 2], rp[CX];
     {
-        const float *r0 = pc + (size_t)max(y0 - 1, 0) * rowE;
-        const float *r1 = pc + (size_t)y0 * rowE;
+        const size_t r0 = (size_t)max(y0 - 1, 0) * rowE, r1 = (size_t)y0 * rowE;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) rm[i] = ldb(r0 + (ptrdiff_t)i * Bp, o4);
+        for (int i = 0; i < CX; ++i) rm[i] = P(r0 + (ptrdiff_t)i * Bp);
 #pragma unroll
-        for (int i = 0; i < CX + 2; ++i) rc[i] = ldb(r1 + (ptrdiff_t)(i - 1) * Bp, o4);
+        for (int i = 0; i < CX + 2; ++i) rc[i] = P(r1 + (ptrdiff_t)(i - 1) * Bp);
     }
     for (int y = y0; y < y1; ++y) {
-        const float *rpn = pc + (size_t)min(y + 1, ny - 1) * rowE;
+        const size_t rn = (size_t)min(y + 1, ny - 1) * rowE;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) rp[i] = ldb(rpn + (ptrdiff_t)i * Bp, o4);
+        for (int i = 0; i < CX; ++i) rp[i] = P(rn + (ptrdiff_t)i * Bp);
         const bool yint = (y >= 1) && (y <= ny - 2);
         const size_t ro = (size_t)y * rowE;
         const float2 *spr = sp2 + (size_t)y * nx;
@@ -281,16 +249,15 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         for (int i = 0; i < CX; ++i) {
             const int x = x0 + i;
             if (x < nx) {
-                float lu = 0.0f;
-                if (yint && x >= 1 && x <= nx - 2) lu = lap5(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
-                float inc = smul(ldb(spr + i, oc), lu);
+                double lu = 0.0;
+                if (yint && x >= 1 && x <= nx - 2) lu = lap5d(rc[i + 1], rp[i], rm[i], rc[i + 2], rc[i]);
+                double inc = s2d(ldb(spr + i, oc)) * lu;
                 if (y * nx + x == my_src) inc = inc + my_inc;
                 const size_t ei = ro + (size_t)i * Bp;
-                float lo;
-                float v = leap_hilo(rc[i + 1], bf2f(ldb(pcl + ei, o2)), ldb(pp + ei, o4),
-                                    bf2f(ldb(ppl + ei, o2)), inc, lo);
-                if (SPONGE) v = scale_hilo(v, lo, sponge[(size_t)y * nx + x], lo);
-                const uint16_t lb = f2bf(lo);
+                double nv = (2.0 * rc[i + 1] - hl(ldb(pp + ei, o4), ldb(ppl + ei, o2))) + inc;
+                if (SPONGE) nv = nv * (double)sponge[(size_t)y * nx + x];
+                uint16_t lb;
+                const float v = split_hl(nv, lb);
                 stb(pn + ei, o4, v);
                 stb(pnl + ei, o2, lb);
                 if (has_rec) {
@@ -304,38 +271,40 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         if (y + 1 < y1) {
 #pragma unroll
             for (int i = 0; i < CX; ++i) { rm[i] = rc[i + 1]; rc[i + 1] = rp[i]; }
-            rc[0] = ldb(rpn - Bp, o4);
-            rc[CX + 1] = ldb(rpn + (ptrdiff_t)CX * Bp, o4);
+            rc[0] = P(rn - (ptrdiff_t)Bp);
+            rc[CX + 1] = P(rn + (ptrdiff_t)CX * Bp);
         }
     }
 }
 
-// Row index clamped to the grid (halo rows of a wavefront read the edge row).
-__device__ __forceinline__ size_t clamp_row(int y, int ny) { return (size_t)min(max(y, 0), ny - 1); }
-
 // ---------------------------------------------------------------------------
 // Batched 2D forward, two levels per launch (temporal blocking, no sponge):
 // level n (A) from levels n-1 (pc) and n-2 (pp), then level n+1 (B) from A and
-// pc.  A wavefront marches its rows with B one row behind A, so the three A
-// rows around B's row sit in registers.  Each lane computes A on its CX cells
-// plus one halo column on either side, and on one halo row above and below
-// its row range: the same operations on the same operands as the owner of
-// those cells, hence the same values.  Both levels and their traces are
-// bit-identical to two single-level launches; per shot-cell the pair reads
-// 12 bytes and writes 12 instead of reading 24 and writing 12.
-//   sinc_a[b] = S_src w[n-1], sinc_b[b] = S_src w[n];
+// pc, both in FP64 as k_fwd2d.  A wavefront marches its rows with B one row
+// behind A, so the three A rows around B's row sit in registers (as the values
+// A's stored hi + lo represent, which is what a separate launch would read).
+// Each lane computes A on its CX cells plus one halo column on either side,
+// and on one halo row above and below its row range: the same operations on
+// the same operands as the owner of those cells, hence the same values.  Both
+// levels and their traces are bit-identical to two single-level launches; per
+// shot-cell the pair reads 12 bytes and writes 12 instead of reading 24 and
+// writing 12.  The next row's loads are issued one iteration ahead.
+//   sincd_a[b] = S_src w[n-1], sincd_b[b] = S_src w[n] (FP64);
 //   rec_a / rec_b: trace rows of levels n and n+1.
 // Halo columns read one or two cells beyond the strip; the guard zones of the
 // operands cover the reads before the first and after the last row.
 // ---------------------------------------------------------------------------
+// Row index clamped to the grid (halo rows of a wavefront read the edge row).
+__device__ __forceinline__ size_t clamp_row(int y, int ny) { return (size_t)min(max(y, 0), ny - 1); }
+
 template <int LB, int CX, bool REC>
 __global__ void __launch_bounds__(kBlock)
 k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
            float *__restrict__ pb, const uint16_t *__restrict__ pcl,
            const uint16_t *__restrict__ ppl, uint16_t *__restrict__ pal,
            uint16_t *__restrict__ pbl, const float2 *__restrict__ sp2,
-           const int *__restrict__ src_cell, const float *__restrict__ sinc_a,
-           const float *__restrict__ sinc_b, const int *__restrict__ rec_head,
+           const int *__restrict__ src_cell, const double *__restrict__ sincd_a,
+           const double *__restrict__ sincd_b, const int *__restrict__ rec_head,
            const int *__restrict__ rec_next, const int *__restrict__ row_rec,
            float *__restrict__ rec_a, float *__restrict__ rec_lo_a, float *__restrict__ rec_b,
            float *__restrict__ rec_lo_b, int ny, int nx, int Bp, int RY, int nxs, int nyc,
@@ -355,58 +324,62 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
     const int y0 = yc * RY;
     const int y1 = min(y0 + RY, ny);
     const int my_src = src_cell[b];
-    const float inc_a = sinc_a[b], inc_b = sinc_b[b];
+    const double inc_a = sincd_a[b], inc_b = sincd_b[b];
     const size_t rowE = (size_t)nx * Bp;
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
     const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
     const unsigned oc = (unsigned)x0 * 8u;
-    // pc rows around the front row yf (index j <-> column x0-2+j for uc/un,
-    // x0-1+j for um): um = yf-1, uc = yf, un = yf+1.  un and the low parts /
-    // level n-2 of row yf (cl, ph, pl) are loaded one iteration ahead, so the
-    // loads of the next row are in flight while this row is computed.
-    float um[CA], uc[CA + 2], un[CA + 2];
-    float cl[CA], ph[CA], pl[CA];
-    // A rows yf-2, yf-1, yf (index j <-> column x0-1+j)
-    float am[CA], ac[CA], an[CA];
-    float alo[CX], clo[CX];  // row yf-1, own columns: low parts of A and of pc
+    // Full state (FP64) around the front row yf (index j <-> column x0-2+j for
+    // uc/un, x0-1+j for um): um = pc row yf-1, uc = yf, un = yf+1.  un and q
+    // (level n-2 on row yf, columns x0-1 ..) come from raw loads issued one
+    // iteration ahead (nh/nl, qh/ql).
+    double um[CA], uc[CA + 2], un[CA + 2];
+    float nh[CA + 2], qh[CA];
+    uint16_t nl[CA + 2], ql[CA];
+    // A rows yf-2, yf-1, yf (index j <-> column x0-1+j), as stored
+    double am[CA], ac[CA], an[CA];
     {
-        const float *r0 = pc + clamp_row(y0 - 2, ny) * rowE;
-        const float *r1 = pc + clamp_row(y0 - 1, ny) * rowE;
-        const float *r2 = pc + clamp_row(y0, ny) * rowE;
-        const size_t e1 = clamp_row(y0 - 1, ny) * rowE;
+        const size_t r0 = clamp_row(y0 - 2, ny) * rowE, r1 = clamp_row(y0 - 1, ny) * rowE;
+        const size_t r2 = clamp_row(y0, ny) * rowE;
 #pragma unroll
         for (int j = 0; j < CA; ++j) {
-            um[j] = ldb(r0 + (ptrdiff_t)(j - 1) * Bp, o4);
-            const ptrdiff_t e = (ptrdiff_t)e1 + (ptrdiff_t)(j - 1) * Bp;
-            cl[j] = bf2f(ldb(pcl + e, o2));
-            ph[j] = ldb(pp + e, o4);
-            pl[j] = bf2f(ldb(ppl + e, o2));
+            const ptrdiff_t c = (ptrdiff_t)(j - 1) * Bp;
+            um[j] = hl(ldb(pc + r0 + c, o4), ldb(pcl + r0 + c, o2));
+            qh[j] = ldb(pp + r1 + c, o4);
+            ql[j] = ldb(ppl + r1 + c, o2);
         }
 #pragma unroll
         for (int j = 0; j < CA + 2; ++j) {
-            uc[j] = ldb(r1 + (ptrdiff_t)(j - 2) * Bp, o4);
-            un[j] = ldb(r2 + (ptrdiff_t)(j - 2) * Bp, o4);
+            const ptrdiff_t c = (ptrdiff_t)(j - 2) * Bp;
+            uc[j] = hl(ldb(pc + r1 + c, o4), ldb(pcl + r1 + c, o2));
+            nh[j] = ldb(pc + r2 + c, o4);
+            nl[j] = ldb(pcl + r2 + c, o2);
         }
 #pragma unroll
-        for (int j = 0; j < CA; ++j) { am[j] = 0.0f; ac[j] = 0.0f; an[j] = 0.0f; }
-#pragma unroll
-        for (int i = 0; i < CX; ++i) { alo[i] = 0.0f; clo[i] = 0.0f; }
+        for (int j = 0; j < CA; ++j) { am[j] = 0.0; ac[j] = 0.0; an[j] = 0.0; }
     }
     for (int yf = y0 - 1; yf <= y1; ++yf) {
         const int yr = (int)clamp_row(yf, ny);  // row actually read for halo rows
-        // ---- loads for the next iteration: pc row yf+2, pcl/pp/ppl row yf+1
-        float nn[CA + 2], ncl[CA], nph[CA], npl[CA];
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) un[j] = hl(nh[j], nl[j]);
+        double q[CA];
+#pragma unroll
+        for (int j = 0; j < CA; ++j) q[j] = hl(qh[j], ql[j]);
+        // ---- loads for the next iteration: pc row yf+2, pp row yf+1
         {
-            const float *rnn = pc + clamp_row(yf + 2, ny) * rowE;
+            const size_t rnn = clamp_row(yf + 2, ny) * rowE;
             const size_t en = clamp_row(yf + 1, ny) * rowE;
 #pragma unroll
-            for (int j = 0; j < CA + 2; ++j) nn[j] = ldb(rnn + (ptrdiff_t)(j - 2) * Bp, o4);
+            for (int j = 0; j < CA + 2; ++j) {
+                const ptrdiff_t c = (ptrdiff_t)(j - 2) * Bp;
+                nh[j] = ldb(pc + rnn + c, o4);
+                nl[j] = ldb(pcl + rnn + c, o2);
+            }
 #pragma unroll
             for (int j = 0; j < CA; ++j) {
-                const ptrdiff_t e = (ptrdiff_t)en + (ptrdiff_t)(j - 1) * Bp;
-                ncl[j] = bf2f(ldb(pcl + e, o2));
-                nph[j] = ldb(pp + e, o4);
-                npl[j] = bf2f(ldb(ppl + e, o2));
+                const ptrdiff_t c = (ptrdiff_t)(j - 1) * Bp;
+                qh[j] = ldb(pp + en + c, o4);
+                ql[j] = ldb(ppl + en + c, o2);
             }
         }
         // ---- level n on row yf, columns x0-1 .. x0+CX
@@ -415,32 +388,27 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
         const size_t ro = (size_t)yr * rowE;
         const float2 *spr = sp2 + (size_t)yr * nx;
         const bool rec_a_row = REC && own_a && row_rec[yr] > 0;
-        float nlo[CX], ncx[CX];
 #pragma unroll
         for (int j = 0; j < CA; ++j) {
             const int x = x0 - 1 + j;
-            float lu = 0.0f;
-            if (yint && x >= 1 && x <= nx - 2) lu = lap5(uc[j + 1], un[j + 1], um[j], uc[j + 2], uc[j]);
-            float inc = smul(ldb(spr + (j - 1), oc), lu);
+            double lu = 0.0;
+            if (yint && x >= 1 && x <= nx - 2) lu = lap5d(uc[j + 1], un[j + 1], um[j], uc[j + 2], uc[j]);
+            double inc = s2d(ldb(spr + (j - 1), oc)) * lu;
             if (yf * nx + x == my_src) inc = inc + inc_a;
-            float lo;
-            const float v = leap_hilo(uc[j + 1], cl[j], ph[j], pl[j], inc, lo);
-            const uint16_t lb = f2bf(lo);
+            const double nv = (2.0 * uc[j + 1] - q[j]) + inc;
+            uint16_t lb;
+            const float v = split_hl(nv, lb);
             am[j] = ac[j];
             ac[j] = an[j];
-            an[j] = v;
-            if (j >= 1 && j <= CX) {
-                nlo[j - 1] = bf2f(lb);
-                ncx[j - 1] = cl[j];
-                if (own_a && x < nx) {
-                    const ptrdiff_t e = (ptrdiff_t)ro + (ptrdiff_t)(j - 1) * Bp;
-                    stb(pa + e, o4, v);
-                    stb(pal + e, o2, lb);
-                    if (rec_a_row) {
-                        for (int r = rec_head[yr * nx + x]; r >= 0; r = rec_next[r]) {
-                            rec_a[(size_t)r * Bp + b] = v;
-                            rec_lo_a[(size_t)r * Bp + b] = bf2f(lb);
-                        }
+            an[j] = hl(v, lb);
+            if (j >= 1 && j <= CX && own_a && x < nx) {
+                const ptrdiff_t e = (ptrdiff_t)ro + (ptrdiff_t)(j - 1) * Bp;
+                stb(pa + e, o4, v);
+                stb(pal + e, o2, lb);
+                if (rec_a_row) {
+                    for (int r = rec_head[yr * nx + x]; r >= 0; r = rec_next[r]) {
+                        rec_a[(size_t)r * Bp + b] = v;
+                        rec_lo_a[(size_t)r * Bp + b] = bf2f(lb);
                     }
                 }
             }
@@ -456,14 +424,14 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
             for (int i = 0; i < CX; ++i) {
                 const int x = x0 + i;
                 if (x < nx) {
-                    float lu = 0.0f;
+                    double lu = 0.0;
                     if (yint_b && x >= 1 && x <= nx - 2)
-                        lu = lap5(ac[i + 1], an[i + 1], am[i + 1], ac[i + 2], ac[i]);
-                    float inc = smul(ldb(spb + i, oc), lu);
+                        lu = lap5d(ac[i + 1], an[i + 1], am[i + 1], ac[i + 2], ac[i]);
+                    double inc = s2d(ldb(spb + i, oc)) * lu;
                     if (y * nx + x == my_src) inc = inc + inc_b;
-                    float lo;
-                    const float v = leap_hilo(ac[i + 1], alo[i], um[i + 1], clo[i], inc, lo);
-                    const uint16_t lb = f2bf(lo);
+                    const double nv = (2.0 * ac[i + 1] - um[i + 1]) + inc;
+                    uint16_t lb;
+                    const float v = split_hl(nv, lb);
                     const size_t e = rob + (size_t)i * Bp;
                     stb(pb + e, o4, v);
                     stb(pbl + e, o2, lb);
@@ -478,16 +446,9 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
         }
         // ---- advance
 #pragma unroll
-        for (int i = 0; i < CX; ++i) { alo[i] = nlo[i]; clo[i] = ncx[i]; }
+        for (int j = 0; j < CA; ++j) um[j] = uc[j + 1];
 #pragma unroll
-        for (int j = 0; j < CA; ++j) {
-            um[j] = uc[j + 1];
-            cl[j] = ncl[j];
-            ph[j] = nph[j];
-            pl[j] = npl[j];
-        }
-#pragma unroll
-        for (int j = 0; j < CA + 2; ++j) { uc[j] = un[j]; un[j] = nn[j]; }
+        for (int j = 0; j < CA + 2; ++j) uc[j] = un[j];
     }
 }
 
@@ -495,8 +456,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
 // Batched 2D adjoint step with the imaging condition.
 //   l1 = lam^{k+1}, l2 = lam^{k+2}, lk = lam^k (output)
 //   no sponge: pa = p^{k-1};  sponge: pa = p^k, pb = p^{k-1}, pc2 = p^{k-2}
-//   res = [n_rec][Bp] residual row of level k; srcd2[b] = recorded increment
-//   of level k at shot b's source cell
+//   res = [n_rec][Bp] residual row of level k; sinc[b] = S_src w[k-1]
 //   g_hi/g_lo: [ng][N] Neumaier accumulators of sum_shots sum_k lam^k d2p^k
 // Addressing as k_fwd2d (lanes past the last column still load, from the
 // guard zone, and join the reductions with zero contributions).
@@ -858,13 +818,26 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
 // Batched 3D forward step: each lane owns a CX x CY patch and marches RZ
 // planes along z.  Row starts (y clamped, wave-uniform) and column steps go
 // into the scalar base; a lane keeps one byte offset (addressing as k_fwd2d).
+//
+// The update is done in FP64 from the full stored state P = hi + lo of all
+// seven stencil points, with S' and the source term S_src w[n-1] (sincd) in
+// FP64, and the result split back into FP32 hi + bfloat16 lo.  In 3D the
+// gradient sums lam^k d2p^k over many time levels with strong cancellation,
+// so an increment rounded to FP32 at every step (or a Laplacian of the high
+// parts only) leaves a wavefield error of a few 1e-7 that the sum turns into a
+// gradient error of several 1e-4; with the increment kept in FP64 the
+// gradient is within ~2e-6 of the FP64 reference.  CDNA FP64 vector
+// arithmetic runs at the FP32 rate, and the storage (6 bytes per value) and
+// so the HBM traffic are unchanged.  srcd2[b] records the increment applied
+// at shot b's source cell for the adjoint's imaging.
 // ---------------------------------------------------------------------------
+
 template <int LB, int CX, int CY, bool REC>
 __global__ void __launch_bounds__(kBlock)
 k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
         const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
         uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
-        const int *__restrict__ src_cell, const float *__restrict__ sinc,
+        const int *__restrict__ src_cell, const double *__restrict__ sincd,
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
         const int *__restrict__ row_rec, float *__restrict__ rec, float *__restrict__ rec_lo,
         float *__restrict__ srcd2, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
@@ -885,9 +858,8 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     const int z0 = zc * RZ;
     const int z1 = min(z0 + RZ, nz);
     if (x0 >= nx) return;
-    (void)src_cell;
-    (void)sinc;
-    (void)srcd2;
+    const int my_src = src_cell[b];
+    const double my_inc = sincd[b];
     const size_t rowE = (size_t)nx * Bp;
     const size_t planeE = rowE * ny;
     const size_t planeC = (size_t)nx * ny;
@@ -896,40 +868,42 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     const unsigned oc = (unsigned)x0 * 8u;
     // row j of the patch (j = -1..CY), clamped into the grid; wave-uniform
     auto row = [&](int j) -> size_t { return (size_t)min(max(y0 + j, 0), ny - 1); };
-    auto F = [&](const float *plane, int j, int i) -> float {
-        return ldb(plane + row(j) * rowE + (ptrdiff_t)i * Bp, o4);
+    // full state hi + lo of cell (row j, column i) of a plane, in FP64
+    auto P = [&](size_t plane, int j, int i) -> double {
+        const size_t e = plane + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+        return (double)ldb(pc + e, o4) + (double)bf2f(ldb(pcl + e, o2));
     };
-    float zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
-    auto load_halo = [&](const float *pl) {
+    double zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
+    auto load_halo = [&](size_t pl) {
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
-            zcu[0][i + 1] = F(pl, -1, i);
-            zcu[CY + 1][i + 1] = F(pl, CY, i);
+            zcu[0][i + 1] = P(pl, -1, i);
+            zcu[CY + 1][i + 1] = P(pl, CY, i);
         }
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
-            zcu[j + 1][0] = F(pl, j, -1);
-            zcu[j + 1][CX + 1] = F(pl, j, CX);
+            zcu[j + 1][0] = P(pl, j, -1);
+            zcu[j + 1][CX + 1] = P(pl, j, CX);
         }
     };
     {
-        const float *pm = pc + (size_t)max(z0 - 1, 0) * planeE;
-        const float *p0 = pc + (size_t)z0 * planeE;
+        const size_t pm = (size_t)max(z0 - 1, 0) * planeE;
+        const size_t p0 = (size_t)z0 * planeE;
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
-                zm[j][i] = F(pm, j, i);
-                zcu[j + 1][i + 1] = F(p0, j, i);
+                zm[j][i] = P(pm, j, i);
+                zcu[j + 1][i + 1] = P(p0, j, i);
             }
         load_halo(p0);
     }
     for (int z = z0; z < z1; ++z) {
-        const float *pnz = pc + (size_t)min(z + 1, nz - 1) * planeE;
+        const size_t pnz = (size_t)min(z + 1, nz - 1) * planeE;
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
-            for (int i = 0; i < CX; ++i) zp[j][i] = F(pnz, j, i);
+            for (int i = 0; i < CX; ++i) zp[j][i] = P(pnz, j, i);
         const bool zint = (z >= 1) && (z <= nz - 2);
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
@@ -942,20 +916,23 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             for (int i = 0; i < CX; ++i) {
                 const int x = x0 + i;
                 if (x < nx && y < ny) {
-                    float lu = 0.0f;
+                    double lu = 0.0;
                     if (yint && x >= 1 && x <= nx - 2)
-                        lu = lap7(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
-                                  zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
-                    const float inc = smul(ldb(spr + i, oc), lu);
+                        lu = lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
+                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
+                    const float2 s2 = ldb(spr + i, oc);
+                    double inc = ((double)s2.x + (double)s2.y) * lu;
+                    const int c = (z * ny + y) * nx + x;
+                    if (c == my_src) inc = inc + my_inc;
                     const size_t ei = ro + (size_t)i * Bp;
-                    float lo;
-                    const float v = leap_hilo(zcu[j + 1][i + 1], bf2f(ldb(pcl + ei, o2)),
-                                              ldb(pp + ei, o4), bf2f(ldb(ppl + ei, o2)), inc, lo);
-                    const uint16_t lb = f2bf(lo);
+                    const double q = (double)ldb(pp + ei, o4) + (double)bf2f(ldb(ppl + ei, o2));
+                    const double nv = (2.0 * zcu[j + 1][i + 1] - q) + inc;
+                    const float v = (float)nv;
+                    const uint16_t lb = f2bf((float)(nv - (double)v));
                     stb(pn + ei, o4, v);
                     stb(pnl + ei, o2, lb);
+                    if (c == my_src) srcd2[b] = (float)inc;
                     if (has_rec) {
-                        const int c = (z * ny + y) * nx + x;
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
                             rec[(size_t)r * Bp + b] = v;
                             rec_lo[(size_t)r * Bp + b] = bf2f(lb);
@@ -1143,7 +1120,7 @@ __global__ void __launch_bounds__(kBlock)
 k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
             const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
             uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
-            const float *__restrict__ sponge, int src, const float *__restrict__ sinc,
+            const float *__restrict__ sponge, int src, const double *__restrict__ sincd,
             const int *__restrict__ rec_head, const int *__restrict__ rec_next,
             float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int nz,
             int ny, int nx, int ndim) {
@@ -1154,30 +1131,24 @@ k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *_
     const int x = c % nx;
     const int y = (c / nx) % ny;
     const int z = c / (nx * ny);
-    float lu = 0.0f;
-    const float pcv = pc[c];
-    if (ndim == 2) {
-        if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
-            lu = lap5(pcv, pc[c + nx], pc[c - nx], pc[c + 1], pc[c - 1]);
-    } else {
+    // FP64 update from the full hi + lo state (as the batched kernels)
+    auto P = [&](int e) { return hl(pc[e], pcl[e]); };
+    double lu = 0.0;
+    if (ndim == 3) {
         const int sz = nx * ny;
         if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
-            lu = lap7(pcv, pc[c + sz], pc[c - sz], pc[c + nx], pc[c - nx], pc[c + 1], pc[c - 1]);
+            lu = lap7d(P(c), P(c + sz), P(c - sz), P(c + nx), P(c - nx), P(c + 1), P(c - 1));
+    } else {
+        if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+            lu = lap5d(P(c), P(c + nx), P(c - nx), P(c + 1), P(c - 1));
     }
-    if (c == src && ndim == 3) {
-        source_cell_update<SPONGE, 3, REC>(
-                pc, pcl, pp, ppl, pn, pnl, sp2, sponge, *sinc, (size_t)c, c, 1, (size_t)nx,
-                (size_t)nx * ny,
-                z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2, rec_head,
-                rec_next, rec, rec_lo, 1, 0, srcd2);
-        return;
-    }
-    float inc = smul(sp2[c], lu);
-    if (c == src) inc = inc + *sinc;  // 2D: source added in FP32 as the batched path does
-    float lo;
-    float v = leap_hilo(pcv, bf2f(pcl[c]), pp[c], bf2f(ppl[c]), inc, lo);
-    if (SPONGE) v = scale_hilo(v, lo, sponge[c], lo);
-    const uint16_t lb = f2bf(lo);
+    double inc = s2d(sp2[c]) * lu;
+    if (c == src) inc = inc + *sincd;
+    double nv = (2.0 * P(c) - hl(pp[c], ppl[c])) + inc;
+    if (SPONGE) nv = nv * (double)sponge[c];
+    if (c == src) *srcd2 = (float)inc;
+    uint16_t lb;
+    const float v = split_hl(nv, lb);
     pn[c] = v;
     pnl[c] = lb;
     if (REC) {
@@ -1240,37 +1211,8 @@ k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *_
 }
 
 // ---------------------------------------------------------------------------
-// Batched source-cell update: one thread per shot, launched after each
-// forward step of the batched path (layout cell * Bp + shot).
+// Set-up and trace kernels
 // ---------------------------------------------------------------------------
-template <bool SPONGE, int NDIM, bool REC>
-__global__ void __launch_bounds__(64)
-k_src_fix(const float *__restrict__ pc, const uint16_t *__restrict__ pcl,
-          const float *__restrict__ pp, const uint16_t *__restrict__ ppl, float *__restrict__ pn,
-          uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
-          const float *__restrict__ sponge, const int *__restrict__ src_cell,
-          const float *__restrict__ sinc, const int *__restrict__ rec_head,
-          const int *__restrict__ rec_next, float *__restrict__ rec, float *__restrict__ rec_lo,
-          float *__restrict__ srcd2, int nz, int ny, int nx, int Bp) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= Bp) return;
-    const int c = src_cell[b];
-    if (c < 0) return;
-    const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
-    bool in = y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2;
-    if (NDIM == 3) in = in && z >= 1 && z <= nz - 2;
-    const size_t BP = (size_t)Bp;
-    source_cell_update<SPONGE, NDIM, REC>(pc, pcl, pp, ppl, pn, pnl, sp2, sponge, sinc[b],
-                                          (size_t)c * BP + b, c, BP, (size_t)nx * BP,
-                                          (size_t)nx * ny * BP, in, rec_head, rec_next, rec,
-                                          rec_lo, BP, b, srcd2);
-}
-
-// ---------------------------------------------------------------------------
-// Set-up, residual, reductions and layout kernels.
-// ---------------------------------------------------------------------------
-
-// S = dt^2/m (fwi.py:52), S' = S / h^2 as an FP32 hi/lo pair.
 __global__ void k_setup_s(const double *__restrict__ m, float2 *__restrict__ sp2, int64_t N,
                           double dt2, double inv_h2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1282,15 +1224,16 @@ __global__ void k_setup_s(const double *__restrict__ m, float2 *__restrict__ sp2
 
 // sinc[n][b] = S(src_b) * w[n-1] for n >= 1 (0 for padded shots and n = 0).
 __global__ void k_setup_sinc(const double *__restrict__ m, const double *__restrict__ w,
-                             const int *__restrict__ src_cell, float *__restrict__ sinc, int nt,
-                             int Bp, double dt2) {
+                             const int *__restrict__ src_cell, float *__restrict__ sinc,
+                             double *__restrict__ sincd, int nt, int Bp, double dt2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nt * Bp) return;
     const int n = (int)(i / Bp), b = (int)(i % Bp);
     const int s = src_cell[b];
-    float v = 0.0f;
-    if (n >= 1 && s >= 0) v = (float)((dt2 / m[s]) * w[n - 1]);
-    sinc[i] = v;
+    double v = 0.0;
+    if (n >= 1 && s >= 0) v = (dt2 / m[s]) * w[n - 1];
+    sinc[i] = (float)v;
+    sincd[i] = v;
 }
 
 __global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
@@ -1304,39 +1247,75 @@ __global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
         p[i] = v;
 }
 
-// dobsT[n][j][b] = dobs[b0 + b][n][j] for b < nb, 0 for padded shots.
-__global__ void k_dobs_to_batched(const float *__restrict__ dobs, float *__restrict__ dT, int nt,
-                                  int n_rec, int Bp, int b0, int nb) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t tot = (int64_t)nt * n_rec * Bp;
-    if (i >= tot) return;
-    const int b = (int)(i % Bp);
-    const int64_t nj = i / Bp;
-    dT[i] = b < nb ? dobs[((int64_t)(b0 + b) * nt) * n_rec + nj] : 0.0f;
+// Traces move between the caller's layout data[b][n][j] (FP64) and the
+// batched layout [n][j][b] (FP32 hi + lo) through 32 x 32 LDS tiles, so both
+// the reads and the writes are coalesced.  Grid: (ceil(nt*n_rec / 32),
+// ceil(Bp / 32)), 256 threads.
+constexpr int kTT = 32;
+
+// dT[nj][b] + dT_lo[nj][b] = dobs[b0 + b][nj] (b < nb; padded shots get 0).
+__global__ void __launch_bounds__(256)
+k_dobs_to_batched(const double *__restrict__ dobs, float *__restrict__ dT,
+                  float *__restrict__ dT_lo, int64_t NJ, int Bp, int b0, int nb) {
+    __shared__ double tile[kTT][kTT + 1];
+    const int64_t nj0 = (int64_t)blockIdx.x * kTT;
+    const int bb0 = (int)blockIdx.y * kTT;
+    const int tx = threadIdx.x & (kTT - 1), ty = threadIdx.x / kTT;
+#pragma unroll
+    for (int r = ty; r < kTT; r += 256 / kTT) {
+        const int b = bb0 + r;
+        const int64_t nj = nj0 + tx;
+        tile[r][tx] = (b < nb && nj < NJ) ? dobs[(int64_t)(b0 + b) * NJ + nj] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = ty; r < kTT; r += 256 / kTT) {
+        const int64_t nj = nj0 + r;
+        const int b = bb0 + tx;
+        if (nj < NJ && b < Bp) {
+            const double v = tile[tx][r];
+            const float h = (float)v;
+            dT[nj * Bp + b] = h;
+            dT_lo[nj * Bp + b] = (float)(v - (double)h);
+        }
+    }
 }
 
-// data[b0 + b][n][j] = rec[n][j][b] for b < nb.
-__global__ void k_batched_to_data(const float *__restrict__ rec, const float *__restrict__ rec_lo,
-                                  float *__restrict__ data, int nt, int n_rec, int Bp, int b0,
-                                  int nb) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t tot = (int64_t)nb * nt * n_rec;
-    if (i >= tot) return;
-    const int64_t nj = i % ((int64_t)nt * n_rec);
-    const int b = (int)(i / ((int64_t)nt * n_rec));
-    data[(int64_t)(b0 + b) * nt * n_rec + nj] = rec[nj * Bp + b] + rec_lo[nj * Bp + b];
+// data[b0 + b][nj] = rec[nj][b] + rec_lo[nj][b] in FP64, for b < nb.
+__global__ void __launch_bounds__(256)
+k_batched_to_data(const float *__restrict__ rec, const float *__restrict__ rec_lo,
+                  double *__restrict__ data, int64_t NJ, int Bp, int b0, int nb) {
+    __shared__ double tile[kTT][kTT + 1];
+    const int64_t nj0 = (int64_t)blockIdx.x * kTT;
+    const int bb0 = (int)blockIdx.y * kTT;
+    const int tx = threadIdx.x & (kTT - 1), ty = threadIdx.x / kTT;
+#pragma unroll
+    for (int r = ty; r < kTT; r += 256 / kTT) {
+        const int64_t nj = nj0 + r;
+        const int b = bb0 + tx;
+        tile[r][tx] = (nj < NJ && b < Bp) ? (double)rec[nj * Bp + b] + (double)rec_lo[nj * Bp + b]
+                                          : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = ty; r < kTT; r += 256 / kTT) {
+        const int b = bb0 + r;
+        const int64_t nj = nj0 + tx;
+        if (b < nb && nj < NJ) data[(int64_t)(b0 + b) * NJ + nj] = tile[tx][r];
+    }
 }
 
 // L2 residual r = dsyn - dobs over `count` entries (padded entries are 0 - 0),
-// and per-block Neumaier partial sums of r^2 (fwi.py:155-157).
+// both as FP32 hi + lo pairs: r = (s_hi - o_hi) + (s_lo - o_lo); and
+// per-block Neumaier partial sums of r^2 (fwi.py:155-157).
 __global__ void __launch_bounds__(kBlock)
 k_residual_l2(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
-              const float *__restrict__ dobs, float *__restrict__ res, int64_t count,
-              float2 *__restrict__ part) {
+              const float *__restrict__ dobs, const float *__restrict__ dobs_lo,
+              float *__restrict__ res, int64_t count, float2 *__restrict__ part) {
     float s = 0.0f, c = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const float r = (dsyn[i] - dobs[i]) + dsyn_lo[i];
+        const float r = (dsyn[i] - dobs[i]) + (dsyn_lo[i] - dobs_lo[i]);
         if (res) res[i] = r;
         neumaier(s, c, r * r);
     }
@@ -1374,14 +1353,15 @@ __global__ void k_reduce_partials(const float2 *__restrict__ part, int n, double
 // stats[t] = (|s|^2, |o|^2, <s,o>) in FP64; layout of traces [n][j][b] with
 // stride n_rec*Bp between time samples.
 __global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
-                            const float *__restrict__ dobs, double *__restrict__ stats, int nt,
-                            int64_t ntr) {
+                            const float *__restrict__ dobs, const float *__restrict__ dobs_lo,
+                            double *__restrict__ stats, int nt, int64_t ntr) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntr) return;
     double ss = 0.0, oo = 0.0, so = 0.0;
     for (int n = 0; n < nt; ++n) {
         const int64_t q = (int64_t)n * ntr + t;
-        const double s = (double)dsyn[q] + (double)dsyn_lo[q], o = dobs[q];
+        const double s = (double)dsyn[q] + (double)dsyn_lo[q];
+        const double o = (double)dobs[q] + (double)dobs_lo[q];
         ss += s * s;
         oo += o * o;
         so += s * o;
@@ -1394,7 +1374,8 @@ __global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restr
 // GCN adjoint source res = -(o/no - c s/ns)/ns and per-trace misfit 1 - c
 // (fwi.py:164-167); tmask[t] = 0 for padded traces.
 __global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
-                               const float *__restrict__ dobs, const double *__restrict__ stats,
+                               const float *__restrict__ dobs, const float *__restrict__ dobs_lo,
+                               const double *__restrict__ stats,
                                const unsigned char *__restrict__ tmask, float *__restrict__ res,
                                int nt, int64_t ntr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1405,7 +1386,8 @@ __global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__re
     const double ns = sqrt(stats[3 * t]) + eps, no = sqrt(stats[3 * t + 1]) + eps;
     const double c = stats[3 * t + 2] / (ns * no);
     const double s = (double)dsyn[i] + (double)dsyn_lo[i];
-    res[i] = (float)(-((double)dobs[i] / no - c * (s / ns)) / ns);
+    const double o = (double)dobs[i] + (double)dobs_lo[i];
+    res[i] = (float)(-(o / no - c * (s / ns)) / ns);
 }
 
 __global__ void k_gcn_misfit(const double *__restrict__ stats, const unsigned char *__restrict__ tmask,

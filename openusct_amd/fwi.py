@@ -127,7 +127,7 @@ def _dobs_dev(dobs, plan):
     want = (plan.n_tx, plan.nt, plan.n_rec)
     if tuple(np.shape(dobs)) != want:
         raise ValueError(f"dobs has shape {tuple(np.shape(dobs))}, expected {want}")
-    return _to_dev(dobs, torch.float32)
+    return _to_dev(dobs, torch.float64)
 
 
 def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, misfit_type="l2",

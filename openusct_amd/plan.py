@@ -127,14 +127,14 @@ class Plan:
         want = (self.n_tx, self.nt, self.n_rec)
         if tuple(dobs.shape) != want:
             raise ValueError(f"dobs has shape {tuple(dobs.shape)}, expected {want}")
-        return dobs.to(device=self.device, dtype=torch.float32).contiguous()
+        return dobs.to(device=self.device, dtype=torch.float64).contiguous()
 
     # -- operations -------------------------------------------------------------
     def forward_fmc(self, m, wavelet, sponge=None, out=None, stream=None):
-        """Traces (n_tx, nt, n_rec) float32 on the device (row 0 is zero)."""
+        """Traces (n_tx, nt, n_rec) float64 on the device (row 0 is zero)."""
         m, w, sp = self._model(m), self._wavelet(wavelet), self._sponge(sponge)
         if out is None:
-            out = torch.empty((self.n_tx, self.nt, self.n_rec), dtype=torch.float32,
+            out = torch.empty((self.n_tx, self.nt, self.n_rec), dtype=torch.float64,
                               device=self.device)
         _lib.check(_lib.load().usct_forward_fmc(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(out),
                                                 _stream_handle(stream)))
@@ -196,7 +196,7 @@ def footprint_bytes(shape, nt, n_tx, n_rec, ckpt_every=0, gradient=True):
     fields = 5 + ((K + 2) + 2 * max(NS - 2, 0) if gradient else 0)
     halves = 5 + (2 * max(NS - 2, 0) if gradient else 0)
     traces = 4 * nt * n_rec * Bp
-    b = fields * field + halves * (field // 2) + 4 * traces
+    b = fields * field + halves * (field // 2) + 5 * traces
     b += (4 * fields + 2 * halves) * 2 * 64 * Bp  # guard zones
     b += 8 * N + 4 * N + 8 * N * max(Bp // 16, 1) + 8 * nt * Bp + 24 * n_rec * Bp + 8 * N
     return int(b + (1 << 20))

@@ -143,7 +143,7 @@ class Server:
             m = torch.as_tensor(req.m, dtype=torch.float64).to(self.device, non_blocking=False)
             w = torch.as_tensor(req.wavelet, dtype=torch.float64).to(self.device)
             if req.kind == "gradient":
-                d = torch.as_tensor(req.dobs, dtype=torch.float32).to(self.device)
+                d = torch.as_tensor(req.dobs, dtype=torch.float64).to(self.device)
                 fl.J, fl.grad = plan.misfit_and_gradient(m, w, d, stream=stream)
                 fl.tensors += [m, w, d]
             elif req.kind == "forward":

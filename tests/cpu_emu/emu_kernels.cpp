@@ -43,7 +43,8 @@ struct Problem {
     unsigned blocks;
     float2 *sp2;
     int *src_cell, *rec_head, *rec_next, *row_rec;
-    float *sinc;  // [3][Bp]
+    float *sinc;    // [3][Bp]
+    double *sincd;  // [3][Bp], the same values in FP64
     std::mt19937 rng;
 
     Problem(int ny_, int nx_, int LB_, int ng_, int CX_, int RY_, uint32_t seed)
@@ -76,8 +77,12 @@ struct Problem {
         }
         src_cell[Bp - 1] = -1;
         sinc = alloc<float>(3 * (int64_t)Bp, 0);
+        sincd = alloc<double>(3 * (int64_t)Bp, 0);
         std::uniform_real_distribution<float> w(-2.0f, 2.0f);
-        for (int i = 0; i < 3 * Bp; ++i) sinc[i] = w(rng);
+        for (int i = 0; i < 3 * Bp; ++i) {
+            sincd[i] = (double)w(rng) * 1.0000001234567;
+            sinc[i] = (float)sincd[i];
+        }
         // receivers: random cells, a duplicate pair, the four corners, strip edges
         std::vector<int> rc;
         for (int i = 0; i < 12; ++i) rc.push_back(cy(rng) * nx + cx(rng));
@@ -128,6 +133,57 @@ bool same(const char *what, const T *a, const T *b, int64_t n, const char *tag) 
     return false;
 }
 
+// One 2D forward level by a direct loop over cells: the FP64 update the
+// kernels implement (full hi + lo state, FP64 S', source and sponge).
+bool direct_fwd2d(const Problem &P, const float *pc, const uint16_t *pcl, const float *pp,
+                  const uint16_t *ppl, const double *sincd, const float *sponge, const float *pn,
+                  const uint16_t *pnl, const char *tag) {
+    auto Pv = [&](int64_t c, int b) { return hl(pc[c * P.Bp + b], pcl[c * P.Bp + b]); };
+    for (int y = 0; y < P.ny; ++y)
+        for (int x = 0; x < P.nx; ++x)
+            for (int b = 0; b < P.Bp; ++b) {
+                const int64_t c = (int64_t)y * P.nx + x;
+                double lu = 0.0;
+                if (y >= 1 && y <= P.ny - 2 && x >= 1 && x <= P.nx - 2)
+                    lu = lap5d(Pv(c, b), Pv(c + P.nx, b), Pv(c - P.nx, b), Pv(c + 1, b), Pv(c - 1, b));
+                double inc = s2d(P.sp2[c]) * lu;
+                if (c == P.src_cell[b]) inc = inc + sincd[b];
+                double nv = (2.0 * Pv(c, b) - hl(pp[c * P.Bp + b], ppl[c * P.Bp + b])) + inc;
+                if (sponge) nv = nv * (double)sponge[c];
+                uint16_t lb;
+                const float v = split_hl(nv, lb);
+                if (std::memcmp(&v, &pn[c * P.Bp + b], 4) != 0 || lb != pnl[c * P.Bp + b]) {
+                    std::printf("MISMATCH %s direct FP64 update at cell %lld shot %d\n", tag,
+                                (long long)c, b);
+                    return false;
+                }
+            }
+    return true;
+}
+
+template <int LB, int CX, bool SPONGE>
+bool check_fwd_direct(int ny, int nx, int ng, int RY, uint32_t seed) {
+    Problem P(ny, nx, LB, ng, CX, RY, seed);
+    char tag[160];
+    std::snprintf(tag, sizeof tag, "fwd-direct ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d sponge=%d", ny,
+                  nx, LB, ng, CX, RY, (int)SPONGE);
+    float *pc = P.field(1.0f), *pp = P.field(1.0f), *pn = P.zeros();
+    uint16_t *pcl = P.low(1e-7f), *ppl = P.low(1e-7f), *pnl = P.zeros_lo();
+    float *sp = alloc<float>(P.N, 0);
+    std::uniform_real_distribution<float> d(0.5f, 1.0f);
+    for (int64_t c = 0; c < P.N; ++c) sp[c] = d(P.rng);
+    float *ra = P.traces(), *rl = P.traces();
+    const double *sa = P.sincd + P.Bp;
+    emu::launch(P.blocks, kBlock, [&] {
+        k_fwd2d<LB, CX, SPONGE, true>(pc, pp, pn, pcl, ppl, pnl, P.sp2, sp, P.src_cell, sa,
+                                      P.rec_head, P.rec_next, P.row_rec, ra, rl, P.ny, P.nx, P.Bp,
+                                      P.RY, P.nxs, P.nyc, P.ng);
+    });
+    const bool ok = direct_fwd2d(P, pc, pcl, pp, ppl, sa, SPONGE ? sp : nullptr, pn, pnl, tag);
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
 template <int LB, int CX, bool REC>
 bool check_fwd(int ny, int nx, int ng, int RY, uint32_t seed) {
     Problem P(ny, nx, LB, ng, CX, RY, seed);
@@ -140,16 +196,16 @@ bool check_fwd(int ny, int nx, int ng, int RY, uint32_t seed) {
     uint16_t *a1l = P.zeros_lo(), *b1l = P.zeros_lo(), *a2l = P.zeros_lo(), *b2l = P.zeros_lo();
     float *ra1 = P.traces(), *ral1 = P.traces(), *rb1 = P.traces(), *rbl1 = P.traces();
     float *ra2 = P.traces(), *ral2 = P.traces(), *rb2 = P.traces(), *rbl2 = P.traces();
-    const float *sa = P.sinc + P.Bp, *sb = P.sinc + 2 * P.Bp;
+    const double *sa = P.sincd + P.Bp, *sb = P.sincd + 2 * P.Bp;
     emu::launch(P.blocks, kBlock, [&] {
         k_fwd2d<LB, CX, false, REC>(pc, pp, a1, pcl, ppl, a1l, P.sp2, nullptr, P.src_cell, sa,
-                                    P.rec_head, P.rec_next, P.row_rec, ra1, ral1, nullptr, P.ny,
-                                    P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+                                    P.rec_head, P.rec_next, P.row_rec, ra1, ral1, P.ny, P.nx, P.Bp,
+                                    P.RY, P.nxs, P.nyc, P.ng);
     });
     emu::launch(P.blocks, kBlock, [&] {
         k_fwd2d<LB, CX, false, REC>(a1, pc, b1, a1l, pcl, b1l, P.sp2, nullptr, P.src_cell, sb,
-                                    P.rec_head, P.rec_next, P.row_rec, rb1, rbl1, nullptr, P.ny,
-                                    P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+                                    P.rec_head, P.rec_next, P.row_rec, rb1, rbl1, P.ny, P.nx, P.Bp,
+                                    P.RY, P.nxs, P.nyc, P.ng);
     });
     emu::launch(P.blocks, kBlock, [&] {
         k_fwd2d_x2<LB, CX, REC>(pc, pp, a2, b2, pcl, ppl, a2l, b2l, P.sp2, P.src_cell, sa, sb,
@@ -175,6 +231,8 @@ bool fwd_cases(uint32_t seed) {
             ok &= check_fwd<LB, CX, true>(s[0], s[1], 2, RY, seed++);
         }
     ok &= check_fwd<LB, CX, false>(29, 37, 1, 4, seed++);
+    ok &= check_fwd_direct<LB, CX, false>(29, 37, 2, 8, seed++);
+    ok &= check_fwd_direct<LB, CX, true>(9, 13, 1, 4, seed++);
     return ok;
 }
 
@@ -231,6 +289,99 @@ bool adj_cases(uint32_t seed) {
     return ok;
 }
 
+// 3D forward: k_fwd3d against a direct loop over cells of the same FP64 update
+// (full hi + lo state, FP64 S' and source term, split into FP32 hi + bf16 lo).
+template <int LB, int CX, int CY>
+bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
+    const int Bp = LB * ng, W = 64 / LB;
+    const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
+    std::mt19937 rng(seed);
+    char tag[160];
+    std::snprintf(tag, sizeof tag, "fwd3d nz=%d ny=%d nx=%d LB=%d ng=%d CX=%d CY=%d RZ=%d", nz, ny,
+                  nx, LB, ng, CX, CY, RZ);
+    std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
+        dl(-1e-7f, 1e-7f);
+    float2 *sp2 = alloc<float2>(N, 1);
+    for (int64_t c = 0; c < N; ++c) sp2[c] = make_float2(u(rng), tt(rng));
+    float *pc = alloc<float>(FE, Bp), *pp = alloc<float>(FE, Bp), *pn = alloc<float>(FE, Bp);
+    uint16_t *pcl = alloc<uint16_t>(FE, Bp), *ppl = alloc<uint16_t>(FE, Bp),
+             *pnl = alloc<uint16_t>(FE, Bp);
+    for (int64_t i = 0; i < FE; ++i) {
+        pc[i] = d(rng);
+        pp[i] = d(rng);
+        pcl[i] = f2bf(dl(rng));
+        ppl[i] = f2bf(dl(rng));
+    }
+    int *src = alloc<int>(Bp, 0);
+    std::uniform_int_distribution<int> cz(0, nz - 1), cy(0, ny - 1), cx(0, nx - 1);
+    for (int b = 0; b < Bp; ++b) src[b] = (cz(rng) * ny + cy(rng)) * nx + (b % 3 == 0 ? 0 : cx(rng));
+    src[Bp - 1] = -1;
+    double *sincd = alloc<double>(Bp, 0);
+    for (int b = 0; b < Bp; ++b) sincd[b] = (double)d(rng) * 1.2345678901;
+    float *srcd2 = alloc<float>(Bp, 0);
+    std::vector<int> rc = {0, (int)N - 1, (int)(N / 2), (int)(N / 3), (int)(N / 3)};
+    const int n_rec = (int)rc.size();
+    int *rec_head = alloc<int>(N, 0), *rec_next = alloc<int>(n_rec, 0);
+    int *row_rec = alloc<int>((int64_t)nz * ny, 0);
+    for (int64_t c = 0; c < N; ++c) rec_head[c] = -1;
+    for (int j = n_rec - 1; j >= 0; --j) {
+        rec_next[j] = rec_head[rc[j]];
+        rec_head[rc[j]] = j;
+    }
+    for (int j = 0; j < n_rec; ++j) row_rec[rc[j] / nx]++;
+    float *rec = alloc<float>((int64_t)n_rec * Bp, 0), *rec_lo = alloc<float>((int64_t)n_rec * Bp, 0);
+    const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
+    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
+    emu::launch(blocks, kBlock, [&] {
+        k_fwd3d<LB, CX, CY, true>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head, rec_next,
+                                  row_rec, rec, rec_lo, srcd2, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
+                                  ng);
+    });
+    bool ok = true;
+    auto Pv = [&](int64_t c, int b) { return (double)pc[c * Bp + b] + (double)bf2f(pcl[c * Bp + b]); };
+    for (int z = 0; z < nz && ok; ++z)
+        for (int y = 0; y < ny && ok; ++y)
+            for (int x = 0; x < nx && ok; ++x)
+                for (int b = 0; b < Bp; ++b) {
+                    const int64_t c = ((int64_t)z * ny + y) * nx + x;
+                    double lu = 0.0;
+                    if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+                        lu = lap7d(Pv(c, b), Pv(c + (int64_t)nx * ny, b), Pv(c - (int64_t)nx * ny, b),
+                                   Pv(c + nx, b), Pv(c - nx, b), Pv(c + 1, b), Pv(c - 1, b));
+                    double inc = ((double)sp2[c].x + (double)sp2[c].y) * lu;
+                    if (c == src[b]) inc = inc + sincd[b];
+                    const double q = (double)pp[c * Bp + b] + (double)bf2f(ppl[c * Bp + b]);
+                    const double nv = (2.0 * Pv(c, b) - q) + inc;
+                    const float v = (float)nv;
+                    const uint16_t lb = f2bf((float)(nv - (double)v));
+                    if (std::memcmp(&v, &pn[c * Bp + b], 4) != 0 || lb != pnl[c * Bp + b]) {
+                        std::printf("MISMATCH %s cell %lld shot %d\n", tag, (long long)c, b);
+                        ok = false;
+                        break;
+                    }
+                    if (c == src[b] && srcd2[b] != (float)inc) {
+                        std::printf("MISMATCH %s srcd2 shot %d\n", tag, b);
+                        ok = false;
+                    }
+                    for (int r = rec_head[c]; r >= 0; r = rec_next[r])
+                        if (rec[(int64_t)r * Bp + b] != v || rec_lo[(int64_t)r * Bp + b] != bf2f(lb)) {
+                            std::printf("MISMATCH %s trace %d shot %d\n", tag, r, b);
+                            ok = false;
+                        }
+                }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+template <int LB, int CX, int CY>
+bool fwd3d_cases(uint32_t seed) {
+    bool ok = true;
+    ok &= check_fwd3d<LB, CX, CY>(9, 7, 11, 2, 4, seed++);
+    ok &= check_fwd3d<LB, CX, CY>(5, 6, 13, 1, 64, seed++);
+    ok &= check_fwd3d<LB, CX, CY>(12, 9, 5, 2, 8, seed++);
+    return ok;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -250,6 +401,12 @@ int main(int argc, char **argv) {
         ok &= adj_cases<32, 4>(seed + 700);
         ok &= adj_cases<64, 4>(seed + 800);
         ok &= adj_cases<64, 8>(seed + 900);
+    }
+    if (only.empty() || only == "fwd3d") {
+        ok &= fwd3d_cases<64, 4, 2>(seed + 1000);
+        ok &= fwd3d_cases<64, 4, 4>(seed + 1100);
+        ok &= fwd3d_cases<32, 8, 2>(seed + 1200);
+        ok &= fwd3d_cases<16, 4, 2>(seed + 1300);
     }
     std::printf(ok ? "ALL EQUAL\n" : "DIFFERENCES FOUND\n");
     return ok ? 0 : 1;

@@ -35,7 +35,7 @@ def test_autograd_matches_chain_rule_and_oracle():
     dc_ref = -2.0 * g_ref / c0 ** 3
     with _plan_for(prob) as plan:
         w = torch.tensor(prob.wavelet, device=dev)
-        d = torch.tensor(dobs, dtype=torch.float32, device=dev)
+        d = torch.tensor(dobs, dtype=torch.float64, device=dev)
         c = torch.tensor(c0, device=dev, requires_grad=True)
         J = misfit_from_velocity(c, plan, w, d)
         J.backward()
@@ -69,7 +69,7 @@ def test_autograd_directional_finite_difference():
     with _plan_for(prob) as plan:
         c = torch.tensor(c0, device=dev, requires_grad=True)
         J = misfit_from_velocity(c, plan, torch.tensor(prob.wavelet, device=dev),
-                                 torch.tensor(dobs, dtype=torch.float32, device=dev))
+                                 torch.tensor(dobs, dtype=torch.float64, device=dev))
         J.backward()
         ad = float((c.grad.cpu().numpy() * v).sum())
     eps = 1.0  # m/s at the bump peaks
@@ -92,7 +92,7 @@ def test_gradient_descent_reduces_misfit():
     with _plan_for(prob) as plan:
         c, hist = gradient_descent(torch.tensor(np.sqrt(1 / prob.m0), device=dev), plan,
                                    torch.tensor(prob.wavelet, device=dev),
-                                   torch.tensor(dobs, dtype=torch.float32, device=dev),
+                                   torch.tensor(dobs, dtype=torch.float64, device=dev),
                                    steps=5, step_frac=0.005, c_bounds=(1400.0, 1800.0))
     print("[case] GD misfit history", hist)
     assert hist[-1] < hist[0]

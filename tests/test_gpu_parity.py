@@ -100,11 +100,11 @@ def test_gcn_2d_golden(method):
     assert e_J < TOL and e_g < TOL
 
 
-# 3D FP32 gradients near a point source: the field's singularity makes the
-# Laplacian and source term cancel in the cells around it.  Measured bounds:
-# test_gradient_3d.py's grid (about 3 points per shortest wavelength) and a
-# 10-points-per-wavelength grid.  Traces and misfits meet 1e-5 on both.
-TOL_3D = {"fmc_3d.npz": 1e-3, "fmc_3d_fine.npz": 1e-4}
+# 3D gradients sum lam^k d2p^k over time with strong cancellation: a forward
+# increment rounded to FP32 at every step leaves a gradient error of several
+# 1e-4 on test_gradient_3d.py's grid.  The 3D forward update is formed in FP64
+# from the stored hi + lo state, which brings it within the 1e-5 bar.
+TOL_3D = {"fmc_3d.npz": TOL, "fmc_3d_fine.npz": TOL}
 
 
 @pytest.mark.parametrize("name", ["fmc_3d.npz", "fmc_3d_fine.npz"])
@@ -296,7 +296,7 @@ def _oracle_vs_device(prob, method="batched", ckpt_every=0):
                              torch.tensor(prob.wavelet, device=dev)).cpu().numpy()
         J, g = plan.misfit_and_gradient(torch.tensor(prob.m0, device=dev),
                                         torch.tensor(prob.wavelet, device=dev),
-                                        torch.tensor(dobs, dtype=torch.float32, device=dev))
+                                        torch.tensor(dobs, dtype=torch.float64, device=dev))
     e = (rel_l2(d, dobs), abs(J.item() - J_ref) / J_ref, rel_l2(g.cpu().numpy(), g_ref))
     print(f"{prob.name}: traces {e[0]:.2e}  J {e[1]:.2e}  gradient {e[2]:.2e}")
     return e
@@ -322,9 +322,8 @@ def test_c2_subset_plain_vs_oracle():
 
 
 def test_c4_reduced_vs_oracle():
-    """C4 spacing, time step and array on a 64^3 grid, 300 steps, 4 shots.
-    Traces and J at 1e-5; the gradient near the 3D point sources is measured
-    at 2.8e-4 (bound 5e-4)."""
+    """C4 spacing, time step and array on a 64^3 grid, 300 steps, 4 shots,
+    checkpointed every 50 steps."""
     from openusct_amd import configs
     e = _oracle_vs_device(configs.c4(n=64, nt_scale=0.15, shots=4), ckpt_every=50)
-    assert e[0] < TOL and e[1] < TOL and e[2] < 5e-4
+    assert e[0] < TOL and e[1] < TOL and e[2] < TOL

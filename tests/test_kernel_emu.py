@@ -5,7 +5,9 @@ g++ against a stand-in for the HIP device API (tests/cpu_emu/hip/hip_runtime.h)
 and runs the kernels lane by lane under AddressSanitizer.  It checks that the
 two-level (temporally blocked) forward and adjoint kernels produce bit for bit
 what two single-level launches produce -- fields, bfloat16 low parts, traces
-and gradient accumulators -- over odd grids, sources and receivers on borders
+and gradient accumulators -- and that the 2D (with and without the sponge)
+and 3D forward kernels equal a direct loop over cells of their FP64 update,
+over odd grids, sources and receivers on borders
 and strip edges, duplicate receivers and every lane-group / tile shape the
 autotuner may pick, and that no access leaves the guard zones the library
 allocates (kGuardCells in usct.hip).
@@ -44,7 +46,7 @@ def emu_binary(tmp_path_factory):
 
 
 @pytest.mark.parametrize("seed", [1, 2])
-@pytest.mark.parametrize("which", ["fwd", "adj"])
+@pytest.mark.parametrize("which", ["fwd", "adj", "fwd3d"])
 def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
     r = subprocess.run([emu_binary, str(seed), which], capture_output=True, text=True,
@@ -52,4 +54,4 @@ def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     tail = "\n".join((r.stdout + r.stderr).splitlines()[-20:])
     assert r.returncode == 0, tail
     assert "ALL EQUAL" in r.stdout, tail
-    assert r.stdout.count("ok ") == 65, tail
+    assert r.stdout.count("ok ") == {"fwd": 75, "adj": 65, "fwd3d": 12}[which], tail
