@@ -44,7 +44,35 @@ WORKLOADS = {
     "C4": "3D cylinder 256^3, dx=0.25mm, dt=30ns, nt=2000, 8 rings x 64 elements R=28mm, "
           "64 shots, Ricker 500kHz, FP32 fields, checkpoint every 50 steps; one step = one "
           "misfit+gradient",
+    "C5": "serving mix: Poisson-arriving FWI-gradient and forward requests (50/50), grids 2D "
+          "{256^2, 512^2, 1024^2} and 3D {128^3, 192^3, 256^3}, 16-256 shots, seeded phantoms, "
+          "admitted by HBM footprint; value = aggregate Gpt-updates/s over the run",
 }
+
+
+def serving(n_requests, nt_scale, rate=150.0, seed=5, with_3d=True):
+    """Run the serving mix (make_mix: seed 5, arrivals for ~80 % utilisation at
+    `rate` Gpt-updates/s) through the continuous-batching server; observed data
+    of gradient requests is synthesised on the device first (not timed)."""
+    from openusct_amd.serve import Server, fill_observed, make_mix, summarize
+
+    reqs = make_mix(n_requests, seed=seed, nt_scale=nt_scale, rate=rate, with_3d=with_3d)
+    t0 = time.perf_counter()
+    fill_observed(reqs)
+    print(f"[bench] serving: {n_requests} requests, observed data ready in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    step = max(1, n_requests // 10)
+
+    def progress(k, n):
+        if k % step == 0 or k == n:
+            print(f"[bench] serving: {k}/{n} done", file=sys.stderr, flush=True)
+
+    res = Server().run(reqs, progress=progress)
+    s = summarize(res)
+    s["gradient_requests"] = sum(1 for r in reqs if r.kind == "gradient")
+    s["sample"] = (f"{n_requests} requests of the serving mix (seed 5), time axes x{nt_scale}, "
+                   f"Poisson arrivals for ~80 % load at {rate:g} Gpt-updates/s")
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
 
 
 def workload(name="C2"):
@@ -118,6 +146,14 @@ def main():
                     help="skip the per-launch event timing pass (for counter runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run an elementwise copy of this many MiB (counter calibration)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the serving sample reported beside the main line "
+                         "(C5: of the serving run itself); 0 = skip")
+    ap.add_argument("--gd-steps", type=int, default=-1,
+                    help="steps of gradient descent through the sound-speed autograd Function "
+                         "after the timed gradients (default: 5 for C3, else 0)")
+    ap.add_argument("--serve-nt-scale", type=float, default=0.05,
+                    help="time-axis scale of the serving requests")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -138,6 +174,17 @@ def main():
             y = x * 1.0
         torch.cuda.synchronize()
         del x, y
+    if args.workload == "C5":
+        sv = serving(args.serve_requests or 100, args.serve_nt_scale)
+        print(json.dumps({
+            "metric": "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency",
+            "value": sv["gpt_updates_per_s"], "unit": "Gpt-updates/s", "n_gpus": 1, "steps": 1,
+            "warmup": 0, "ms_per_step": round(sv["makespan_s"] * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+            "data": "synthetic (seeded phantoms, observed data simulated on the device)",
+            "config": {"workload": WORKLOADS["C5"]}, "serving": sv,
+            "roofline": None, "cpu_baseline": None}), flush=True)
+        return
     prob = workload(args.workload)
     shape, h, dt, nt = prob.shape, prob.h, prob.dt, prob.nt
     N = int(np.prod(shape))
@@ -171,6 +218,22 @@ def main():
     value = upd / (ms / 1e3) / 1e9
     print(f"[bench] batched {ms:.1f} ms/gradient, {value:.1f} Gpt-updates/s", file=sys.stderr,
           flush=True)
+
+    gd = None
+    gd_steps = args.gd_steps if args.gd_steps >= 0 else (5 if args.workload == "C3" else 0)
+    if gd_steps:
+        from openusct_amd.autograd import gradient_descent
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c0 = torch.sqrt(1.0 / md)
+        _, hist = gradient_descent(c0, plan, wd, dobs, steps=gd_steps, step_frac=0.005,
+                                   c_bounds=(1400.0, 1800.0))
+        torch.cuda.synchronize()
+        gd = {"steps": gd_steps, "misfit_history": hist,
+              "seconds": round(time.perf_counter() - t0, 2),
+              "through": "openusct_amd.autograd.SoundSpeedMisfit (c -> J, backward = device adjoint)"}
+        print(f"[bench] gradient descent {gd_steps} steps in {gd['seconds']} s: {hist}",
+              file=sys.stderr, flush=True)
 
     if args.no_profile:
         prof = {"fwd_ms": float("nan"), "adj_ms": float("nan"), "n_fwd": 0, "n_adj": 0}
@@ -211,6 +274,12 @@ def main():
         print(f"[bench] plain {tp * 1e3:.1f} ms/gradient over {ps} shots, {plain:.1f} "
               f"Gpt-updates/s", file=sys.stderr, flush=True)
         pplan.close()
+
+    sv = None
+    if args.serve_requests:
+        sv = serving(args.serve_requests, args.serve_nt_scale)
+        print(f"[bench] serving p50 {sv['p50_s']:.3f} s, p99 {sv['p99_s']:.3f} s, "
+              f"{sv['gpt_updates_per_s']:.1f} Gpt-updates/s", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -272,6 +341,8 @@ def main():
             "gradient_level": round(value * BYTES_PER_UPDATE, 1),
         },
         "cpu_baseline": cpu,
+        "serving": sv,
+        "gradient_descent": gd,
     }
     print(json.dumps(line), flush=True)
 

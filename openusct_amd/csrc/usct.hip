@@ -783,47 +783,54 @@ int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
 
 int choose_batched(usct_plan *P, int ckpt_every) {
     const int B = P->n_tx;
-    // lanes per shot group: the widest group with at most 1/8 padding waste
-    int lb = 64;
-    for (int c : {64, 32, 16}) {
-        lb = c;
-        const int64_t pad = cdiv(B, c) * c;
-        if ((pad - B) * 8 <= pad) break;
-    }
-    P->tf.LB = P->ta.LB = lb;
     const int nt = P->nt;
-    const int64_t total_groups = cdiv(B, lb);
+    // HBM bytes streamed per shot-cell and time level (forward, adjoint,
+    // re-computed forward): the two-level 2D kernels move about 13-14 bytes per
+    // level, the single-level ones 18-20.
+    const bool x2 = P->ndim == 2 && !P->sponge && !(P->flags & USCT_FLAG_NO_FUSE);
+    const double b_fwd = x2 ? 13.0 : 19.0, b_adj = x2 ? 14.0 : 17.0;
     MemPlan best{0, 0, 0, 0, 0, 1e300};
-    // Try shot-group counts per pass from all groups down to one.
-    for (int64_t gpp = total_groups; gpp >= 1; --gpp) {
-        const int Bp = (int)(gpp * lb);
-        const int passes = (int)cdiv(total_groups, gpp);
-        // candidate K values: whole history, requested, and the memory minimiser
-        std::vector<int> Ks;
-        if (ckpt_every > 0) Ks.push_back(std::min(std::max(ckpt_every, 2), std::max(nt - 1, 2)));
-        else {
-            Ks.push_back(std::max(nt - 1, 2));
-            int kmin = std::max(2, (int)std::lround(std::sqrt(2.0 * std::max(nt - 1, 1))));
-            Ks.push_back(kmin);
-        }
-        for (int K : Ks) {
-            const int NS = (int)std::max<int64_t>(1, cdiv(nt - 1, K));
-            const int64_t bytes = batched_bytes(P, Bp, K, NS, !(P->flags & USCT_FLAG_FORWARD_ONLY));
-            if (bytes > P->budget) continue;
-            // cost: bytes streamed per shot-level (fwd 12, adj 17, recompute 12 if NS > 1),
-            // padded shots count, narrower lane groups stream a little less efficiently.
-            const double per = 19.0 + 17.0 + (NS > 1 ? 19.0 * (1.0 - 1.0 / NS) : 0.0);
-            const double eff = lb == 64 ? 1.0 : (lb == 32 ? 0.95 : 0.88);
-            // launch overhead of ~3 us per step launch, in streamed-byte units at 5 TB/s
-            const double launches = (double)passes * nt * (NS > 1 ? 3.0 : 2.0);
-            const double cost = (double)passes * Bp * per * (double)P->N * nt / eff +
-                                launches * 15e6;
-            if (cost < best.cost) best = MemPlan{Bp, passes, K, NS, bytes, cost};
+    int best_lb = 64;
+    // Every lane-group width, every number of groups per pass (padded shots
+    // count in the cost), whole history or checkpoints: the cheapest plan that
+    // fits the budget.  Narrow groups let a pass hold few shots when a wide one
+    // does not fit.
+    for (int lb : {64, 32, 16}) {
+        const int64_t total_groups = cdiv(B, lb);
+        for (int64_t gpp = total_groups; gpp >= 1; --gpp) {
+            const int Bp = (int)(gpp * lb);
+            const int passes = (int)cdiv(total_groups, gpp);
+            // candidate K values: whole history, requested, and the memory minimiser
+            std::vector<int> Ks;
+            if (ckpt_every > 0) Ks.push_back(std::min(std::max(ckpt_every, 2), std::max(nt - 1, 2)));
+            else {
+                Ks.push_back(std::max(nt - 1, 2));
+                int kmin = std::max(2, (int)std::lround(std::sqrt(2.0 * std::max(nt - 1, 1))));
+                Ks.push_back(kmin);
+            }
+            for (int K : Ks) {
+                const int NS = (int)std::max<int64_t>(1, cdiv(nt - 1, K));
+                const int64_t bytes =
+                    batched_bytes(P, Bp, K, NS, !(P->flags & USCT_FLAG_FORWARD_ONLY));
+                if (bytes > P->budget) continue;
+                const double per = b_fwd + b_adj + (NS > 1 ? b_fwd * (1.0 - 1.0 / NS) : 0.0);
+                // narrower lane groups stream a little less efficiently
+                const double eff = lb == 64 ? 1.0 : (lb == 32 ? 0.95 : 0.88);
+                // launch overhead of ~3 us per step launch, in streamed-byte units at 5 TB/s
+                const double launches = (double)passes * nt * (NS > 1 ? 3.0 : 2.0);
+                const double cost = (double)passes * Bp * per * (double)P->N * nt / eff +
+                                    launches * 15e6;
+                if (cost < best.cost) {
+                    best = MemPlan{Bp, passes, K, NS, bytes, cost};
+                    best_lb = lb;
+                }
+            }
         }
     }
     if (best.Bp == 0)
         return fail(USCT_ENOMEM, "batched plan does not fit the memory budget of %lld bytes",
                     (long long)P->budget);
+    P->tf.LB = P->ta.LB = best_lb;
     P->Bp = best.Bp;
     P->passes = best.passes;
     P->K = best.K;

@@ -818,18 +818,10 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
 // Batched 3D forward step: each lane owns a CX x CY patch and marches RZ
 // planes along z.  Row starts (y clamped, wave-uniform) and column steps go
 // into the scalar base; a lane keeps one byte offset (addressing as k_fwd2d).
-//
-// The update is done in FP64 from the full stored state P = hi + lo of all
-// seven stencil points, with S' and the source term S_src w[n-1] (sincd) in
-// FP64, and the result split back into FP32 hi + bfloat16 lo.  In 3D the
-// gradient sums lam^k d2p^k over many time levels with strong cancellation,
-// so an increment rounded to FP32 at every step (or a Laplacian of the high
-// parts only) leaves a wavefield error of a few 1e-7 that the sum turns into a
-// gradient error of several 1e-4; with the increment kept in FP64 the
-// gradient is within ~2e-6 of the FP64 reference.  CDNA FP64 vector
-// arithmetic runs at the FP32 rate, and the storage (6 bytes per value) and
-// so the HBM traffic are unchanged.  srcd2[b] records the increment applied
-// at shot b's source cell for the adjoint's imaging.
+// The update is the FP64 one of k_fwd2d (seven-point Laplacian of the full
+// hi + lo state); the own cells of the next plane are loaded one plane ahead.
+// srcd2[b] records the increment applied at shot b's source cell for the
+// adjoint's imaging.
 // ---------------------------------------------------------------------------
 
 template <int LB, int CX, int CY, bool REC>
@@ -874,6 +866,10 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         return (double)ldb(pc + e, o4) + (double)bf2f(ldb(pcl + e, o2));
     };
     double zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
+    // raw loads issued one plane ahead: own cells of plane z+1 (nh/nl) and of
+    // level n-2 on plane z (qh/ql)
+    float nh[CY][CX], qh[CY][CX];
+    uint16_t nl[CY][CX], ql[CY][CX];
     auto load_halo = [&](size_t pl) {
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
@@ -886,6 +882,20 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             zcu[j + 1][CX + 1] = P(pl, j, CX);
         }
     };
+    auto load_raw = [&](size_t pl, size_t ppl_, float (&h)[CY][CX], uint16_t (&l)[CY][CX],
+                        float (&qh_)[CY][CX], uint16_t (&ql_)[CY][CX]) {
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const size_t e = pl + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+                h[j][i] = ldb(pc + e, o4);
+                l[j][i] = ldb(pcl + e, o2);
+                const size_t eq = ppl_ + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+                qh_[j][i] = ldb(pp + eq, o4);
+                ql_[j][i] = ldb(ppl + eq, o2);
+            }
+    };
     {
         const size_t pm = (size_t)max(z0 - 1, 0) * planeE;
         const size_t p0 = (size_t)z0 * planeE;
@@ -897,13 +907,19 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                 zcu[j + 1][i + 1] = P(p0, j, i);
             }
         load_halo(p0);
+        load_raw((size_t)min(z0 + 1, nz - 1) * planeE, p0, nh, nl, qh, ql);
     }
     for (int z = z0; z < z1; ++z) {
         const size_t pnz = (size_t)min(z + 1, nz - 1) * planeE;
+        double q[CY][CX];
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
-            for (int i = 0; i < CX; ++i) zp[j][i] = P(pnz, j, i);
+            for (int i = 0; i < CX; ++i) {
+                zp[j][i] = hl(nh[j][i], nl[j][i]);
+                q[j][i] = hl(qh[j][i], ql[j][i]);
+            }
+        if (z + 1 < z1) load_raw((size_t)min(z + 2, nz - 1) * planeE, pnz, nh, nl, qh, ql);
         const bool zint = (z >= 1) && (z <= nz - 2);
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
@@ -920,15 +936,13 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     if (yint && x >= 1 && x <= nx - 2)
                         lu = lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
                                    zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
-                    const float2 s2 = ldb(spr + i, oc);
-                    double inc = ((double)s2.x + (double)s2.y) * lu;
+                    double inc = s2d(ldb(spr + i, oc)) * lu;
                     const int c = (z * ny + y) * nx + x;
                     if (c == my_src) inc = inc + my_inc;
                     const size_t ei = ro + (size_t)i * Bp;
-                    const double q = (double)ldb(pp + ei, o4) + (double)bf2f(ldb(ppl + ei, o2));
-                    const double nv = (2.0 * zcu[j + 1][i + 1] - q) + inc;
-                    const float v = (float)nv;
-                    const uint16_t lb = f2bf((float)(nv - (double)v));
+                    const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
+                    uint16_t lb;
+                    const float v = split_hl(nv, lb);
                     stb(pn + ei, o4, v);
                     stb(pnl + ei, o2, lb);
                     if (c == my_src) srcd2[b] = (float)inc;
@@ -1017,6 +1031,19 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
             fc[j + 1][CX + 1] = F(fz, j, CX);
         }
     };
+    // raw loads issued one plane ahead: own cells of lam^{k+1} and p^{k-1} on
+    // plane z+1 (an, fn) and lam^{k+2} on plane z (l2n)
+    float an[CY][CX], fn[CY][CX], l2n[CY][CX];
+    auto load_raw = [&](size_t pl, size_t pl2) {
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                an[j][i] = F(l1 + pl, j, i);
+                fn[j][i] = F(pa + pl, j, i);
+                l2n[j][i] = F(l2 + pl2, j, i);
+            }
+    };
     {
         const int zmi = max(z0 - 1, 0);
         const float *lm = l1 + (size_t)zmi * planeE, *l0 = l1 + (size_t)z0 * planeE;
@@ -1034,21 +1061,25 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                 fc[j + 1][i + 1] = F(f0, j, i);
             }
         load_halo(l0, s0, f0);
+        load_raw((size_t)min(z0 + 1, nz - 1) * planeE, (size_t)z0 * planeE);
     }
     for (int z = z0; z < z1; ++z) {
         const int zpi = min(z + 1, nz - 1);
         const float *lnz = l1 + (size_t)zpi * planeE;
         const float2 *snz = sp2 + (size_t)zpi * planeC;
         const float *fnz = pa + (size_t)zpi * planeE;
+        float l2c[CY][CX];
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
-                const float v = F(lnz, j, i);
+                const float v = an[j][i];
                 lpn[j][i] = v;
                 qp[j][i] = smul(ldb(snz + row(j) * nx + i, oc), v);
-                fp[j][i] = F(fnz, j, i);
+                fp[j][i] = fn[j][i];
+                l2c[j][i] = l2n[j][i];
             }
+        if (z + 1 < z1) load_raw((size_t)min(z + 2, nz - 1) * planeE, (size_t)zpi * planeE);
         const bool zint = (z >= 1) && (z <= nz - 2);
         float tv[CC];
 #pragma unroll
@@ -1068,7 +1099,7 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
                                                qc[j][i + 1], qc[j + 1][i + 2], qc[j + 1][i])
                                         : 0.0f;
                     const size_t ei = ro + (size_t)i * Bp;
-                    float v = leap(lc[j][i], ldb(l2 + ei, o4), lq);
+                    float v = leap(lc[j][i], l2c[j][i], lq);
                     const int c = (z * ny + y) * nx + x;
                     if (has_rec) {
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r])

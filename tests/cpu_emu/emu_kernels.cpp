@@ -373,12 +373,103 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     return ok;
 }
 
+// 3D adjoint: lam^k against a direct loop (bit for bit: the same FP32 ops in
+// the same order), and the imaging sums against a direct FP64 sum (the kernel
+// reduces over a lane group first, in another order).
+template <int LB, int CX, int CY>
+bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
+    const int Bp = LB * ng, W = 64 / LB;
+    const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
+    std::mt19937 rng(seed);
+    char tag[160];
+    std::snprintf(tag, sizeof tag, "adj3d nz=%d ny=%d nx=%d LB=%d ng=%d CX=%d CY=%d RZ=%d", nz, ny,
+                  nx, LB, ng, CX, CY, RZ);
+    std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f);
+    float2 *sp2 = alloc<float2>(N, 1);
+    for (int64_t c = 0; c < N; ++c) sp2[c] = make_float2(u(rng), tt(rng));
+    float *l1 = alloc<float>(FE, Bp), *l2 = alloc<float>(FE, Bp), *pa = alloc<float>(FE, Bp),
+          *lk = alloc<float>(FE, Bp);
+    for (int64_t i = 0; i < FE; ++i) {
+        l1[i] = d(rng);
+        l2[i] = d(rng);
+        pa[i] = d(rng);
+    }
+    int *src = alloc<int>(Bp, 0);
+    std::uniform_int_distribution<int> cz(0, nz - 1), cy(0, ny - 1), cx(0, nx - 1);
+    for (int b = 0; b < Bp; ++b) src[b] = (cz(rng) * ny + cy(rng)) * nx + cx(rng);
+    src[Bp - 1] = -1;
+    float *sinc = alloc<float>(Bp, 0), *srcd2 = alloc<float>(Bp, 0);
+    for (int b = 0; b < Bp; ++b) { sinc[b] = d(rng); srcd2[b] = d(rng); }
+    std::vector<int> rc = {0, (int)N - 1, (int)(N / 2), (int)(N / 3), (int)(N / 3)};
+    const int n_rec = (int)rc.size();
+    int *rec_head = alloc<int>(N, 0), *rec_next = alloc<int>(n_rec, 0);
+    int *row_rec = alloc<int>((int64_t)nz * ny, 0);
+    for (int64_t c = 0; c < N; ++c) rec_head[c] = -1;
+    for (int j = n_rec - 1; j >= 0; --j) {
+        rec_next[j] = rec_head[rc[j]];
+        rec_head[rc[j]] = j;
+    }
+    for (int j = 0; j < n_rec; ++j) row_rec[rc[j] / nx]++;
+    float *res = alloc<float>((int64_t)n_rec * Bp, 0);
+    for (int64_t i = 0; i < (int64_t)n_rec * Bp; ++i) res[i] = d(rng);
+    float *gh = alloc<float>(ng * N, 0), *gl = alloc<float>(ng * N, 0);
+    const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
+    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
+    emu::launch(blocks, kBlock, [&] {
+        k_adj3d<LB, CX, CY>(l1, l2, lk, pa, sp2, src, sinc, rec_head, rec_next, row_rec, res,
+                            srcd2, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
+    });
+    bool ok = true;
+    const int64_t sz = (int64_t)nx * ny;
+    std::vector<double> gref(ng * N, 0.0);
+    for (int64_t c = 0; c < N && ok; ++c) {
+        const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / sz);
+        const bool in = z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2;
+        for (int b = 0; b < Bp; ++b) {
+            auto q = [&](int64_t e) { return smul(sp2[e], l1[e * Bp + b]); };
+            auto f = [&](int64_t e) { return pa[e * Bp + b]; };
+            const float lq = in ? lap7(q(c), q(c + sz), q(c - sz), q(c + nx), q(c - nx), q(c + 1),
+                                       q(c - 1))
+                                : 0.0f;
+            float v = leap(l1[c * Bp + b], l2[c * Bp + b], lq);
+            for (int r = rec_head[c]; r >= 0; r = rec_next[r]) v = v - res[(int64_t)r * Bp + b];
+            if (std::memcmp(&v, &lk[c * Bp + b], 4) != 0) {
+                std::printf("MISMATCH %s lam cell %lld shot %d\n", tag, (long long)c, b);
+                ok = false;
+                break;
+            }
+            const float lp = in ? lap7(f(c), f(c + sz), f(c - sz), f(c + nx), f(c - nx), f(c + 1),
+                                       f(c - 1))
+                                : 0.0f;
+            float d2 = smul(sp2[c], lp);
+            if (c == src[b]) d2 = srcd2[b];
+            gref[(b / LB) * N + c] += (double)(v * d2);
+        }
+    }
+    double num = 0, den = 0;
+    for (int64_t i = 0; i < ng * N; ++i) {
+        const double e = ((double)gh[i] + (double)gl[i]) - gref[i];
+        num += e * e;
+        den += gref[i] * gref[i];
+    }
+    if (ok && !(num <= 1e-12 * den)) {
+        std::printf("MISMATCH %s imaging sums rel err %.3e\n", tag, std::sqrt(num / den));
+        ok = false;
+    }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
 template <int LB, int CX, int CY>
 bool fwd3d_cases(uint32_t seed) {
     bool ok = true;
     ok &= check_fwd3d<LB, CX, CY>(9, 7, 11, 2, 4, seed++);
     ok &= check_fwd3d<LB, CX, CY>(5, 6, 13, 1, 64, seed++);
     ok &= check_fwd3d<LB, CX, CY>(12, 9, 5, 2, 8, seed++);
+    if (CX * CY <= LB) {  // the adjoint reduces a lane's cells over its lane group
+        ok &= check_adj3d<LB, CX, CY>(9, 7, 11, 2, 4, seed++);
+        ok &= check_adj3d<LB, CX, CY>(5, 6, 13, 1, 64, seed++);
+    }
     return ok;
 }
 
