@@ -80,7 +80,7 @@ def workload(name="C2"):
     return configs.BUILDERS[name]()
 
 
-def cpu_baseline(budget_s=15.0):
+def cpu_baseline(budget_s=25.0):
     """Time the reference's own C++ core (oracle/_ref, built from its sources) on a
     bounded sample of the benchmark workload: the 512^2 / 128-element ring problem
     (nt = 1000) with its first k shots, k sized so the timed gradient takes about

@@ -45,30 +45,43 @@ def misfit_from_velocity(c, plan, wavelet, dobs, misfit_type="l2", sponge=None):
 
 
 def gradient_descent(c0, plan, wavelet, dobs, steps=5, step_frac=0.02, c_bounds=None,
-                     mask=None, misfit_type="l2"):
-    """Plain steepest descent on c through :class:`SoundSpeedMisfit`.
+                     mask=None, misfit_type="l2", n_linesearch=6):
+    """Steepest descent on c through :class:`SoundSpeedMisfit`, with the step rule
+    and halving line search of simulation/ringfwi/fwi.py:637-660 applied to c.
 
-    Each step: J = misfit(c); J.backward(); c -= step_frac * mean(c) * grad / max|grad|
-    (the step rule of simulation/ringfwi/fwi.py:642, applied to c), optionally
-    restricted to ``mask`` and clipped to ``c_bounds``.  Returns (c, history).
+    Each step: J = misfit(c); J.backward(); direction -grad (times ``mask``);
+    step = step_frac * mean(c) / max|grad|; try c - step * grad (clipped to
+    ``c_bounds``) and halve the step until the misfit decreases, at most
+    ``n_linesearch`` tries (the trial misfits are forward-only evaluations); stop
+    when no trial decreases it.  Returns (c, history of accepted misfits).
     """
     c = c0.detach().clone().to(torch.float64).requires_grad_(True)
     history = []
+    J = misfit_from_velocity(c, plan, wavelet, dobs, misfit_type)
+    J.backward()
     for _ in range(steps):
-        if c.grad is not None:
-            c.grad = None
-        J = misfit_from_velocity(c, plan, wavelet, dobs, misfit_type)
-        J.backward()
         history.append(float(J.item()))
         with torch.no_grad():
             d = c.grad if mask is None else c.grad * mask
-            dmax = d.abs().max()
-            if float(dmax) == 0.0:
+            dmax = float(d.abs().max())
+            if dmax == 0.0:
                 break
-            c -= step_frac * c.mean() * d / dmax
-            if c_bounds is not None:
-                c.clamp_(c_bounds[0], c_bounds[1])
-    with torch.no_grad():
+            step = step_frac * float(c.mean()) / dmax
+            accepted = False
+            for _ in range(n_linesearch):
+                c_try = c - step * d
+                if c_bounds is not None:
+                    c_try.clamp_(c_bounds[0], c_bounds[1])
+                J_try = plan.misfit(1.0 / (c_try * c_try), wavelet, dobs,
+                                    misfit_type=misfit_type)
+                if float(J_try.item()) < float(J.item()):
+                    accepted = True
+                    break
+                step *= 0.5
+        if not accepted:
+            break
+        c = c_try.detach().requires_grad_(True)
         J = misfit_from_velocity(c, plan, wavelet, dobs, misfit_type)
+        J.backward()
     history.append(float(J.item()))
     return c.detach(), history
