@@ -77,6 +77,9 @@ typedef struct {
     int fwd_lanes_per_group;  /* the same four for the forward kernel */
     int fwd_tile_x, fwd_tile_y;
     int fwd_rows_per_wave;
+    int tile_kind;            /* adjoint, 3D: 0 = cells per lane (tile_x x tile_y),
+                                 1 = LDS-staged work-group tile of tile_x x tile_y cells */
+    int fwd_tile_kind;        /* the same for the forward kernel */
 } usct_plan_info;
 
 /*

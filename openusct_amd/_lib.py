@@ -44,6 +44,8 @@ class PlanInfo(ctypes.Structure):
         ("fwd_tile_x", ctypes.c_int),
         ("fwd_tile_y", ctypes.c_int),
         ("fwd_rows_per_wave", ctypes.c_int),
+        ("tile_kind", ctypes.c_int),
+        ("fwd_tile_kind", ctypes.c_int),
     ]
 
     def as_dict(self):

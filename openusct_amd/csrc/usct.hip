@@ -73,7 +73,16 @@ enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2 };
 
 struct Tiling {
     int LB, CX, CY, RY;  // lanes per shot group, cells per lane (x, y), rows per wavefront
+    int lds = 0;         // 3D: 1 = LDS-staged work-group tiles of CX x CY cells (patch per
+                         // lane from lds_patch), rows = planes per work-group
 };
+
+// Lane patch of an LDS tile: 256 lanes = LB shots x (TX/PX) x (TY/PY) patches.
+inline void lds_patch(int LB, int TX, int TY, int &PX, int &PY) {
+    const int cells = TX * TY * LB / 256;  // cells per lane
+    PX = cells >= 4 ? 4 : 2;
+    PY = cells / PX;
+}
 
 struct Profile {
     std::vector<hipEvent_t> fwd_ev, adj_ev;  // start/stop pairs
@@ -95,7 +104,7 @@ struct usct_plan {
     // Tiles of the forward and of the adjoint kernels (chosen independently: the
     // field layout does not depend on them; the gradient accumulators use the
     // adjoint's lane groups).
-    Tiling tf{64, 8, 1, 32}, ta{64, 8, 1, 32};
+    Tiling tf{64, 8, 1, 32, 0}, ta{64, 8, 1, 32, 0};
     int Bp = 1;        // padded shots per pass
     int passes = 1;
     int K = 0, NS = 1;  // checkpoint interval and number of windows
@@ -266,6 +275,33 @@ void fwd3d_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
     else fwd3d_t<LB, 4, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
 }
 
+template <int LB, int TX, int TY, int PX, int PY, bool REC>
+void fwd3d_lds_t(hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P, const float *sinc,
+                 float *rec, float *rl) {
+    const Tiling &T = P->tf;
+    const int ntx = (int)cdiv(P->nx, TX), nty = (int)cdiv(P->ny, TY);
+    const int nzc = (int)cdiv(P->nz, T.RY), ng = P->Bp / LB;
+    const dim3 g((unsigned)((int64_t)ng * ntx * nty * nzc));
+    float *d2 = P->srcd2 + (sinc - P->sinc);
+    const double *sd = P->sincd + (sinc - P->sinc);
+    hipLaunchKernelGGL((k_fwd3d_lds<LB, TX, TY, PX, PY, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi,
+                       pn.hi, pc.lo, pp.lo, pn.lo, P->sp2, P->src_cell, sd, P->rec_head,
+                       P->rec_next, P->row_rec, rec, rl, d2, P->nz, P->ny, P->nx, P->Bp, T.RY, ntx,
+                       nty, nzc, ng);
+}
+
+// LDS tile shapes (the autotuner's candidates): LB 16 -> 8x8 or 16x8, LB 32 -> 8x8,
+// LB 64 -> 8x4 cells per work-group.
+template <bool REC>
+void fwd3d_lds_sel(hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P, const float *sinc,
+                   float *rec, float *rl) {
+    const Tiling &T = P->tf;
+    if (T.LB == 16 && T.CX == 16) fwd3d_lds_t<16, 16, 8, 4, 2, REC>(s, pc, pp, pn, P, sinc, rec, rl);
+    else if (T.LB == 16) fwd3d_lds_t<16, 8, 8, 2, 2, REC>(s, pc, pp, pn, P, sinc, rec, rl);
+    else if (T.LB == 32) fwd3d_lds_t<32, 8, 8, 4, 2, REC>(s, pc, pp, pn, P, sinc, rec, rl);
+    else fwd3d_lds_t<64, 8, 4, 4, 2, REC>(s, pc, pp, pn, P, sinc, rec, rl);
+}
+
 template <bool REC>
 void fwd3d_sel(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
                const float *sinc, float *rec, float *rl, const Tile &t) {
@@ -292,8 +328,13 @@ void launch_fwd_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl p
             else fwd2d_sel<false, false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
         }
     } else {
-        if (record) fwd3d_sel<true>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
-        else fwd3d_sel<false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+        if (P->tf.lds) {
+            if (record) fwd3d_lds_sel<true>(s, pc, pp, pn, P, sinc, rec, rl);
+            else fwd3d_lds_sel<false>(s, pc, pp, pn, P, sinc, rec, rl);
+        } else {
+            if (record) fwd3d_sel<true>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+            else fwd3d_sel<false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+        }
     }
 }
 
@@ -415,6 +456,19 @@ void adj3d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk
     else adj3d_t<LB, 4, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
 }
 
+template <int LB, int TX, int TY, int PX, int PY>
+void adj3d_lds_t(hipStream_t s, const float *l1, const float *l2, float *lk, const float *pa,
+                 const usct_plan *P, const float *sinc, const float *res) {
+    const Tiling &T = P->ta;
+    const int ntx = (int)cdiv(P->nx, TX), nty = (int)cdiv(P->ny, TY);
+    const int nzc = (int)cdiv(P->nz, T.RY), ng = P->Bp / LB;
+    const dim3 g((unsigned)((int64_t)ng * ntx * nty * nzc));
+    const float *d2 = P->srcd2 + (sinc - P->sinc);
+    hipLaunchKernelGGL((k_adj3d_lds<LB, TX, TY, PX, PY>), g, dim3(kBlock), 0, s, l1, l2, lk, pa,
+                       P->sp2, P->src_cell, P->rec_head, P->rec_next, P->row_rec, res, d2,
+                       P->g_hi, P->g_lo, P->nz, P->ny, P->nx, P->Bp, T.RY, ntx, nty, nzc, ng);
+}
+
 void launch_adj_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
                         float *lk, const float *pa, const float *pb, const float *pc2, int k) {
     const Tile t = tile_of(P, P->ta);
@@ -424,6 +478,12 @@ void launch_adj_batched(const usct_plan *P, hipStream_t s, const float *l1, cons
     if (P->ndim == 2) {
         if (P->sponge) adj2d_sel<true>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
         else adj2d_sel<false>(g, s, l1, l2, lk, pa, pb, pc2, P, sinc, res, t);
+    } else if (P->ta.lds) {
+        const Tiling &T = P->ta;
+        if (T.LB == 16 && T.CX == 16) adj3d_lds_t<16, 16, 8, 4, 2>(s, l1, l2, lk, pa, P, sinc, res);
+        else if (T.LB == 16) adj3d_lds_t<16, 8, 8, 2, 2>(s, l1, l2, lk, pa, P, sinc, res);
+        else if (T.LB == 32) adj3d_lds_t<32, 8, 8, 4, 2>(s, l1, l2, lk, pa, P, sinc, res);
+        else adj3d_lds_t<64, 8, 4, 4, 2>(s, l1, l2, lk, pa, P, sinc, res);
     } else {
         switch (P->ta.LB) {
             case 16: adj3d_lb<16>(g, s, l1, l2, lk, pa, P, sinc, res, t); break;
@@ -1020,15 +1080,24 @@ int autotune(usct_plan *P) {
         if (P->ndim == 2) {
             for (int lb : lbs)
                 for (int cx : {4, 8})
-                    for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry});
+                    for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry, 0});
         } else {
             for (int lb : lbs)
                 for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}})
                     for (int rz : {4, 8, 16, 32, 64}) {
                         // the adjoint reduces a lane's CX*CY cells over its lane group
                         if (adjoint && pc.first * pc.second > lb) continue;
-                        cands.push_back({lb, pc.first, pc.second, rz});
+                        cands.push_back({lb, pc.first, pc.second, rz, 0});
                     }
+            // LDS-staged work-group tiles (TX x TY cells)
+            for (int lb : lbs) {
+                std::vector<std::pair<int, int>> tiles;
+                if (lb == 16) tiles = {{8, 8}, {16, 8}};
+                else if (lb == 32) tiles = {{8, 8}};
+                else tiles = {{8, 4}};
+                for (auto tl : tiles)
+                    for (int rz : {8, 16, 32, 64}) cands.push_back({lb, tl.first, tl.second, rz, 1});
+            }
         }
         Tiling &T = adjoint ? P->ta : P->tf;
         double best = 1e300;
@@ -1037,8 +1106,9 @@ int autotune(usct_plan *P) {
             T = c;
             const double t = time_tile(P, adjoint != 0);
             if (log && *log == '1')
-                std::fprintf(stderr, "[usct tune] %s lanes %d tile %dx%d rows %d: %.3f ms\n",
-                             adjoint ? "adjoint" : "forward", c.LB, c.CX, c.CY, c.RY, t);
+                std::fprintf(stderr, "[usct tune] %s lanes %d %s %dx%d rows %d: %.3f ms\n",
+                             adjoint ? "adjoint" : "forward", c.LB, c.lds ? "lds-tile" : "patch",
+                             c.CX, c.CY, c.RY, t);
             if (t < best) { best = t; bt = c; }
         }
         T = bt;
@@ -1169,7 +1239,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     P->method = method == USCT_METHOD_PLAIN ? USCT_METHOD_PLAIN : USCT_METHOD_BATCHED;
     P->flags = flags;
     P->sponge = (flags & USCT_FLAG_SPONGE) != 0;
-    if (ndim == 3) P->tf = P->ta = Tiling{64, 4, 4, 32};
+    if (ndim == 3) P->tf = P->ta = Tiling{64, 4, 4, 32, 0};
     int rc = 0;
     auto bail = [&](int code) {
         free_plan_memory(P);
@@ -1193,7 +1263,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     if ((rc = dalloc_t(P, &P->J_acc, 1))) return bail(rc);
     if ((rc = dalloc_t(P, &P->part, P->npart))) return bail(rc);
     if (P->method == USCT_METHOD_PLAIN) {
-        P->tf = P->ta = Tiling{1, 1, 1, 1};
+        P->tf = P->ta = Tiling{1, 1, 1, 1, 0};
         P->Bp = 1;
         P->passes = n_tx;
         P->K = nt;
@@ -1238,6 +1308,8 @@ int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
     info->fwd_tile_x = P->tf.CX;
     info->fwd_tile_y = P->tf.CY;
     info->fwd_rows_per_wave = P->tf.RY;
+    info->tile_kind = P->ta.lds;
+    info->fwd_tile_kind = P->tf.lds;
     info->shots_padded = P->method == USCT_METHOD_PLAIN ? P->n_tx : P->Bp * P->passes;
     info->shot_passes = P->passes;
     info->shots_per_pass = P->Bp;

@@ -969,6 +969,431 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 }
 
 // ---------------------------------------------------------------------------
+// LDS-tiled 3D kernels.  A work-group of 256 lanes owns a TX x TY tile of cells
+// (for LB shots) and marches RZ planes along z; lane = (shot, patch), each lane
+// a PX x PY patch of the tile.  The centre plane of the tile, with a one-cell
+// halo ring (no corners), is staged in LDS (double-buffered: plane z+1 is
+// written while plane z is read; one barrier per plane), so a lane keeps only
+// its own cells of the three planes in registers and the tile's halo is read
+// from HBM/L2 once per plane, not once per lane.  The own cells and the ring of
+// the next plane are loaded one plane ahead.
+// ---------------------------------------------------------------------------
+template <int TX, int TY>
+struct Ring {  // halo cells of a TX x TY tile, without corners: r -> (hx, hy) in the padded tile
+    static constexpr int N = 2 * TX + 2 * TY;
+    static __device__ __forceinline__ void at(int r, int &hx, int &hy) {
+        if (r < TX) { hx = r + 1; hy = 0; }
+        else if (r < 2 * TX) { hx = r - TX + 1; hy = TY + 1; }
+        else if (r < 2 * TX + TY) { hx = 0; hy = r - 2 * TX + 1; }
+        else { hx = TX + 1; hy = r - 2 * TX - TY + 1; }
+    }
+};
+
+// 3D forward level (FP64 update of k_fwd2d / k_fwd3d) with an LDS-staged tile.
+// Grid: ng * ntx * nty * nzc work-groups of 256 lanes.
+template <int LB, int TX, int TY, int PX, int PY, bool REC>
+__global__ void __launch_bounds__(kBlock)
+k_fwd3d_lds(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
+            const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
+            uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+            const int *__restrict__ src_cell, const double *__restrict__ sincd,
+            const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+            const int *__restrict__ row_rec, float *__restrict__ rec, float *__restrict__ rec_lo,
+            float *__restrict__ srcd2, int nz, int ny, int nx, int Bp, int RZ, int ntx, int nty,
+            int nzc, int ng) {
+    constexpr int TCX = TX / PX, TCY = TY / PY;
+    static_assert(TCX * PX == TX && TCY * PY == TY && TCX * TCY * LB == kBlock, "tile shape");
+    constexpr int HX = TX + 2, HY = TY + 2;
+    constexpr int RN = Ring<TX, TY>::N * LB;            // ring entries (cell, shot)
+    constexpr int RPL = (RN + kBlock - 1) / kBlock;     // ring entries per lane
+    __shared__ double sh[2][HY * HX * LB];
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int sg = blk % ng;
+    int t = blk / ng;
+    const int txi = t % ntx;
+    t /= ntx;
+    const int tyi = t % nty, zc = t / nty;
+    if (zc >= nzc) return;  // uniform over the work-group
+    const int tid = threadIdx.x;
+    const int bl = tid % LB, col = tid / LB;
+    const int cx = col % TCX, cy = col / TCX;
+    const int b = sg * LB + bl;
+    const int X0 = txi * TX, Y0 = tyi * TY;
+    const int x0 = X0 + cx * PX, y0 = Y0 + cy * PY;
+    const int z0 = zc * RZ, z1 = min(z0 + RZ, nz);
+    const int my_src = src_cell[b];
+    const double my_inc = sincd[b];
+    const size_t planeE = (size_t)nx * ny * Bp;
+    // Per-lane 32-bit byte offsets from the plane base shifted back by one cell
+    // (so the halo column x = -1 is offset 0): own cells (y clamped) and ring entries.
+    auto off = [&](int y, int x, int shot) -> unsigned {
+        return (unsigned)(((min(max(y, 0), ny - 1) * nx + x + 1) * Bp + shot));
+    };
+    unsigned oo[PY][PX], ro[RPL];
+    int r_hx[RPL], r_hy[RPL];
+    bool r_on[RPL];
+#pragma unroll
+    for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < PX; ++ii) oo[jj][ii] = off(y0 + jj, x0 + ii, b);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        const int e = tid + q * kBlock;
+        r_on[q] = e < RN;
+        Ring<TX, TY>::at(r_on[q] ? e / LB : 0, r_hx[q], r_hy[q]);
+        ro[q] = off(Y0 + r_hy[q] - 1, X0 + r_hx[q] - 1, sg * LB + e % LB);
+    }
+    auto sidx = [&](int hy, int hx, int shot) { return (hy * HX + hx) * LB + shot; };
+    // plane bases (wave-uniform), one cell before the plane
+    auto F4 = [&](const float *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - Bp; };
+    auto F2 = [&](const uint16_t *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - Bp; };
+    auto W4 = [&](float *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - Bp; };
+    auto W2 = [&](uint16_t *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - Bp; };
+    double zm[PY][PX], zcn[PY][PX], zp[PY][PX];
+    float nh[PY][PX], qh[PY][PX], rh[RPL];
+    uint16_t nl[PY][PX], ql[PY][PX], rl[RPL];
+    // raw loads of plane zn (own cells, ring) and of level n-2 on plane zq (own cells)
+    auto load_raw = [&](int zn, int zq) {
+        const float *a = F4(pc, zn), *aq = F4(pp, zq);
+        const uint16_t *al = F2(pcl, zn), *aql = F2(ppl, zq);
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                nh[jj][ii] = ldb(a, oo[jj][ii] * 4u);
+                nl[jj][ii] = ldb(al, oo[jj][ii] * 2u);
+                qh[jj][ii] = ldb(aq, oo[jj][ii] * 4u);
+                ql[jj][ii] = ldb(aql, oo[jj][ii] * 2u);
+            }
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            if (!r_on[q]) continue;
+            rh[q] = ldb(a, ro[q] * 4u);
+            rl[q] = ldb(al, ro[q] * 2u);
+        }
+    };
+    // put plane (own cells, ring from rh/rl) into buffer s
+    auto stage = [&](int s, const double (&own)[PY][PX]) {
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii)
+                sh[s][sidx(cy * PY + jj + 1, cx * PX + ii + 1, bl)] = own[jj][ii];
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            if (!r_on[q]) continue;
+            const int e = tid + q * kBlock;
+            sh[s][sidx(r_hy[q], r_hx[q], e % LB)] = hl(rh[q], rl[q]);
+        }
+    };
+    {
+        const float *am = F4(pc, max(z0 - 1, 0));
+        const uint16_t *aml = F2(pcl, max(z0 - 1, 0));
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii)
+                zm[jj][ii] = hl(ldb(am, oo[jj][ii] * 4u), ldb(aml, oo[jj][ii] * 2u));
+        load_raw(z0, z0);  // plane z0 (staged now) and level n-2 on plane z0
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) zcn[jj][ii] = hl(nh[jj][ii], nl[jj][ii]);
+        stage(0, zcn);
+        load_raw(min(z0 + 1, nz - 1), z0);
+        __syncthreads();
+    }
+    for (int z = z0; z < z1; ++z) {
+        const int cur = (z - z0) & 1;
+        double q[PY][PX];
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                zp[jj][ii] = hl(nh[jj][ii], nl[jj][ii]);
+                q[jj][ii] = hl(qh[jj][ii], ql[jj][ii]);
+            }
+        if (z + 1 < z1) {
+            stage(cur ^ 1, zp);
+            load_raw(min(z + 2, nz - 1), z + 1);
+        }
+        const bool zint = (z >= 1) && (z <= nz - 2);
+        float *on = W4(pn, z);
+        uint16_t *onl = W2(pnl, z);
+        const float2 *spz = sp2 + (size_t)z * nx * ny;
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj) {
+            const int y = y0 + jj;
+            const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const bool has_rec = REC && y < ny && row_rec[z * ny + y] > 0;
+            const int ly = cy * PY + jj + 1;
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                const int x = x0 + ii;
+                const int lx = cx * PX + ii + 1;
+                if (x < nx && y < ny) {
+                    double lu = 0.0;
+                    if (yint && x >= 1 && x <= nx - 2) {
+                        const double ym = jj > 0 ? zcn[jj - 1][ii] : sh[cur][sidx(ly - 1, lx, bl)];
+                        const double yp = jj < PY - 1 ? zcn[jj + 1][ii] : sh[cur][sidx(ly + 1, lx, bl)];
+                        const double xm = ii > 0 ? zcn[jj][ii - 1] : sh[cur][sidx(ly, lx - 1, bl)];
+                        const double xp = ii < PX - 1 ? zcn[jj][ii + 1] : sh[cur][sidx(ly, lx + 1, bl)];
+                        lu = lap7d(zcn[jj][ii], zp[jj][ii], zm[jj][ii], yp, ym, xp, xm);
+                    }
+                    const int c = (z * ny + y) * nx + x;
+                    double inc = s2d(spz[y * nx + x]) * lu;
+                    if (c == my_src) inc = inc + my_inc;
+                    const double nv = (2.0 * zcn[jj][ii] - q[jj][ii]) + inc;
+                    uint16_t lb;
+                    const float v = split_hl(nv, lb);
+                    stb(on, oo[jj][ii] * 4u, v);
+                    stb(onl, oo[jj][ii] * 2u, lb);
+                    if (c == my_src) srcd2[b] = (float)inc;
+                    if (has_rec) {
+                        for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
+                            rec[(size_t)r * Bp + b] = v;
+                            rec_lo[(size_t)r * Bp + b] = bf2f(lb);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                zm[jj][ii] = zcn[jj][ii];
+                zcn[jj][ii] = zp[jj][ii];
+            }
+        __syncthreads();
+    }
+}
+
+// 3D adjoint level with the imaging condition (as k_adj3d), LDS-staged tiles of
+// S' lam^{k+1} and p^{k-1}.  Grid as k_fwd3d_lds.
+template <int LB, int TX, int TY, int PX, int PY>
+__global__ void __launch_bounds__(kBlock)
+k_adj3d_lds(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
+            const float *__restrict__ pa, const float2 *__restrict__ sp2,
+            const int *__restrict__ src_cell, const int *__restrict__ rec_head,
+            const int *__restrict__ rec_next, const int *__restrict__ row_rec,
+            const float *__restrict__ res, const float *__restrict__ srcd2,
+            float *__restrict__ g_hi, float *__restrict__ g_lo, int nz, int ny, int nx, int Bp,
+            int RZ, int ntx, int nty, int nzc, int ng) {
+    constexpr int TCX = TX / PX, TCY = TY / PY, PC = PX * PY;
+    static_assert(TCX * PX == TX && TCY * PY == TY && TCX * TCY * LB == kBlock, "tile shape");
+    static_assert(PC <= LB, "a patch is reduced over its lane group");
+    constexpr int HX = TX + 2, HY = TY + 2;
+    constexpr int RN = Ring<TX, TY>::N * LB;
+    constexpr int RPL = (RN + kBlock - 1) / kBlock;
+    __shared__ float shq[2][HY * HX * LB], shf[2][HY * HX * LB];
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int sg = blk % ng;
+    int t = blk / ng;
+    const int txi = t % ntx;
+    t /= ntx;
+    const int tyi = t % nty, zc = t / nty;
+    if (zc >= nzc) return;  // uniform over the work-group
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int bl = tid % LB, col = tid / LB;
+    const int cx = col % TCX, cy = col / TCX;
+    const int b = sg * LB + bl;
+    const int X0 = txi * TX, Y0 = tyi * TY;
+    const int x0 = X0 + cx * PX, y0 = Y0 + cy * PY;
+    const int z0 = zc * RZ, z1 = min(z0 + RZ, nz);
+    const int my_src = src_cell[b];
+    const float my_d2 = srcd2[b];
+    const size_t planeE = (size_t)nx * ny * Bp, planeC = (size_t)nx * ny;
+    const size_t N = planeC * nz;
+    // per-lane 32-bit offsets (elements) from a plane base shifted back by one
+    // cell: own cells, ring entries; cell offsets likewise for S'
+    auto off = [&](int y, int x, int shot) -> unsigned {
+        return (unsigned)(((min(max(y, 0), ny - 1) * nx + x + 1) * Bp + shot));
+    };
+    auto coff = [&](int y, int x) -> unsigned {
+        return (unsigned)(min(max(y, 0), ny - 1) * nx + x + 1);
+    };
+    unsigned oo[PY][PX], oc[PY][PX], ro[RPL], rc[RPL];
+    int r_hx[RPL], r_hy[RPL];
+    bool r_on[RPL];
+#pragma unroll
+    for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < PX; ++ii) {
+            oo[jj][ii] = off(y0 + jj, x0 + ii, b);
+            oc[jj][ii] = coff(y0 + jj, x0 + ii);
+        }
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        const int e = tid + q * kBlock;
+        r_on[q] = e < RN;
+        Ring<TX, TY>::at(r_on[q] ? e / LB : 0, r_hx[q], r_hy[q]);
+        ro[q] = off(Y0 + r_hy[q] - 1, X0 + r_hx[q] - 1, sg * LB + e % LB);
+        rc[q] = coff(Y0 + r_hy[q] - 1, X0 + r_hx[q] - 1);
+    }
+    auto sidx = [&](int hy, int hx, int shot) { return (hy * HX + hx) * LB + shot; };
+    auto F4 = [&](const float *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - Bp; };
+    auto W4 = [&](float *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - Bp; };
+    auto S2 = [&](int z) { return sp2 + (ptrdiff_t)z * (ptrdiff_t)planeC - 1; };
+    float qm[PY][PX], qcn[PY][PX], qp[PY][PX], lcn[PY][PX];
+    float fm[PY][PX], fcn[PY][PX], fp[PY][PX];
+    // raw loads one plane ahead: lam^{k+1} and p^{k-1} of plane zn (own cells and
+    // ring), lam^{k+2} of plane z2 (own cells)
+    float ln[PY][PX], fnr[PY][PX], l2n[PY][PX], rlam[RPL], rf[RPL];
+    auto load_raw = [&](int zn, int z2) {
+        const float *a = F4(l1, zn), *f = F4(pa, zn), *a2 = F4(l2, z2);
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                ln[jj][ii] = ldb(a, oo[jj][ii] * 4u);
+                fnr[jj][ii] = ldb(f, oo[jj][ii] * 4u);
+                l2n[jj][ii] = ldb(a2, oo[jj][ii] * 4u);
+            }
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            if (!r_on[q]) continue;
+            rlam[q] = ldb(a, ro[q] * 4u);
+            rf[q] = ldb(f, ro[q] * 4u);
+        }
+    };
+    // stage plane zn (S' lam and p, own cells and ring) into buffer s
+    auto stage = [&](int s, int zn, const float (&qo)[PY][PX], const float (&fo)[PY][PX]) {
+        const float2 *sz = S2(zn);
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                const int si = sidx(cy * PY + jj + 1, cx * PX + ii + 1, bl);
+                shq[s][si] = qo[jj][ii];
+                shf[s][si] = fo[jj][ii];
+            }
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            if (!r_on[q]) continue;
+            const int e = tid + q * kBlock;
+            const int si = sidx(r_hy[q], r_hx[q], e % LB);
+            shq[s][si] = smul(ldb(sz, rc[q] * 8u), rlam[q]);
+            shf[s][si] = rf[q];
+        }
+    };
+    auto own_q = [&](int zn, float (&qo)[PY][PX]) {
+        const float2 *sz = S2(zn);
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) qo[jj][ii] = smul(ldb(sz, oc[jj][ii] * 8u), ln[jj][ii]);
+    };
+    {
+        const int zmi = max(z0 - 1, 0);
+        const float *am = F4(l1, zmi), *fmz = F4(pa, zmi);
+        const float2 *sm = S2(zmi);
+#pragma unroll
+        for (int j = 0; j < PY; ++j)
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                qm[j][i] = smul(ldb(sm, oc[j][i] * 8u), ldb(am, oo[j][i] * 4u));
+                fm[j][i] = ldb(fmz, oo[j][i] * 4u);
+            }
+        load_raw(z0, z0);
+        own_q(z0, qcn);
+#pragma unroll
+        for (int j = 0; j < PY; ++j)
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                lcn[j][i] = ln[j][i];
+                fcn[j][i] = fnr[j][i];
+            }
+        stage(0, z0, qcn, fcn);
+        load_raw(min(z0 + 1, nz - 1), z0);
+        __syncthreads();
+    }
+    for (int z = z0; z < z1; ++z) {
+        const int cur = (z - z0) & 1;
+        const int zpi = min(z + 1, nz - 1);
+        float l2c[PY][PX], lnx[PY][PX];
+        own_q(zpi, qp);
+#pragma unroll
+        for (int j = 0; j < PY; ++j)
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                fp[j][i] = fnr[j][i];
+                l2c[j][i] = l2n[j][i];
+                lnx[j][i] = ln[j][i];
+            }
+        if (z + 1 < z1) {
+            stage(cur ^ 1, zpi, qp, fp);
+            load_raw(min(z + 2, nz - 1), z + 1);
+        }
+        const bool zint = (z >= 1) && (z <= nz - 2);
+        float tv[PC];
+#pragma unroll
+        for (int j = 0; j < PY; ++j) {
+            const int y = y0 + j;
+            const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const bool has_rec = y < ny && row_rec[z * ny + y] > 0;
+            const int ly = cy * PY + j + 1;
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                const int x = x0 + i;
+                const int lx = cx * PX + i + 1;
+                tv[j * PX + i] = 0.0f;
+                if (x < nx && y < ny) {
+                    const bool in = yint && x >= 1 && x <= nx - 2;
+                    float lq = 0.0f, lp = 0.0f;
+                    if (in) {
+                        const float qym = j > 0 ? qcn[j - 1][i] : shq[cur][sidx(ly - 1, lx, bl)];
+                        const float qyp = j < PY - 1 ? qcn[j + 1][i] : shq[cur][sidx(ly + 1, lx, bl)];
+                        const float qxm = i > 0 ? qcn[j][i - 1] : shq[cur][sidx(ly, lx - 1, bl)];
+                        const float qxp = i < PX - 1 ? qcn[j][i + 1] : shq[cur][sidx(ly, lx + 1, bl)];
+                        lq = lap7(qcn[j][i], qp[j][i], qm[j][i], qyp, qym, qxp, qxm);
+                        const float fym = j > 0 ? fcn[j - 1][i] : shf[cur][sidx(ly - 1, lx, bl)];
+                        const float fyp = j < PY - 1 ? fcn[j + 1][i] : shf[cur][sidx(ly + 1, lx, bl)];
+                        const float fxm = i > 0 ? fcn[j][i - 1] : shf[cur][sidx(ly, lx - 1, bl)];
+                        const float fxp = i < PX - 1 ? fcn[j][i + 1] : shf[cur][sidx(ly, lx + 1, bl)];
+                        lp = lap7(fcn[j][i], fp[j][i], fm[j][i], fyp, fym, fxp, fxm);
+                    }
+                    float v = leap(lcn[j][i], l2c[j][i], lq);
+                    const int c = (z * ny + y) * nx + x;
+                    if (has_rec) {
+                        for (int r = rec_head[c]; r >= 0; r = rec_next[r])
+                            v = v - res[(size_t)r * Bp + b];
+                    }
+                    stb(W4(lk, z), oo[j][i] * 4u, v);
+                    float d2 = smul(ldb(S2(z), oc[j][i] * 8u), lp);
+                    if (c == my_src) d2 = my_d2;
+                    tv[j * PX + i] = v * d2;
+                }
+            }
+        }
+        const float s = reduce_scatter<LB, PC>(tv, lane);
+        if ((bl & (LB / PC - 1)) == 0) {
+            const int ci = sum_cell<LB, PC>(lane);
+            const int x = x0 + ci % PX, y = y0 + ci / PX;
+            if (x < nx && y < ny) {
+                const size_t gi = (size_t)sg * N + ((size_t)z * ny + y) * nx + x;
+                float hi = g_hi[gi], lo = g_lo[gi];
+                neumaier(hi, lo, s);
+                g_hi[gi] = hi;
+                g_lo[gi] = lo;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < PY; ++j)
+#pragma unroll
+            for (int i = 0; i < PX; ++i) {
+                qm[j][i] = qcn[j][i];
+                qcn[j][i] = qp[j][i];
+                lcn[j][i] = lnx[j][i];
+                fm[j][i] = fcn[j][i];
+                fcn[j][i] = fp[j][i];
+            }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched 3D adjoint step with the imaging condition (no sponge in 3D:
 // the reference sponge is 2D only, solver.py:30).  Addressing as k_fwd3d.
 // ---------------------------------------------------------------------------

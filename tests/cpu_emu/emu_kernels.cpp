@@ -291,14 +291,16 @@ bool adj_cases(uint32_t seed) {
 
 // 3D forward: k_fwd3d against a direct loop over cells of the same FP64 update
 // (full hi + lo state, FP64 S' and source term, split into FP32 hi + bf16 lo).
-template <int LB, int CX, int CY>
+// KIND 0: k_fwd3d with CX x CY patches per lane; KIND 1: k_fwd3d_lds with a
+// TX x TY tile per work-group (CX, CY = TX, TY) of PX x PY patches per lane.
+template <int LB, int CX, int CY, int KIND = 0, int PX = 1, int PY = 1>
 bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     const int Bp = LB * ng, W = 64 / LB;
     const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
     std::mt19937 rng(seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, "fwd3d nz=%d ny=%d nx=%d LB=%d ng=%d CX=%d CY=%d RZ=%d", nz, ny,
-                  nx, LB, ng, CX, CY, RZ);
+    std::snprintf(tag, sizeof tag, "%s nz=%d ny=%d nx=%d LB=%d ng=%d C/T=%dx%d P=%dx%d RZ=%d",
+                  KIND ? "fwd3d-lds" : "fwd3d", nz, ny, nx, LB, ng, CX, CY, PX, PY, RZ);
     std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
         dl(-1e-7f, 1e-7f);
     float2 *sp2 = alloc<float2>(N, 1);
@@ -330,13 +332,24 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     }
     for (int j = 0; j < n_rec; ++j) row_rec[rc[j] / nx]++;
     float *rec = alloc<float>((int64_t)n_rec * Bp, 0), *rec_lo = alloc<float>((int64_t)n_rec * Bp, 0);
-    const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
-    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
-    emu::launch(blocks, kBlock, [&] {
-        k_fwd3d<LB, CX, CY, true>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head, rec_next,
-                                  row_rec, rec, rec_lo, srcd2, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
-                                  ng);
-    });
+    if constexpr (KIND == 0) {
+        const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
+        const int nzc = (int)cdiv(nz, RZ);
+        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
+        emu::launch(blocks, kBlock, [&] {
+            k_fwd3d<LB, CX, CY, true>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head,
+                                      rec_next, row_rec, rec, rec_lo, srcd2, nz, ny, nx, Bp, RZ,
+                                      nxs, nys, nzc, ng);
+        });
+    } else {
+        const int ntx = (int)cdiv(nx, CX), nty = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
+        const unsigned blocks = (unsigned)((int64_t)ng * ntx * nty * nzc);
+        emu::launch(blocks, kBlock, [&] {
+            k_fwd3d_lds<LB, CX, CY, PX, PY, true>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd,
+                                                  rec_head, rec_next, row_rec, rec, rec_lo, srcd2,
+                                                  nz, ny, nx, Bp, RZ, ntx, nty, nzc, ng);
+        });
+    }
     bool ok = true;
     auto Pv = [&](int64_t c, int b) { return (double)pc[c * Bp + b] + (double)bf2f(pcl[c * Bp + b]); };
     for (int z = 0; z < nz && ok; ++z)
@@ -376,14 +389,14 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
 // 3D adjoint: lam^k against a direct loop (bit for bit: the same FP32 ops in
 // the same order), and the imaging sums against a direct FP64 sum (the kernel
 // reduces over a lane group first, in another order).
-template <int LB, int CX, int CY>
+template <int LB, int CX, int CY, int KIND = 0, int PX = 1, int PY = 1>
 bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     const int Bp = LB * ng, W = 64 / LB;
     const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
     std::mt19937 rng(seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, "adj3d nz=%d ny=%d nx=%d LB=%d ng=%d CX=%d CY=%d RZ=%d", nz, ny,
-                  nx, LB, ng, CX, CY, RZ);
+    std::snprintf(tag, sizeof tag, "%s nz=%d ny=%d nx=%d LB=%d ng=%d C/T=%dx%d P=%dx%d RZ=%d",
+                  KIND ? "adj3d-lds" : "adj3d", nz, ny, nx, LB, ng, CX, CY, PX, PY, RZ);
     std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f);
     float2 *sp2 = alloc<float2>(N, 1);
     for (int64_t c = 0; c < N; ++c) sp2[c] = make_float2(u(rng), tt(rng));
@@ -413,12 +426,23 @@ bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     float *res = alloc<float>((int64_t)n_rec * Bp, 0);
     for (int64_t i = 0; i < (int64_t)n_rec * Bp; ++i) res[i] = d(rng);
     float *gh = alloc<float>(ng * N, 0), *gl = alloc<float>(ng * N, 0);
-    const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
-    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
-    emu::launch(blocks, kBlock, [&] {
-        k_adj3d<LB, CX, CY>(l1, l2, lk, pa, sp2, src, sinc, rec_head, rec_next, row_rec, res,
-                            srcd2, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
-    });
+    if constexpr (KIND == 0) {
+        const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
+        const int nzc = (int)cdiv(nz, RZ);
+        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
+        emu::launch(blocks, kBlock, [&] {
+            k_adj3d<LB, CX, CY>(l1, l2, lk, pa, sp2, src, sinc, rec_head, rec_next, row_rec, res,
+                                srcd2, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
+        });
+    } else {
+        const int ntx = (int)cdiv(nx, CX), nty = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
+        const unsigned blocks = (unsigned)((int64_t)ng * ntx * nty * nzc);
+        emu::launch(blocks, kBlock, [&] {
+            k_adj3d_lds<LB, CX, CY, PX, PY>(l1, l2, lk, pa, sp2, src, rec_head, rec_next, row_rec,
+                                            res, srcd2, gh, gl, nz, ny, nx, Bp, RZ, ntx, nty, nzc,
+                                            ng);
+        });
+    }
     bool ok = true;
     const int64_t sz = (int64_t)nx * ny;
     std::vector<double> gref(ng * N, 0.0);
@@ -473,6 +497,24 @@ bool fwd3d_cases(uint32_t seed) {
     return ok;
 }
 
+bool lds_cases(uint32_t seed) {
+    bool ok = true;
+    // tiles that do and do not divide the grid, shot groups of 16 / 32 / 64
+    ok &= check_fwd3d<16, 8, 8, 1, 2, 2>(9, 7, 11, 2, 4, seed++);
+    ok &= check_fwd3d<16, 8, 8, 1, 2, 2>(12, 17, 19, 1, 64, seed++);
+    ok &= check_fwd3d<16, 16, 8, 1, 4, 2>(7, 9, 21, 2, 3, seed++);
+    ok &= check_fwd3d<32, 8, 8, 1, 4, 2>(6, 10, 9, 2, 8, seed++);
+    ok &= check_fwd3d<64, 8, 4, 1, 4, 2>(5, 6, 13, 1, 4, seed++);
+    ok &= check_fwd3d<16, 8, 8, 1, 2, 2>(3, 3, 3, 1, 2, seed++);
+    ok &= check_adj3d<16, 8, 8, 1, 2, 2>(9, 7, 11, 2, 4, seed++);
+    ok &= check_adj3d<16, 8, 8, 1, 2, 2>(12, 17, 19, 1, 64, seed++);
+    ok &= check_adj3d<16, 16, 8, 1, 4, 2>(7, 9, 21, 2, 3, seed++);
+    ok &= check_adj3d<32, 8, 8, 1, 4, 2>(6, 10, 9, 2, 8, seed++);
+    ok &= check_adj3d<64, 8, 4, 1, 4, 2>(5, 6, 13, 1, 4, seed++);
+    ok &= check_adj3d<16, 8, 8, 1, 2, 2>(3, 3, 3, 1, 2, seed++);
+    return ok;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -492,6 +534,9 @@ int main(int argc, char **argv) {
         ok &= adj_cases<32, 4>(seed + 700);
         ok &= adj_cases<64, 4>(seed + 800);
         ok &= adj_cases<64, 8>(seed + 900);
+    }
+    if (only.empty() || only == "lds") {
+        ok &= lds_cases(seed + 2000);
     }
     if (only.empty() || only == "fwd3d") {
         ok &= fwd3d_cases<64, 4, 2>(seed + 1000);

@@ -4,12 +4,12 @@
 // library).  A wavefront's 64 lanes run as 64 cooperative contexts on the
 // calling thread (x86-64 only: a six-register stack switch, cheap next to
 // ucontext, whose signal-mask syscall and sanitizer hook dominate otherwise),
-// switched round-robin: __shfl_xor stores the lane's
+// switched round-robin, one work-group at a time: __shfl_xor stores the lane's
 // value, yields until every live lane has stored, reads its partner's, and
-// yields again before the slot can be reused.  A lane that returns early drops
-// out, as an inactive lane would.  Work-groups and their wavefronts run one
-// wavefront at a time.  Kernels that use __shared__ memory or __syncthreads
-// compile but must not be run here.
+// yields again before the slot can be reused; __syncthreads yields once.  A
+// lane that returns early drops out, as an inactive lane would.  __shared__
+// arrays are static: one instance, shared by the lanes of the running
+// work-group (work-groups run one after another).
 #pragma once
 
 #include <algorithm>
@@ -46,7 +46,7 @@ inline uint32_t __float_as_uint(float f) {
 }
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
-inline void __syncthreads() {}
+
 
 namespace emu {
 
@@ -82,21 +82,22 @@ emu_switch:
 )");
 
 constexpr size_t kStack = 256 << 10;
+constexpr int kMaxLanes = 256;  // one work-group
 
 struct Wave {
     void *sched = nullptr;
-    void *lane[64] = {};
+    void *lane[kMaxLanes] = {};
     std::vector<std::unique_ptr<char[]>> stacks;
-    bool done[64] = {};
-    int cur = 0;
-    unsigned block = 0, wave = 0;
+    bool done[kMaxLanes] = {};
+    int cur = 0, nlanes = 0;
+    unsigned block = 0;
     const std::function<void()> *fn = nullptr;
-    float xch[64] = {};
+    float xch[kMaxLanes] = {};
 };
 inline Wave W;
 
 inline void enter(unsigned l) {
-    tid.x = W.wave * 64 + l;
+    tid.x = l;
     bid.x = W.block;
 }
 
@@ -109,20 +110,24 @@ inline void enter(unsigned l) {
     __builtin_unreachable();
 }
 
-// Back to the scheduler; resumes after every other live lane has yielded once.
+// Back to the scheduler; resumes after every other live lane of the
+// work-group has yielded once (a barrier over the live lanes).
 inline void yield() {
     const int l = W.cur;
     emu_switch(&W.lane[l], W.sched);
     enter((unsigned)l);
 }
 
-inline void run_wave(unsigned block, unsigned w, const std::function<void()> &fn) {
+// All lanes of one work-group run together, round-robin; a round ends when
+// every live lane has yielded, so __syncthreads and the shuffle exchanges are
+// barriers over the work-group (a wave's lanes yield in lockstep).
+inline void run_block(unsigned block, unsigned threads, const std::function<void()> &fn) {
     if (W.stacks.empty())
-        for (int l = 0; l < 64; ++l) W.stacks.emplace_back(new char[kStack]);
+        for (int l = 0; l < kMaxLanes; ++l) W.stacks.emplace_back(new char[kStack]);
     W.block = block;
-    W.wave = w;
     W.fn = &fn;
-    for (int l = 0; l < 64; ++l) {
+    W.nlanes = (int)threads;
+    for (int l = 0; l < W.nlanes; ++l) {
         W.done[l] = false;
         // initial frame: six zero registers, the entry address, a zero return
         // address slot; the entry sees the stack pointer as after a call
@@ -135,7 +140,7 @@ inline void run_wave(unsigned block, unsigned w, const std::function<void()> &fn
     }
     for (bool live = true; live;) {
         live = false;
-        for (int l = 0; l < 64; ++l) {
+        for (int l = 0; l < W.nlanes; ++l) {
             if (W.done[l]) continue;
             W.cur = l;
             emu_switch(&W.sched, W.lane[l]);
@@ -144,19 +149,18 @@ inline void run_wave(unsigned block, unsigned w, const std::function<void()> &fn
     }
 }
 
-// Run a kernel body over `blocks` work-groups of `threads` lanes.
+// Run a kernel body over `blocks` work-groups of `threads` lanes (<= 256).
 inline void launch(unsigned blocks, unsigned threads, const std::function<void()> &body) {
     gdim = U3{blocks, 1, 1};
     bdim = U3{threads, 1, 1};
-    for (unsigned b = 0; b < blocks; ++b)
-        for (unsigned w = 0; w < threads / 64; ++w) run_wave(b, w, body);
+    for (unsigned b = 0; b < blocks; ++b) run_block(b, threads, body);
 }
 
 inline float shfl_xor(float v, int off) {
-    const int l = W.cur;
+    const int l = W.cur;  // work-group lane; the partner is in the same wavefront
     W.xch[l] = v;
     yield();
-    const float r = W.xch[l ^ off];
+    const float r = W.xch[(l & ~63) | ((l & 63) ^ off)];
     yield();
     return r;
 }
@@ -169,3 +173,4 @@ inline float shfl_xor(float v, int off) {
 #define blockDim emu::bdim
 
 inline float __shfl_xor(float v, int off, int /*width*/) { return emu::shfl_xor(v, off); }
+inline void __syncthreads() { emu::yield(); }
