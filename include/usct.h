@@ -117,6 +117,17 @@ int usct_misfit_and_gradient(usct_plan *plan, const double *m, const double *wav
                              const double *sponge, const double *dobs, int misfit_type,
                              double *J, double *grad, void *stream);
 
+/* Gauss-Newton Hessian-vector product hv = (J^T J) v summed over the plan's
+ * transmits (simulation/ringfwi/fwi.py:410-431, gn_hessian_vector): the exact
+ * discrete Born forward of v (fwi.py:350-378) on each transmit's background
+ * wavefield, then the exact adjoint of those Born traces (fwi.py:382-407),
+ * divided by m.  v, hv: device N doubles.  The background wavefield is
+ * recomputed each call (checkpointed, as for the gradient); the plan needs
+ * gradient workspace (no USCT_FLAG_FORWARD_ONLY) and allocates the Born state
+ * (three levels of the plan's field size, 6 bytes per element) on first use. */
+int usct_gn_hessian_vector(usct_plan *plan, const double *m, const double *wavelet,
+                           const double *sponge, const double *v, double *hv, void *stream);
+
 /*
  * Time one gradient launch by launch with HIP events on the plan's stream.
  * levels_per_launch: time levels one launch advances (2 when the plan runs

@@ -69,7 +69,7 @@ int fail(int code, const char *fmt, ...) {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2 };
+enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2, MODE_GNHV = 3 };
 
 struct Tiling {
     int LB, CX, CY, RY;  // lanes per shot group, cells per lane (x, y), rows per wavefront
@@ -135,6 +135,11 @@ struct usct_plan {
     float *g_hi = nullptr, *g_lo = nullptr;
     double *J_acc = nullptr;
     double *gstats = nullptr;
+    // Gauss-Newton: Born state over three levels (FP32 hi + bfloat16 lo) and
+    // v/m, allocated by the first Hessian-vector product
+    float *Dh[3] = {nullptr, nullptr, nullptr};
+    uint16_t *Dl[3] = {nullptr, nullptr, nullptr};
+    float2 *vm2 = nullptr;
     unsigned char *tmask = nullptr;  // [passes][n_rec*Bp]
     unsigned char *tmask_cur = nullptr;
     // execution
@@ -191,6 +196,11 @@ void free_plan_memory(usct_plan *P) {
     P->Wslot.clear();
     P->CK.clear();
     P->CKlo.clear();
+    for (int i = 0; i < 3; ++i) {
+        P->Dh[i] = nullptr;
+        P->Dl[i] = nullptr;
+    }
+    P->vm2 = nullptr;
     P->bytes = 0;
 }
 
@@ -272,6 +282,7 @@ void fwd3d_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
               const float *sinc, float *rec, float *rl, const Tile &t) {
     if (P->tf.CX == 4 && P->tf.CY == 4) fwd3d_t<LB, 4, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     else if (P->tf.CX == 8 && P->tf.CY == 2) fwd3d_t<LB, 8, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else if (P->tf.CX == 2 && P->tf.CY == 2) fwd3d_t<LB, 2, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     else fwd3d_t<LB, 4, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
 }
 
@@ -453,6 +464,7 @@ void adj3d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk
               const Tile &t) {
     if (P->ta.CX == 4 && P->ta.CY == 4) adj3d_t<LB, 4, 4>(g, s, l1, l2, lk, pa, P, sinc, res, t);
     else if (P->ta.CX == 8 && P->ta.CY == 2) adj3d_t<LB, 8, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else if (P->ta.CX == 2 && P->ta.CY == 2) adj3d_t<LB, 2, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
     else adj3d_t<LB, 4, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
 }
 
@@ -582,9 +594,42 @@ void launch_residual(usct_plan *P, hipStream_t s, int misfit, bool want_res) {
     }
 }
 
-// The launch sequence of one batched pass (captured into a hipGraph).
+// Born state of levels -1 and 0 (zero), before the first Born step of a pass.
+void born_clear(usct_plan *P, hipStream_t s) {
+    const int64_t FE = P->field_elems();
+    for (int i : {0, 2}) {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(FE)), dim3(256), 0, s, P->Dh[i], FE, 0.0f);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(cdiv(FE, 2))), dim3(256), 0, s,
+                           reinterpret_cast<float *>(P->Dl[i]), cdiv(FE, 2), 0.0f);
+    }
+}
+
+// Born level n (state in Dh/Dl[n % 3]) from the background levels n, n-1, n-2;
+// its receiver samples become row n of the adjoint source res.
+void launch_born(usct_plan *P, hipStream_t s, int n, Lvl u0, Lvl u1, Lvl u2) {
+    const int64_t FE = P->field_elems();
+    const int a = n % 3, b = (n + 2) % 3, c = (n + 1) % 3;  // levels n, n-1, n-2
+    float *res = P->res + (int64_t)n * P->n_rec * P->Bp;
+    const dim3 g((unsigned)cdiv(FE, 256));
+    if (P->sponge)
+        hipLaunchKernelGGL(k_born<true>, g, dim3(256), 0, s, u0.hi, u0.lo, u1.hi, u1.lo, u2.hi,
+                           u2.lo, P->Dh[b], P->Dl[b], P->Dh[c], P->Dl[c], P->Dh[a], P->Dl[a],
+                           P->sp2, P->vm2, P->spg, P->rec_head, P->rec_next, res, P->nz, P->ny,
+                           P->nx, P->ndim, P->Bp);
+    else
+        hipLaunchKernelGGL(k_born<false>, g, dim3(256), 0, s, u0.hi, u0.lo, u1.hi, u1.lo, u2.hi,
+                           u2.lo, P->Dh[b], P->Dl[b], P->Dh[c], P->Dl[c], P->Dh[a], P->Dl[a],
+                           P->sp2, P->vm2, P->spg, P->rec_head, P->rec_next, res, P->nz, P->ny,
+                           P->nx, P->ndim, P->Bp);
+}
+
+// The launch sequence of one batched pass (captured into a hipGraph).  The
+// Gauss-Newton product runs the Born level of each background level right
+// after it (levels n-2 .. n+1 are resident then) and takes the Born traces,
+// instead of a residual, as the adjoint's source.
 void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile *pr) {
-    const bool store = mode == MODE_GRAD;
+    const bool born = mode == MODE_GNHV;
+    const bool store = mode == MODE_GRAD || born;
     const int nt = P->nt;
     // rec row 0 = 0 (fwi.py:58: rec starts as zeros, level 0 is never written)
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
@@ -592,14 +637,20 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
                        P->recb_lo, (int64_t)P->n_rec * P->Bp, 0.0f);
     const bool x2 = fused(P);
+    if (born) born_clear(P, s);
     for (int n = 1; n < nt;) {
         const bool two = x2 && n + 1 < nt;
-        n += fwd_step(P, s, fwd_level(P, n - 2, store), fwd_level(P, n - 1, store),
-                      fwd_level(P, n, store), two ? fwd_level(P, n + 1, store) : Lvl{},
-                      n, two, true, pr);
+        const int adv = fwd_step(P, s, fwd_level(P, n - 2, store), fwd_level(P, n - 1, store),
+                                 fwd_level(P, n, store), two ? fwd_level(P, n + 1, store) : Lvl{},
+                                 n, two, true, pr);
+        if (born)
+            for (int l = n; l < n + adv; ++l)
+                launch_born(P, s, l, fwd_level(P, l, true), fwd_level(P, l - 1, true),
+                            fwd_level(P, l - 2, true));
+        n += adv;
     }
     if (mode == MODE_FWD) return;
-    launch_residual(P, s, misfit, store);
+    if (!born) launch_residual(P, s, misfit, store);
     if (!store) return;
     for (int w = P->NS - 1; w >= 0; --w) {
         const int base = w * P->K;
@@ -737,7 +788,8 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
         HIPX(hipMemsetAsync(P->g_hi, 0, sizeof(float) * N, s));
         HIPX(hipMemsetAsync(P->g_lo, 0, sizeof(float) * N, s));
     }
-    const bool store = mode == MODE_GRAD;
+    const bool born = mode == MODE_GNHV;
+    const bool store = mode == MODE_GRAD || born;
     const int nt = P->nt, nr = P->n_rec;
     const int nd = P->ndim;
     // History slots: level n in Wslot[n] (n >= 1), levels <= 0 are Z.
@@ -753,6 +805,7 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
                            P->src_all + b, P->sinc, P->sincd, nt, 1, P->dt * P->dt);
         HIPX(hipMemsetAsync(P->recb, 0, sizeof(float) * nr, s));
         HIPX(hipMemsetAsync(P->recb_lo, 0, sizeof(float) * nr, s));
+        if (born) born_clear(P, s);
         for (int n = 1; n < nt; ++n) {
             if (pr) ev_start(pr, pr->fwd_ev, s);
             if (P->sponge)
@@ -771,6 +824,9 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
                 ev_stop(pr, pr->fwd_ev, s);
                 pr->fwd_lv.push_back(1);
             }
+            if (born)
+                launch_born(P, s, n, {lvl(n), llo(n)}, {lvl(n - 1), llo(n - 1)},
+                            {lvl(n - 2), llo(n - 2)});
         }
         const int64_t NJ = (int64_t)nt * nr;
         const dim3 gt((unsigned)cdiv(NJ, kTT), 1);
@@ -778,12 +834,14 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
             hipLaunchKernelGGL(k_batched_to_data, gt, dim3(256), 0, s, P->recb, P->recb_lo, data,
                                NJ, 1, b, 1);
         if (mode == MODE_FWD) continue;
-        // residual of this shot against dobs[b] (FP64 -> FP32 hi + lo)
-        hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT, P->dobsT_lo,
-                           NJ, 1, b, 1);
-        if (misfit == USCT_MISFIT_GCN)
-            HIPX(hipMemcpyAsync(P->tmask_cur, P->tmask, nr, hipMemcpyDeviceToDevice, s));
-        launch_residual(P, s, misfit, store);
+        if (!born) {
+            // residual of this shot against dobs[b] (FP64 -> FP32 hi + lo)
+            hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT,
+                               P->dobsT_lo, NJ, 1, b, 1);
+            if (misfit == USCT_MISFIT_GCN)
+                HIPX(hipMemcpyAsync(P->tmask_cur, P->tmask, nr, hipMemcpyDeviceToDevice, s));
+            launch_residual(P, s, misfit, store);
+        }
         if (!store) continue;
         for (int k = nt - 1; k >= 1; --k) {
             const float *pa, *pb = nullptr, *pc2 = nullptr;
@@ -1083,7 +1141,7 @@ int autotune(usct_plan *P) {
                     for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry, 0});
         } else {
             for (int lb : lbs)
-                for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}})
+                for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}, {2, 2}})
                     for (int rz : {4, 8, 16, 32, 64}) {
                         // the adjoint reduces a lane's CX*CY cells over its lane group
                         if (adjoint && pc.first * pc.second > lb) continue;
@@ -1181,9 +1239,20 @@ int leave(usct_plan *P, void *stream) {
     return 0;
 }
 
+// Born state and v/m buffers, allocated once per plan (Gauss-Newton only).
+int ensure_gn(usct_plan *P) {
+    if (P->vm2) return 0;
+    const int64_t FE = P->field_elems();
+    for (int i = 0; i < 3; ++i) {
+        TRY(dalloc_t(P, &P->Dh[i], FE));
+        TRY(dalloc_t(P, &P->Dl[i], 2 * cdiv(FE, 2)));
+    }
+    return dalloc_t(P, &P->vm2, P->N);
+}
+
 int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
         const double *sponge, const double *dobs, double *data, double *J, double *grad,
-        void *stream, Profile *pr) {
+        void *stream, Profile *pr, const double *v = nullptr) {
     if (!P) return fail(USCT_EINVAL, "null plan");
     if (!m || !w) return fail(USCT_EINVAL, "m and wavelet are required");
     if (misfit != USCT_MISFIT_L2 && misfit != USCT_MISFIT_GCN)
@@ -1191,16 +1260,22 @@ int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
     if ((sponge != nullptr) != P->sponge)
         return fail(USCT_EINVAL, sponge ? "sponge passed to a plan created without USCT_FLAG_SPONGE"
                                         : "plan created with USCT_FLAG_SPONGE needs a sponge");
-    if (mode != MODE_FWD && !dobs) return fail(USCT_EINVAL, "dobs is required");
-    if (mode == MODE_GRAD && (P->flags & USCT_FLAG_FORWARD_ONLY))
+    if ((mode == MODE_MISFIT || mode == MODE_GRAD) && !dobs)
+        return fail(USCT_EINVAL, "dobs is required");
+    if ((mode == MODE_GRAD || mode == MODE_GNHV) && (P->flags & USCT_FLAG_FORWARD_ONLY))
         return fail(USCT_ENOTSUP, "plan was created with USCT_FLAG_FORWARD_ONLY");
+    if (mode == MODE_GNHV && !v) return fail(USCT_EINVAL, "v is required");
     if (P->n_tx == 0) {
         TRY(enter(P, stream));
         if (J) HIPX(hipMemsetAsync(J, 0, sizeof(double), P->stream));
         if (grad) HIPX(hipMemsetAsync(grad, 0, sizeof(double) * P->N, P->stream));
         return leave(P, stream);
     }
+    if (mode == MODE_GNHV) TRY(ensure_gn(P));
     TRY(enter(P, stream));
+    if (mode == MODE_GNHV)
+        hipLaunchKernelGGL(k_setup_vm, dim3(grid_for(P->N)), dim3(256), 0, P->stream, v, m,
+                           P->vm2, P->N);
     int rc = P->method == USCT_METHOD_PLAIN
                  ? run_plain(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr)
                  : run_batched(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr);
@@ -1339,6 +1414,13 @@ int usct_misfit_and_gradient(usct_plan *P, const double *m, const double *wavele
     if (!J || !grad) return fail(USCT_EINVAL, "J and grad are required");
     return run(P, MODE_GRAD, misfit_type, m, wavelet, sponge, dobs, nullptr, J, grad, stream,
                nullptr);
+}
+
+int usct_gn_hessian_vector(usct_plan *P, const double *m, const double *wavelet,
+                           const double *sponge, const double *v, double *hv, void *stream) {
+    if (!hv) return fail(USCT_EINVAL, "hv is required");
+    return run(P, MODE_GNHV, USCT_MISFIT_L2, m, wavelet, sponge, nullptr, nullptr, nullptr, hv,
+               stream, nullptr, v);
 }
 
 int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,

@@ -1667,8 +1667,74 @@ k_adj_plain(const float *__restrict__ l1, const float *__restrict__ l2, float *_
 }
 
 // ---------------------------------------------------------------------------
+// Born (linearised) forward step, one thread per shot-cell of the [cell][shot]
+// layout (Bp shots per cell; the plain path has Bp = 1), fwi.py:366-377:
+//   dp^n = 2 dp^{n-1} - dp^{n-2} + S L(dp^{n-1}) - (v/m)(U^n - 2U^{n-1} + U^{n-2}),
+// times the sponge.  Background levels U (u0 = U^n, u1 = U^{n-1}, u2 = U^{n-2})
+// and the Born state are FP32 hi + bfloat16 lo, combined and updated in FP64
+// as in the forward; the Laplacian is zero on border cells.  Each receiver j on
+// the cell takes dp^n into res[j * Bp + b] (row n of the adjoint's source).
+// Neighbour offsets are +-Bp (x), +-nx Bp (y), +-nx ny Bp (z).
+// ---------------------------------------------------------------------------
+template <bool SPONGE>
+__global__ void __launch_bounds__(256)
+k_born(const float *__restrict__ u0, const uint16_t *__restrict__ u0l,
+       const float *__restrict__ u1, const uint16_t *__restrict__ u1l,
+       const float *__restrict__ u2, const uint16_t *__restrict__ u2l,
+       const float *__restrict__ d1, const uint16_t *__restrict__ d1l,
+       const float *__restrict__ d2, const uint16_t *__restrict__ d2l,
+       float *__restrict__ dn, uint16_t *__restrict__ dnl, const float2 *__restrict__ sp2,
+       const float2 *__restrict__ vm2, const float *__restrict__ sponge,
+       const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+       float *__restrict__ res, int nz, int ny, int nx, int ndim, int Bp) {
+    const int64_t total = (int64_t)nz * ny * nx * Bp;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    int c, b;
+    if (total <= (int64_t)0xffffffffu) {
+        const uint32_t u = (uint32_t)i;
+        c = (int)(u / (uint32_t)Bp);
+        b = (int)(u - (uint32_t)c * (uint32_t)Bp);
+    } else {
+        c = (int)(i / Bp);
+        b = (int)(i - (int64_t)c * Bp);
+    }
+    const int x = c % nx;
+    const int y = (c / nx) % ny;
+    const int z = c / (nx * ny);
+    auto D = [&](int64_t e) { return hl(d1[e], d1l[e]); };
+    const int64_t sx = Bp, sy = (int64_t)nx * Bp;
+    double lu = 0.0;
+    if (ndim == 3) {
+        const int64_t sz = sy * ny;
+        if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+            lu = lap7d(D(i), D(i + sz), D(i - sz), D(i + sy), D(i - sy), D(i + sx), D(i - sx));
+    } else if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2) {
+        lu = lap5d(D(i), D(i + sy), D(i - sy), D(i + sx), D(i - sx));
+    }
+    const double d2u = (hl(u0[i], u0l[i]) - 2.0 * hl(u1[i], u1l[i])) + hl(u2[i], u2l[i]);
+    double nv = (2.0 * D(i) - hl(d2[i], d2l[i])) + s2d(sp2[c]) * lu - s2d(vm2[c]) * d2u;
+    if (SPONGE) nv = nv * (double)sponge[c];
+    uint16_t lb;
+    const float v = split_hl(nv, lb);
+    dn[i] = v;
+    dnl[i] = lb;
+    for (int j = rec_head[c]; j >= 0; j = rec_next[j]) res[(int64_t)j * Bp + b] = v;
+}
+
+// ---------------------------------------------------------------------------
 // Set-up and trace kernels
 // ---------------------------------------------------------------------------
+// vm2[c] = v[c] / m[c] as FP32 hi + lo (the Born source's scale).
+__global__ void k_setup_vm(const double *__restrict__ v, const double *__restrict__ m,
+                           float2 *__restrict__ vm2, int64_t N) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double s = v[i] / m[i];
+    const float hi = (float)s;
+    vm2[i] = make_float2(hi, (float)(s - (double)hi));
+}
+
 __global__ void k_setup_s(const double *__restrict__ m, float2 *__restrict__ sp2, int64_t N,
                           double dt2, double inv_h2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

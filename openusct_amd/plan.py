@@ -163,6 +163,19 @@ class Plan:
             _ptr(J), _ptr(g), _stream_handle(stream)))
         return J, g
 
+    def gn_hessian_vector(self, m, wavelet, v, sponge=None, out=None, stream=None):
+        """Gauss-Newton Hessian-vector product (J^T J) v (fwi.py:410-431), float64
+        with the model's shape; the background wavefield is recomputed per call."""
+        m, w, sp = self._model(m), self._wavelet(wavelet), self._sponge(sponge)
+        if not isinstance(v, torch.Tensor) or tuple(v.shape) != self.shape:
+            raise ValueError(f"v must be a tensor of shape {self.shape}")
+        v = v.to(device=self.device, dtype=torch.float64).contiguous()
+        if out is None:
+            out = torch.empty(self.shape, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().usct_gn_hessian_vector(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(v),
+                                                      _ptr(out), _stream_handle(stream)))
+        return out
+
     def profile_gradient(self, m, wavelet, dobs, sponge=None, stream=None):
         """Kernel timing of one gradient: mean duration (ms) of the forward and of the
         adjoint launches that advance ``levels_per_launch`` time levels each, and the

@@ -48,6 +48,10 @@ def _lib():
         lib.oracle_simulate_wavefield.argtypes = [dp, ip, ctypes.c_int, ctypes.c_double,
                                                   ctypes.c_double, ctypes.c_int,
                                                   ctypes.c_int64, dp, dp, dp]
+        lib.oracle_gn_hessian_vector.argtypes = [dp, ip, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_int, lp,
+                                                 ctypes.c_int, lp, ctypes.c_int, dp, dp, dp, dp]
+        lib.oracle_born_fmc.argtypes = lib.oracle_gn_hessian_vector.argtypes
         lib.oracle_num_threads.restype = ctypes.c_int
         _LIB = lib
     return _LIB
@@ -127,6 +131,36 @@ def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None,
     no = np.sqrt(np.sum(dobs * dobs, axis=1)) + eps
     c = np.sum((d / ns[:, None]) * (dobs / no[:, None]), axis=1)
     return float(np.sum(1.0 - c))
+
+
+def gn_hessian_vector(m, geom, wavelet, dt, h, nt, v, sponge=None, src_list=None):
+    """Gauss-Newton Hessian-vector product (J^T J) v (fwi.py:406-431), FP64."""
+    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge)
+    w = np.ascontiguousarray(wavelet, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    if v.shape != m.shape:
+        raise ValueError(f"v shape {v.shape} != {m.shape}")
+    hv = np.zeros(m.shape)
+    rc = _lib().oracle_gn_hessian_vector(
+        _d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), m.ndim, float(h), float(dt),
+        int(nt), _l(tx), len(tx), _l(rec), len(rec), _d(w), _d(v), _d(sp), _d(hv))
+    if rc:
+        raise ValueError(f"oracle_gn_hessian_vector failed ({rc})")
+    return hv
+
+
+def born_fmc(m, geom, wavelet, dt, h, nt, v, sponge=None, src_list=None):
+    """Born (linearised) traces J v of every transmit (fwi.py:351-378), FP64."""
+    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge)
+    w = np.ascontiguousarray(wavelet, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    dd = np.zeros((len(tx), nt, len(rec)))
+    rc = _lib().oracle_born_fmc(
+        _d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), m.ndim, float(h), float(dt),
+        int(nt), _l(tx), len(tx), _l(rec), len(rec), _d(w), _d(v), _d(sp), _d(dd))
+    if rc:
+        raise ValueError(f"oracle_born_fmc failed ({rc})")
+    return dd
 
 
 def simulate_wavefield(m, geom, src_element, wavelet, dt, h, nt, sponge=None):

@@ -22,6 +22,12 @@
  *   gradient       simulation/ringfwi/fwi.py:266-332 (k = nt-1..1, residual at
  *                  k = 0 never injected, duplicate receivers accumulate,
  *                  d2p = U[k] - 2U[k-1] + (U[k-2] if k >= 2 else 0), g /= m)
+ *   Born forward   simulation/ringfwi/fwi.py:351-378 (dp^n = 2dp^{n-1} - dp^{n-2}
+ *                  + S*L(dp^{n-1}) - (v/m)(U^n - 2U^{n-1} + U^{n-2}), times the sponge;
+ *                  traces dp^n at the receivers, row 0 = 0)
+ *   J^T apply      simulation/ringfwi/fwi.py:381-403 (the gradient's adjoint sweep
+ *                  with the Born traces as residual)
+ *   GN H v         simulation/ringfwi/fwi.py:406-431 (sum over transmits)
  *   loop structure and OpenMP placement follow software/core/libuap.hpp:29-224
  *   (rows/planes split across threads, serial receiver/source updates).
  *
@@ -214,6 +220,42 @@ static double adjoint_residual(const double *dsyn, const double *dobs, int nt, i
 }
 
 /*
+ * Adjoint sweep of one transmit (fwi.py:310-326): lam^k for k = nt-1..1 with the
+ * residual res (nt x n_rec) injected at the receivers, imaging
+ * grad += lam^k (U^k - 2U^{k-1} + U^{k-2}).  Work arrays l1, l2, lk, tmp, lap: N.
+ */
+static void adjoint_image(const double *S, const double *U, const double *res, const int *dims,
+                          int ndim, double inv_h2, int nt, const int64_t *rec_lin, int n_rec,
+                          const double *sponge, double *l1, double *l2, double *lk, double *tmp,
+                          double *lap, double *grad) {
+    const int64_t N = cells_of(dims, ndim);
+    memset(l1, 0, sizeof(double) * N);
+    memset(l2, 0, sizeof(double) * N);
+    for (int k = nt - 1; k >= 1; --k) {
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) tmp[i] = S[i] * l1[i];
+        lap_fp64(tmp, lap, dims, ndim, inv_h2);
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            double v = 2.0 * l1[i] - l2[i] + lap[i];
+            if (sponge) v *= sponge[i];
+            lk[i] = v;
+        }
+        const double *rr = res + (int64_t)k * n_rec;
+        for (int j = 0; j < n_rec; ++j) lk[rec_lin[j]] -= rr[j];
+        const double *Uk = U + (int64_t)k * N;
+        const double *Uk1 = U + (int64_t)(k - 1) * N;
+        const double *Uk2 = k >= 2 ? U + (int64_t)(k - 2) * N : NULL;
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            double d2p = Uk[i] - 2.0 * Uk1[i] + (Uk2 ? Uk2[i] : 0.0);
+            grad[i] += lk[i] * d2p;
+        }
+        double *t2 = l2; l2 = l1; l1 = lk; lk = t2;
+    }
+}
+
+/*
  * Misfit and exact discrete-adjoint gradient w.r.t. m (fwi.py:266-332).
  * dobs: n_tx * nt * n_rec; grad: N; J returned through *J_out.
  */
@@ -252,35 +294,134 @@ int oracle_misfit_and_gradient(const double *m, const int *dims, int ndim, doubl
         if (rc) break;
         kadd(&J, adjoint_residual(dsyn, dobs + (int64_t)t * nt * n_rec, nt, n_rec,
                                   misfit_type, res));
-        memset(l1, 0, sizeof(double) * N);
-        memset(l2, 0, sizeof(double) * N);
-        for (int k = nt - 1; k >= 1; --k) {
-            #pragma omp parallel for schedule(static)
-            for (int64_t i = 0; i < N; ++i) tmp[i] = S[i] * l1[i];
-            lap_fp64(tmp, lap, dims, ndim, inv_h2);
-            #pragma omp parallel for schedule(static)
-            for (int64_t i = 0; i < N; ++i) {
-                double v = 2.0 * l1[i] - l2[i] + lap[i];
-                if (sponge) v *= sponge[i];
-                lk[i] = v;
-            }
-            const double *rr = res + (int64_t)k * n_rec;
-            for (int j = 0; j < n_rec; ++j) lk[rec_lin[j]] -= rr[j];
-            const double *Uk = U + (int64_t)k * N;
-            const double *Uk1 = U + (int64_t)(k - 1) * N;
-            const double *Uk2 = k >= 2 ? U + (int64_t)(k - 2) * N : NULL;
-            #pragma omp parallel for schedule(static)
-            for (int64_t i = 0; i < N; ++i) {
-                double d2p = Uk[i] - 2.0 * Uk1[i] + (Uk2 ? Uk2[i] : 0.0);
-                grad[i] += lk[i] * d2p;
-            }
-            double *t2 = l2; l2 = l1; l1 = lk; lk = t2;
-        }
+        adjoint_image(S, U, res, dims, ndim, inv_h2, nt, rec_lin, n_rec, sponge, l1, l2, lk, tmp,
+                      lap, grad);
     }
     if (!rc) for (int64_t i = 0; i < N; ++i) grad[i] /= m[i];
     *J_out = J.s + J.c;
     free(S); free(U); free(dsyn); free(res); free(l1); free(l2); free(lk); free(tmp);
     free(lap);
+    return rc;
+}
+
+/*
+ * Born forward of one transmit (fwi.py:351-378) given its background history U
+ * (nt x N): dp^n = 2dp^{n-1} - dp^{n-2} + S*L(dp^{n-1}) - vm (U^n - 2U^{n-1} + U^{n-2}),
+ * times the sponge; dd[n*n_rec + j] = dp^n at receiver j (row 0 = 0).
+ * Work arrays a, b, c, lap: N.
+ */
+static void born_traces(const double *S, const double *vm, const double *U, const int *dims,
+                        int ndim, double inv_h2, int nt, const int64_t *rec_lin, int n_rec,
+                        const double *sponge, double *a, double *b, double *c, double *lap,
+                        double *dd) {
+    const int64_t N = cells_of(dims, ndim);
+    double *dp_prev = a, *dp_cur = b, *dp_new = c;
+    memset(dp_prev, 0, sizeof(double) * N);
+    memset(dp_cur, 0, sizeof(double) * N);
+    for (int j = 0; j < n_rec; ++j) dd[j] = 0.0;
+    for (int n = 1; n < nt; ++n) {
+        lap_fp64(dp_cur, lap, dims, ndim, inv_h2);
+        const double *Un = U + (int64_t)n * N;
+        const double *Un1 = U + (int64_t)(n - 1) * N;
+        const double *Un2 = n >= 2 ? U + (int64_t)(n - 2) * N : NULL;
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; ++i) {
+            const double d2U = Un[i] - 2.0 * Un1[i] + (Un2 ? Un2[i] : 0.0);
+            double w = 2.0 * dp_cur[i] - dp_prev[i] + S[i] * lap[i] - vm[i] * d2U;
+            if (sponge) w *= sponge[i];
+            dp_new[i] = w;
+        }
+        double *t3 = dp_prev; dp_prev = dp_cur; dp_cur = dp_new; dp_new = t3;
+        double *row = dd + (int64_t)n * n_rec;
+        for (int j = 0; j < n_rec; ++j) row[j] = dp_cur[rec_lin[j]];
+    }
+}
+
+/*
+ * Gauss-Newton Hessian-vector product (J^T J) v, summed over transmits
+ * (fwi.py:406-431): per transmit the background forward with history, the Born
+ * forward of v (fwi.py:351-378) and the adjoint sweep with the Born traces as
+ * residual (fwi.py:381-403); hv = sum / m.
+ */
+int oracle_gn_hessian_vector(const double *m, const int *dims, int ndim, double h, double dt,
+                             int nt, const int64_t *tx_lin, int n_tx, const int64_t *rec_lin,
+                             int n_rec, const double *wavelet, const double *v,
+                             const double *sponge, double *hv) {
+    if (check_args(dims, ndim, nt, rec_lin, n_rec)) return -1;
+    if (check_args(dims, ndim, nt, tx_lin, n_tx)) return -1;
+    const int64_t N = cells_of(dims, ndim);
+    const double inv_h2 = 1.0 / (h * h);
+    const double dt2 = dt * dt;
+    double *S = (double *)malloc(sizeof(double) * N);
+    double *vm = (double *)malloc(sizeof(double) * N);
+    double *U = (double *)malloc(sizeof(double) * N * (int64_t)nt);
+    double *dd = (double *)calloc((size_t)nt * n_rec, sizeof(double));
+    double *a = (double *)malloc(sizeof(double) * N);
+    double *b = (double *)malloc(sizeof(double) * N);
+    double *c = (double *)malloc(sizeof(double) * N);
+    double *l1 = (double *)malloc(sizeof(double) * N);
+    double *l2 = (double *)malloc(sizeof(double) * N);
+    double *lk = (double *)malloc(sizeof(double) * N);
+    double *tmp = (double *)malloc(sizeof(double) * N);
+    double *lap = (double *)malloc(sizeof(double) * N);
+    if (!S || !vm || !U || !dd || !a || !b || !c || !l1 || !l2 || !lk || !tmp || !lap) {
+        free(S); free(vm); free(U); free(dd); free(a); free(b); free(c); free(l1); free(l2);
+        free(lk); free(tmp); free(lap);
+        return -3;
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        S[i] = dt2 / m[i];
+        vm[i] = v[i] / m[i];
+        hv[i] = 0.0;
+    }
+    int rc = 0;
+    for (int t = 0; t < n_tx && !rc; ++t) {
+        rc = oracle_forward(m, dims, ndim, h, dt, nt, tx_lin[t], wavelet, NULL, 0, sponge, NULL,
+                            U);
+        if (rc) break;
+        born_traces(S, vm, U, dims, ndim, inv_h2, nt, rec_lin, n_rec, sponge, a, b, c, lap, dd);
+        adjoint_image(S, U, dd, dims, ndim, inv_h2, nt, rec_lin, n_rec, sponge, l1, l2, lk, tmp,
+                      lap, hv);
+    }
+    if (!rc) for (int64_t i = 0; i < N; ++i) hv[i] /= m[i];
+    free(S); free(vm); free(U); free(dd); free(a); free(b); free(c); free(l1); free(l2); free(lk);
+    free(tmp); free(lap);
+    return rc;
+}
+
+/* Born traces J v of every transmit (fwi.py:351-378); dd: n_tx * nt * n_rec. */
+int oracle_born_fmc(const double *m, const int *dims, int ndim, double h, double dt, int nt,
+                    const int64_t *tx_lin, int n_tx, const int64_t *rec_lin, int n_rec,
+                    const double *wavelet, const double *v, const double *sponge, double *dd) {
+    if (check_args(dims, ndim, nt, rec_lin, n_rec)) return -1;
+    if (check_args(dims, ndim, nt, tx_lin, n_tx)) return -1;
+    const int64_t N = cells_of(dims, ndim);
+    const double inv_h2 = 1.0 / (h * h);
+    const double dt2 = dt * dt;
+    double *S = (double *)malloc(sizeof(double) * N);
+    double *vm = (double *)malloc(sizeof(double) * N);
+    double *U = (double *)malloc(sizeof(double) * N * (int64_t)nt);
+    double *a = (double *)malloc(sizeof(double) * N);
+    double *b = (double *)malloc(sizeof(double) * N);
+    double *c = (double *)malloc(sizeof(double) * N);
+    double *lap = (double *)malloc(sizeof(double) * N);
+    if (!S || !vm || !U || !a || !b || !c || !lap) {
+        free(S); free(vm); free(U); free(a); free(b); free(c); free(lap);
+        return -3;
+    }
+    for (int64_t i = 0; i < N; ++i) {
+        S[i] = dt2 / m[i];
+        vm[i] = v[i] / m[i];
+    }
+    int rc = 0;
+    for (int t = 0; t < n_tx && !rc; ++t) {
+        rc = oracle_forward(m, dims, ndim, h, dt, nt, tx_lin[t], wavelet, NULL, 0, sponge, NULL,
+                            U);
+        if (!rc)
+            born_traces(S, vm, U, dims, ndim, inv_h2, nt, rec_lin, n_rec, sponge, a, b, c, lap,
+                        dd + (int64_t)t * nt * n_rec);
+    }
+    free(S); free(vm); free(U); free(a); free(b); free(c); free(lap);
     return rc;
 }
 

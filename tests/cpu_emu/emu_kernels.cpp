@@ -515,6 +515,113 @@ bool lds_cases(uint32_t seed) {
     return ok;
 }
 
+// Born step: every shot-cell of a grid (2D: nz = 1) against a direct FP64 loop
+// of its update; fields carry no guard zone, so under AddressSanitizer any
+// neighbour read outside the field aborts.
+bool check_born(int nz, int ny, int nx, int Bp, int ndim, bool sponge, uint32_t seed) {
+    std::mt19937 rng(seed);
+    const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
+    std::uniform_real_distribution<float> val(-1.0f, 1.0f), u(0.05f, 0.24f), t(-1e-9f, 1e-9f);
+    auto field = [&](float *&h, uint16_t *&l) {
+        h = alloc<float>(FE, 0);
+        l = alloc<uint16_t>(FE, 0);
+        for (int64_t i = 0; i < FE; ++i) {
+            h[i] = val(rng);
+            l[i] = f2bf(1e-8f * val(rng));
+        }
+    };
+    float *u0, *u1, *u2, *d1, *d2;
+    uint16_t *u0l, *u1l, *u2l, *d1l, *d2l;
+    field(u0, u0l);
+    field(u1, u1l);
+    field(u2, u2l);
+    field(d1, d1l);
+    field(d2, d2l);
+    float *dn = alloc<float>(FE, 0);
+    uint16_t *dnl = alloc<uint16_t>(FE, 0);
+    float2 *sp2 = alloc<float2>(N, 0), *vm2 = alloc<float2>(N, 0);
+    float *spg = alloc<float>(N, 0);
+    for (int64_t c = 0; c < N; ++c) {
+        sp2[c] = make_float2(u(rng), t(rng));
+        vm2[c] = make_float2(val(rng), t(rng));
+        spg[c] = sponge ? 0.5f + 0.5f * u(rng) : 1.0f;
+    }
+    // receivers: the first and last cell, a duplicate, random cells
+    const int n_rec = 9;
+    std::vector<int64_t> rc(n_rec);
+    std::uniform_int_distribution<int64_t> cell(0, N - 1);
+    for (int j = 0; j < n_rec; ++j) rc[j] = cell(rng);
+    rc[0] = 0;
+    rc[1] = N - 1;
+    rc[2] = rc[3];
+    int *rec_head = alloc<int>(N, 0), *rec_next = alloc<int>(n_rec, 0);
+    for (int64_t c = 0; c < N; ++c) rec_head[c] = -1;
+    for (int j = n_rec - 1; j >= 0; --j) {
+        rec_next[j] = rec_head[rc[j]];
+        rec_head[rc[j]] = j;
+    }
+    float *res = alloc<float>((int64_t)n_rec * Bp, 0);
+    emu::launch((unsigned)cdiv(FE, 256), 256, [&] {
+        if (sponge)
+            k_born<true>(u0, u0l, u1, u1l, u2, u2l, d1, d1l, d2, d2l, dn, dnl, sp2, vm2, spg,
+                         rec_head, rec_next, res, nz, ny, nx, ndim, Bp);
+        else
+            k_born<false>(u0, u0l, u1, u1l, u2, u2l, d1, d1l, d2, d2l, dn, dnl, sp2, vm2, spg,
+                          rec_head, rec_next, res, nz, ny, nx, ndim, Bp);
+    });
+    char tag[96];
+    std::snprintf(tag, sizeof tag, "born %dD %dx%dx%d Bp=%d sponge=%d", ndim, nz, ny, nx, Bp,
+                  (int)sponge);
+    bool ok = true;
+    const int64_t sy = (int64_t)nx, sz = (int64_t)nx * ny;
+    for (int64_t c = 0; c < N && ok; ++c) {
+        const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / sz);
+        for (int b = 0; b < Bp; ++b) {
+            auto D = [&](int64_t e) { return hl(d1[e * Bp + b], d1l[e * Bp + b]); };
+            auto U = [&](const float *h, const uint16_t *l) {
+                return hl(h[c * Bp + b], l[c * Bp + b]);
+            };
+            double lu = 0.0;
+            if (ndim == 3) {
+                if (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+                    lu = lap7d(D(c), D(c + sz), D(c - sz), D(c + sy), D(c - sy), D(c + 1), D(c - 1));
+            } else if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2) {
+                lu = lap5d(D(c), D(c + sy), D(c - sy), D(c + 1), D(c - 1));
+            }
+            const double d2u = (U(u0, u0l) - 2.0 * U(u1, u1l)) + U(u2, u2l);
+            double nv = (2.0 * D(c) - U(d2, d2l)) + s2d(sp2[c]) * lu - s2d(vm2[c]) * d2u;
+            if (sponge) nv = nv * (double)spg[c];
+            uint16_t lb;
+            const float v = split_hl(nv, lb);
+            if (std::memcmp(&v, &dn[c * Bp + b], 4) != 0 || lb != dnl[c * Bp + b]) {
+                std::printf("MISMATCH %s cell %lld shot %d\n", tag, (long long)c, b);
+                ok = false;
+                break;
+            }
+        }
+    }
+    // each receiver holds its cell's new level (duplicates both)
+    for (int j = 0; j < n_rec && ok; ++j)
+        for (int b = 0; b < Bp; ++b)
+            if (std::memcmp(&res[(int64_t)j * Bp + b], &dn[rc[j] * Bp + b], 4) != 0) {
+                std::printf("MISMATCH %s receiver %d shot %d\n", tag, j, b);
+                ok = false;
+                break;
+            }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+bool born_cases(uint32_t seed) {
+    bool ok = true;
+    ok &= check_born(1, 13, 11, 16, 2, false, seed++);
+    ok &= check_born(1, 9, 7, 3, 2, true, seed++);
+    ok &= check_born(1, 5, 6, 1, 2, true, seed++);  // plain layout
+    ok &= check_born(7, 9, 11, 4, 3, false, seed++);
+    ok &= check_born(5, 6, 3, 1, 3, false, seed++);
+    return ok;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -538,11 +645,15 @@ int main(int argc, char **argv) {
     if (only.empty() || only == "lds") {
         ok &= lds_cases(seed + 2000);
     }
+    if (only.empty() || only == "born") {
+        ok &= born_cases(seed + 3000);
+    }
     if (only.empty() || only == "fwd3d") {
         ok &= fwd3d_cases<64, 4, 2>(seed + 1000);
         ok &= fwd3d_cases<64, 4, 4>(seed + 1100);
         ok &= fwd3d_cases<32, 8, 2>(seed + 1200);
         ok &= fwd3d_cases<16, 4, 2>(seed + 1300);
+        ok &= fwd3d_cases<16, 2, 2>(seed + 1400);
     }
     std::printf(ok ? "ALL EQUAL\n" : "DIFFERENCES FOUND\n");
     return ok ? 0 : 1;

@@ -274,6 +274,25 @@ def test_autotuned_plan_matches_default():
     assert rel_l2(gb.cpu().numpy(), ga.cpu().numpy()) < 1e-6
 
 
+@pytest.mark.parametrize("sponge", [False, True])
+def test_simulate_wavefield_vs_oracle(sponge):
+    """Full pressure history of one transmit (fwi.py:92-99) against the oracle's,
+    with and without the sponge."""
+    from oracle import oracle as O
+    from openusct_amd import fwi, solver
+    f, geom, h, dt, _, w, _ = _fmc2d()
+    nt = 120
+    sp = solver.make_sponge(f["m_true"].shape, width=8) if sponge else None
+    print(f"[case] simulate_wavefield 61x61 nt={nt} sponge={sponge}", flush=True)
+    hist = fwi.simulate_wavefield(f["m_true"], geom, 3, w, dt, h, nt, sp)
+    ref = O.simulate_wavefield(f["m_true"], geom, 3, w, dt, h, nt, sp)
+    assert hist.shape == ref.shape == (nt,) + f["m_true"].shape
+    assert np.all(hist[0] == 0.0)
+    e = rel_l2(hist, ref)
+    print(f"history {e:.2e}")
+    assert e < TOL
+
+
 # ---------------------------------------------------------------------------
 # Configuration-scale parity against the FP64 oracle
 # ---------------------------------------------------------------------------

@@ -309,3 +309,91 @@ def test_planted_mistake_is_caught():
     assert rel_l2(-g, f["nosponge_g"]) > 1.0
     # a dropped k >= 2 term (d2p = U[k] - 2U[k-1]) also moves the gradient far away
     assert rel_l2(g, f["nosponge_g"]) < 1e-12
+
+
+def _gn_setup():
+    """test_gauss_newton.py:21-28's problem: 8-element ring, 61^2 at 1 mm,
+    3000 m/s coupling medium around a 1480 m/s bath, nt = 300."""
+    from openusct_amd import phantom
+    from openusct_amd.geometry import RingArray
+    from openusct_amd.sources import ricker
+    h = 1.0e-3
+    geom = RingArray(n_elements=8, radius_m=0.024, domain_m=0.06, h=h)
+    n = geom.n
+    dt, nt = 1.2e-7, 300
+    w = ricker(nt, dt, 0.4e6)
+    m0 = phantom.velocity_to_m(phantom.coupling_background((n, n), 3000.0, 1480.0, 0.02, h))
+    return geom, n, h, dt, nt, w, m0
+
+
+def _interior_v(n, m0, sp, seed):
+    """A random direction supported where the sponge is 1 (5-cell margin without one)."""
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((n, n)) * float(np.mean(m0))
+    keep = np.zeros((n, n), bool)
+    keep[5:-5, 5:-5] = True
+    if sp is not None:
+        keep &= sp == 1.0
+    return np.where(keep, v, 0.0), keep
+
+
+@pytest.mark.parametrize("with_sponge", [False, True])
+def test_gn_hessian_vector_matches_fd_of_gradient(with_sponge):
+    """At a zero-residual model the central difference of the exact gradient is
+    the Gauss-Newton Hessian applied to v (test_gauss_newton.py:30-51): a wrong
+    sign, factor or time index in the Born source breaks this.  With a sponge,
+    the Born source and the imaging condition use U^n - 2U^{n-1} + U^{n-2}
+    (fwi.py:368, 399), which equals the sponged scheme's own second difference
+    only where the sponge is 1, so there v is supported and the two are compared;
+    the sponge products of the Born and adjoint sweeps (fwi.py:370-371, 394-395)
+    act everywhere and are checked by this identity."""
+    from openusct_amd import solver
+    geom, n, h, dt, nt, w, m0 = _gn_setup()
+    sp = solver.make_sponge((n, n), width=8) if with_sponge else None
+    sl = [0, 4]
+    dobs = O.forward_fmc(m0, geom, w, dt, h, nt, sp, sl)
+    v, keep = _interior_v(n, m0, sp, 2)
+    hv = O.gn_hessian_vector(m0, geom, w, dt, h, nt, v, sp, sl)
+    eps = 1e-4
+    _, gp = O.misfit_and_gradient(m0 + eps * v, geom, w, dt, h, nt, dobs, sp, sl)
+    _, gm = O.misfit_and_gradient(m0 - eps * v, geom, w, dt, h, nt, dobs, sp, sl)
+    fd = (gp - gm) / (2.0 * eps)
+    rel = np.max(np.abs(hv - fd)[keep]) / np.max(np.abs(fd[keep]))
+    assert rel < 1e-5, rel
+    if with_sponge:
+        # the sponge changes the result by far more than the tolerance
+        hv0 = O.gn_hessian_vector(m0, geom, w, dt, h, nt, v, None, sl)
+        assert np.max(np.abs(hv0 - hv)[keep]) > 1e-3 * np.max(np.abs(hv[keep]))
+
+
+def test_gn_hessian_is_symmetric_positive():
+    """J^T J is symmetric and positive semi-definite: <u, Hv> = <Hu, v>, <v, Hv> >= 0."""
+    geom, n, h, dt, nt, w, m0 = _gn_setup()
+    nt = 150
+    w = w[:nt]
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((n, n)) * float(np.mean(m0))
+    v = rng.standard_normal((n, n)) * float(np.mean(m0))
+    hu = O.gn_hessian_vector(m0, geom, w, dt, h, nt, u, None, [1, 5])
+    hv = O.gn_hessian_vector(m0, geom, w, dt, h, nt, v, None, [1, 5])
+    a, b = float(np.sum(u * hv)), float(np.sum(hu * v))
+    assert abs(a - b) <= 1e-10 * max(abs(a), abs(b))
+    assert float(np.sum(v * hv)) > 0.0 and float(np.sum(u * hu)) > 0.0
+
+
+@pytest.mark.parametrize("with_sponge", [False, True])
+def test_born_traces_match_fd_of_forward(with_sponge):
+    """The Born traces are the linearisation of the forward (fwi.py:343-348):
+    J v = d/de forward(m + e v) at e = 0, for v supported where the sponge is 1."""
+    from openusct_amd import solver
+    geom, n, h, dt, nt, w, m0 = _gn_setup()
+    nt = 200
+    w = w[:nt]
+    sp = solver.make_sponge((n, n), width=8) if with_sponge else None
+    v, _ = _interior_v(n, m0, sp, 4)
+    dd = O.born_fmc(m0, geom, w, dt, h, nt, v, sp, [2, 6])
+    eps = 1e-4
+    fd = (O.forward_fmc(m0 + eps * v, geom, w, dt, h, nt, sp, [2, 6]) -
+          O.forward_fmc(m0 - eps * v, geom, w, dt, h, nt, sp, [2, 6])) / (2.0 * eps)
+    rel = np.max(np.abs(dd - fd)) / np.max(np.abs(fd))
+    assert rel < 1e-6, rel
