@@ -338,6 +338,31 @@ def invert(m0, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, n_ite
     return m, history
 
 
+def p_window_weights(dobs, dt, f0, onset_frac=0.2, keep_cycles=2.5, taper_cycles=0.75):
+    """Per-trace P-arrival mute windows from observed data (fwi.py:670-699).
+
+    Onset = first sample whose Hilbert envelope exceeds ``onset_frac`` of the
+    trace's envelope maximum; the window is 1 for ``keep_cycles`` periods of f0
+    after the onset, then a raised-cosine roll-off over ``taper_cycles`` periods
+    (at least one sample).  Returns a list over transmits of (nt, n_rx) weights,
+    the ``time_weights`` argument of misfit / misfit_and_gradient.
+    """
+    from scipy.signal import hilbert
+
+    dobs = np.asarray(dobs)
+    n_tx, nt, _ = dobs.shape
+    keep = keep_cycles / f0 / dt
+    taper = max(1.0, taper_cycles / f0 / dt)
+    t = np.arange(nt)[:, None]
+    out = []
+    for i in range(n_tx):
+        env = np.abs(hilbert(dobs[i], axis=0))
+        onset = np.argmax(env > onset_frac * (env.max(axis=0, keepdims=True) + 1e-30), axis=0)
+        x = np.clip((onset + keep + taper - t) / taper, 0.0, 1.0)
+        out.append(0.5 - 0.5 * np.cos(np.pi * x))
+    return out
+
+
 def fwi_from_dataset(dataset, start_c=None, update_frac=0.95, m_bounds=None, n_iter=12,
                      step_frac=0.03, verbose=False, **invert_kw):
     """FWI on a Dataset-like object (fwi.py:702-774): grid from its ground truth

@@ -119,3 +119,81 @@ def remap_positions(element_pos, n, h):
     domain = (n - 1) * h
     abspos = element_pos + domain / 2.0
     return [tuple(int(round(p[a] / h)) for a in reversed(range(dim))) for p in abspos]
+
+
+def _tangents(normal):
+    """Unit vectors spanning the aperture plane of an element with outward
+    ``normal``: one in 2D (the normal turned by +90 degrees), two in 3D (the
+    normal crossed with +z, or with +x when the normal is within ~25 degrees of
+    z, then the normal crossed with that) (geometry.py:199-209)."""
+    normal = np.asarray(normal, dtype=np.float64)
+    if normal.size == 2:
+        return [np.array([-normal[1], normal[0]])]
+    axis = np.array([0.0, 0.0, 1.0])
+    if abs(np.dot(normal, axis)) > 0.9:
+        axis = np.array([1.0, 0.0, 0.0])
+    t1 = np.cross(normal, axis)
+    t1 /= np.linalg.norm(t1)
+    return [t1, np.cross(normal, t1)]
+
+
+def build_footprints(geom, width_m, shape="flat", height_m=None, max_pts=9):
+    """Finite-aperture element footprints on the grid (geometry.py:212-278).
+
+    Each element is a flat aperture in the plane tangent to the array at the
+    element: ``"rect"`` (``width_m`` across, ``height_m`` along the array axis in
+    3D; a segment in 2D), ``"disc"`` (ellipse with those axes), ``"flat"`` (a
+    square: height = width) or ``"point"`` (also any aperture no wider than one
+    cell).  Each extent is sampled at min(round(extent / h) + 1, max_pts)
+    evenly spaced offsets, the first tangent's offsets varying fastest; every
+    sample is rounded (half to even) to its grid node and clipped to the grid;
+    repeated nodes are kept once, in first-seen order.  Cylinder apertures are
+    oriented about the radial direction in the ring plane, so height runs
+    along z.  Returns, per element, (list of index tuples, weights 1 / count).
+    """
+    pos = np.asarray(geom.element_positions, dtype=np.float64)
+    dim = pos.shape[1]
+    half = geom.domain_m / 2.0
+    h, n = geom.h, geom.n
+    if height_m is None or shape == "flat":
+        height_m = width_m
+    point = shape == "point" or (width_m <= h and height_m <= h)
+    cylinder = getattr(geom, "array_type", "") == "cylinder"
+
+    def offsets(extent):
+        k = int(np.clip(round(extent / h) + 1, 1, max_pts))
+        return np.linspace(-extent / 2.0, extent / 2.0, k) if k > 1 else np.zeros(1)
+
+    out = []
+    for e in range(len(pos)):
+        if point:
+            out.append(([geom.element_index(e)], np.array([1.0])))
+            continue
+        p = pos[e]
+        radial = p.copy()
+        if cylinder:
+            radial[2] = 0.0
+        r = np.linalg.norm(radial)
+        normal = radial / r if r > 0 else np.eye(dim)[0]
+        tang = _tangents(normal)
+        axes = [offsets(width_m)] + ([offsets(height_m)] if len(tang) > 1 else [])
+        if len(axes) == 1:
+            samples = [(a,) for a in axes[0]]
+        else:
+            samples = [(a, b) for b in axes[1] for a in axes[0]]
+        nodes = {}
+        for s in samples:
+            if shape == "disc":
+                u = s[0] / (width_m / 2.0 + 1e-30)
+                v = s[1] / (height_m / 2.0 + 1e-30) if len(s) > 1 else 0.0
+                if u * u + v * v > 1.0 + 1e-9:
+                    continue
+            shift = 0
+            for c, t in zip(s, tang):
+                shift = shift + c * t
+            xyz = p + half + shift
+            key = tuple(int(np.clip(round(xyz[d] / h), 0, n - 1)) for d in range(dim - 1, -1, -1))
+            nodes.setdefault(key, True)
+        cells = list(nodes)
+        out.append((cells, np.full(len(cells), 1.0 / len(cells))))
+    return out

@@ -9,6 +9,9 @@ The Python signatures follow the reference's drop-in protocol
 fwi.py:266-332 ``misfit_and_gradient``): ``geom`` only needs ``idx``,
 ``n_elements`` and ``element_index(k)``; ``src_list`` indexes elements and every
 element receives.  The sponge is applied, as in the Python reference.
+``footprints`` (geometry.py:212-278 lists of (cells, weights) per element) and
+``time_weights`` (per-transmit (nt, n_rx) windows) follow fwi.py:74-89,
+255-262 and 298-319.
 """
 
 from __future__ import annotations
@@ -38,20 +41,14 @@ def _lib():
         dp = ctypes.POINTER(ctypes.c_double)
         ip = ctypes.POINTER(ctypes.c_int)
         lp = ctypes.POINTER(ctypes.c_int64)
-        lib.oracle_forward_fmc.argtypes = [dp, ip, ctypes.c_int, ctypes.c_double,
-                                           ctypes.c_double, ctypes.c_int, lp, ctypes.c_int,
-                                           lp, ctypes.c_int, dp, dp, dp]
-        lib.oracle_misfit_and_gradient.argtypes = [dp, ip, ctypes.c_int, ctypes.c_double,
-                                                   ctypes.c_double, ctypes.c_int, lp,
-                                                   ctypes.c_int, lp, ctypes.c_int, dp, dp,
-                                                   ctypes.c_int, dp, dp, dp]
-        lib.oracle_simulate_wavefield.argtypes = [dp, ip, ctypes.c_int, ctypes.c_double,
-                                                  ctypes.c_double, ctypes.c_int,
-                                                  ctypes.c_int64, dp, dp, dp]
-        lib.oracle_gn_hessian_vector.argtypes = [dp, ip, ctypes.c_int, ctypes.c_double,
-                                                 ctypes.c_double, ctypes.c_int, lp,
-                                                 ctypes.c_int, lp, ctypes.c_int, dp, dp, dp, dp]
-        lib.oracle_born_fmc.argtypes = lib.oracle_gn_hessian_vector.argtypes
+        i, d = ctypes.c_int, ctypes.c_double
+        grid = [dp, ip, i, d, d, i]
+        grp = [i, lp, lp, dp]
+        lib.oracle_forward_fmc_g.argtypes = grid + grp + grp + [dp, dp, dp]
+        lib.oracle_misfit_and_gradient_g.argtypes = grid + grp + grp + [dp, dp, dp, i, dp, dp,
+                                                                         dp]
+        lib.oracle_gn_g.argtypes = grid + grp + grp + [dp, dp, dp, dp, dp]
+        lib.oracle_simulate_wavefield.argtypes = [dp, ip, i, d, d, i, ctypes.c_int64, dp, dp, dp]
         lib.oracle_num_threads.restype = ctypes.c_int
         _LIB = lib
     return _LIB
@@ -75,97 +72,133 @@ def linear_indices(idx, shape):
     return np.ravel_multi_index(tuple(idx.T), shape).astype(np.int64)
 
 
-def _prep(m, geom, src_list, sponge):
+class _Groups:
+    """Elements as CSR groups of cells with weights (footprints, geometry.py:212-278;
+    a point element is one cell of weight 1)."""
+
+    def __init__(self, elements, shape):
+        cells, w, off = [], [], [0]
+        for idxs, wt in elements:
+            lin = linear_indices(list(idxs), shape)
+            wt = np.asarray(wt, dtype=np.float64).reshape(-1)
+            if len(lin) != len(wt):
+                raise ValueError("footprint cells and weights differ in length")
+            cells.extend(int(c) for c in lin)
+            w.extend(float(x) for x in wt)
+            off.append(len(cells))
+        self.n = len(off) - 1
+        self.off = np.asarray(off, dtype=np.int64)
+        self.cells = np.asarray(cells if cells else [0], dtype=np.int64)
+        self.w = np.asarray(w if w else [0.0], dtype=np.float64)
+
+    def args(self):
+        return (self.n, _l(self.off), _l(self.cells), _d(self.w))
+
+
+def _prep(m, geom, src_list, sponge, footprints=None):
     m = np.ascontiguousarray(m, dtype=np.float64)
-    rec = linear_indices(geom.idx, m.shape)
     if src_list is None:
         src_list = list(range(geom.n_elements))
-    tx = np.ascontiguousarray(rec[np.asarray(list(src_list), dtype=np.int64)]) \
-        if len(src_list) else np.zeros(0, np.int64)
+    src_list = [int(s) for s in src_list]
+    if footprints is None:
+        elements = [([tuple(ix)], [1.0]) for ix in geom.idx]
+    else:
+        elements = list(footprints)
+        if len(elements) != geom.n_elements:
+            raise ValueError("footprints must hold one entry per element")
+    rec = _Groups(elements, m.shape)
+    tx = _Groups([elements[s] for s in src_list], m.shape)
     dims = np.asarray(m.shape, dtype=np.int32)
     sp = None if sponge is None else np.ascontiguousarray(sponge, dtype=np.float64)
     return m, rec, tx, dims, sp
 
 
-def forward_fmc(m, geom, wavelet, dt, h, nt, sponge=None, src_list=None):
-    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge)
+def _grid(m, dims, h, dt, nt):
+    return (_d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), m.ndim, float(h),
+            float(dt), int(nt))
+
+
+def forward_fmc(m, geom, wavelet, dt, h, nt, sponge=None, src_list=None, footprints=None):
+    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge, footprints)
     w = np.ascontiguousarray(wavelet, dtype=np.float64)
-    data = np.zeros((len(tx), nt, len(rec)))
-    rc = _lib().oracle_forward_fmc(_d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                   m.ndim, float(h), float(dt), int(nt), _l(tx), len(tx),
-                                   _l(rec), len(rec), _d(w), _d(sp), _d(data))
+    data = np.zeros((tx.n, nt, rec.n))
+    rc = _lib().oracle_forward_fmc_g(*_grid(m, dims, h, dt, nt), *tx.args(), *rec.args(), _d(w),
+                                     _d(sp), _d(data))
     if rc:
-        raise ValueError(f"oracle_forward_fmc failed ({rc})")
+        raise ValueError(f"oracle_forward_fmc_g failed ({rc})")
     return data
 
 
 _CODES = {"l2": 0, "gcn": 1}
 
 
-def misfit_and_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None,
-                        misfit_type="l2"):
-    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge)
+def _misfit_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge, src_list, misfit_type,
+                     footprints, time_weights, want_grad):
+    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge, footprints)
     w = np.ascontiguousarray(wavelet, dtype=np.float64)
     dobs = np.ascontiguousarray(dobs, dtype=np.float64)
-    if dobs.shape != (len(tx), nt, len(rec)):
-        raise ValueError(f"dobs shape {dobs.shape} != {(len(tx), nt, len(rec))}")
-    g = np.zeros(m.shape)
+    if dobs.shape != (tx.n, nt, rec.n):
+        raise ValueError(f"dobs shape {dobs.shape} != {(tx.n, nt, rec.n)}")
+    tw = None
+    if time_weights is not None:
+        tw = np.ascontiguousarray(np.stack([np.asarray(x, np.float64) for x in time_weights]))
+        if tw.shape != dobs.shape:
+            raise ValueError(f"time_weights shape {tw.shape} != {dobs.shape}")
+    g = np.zeros(m.shape) if want_grad else None
     J = ctypes.c_double(0.0)
-    rc = _lib().oracle_misfit_and_gradient(
-        _d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), m.ndim, float(h),
-        float(dt), int(nt), _l(tx), len(tx), _l(rec), len(rec), _d(w), _d(dobs),
-        _CODES[misfit_type], _d(sp), _d(g), ctypes.byref(J))
+    rc = _lib().oracle_misfit_and_gradient_g(*_grid(m, dims, h, dt, nt), *tx.args(), *rec.args(),
+                                             _d(w), _d(dobs), _d(tw), _CODES[misfit_type],
+                                             _d(sp), _d(g), ctypes.byref(J))
     if rc:
-        raise ValueError(f"oracle_misfit_and_gradient failed ({rc})")
+        raise ValueError(f"oracle_misfit_and_gradient_g failed ({rc})")
     return float(J.value), g
 
 
-def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None,
-           misfit_type="l2"):
-    d = forward_fmc(m, geom, wavelet, dt, h, nt, sponge, src_list)
-    if misfit_type == "l2":
-        r = d - dobs
-        return 0.5 * float(np.sum(r * r))
-    eps = 1e-12
-    ns = np.sqrt(np.sum(d * d, axis=1)) + eps
-    no = np.sqrt(np.sum(dobs * dobs, axis=1)) + eps
-    c = np.sum((d / ns[:, None]) * (dobs / no[:, None]), axis=1)
-    return float(np.sum(1.0 - c))
+def misfit_and_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None,
+                        misfit_type="l2", footprints=None, time_weights=None):
+    """Misfit and exact adjoint gradient (fwi.py:266-332), FP64."""
+    return _misfit_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge, src_list, misfit_type,
+                            footprints, time_weights, True)
 
 
-def gn_hessian_vector(m, geom, wavelet, dt, h, nt, v, sponge=None, src_list=None):
-    """Gauss-Newton Hessian-vector product (J^T J) v (fwi.py:406-431), FP64."""
-    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge)
+def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, misfit_type="l2",
+           footprints=None, time_weights=None):
+    """Misfit only (fwi.py:246-263), FP64."""
+    return _misfit_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge, src_list, misfit_type,
+                            footprints, time_weights, False)[0]
+
+
+def _gn(m, geom, wavelet, dt, h, nt, v, sponge, src_list, footprints, born):
+    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge, footprints)
     w = np.ascontiguousarray(wavelet, dtype=np.float64)
     v = np.ascontiguousarray(v, dtype=np.float64)
     if v.shape != m.shape:
         raise ValueError(f"v shape {v.shape} != {m.shape}")
-    hv = np.zeros(m.shape)
-    rc = _lib().oracle_gn_hessian_vector(
-        _d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), m.ndim, float(h), float(dt),
-        int(nt), _l(tx), len(tx), _l(rec), len(rec), _d(w), _d(v), _d(sp), _d(hv))
+    hv = None if born else np.zeros(m.shape)
+    dd = np.zeros((tx.n, nt, rec.n)) if born else None
+    rc = _lib().oracle_gn_g(*_grid(m, dims, h, dt, nt), *tx.args(), *rec.args(), _d(w), _d(v),
+                            _d(sp), _d(hv), _d(dd))
     if rc:
-        raise ValueError(f"oracle_gn_hessian_vector failed ({rc})")
-    return hv
+        raise ValueError(f"oracle_gn_g failed ({rc})")
+    return dd if born else hv
 
 
-def born_fmc(m, geom, wavelet, dt, h, nt, v, sponge=None, src_list=None):
-    """Born (linearised) traces J v of every transmit (fwi.py:351-378), FP64."""
-    m, rec, tx, dims, sp = _prep(m, geom, src_list, sponge)
-    w = np.ascontiguousarray(wavelet, dtype=np.float64)
-    v = np.ascontiguousarray(v, dtype=np.float64)
-    dd = np.zeros((len(tx), nt, len(rec)))
-    rc = _lib().oracle_born_fmc(
-        _d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), m.ndim, float(h), float(dt),
-        int(nt), _l(tx), len(tx), _l(rec), len(rec), _d(w), _d(v), _d(sp), _d(dd))
-    if rc:
-        raise ValueError(f"oracle_born_fmc failed ({rc})")
-    return dd
+def gn_hessian_vector(m, geom, wavelet, dt, h, nt, v, sponge=None, src_list=None,
+                      footprints=None):
+    """Gauss-Newton Hessian-vector product (J^T J) v (fwi.py:410-431), FP64."""
+    return _gn(m, geom, wavelet, dt, h, nt, v, sponge, src_list, footprints, False)
+
+
+def born_fmc(m, geom, wavelet, dt, h, nt, v, sponge=None, src_list=None, footprints=None):
+    """Born (linearised) traces J v of every transmit (fwi.py:350-378), FP64."""
+    return _gn(m, geom, wavelet, dt, h, nt, v, sponge, src_list, footprints, True)
 
 
 def simulate_wavefield(m, geom, src_element, wavelet, dt, h, nt, sponge=None):
-    m, rec, _, dims, sp = _prep(m, geom, [], sponge)
-    src = int(rec[src_element])
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    dims = np.asarray(m.shape, dtype=np.int32)
+    sp = None if sponge is None else np.ascontiguousarray(sponge, dtype=np.float64)
+    src = int(linear_indices([geom.element_index(int(src_element))], m.shape)[0])
     w = np.ascontiguousarray(wavelet, dtype=np.float64)
     hist = np.zeros((nt,) + m.shape)
     rc = _lib().oracle_simulate_wavefield(_d(m), dims.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),

@@ -2,7 +2,7 @@
 
 Run in a container that holds the reference checkout (never on a GPU host):
 
-    python tests/golden/make_golden.py [/path/to/reference]
+    python tests/golden/make_golden.py [/path/to/reference] [--only=base,apertures]
 
 It imports ``ringfwi`` from ``<reference>/simulation`` and records inputs and
 outputs of the functions whose behaviour only the reference itself pins:
@@ -18,7 +18,12 @@ outputs of the functions whose behaviour only the reference itself pins:
 * forward / gradient on the reference's own test problems
   (simulation/tests/test_gradient.py:20-40, test_gradient_3d.py:18-37), with
   duplicate receivers (fwi.py:321-322 accumulates them) and the GCN misfit,
-  plus a better-resolved 3D cylinder problem (fmc_3d_fine.npz).
+  plus a better-resolved 3D cylinder problem (fmc_3d_fine.npz);
+* finite-aperture footprints (geometry.py:212-278) and forward / gradient with
+  them in 2D and 3D (fwi.py:74-89, 316-319), P-window time weights
+  (fwi.py:670-699) and windowed misfits (fwi.py:255-262, 298-301), and
+  Gauss-Newton products with a sponge and with footprints (fwi.py:410-431),
+  whose sponge part is the reference's own formula.
 
 The fixtures are data only (numpy .npz, no pickles).
 """
@@ -192,5 +197,130 @@ def main(ref_root):
     print("golden fixtures written to", HERE)
 
 
+def main_apertures(ref_root):
+    """Footprints, P-window time weights and Gauss-Newton products."""
+    sys.path.insert(0, os.path.join(ref_root, "simulation"))
+    from ringfwi import fwi, phantom  # noqa: E402
+    from ringfwi.geometry import CylinderArray, RingArray, build_footprints  # noqa: E402
+    from ringfwi.solver import make_sponge  # noqa: E402
+    from ringfwi.sources import ricker  # noqa: E402
+
+    def flat(fps):
+        cells = np.array([ix for idxs, _ in fps for ix in idxs], dtype=np.int64)
+        counts = np.array([len(idxs) for idxs, _ in fps], dtype=np.int64)
+        w = np.concatenate([np.asarray(wt, dtype=np.float64) for _, wt in fps])
+        return cells, counts, w
+
+    # -- build_footprints (geometry.py:212-278) ------------------------------
+    fp = {}
+    cases = [("ring", (8, 0.024, 0.06, 1e-3), 0.003, "flat", None, 9),
+             ("ring", (8, 0.024, 0.06, 1e-3), 0.003, "rect", 0.001, 9),
+             ("ring", (8, 0.024, 0.06, 1e-3), 0.0, "point", None, 9),
+             ("ring", (8, 0.024, 0.06, 1e-3), 0.005, "disc", None, 9),
+             ("ring", (64, 0.022, 0.051, 2e-4), 0.0011, "rect", None, 9),
+             ("ring", (128, 0.022, 0.1022, 2e-4), 0.003, "flat", None, 5),
+             ("cyl", (2, 8, 0.010, 0.012, 0.028, 0.028 / 33.0), 0.004, "rect", 0.004, 9),
+             ("cyl", (2, 8, 0.010, 0.012, 0.028, 0.028 / 33.0), 0.004, "disc", 0.004, 9),
+             ("cyl", (2, 8, 0.010, 0.012, 0.028, 0.028 / 33.0), 0.002, "rect", 0.006, 9),
+             ("cyl", (2, 6, 0.010, 0.012, 0.03, 1.5e-3), 0.003, "flat", None, 9),
+             ("cyl", (8, 64, 0.028, 0.028, 0.06375, 2.5e-4), 0.001, "disc", 0.0015, 9)]
+    for i, (kind, prm, width, shape, height, mp) in enumerate(cases):
+        g = RingArray(*prm) if kind == "ring" else CylinderArray(*prm)
+        cells, counts, w = flat(build_footprints(g, width, shape, height_m=height, max_pts=mp))
+        fp[f"case{i}_kind"] = np.array(kind)
+        fp[f"case{i}_params"] = np.array(prm, dtype=np.float64)
+        fp[f"case{i}_args"] = np.array([width, np.nan if height is None else height, mp])
+        fp[f"case{i}_shape"] = np.array(shape)
+        fp[f"case{i}_cells"] = cells
+        fp[f"case{i}_counts"] = counts
+        fp[f"case{i}_w"] = w
+    np.savez_compressed(os.path.join(HERE, "footprints.npz"), **fp)
+
+    # -- apertures and time windows on test_gradient_aperture.py's problem -----
+    h = 1.0e-3
+    geom = RingArray(n_elements=8, radius_m=0.024, domain_m=0.06, h=h)
+    n = geom.n
+    dt, nt, f0 = 1.2e-7, 350, 0.4e6
+    wav = ricker(nt, dt, f0)
+    c_true = phantom.coupling_background((n, n), 3000.0, 1480.0, 0.02, h)
+    c_true = phantom.add_inclusion(c_true, (0.5, 0.5), 0.006, 3400.0, h)
+    m_true = phantom.velocity_to_m(c_true)
+    m0 = phantom.velocity_to_m(phantom.coupling_background((n, n), 3000.0, 1480.0, 0.02, h))
+    fps = build_footprints(geom, width_m=0.003, shape="flat")
+    cells, counts, w = flat(fps)
+    src_list = [0, 4]
+    sp = make_sponge((n, n), width=8)
+    out = dict(idx=np.array(geom.idx, dtype=np.int64), h=h, dt=dt, nt=nt, f0=f0, wavelet=wav,
+               m_true=m_true, m0=m0, src_list=np.array(src_list), fp_cells=cells,
+               fp_counts=counts, fp_w=w, sponge=sp)
+    for tag, s_ in (("nosponge", None), ("sponge", sp)):
+        dobs = fwi.forward_fmc(m_true, geom, wav, dt, h, nt, s_, src_list, footprints=fps)
+        out[f"{tag}_dobs"] = dobs
+        for mt in ("l2", "gcn"):
+            J, g = fwi.misfit_and_gradient(m0, geom, wav, dt, h, nt, dobs, s_, src_list, mt,
+                                           footprints=fps)
+            out[f"{tag}_{mt}_J"] = np.array(J)
+            out[f"{tag}_{mt}_g"] = g
+    # time windows (fwi.py:670-699) on the point-element data, then L2 / GCN with them
+    dobs_pt = fwi.forward_fmc(m_true, geom, wav, dt, h, nt, None, src_list)
+    tw = fwi.p_window_weights(dobs_pt, dt, f0)
+    out["pt_dobs"] = dobs_pt
+    out["tw"] = np.stack(tw)
+    for mt in ("l2", "gcn"):
+        J, g = fwi.misfit_and_gradient(m0, geom, wav, dt, h, nt, dobs_pt, None, src_list, mt,
+                                       time_weights=tw)
+        out[f"tw_{mt}_J"] = np.array(J)
+        out[f"tw_{mt}_g"] = g
+        out[f"tw_{mt}_misfit"] = np.array(fwi.misfit(m0, geom, wav, dt, h, nt, dobs_pt, None,
+                                                     src_list, mt, time_weights=tw))
+    # windows and footprints together
+    twf = fwi.p_window_weights(out["nosponge_dobs"], dt, f0, onset_frac=0.15, keep_cycles=2.0)
+    out["twf"] = np.stack(twf)
+    J, g = fwi.misfit_and_gradient(m0, geom, wav, dt, h, nt, out["nosponge_dobs"], None,
+                                   src_list, "l2", footprints=fps, time_weights=twf)
+    out["twf_J"] = np.array(J)
+    out["twf_g"] = g
+    # Gauss-Newton products (fwi.py:410-431): v over the whole grid
+    rng = np.random.default_rng(8)
+    v = rng.standard_normal((n, n)) * float(np.mean(m0))
+    out["gn_v"] = v
+    nt_gn = 250
+    out["gn_nt"] = nt_gn
+    out["gn_sponge_hv"] = fwi.gn_hessian_vector(m0, geom, wav[:nt_gn], dt, h, nt_gn, v, sp,
+                                                src_list)
+    out["gn_fp_hv"] = fwi.gn_hessian_vector(m0, geom, wav[:nt_gn], dt, h, nt_gn, v, None,
+                                            src_list, footprints=fps)
+    np.savez_compressed(os.path.join(HERE, "aperture_2d.npz"), **out)
+
+    # -- 3D apertures (test_gradient_3d.py's cylinder, rect elements) ---------
+    h = 1.5e-3
+    cyl = CylinderArray(n_rings=2, per_ring=6, radius_m=0.010, height_m=0.012,
+                        domain_m=0.03, h=h)
+    n = cyl.n
+    dt, nt = 1.5e-7, 140
+    wav = ricker(nt, dt, 0.35e6)
+    c_true = phantom.cylinder_background((n, n, n), 3000.0, 1480.0, 0.008, h)
+    c_true = phantom.add_sphere(c_true, (0.5, 0.5, 0.5), 0.004, 2700.0, h)
+    m_true = phantom.velocity_to_m(c_true)
+    m0 = phantom.velocity_to_m(phantom.cylinder_background((n, n, n), 3000.0, 1480.0, 0.008, h))
+    fps = build_footprints(cyl, 0.0045, "rect", height_m=0.003)
+    cells, counts, w = flat(fps)
+    src_list = [0, 6]
+    dobs = fwi.forward_fmc(m_true, cyl, wav, dt, h, nt, None, src_list, footprints=fps)
+    J, g = fwi.misfit_and_gradient(m0, cyl, wav, dt, h, nt, dobs, None, src_list,
+                                   footprints=fps)
+    np.savez_compressed(os.path.join(HERE, "aperture_3d.npz"),
+                        idx=np.array(cyl.idx, dtype=np.int64), h=h, dt=dt, nt=nt, wavelet=wav,
+                        m_true=m_true, m0=m0, src_list=np.array(src_list), fp_cells=cells,
+                        fp_counts=counts, fp_w=w, dobs=dobs, J=np.array(J), g=g)
+    print("aperture / window / Gauss-Newton fixtures written to", HERE)
+
+
+SECTIONS = {"base": main, "apertures": main_apertures}
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    args = [a for a in sys.argv[1:] if not a.startswith("--only=")]
+    only = [a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--only=")]
+    names = only[0].split(",") if only else list(SECTIONS)
+    for name in names:
+        SECTIONS[name](args[0] if args else "/root/reference")

@@ -28,3 +28,14 @@ def rel_l2(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def footprints_from(cells, counts, w):
+    """Footprint list [(index tuples, weights)] from a fixture's flat arrays."""
+    out, o = [], 0
+    for c in np.asarray(counts):
+        c = int(c)
+        out.append(([tuple(int(v) for v in t) for t in np.asarray(cells)[o:o + c]],
+                    np.asarray(w)[o:o + c]))
+        o += c
+    return out

@@ -188,3 +188,39 @@ def test_footprint_model_orders():
     assert 0 < small < big
     assert footprint_bytes((512, 512), 1000, 128, 128, ckpt_every=50) < \
         footprint_bytes((512, 512), 1000, 128, 128, ckpt_every=999)
+
+
+def test_build_footprints_match_reference():
+    """geometry.py:212-278 point lists (order, rounding, clipping, disc trimming)."""
+    from openusct_amd.geometry import CylinderArray, RingArray, build_footprints
+    f = golden("footprints.npz")
+    i = 0
+    while f"case{i}_kind" in f:
+        prm = f[f"case{i}_params"]
+        if str(f[f"case{i}_kind"]) == "ring":
+            g = RingArray(int(prm[0]), float(prm[1]), float(prm[2]), float(prm[3]))
+        else:
+            g = CylinderArray(int(prm[0]), int(prm[1]), *[float(x) for x in prm[2:]])
+        width, height, mp = f[f"case{i}_args"]
+        fps = build_footprints(g, float(width), str(f[f"case{i}_shape"]),
+                               height_m=None if np.isnan(height) else float(height),
+                               max_pts=int(mp))
+        cells = np.array([ix for idxs, _ in fps for ix in idxs], dtype=np.int64)
+        counts = np.array([len(idxs) for idxs, _ in fps])
+        w = np.concatenate([wt for _, wt in fps])
+        assert np.array_equal(counts, f[f"case{i}_counts"]), i
+        assert np.array_equal(cells, f[f"case{i}_cells"]), i
+        assert np.array_equal(w, f[f"case{i}_w"]), i
+        i += 1
+    assert i >= 10
+
+
+def test_p_window_weights_match_reference():
+    from openusct_amd import fwi
+    f = golden("aperture_2d.npz")
+    tw = fwi.p_window_weights(f["pt_dobs"], float(f["dt"]), float(f["f0"]))
+    assert np.max(np.abs(np.stack(tw) - f["tw"])) <= 1e-12
+    twf = fwi.p_window_weights(f["nosponge_dobs"], float(f["dt"]), float(f["f0"]),
+                               onset_frac=0.15, keep_cycles=2.0)
+    assert np.max(np.abs(np.stack(twf) - f["twf"])) <= 1e-12
+    assert all(((w >= 0) & (w <= 1)).all() for w in tw)

@@ -397,3 +397,114 @@ def test_born_traces_match_fd_of_forward(with_sponge):
           O.forward_fmc(m0 - eps * v, geom, w, dt, h, nt, sp, [2, 6])) / (2.0 * eps)
     rel = np.max(np.abs(dd - fd)) / np.max(np.abs(fd))
     assert rel < 1e-6, rel
+
+
+def _aperture():
+    from helpers import footprints_from
+    f = golden("aperture_2d.npz")
+    geom = IdxGeom(f["idx"])
+    fps = footprints_from(f["fp_cells"], f["fp_counts"], f["fp_w"])
+    return f, geom, fps, float(f["h"]), float(f["dt"]), int(f["nt"]), f["wavelet"], \
+        [int(s) for s in f["src_list"]]
+
+
+def _close(a, b, tol=1e-12):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.max(np.abs(a - b)) <= tol * np.max(np.abs(b))
+
+
+@pytest.mark.parametrize("tag", ["nosponge", "sponge"])
+def test_oracle_matches_golden_apertures_2d(tag):
+    """Finite-aperture transmit and receive (fwi.py:74-89, 316-319), L2 and GCN."""
+    f, geom, fps, h, dt, nt, w, src = _aperture()
+    sp = f["sponge"] if tag == "sponge" else None
+    assert max(len(c) for c, _ in fps) > 1
+    d = O.forward_fmc(f["m_true"], geom, w, dt, h, nt, sp, src, footprints=fps)
+    assert _close(d, f[f"{tag}_dobs"])
+    for mt in ("l2", "gcn"):
+        J, g = O.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f[f"{tag}_dobs"], sp, src, mt,
+                                     footprints=fps)
+        assert abs(J - float(f[f"{tag}_{mt}_J"])) <= 1e-11 * abs(float(f[f"{tag}_{mt}_J"]))
+        assert _close(g, f[f"{tag}_{mt}_g"], 1e-11), mt
+
+
+def test_oracle_matches_golden_time_windows():
+    """Windowed misfits (fwi.py:255-262, 298-301), alone and with footprints."""
+    f, geom, fps, h, dt, nt, w, src = _aperture()
+    tw = list(f["tw"])
+    for mt in ("l2", "gcn"):
+        J, g = O.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f["pt_dobs"], None, src, mt,
+                                     time_weights=tw)
+        assert abs(J - float(f[f"tw_{mt}_J"])) <= 1e-11 * abs(float(f[f"tw_{mt}_J"]))
+        assert _close(g, f[f"tw_{mt}_g"], 1e-11), mt
+        Jm = O.misfit(f["m0"], geom, w, dt, h, nt, f["pt_dobs"], None, src, mt, time_weights=tw)
+        assert abs(Jm - float(f[f"tw_{mt}_misfit"])) <= 1e-11 * abs(float(f[f"tw_{mt}_misfit"]))
+    J, g = O.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f["nosponge_dobs"], None, src,
+                                 "l2", footprints=fps, time_weights=list(f["twf"]))
+    assert abs(J - float(f["twf_J"])) <= 1e-11 * float(f["twf_J"])
+    assert _close(g, f["twf_g"], 1e-11)
+
+
+def test_oracle_matches_golden_gauss_newton_sponge_and_footprints():
+    """The reference's own Gauss-Newton product with a sponge over the whole grid
+    (where its Born source is not the sponged scheme's linearisation) and with
+    footprints (fwi.py:410-431)."""
+    f, geom, fps, h, dt, _, w, src = _aperture()
+    nt = int(f["gn_nt"])
+    hv = O.gn_hessian_vector(f["m0"], geom, w[:nt], dt, h, nt, f["gn_v"], f["sponge"], src)
+    assert _close(hv, f["gn_sponge_hv"], 1e-11)
+    hv = O.gn_hessian_vector(f["m0"], geom, w[:nt], dt, h, nt, f["gn_v"], None, src,
+                             footprints=fps)
+    assert _close(hv, f["gn_fp_hv"], 1e-11)
+
+
+def test_oracle_matches_golden_apertures_3d():
+    from helpers import footprints_from
+    f = golden("aperture_3d.npz")
+    geom = IdxGeom(f["idx"])
+    fps = footprints_from(f["fp_cells"], f["fp_counts"], f["fp_w"])
+    assert max(len(c) for c, _ in fps) > 1
+    src = [int(s) for s in f["src_list"]]
+    h, dt, nt, w = float(f["h"]), float(f["dt"]), int(f["nt"]), f["wavelet"]
+    d = O.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src, footprints=fps)
+    assert _close(d, f["dobs"])
+    J, g = O.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, f["dobs"], None, src,
+                                 footprints=fps)
+    assert abs(J - float(f["J"])) <= 1e-12 * float(f["J"])
+    assert _close(g, f["g"])
+
+
+def test_oracle_point_footprints_and_unit_windows_are_identities():
+    """One-cell footprints of weight 1 are the point path, unit windows the
+    unwindowed misfit, bit for bit (test_time_weights.py:31-37 asks 1e-12)."""
+    f, geom, h, dt, nt, w, src = _fmc2d()
+    pts = [([tuple(ix)], np.array([1.0])) for ix in geom.idx]
+    d0 = O.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src)
+    d1 = O.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src, footprints=pts)
+    assert np.array_equal(d0, d1)
+    J0, g0 = O.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, d0, None, src)
+    J1, g1 = O.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, d0, None, src, footprints=pts,
+                                   time_weights=[np.ones((nt, len(geom.idx)))] * len(src))
+    assert J0 == J1 and np.array_equal(g0, g1)
+
+
+def test_oracle_aperture_and_window_gradients_match_finite_differences():
+    """The footprint and windowed gradients are exact adjoints (no sponge):
+    directional central differences of the FP64 misfit agree to 1e-5
+    (test_gradient_aperture.py:43-56 asks 5 % of single-cell probes)."""
+    f, geom, fps, h, dt, nt, w, src = _aperture()
+    nt = 200
+    w = w[:nt]
+    dobs = O.forward_fmc(f["m_true"], geom, w, dt, h, nt, None, src, footprints=fps)
+    tw = [np.clip(np.linspace(1.5, -0.5, nt)[:, None] * np.ones((1, len(geom.idx))), 0, 1)] * 2
+    n = f["m0"].shape[0]
+    yy, xx = np.mgrid[0:n, 0:n].astype(float)
+    v = np.exp(-((yy - 30) ** 2 + (xx - 27) ** 2) / 40.0) * float(np.mean(f["m0"]))
+    for kw in (dict(footprints=fps), dict(footprints=fps, time_weights=tw)):
+        J, g = O.misfit_and_gradient(f["m0"], geom, w, dt, h, nt, dobs, None, src, **kw)
+        eps = 3e-5
+        Jp = O.misfit(f["m0"] + eps * v, geom, w, dt, h, nt, dobs, None, src, **kw)
+        Jm = O.misfit(f["m0"] - eps * v, geom, w, dt, h, nt, dobs, None, src, **kw)
+        fd = (Jp - Jm) / (2 * eps)
+        ad = float(np.sum(g * v))
+        assert abs(fd - ad) <= 1e-5 * abs(fd), (kw.keys(), fd, ad)
