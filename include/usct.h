@@ -17,6 +17,12 @@
  *   usct_misfit_and_gradient_direct software/core/libuap.hpp:169-224
  *                                   (uap::misfit_and_gradient),
  *                                   software/python/_uap.cpp:41-55
+ *   usct_plan_create_groups / usct_misfit_tw / usct_misfit_and_gradient_tw
+ *                                   finite-aperture elements and time windows
+ *                                   (simulation/ringfwi/fwi.py:74-89, 246-332),
+ *                                   which the reference runs in Python only
+ *                                   (fwi.py:594-615)
+ *   usct_gn_hessian_vector          simulation/ringfwi/fwi.py:410-431
  *   usct_plan_* / usct_forward_fmc / usct_misfit / usct_misfit_and_gradient
  *                                   the same three operations (plus misfit-only,
  *                                   software/python/uap.py:56-67) split into a
@@ -93,6 +99,22 @@ typedef struct {
 int usct_plan_create(usct_plan **plan, int ndim, const int *dims, double h, double dt,
                      int nt, const int64_t *tx_lin, int n_tx, const int64_t *rec_lin,
                      int n_rec, int method, int ckpt_every, int64_t mem_budget, int flags);
+
+/*
+ * Create a plan whose elements are finite apertures: each transmit and each
+ * receiver is a group of cells with weights (simulation/ringfwi/geometry.py:212-278
+ * build_footprints; fwi.py:74-89 drive every cell of the transmit group with
+ * wavelet * w_i and record sum_i w_i p[cell_i] per receiver group; fwi.py:316-319
+ * spread the adjoint source the same way).  Groups are in CSR form: group g
+ * holds cells[off[g] .. off[g+1]) with weights w[...], off[0] = 0, no empty
+ * group.  Other arguments as usct_plan_create.  Such plans advance one time
+ * level per launch; every entry point taking a plan accepts them.
+ */
+int usct_plan_create_groups(usct_plan **plan, int ndim, const int *dims, double h, double dt,
+                            int nt, int n_tx, const int64_t *tx_off, const int64_t *tx_cells,
+                            const double *tx_w, int n_rec, const int64_t *rec_off,
+                            const int64_t *rec_cells, const double *rec_w, int method,
+                            int ckpt_every, int64_t mem_budget, int flags);
 int usct_plan_destroy(usct_plan *plan);
 int usct_plan_get_info(const usct_plan *plan, usct_plan_info *info);
 
@@ -116,6 +138,18 @@ int usct_misfit(usct_plan *plan, const double *m, const double *wavelet,
 int usct_misfit_and_gradient(usct_plan *plan, const double *m, const double *wavelet,
                              const double *sponge, const double *dobs, int misfit_type,
                              double *J, double *grad, void *stream);
+
+/* Windowed misfit (fwi.py:246-263 with time_weights, fwi.py:255-262): the misfit
+ * of W*dsyn against W*dobs per transmit, and with it the gradient, whose adjoint
+ * source is multiplied by W again (fwi.py:298-301).  time_weights: device
+ * n_tx*nt*n_rec doubles in the layout of dobs, or NULL (no window). */
+int usct_misfit_tw(usct_plan *plan, const double *m, const double *wavelet,
+                   const double *sponge, const double *dobs, const double *time_weights,
+                   int misfit_type, double *J, void *stream);
+int usct_misfit_and_gradient_tw(usct_plan *plan, const double *m, const double *wavelet,
+                                const double *sponge, const double *dobs,
+                                const double *time_weights, int misfit_type, double *J,
+                                double *grad, void *stream);
 
 /* Gauss-Newton Hessian-vector product hv = (J^T J) v summed over the plan's
  * transmits (simulation/ringfwi/fwi.py:410-431, gn_hessian_vector): the exact

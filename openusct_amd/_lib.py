@@ -20,8 +20,9 @@ MISFIT_CODES = {"l2": 0, "gcn": 1}
 
 # Every symbol declared in include/usct.h.
 EXPORTS = (
-    "usct_plan_create", "usct_plan_destroy", "usct_plan_get_info", "usct_forward_fmc",
-    "usct_misfit", "usct_misfit_and_gradient", "usct_gn_hessian_vector", "usct_profile_gradient",
+    "usct_plan_create", "usct_plan_create_groups", "usct_plan_destroy", "usct_plan_get_info", "usct_forward_fmc",
+    "usct_misfit", "usct_misfit_and_gradient", "usct_misfit_tw", "usct_misfit_and_gradient_tw",
+    "usct_gn_hessian_vector", "usct_profile_gradient",
     "usct_forward_fmc_direct", "usct_misfit_and_gradient_direct", "usct_last_error",
     "usct_version",
 )
@@ -74,12 +75,17 @@ def load(path=LIB_PATH):
     intp = ctypes.POINTER(ctypes.c_int)
     lib.usct_plan_create.argtypes = [ctypes.POINTER(vp), ip, intp, dp, dp, ip, i64p, ip, i64p, ip,
                                      ip, ip, ctypes.c_int64, ip]
+    lib.usct_plan_create_groups.argtypes = [ctypes.POINTER(vp), ip, intp, dp, dp, ip,
+                                            ip, i64p, i64p, vp, ip, i64p, i64p, vp,
+                                            ip, ip, ctypes.c_int64, ip]
     lib.usct_plan_destroy.argtypes = [vp]
     lib.usct_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
     lib.usct_forward_fmc.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.usct_misfit.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
     lib.usct_misfit_and_gradient.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp, vp]
     lib.usct_gn_hessian_vector.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.usct_misfit_tw.argtypes = [vp, vp, vp, vp, vp, vp, ip, vp, vp]
+    lib.usct_misfit_and_gradient_tw.argtypes = [vp, vp, vp, vp, vp, vp, ip, vp, vp, vp]
     lib.usct_profile_gradient.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_double), intp, intp, intp, vp]
     lib.usct_forward_fmc_direct.argtypes = [vp, intp, ip, dp, dp, ip, i64p, ip, i64p, ip, vp, fp,

@@ -140,6 +140,26 @@ struct usct_plan {
     float *Dh[3] = {nullptr, nullptr, nullptr};
     uint16_t *Dl[3] = {nullptr, nullptr, nullptr};
     float2 *vm2 = nullptr;
+    // Finite-aperture elements (groups of cells with weights).  Per-pass lists
+    // are padded to fp_SE transmit entries and fp_FC fix cells and selected by
+    // *fp_pass.
+    bool fp = false;
+    int fp_SE = 0, fp_FC = 0;
+    int *fp_pass = nullptr, *fp_pass_ids = nullptr;
+    int *fp_eb = nullptr, *fp_ec = nullptr;   // transmit entries: shot, cell
+    double *fp_ew = nullptr, *fp_ecoef = nullptr;  // weight, S_c * weight
+    int *fp_roff = nullptr, *fp_rcell = nullptr;  // receiver groups (CSR)
+    double *fp_rw = nullptr;
+    int *fp_fcell = nullptr, *fp_frb = nullptr, *fp_fre = nullptr, *fp_fsb = nullptr,
+        *fp_fse = nullptr;                     // fix cells and their entry ranges
+    int *fp_rj = nullptr, *fp_sb = nullptr, *fp_sc = nullptr;  // fix entries: receiver;
+                                                                // shot and cell
+    float *fp_rwf = nullptr;                   // receiver weight (FP32)
+    double *fp_sw = nullptr, *fp_scoef = nullptr;  // transmit weight, S_c * weight
+    int64_t fp_nse = 0;                        // fix transmit entries over all passes
+    double *wav = nullptr;                     // the call's wavelet (nt)
+    // time windows of the current pass, batched layout (allocated on first use)
+    float *twT = nullptr, *twT_lo = nullptr;
     unsigned char *tmask = nullptr;  // [passes][n_rec*Bp]
     unsigned char *tmask_cur = nullptr;
     // execution
@@ -201,6 +221,7 @@ void free_plan_memory(usct_plan *P) {
         P->Dl[i] = nullptr;
     }
     P->vm2 = nullptr;
+    P->twT = P->twT_lo = nullptr;
     P->bytes = 0;
 }
 
@@ -386,7 +407,8 @@ void launch_fwd2_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl 
 
 // Whether the plan advances two time levels per launch.
 bool fused(const usct_plan *P) {
-    return P->method == USCT_METHOD_BATCHED && P->ndim == 2 && !P->sponge &&
+    // footprint plans correct every level after its launch, so one level per launch
+    return P->method == USCT_METHOD_BATCHED && P->ndim == 2 && !P->sponge && !P->fp &&
            !(P->flags & USCT_FLAG_NO_FUSE);
 }
 
@@ -573,23 +595,24 @@ int fwd_step(usct_plan *P, hipStream_t s, Lvl l2, Lvl l1, Lvl a, Lvl b, int n, b
 }
 
 // Residual and misfit of the traces in recb against dobsT for the current pass.
-void launch_residual(usct_plan *P, hipStream_t s, int misfit, bool want_res) {
+void launch_residual(usct_plan *P, hipStream_t s, int misfit, bool want_res, bool windowed) {
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
     const int64_t count = (int64_t)P->nt * ntr;
+    const float *W = windowed ? P->twT : nullptr;
     if (misfit == USCT_MISFIT_L2) {
         const int nb = (int)std::min<int64_t>(P->npart, grid_for(count));
         hipLaunchKernelGGL(k_residual_l2, dim3(nb), dim3(kBlock), 0, s, P->recb, P->recb_lo,
-                           P->dobsT, P->dobsT_lo, want_res ? P->res : nullptr, count, P->part);
+                           P->dobsT, P->dobsT_lo, W, want_res ? P->res : nullptr, count, P->part);
         hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, s, P->part, nb, 0.5,
                            P->J_acc, 1);
     } else {
         hipLaunchKernelGGL(k_gcn_stats, dim3(grid_for(ntr)), dim3(256), 0, s, P->recb, P->recb_lo,
-                           P->dobsT, P->dobsT_lo, P->gstats, P->nt, ntr);
+                           P->dobsT, P->dobsT_lo, W, P->gstats, P->nt, ntr);
         hipLaunchKernelGGL(k_gcn_misfit, dim3(1), dim3(64), 0, s, P->gstats, P->tmask_cur, ntr,
                            P->J_acc, 1);
         if (want_res)
             hipLaunchKernelGGL(k_gcn_residual, dim3(grid_for(count)), dim3(256), 0, s, P->recb,
-                               P->recb_lo, P->dobsT, P->dobsT_lo, P->gstats, P->tmask_cur,
+                               P->recb_lo, P->dobsT, P->dobsT_lo, W, P->gstats, P->tmask_cur,
                                P->res, P->nt, ntr);
     }
 }
@@ -623,11 +646,51 @@ void launch_born(usct_plan *P, hipStream_t s, int n, Lvl u0, Lvl u1, Lvl u2) {
                            P->nx, P->ndim, P->Bp);
 }
 
+// Footprint plans: weighted transmit cells of level n, then (record) the
+// receiver groups into trace row n.
+void fp_forward_fix(usct_plan *P, hipStream_t s, Lvl lv, int n, bool record) {
+    hipLaunchKernelGGL(k_fp_source, dim3(grid_for(P->fp_SE)), dim3(256), 0, s, lv.hi, lv.lo,
+                       P->fp_pass, P->fp_SE, P->fp_eb, P->fp_ec, P->fp_ecoef, P->wav, n,
+                       P->sponge ? P->spg : nullptr, P->Bp);
+    if (!record) return;
+    const int64_t ntr = (int64_t)P->n_rec * P->Bp;
+    hipLaunchKernelGGL(k_fp_record, dim3(grid_for(ntr)), dim3(256), 0, s, lv.hi, lv.lo, P->fp_roff,
+                       P->fp_rcell, P->fp_rw, P->recb + (int64_t)n * ntr,
+                       P->recb_lo + (int64_t)n * ntr, P->n_rec, P->Bp);
+}
+
+// Footprint plans: receiver groups of Born level n into row n of the adjoint source.
+void fp_born_record(usct_plan *P, hipStream_t s, int n) {
+    const int64_t ntr = (int64_t)P->n_rec * P->Bp;
+    hipLaunchKernelGGL(k_fp_record, dim3(grid_for(ntr)), dim3(256), 0, s, P->Dh[n % 3],
+                       P->Dl[n % 3], P->fp_roff, P->fp_rcell, P->fp_rw,
+                       P->res + (int64_t)n * ntr, nullptr, P->n_rec, P->Bp);
+}
+
+// Footprint plans: receiver injection and imaging corrections after adjoint level k.
+void fp_adjoint_fix(usct_plan *P, hipStream_t s, float *lk, const float *pa, const float *pb,
+                    const float *pc2, int k) {
+    const int ng = P->ng_adj();
+    const float *res = P->res + (int64_t)k * P->n_rec * P->Bp;
+    const dim3 g(grid_for((int64_t)P->fp_FC * ng));
+    if (P->sponge)
+        hipLaunchKernelGGL(k_fp_adjoint<true>, g, dim3(256), 0, s, lk, res, pa, pb, pc2, P->sp2,
+                           P->fp_pass, P->fp_FC, P->fp_fcell, P->fp_frb, P->fp_fre, P->fp_rj,
+                           P->fp_rwf, P->fp_fsb, P->fp_fse, P->fp_sb, P->fp_scoef, P->wav, k,
+                           P->g_hi, P->g_lo, P->ta.LB, ng, P->nz, P->ny, P->nx, P->ndim, P->Bp);
+    else
+        hipLaunchKernelGGL(k_fp_adjoint<false>, g, dim3(256), 0, s, lk, res, pa, pb, pc2, P->sp2,
+                           P->fp_pass, P->fp_FC, P->fp_fcell, P->fp_frb, P->fp_fre, P->fp_rj,
+                           P->fp_rwf, P->fp_fsb, P->fp_fse, P->fp_sb, P->fp_scoef, P->wav, k,
+                           P->g_hi, P->g_lo, P->ta.LB, ng, P->nz, P->ny, P->nx, P->ndim, P->Bp);
+}
+
 // The launch sequence of one batched pass (captured into a hipGraph).  The
 // Gauss-Newton product runs the Born level of each background level right
 // after it (levels n-2 .. n+1 are resident then) and takes the Born traces,
 // instead of a residual, as the adjoint's source.
-void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile *pr) {
+void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool windowed,
+                      Profile *pr) {
     const bool born = mode == MODE_GNHV;
     const bool store = mode == MODE_GRAD || born;
     const int nt = P->nt;
@@ -643,14 +706,18 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
         const int adv = fwd_step(P, s, fwd_level(P, n - 2, store), fwd_level(P, n - 1, store),
                                  fwd_level(P, n, store), two ? fwd_level(P, n + 1, store) : Lvl{},
                                  n, two, true, pr);
+        if (P->fp)
+            for (int l = n; l < n + adv; ++l) fp_forward_fix(P, s, fwd_level(P, l, store), l, true);
         if (born)
-            for (int l = n; l < n + adv; ++l)
+            for (int l = n; l < n + adv; ++l) {
                 launch_born(P, s, l, fwd_level(P, l, true), fwd_level(P, l - 1, true),
                             fwd_level(P, l - 2, true));
+                if (P->fp) fp_born_record(P, s, l);
+            }
         n += adv;
     }
     if (mode == MODE_FWD) return;
-    if (!born) launch_residual(P, s, misfit, store);
+    if (!born) launch_residual(P, s, misfit, store, windowed);
     if (!store) return;
     for (int w = P->NS - 1; w >= 0; --w) {
         const int base = w * P->K;
@@ -658,9 +725,12 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
         if (w < P->NS - 1) {
             for (int n = base + 1; n <= khi;) {
                 const bool two = x2 && n + 1 <= khi;
-                n += fwd_step(P, s, win_level(P, w, n - 2), win_level(P, w, n - 1),
-                              win_level(P, w, n), two ? win_level(P, w, n + 1) : Lvl{}, n, two,
-                              false, pr);
+                const int adv = fwd_step(P, s, win_level(P, w, n - 2), win_level(P, w, n - 1),
+                                         win_level(P, w, n), two ? win_level(P, w, n + 1) : Lvl{},
+                                         n, two, false, pr);
+                if (P->fp)
+                    for (int l = n; l < n + adv; ++l) fp_forward_fix(P, s, win_level(P, w, l), l, false);
+                n += adv;
             }
         }
         for (int k = khi; k >= base + 1;) {
@@ -681,6 +751,7 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, Profile
                 }
                 launch_adj_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2),
                                    lam_level(P, k), pa, pb, pc2, k);
+                if (P->fp) fp_adjoint_fix(P, s, lam_level(P, k), pa, pb, pc2, k);
             }
             if (pr) {
                 ev_stop(pr, pr->adj_ev, s);
@@ -697,18 +768,18 @@ int check_launch() {
     return 0;
 }
 
-int run_pass_sequence(usct_plan *P, int mode, int misfit, Profile *pr) {
+int run_pass_sequence(usct_plan *P, int mode, int misfit, bool windowed, Profile *pr) {
     const bool use_graph = !(P->flags & USCT_FLAG_NO_GRAPH) && pr == nullptr;
     if (!use_graph) {
-        batched_sequence(P, P->stream, mode, misfit, pr);
+        batched_sequence(P, P->stream, mode, misfit, windowed, pr);
         return check_launch();
     }
-    const int key = mode * 4 + misfit;
+    const int key = (mode * 4 + misfit) * 2 + (windowed ? 1 : 0);
     auto it = P->graphs.find(key);
     if (it == P->graphs.end()) {
         hipGraph_t graph;
         HIPX(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
-        batched_sequence(P, P->stream, mode, misfit, nullptr);
+        batched_sequence(P, P->stream, mode, misfit, windowed, nullptr);
         hipError_t le = hipGetLastError();
         HIPX(hipStreamEndCapture(P->stream, &graph));
         if (le != hipSuccess) {
@@ -730,7 +801,7 @@ int run_pass_sequence(usct_plan *P, int mode, int misfit, Profile *pr) {
 // ---------------------------------------------------------------------------
 int run_batched(usct_plan *P, int mode, int misfit, const double *m, const double *w,
                 const double *sponge, const double *dobs, double *data, double *J, double *grad,
-                Profile *pr) {
+                Profile *pr, const double *tw) {
     hipStream_t s = P->stream;
     const int64_t N = P->N;
     hipLaunchKernelGGL(k_setup_s, dim3(grid_for(N)), dim3(256), 0, s, m, P->sp2, N,
@@ -757,8 +828,14 @@ int run_batched(usct_plan *P, int mode, int misfit, const double *m, const doubl
         if (dobs)
             hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT,
                                P->dobsT_lo, NJ, P->Bp, b0, nb);
+        if (tw)
+            hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, tw, P->twT, P->twT_lo, NJ,
+                               P->Bp, b0, nb);
+        if (P->fp)
+            HIPX(hipMemcpyAsync(P->fp_pass, P->fp_pass_ids + p, sizeof(int),
+                                hipMemcpyDeviceToDevice, s));
         TRY(check_launch());
-        TRY(run_pass_sequence(P, mode, misfit, pr));
+        TRY(run_pass_sequence(P, mode, misfit, tw != nullptr, pr));
         if (data)
             hipLaunchKernelGGL(k_batched_to_data, gt, dim3(256), 0, s, P->recb, P->recb_lo, data,
                                NJ, P->Bp, b0, nb);
@@ -775,7 +852,7 @@ int run_batched(usct_plan *P, int mode, int misfit, const double *m, const doubl
 // ---------------------------------------------------------------------------
 int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double *w,
               const double *sponge, const double *dobs, double *data, double *J, double *grad,
-              Profile *pr) {
+              Profile *pr, const double *tw) {
     hipStream_t s = P->stream;
     const int64_t N = P->N;
     const unsigned gN = grid_for(N);
@@ -799,7 +876,11 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
     };
     auto llo = [&](int n) -> uint16_t * { return n <= 0 ? P->Zlo : P->Lo[n % 4]; };
     for (int b = 0; b < P->n_tx; ++b) {
-        const int src = (int)P->tx[b];
+        // footprint plans add their weighted sources after each level instead
+        const int src = P->fp ? -1 : (int)P->tx[b];
+        if (P->fp)
+            HIPX(hipMemcpyAsync(P->fp_pass, P->fp_pass_ids + b, sizeof(int),
+                                hipMemcpyDeviceToDevice, s));
         // S_src * w[n-1] per level, same FP64 product as the batched path.
         hipLaunchKernelGGL(k_setup_sinc, dim3(grid_for(nt)), dim3(256), 0, s, m, w,
                            P->src_all + b, P->sinc, P->sincd, nt, 1, P->dt * P->dt);
@@ -824,9 +905,12 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
                 ev_stop(pr, pr->fwd_ev, s);
                 pr->fwd_lv.push_back(1);
             }
-            if (born)
+            if (P->fp) fp_forward_fix(P, s, {lvl(n), llo(n)}, n, true);
+            if (born) {
                 launch_born(P, s, n, {lvl(n), llo(n)}, {lvl(n - 1), llo(n - 1)},
                             {lvl(n - 2), llo(n - 2)});
+                if (P->fp) fp_born_record(P, s, n);
+            }
         }
         const int64_t NJ = (int64_t)nt * nr;
         const dim3 gt((unsigned)cdiv(NJ, kTT), 1);
@@ -838,9 +922,12 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
             // residual of this shot against dobs[b] (FP64 -> FP32 hi + lo)
             hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT,
                                P->dobsT_lo, NJ, 1, b, 1);
+            if (tw)
+                hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, tw, P->twT, P->twT_lo,
+                                   NJ, 1, b, 1);
             if (misfit == USCT_MISFIT_GCN)
                 HIPX(hipMemcpyAsync(P->tmask_cur, P->tmask, nr, hipMemcpyDeviceToDevice, s));
-            launch_residual(P, s, misfit, store);
+            launch_residual(P, s, misfit, store, tw != nullptr);
         }
         if (!store) continue;
         for (int k = nt - 1; k >= 1; --k) {
@@ -860,6 +947,7 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
                                    P->spg, src, P->sinc + k, P->srcd2 + k, P->rec_head, P->rec_next,
                                    P->res + (int64_t)k * nr, P->g_hi, P->g_lo, P->nz, P->ny, P->nx,
                                    nd);
+            if (P->fp) fp_adjoint_fix(P, s, lk, pa, pb, pc2, k);
             if (pr) {
                 ev_stop(pr, pr->adj_ev, s);
                 pr->adj_lv.push_back(1);
@@ -1250,9 +1338,17 @@ int ensure_gn(usct_plan *P) {
     return dalloc_t(P, &P->vm2, P->N);
 }
 
+// Time-window traces of a pass (batched layout), allocated on first use.
+int ensure_windows(usct_plan *P) {
+    if (P->twT) return 0;
+    const int64_t count = (int64_t)P->nt * P->n_rec * P->Bp;
+    TRY(dalloc_t(P, &P->twT_lo, count));
+    return dalloc_t(P, &P->twT, count);
+}
+
 int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
         const double *sponge, const double *dobs, double *data, double *J, double *grad,
-        void *stream, Profile *pr, const double *v = nullptr) {
+        void *stream, Profile *pr, const double *v = nullptr, const double *tw = nullptr) {
     if (!P) return fail(USCT_EINVAL, "null plan");
     if (!m || !w) return fail(USCT_EINVAL, "m and wavelet are required");
     if (misfit != USCT_MISFIT_L2 && misfit != USCT_MISFIT_GCN)
@@ -1272,15 +1368,178 @@ int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
         return leave(P, stream);
     }
     if (mode == MODE_GNHV) TRY(ensure_gn(P));
+    if (tw) TRY(ensure_windows(P));
     TRY(enter(P, stream));
     if (mode == MODE_GNHV)
         hipLaunchKernelGGL(k_setup_vm, dim3(grid_for(P->N)), dim3(256), 0, P->stream, v, m,
                            P->vm2, P->N);
+    if (P->fp) {
+        // the call's wavelet and S_c w_i of every transmit cell, read by the
+        // (captured) footprint kernels from plan-owned buffers
+        HIPX(hipMemcpyAsync(P->wav, w, sizeof(double) * P->nt, hipMemcpyDeviceToDevice,
+                            P->stream));
+        const double dt2 = P->dt * P->dt;
+        const int64_t ne = (int64_t)P->fp_SE * P->passes;
+        hipLaunchKernelGGL(k_fp_coef, dim3(grid_for(ne)), dim3(256), 0, P->stream, P->fp_ec,
+                           P->fp_ew, P->fp_ecoef, ne, m, dt2);
+        if (P->fp_nse)
+            hipLaunchKernelGGL(k_fp_coef, dim3(grid_for(P->fp_nse)), dim3(256), 0, P->stream,
+                               P->fp_sc, P->fp_sw, P->fp_scoef, P->fp_nse, m, dt2);
+    }
     int rc = P->method == USCT_METHOD_PLAIN
-                 ? run_plain(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr)
-                 : run_batched(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr);
+                 ? run_plain(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr, tw)
+                 : run_batched(P, mode, misfit, m, w, sponge, dobs, data, J, grad, pr, tw);
     int rc2 = leave(P, stream);
     return rc ? rc : rc2;
+}
+
+// Element groups in CSR form (group g: cells[off[g] .. off[g+1]) with weights w).
+int check_groups_arg(const char *what, int n, const int64_t *off, const int64_t *cells,
+                     const double *w, int64_t N) {
+    if (n > 0 && (!off || !cells || !w)) return fail(USCT_EINVAL, "%s groups: null argument", what);
+    if (n > 0 && off[0] != 0) return fail(USCT_EINVAL, "%s_off[0] must be 0", what);
+    for (int g = 0; g < n; ++g) {
+        if (off[g + 1] <= off[g]) return fail(USCT_EINVAL, "%s group %d is empty", what, g);
+        for (int64_t i = off[g]; i < off[g + 1]; ++i) {
+            if (cells[i] < 0 || cells[i] >= N)
+                return fail(USCT_EINVAL, "%s group %d: cell %lld outside the grid of %lld cells",
+                            what, g, (long long)cells[i], (long long)N);
+            if (!std::isfinite(w[i]))
+                return fail(USCT_EINVAL, "%s group %d: weight %lld is not finite", what, g,
+                            (long long)(i - off[g]));
+        }
+    }
+    return 0;
+}
+
+template <class T>
+int upload(usct_plan *P, T **dst, const std::vector<T> &v) {
+    TRY(dalloc_t(P, dst, std::max<int64_t>((int64_t)v.size(), 1)));
+    if (!v.empty()) HIPX(hipMemcpy(*dst, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Device lists of a plan whose elements are groups of cells (its passes fixed):
+// the stencil kernels get no point source and no receivers, and the footprint
+// kernels take the transmit entries per pass, the receiver groups and, per pass,
+// the fix cells (every receiver cell and the pass's transmit cells) with their
+// receiver entries in receiver order and their transmit entries.
+int attach_groups(usct_plan *P, const int64_t *tx_off, const int64_t *tx_cells,
+                  const double *tx_w, const int64_t *rec_off, const int64_t *rec_cells,
+                  const double *rec_w) {
+    const int Bp = P->Bp, passes = P->passes, n_rec = P->n_rec;
+    if ((int64_t)passes * Bp > 0)
+        HIPX(hipMemset(P->src_all, 0xff, sizeof(int) * (size_t)passes * Bp));
+    HIPX(hipMemset(P->rec_head, 0xff, sizeof(int) * (size_t)P->N));
+    HIPX(hipMemset(P->row_rec, 0, sizeof(int) * (size_t)(P->N / P->nx)));
+    std::vector<int> roff(n_rec + 1, 0), rcell;
+    std::vector<double> rw;
+    std::map<int, std::vector<std::pair<int, float>>> rec_at;
+    for (int j = 0; j < n_rec; ++j) {
+        for (int64_t i = rec_off[j]; i < rec_off[j + 1]; ++i) {
+            rcell.push_back((int)rec_cells[i]);
+            rw.push_back(rec_w[i]);
+            rec_at[(int)rec_cells[i]].push_back({j, (float)rec_w[i]});
+        }
+        roff[j + 1] = (int)rcell.size();
+    }
+    // transmit entries per pass; a cell repeated within one group is driven once
+    // with the summed weight
+    std::vector<std::vector<std::tuple<int, int, double>>> ent(std::max(passes, 1));
+    std::vector<std::map<int, std::vector<std::pair<int, double>>>> src_at(std::max(passes, 1));
+    for (int t = 0; t < P->n_tx; ++t) {
+        const int p = t / Bp, b = t % Bp;
+        std::vector<std::pair<int, double>> cw;
+        for (int64_t i = tx_off[t]; i < tx_off[t + 1]; ++i) {
+            const int c = (int)tx_cells[i];
+            auto it = std::find_if(cw.begin(), cw.end(), [c](const auto &e) { return e.first == c; });
+            if (it == cw.end()) cw.push_back({c, tx_w[i]});
+            else it->second += tx_w[i];
+        }
+        for (auto &e : cw) {
+            ent[p].push_back({b, e.first, e.second});
+            src_at[p][e.first].push_back({b, e.second});
+        }
+    }
+    int SE = 0, FC = 0;
+    std::vector<std::vector<int>> fix(std::max(passes, 1));
+    for (int p = 0; p < passes; ++p) {
+        SE = std::max(SE, (int)ent[p].size());
+        std::vector<int> &f = fix[p];
+        for (auto &kv : rec_at) f.push_back(kv.first);
+        for (auto &kv : src_at[p])
+            if (!rec_at.count(kv.first)) f.push_back(kv.first);
+        std::sort(f.begin(), f.end());
+        FC = std::max(FC, (int)f.size());
+    }
+    std::vector<int> eb((size_t)passes * SE, -1), ec((size_t)passes * SE, -1);
+    std::vector<double> ew((size_t)passes * SE, 0.0);
+    std::vector<int> fcell((size_t)passes * FC, -1), frb((size_t)passes * FC, 0),
+        fre((size_t)passes * FC, 0), fsb((size_t)passes * FC, 0), fse((size_t)passes * FC, 0);
+    std::vector<int> rj, sb, sc;
+    std::vector<float> rwf;
+    std::vector<double> sw;
+    for (int p = 0; p < passes; ++p) {
+        for (size_t e = 0; e < ent[p].size(); ++e) {
+            const size_t q = (size_t)p * SE + e;
+            eb[q] = std::get<0>(ent[p][e]);
+            ec[q] = std::get<1>(ent[p][e]);
+            ew[q] = std::get<2>(ent[p][e]);
+        }
+        for (size_t f = 0; f < fix[p].size(); ++f) {
+            const size_t q = (size_t)p * FC + f;
+            const int c = fix[p][f];
+            fcell[q] = c;
+            frb[q] = (int)rj.size();
+            auto r = rec_at.find(c);
+            if (r != rec_at.end())
+                for (auto &e : r->second) {
+                    rj.push_back(e.first);
+                    rwf.push_back(e.second);
+                }
+            fre[q] = (int)rj.size();
+            fsb[q] = (int)sb.size();
+            auto t = src_at[p].find(c);
+            if (t != src_at[p].end())
+                for (auto &e : t->second) {
+                    sb.push_back(e.first);
+                    sc.push_back(c);
+                    sw.push_back(e.second);
+                }
+            fse[q] = (int)sb.size();
+        }
+    }
+    if (rj.size() > (size_t)INT32_MAX || sb.size() > (size_t)INT32_MAX)
+        return fail(USCT_EINVAL, "footprint lists exceed 2^31 entries");
+    std::vector<int> ids(std::max(passes, 1));
+    for (int p = 0; p < (int)ids.size(); ++p) ids[p] = p;
+    TRY(upload(P, &P->fp_roff, roff));
+    TRY(upload(P, &P->fp_rcell, rcell));
+    TRY(upload(P, &P->fp_rw, rw));
+    TRY(upload(P, &P->fp_eb, eb));
+    TRY(upload(P, &P->fp_ec, ec));
+    TRY(upload(P, &P->fp_ew, ew));
+    TRY(upload(P, &P->fp_fcell, fcell));
+    TRY(upload(P, &P->fp_frb, frb));
+    TRY(upload(P, &P->fp_fre, fre));
+    TRY(upload(P, &P->fp_fsb, fsb));
+    TRY(upload(P, &P->fp_fse, fse));
+    TRY(upload(P, &P->fp_rj, rj));
+    TRY(upload(P, &P->fp_rwf, rwf));
+    TRY(upload(P, &P->fp_sb, sb));
+    TRY(upload(P, &P->fp_sc, sc));
+    TRY(upload(P, &P->fp_sw, sw));
+    TRY(upload(P, &P->fp_pass_ids, ids));
+    TRY(dalloc_t(P, &P->fp_pass, 1));
+    HIPX(hipMemcpy(P->fp_pass, ids.data(), sizeof(int), hipMemcpyHostToDevice));
+    TRY(dalloc_t(P, &P->fp_ecoef, std::max<int64_t>((int64_t)eb.size(), 1)));
+    TRY(dalloc_t(P, &P->fp_scoef, std::max<int64_t>((int64_t)sb.size(), 1)));
+    TRY(dalloc_t(P, &P->wav, P->nt));
+    P->fp_SE = SE;
+    P->fp_FC = FC;
+    P->fp_nse = (int64_t)sb.size();
+    P->fp = true;
+    return 0;
 }
 
 }  // namespace
@@ -1361,6 +1620,37 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     return 0;
 }
 
+int usct_plan_create_groups(usct_plan **out, int ndim, const int *dims, double h, double dt,
+                            int nt, int n_tx, const int64_t *tx_off, const int64_t *tx_cells,
+                            const double *tx_w, int n_rec, const int64_t *rec_off,
+                            const int64_t *rec_cells, const double *rec_w, int method,
+                            int ckpt_every, int64_t mem_budget, int flags) {
+    if (!out || !dims) return fail(USCT_EINVAL, "null argument");
+    *out = nullptr;
+    if (ndim != 2 && ndim != 3) return fail(USCT_EINVAL, "ndim must be 2 or 3 (got %d)", ndim);
+    int64_t N = 1;
+    for (int d = 0; d < ndim; ++d) {
+        if (dims[d] < 1) return fail(USCT_EINVAL, "dims[%d] = %d must be positive", d, dims[d]);
+        N *= dims[d];
+    }
+    if (n_tx < 0 || n_rec < 1) return fail(USCT_EINVAL, "need n_tx >= 0 and n_rec >= 1");
+    TRY(check_groups_arg("tx", n_tx, tx_off, tx_cells, tx_w, N));
+    TRY(check_groups_arg("rec", n_rec, rec_off, rec_cells, rec_w, N));
+    std::vector<int64_t> tx_first(std::max(n_tx, 1)), rec_first(n_rec);
+    for (int t = 0; t < n_tx; ++t) tx_first[t] = tx_cells[tx_off[t]];
+    for (int j = 0; j < n_rec; ++j) rec_first[j] = rec_cells[rec_off[j]];
+    usct_plan *P = nullptr;
+    TRY(usct_plan_create(&P, ndim, dims, h, dt, nt, tx_first.data(), n_tx, rec_first.data(), n_rec,
+                         method, ckpt_every, mem_budget, flags));
+    const int rc = attach_groups(P, tx_off, tx_cells, tx_w, rec_off, rec_cells, rec_w);
+    if (rc) {
+        usct_plan_destroy(P);
+        return rc;
+    }
+    *out = P;
+    return 0;
+}
+
 int usct_plan_destroy(usct_plan *P) {
     if (!P) return 0;
     if (P->stream) (void)hipStreamSynchronize(P->stream);
@@ -1414,6 +1704,23 @@ int usct_misfit_and_gradient(usct_plan *P, const double *m, const double *wavele
     if (!J || !grad) return fail(USCT_EINVAL, "J and grad are required");
     return run(P, MODE_GRAD, misfit_type, m, wavelet, sponge, dobs, nullptr, J, grad, stream,
                nullptr);
+}
+
+int usct_misfit_tw(usct_plan *P, const double *m, const double *wavelet, const double *sponge,
+                   const double *dobs, const double *time_weights, int misfit_type, double *J,
+                   void *stream) {
+    if (!J) return fail(USCT_EINVAL, "J is required");
+    return run(P, MODE_MISFIT, misfit_type, m, wavelet, sponge, dobs, nullptr, J, nullptr, stream,
+               nullptr, nullptr, time_weights);
+}
+
+int usct_misfit_and_gradient_tw(usct_plan *P, const double *m, const double *wavelet,
+                                const double *sponge, const double *dobs,
+                                const double *time_weights, int misfit_type, double *J,
+                                double *grad, void *stream) {
+    if (!J || !grad) return fail(USCT_EINVAL, "J and grad are required");
+    return run(P, MODE_GRAD, misfit_type, m, wavelet, sponge, dobs, nullptr, J, grad, stream,
+               nullptr, nullptr, time_weights);
 }
 
 int usct_gn_hessian_vector(usct_plan *P, const double *m, const double *wavelet,

@@ -1723,6 +1723,132 @@ k_born(const float *__restrict__ u0, const uint16_t *__restrict__ u0l,
 }
 
 // ---------------------------------------------------------------------------
+// Finite-aperture elements (footprints, fwi.py:74-89, 316-319): an element is a
+// group of cells with weights.  Plans with such elements run the stencil
+// kernels with no point source and no receivers; after each level these
+// kernels add the weighted sources, record the receiver groups and correct the
+// adjoint.  Lists of the current shot pass are padded to a fixed stride and
+// selected by *pass (a device int set before each pass), so one captured
+// graph serves every pass.
+// ---------------------------------------------------------------------------
+
+// coef[e] = S_c w_e = (dt^2 / m[cell_e]) w_e for every transmit entry (set up per
+// call, outside the captured sequence, from the caller's model).
+__global__ void k_fp_coef(const int *__restrict__ cell, const double *__restrict__ wt,
+                          double *__restrict__ coef, int64_t count, const double *__restrict__ m,
+                          double dt2) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    coef[e] = cell[e] >= 0 ? (dt2 / m[cell[e]]) * wt[e] : 0.0;
+}
+
+// Transmit entries of the pass (shot eb, cell ec, coefficient ecoef = S_c w_i;
+// eb < 0 pads): level n gains ecoef wav[n-1], times the sponge, added in FP64 to
+// the stored hi + lo (lap[cell] += wavelet[n-1] * w_i before p_new = 2p - p_prev
+// + S lap, fwi.py:64-69).  Each (cell, shot) appears once per pass.
+__global__ void k_fp_source(float *__restrict__ hi, uint16_t *__restrict__ lo,
+                            const int *__restrict__ pass, int stride, const int *__restrict__ eb,
+                            const int *__restrict__ ec, const double *__restrict__ ecoef,
+                            const double *__restrict__ wav, int n,
+                            const float *__restrict__ sponge, int Bp) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= stride) return;
+    const int64_t q = (int64_t)(*pass) * stride + e;
+    const int b = eb[q];
+    if (b < 0) return;
+    const int c = ec[q];
+    const int64_t i = (int64_t)c * Bp + b;
+    double add = ecoef[q] * wav[n - 1];
+    if (sponge) add = add * (double)sponge[c];
+    uint16_t l;
+    const float h = split_hl(hl(hi[i], lo[i]) + add, l);
+    hi[i] = h;
+    lo[i] = l;
+}
+
+// Receiver group j of shot b at one level: sum_i w_i p[cell_i] in FP64, in
+// footprint order (fwi.py:74-76), into out_hi[j * Bp + b] and, when out_lo is
+// given, the FP32 remainder.  Thread per (j, b).
+__global__ void k_fp_record(const float *__restrict__ hi, const uint16_t *__restrict__ lo,
+                            const int *__restrict__ roff, const int *__restrict__ rcell,
+                            const double *__restrict__ rw, float *__restrict__ out_hi,
+                            float *__restrict__ out_lo, int n_rec, int Bp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_rec * Bp) return;
+    const int j = (int)(t / Bp), b = (int)(t - (int64_t)j * Bp);
+    double acc = 0.0;
+    for (int k = roff[j]; k < roff[j + 1]; ++k) {
+        const int64_t i = (int64_t)rcell[k] * Bp + b;
+        acc = acc + rw[k] * hl(hi[i], lo[i]);
+    }
+    const float h = (float)acc;
+    out_hi[t] = h;
+    if (out_lo) out_lo[t] = (float)(acc - (double)h);
+}
+
+// After adjoint level k: lk holds lam'^k from the stencil kernel (sponge
+// applied, no receiver injected) and the group accumulators hold lam'^k d2 with
+// d2 the kernel's second difference (no sponge: S' lap(p^{k-1}) with no source
+// term; sponge: p^k - 2p^{k-1} + p^{k-2} of the stored levels).  For each fix
+// cell c of the pass and shot group g (one thread, shots in order):
+//   lam^k[c] = lam'^k[c] - sum_j res[k][j] w_j   (receiver order, fwi.py:316-319)
+//   g[g][c] += (lam^k - lam'^k) d2 + lam^k s     (s = S_c w_i wav[k-1] for a transmit
+//                                                 cell of that shot; 0 with the sponge,
+//                                                 whose d2 already holds the source)
+// Lists per fix cell: receiver entries [frb, fre) of (rj, rw), transmit entries
+// [fsb, fse) of (sb, scoef = S_c w_i).  fcell < 0 pads.
+template <bool SPONGE>
+__global__ void __launch_bounds__(256)
+k_fp_adjoint(float *__restrict__ lk, const float *__restrict__ res, const float *__restrict__ pa,
+             const float *__restrict__ pb, const float *__restrict__ pc2,
+             const float2 *__restrict__ sp2, const int *__restrict__ pass, int stride,
+             const int *__restrict__ fcell, const int *__restrict__ frb,
+             const int *__restrict__ fre, const int *__restrict__ rj,
+             const float *__restrict__ rw, const int *__restrict__ fsb,
+             const int *__restrict__ fse, const int *__restrict__ sb,
+             const double *__restrict__ scoef, const double *__restrict__ wav, int k,
+             float *__restrict__ g_hi,
+             float *__restrict__ g_lo, int LB, int ng, int nz, int ny, int nx, int ndim, int Bp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)stride * ng) return;
+    const int f = (int)(t / ng), g = (int)(t - (int64_t)f * ng);
+    const int64_t q = (int64_t)(*pass) * stride + f;
+    const int c = fcell[q];
+    if (c < 0) return;
+    const int x = c % nx, y = (c / nx) % ny, z = c / (nx * ny);
+    const bool in = ndim == 3 ? (z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 &&
+                                 x <= nx - 2)
+                              : (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2);
+    const int64_t sx = Bp, sy = (int64_t)nx * Bp, sz = sy * ny;
+    const int64_t gi = (int64_t)g * nz * ny * nx + c;
+    float gh = g_hi[gi], gl = g_lo[gi];
+    for (int bb = 0; bb < LB; ++bb) {
+        const int b = g * LB + bb;
+        const int64_t i = (int64_t)c * Bp + b;
+        const float l0 = lk[i];
+        float v = l0;
+        for (int r = frb[q]; r < fre[q]; ++r) v = v - res[(int64_t)rj[r] * Bp + b] * rw[r];
+        lk[i] = v;
+        float d2, src = 0.0f;
+        if (SPONGE) {
+            d2 = (pa[i] - 2.0f * pb[i]) + pc2[i];
+        } else {
+            float lp = 0.0f;
+            if (in)
+                lp = ndim == 3 ? lap7(pa[i], pa[i + sz], pa[i - sz], pa[i + sy], pa[i - sy],
+                                      pa[i + sx], pa[i - sx])
+                               : lap5(pa[i], pa[i + sy], pa[i - sy], pa[i + sx], pa[i - sx]);
+            d2 = smul(sp2[c], lp);
+            for (int e = fsb[q]; e < fse[q]; ++e)
+                if (sb[e] == b) src = src + (float)(scoef[e] * wav[k - 1]);
+        }
+        neumaier(gh, gl, (v - l0) * d2 + v * src);
+    }
+    g_hi[gi] = gh;
+    g_lo[gi] = gl;
+}
+
+// ---------------------------------------------------------------------------
 // Set-up and trace kernels
 // ---------------------------------------------------------------------------
 // vm2[c] = v[c] / m[c] as FP32 hi + lo (the Born source's scale).
@@ -1833,12 +1959,15 @@ k_batched_to_data(const float *__restrict__ rec, const float *__restrict__ rec_l
 __global__ void __launch_bounds__(kBlock)
 k_residual_l2(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
               const float *__restrict__ dobs, const float *__restrict__ dobs_lo,
-              float *__restrict__ res, int64_t count, float2 *__restrict__ part) {
+              const float *__restrict__ W, float *__restrict__ res, int64_t count,
+              float2 *__restrict__ part) {
     float s = 0.0f, c = 0.0f;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
          i += (int64_t)gridDim.x * blockDim.x) {
-        const float r = (dsyn[i] - dobs[i]) + (dsyn_lo[i] - dobs_lo[i]);
-        if (res) res[i] = r;
+        float r = (dsyn[i] - dobs[i]) + (dsyn_lo[i] - dobs_lo[i]);
+        // windowed (fwi.py:298-301): residual of W s against W o, adjoint source times W
+        if (W) r = r * W[i];
+        if (res) res[i] = W ? r * W[i] : r;
         neumaier(s, c, r * r);
     }
     __shared__ float sh[kBlock], ch[kBlock];
@@ -1876,14 +2005,16 @@ __global__ void k_reduce_partials(const float2 *__restrict__ part, int n, double
 // stride n_rec*Bp between time samples.
 __global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
                             const float *__restrict__ dobs, const float *__restrict__ dobs_lo,
-                            double *__restrict__ stats, int nt, int64_t ntr) {
+                            const float *__restrict__ W, double *__restrict__ stats, int nt,
+                            int64_t ntr) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntr) return;
     double ss = 0.0, oo = 0.0, so = 0.0;
     for (int n = 0; n < nt; ++n) {
         const int64_t q = (int64_t)n * ntr + t;
-        const double s = (double)dsyn[q] + (double)dsyn_lo[q];
-        const double o = (double)dobs[q] + (double)dobs_lo[q];
+        const double wq = W ? (double)W[q] : 1.0;
+        const double s = ((double)dsyn[q] + (double)dsyn_lo[q]) * wq;
+        const double o = ((double)dobs[q] + (double)dobs_lo[q]) * wq;
         ss += s * s;
         oo += o * o;
         so += s * o;
@@ -1894,10 +2025,11 @@ __global__ void k_gcn_stats(const float *__restrict__ dsyn, const float *__restr
 }
 
 // GCN adjoint source res = -(o/no - c s/ns)/ns and per-trace misfit 1 - c
-// (fwi.py:164-167); tmask[t] = 0 for padded traces.
+// (fwi.py:164-167); tmask[t] = 0 for padded traces.  With a window W the
+// traces are W s and W o and the source is multiplied by W (fwi.py:298-301).
 __global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__restrict__ dsyn_lo,
                                const float *__restrict__ dobs, const float *__restrict__ dobs_lo,
-                               const double *__restrict__ stats,
+                               const float *__restrict__ W, const double *__restrict__ stats,
                                const unsigned char *__restrict__ tmask, float *__restrict__ res,
                                int nt, int64_t ntr) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1907,9 +2039,10 @@ __global__ void k_gcn_residual(const float *__restrict__ dsyn, const float *__re
     const double eps = 1e-12;
     const double ns = sqrt(stats[3 * t]) + eps, no = sqrt(stats[3 * t + 1]) + eps;
     const double c = stats[3 * t + 2] / (ns * no);
-    const double s = (double)dsyn[i] + (double)dsyn_lo[i];
-    const double o = (double)dobs[i] + (double)dobs_lo[i];
-    res[i] = (float)(-(o / no - c * (s / ns)) / ns);
+    const double wq = W ? (double)W[i] : 1.0;
+    const double s = ((double)dsyn[i] + (double)dsyn_lo[i]) * wq;
+    const double o = ((double)dobs[i] + (double)dobs_lo[i]) * wq;
+    res[i] = (float)((-(o / no - c * (s / ns)) / ns) * wq);
 }
 
 __global__ void k_gcn_misfit(const double *__restrict__ stats, const unsigned char *__restrict__ tmask,

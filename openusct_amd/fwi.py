@@ -65,18 +65,31 @@ def _tx_rec(geom, shape, src_list):
     return tx, rec
 
 
-def get_plan(shape, h, dt, nt, tx, rec, sponge, method=None):
-    """Plan for this problem, reused across calls (line searches, iterations)."""
+def _groups_key(groups):
+    if groups is None:
+        return None
+    return tuple((np.asarray(c, np.int64).tobytes(), np.asarray(w, np.float64).tobytes())
+                 for c, w in groups)
+
+
+def get_plan(shape, h, dt, nt, tx, rec, sponge, method=None, groups=None):
+    """Plan for this problem, reused across calls (line searches, iterations).
+    ``groups`` = (transmit groups, receiver groups) for finite-aperture elements."""
     method = method or METHOD
+    gk = None if groups is None else (_groups_key(groups[0]), _groups_key(groups[1]))
     key = (tuple(shape), float(h), float(dt), int(nt), tx.tobytes(), rec.tobytes(), bool(sponge),
-           method, torch.cuda.current_device())
+           method, gk, torch.cuda.current_device())
     plan = _plans.get(key)
     if plan is not None:
         _plans.move_to_end(key)
         return plan
     while len(_plans) >= _CACHE_SIZE:
         _plans.popitem(last=False)[1].close()
-    plan = Plan(shape, h, dt, nt, tx, rec, method=method, sponge=sponge)
+    if groups is None:
+        plan = Plan(shape, h, dt, nt, tx, rec, method=method, sponge=sponge)
+    else:
+        plan = Plan(shape, h, dt, nt, tx, rec, method=method, sponge=sponge,
+                    tx_groups=groups[0], rec_groups=groups[1])
     _plans[key] = plan
     return plan
 
@@ -96,7 +109,26 @@ def _shape_of(m):
     return tuple(m.shape)
 
 
-def _setup(m, geom, wavelet, dt, h, nt, sponge, src_list, method=None):
+def _element_groups(geom, shape, src_list, footprints):
+    """Transmit and receiver groups (linear cells, weights) from footprints
+    (geometry.py:212-278 format: per element a list of index tuples and weights)."""
+    if len(footprints) != geom.n_elements:
+        raise ValueError(f"footprints has {len(footprints)} entries, the array "
+                         f"{geom.n_elements} elements")
+    rec = []
+    for idxs, w in footprints:
+        w = np.asarray(w, dtype=np.float64).reshape(-1)
+        if len(idxs) != len(w) or len(w) == 0:
+            raise ValueError("each footprint needs one weight per grid point")
+        rec.append((linear_indices(list(idxs), shape), w))
+    src = list(range(geom.n_elements)) if src_list is None else [int(s) for s in src_list]
+    for s in src:
+        if not 0 <= s < geom.n_elements:
+            raise ValueError(f"src_list entry {s} is not an element of the array")
+    return [rec[s] for s in src], rec
+
+
+def _setup(m, geom, wavelet, dt, h, nt, sponge, src_list, method=None, footprints=None):
     shape = _shape_of(m)
     if len(shape) not in (2, 3):
         raise ValueError(f"m must be 2D or 3D, got shape {shape}")
@@ -107,7 +139,8 @@ def _setup(m, geom, wavelet, dt, h, nt, sponge, src_list, method=None):
     if sponge is not None and len(shape) != 2:
         raise NotImplementedError("the sponge is defined for 2D grids only")
     tx, rec = _tx_rec(geom, shape, src_list)
-    plan = get_plan(shape, h, dt, nt, tx, rec, sponge is not None, method)
+    groups = None if footprints is None else _element_groups(geom, shape, src_list, footprints)
+    plan = get_plan(shape, h, dt, nt, tx, rec, sponge is not None, method, groups)
     md = _to_dev(m, torch.float64)
     wd = _to_dev(np.asarray(wavelet, dtype=np.float64)[:nt], torch.float64)
     sd = None if sponge is None else _to_dev(sponge, torch.float64)
@@ -115,10 +148,10 @@ def _setup(m, geom, wavelet, dt, h, nt, sponge, src_list, method=None):
 
 
 def forward_fmc(m, geom, wavelet, dt, h, nt, sponge=None, src_list=None, footprints=None):
-    """Full-matrix capture (fwi.py:102-119): data (n_src, nt, n_elements) float64."""
-    if footprints is not None:
-        raise NotImplementedError("finite-aperture footprints are not implemented on the device")
-    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list)
+    """Full-matrix capture (fwi.py:102-119): data (n_src, nt, n_elements) float64.
+    ``footprints`` (geometry.build_footprints) models finite-aperture elements."""
+    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list,
+                              footprints=footprints)
     d = plan.forward_fmc(md, wd, sd)
     return d.double().cpu().numpy()
 
@@ -130,27 +163,39 @@ def _dobs_dev(dobs, plan):
     return _to_dev(dobs, torch.float64)
 
 
+def _windows_dev(time_weights, plan):
+    """time_weights: a list over transmits of (nt, n_rx) windows (fwi.py:250-253)."""
+    if time_weights is None:
+        return None
+    tw = [np.asarray(w, dtype=np.float64) for w in time_weights]
+    if len(tw) != plan.n_tx or any(w.shape != (plan.nt, plan.n_rec) for w in tw):
+        raise ValueError(f"time_weights must be {plan.n_tx} arrays of shape "
+                         f"{(plan.nt, plan.n_rec)}")
+    return _to_dev(np.stack(tw) if tw else np.zeros((0, plan.nt, plan.n_rec)), torch.float64)
+
+
 def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, misfit_type="l2",
            footprints=None, time_weights=None):
-    """Misfit only (fwi.py:246-263)."""
-    if footprints is not None or time_weights is not None:
-        raise NotImplementedError("footprints / time_weights are not implemented on the device")
+    """Misfit only (fwi.py:246-263), optionally with footprints and time windows."""
     if misfit_type not in ("l2", "gcn"):
         raise NotImplementedError(f"misfit_type {misfit_type!r} is not implemented on the device")
-    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list)
-    J = plan.misfit(md, wd, _dobs_dev(dobs, plan), sd, misfit_type)
+    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list,
+                              footprints=footprints)
+    J = plan.misfit(md, wd, _dobs_dev(dobs, plan), sd, misfit_type,
+                    time_weights=_windows_dev(time_weights, plan))
     return float(J.item())
 
 
 def misfit_and_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None,
                         misfit_type="l2", footprints=None, time_weights=None):
-    """Misfit and exact discrete-adjoint gradient w.r.t. m (fwi.py:266-332)."""
-    if footprints is not None or time_weights is not None:
-        raise NotImplementedError("footprints / time_weights are not implemented on the device")
+    """Misfit and exact discrete-adjoint gradient w.r.t. m (fwi.py:266-332),
+    optionally with finite-aperture footprints and time windows."""
     if misfit_type not in ("l2", "gcn"):
         raise NotImplementedError(f"misfit_type {misfit_type!r} is not implemented on the device")
-    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list)
-    J, g = plan.misfit_and_gradient(md, wd, _dobs_dev(dobs, plan), sd, misfit_type)
+    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list,
+                              footprints=footprints)
+    J, g = plan.misfit_and_gradient(md, wd, _dobs_dev(dobs, plan), sd, misfit_type,
+                                    time_weights=_windows_dev(time_weights, plan))
     return float(J.item()), g.cpu().numpy()
 
 
@@ -183,9 +228,8 @@ def gn_hessian_vector(m, geom, wavelet, dt, h, nt, v, sponge=None, src_list=None
     accepted and not needed: the device recomputes the background wavefield
     from checkpoints inside each call, which gives the same histories.
     """
-    if footprints is not None:
-        raise NotImplementedError("finite-aperture footprints are not implemented on the device")
-    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list)
+    plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list,
+                              footprints=footprints)
     if tuple(np.shape(v)) != plan.shape:
         raise ValueError(f"v has shape {tuple(np.shape(v))}, expected {plan.shape}")
     return plan.gn_hessian_vector(md, wd, _to_dev(v, torch.float64), sd).cpu().numpy()
@@ -196,11 +240,13 @@ def _dot(a, b):
 
 
 def _invert_gauss_newton(m0, geom, wavelet, dt, h, nt, dobs, sponge, src_list, n_iter,
-                         update_mask, m_bounds, n_linesearch, verbose, progress, n_cg, gn_damp):
+                         update_mask, m_bounds, n_linesearch, verbose, progress, n_cg, gn_damp,
+                         footprints=None):
     """Truncated Gauss-Newton (fwi.py:434-531): conjugate gradients on
     (J^T J + mu I) dm = -g with mu = gn_damp <g, Hg> / <g, g>, then a halving
     line search from step 1.  Model, gradient and CG vectors stay on the device."""
-    plan, m, wd, sd = _setup(m0, geom, wavelet, dt, h, nt, sponge, src_list)
+    plan, m, wd, sd = _setup(m0, geom, wavelet, dt, h, nt, sponge, src_list,
+                             footprints=footprints)
     dd = _dobs_dev(dobs, plan)
     m = m.clone()
     lo = hi = None
@@ -283,17 +329,22 @@ def invert(m0, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, n_ite
             raise ValueError("Gauss-Newton is defined for the L2 misfit; select misfit_type='l2'.")
         if time_weights is not None:
             raise ValueError("Gauss-Newton does not support time_weights yet.")
-        if footprints is not None:
-            raise NotImplementedError(
-                "finite-aperture footprints are not implemented on the device")
         return _invert_gauss_newton(m0, geom, wavelet, dt, h, nt, dobs, sponge, src_list,
                                     n_iter, update_mask, m_bounds, n_linesearch, verbose,
-                                    progress, n_cg, gn_damp)
-    be = backend if backend is not None else _THIS
-    mg = lambda mm: be.misfit_and_gradient(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,
-                                           misfit_type=misfit_type)
-    mf = lambda mm: be.misfit(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,
-                              misfit_type=misfit_type)
+                                    progress, n_cg, gn_damp, footprints)
+    if footprints is not None or time_weights is not None:
+        # the reference evaluates these on its Python path whatever the backend
+        # (fwi.py:604-615); here the device runs them
+        mg = lambda mm: misfit_and_gradient(mm, geom, wavelet, dt, h, nt, dobs, sponge,  # noqa
+                                            src_list, misfit_type, footprints, time_weights)
+        mf = lambda mm: misfit(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,  # noqa
+                               misfit_type, footprints, time_weights)
+    else:
+        be = backend if backend is not None else _THIS
+        mg = lambda mm: be.misfit_and_gradient(mm, geom, wavelet, dt, h, nt, dobs, sponge,  # noqa
+                                               src_list, misfit_type=misfit_type)
+        mf = lambda mm: be.misfit(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,  # noqa
+                                  misfit_type=misfit_type)
     m = np.array(m0, dtype=np.float64, copy=True)
     lo = hi = None
     if m_bounds is not None:

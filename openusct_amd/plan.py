@@ -46,13 +46,25 @@ class Plan:
 
     def __init__(self, shape, h, dt, nt, tx_lin, rec_lin, method="batched", ckpt_every=0,
                  mem_budget=0, sponge=False, autotune=True, graph=True, forward_only=False,
-                 fuse=True):
+                 fuse=True, tx_groups=None, rec_groups=None):
+        """``tx_lin`` / ``rec_lin``: one linear cell per transmit / receiver.  With
+        ``tx_groups`` and ``rec_groups`` (lists of (linear cells, weights) per
+        element, as finite-aperture footprints) those replace the point cells and
+        ``tx_lin`` / ``rec_lin`` are ignored."""
         lib = _lib.load()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
         self.h, self.dt, self.nt = float(h), float(dt), int(nt)
-        self.tx = np.ascontiguousarray(tx_lin, dtype=np.int64).reshape(-1)
-        self.rec = np.ascontiguousarray(rec_lin, dtype=np.int64).reshape(-1)
+        self.groups = tx_groups is not None or rec_groups is not None
+        if self.groups:
+            if tx_groups is None or rec_groups is None:
+                raise ValueError("tx_groups and rec_groups go together")
+            tx_csr, rec_csr = _csr(tx_groups), _csr(rec_groups)
+            self.tx = tx_csr[1][tx_csr[0][:-1]] if len(tx_groups) else np.zeros(0, np.int64)
+            self.rec = rec_csr[1][rec_csr[0][:-1]]
+        else:
+            self.tx = np.ascontiguousarray(tx_lin, dtype=np.int64).reshape(-1)
+            self.rec = np.ascontiguousarray(rec_lin, dtype=np.int64).reshape(-1)
         self.n_tx, self.n_rec = len(self.tx), len(self.rec)
         self.N = int(np.prod(self.shape))
         self.sponge = bool(sponge)
@@ -64,11 +76,18 @@ class Plan:
         dims = (ctypes.c_int * self.ndim)(*self.shape)
         handle = ctypes.c_void_p()
         torch.cuda.init()
-        rc = lib.usct_plan_create(
-            ctypes.byref(handle), self.ndim, dims, self.h, self.dt, self.nt,
-            self.tx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), self.n_tx,
-            self.rec.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), self.n_rec,
-            _METHODS[method], int(ckpt_every), int(mem_budget), flags)
+        i64 = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))  # noqa: E731
+        if self.groups:
+            rc = lib.usct_plan_create_groups(
+                ctypes.byref(handle), self.ndim, dims, self.h, self.dt, self.nt, self.n_tx,
+                i64(tx_csr[0]), i64(tx_csr[1]), tx_csr[2].ctypes.data, self.n_rec,
+                i64(rec_csr[0]), i64(rec_csr[1]), rec_csr[2].ctypes.data, _METHODS[method],
+                int(ckpt_every), int(mem_budget), flags)
+        else:
+            rc = lib.usct_plan_create(
+                ctypes.byref(handle), self.ndim, dims, self.h, self.dt, self.nt, i64(self.tx),
+                self.n_tx, i64(self.rec), self.n_rec, _METHODS[method], int(ckpt_every),
+                int(mem_budget), flags)
         _lib.check(rc)
         self._h = handle
         self.device = torch.device("cuda", torch.cuda.current_device())
@@ -123,11 +142,14 @@ class Plan:
             raise ValueError(f"sponge has shape {tuple(sponge.shape)}, expected {self.shape}")
         return sponge
 
-    def _dobs(self, dobs):
+    def _dobs(self, dobs, what="dobs"):
         want = (self.n_tx, self.nt, self.n_rec)
         if tuple(dobs.shape) != want:
-            raise ValueError(f"dobs has shape {tuple(dobs.shape)}, expected {want}")
+            raise ValueError(f"{what} has shape {tuple(dobs.shape)}, expected {want}")
         return dobs.to(device=self.device, dtype=torch.float64).contiguous()
+
+    def _windows(self, time_weights):
+        return None if time_weights is None else self._dobs(time_weights, "time_weights")
 
     # -- operations -------------------------------------------------------------
     def forward_fmc(self, m, wavelet, sponge=None, out=None, stream=None):
@@ -140,27 +162,46 @@ class Plan:
                                                 _stream_handle(stream)))
         return out
 
-    def misfit(self, m, wavelet, dobs, sponge=None, misfit_type="l2", stream=None):
-        """Misfit as a 1-element float64 device tensor."""
+    def misfit(self, m, wavelet, dobs, sponge=None, misfit_type="l2", stream=None,
+               time_weights=None):
+        """Misfit as a 1-element float64 device tensor; ``time_weights`` (the
+        shape of dobs) windows every trace (fwi.py:255-262)."""
         if misfit_type not in _lib.MISFIT_CODES:
             raise ValueError(f"unknown misfit_type {misfit_type!r}")
         m, w, sp, d = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(dobs)
+        tw = self._windows(time_weights)
         J = torch.empty(1, dtype=torch.float64, device=self.device)
-        _lib.check(_lib.load().usct_misfit(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d),
-                                           _lib.MISFIT_CODES[misfit_type], _ptr(J),
-                                           _stream_handle(stream)))
+        lib = _lib.load()
+        if tw is None:
+            rc = lib.usct_misfit(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d),
+                                 _lib.MISFIT_CODES[misfit_type], _ptr(J), _stream_handle(stream))
+        else:
+            rc = lib.usct_misfit_tw(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d), _ptr(tw),
+                                    _lib.MISFIT_CODES[misfit_type], _ptr(J),
+                                    _stream_handle(stream))
+        _lib.check(rc)
         return J
 
-    def misfit_and_gradient(self, m, wavelet, dobs, sponge=None, misfit_type="l2", stream=None):
-        """(J, g): J a 1-element float64 tensor, g float64 with the model's shape."""
+    def misfit_and_gradient(self, m, wavelet, dobs, sponge=None, misfit_type="l2", stream=None,
+                            time_weights=None):
+        """(J, g): J a 1-element float64 tensor, g float64 with the model's shape;
+        ``time_weights`` as for misfit (fwi.py:298-301)."""
         if misfit_type not in _lib.MISFIT_CODES:
             raise ValueError(f"unknown misfit_type {misfit_type!r}")
         m, w, sp, d = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(dobs)
+        tw = self._windows(time_weights)
         J = torch.empty(1, dtype=torch.float64, device=self.device)
         g = torch.empty(self.shape, dtype=torch.float64, device=self.device)
-        _lib.check(_lib.load().usct_misfit_and_gradient(
-            self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d), _lib.MISFIT_CODES[misfit_type],
-            _ptr(J), _ptr(g), _stream_handle(stream)))
+        lib = _lib.load()
+        if tw is None:
+            rc = lib.usct_misfit_and_gradient(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d),
+                                              _lib.MISFIT_CODES[misfit_type], _ptr(J), _ptr(g),
+                                              _stream_handle(stream))
+        else:
+            rc = lib.usct_misfit_and_gradient_tw(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d),
+                                                 _ptr(tw), _lib.MISFIT_CODES[misfit_type],
+                                                 _ptr(J), _ptr(g), _stream_handle(stream))
+        _lib.check(rc)
         return J, g
 
     def gn_hessian_vector(self, m, wavelet, v, sponge=None, out=None, stream=None):
@@ -188,6 +229,21 @@ class Plan:
             ctypes.byref(nf), ctypes.byref(na), ctypes.byref(lv), _stream_handle(stream)))
         return {"fwd_ms": fwd.value, "adj_ms": adj.value, "n_fwd": nf.value, "n_adj": na.value,
                 "levels_per_launch": lv.value}
+
+
+def _csr(groups):
+    """(off, cells, weights) arrays of a list of (linear cells, weights) groups."""
+    off, cells, w = [0], [], []
+    for c, wt in groups:
+        c = np.asarray(c, dtype=np.int64).reshape(-1)
+        wt = np.asarray(wt, dtype=np.float64).reshape(-1)
+        if len(c) != len(wt) or len(c) == 0:
+            raise ValueError("each group needs one weight per cell and at least one cell")
+        cells.extend(c.tolist())
+        w.extend(wt.tolist())
+        off.append(len(cells))
+    return (np.asarray(off, dtype=np.int64), np.asarray(cells or [0], dtype=np.int64),
+            np.asarray(w or [0.0], dtype=np.float64))
 
 
 def footprint_bytes(shape, nt, n_tx, n_rec, ckpt_every=0, gradient=True):

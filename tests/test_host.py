@@ -141,12 +141,17 @@ def test_fwi_rejects_unsupported_options_before_touching_the_device():
     g = IdxGeom([(1, 1)])
     m = np.full((8, 8), 1.0 / 1500.0 ** 2)
     with pytest.raises(NotImplementedError):
-        fwi.forward_fmc(m, g, np.zeros(4), 1e-8, 1e-3, 4, footprints=[])
-    with pytest.raises(NotImplementedError):
         fwi.misfit(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)), misfit_type="envelope")
     with pytest.raises(NotImplementedError):
         fwi.misfit_and_gradient(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)),
-                                time_weights=[np.ones((4, 1))])
+                                misfit_type="traveltime", time_weights=[np.ones((4, 1))])
+    with pytest.raises(ValueError):  # one footprint per element
+        fwi._element_groups(g, (8, 8), None, [])
+    with pytest.raises(ValueError):  # a weight per point
+        fwi._element_groups(g, (8, 8), None, [([(1, 1), (1, 2)], np.array([1.0]))])
+    with pytest.raises(ValueError):
+        fwi.invert(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)),
+                   optimizer="gauss-newton", misfit_type="gcn")
 
 
 def test_plugin_registry():
@@ -224,3 +229,21 @@ def test_p_window_weights_match_reference():
                                onset_frac=0.15, keep_cycles=2.0)
     assert np.max(np.abs(np.stack(twf) - f["twf"])) <= 1e-12
     assert all(((w >= 0) & (w <= 1)).all() for w in tw)
+
+
+def test_element_groups_to_csr():
+    from openusct_amd import fwi
+    from openusct_amd.geometry import RingArray, build_footprints
+    from openusct_amd.plan import _csr
+    g = RingArray(n_elements=8, radius_m=0.024, domain_m=0.06, h=1e-3)
+    fps = build_footprints(g, 0.003, "flat")
+    tx, rec = fwi._element_groups(g, (g.n, g.n), [0, 4], fps)
+    assert len(tx) == 2 and len(rec) == 8
+    assert np.array_equal(tx[1][0], rec[4][0])
+    off, cells, w = _csr(rec)
+    assert off[0] == 0 and off[-1] == len(cells) == sum(len(c) for c, _ in fps)
+    for j, (c, wt) in enumerate(rec):
+        assert np.array_equal(cells[off[j]:off[j + 1]], c)
+        assert np.array_equal(w[off[j]:off[j + 1]], wt)
+    with pytest.raises(ValueError):
+        _csr([(np.array([3]), np.array([]))])
