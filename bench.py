@@ -50,10 +50,12 @@ WORKLOADS = {
 }
 
 
-def serving(n_requests, nt_scale, rate=150.0, seed=5, with_3d=True):
+def serving(n_requests, nt_scale, rate=75.0, seed=5, with_3d=True):
     """Run the serving mix (make_mix: seed 5, arrivals for ~80 % utilisation at
-    `rate` Gpt-updates/s) through the continuous-batching server; observed data
-    of gradient requests is synthesised on the device first (not timed)."""
+    `rate` Gpt-updates/s, about what the mix of 2D / 3D gradient and forward
+    requests sustains on one MI355X) through the continuous-batching server;
+    observed data of gradient requests is synthesised on the device first (not
+    timed)."""
     from openusct_amd.serve import Server, fill_observed, make_mix, summarize
 
     reqs = make_mix(n_requests, seed=seed, nt_scale=nt_scale, rate=rate, with_3d=with_3d)
@@ -154,6 +156,9 @@ def main():
                          "after the timed gradients (default: 5 for C3, else 0)")
     ap.add_argument("--serve-nt-scale", type=float, default=0.05,
                     help="time-axis scale of the serving requests")
+    ap.add_argument("--gn-products", type=int, default=-1,
+                    help="Gauss-Newton Hessian-vector products timed on the same plan "
+                         "(default: 3 for C2, else 0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -234,6 +239,29 @@ def main():
               "through": "openusct_amd.autograd.SoundSpeedMisfit (c -> J, backward = device adjoint)"}
         print(f"[bench] gradient descent {gd_steps} steps in {gd['seconds']} s: {hist}",
               file=sys.stderr, flush=True)
+
+    gn = None
+    gn_k = args.gn_products if args.gn_products >= 0 else (3 if args.workload == "C2" else 0)
+    if gn_k:
+        # Gauss-Newton products (J^T J) v: background forward, Born forward and
+        # adjoint sweep per call; v a smooth direction of model size
+        v = (g / g.abs().max()) * md.mean()
+        plan.gn_hessian_vector(md, wd, v)  # first call allocates the Born state
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(gn_k):
+            hv = plan.gn_hessian_vector(md, wd, v)
+        torch.cuda.synchronize()
+        ms_hv = (time.perf_counter() - t0) / gn_k * 1e3
+        upd_hv = 3.0 * N * (nt - 1) * len(tx)
+        gn = {"products": gn_k, "ms_per_product": round(ms_hv, 3),
+              "vs_gradient": round(ms_hv / ms, 3),
+              "gpt_updates_per_s": round(upd_hv / (ms_hv / 1e3) / 1e9, 3),
+              "counted": "3 sweeps of N (nt-1) shots: background forward, Born forward, adjoint",
+              "finite": bool(torch.isfinite(hv).all().item())}
+        print(f"[bench] Gauss-Newton product {ms_hv:.1f} ms ({gn['vs_gradient']}x a gradient), "
+              f"{gn['gpt_updates_per_s']} Gpt-updates/s", file=sys.stderr, flush=True)
+        del hv, v
 
     if args.no_profile:
         prof = {"fwd_ms": float("nan"), "adj_ms": float("nan"), "n_fwd": 0, "n_adj": 0}
@@ -343,6 +371,7 @@ def main():
         "cpu_baseline": cpu,
         "serving": sv,
         "gradient_descent": gd,
+        "gauss_newton": gn,
     }
     print(json.dumps(line), flush=True)
 
