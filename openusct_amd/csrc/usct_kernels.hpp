@@ -40,6 +40,27 @@ namespace usct {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;  // 4 wavefronts
 
+// Minimum wavefronts per SIMD the compiler must leave room for (caps the
+// registers of a kernel family); 0 = the compiler's choice.  The 2D kernels
+// run 3 per SIMD (measured on the 512^2, 128-shot gradient: 3% faster than the
+// compiler's choice; 4 spills and is 1.8x slower); the 3D kernels gain nothing.
+#ifndef USCT_OCC2D
+#define USCT_OCC2D 3
+#endif
+#ifndef USCT_OCC3D
+#define USCT_OCC3D 0
+#endif
+#if USCT_OCC2D > 0
+#define USCT_ATTR2D __attribute__((amdgpu_waves_per_eu(USCT_OCC2D)))
+#else
+#define USCT_ATTR2D
+#endif
+#if USCT_OCC3D > 0
+#define USCT_ATTR3D __attribute__((amdgpu_waves_per_eu(USCT_OCC3D)))
+#else
+#define USCT_ATTR3D
+#endif
+
 // Blocks are dispatched round-robin over the 8 XCDs (block i -> XCD i % 8).
 // Give each XCD a contiguous range of logical blocks so neighbouring strips,
 // which share halo rows, share an L2.
@@ -200,7 +221,7 @@ __device__ __forceinline__ float split_hl(double v, uint16_t &lo) {
 // border cells, whose Laplacian is zero.
 // ---------------------------------------------------------------------------
 template <int LB, int CX, bool SPONGE, bool REC>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
 k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
         const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
         uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
@@ -298,7 +319,7 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 __device__ __forceinline__ size_t clamp_row(int y, int ny) { return (size_t)min(max(y, 0), ny - 1); }
 
 template <int LB, int CX, bool REC>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
 k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
            float *__restrict__ pb, const uint16_t *__restrict__ pcl,
            const uint16_t *__restrict__ ppl, uint16_t *__restrict__ pal,
@@ -462,7 +483,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
 // guard zone, and join the reductions with zero contributions).
 // ---------------------------------------------------------------------------
 template <int LB, int CX, bool SPONGE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
 k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
         const float *__restrict__ pa, const float *__restrict__ pb, const float *__restrict__ pc2,
         const float2 *__restrict__ sp2, const float *__restrict__ sponge,
@@ -605,7 +626,7 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 //   sinc_u/sinc_v: S_src w[k-1], S_src w[k-2]; res_u/res_v: residual rows k, k-1.
 // ---------------------------------------------------------------------------
 template <int LB, int CX>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
 k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lu,
            float *__restrict__ lv, const float *__restrict__ pa1, const float *__restrict__ pa2,
            const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
@@ -684,8 +705,19 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
 #pragma unroll
         for (int i = 0; i < CX; ++i) { l1p[i] = 0.0f; up[i] = 0.0f; }
     }
+    // the lane's reduced cell (the same on every row); its accumulators of the row
+    // updated at the end of an iteration are loaded at the iteration's start
+    const int gx = x0 + sum_cell<LB, CX>(lane);
+    const bool gval = (sb & (LB / CX - 1)) == 0 && gx < nx;
+    float *const ghb = g_hi + (size_t)sg * N + gx;
+    float *const glb = g_lo + (size_t)sg * N + gx;
     for (int yf = y0 - 1; yf <= y1; ++yf) {
         const size_t yr = clamp_row(yf, ny), yn = clamp_row(yf + 1, ny);
+        float gh0 = 0.0f, gl0 = 0.0f;
+        if (gval && yf - 1 >= y0 && yf - 1 < y1) {
+            gh0 = ghb[(size_t)(yf - 1) * nx];
+            gl0 = glb[(size_t)(yf - 1) * nx];
+        }
         // ---- loads for the next iteration
         float ln2[CA + 2], l2n[CA], fn2[CX + 2], hn[CX + 2];
         {
@@ -777,17 +809,12 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
                 }
             }
             const float s_v = reduce_scatter<LB, CX>(tv, lane);
-            if ((sb & (LB / CX - 1)) == 0) {
-                const int x = x0 + sum_cell<LB, CX>(lane);
-                if (x < nx) {
-                    float *gh = g_hi + (size_t)sg * N + (size_t)y * nx;
-                    float *gl = g_lo + (size_t)sg * N + (size_t)y * nx;
-                    float hi = gh[x], lo = gl[x];
-                    neumaier(hi, lo, s_prev);
-                    neumaier(hi, lo, s_v);
-                    gh[x] = hi;
-                    gl[x] = lo;
-                }
+            if (gval) {
+                float hi = gh0, lo = gl0;
+                neumaier(hi, lo, s_prev);
+                neumaier(hi, lo, s_v);
+                ghb[(size_t)y * nx] = hi;
+                glb[(size_t)y * nx] = lo;
             }
         }
         // ---- advance
@@ -825,7 +852,7 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
 // ---------------------------------------------------------------------------
 
 template <int LB, int CX, int CY, bool REC>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
         const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
         uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
@@ -992,7 +1019,7 @@ struct Ring {  // halo cells of a TX x TY tile, without corners: r -> (hx, hy) i
 // 3D forward level (FP64 update of k_fwd2d / k_fwd3d) with an LDS-staged tile.
 // Grid: ng * ntx * nty * nzc work-groups of 256 lanes.
 template <int LB, int TX, int TY, int PX, int PY, bool REC>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_fwd3d_lds(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
             const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
             uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
@@ -1172,7 +1199,7 @@ k_fwd3d_lds(const float *__restrict__ pc, const float *__restrict__ pp, float *_
 // 3D adjoint level with the imaging condition (as k_adj3d), LDS-staged tiles of
 // S' lam^{k+1} and p^{k-1}.  Grid as k_fwd3d_lds.
 template <int LB, int TX, int TY, int PX, int PY>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_adj3d_lds(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
             const float *__restrict__ pa, const float2 *__restrict__ sp2,
             const int *__restrict__ src_cell, const int *__restrict__ rec_head,
@@ -1309,10 +1336,27 @@ k_adj3d_lds(const float *__restrict__ l1, const float *__restrict__ l2, float *_
         load_raw(min(z0 + 1, nz - 1), z0);
         __syncthreads();
     }
+    // the lane's reduced cell (the same on every plane) and its gradient
+    // accumulators, loaded one plane ahead so the update does not wait on them
+    const bool gown = (bl & (LB / PC - 1)) == 0;
+    const int gci = sum_cell<LB, PC>(lane);
+    const int gx = x0 + gci % PX, gy = y0 + gci / PX;
+    const bool gval = gown && gx < nx && gy < ny;
+    const size_t gbase = (size_t)sg * N + (size_t)gy * nx + gx;
+    float ghn = 0.0f, gln = 0.0f;
+    if (gval) {
+        ghn = g_hi[gbase + (size_t)z0 * planeC];
+        gln = g_lo[gbase + (size_t)z0 * planeC];
+    }
     for (int z = z0; z < z1; ++z) {
         const int cur = (z - z0) & 1;
         const int zpi = min(z + 1, nz - 1);
         float l2c[PY][PX], lnx[PY][PX];
+        const float gh0 = ghn, gl0 = gln;
+        if (gval && z + 1 < z1) {
+            ghn = g_hi[gbase + (size_t)(z + 1) * planeC];
+            gln = g_lo[gbase + (size_t)(z + 1) * planeC];
+        }
         own_q(zpi, qp);
 #pragma unroll
         for (int j = 0; j < PY; ++j)
@@ -1368,16 +1412,11 @@ k_adj3d_lds(const float *__restrict__ l1, const float *__restrict__ l2, float *_
             }
         }
         const float s = reduce_scatter<LB, PC>(tv, lane);
-        if ((bl & (LB / PC - 1)) == 0) {
-            const int ci = sum_cell<LB, PC>(lane);
-            const int x = x0 + ci % PX, y = y0 + ci / PX;
-            if (x < nx && y < ny) {
-                const size_t gi = (size_t)sg * N + ((size_t)z * ny + y) * nx + x;
-                float hi = g_hi[gi], lo = g_lo[gi];
-                neumaier(hi, lo, s);
-                g_hi[gi] = hi;
-                g_lo[gi] = lo;
-            }
+        if (gval) {
+            float hi = gh0, lo = gl0;
+            neumaier(hi, lo, s);
+            g_hi[gbase + (size_t)z * planeC] = hi;
+            g_lo[gbase + (size_t)z * planeC] = lo;
         }
 #pragma unroll
         for (int j = 0; j < PY; ++j)
@@ -1398,7 +1437,7 @@ k_adj3d_lds(const float *__restrict__ l1, const float *__restrict__ l2, float *_
 // the reference sponge is 2D only, solver.py:30).  Addressing as k_fwd3d.
 // ---------------------------------------------------------------------------
 template <int LB, int CX, int CY>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
         const float *__restrict__ pa, const float2 *__restrict__ sp2,
         const int *__restrict__ src_cell, const float *__restrict__ sinc,
@@ -1488,12 +1527,29 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         load_halo(l0, s0, f0);
         load_raw((size_t)min(z0 + 1, nz - 1) * planeE, (size_t)z0 * planeE);
     }
+    // the lane's reduced cell (the same on every plane) and its gradient
+    // accumulators, loaded one plane ahead so the update does not wait on them
+    const bool gown = (sb & (LB / CC - 1)) == 0;
+    const int gci = sum_cell<LB, CC>(lane);
+    const int gx = x0 + gci % CX, gy = y0 + gci / CX;
+    const bool gval = gown && gx < nx && gy < ny;
+    const size_t gbase = (size_t)sg * N + (size_t)gy * nx + gx;
+    float ghn = 0.0f, gln = 0.0f;
+    if (gval) {
+        ghn = g_hi[gbase + (size_t)z0 * planeC];
+        gln = g_lo[gbase + (size_t)z0 * planeC];
+    }
     for (int z = z0; z < z1; ++z) {
         const int zpi = min(z + 1, nz - 1);
         const float *lnz = l1 + (size_t)zpi * planeE;
         const float2 *snz = sp2 + (size_t)zpi * planeC;
         const float *fnz = pa + (size_t)zpi * planeE;
         float l2c[CY][CX];
+        const float gh0 = ghn, gl0 = gln;
+        if (gval && z + 1 < z1) {
+            ghn = g_hi[gbase + (size_t)(z + 1) * planeC];
+            gln = g_lo[gbase + (size_t)(z + 1) * planeC];
+        }
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
@@ -1541,16 +1597,11 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
             }
         }
         const float s = reduce_scatter<LB, CC>(tv, lane);
-        if ((sb & (LB / CC - 1)) == 0) {
-            const int ci = sum_cell<LB, CC>(lane);
-            const int x = x0 + ci % CX, y = y0 + ci / CX;
-            if (x < nx && y < ny) {
-                const size_t gi = (size_t)sg * N + ((size_t)z * ny + y) * nx + x;
-                float hi = g_hi[gi], lo = g_lo[gi];
-                neumaier(hi, lo, s);
-                g_hi[gi] = hi;
-                g_lo[gi] = lo;
-            }
+        if (gval) {
+            float hi = gh0, lo = gl0;
+            neumaier(hi, lo, s);
+            g_hi[gbase + (size_t)z * planeC] = hi;
+            g_lo[gbase + (size_t)z * planeC] = lo;
         }
         if (z + 1 < z1) {
 #pragma unroll
