@@ -50,12 +50,12 @@ WORKLOADS = {
 }
 
 
-def serving(n_requests, nt_scale, rate=75.0, seed=5, with_3d=True):
+def serving(n_requests, nt_scale, rate=150.0, seed=5, with_3d=True):
     """Run the serving mix (make_mix: seed 5, arrivals for ~80 % utilisation at
-    `rate` Gpt-updates/s, about what the mix of 2D / 3D gradient and forward
-    requests sustains on one MI355X) through the continuous-batching server;
-    observed data of gradient requests is synthesised on the device first (not
-    timed)."""
+    `rate` Gpt-updates/s; the mix of 2D / 3D gradient and forward requests at
+    time axes x0.1 sustains about 160 on one MI355X when all requests are
+    queued at once) through the continuous-batching server; observed data of
+    gradient requests is synthesised on the device first (not timed)."""
     from openusct_amd.serve import Server, fill_observed, make_mix, summarize
 
     reqs = make_mix(n_requests, seed=seed, nt_scale=nt_scale, rate=rate, with_3d=with_3d)

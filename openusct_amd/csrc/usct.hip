@@ -104,7 +104,10 @@ struct usct_plan {
     // Tiles of the forward and of the adjoint kernels (chosen independently: the
     // field layout does not depend on them; the gradient accumulators use the
     // adjoint's lane groups).
-    Tiling tf{64, 8, 1, 32, 0}, ta{64, 8, 1, 32, 0};
+    // Defaults (plans without the autotuner: serving, tests) are the shapes the
+    // tuner picks on the 512^2 / 256^3 workloads: 2D forward 8 and adjoint 4
+    // cells per lane, 3D 2 x 2 lane patches.
+    Tiling tf{64, 8, 1, 32, 0}, ta{64, 4, 1, 32, 0};
     int Bp = 1;        // padded shots per pass
     int passes = 1;
     int K = 0, NS = 1;  // checkpoint interval and number of windows
@@ -1573,7 +1576,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     P->method = method == USCT_METHOD_PLAIN ? USCT_METHOD_PLAIN : USCT_METHOD_BATCHED;
     P->flags = flags;
     P->sponge = (flags & USCT_FLAG_SPONGE) != 0;
-    if (ndim == 3) P->tf = P->ta = Tiling{64, 4, 4, 32, 0};
+    if (ndim == 3) P->tf = P->ta = Tiling{64, 2, 2, 32, 0};
     int rc = 0;
     auto bail = [&](int code) {
         free_plan_memory(P);
