@@ -186,6 +186,22 @@ __device__ __forceinline__ T ldn(const T *base, unsigned boff) {
 #endif
     return ldb(base, boff);
 }
+template <class T>
+__device__ __forceinline__ T ld_nt(const T *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
+template <class T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
 template <int NT, class T>
 __device__ __forceinline__ void stn(T *base, unsigned boff, T v) {
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -1794,13 +1810,16 @@ k_born(const float *__restrict__ u0, const uint16_t *__restrict__ u0l,
     } else if (y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2) {
         lu = lap5d(D(i), D(i + sy), D(i - sy), D(i + sx), D(i - sx));
     }
-    const double d2u = (hl(u0[i], u0l[i]) - 2.0 * hl(u1[i], u1l[i])) + hl(u2[i], u2l[i]);
-    double nv = (2.0 * D(i) - hl(d2[i], d2l[i])) + s2d(sp2[c]) * lu - s2d(vm2[c]) * d2u;
+    // background levels and dp^{n-2} are read once: non-temporal (no L2 retention)
+    const double d2u = (hl(ld_nt(u0 + i), ld_nt(u0l + i)) - 2.0 * hl(ld_nt(u1 + i), ld_nt(u1l + i))) +
+                       hl(ld_nt(u2 + i), ld_nt(u2l + i));
+    double nv = (2.0 * D(i) - hl(ld_nt(d2 + i), ld_nt(d2l + i))) + s2d(sp2[c]) * lu -
+                s2d(vm2[c]) * d2u;
     if (SPONGE) nv = nv * (double)sponge[c];
     uint16_t lb;
     const float v = split_hl(nv, lb);
-    dn[i] = v;
-    dnl[i] = lb;
+    st_nt(dn + i, v);
+    st_nt(dnl + i, lb);
     for (int j = rec_head[c]; j >= 0; j = rec_next[j]) res[(int64_t)j * Bp + b] = v;
 }
 
