@@ -139,7 +139,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="C2", choices=sorted(WORKLOADS))
     ap.add_argument("--shots", type=int, default=0, help="use only the first N shots")
-    ap.add_argument("--plain-shots", type=int, default=128)
+    ap.add_argument("--plain-shots", type=int, default=0,
+                    help="shots of the plain per-shot path timed beside the batched one "
+                         "(default: 128 in 2D, 2 for C4, whose plain plan keeps 134 GB of history)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ckpt-every", type=int, default=0)
@@ -288,8 +290,8 @@ def main():
     torch.cuda.empty_cache()
 
     plain = None
-    if not args.no_plain and args.workload in ("C2", "C2-256"):
-        ps = min(args.plain_shots, len(tx))
+    if not args.no_plain and args.workload in ("C2", "C2-256", "C4"):
+        ps = min(args.plain_shots or (2 if len(shape) == 3 else 128), len(tx))
         pplan = Plan(shape, h, dt, nt, tx[:ps], rec, method="plain")
         pd = pplan.forward_fmc(mt, wd)
         pplan.misfit_and_gradient(md, wd, pd)  # warm-up
