@@ -247,3 +247,37 @@ def test_element_groups_to_csr():
         assert np.array_equal(w[off[j]:off[j + 1]], wt)
     with pytest.raises(ValueError):
         _csr([(np.array([3]), np.array([]))])
+
+
+def test_group_plan_rejects_bad_groups_before_any_device_call():
+    """usct_plan_create_groups checks its CSR groups on the host: empty groups,
+    cells outside the grid, non-finite weights and a bad first offset are
+    USCT_EINVAL with a message, before any HIP call."""
+    from openusct_amd import _lib
+    lib = _lib.load()
+    dims = (ctypes.c_int * 2)(8, 8)
+    i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)  # noqa: E731
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+
+    def create(tx_off, tx_cells, tx_w, rec_off, rec_cells, rec_w):
+        plan = ctypes.c_void_p()
+        P = ctypes.POINTER(ctypes.c_int64)
+        rc = lib.usct_plan_create_groups(
+            ctypes.byref(plan), 2, dims, 1e-3, 1e-7, 10, len(tx_off) - 1,
+            tx_off.ctypes.data_as(P), tx_cells.ctypes.data_as(P), tx_w.ctypes.data,
+            len(rec_off) - 1, rec_off.ctypes.data_as(P), rec_cells.ctypes.data_as(P),
+            rec_w.ctypes.data, 2, 0, 0, 0)
+        return rc, lib.usct_last_error().decode()
+
+    good = (i64([0, 2]), i64([9, 10]), f64([0.5, 0.5]))
+    cases = {
+        "empty": ((i64([0, 0]), i64([0]), f64([1.0])), "empty"),
+        "outside": ((i64([0, 1]), i64([64]), f64([1.0])), "outside the grid"),
+        "weight": ((i64([0, 1]), i64([3]), f64([np.nan])), "not finite"),
+        "offset": ((i64([1, 2]), i64([3, 4]), f64([1.0, 1.0])), "off[0]"),
+    }
+    for name, (bad, msg) in cases.items():
+        rc, err = create(*bad, *good)
+        assert rc == _lib.USCT_EINVAL and msg in err, (name, rc, err)
+        rc, err = create(*good, *bad)
+        assert rc == _lib.USCT_EINVAL and msg in err, (name, rc, err)
