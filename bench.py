@@ -270,20 +270,23 @@ def main():
     else:
         prof = plan.profile_gradient(md, wd, dobs)
     per_pass = len(tx) / max(info["shot_passes"], 1)
-    lv = max(int(prof.get("levels_per_launch", 1)), 1)  # time levels per launch
+    lv = max(int(prof.get("adj_levels", 1)), 1)    # time levels per adjoint launch
+    lvf = max(int(prof.get("fwd_levels", 1)), 1)   # and per forward launch
     adj_bytes = BYTES_ADJ * N * per_pass * lv
-    fwd_bytes = BYTES_FWD * N * per_pass * lv
+    fwd_bytes = BYTES_FWD * N * per_pass * lvf
     adj_gbs = adj_bytes / (prof["adj_ms"] / 1e3) / 1e9
     fwd_gbs = fwd_bytes / (prof["fwd_ms"] / 1e3) / 1e9
-    print(f"[bench] per launch ({lv} level(s)): fwd {prof['fwd_ms']:.3f} ms ({fwd_gbs:.0f} GB/s "
-          f"alg), adj {prof['adj_ms']:.3f} ms ({adj_gbs:.0f} GB/s alg)", file=sys.stderr,
+    print(f"[bench] per launch ({lvf}/{lv} level(s)): fwd {prof['fwd_ms']:.3f} ms ({fwd_gbs:.0f} "
+          f"GB/s alg), adj {prof['adj_ms']:.3f} ms ({adj_gbs:.0f} GB/s alg)", file=sys.stderr,
           flush=True)
     suffix = "_x2" if lv == 2 else ""
     # minimum bytes of the adjoint kernel per launch: per shot-cell it reads
     # lam^{k+1}, lam^{k+2} and one stored level per time level, writes lam per level
     kmin_bytes = (4.0 * (2 + lv) + 4.0 * lv) * N * per_pass
     adj_kernel = ("k_adj2d" if len(shape) == 2 else "k_adj3d") + suffix
-    fwd_kernel = ("k_fwd2d" if len(shape) == 2 else "k_fwd3d") + suffix
+    fwd_kernel = ("k_fwd2d" if len(shape) == 2 else "k_fwd3d") + ("_x2" if lvf == 2 else "")
+    if len(shape) == 3 and info.get("fwd_tile_kind") == 2:
+        fwd_kernel = "k_fwd3d_x2_lds"
     Jv = float(J.item())
     plan.close()
     del dobs
@@ -367,7 +370,7 @@ def main():
             "kernel_min_frac": round(kmin_bytes / (prof["adj_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
             "fwd_kernel": {"name": fwd_kernel, "achieved": round(fwd_gbs, 1),
                            "per_launch_ms": round(prof["fwd_ms"], 4),
-                           "alg_bytes_per_launch": fwd_bytes},
+                           "alg_bytes_per_launch": fwd_bytes, "levels_per_launch": lvf},
             "gradient_level": round(value * BYTES_PER_UPDATE, 1),
         },
         "cpu_baseline": cpu,

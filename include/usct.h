@@ -85,7 +85,8 @@ typedef struct {
     int fwd_rows_per_wave;
     int tile_kind;            /* adjoint, 3D: 0 = cells per lane (tile_x x tile_y),
                                  1 = LDS-staged work-group tile of tile_x x tile_y cells */
-    int fwd_tile_kind;        /* the same for the forward kernel */
+    int fwd_tile_kind;        /* the same for the forward kernel; 2 = LDS-staged tiles
+                                 advancing two time levels per launch */
 } usct_plan_info;
 
 /*
@@ -164,10 +165,10 @@ int usct_gn_hessian_vector(usct_plan *plan, const double *m, const double *wavel
 
 /*
  * Time one gradient launch by launch with HIP events on the plan's stream.
- * levels_per_launch: time levels one launch advances (2 when the plan runs
- *   two-level kernels, else 1);
+ * levels_per_launch: int[2], time levels one forward / one adjoint launch
+ *   advances (2 for two-level kernels, else 1);
  * fwd_ms / adj_ms: mean duration of the forward / adjoint launches that advance
- *   levels_per_launch levels (a trailing single-level launch is not averaged in);
+ *   that many levels (a trailing single-level launch is not averaged in);
  * n_fwd / n_adj: time levels advanced in all (re-computed levels count as forward).
  */
 int usct_profile_gradient(usct_plan *plan, const double *m, const double *wavelet,

@@ -74,7 +74,8 @@ enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2, MODE_GNHV = 3 };
 struct Tiling {
     int LB, CX, CY, RY;  // lanes per shot group, cells per lane (x, y), rows per wavefront
     int lds = 0;         // 3D: 1 = LDS-staged work-group tiles of CX x CY cells (patch per
-                         // lane from lds_patch), rows = planes per work-group
+                         // lane from lds_patch), rows = planes per work-group; 2 = the same
+                         // tiles advancing two forward levels per launch
 };
 
 // Lane patch of an LDS tile: 256 lanes = LB shots x (TX/PX) x (TY/PY) patches.
@@ -393,8 +394,32 @@ void fwd2d_x2_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const us
     else fwd2d_x2_t<LB, 8, REC>(g, s, pc, pp, pa, pb, P, n, t);
 }
 
+// Two 3D forward levels per launch on 8 x 8 LDS tiles of 16 shots.
+template <bool REC>
+void fwd3d_x2_lds_t(hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_plan *P, int n) {
+    constexpr int LB = 16, TX = 8, TY = 8;
+    const Tiling &T = P->tf;
+    const int ntx = (int)cdiv(P->nx, TX), nty = (int)cdiv(P->ny, TY);
+    const int nzc = (int)cdiv(P->nz, T.RY), ng = P->Bp / LB;
+    const dim3 g((unsigned)((int64_t)ng * ntx * nty * nzc));
+    const double *sa = P->sincd + (int64_t)n * P->Bp;
+    float *da = P->srcd2 + (int64_t)n * P->Bp;
+    const int64_t tr = (int64_t)P->n_rec * P->Bp;
+    float *ra = P->recb + (int64_t)n * tr, *rla = P->recb_lo + (int64_t)n * tr;
+    hipLaunchKernelGGL((k_fwd3d_x2_lds<LB, TX, TY, 2, 2, REC>), g, dim3(kBlock), 0, s, pc.hi,
+                       pp.hi, pa.hi, pb.hi, pc.lo, pp.lo, pa.lo, pb.lo, P->sp2, P->src_cell, sa,
+                       sa + P->Bp, P->rec_head, P->rec_next, P->row_rec, ra, rla, ra + tr,
+                       rla + tr, da, da + P->Bp, P->nz, P->ny, P->nx, P->Bp, T.RY, ntx, nty, nzc,
+                       ng);
+}
+
 void launch_fwd2_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb,
                          int n, bool record) {
+    if (P->ndim == 3) {
+        if (record) fwd3d_x2_lds_t<true>(s, pc, pp, pa, pb, P, n);
+        else fwd3d_x2_lds_t<false>(s, pc, pp, pa, pb, P, n);
+        return;
+    }
     const Tile t = tile_of(P, P->tf);
     const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
 #define USCT_X2(LB_)                                                          \
@@ -408,12 +433,17 @@ void launch_fwd2_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl 
 #undef USCT_X2
 }
 
-// Whether the plan advances two time levels per launch.
-bool fused(const usct_plan *P) {
-    // footprint plans correct every level after its launch, so one level per launch
-    return P->method == USCT_METHOD_BATCHED && P->ndim == 2 && !P->sponge && !P->fp &&
+// Whether the plan's forward / adjoint launches advance two time levels: 2D both
+// (no sponge), 3D the forward when its tile is the two-level LDS kind.  Footprint
+// plans correct every level after its launch, so one level per launch.
+bool fusable(const usct_plan *P) {
+    return P->method == USCT_METHOD_BATCHED && !P->sponge && !P->fp &&
            !(P->flags & USCT_FLAG_NO_FUSE);
 }
+bool fused_fwd(const usct_plan *P) {
+    return fusable(P) && (P->ndim == 2 || P->tf.lds == 2);
+}
+bool fused_adj(const usct_plan *P) { return fusable(P) && P->ndim == 2; }
 
 template <int LB, int CX, bool SP>
 void adj2d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
@@ -702,7 +732,7 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
                        P->recb, (int64_t)P->n_rec * P->Bp, 0.0f);
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
                        P->recb_lo, (int64_t)P->n_rec * P->Bp, 0.0f);
-    const bool x2 = fused(P);
+    const bool x2 = fused_fwd(P), x2a = fused_adj(P);
     if (born) born_clear(P, s);
     for (int n = 1; n < nt;) {
         const bool two = x2 && n + 1 < nt;
@@ -737,7 +767,7 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
             }
         }
         for (int k = khi; k >= base + 1;) {
-            const bool two = x2 && k - 1 >= base + 1;
+            const bool two = x2a && k - 1 >= base + 1;
             if (pr) ev_start(pr, pr->adj_ev, s);
             if (two) {
                 launch_adj2_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2),
@@ -1157,7 +1187,7 @@ double time_tile(usct_plan *P, bool adjoint) {
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     const int reps = 4;
-    const bool x2 = fused(P) && P->nt >= 3;  // levels 1 and 2 exist
+    const bool x2 = (adjoint ? fused_adj(P) : fused_fwd(P)) && P->nt >= 3;  // levels 1, 2 exist
     auto step = [&](int r) {
         Lvl L[4];
         for (int i = 0; i < 4; ++i) L[i] = {P->F[(r + i) % 4], P->Lo[(r + i) % 4]};
@@ -1182,7 +1212,7 @@ double time_tile(usct_plan *P, bool adjoint) {
     else (void)hipEventElapsedTime(&ms, a, b);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    return ms;
+    return x2 ? ms / 2.0 : ms;  // per time level: one- and two-level candidates compare
 }
 
 // Autotuner: lane-group width (among those that divide the padded shot count),
@@ -1247,6 +1277,9 @@ int autotune(usct_plan *P) {
                 for (auto tl : tiles)
                     for (int rz : {8, 16, 32, 64}) cands.push_back({lb, tl.first, tl.second, rz, 1});
             }
+            // two forward levels per launch on 8 x 8 tiles of 16 shots
+            if (!adjoint && P->Bp % 16 == 0 && fusable(P))
+                for (int rz : {8, 16, 32, 64}) cands.push_back({16, 8, 8, rz, 2});
         }
         Tiling &T = adjoint ? P->ta : P->tf;
         double best = 1e300;
@@ -1256,7 +1289,8 @@ int autotune(usct_plan *P) {
             const double t = time_tile(P, adjoint != 0);
             if (log && *log == '1')
                 std::fprintf(stderr, "[usct tune] %s lanes %d %s %dx%d rows %d: %.3f ms\n",
-                             adjoint ? "adjoint" : "forward", c.LB, c.lds ? "lds-tile" : "patch",
+                             adjoint ? "adjoint" : "forward", c.LB,
+                             c.lds == 2 ? "lds-tile-x2" : (c.lds ? "lds-tile" : "patch"),
                              c.CX, c.CY, c.RY, t);
             if (t < best) { best = t; bt = c; }
         }
@@ -1746,12 +1780,13 @@ int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,
     Profile pr;
     int rc = run(P, MODE_GRAD, USCT_MISFIT_L2, m, wavelet, sponge, dobs, nullptr, Jd, gd, stream, &pr);
     if (!rc) rc = hipStreamSynchronize(P->stream) == hipSuccess ? 0 : fail(USCT_EHIP, "sync");
-    int lv = 1;
-    for (int l : pr.fwd_lv) lv = std::max(lv, l);
-    for (int l : pr.adj_lv) lv = std::max(lv, l);
-    // mean over the launches that advance lv levels; levels counted over all
-    auto sum = [lv](std::vector<hipEvent_t> &v, const std::vector<int> &lvs, double *ms,
-                    int *levels) {
+    // per kind: mean over the launches that advance the most levels; levels
+    // counted over all launches
+    int lvf = 1, lva = 1;
+    for (int l : pr.fwd_lv) lvf = std::max(lvf, l);
+    for (int l : pr.adj_lv) lva = std::max(lva, l);
+    auto sum = [](std::vector<hipEvent_t> &v, const std::vector<int> &lvs, int lv, double *ms,
+                  int *levels) {
         double s = 0, s_any = 0;
         int c = 0, c_any = 0, total = 0;
         for (size_t i = 0; i + 1 < v.size(); i += 2) {
@@ -1767,9 +1802,12 @@ int usct_profile_gradient(usct_plan *P, const double *m, const double *wavelet,
         if (ms) *ms = c ? s / c : (c_any ? s_any / c_any : 0.0);
         if (levels) *levels = total;
     };
-    sum(pr.fwd_ev, pr.fwd_lv, fwd_ms, n_fwd);
-    sum(pr.adj_ev, pr.adj_lv, adj_ms, n_adj);
-    if (levels_per_launch) *levels_per_launch = lv;
+    sum(pr.fwd_ev, pr.fwd_lv, lvf, fwd_ms, n_fwd);
+    sum(pr.adj_ev, pr.adj_lv, lva, adj_ms, n_adj);
+    if (levels_per_launch) {
+        levels_per_launch[0] = lvf;
+        levels_per_launch[1] = lva;
+    }
     (void)hipFree(Jd);
     (void)hipFree(gd);
     return rc;

@@ -60,6 +60,12 @@ constexpr int kBlock = 256;  // 4 wavefronts
 #else
 #define USCT_ATTR3D
 #endif
+// The two-level 3D forward needs 256+ registers; 2 per SIMD (a 28-byte spill)
+// runs as fast as 1, 3 spills heavily and runs 2.3x slower.
+#ifndef USCT_OCC3DX2
+#define USCT_OCC3DX2 2
+#endif
+#define USCT_ATTR3DX2 __attribute__((amdgpu_waves_per_eu(USCT_OCC3DX2)))
 
 // Blocks are dispatched round-robin over the 8 XCDs (block i -> XCD i % 8).
 // Give each XCD a contiguous range of logical blocks so neighbouring strips,
@@ -1240,6 +1246,341 @@ k_fwd3d_lds(const float *__restrict__ pc, const float *__restrict__ pp, float *_
                 zcn[jj][ii] = zp[jj][ii];
             }
         __syncthreads();
+    }
+}
+
+// Cells of a (TX+4) x (TY+4) square around a TX x TY tile (padded coordinates,
+// tile at 2..TX+1), outside the tile: two rows above, four per tile row, two below.
+template <int TX, int TY>
+struct Halo2 {
+    static constexpr int GX = TX + 4;
+    static constexpr int N = 4 * GX + 4 * TY;
+    static __device__ __forceinline__ void at(int r, int &gx, int &gy) {
+        if (r < 2 * GX) {
+            gy = r / GX;
+            gx = r % GX;
+        } else if (r < 2 * GX + 4 * TY) {
+            const int q = r - 2 * GX, k = q % 4;
+            gy = 2 + q / 4;
+            gx = k < 2 ? k : TX + k;
+        } else {
+            const int q = r - 2 * GX - 4 * TY;
+            gy = TY + 2 + q / GX;
+            gx = q % GX;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Two 3D forward levels per launch (temporal blocking) on LDS-staged tiles.
+// A work-group of 256 lanes owns a TX x TY tile for LB shots (lane = shot x
+// PX x PY patch) and marches planes zz = z0-1 .. z1.  On plane zz it stages
+// level n-1 (A) of the tile with a two-cell halo, computes level n (C) on the
+// tile and its one-cell ring (no corners; each ring (cell, shot) has an owner
+// lane that keeps A of its cell on planes zz-1 .. zz+1) and keeps C, as
+// stored (hi + lo), in LDS; then level n+1 (D) of plane zz-1 follows from C on
+// planes zz-2 .. zz.  Every C and D cell takes the operations of the
+// single-level kernels on the same operands, so the two levels equal two
+// k_fwd3d launches bit for bit.  The planes z0-1 and z1 of C are re-computed
+// from the neighbouring chunks (not stored).  Per shot-cell the launch reads
+// A and B (level n-2) once and writes C and D: 24 B for two levels.
+// ---------------------------------------------------------------------------
+template <int LB, int TX, int TY, int PX, int PY, bool REC>
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3DX2
+k_fwd3d_x2_lds(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
+               float *__restrict__ pb, const uint16_t *__restrict__ pcl,
+               const uint16_t *__restrict__ ppl, uint16_t *__restrict__ pal,
+               uint16_t *__restrict__ pbl, const float2 *__restrict__ sp2,
+               const int *__restrict__ src_cell, const double *__restrict__ sinc_a,
+               const double *__restrict__ sinc_b, const int *__restrict__ rec_head,
+               const int *__restrict__ rec_next, const int *__restrict__ row_rec,
+               float *__restrict__ rec_a, float *__restrict__ rec_a_lo, float *__restrict__ rec_b,
+               float *__restrict__ rec_b_lo, float *__restrict__ srcd2_a,
+               float *__restrict__ srcd2_b, int nz, int ny, int nx, int Bp, int RZ, int ntx,
+               int nty, int nzc, int ng) {
+    constexpr int TCX = TX / PX, TCY = TY / PY;
+    static_assert(TCX * PX == TX && TCY * PY == TY && TCX * TCY * LB == kBlock, "tile shape");
+    constexpr int HX = TX + 2, HY = TY + 2;     // C: tile + ring, padded 0 .. TX+1
+    constexpr int GX = TX + 4, GY = TY + 4;     // A: tile + two-cell halo, padded 0 .. TX+3
+    constexpr int RN = Ring<TX, TY>::N * LB;    // C ring entries (cell, shot)
+    constexpr int RPL = (RN + kBlock - 1) / kBlock;
+    constexpr int AN = Halo2<TX, TY>::N * LB;   // A halo entries
+    constexpr int APL = (AN + kBlock - 1) / kBlock;
+    __shared__ double shA[GY * GX * LB];
+    __shared__ double shC[2][HY * HX * LB];
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int sg = blk % ng;
+    int t = blk / ng;
+    const int txi = t % ntx;
+    t /= ntx;
+    const int tyi = t % nty, zc = t / nty;
+    if (zc >= nzc) return;  // uniform over the work-group
+    const int tid = threadIdx.x;
+    const int bl = tid % LB, col = tid / LB;
+    const int cx = col % TCX, cy = col / TCX;
+    const int b = sg * LB + bl;
+    const int X0 = txi * TX, Y0 = tyi * TY;
+    const int x0 = X0 + cx * PX, y0 = Y0 + cy * PY;
+    const int z0 = zc * RZ, z1 = min(z0 + RZ, nz);
+    const int my_src = src_cell[b];
+    const double inc_a = sinc_a[b], inc_b = sinc_b[b];
+    const size_t planeE = (size_t)nx * ny * Bp, planeC = (size_t)nx * ny;
+    // element offsets from a plane base shifted back by two cells (x = -2 is 0);
+    // rows clamped into the grid (values of out-of-grid rows feed only out-of-grid cells)
+    auto off = [&](int y, int x, int shot) -> unsigned {
+        return (unsigned)(((min(max(y, 0), ny - 1) * nx + x + 2) * Bp + shot));
+    };
+    unsigned oo[PY][PX], ro[RPL], ao[APL];
+    int r_hx[RPL], r_hy[RPL], a_gx[APL], a_gy[APL];
+    bool r_on[RPL], a_on[APL];
+#pragma unroll
+    for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < PX; ++ii) oo[jj][ii] = off(y0 + jj, x0 + ii, b);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+        const int e = tid + q * kBlock;  // shot e % LB == bl (256 is a multiple of LB)
+        r_on[q] = e < RN;
+        Ring<TX, TY>::at(r_on[q] ? e / LB : 0, r_hx[q], r_hy[q]);
+        ro[q] = off(Y0 + r_hy[q] - 1, X0 + r_hx[q] - 1, b);
+    }
+#pragma unroll
+    for (int q = 0; q < APL; ++q) {
+        const int e = tid + q * kBlock;
+        a_on[q] = e < AN;
+        Halo2<TX, TY>::at(a_on[q] ? e / LB : 0, a_gx[q], a_gy[q]);
+        ao[q] = off(Y0 + a_gy[q] - 2, X0 + a_gx[q] - 2, b);
+    }
+    auto aidx = [&](int gy, int gx) { return (gy * GX + gx) * LB + bl; };
+    auto cidx = [&](int hy, int hx) { return (hy * HX + hx) * LB + bl; };
+    auto zcl = [&](int z) { return min(max(z, 0), nz - 1); };
+    auto R4 = [&](const float *f, int z) { return f + (ptrdiff_t)zcl(z) * (ptrdiff_t)planeE - 2 * Bp; };
+    auto R2 = [&](const uint16_t *f, int z) {
+        return f + (ptrdiff_t)zcl(z) * (ptrdiff_t)planeE - 2 * Bp;
+    };
+    auto Wf = [&](float *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - 2 * Bp; };
+    auto Wh = [&](uint16_t *f, int z) { return f + (ptrdiff_t)z * (ptrdiff_t)planeE - 2 * Bp; };
+    auto ld = [&](const float *h, const uint16_t *l, unsigned o) {
+        return hl(ldb(h, o * 4u), ldb(l, o * 2u));
+    };
+    // own cells: A on planes zz-1, zz, zz+1; B on zz; C on zz-2, zz-1, zz
+    double aM[PY][PX], aC[PY][PX], aP[PY][PX], bC[PY][PX], cM2[PY][PX], cM1[PY][PX];
+    // ring cells: A on zz-1, zz, zz+1; B on zz
+    double raM[RPL], raC[RPL], raP[RPL], rbC[RPL];
+    // raw loads one plane ahead: A own and ring on zz+2, B own and ring on zz+1,
+    // A halo on zz+1
+    float nah[PY][PX], nbh[PY][PX], nrah[RPL], nrbh[RPL], nhh[APL];
+    uint16_t nal[PY][PX], nbl[PY][PX], nral[RPL], nrbl[RPL], nhl[APL];
+    float hh[APL];
+    uint16_t hlo[APL];
+    auto load_raw = [&](int za, int zb, int zh) {
+        const float *a = R4(pc, za), *bb = R4(pp, zb), *h = R4(pc, zh);
+        const uint16_t *al = R2(pcl, za), *bbl = R2(ppl, zb), *hl2 = R2(pcl, zh);
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                nah[jj][ii] = ldb(a, oo[jj][ii] * 4u);
+                nal[jj][ii] = ldb(al, oo[jj][ii] * 2u);
+                nbh[jj][ii] = ldb(bb, oo[jj][ii] * 4u);
+                nbl[jj][ii] = ldb(bbl, oo[jj][ii] * 2u);
+            }
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            if (!r_on[q]) continue;
+            nrah[q] = ldb(a, ro[q] * 4u);
+            nral[q] = ldb(al, ro[q] * 2u);
+            nrbh[q] = ldb(bb, ro[q] * 4u);
+            nrbl[q] = ldb(bbl, ro[q] * 2u);
+        }
+#pragma unroll
+        for (int q = 0; q < APL; ++q) {
+            if (!a_on[q]) continue;
+            nhh[q] = ldb(h, ao[q] * 4u);
+            nhl[q] = ldb(hl2, ao[q] * 2u);
+        }
+    };
+    const int zs = z0 - 1, ze = z1;
+    {
+        const float *am = R4(pc, zs - 1), *a0 = R4(pc, zs), *ap = R4(pc, zs + 1), *b0 = R4(pp, zs);
+        const uint16_t *aml = R2(pcl, zs - 1), *a0l = R2(pcl, zs), *apl = R2(pcl, zs + 1),
+                       *b0l = R2(ppl, zs);
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                aM[jj][ii] = ld(am, aml, oo[jj][ii]);
+                aC[jj][ii] = ld(a0, a0l, oo[jj][ii]);
+                aP[jj][ii] = ld(ap, apl, oo[jj][ii]);
+                bC[jj][ii] = ld(b0, b0l, oo[jj][ii]);
+                cM2[jj][ii] = 0.0;
+                cM1[jj][ii] = 0.0;
+            }
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            raM[q] = raC[q] = raP[q] = rbC[q] = 0.0;
+            if (!r_on[q]) continue;
+            raM[q] = ld(am, aml, ro[q]);
+            raC[q] = ld(a0, a0l, ro[q]);
+            raP[q] = ld(ap, apl, ro[q]);
+            rbC[q] = ld(b0, b0l, ro[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < APL; ++q) {
+            hh[q] = 0.0f;
+            hlo[q] = 0;
+            if (!a_on[q]) continue;
+            hh[q] = ldb(a0, ao[q] * 4u);
+            hlo[q] = ldb(a0l, ao[q] * 2u);
+        }
+    }
+    for (int zz = zs; zz <= ze; ++zz) {
+        const int cur = zz & 1;
+        // 1. stage A on plane zz: own cells and the two-cell halo
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii)
+                shA[aidx(cy * PY + jj + 2, cx * PX + ii + 2)] = aC[jj][ii];
+#pragma unroll
+        for (int q = 0; q < APL; ++q)
+            if (a_on[q]) shA[aidx(a_gy[q], a_gx[q])] = hl(hh[q], hlo[q]);
+        if (zz < ze) load_raw(zz + 2, zz + 1, zz + 1);
+        __syncthreads();
+        // 2. C on plane zz: own cells (stored inside the chunk) and ring cells
+        const bool cval = zz >= 0 && zz <= nz - 1;
+        const bool cstore = zz >= z0 && zz < z1;
+        const bool zint = zz >= 1 && zz <= nz - 2;
+        double cC[PY][PX];
+        if (cval) {
+            const float2 *spz = sp2 + (size_t)zz * planeC;
+#pragma unroll
+            for (int jj = 0; jj < PY; ++jj) {
+                const int y = y0 + jj;
+                const bool yint = zint && y >= 1 && y <= ny - 2;
+                const bool has_rec = REC && cstore && y < ny && row_rec[zz * ny + y] > 0;
+                const int gy = cy * PY + jj + 2;
+#pragma unroll
+                for (int ii = 0; ii < PX; ++ii) {
+                    const int x = x0 + ii;
+                    const int gx = cx * PX + ii + 2;
+                    cC[jj][ii] = 0.0;
+                    if (x < nx && y < ny) {
+                        double lu = 0.0;
+                        if (yint && x >= 1 && x <= nx - 2)
+                            lu = lap7d(aC[jj][ii], aP[jj][ii], aM[jj][ii], shA[aidx(gy + 1, gx)],
+                                       shA[aidx(gy - 1, gx)], shA[aidx(gy, gx + 1)],
+                                       shA[aidx(gy, gx - 1)]);
+                        const int c = (zz * ny + y) * nx + x;
+                        double inc = s2d(spz[y * nx + x]) * lu;
+                        if (c == my_src) inc = inc + inc_a;
+                        const double nv = (2.0 * aC[jj][ii] - bC[jj][ii]) + inc;
+                        uint16_t lb;
+                        const float v = split_hl(nv, lb);
+                        cC[jj][ii] = hl(v, lb);
+                        if (cstore) {
+                            stb(Wf(pa, zz), oo[jj][ii] * 4u, v);
+                            stb(Wh(pal, zz), oo[jj][ii] * 2u, lb);
+                            if (c == my_src) srcd2_a[b] = (float)inc;
+                            if (has_rec) {
+                                for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
+                                    rec_a[(size_t)r * Bp + b] = v;
+                                    rec_a_lo[(size_t)r * Bp + b] = bf2f(lb);
+                                }
+                            }
+                        }
+                    }
+                    shC[cur][cidx(cy * PY + jj + 1, cx * PX + ii + 1)] = cC[jj][ii];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) {
+                if (!r_on[q]) continue;
+                const int x = X0 + r_hx[q] - 1, y = Y0 + r_hy[q] - 1;
+                double cv = 0.0;
+                if (x >= 0 && x < nx && y >= 0 && y < ny) {
+                    const int gx = r_hx[q] + 1, gy = r_hy[q] + 1;
+                    double lu = 0.0;
+                    if (zint && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+                        lu = lap7d(raC[q], raP[q], raM[q], shA[aidx(gy + 1, gx)],
+                                   shA[aidx(gy - 1, gx)], shA[aidx(gy, gx + 1)],
+                                   shA[aidx(gy, gx - 1)]);
+                    const int c = (zz * ny + y) * nx + x;
+                    double inc = s2d(spz[y * nx + x]) * lu;
+                    if (c == my_src) inc = inc + inc_a;
+                    const double nv = (2.0 * raC[q] - rbC[q]) + inc;
+                    uint16_t lb;
+                    const float v = split_hl(nv, lb);
+                    cv = hl(v, lb);
+                }
+                shC[cur][cidx(r_hy[q], r_hx[q])] = cv;
+            }
+        }
+        __syncthreads();
+        // 3. D on plane zd = zz - 1 (own cells) from C on planes zz-2 .. zz
+        const int zd = zz - 1;
+        if (zd >= z0 && zd < z1) {
+            const bool dint = zd >= 1 && zd <= nz - 2;
+            const float2 *spz = sp2 + (size_t)zd * planeC;
+#pragma unroll
+            for (int jj = 0; jj < PY; ++jj) {
+                const int y = y0 + jj;
+                const bool yint = dint && y >= 1 && y <= ny - 2;
+                const bool has_rec = REC && y < ny && row_rec[zd * ny + y] > 0;
+                const int hy = cy * PY + jj + 1;
+#pragma unroll
+                for (int ii = 0; ii < PX; ++ii) {
+                    const int x = x0 + ii;
+                    const int hx = cx * PX + ii + 1;
+                    if (x < nx && y < ny) {
+                        double lu = 0.0;
+                        if (yint && x >= 1 && x <= nx - 2)
+                            lu = lap7d(cM1[jj][ii], cC[jj][ii], cM2[jj][ii],
+                                       shC[cur ^ 1][cidx(hy + 1, hx)], shC[cur ^ 1][cidx(hy - 1, hx)],
+                                       shC[cur ^ 1][cidx(hy, hx + 1)], shC[cur ^ 1][cidx(hy, hx - 1)]);
+                        const int c = (zd * ny + y) * nx + x;
+                        double inc = s2d(spz[y * nx + x]) * lu;
+                        if (c == my_src) inc = inc + inc_b;
+                        const double nv = (2.0 * cM1[jj][ii] - aM[jj][ii]) + inc;
+                        uint16_t lb;
+                        const float v = split_hl(nv, lb);
+                        stb(Wf(pb, zd), oo[jj][ii] * 4u, v);
+                        stb(Wh(pbl, zd), oo[jj][ii] * 2u, lb);
+                        if (c == my_src) srcd2_b[b] = (float)inc;
+                        if (has_rec) {
+                            for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
+                                rec_b[(size_t)r * Bp + b] = v;
+                                rec_b_lo[(size_t)r * Bp + b] = bf2f(lb);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        // 4. advance one plane
+#pragma unroll
+        for (int jj = 0; jj < PY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < PX; ++ii) {
+                cM2[jj][ii] = cM1[jj][ii];
+                cM1[jj][ii] = cval ? cC[jj][ii] : 0.0;
+                aM[jj][ii] = aC[jj][ii];
+                aC[jj][ii] = aP[jj][ii];
+                aP[jj][ii] = hl(nah[jj][ii], nal[jj][ii]);
+                bC[jj][ii] = hl(nbh[jj][ii], nbl[jj][ii]);
+            }
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            raM[q] = raC[q];
+            raC[q] = raP[q];
+            raP[q] = r_on[q] ? hl(nrah[q], nral[q]) : 0.0;
+            rbC[q] = r_on[q] ? hl(nrbh[q], nrbl[q]) : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < APL; ++q) {
+            hh[q] = nhh[q];
+            hlo[q] = nhl[q];
+        }
     }
 }
 

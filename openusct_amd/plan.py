@@ -219,16 +219,17 @@ class Plan:
 
     def profile_gradient(self, m, wavelet, dobs, sponge=None, stream=None):
         """Kernel timing of one gradient: mean duration (ms) of the forward and of the
-        adjoint launches that advance ``levels_per_launch`` time levels each, and the
-        number of time levels advanced in all (re-computed levels count as forward)."""
+        adjoint launches that advance ``fwd_levels`` / ``adj_levels`` time levels each,
+        and the number of time levels advanced in all (re-computed levels count as
+        forward)."""
         m, w, sp, d = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(dobs)
         fwd, adj = ctypes.c_double(), ctypes.c_double()
-        nf, na, lv = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nf, na, lv = ctypes.c_int(), ctypes.c_int(), (ctypes.c_int * 2)()
         _lib.check(_lib.load().usct_profile_gradient(
             self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(d), ctypes.byref(fwd), ctypes.byref(adj),
-            ctypes.byref(nf), ctypes.byref(na), ctypes.byref(lv), _stream_handle(stream)))
+            ctypes.byref(nf), ctypes.byref(na), lv, _stream_handle(stream)))
         return {"fwd_ms": fwd.value, "adj_ms": adj.value, "n_fwd": nf.value, "n_adj": na.value,
-                "levels_per_launch": lv.value}
+                "fwd_levels": lv[0], "adj_levels": lv[1]}
 
 
 def _csr(groups):
