@@ -1,4 +1,5 @@
-"""Build libusct.so for gfx950 with hipcc (no cmake, no torch extension)."""
+"""Build libusct.so for gfx950 with hipcc and the host-only dataset-file
+library libusct_h5.so with g++ (no cmake, no torch extension)."""
 
 from __future__ import annotations
 
@@ -8,6 +9,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "usct.hip")
 OUT = os.path.join(HERE, "libusct.so")
+H5_SRC = os.path.join(HERE, "csrc", "usct_h5.cpp")
+H5_OUT = os.path.join(HERE, "libusct_h5.so")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
@@ -28,7 +31,22 @@ def build(verbose=False):
     if verbose and r.stderr:
         print(r.stderr)
     os.replace(tmp, OUT)
+    build_h5(verbose)
     return OUT
+
+
+def build_h5(verbose=False):
+    """Compile the dataset-file reader/writer (host code: OpenMP + zlib)."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = H5_OUT + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall", "-o", tmp, H5_SRC, "-lz"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)
+    if verbose and r.stderr:
+        print(r.stderr)
+    os.replace(tmp, H5_OUT)
+    return H5_OUT
 
 
 if __name__ == "__main__":
