@@ -77,6 +77,69 @@ def serving(n_requests, nt_scale, rate=150.0, seed=5, with_3d=True):
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in s.items()}
 
 
+def dataset_io(traces, prob):
+    """Acquisition-file leg: the workload's own traces written as a dataset file
+    (float32, gzip chunks) and read back -- into host memory by Dataset.load and
+    into HBM by read_fmc -- by the native library, beside h5py (the reference's
+    loader, libhdf5) reading the same file where an h5py-enabled Python exists."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from openusct_amd.dataset import ArrayGeometry, Dataset, read_fmc
+
+    threads = len(os.sched_getaffinity(0))
+    geom = ArrayGeometry(np.asarray(prob.geom.element_positions), prob.geom.radius_m, 5e5,
+                         getattr(prob.geom, "array_type", "ring"))
+    ds = Dataset(geom, traces, 1.0 / prob.dt, prob.wavelet, 5e5, 1500.0,
+                 ground_truth={"c": 1.0 / np.sqrt(prob.m_true), "h_m": prob.h})
+    raw = traces.size * 4
+    d = tempfile.mkdtemp(prefix="usct_io_")
+    try:
+        path = os.path.join(d, "acq.h5")
+        t0 = time.perf_counter()
+        ds.save(path, threads=threads)
+        t_save = time.perf_counter() - t0
+        Dataset.load(path, threads=threads)  # page cache warm, as for every reader below
+        t0 = time.perf_counter()
+        back = Dataset.load(path, threads=threads)
+        t_load = time.perf_counter() - t0
+        exact = bool(np.array_equal(back.data, traces.astype(np.float32)))
+        del back
+        read_fmc(path, "cuda", threads=threads)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev = read_fmc(path, "cuda", threads=threads)
+        torch.cuda.synchronize()
+        t_dev = time.perf_counter() - t0
+        exact = exact and bool(torch.equal(dev.cpu(), torch.from_numpy(traces.astype(np.float32)).double()))
+        del dev
+        h5py_gbs = None
+        py39 = "/opt/conda/bin/python3.9"
+        if os.path.exists(py39):
+            code = ("import sys, time, h5py\n"
+                    "f = h5py.File(sys.argv[1], 'r'); f['acquisition']['data'][()]\n"
+                    "t0 = time.perf_counter(); x = f['acquisition']['data'][()]\n"
+                    "print(time.perf_counter() - t0)\n")
+            r = subprocess.run([py39, "-c", code, path], capture_output=True, text=True, timeout=300)
+            if r.returncode == 0:
+                h5py_gbs = raw / float(r.stdout.split()[-1]) / 1e9
+        out = {"file_bytes": os.path.getsize(path), "raw_bytes": raw, "threads": threads,
+               "save_s": round(t_save, 3), "save_GBps": round(raw / t_save / 1e9, 3),
+               "load_GBps": round(raw / t_load / 1e9, 3), "to_device_GBps": round(raw / t_dev / 1e9, 3),
+               "h5py_load_GBps": None if h5py_gbs is None else round(h5py_gbs, 3),
+               "load_vs_h5py": None if h5py_gbs is None else round(raw / t_load / 1e9 / h5py_gbs, 2),
+               "bit_exact": exact,
+               "what": "float32 traces of this workload; load = Dataset.load into host memory, "
+                       "to_device = read_fmc into HBM as float64 (pinned double buffer)"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def workload(name="C2"):
     from openusct_amd import configs
     return configs.BUILDERS[name]()
@@ -158,6 +221,8 @@ def main():
                          "after the timed gradients (default: 5 for C3, else 0)")
     ap.add_argument("--serve-nt-scale", type=float, default=0.05,
                     help="time-axis scale of the serving requests")
+    ap.add_argument("--no-io", action="store_true",
+                    help="skip the acquisition-file leg (C2 only: write and read its traces)")
     ap.add_argument("--gn-products", type=int, default=-1,
                     help="Gauss-Newton Hessian-vector products timed on the same plan "
                          "(default: 3 for C2, else 0)")
@@ -288,6 +353,9 @@ def main():
     if len(shape) == 3 and info.get("fwd_tile_kind") == 2:
         fwd_kernel = "k_fwd3d_x2_lds"
     Jv = float(J.item())
+    traces = None
+    if args.workload == "C2" and not args.no_io:
+        traces = dobs.cpu().numpy()
     plan.close()
     del dobs
     torch.cuda.empty_cache()
@@ -307,6 +375,19 @@ def main():
         print(f"[bench] plain {tp * 1e3:.1f} ms/gradient over {ps} shots, {plain:.1f} "
               f"Gpt-updates/s", file=sys.stderr, flush=True)
         pplan.close()
+
+    io = None
+    if traces is not None:
+        try:
+            io = dataset_io(traces, prob)
+        except Exception as e:  # the main line must still print
+            io = {"error": repr(e)}
+            print(f"[bench] dataset file leg failed: {e!r}", file=sys.stderr, flush=True)
+        if "error" not in io:
+            print(f"[bench] dataset file {io['file_bytes'] / 1e6:.1f} MB: save {io['save_GBps']} GB/s, "
+                  f"load {io['load_GBps']} GB/s, to HBM {io['to_device_GBps']} GB/s, h5py "
+                  f"{io['h5py_load_GBps']} GB/s, exact {io['bit_exact']}", file=sys.stderr, flush=True)
+        del traces
 
     sv = None
     if args.serve_requests:
@@ -377,6 +458,7 @@ def main():
         "serving": sv,
         "gradient_descent": gd,
         "gauss_newton": gn,
+        "dataset_io": io,
     }
     print(json.dumps(line), flush=True)
 
