@@ -159,7 +159,9 @@ int usct_misfit_and_gradient_tw(usct_plan *plan, const double *m, const double *
  * divided by m.  v, hv: device N doubles.  The background wavefield is
  * recomputed each call (checkpointed, as for the gradient); the plan needs
  * gradient workspace (no USCT_FLAG_FORWARD_ONLY) and allocates the Born state
- * (three levels of the plan's field size, 6 bytes per element) on first use. */
+ * (four levels of the plan's field size, 6 bytes per element) on first use.
+ * Plans that advance two levels per launch (2D, no sponge, no footprints)
+ * advance the background and the Born field together. */
 int usct_gn_hessian_vector(usct_plan *plan, const double *m, const double *wavelet,
                            const double *sponge, const double *v, double *hv, void *stream);
 

@@ -60,11 +60,15 @@ class USCTError(RuntimeError):
 _lib = None
 
 
-def load(path=LIB_PATH):
-    """Load libusct.so and declare its signatures (raises if it is missing)."""
+def load(path=None):
+    """Load libusct.so (``LIB_PATH`` unless ``path`` is given) and declare its
+    signatures (raises if it is missing).  torch is imported first: the library
+    must bind to the HIP runtime torch ships, not load a second one."""
     global _lib
     if _lib is not None:
         return _lib
+    import torch  # noqa: F401
+    path = path or LIB_PATH
     if not os.path.exists(path):
         raise ImportError(
             f"{path} is missing: the HIP extension has not been built "

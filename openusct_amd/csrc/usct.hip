@@ -141,8 +141,8 @@ struct usct_plan {
     double *gstats = nullptr;
     // Gauss-Newton: Born state over three levels (FP32 hi + bfloat16 lo) and
     // v/m, allocated by the first Hessian-vector product
-    float *Dh[3] = {nullptr, nullptr, nullptr};
-    uint16_t *Dl[3] = {nullptr, nullptr, nullptr};
+    float *Dh[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint16_t *Dl[4] = {nullptr, nullptr, nullptr, nullptr};
     float2 *vm2 = nullptr;
     // Finite-aperture elements (groups of cells with weights).  Per-pass lists
     // are padded to fp_SE transmit entries and fp_FC fix cells and selected by
@@ -220,7 +220,7 @@ void free_plan_memory(usct_plan *P) {
     P->Wslot.clear();
     P->CK.clear();
     P->CKlo.clear();
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < 4; ++i) {
         P->Dh[i] = nullptr;
         P->Dl[i] = nullptr;
     }
@@ -650,21 +650,25 @@ void launch_residual(usct_plan *P, hipStream_t s, int misfit, bool want_res, boo
     }
 }
 
+// Born level l lives in Dh/Dl[born_slot(l)]: four buffers, so a two-level
+// launch never writes level n+1 over level n-2.
+int born_slot(int l) { return ((l % 4) + 4) % 4; }
+
 // Born state of levels -1 and 0 (zero), before the first Born step of a pass.
 void born_clear(usct_plan *P, hipStream_t s) {
     const int64_t FE = P->field_elems();
-    for (int i : {0, 2}) {
+    for (int i : {born_slot(0), born_slot(-1)}) {
         hipLaunchKernelGGL(k_fill, dim3(grid_for(FE)), dim3(256), 0, s, P->Dh[i], FE, 0.0f);
         hipLaunchKernelGGL(k_fill, dim3(grid_for(cdiv(FE, 2))), dim3(256), 0, s,
                            reinterpret_cast<float *>(P->Dl[i]), cdiv(FE, 2), 0.0f);
     }
 }
 
-// Born level n (state in Dh/Dl[n % 3]) from the background levels n, n-1, n-2;
-// its receiver samples become row n of the adjoint source res.
+// Born level n from the background levels n, n-1, n-2; its receiver samples
+// become row n of the adjoint source res.
 void launch_born(usct_plan *P, hipStream_t s, int n, Lvl u0, Lvl u1, Lvl u2) {
     const int64_t FE = P->field_elems();
-    const int a = n % 3, b = (n + 2) % 3, c = (n + 1) % 3;  // levels n, n-1, n-2
+    const int a = born_slot(n), b = born_slot(n - 1), c = born_slot(n - 2);
     float *res = P->res + (int64_t)n * P->n_rec * P->Bp;
     const dim3 g((unsigned)cdiv(FE, 256));
     if (P->sponge)
@@ -692,11 +696,49 @@ void fp_forward_fix(usct_plan *P, hipStream_t s, Lvl lv, int n, bool record) {
                        P->recb_lo + (int64_t)n * ntr, P->n_rec, P->Bp);
 }
 
+// Background levels n, n+1 and Born levels n, n+1 in one launch (2D, no
+// sponge, no footprints): k_born2d_x2 on the forward's lane groups, USCT_GNCX
+// cells per lane and USCT_GNRY rows per wavefront (on the 512^2, 128-shot
+// product: 4 cells and 32 rows at two wavefronts per SIMD 308 ms; 2 cells
+// 319-325 ms; 8 or 16 rows 313-338 ms; one wavefront per SIMD 381 ms).
+#ifndef USCT_GNCX
+#define USCT_GNCX 4
+#endif
+#ifndef USCT_GNRY
+#define USCT_GNRY 32
+#endif
+template <int LB>
+void born2_lb(hipStream_t s, const usct_plan *P, int n, Lvl u2, Lvl u1, Lvl a, Lvl b) {
+    constexpr int CX = USCT_GNCX;
+    Tiling tg = P->tf;
+    tg.CX = CX;
+    tg.RY = USCT_GNRY;
+    const Tile t = tile_of(P, tg);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+    const double *sa = P->sincd + (int64_t)n * P->Bp;
+    const int64_t tr = (int64_t)P->n_rec * P->Bp;
+    float *ra = P->res + (int64_t)n * tr;
+    const int dc = born_slot(n - 1), dp = born_slot(n - 2), da = born_slot(n), db = born_slot(n + 1);
+    hipLaunchKernelGGL((k_born2d_x2<LB, CX>), g, dim3(kBlock), 0, s, u1.hi, u2.hi, a.hi, b.hi,
+                       u1.lo, u2.lo, a.lo, b.lo, P->Dh[dc], P->Dh[dp], P->Dh[da], P->Dh[db],
+                       P->Dl[dc], P->Dl[dp], P->Dl[da], P->Dl[db], P->sp2, P->vm2, P->src_cell,
+                       sa, sa + P->Bp, P->rec_head, P->rec_next, P->row_rec, ra, ra + tr, P->ny,
+                       P->nx, P->Bp, tg.RY, t.nxs, t.nyc, P->Bp / tg.LB);
+}
+
+void launch_born2_batched(const usct_plan *P, hipStream_t s, int n, Lvl u2, Lvl u1, Lvl a, Lvl b) {
+    switch (P->tf.LB) {
+        case 16: born2_lb<16>(s, P, n, u2, u1, a, b); break;
+        case 32: born2_lb<32>(s, P, n, u2, u1, a, b); break;
+        default: born2_lb<64>(s, P, n, u2, u1, a, b); break;
+    }
+}
+
 // Footprint plans: receiver groups of Born level n into row n of the adjoint source.
 void fp_born_record(usct_plan *P, hipStream_t s, int n) {
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
-    hipLaunchKernelGGL(k_fp_record, dim3(grid_for(ntr)), dim3(256), 0, s, P->Dh[n % 3],
-                       P->Dl[n % 3], P->fp_roff, P->fp_rcell, P->fp_rw,
+    hipLaunchKernelGGL(k_fp_record, dim3(grid_for(ntr)), dim3(256), 0, s, P->Dh[born_slot(n)],
+                       P->Dl[born_slot(n)], P->fp_roff, P->fp_rcell, P->fp_rw,
                        P->res + (int64_t)n * ntr, nullptr, P->n_rec, P->Bp);
 }
 
@@ -736,6 +778,18 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
     if (born) born_clear(P, s);
     for (int n = 1; n < nt;) {
         const bool two = x2 && n + 1 < nt;
+        if (born && two && P->ndim == 2) {
+            // background and Born levels n, n+1 together (fusable: no sponge, no footprints)
+            if (pr) ev_start(pr, pr->fwd_ev, s);
+            launch_born2_batched(P, s, n, fwd_level(P, n - 2, true), fwd_level(P, n - 1, true),
+                                 fwd_level(P, n, true), fwd_level(P, n + 1, true));
+            if (pr) {
+                ev_stop(pr, pr->fwd_ev, s);
+                pr->fwd_lv.push_back(2);
+            }
+            n += 2;
+            continue;
+        }
         const int adv = fwd_step(P, s, fwd_level(P, n - 2, store), fwd_level(P, n - 1, store),
                                  fwd_level(P, n, store), two ? fwd_level(P, n + 1, store) : Lvl{},
                                  n, two, true, pr);
@@ -1365,14 +1419,16 @@ int leave(usct_plan *P, void *stream) {
 }
 
 // Born state and v/m buffers, allocated once per plan (Gauss-Newton only).
+// The Born fields and v/m carry the guard zones of the forward fields: the
+// two-level Born kernel reads halo columns past the first and last cell.
 int ensure_gn(usct_plan *P) {
     if (P->vm2) return 0;
     const int64_t FE = P->field_elems();
-    for (int i = 0; i < 3; ++i) {
-        TRY(dalloc_t(P, &P->Dh[i], FE));
-        TRY(dalloc_t(P, &P->Dl[i], 2 * cdiv(FE, 2)));
+    for (int i = 0; i < 4; ++i) {
+        TRY(dalloc_guarded(P, &P->Dh[i], FE, P->Bp));
+        TRY(dalloc_guarded(P, &P->Dl[i], 2 * cdiv(FE, 2), P->Bp));
     }
-    return dalloc_t(P, &P->vm2, P->N);
+    return dalloc_guarded(P, &P->vm2, P->N, 1);
 }
 
 // Time-window traces of a pass (batched layout), allocated on first use.

@@ -527,6 +527,220 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
 }
 
 // ---------------------------------------------------------------------------
+// Gauss-Newton, 2D, no sponge: background levels n and n+1 (exactly as
+// k_fwd2d_x2, without trace recording) and the Born levels n and n+1 that use
+// them (exactly as k_born), in one launch.  The Born source -(v/m)(U^l -
+// 2U^{l-1} + U^{l-2}) takes the three background levels from the registers
+// that hold them as stored (hi + lo), so per shot-cell the launch reads two
+// background and two Born levels and writes two of each (48 B per two levels
+// against 24 + 2 x 36 for k_fwd2d_x2 and two k_born launches).  Born traces
+// of level n go to res_a, of level n+1 to res_b.  The Born state rotates over
+// four buffers so that db (level n+1) is never dp (level n-2), which
+// neighbouring wavefronts still read.  With four cells per lane the two
+// fields want 256 VGPRs: two wavefronts per SIMD spill 14 of them and still
+// run faster than one wavefront per SIMD without spills.
+// ---------------------------------------------------------------------------
+#ifndef USCT_OCCGN
+#define USCT_OCCGN 2
+#endif
+template <int LB, int CX>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(USCT_OCCGN)))
+k_born2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
+            float *__restrict__ pb, const uint16_t *__restrict__ pcl,
+            const uint16_t *__restrict__ ppl, uint16_t *__restrict__ pal,
+            uint16_t *__restrict__ pbl, const float *__restrict__ dc, const float *__restrict__ dp,
+            float *__restrict__ da, float *__restrict__ db, const uint16_t *__restrict__ dcl,
+            const uint16_t *__restrict__ dpl, uint16_t *__restrict__ dal,
+            uint16_t *__restrict__ dbl, const float2 *__restrict__ sp2,
+            const float2 *__restrict__ vm2, const int *__restrict__ src_cell,
+            const double *__restrict__ sincd_a, const double *__restrict__ sincd_b,
+            const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+            const int *__restrict__ row_rec, float *__restrict__ res_a,
+            float *__restrict__ res_b, int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+    constexpr int W = kWave / LB;
+    constexpr int CA = CX + 2;  // columns x0-1 .. x0+CX
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    const int t = wv / ng;
+    const int xs = t % nxs, yc = t / nxs;
+    if (yc >= nyc) return;
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    if (x0 >= nx) return;
+    const int y0 = yc * RY;
+    const int y1 = min(y0 + RY, ny);
+    const int my_src = src_cell[b];
+    const double inc_a = sincd_a[b], inc_b = sincd_b[b];
+    const size_t rowE = (size_t)nx * Bp;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    // Background as in k_fwd2d_x2: um / uc / un = level n-1 on rows yf-1, yf,
+    // yf+1; q = level n-2 on row yf; am / ac / an = level n on rows yf-2 .. yf.
+    // The Born field has the same roles: em / ec / en, r, bm / bc / bn.
+    double um[CA], uc[CA + 2], un[CA + 2], am[CA], ac[CA], an[CA];
+    double em[CA], ec[CA + 2], en[CA + 2], bm[CA], bc[CA], bn[CA];
+    float nh[CA + 2], qh[CA], mh[CA + 2], rh[CA];
+    uint16_t nl[CA + 2], ql[CA], ml[CA + 2], rl[CA];
+    {
+        const size_t r0 = clamp_row(y0 - 2, ny) * rowE, r1 = clamp_row(y0 - 1, ny) * rowE;
+        const size_t r2 = clamp_row(y0, ny) * rowE;
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            const ptrdiff_t c = (ptrdiff_t)(j - 1) * Bp;
+            um[j] = hl(ldb(pc + r0 + c, o4), ldb(pcl + r0 + c, o2));
+            em[j] = hl(ldb(dc + r0 + c, o4), ldb(dcl + r0 + c, o2));
+            qh[j] = ldn<USCT_NT2D>(pp + r1 + c, o4);
+            ql[j] = ldn<USCT_NT2D>(ppl + r1 + c, o2);
+            rh[j] = ldn<USCT_NT2D>(dp + r1 + c, o4);
+            rl[j] = ldn<USCT_NT2D>(dpl + r1 + c, o2);
+        }
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) {
+            const ptrdiff_t c = (ptrdiff_t)(j - 2) * Bp;
+            uc[j] = hl(ldb(pc + r1 + c, o4), ldb(pcl + r1 + c, o2));
+            ec[j] = hl(ldb(dc + r1 + c, o4), ldb(dcl + r1 + c, o2));
+            nh[j] = ldb(pc + r2 + c, o4);
+            nl[j] = ldb(pcl + r2 + c, o2);
+            mh[j] = ldb(dc + r2 + c, o4);
+            ml[j] = ldb(dcl + r2 + c, o2);
+        }
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            am[j] = 0.0; ac[j] = 0.0; an[j] = 0.0;
+            bm[j] = 0.0; bc[j] = 0.0; bn[j] = 0.0;
+        }
+    }
+    for (int yf = y0 - 1; yf <= y1; ++yf) {
+        const int yr = (int)clamp_row(yf, ny);
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) {
+            un[j] = hl(nh[j], nl[j]);
+            en[j] = hl(mh[j], ml[j]);
+        }
+        double q[CA], r[CA];
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            q[j] = hl(qh[j], ql[j]);
+            r[j] = hl(rh[j], rl[j]);
+        }
+        {
+            const size_t rnn = clamp_row(yf + 2, ny) * rowE;
+            const size_t enx = clamp_row(yf + 1, ny) * rowE;
+#pragma unroll
+            for (int j = 0; j < CA + 2; ++j) {
+                const ptrdiff_t c = (ptrdiff_t)(j - 2) * Bp;
+                nh[j] = ldb(pc + rnn + c, o4);
+                nl[j] = ldb(pcl + rnn + c, o2);
+                mh[j] = ldb(dc + rnn + c, o4);
+                ml[j] = ldb(dcl + rnn + c, o2);
+            }
+#pragma unroll
+            for (int j = 0; j < CA; ++j) {
+                const ptrdiff_t c = (ptrdiff_t)(j - 1) * Bp;
+                qh[j] = ldn<USCT_NT2D>(pp + enx + c, o4);
+                ql[j] = ldn<USCT_NT2D>(ppl + enx + c, o2);
+                rh[j] = ldn<USCT_NT2D>(dp + enx + c, o4);
+                rl[j] = ldn<USCT_NT2D>(dpl + enx + c, o2);
+            }
+        }
+        // ---- levels n on row yf, columns x0-1 .. x0+CX
+        const bool own_a = yf >= y0 && yf < y1;
+        const bool yint = yf >= 1 && yf <= ny - 2;
+        const size_t ro = (size_t)yr * rowE;
+        const float2 *spr = sp2 + (size_t)yr * nx;
+        const float2 *vmr = vm2 + (size_t)yr * nx;
+        const bool rec_a_row = own_a && row_rec[yr] > 0;
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            const int x = x0 - 1 + j;
+            const bool in = yint && x >= 1 && x <= nx - 2;
+            const double sv = s2d(ldb(spr + (j - 1), oc));
+            double lu = 0.0;
+            if (in) lu = lap5d(uc[j + 1], un[j + 1], um[j], uc[j + 2], uc[j]);
+            double inc = sv * lu;
+            if (yf * nx + x == my_src) inc = inc + inc_a;
+            const double nv = (2.0 * uc[j + 1] - q[j]) + inc;
+            uint16_t lb;
+            const float v = split_hl(nv, lb);
+            am[j] = ac[j];
+            ac[j] = an[j];
+            an[j] = hl(v, lb);
+            double ld = 0.0;
+            if (in) ld = lap5d(ec[j + 1], en[j + 1], em[j], ec[j + 2], ec[j]);
+            const double d2u = (an[j] - 2.0 * uc[j + 1]) + q[j];
+            const double dv = (2.0 * ec[j + 1] - r[j]) + sv * ld - s2d(ldb(vmr + (j - 1), oc)) * d2u;
+            uint16_t dlb;
+            const float w = split_hl(dv, dlb);
+            bm[j] = bc[j];
+            bc[j] = bn[j];
+            bn[j] = hl(w, dlb);
+            if (j >= 1 && j <= CX && own_a && x < nx) {
+                const ptrdiff_t e = (ptrdiff_t)ro + (ptrdiff_t)(j - 1) * Bp;
+                stn<USCT_NT2D>(pa + e, o4, v);
+                stn<USCT_NT2D>(pal + e, o2, lb);
+                stn<USCT_NT2D>(da + e, o4, w);
+                stn<USCT_NT2D>(dal + e, o2, dlb);
+                if (rec_a_row)
+                    for (int rr = rec_head[yr * nx + x]; rr >= 0; rr = rec_next[rr])
+                        res_a[(size_t)rr * Bp + b] = w;
+            }
+        }
+        // ---- levels n+1 on row y = yf-1, own columns
+        const int y = yf - 1;
+        if (y >= y0 && y < y1) {
+            const bool yint_b = y >= 1 && y <= ny - 2;
+            const size_t rob = (size_t)y * rowE;
+            const float2 *spb = sp2 + (size_t)y * nx;
+            const float2 *vmb = vm2 + (size_t)y * nx;
+            const bool rec_b_row = row_rec[y] > 0;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                if (x < nx) {
+                    const bool in = yint_b && x >= 1 && x <= nx - 2;
+                    const double sv = s2d(ldb(spb + i, oc));
+                    double lu = 0.0;
+                    if (in) lu = lap5d(ac[i + 1], an[i + 1], am[i + 1], ac[i + 2], ac[i]);
+                    double inc = sv * lu;
+                    if (y * nx + x == my_src) inc = inc + inc_b;
+                    const double nv = (2.0 * ac[i + 1] - um[i + 1]) + inc;
+                    uint16_t lb;
+                    const float v = split_hl(nv, lb);
+                    double ld = 0.0;
+                    if (in) ld = lap5d(bc[i + 1], bn[i + 1], bm[i + 1], bc[i + 2], bc[i]);
+                    const double d2u = (hl(v, lb) - 2.0 * ac[i + 1]) + um[i + 1];
+                    const double dv = (2.0 * bc[i + 1] - em[i + 1]) + sv * ld - s2d(ldb(vmb + i, oc)) * d2u;
+                    uint16_t dlb;
+                    const float w = split_hl(dv, dlb);
+                    const size_t e = rob + (size_t)i * Bp;
+                    stn<USCT_NT2D>(pb + e, o4, v);
+                    stn<USCT_NT2D>(pbl + e, o2, lb);
+                    stn<USCT_NT2D>(db + e, o4, w);
+                    stn<USCT_NT2D>(dbl + e, o2, dlb);
+                    if (rec_b_row)
+                        for (int rr = rec_head[y * nx + x]; rr >= 0; rr = rec_next[rr])
+                            res_b[(size_t)rr * Bp + b] = w;
+                }
+            }
+        }
+        // ---- advance
+#pragma unroll
+        for (int j = 0; j < CA; ++j) {
+            um[j] = uc[j + 1];
+            em[j] = ec[j + 1];
+        }
+#pragma unroll
+        for (int j = 0; j < CA + 2; ++j) {
+            uc[j] = un[j];
+            ec[j] = en[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batched 2D adjoint step with the imaging condition.
 //   l1 = lam^{k+1}, l2 = lam^{k+2}, lk = lam^k (output)
 //   no sponge: pa = p^{k-1};  sponge: pa = p^k, pb = p^{k-1}, pc2 = p^{k-2}

@@ -755,6 +755,71 @@ bool born_cases(uint32_t seed) {
     return ok;
 }
 
+// Gauss-Newton two-level launch: background and Born levels n, n+1 of one
+// k_born2d_x2 launch against k_fwd2d, k_born, k_fwd2d, k_born (fields, low
+// parts and the Born traces of both levels), fields with the library's guard
+// zones.
+template <int LB, int CX>
+bool check_born_x2(int ny, int nx, int ng, int RY, uint32_t seed) {
+    Problem P(ny, nx, LB, ng, CX, RY, seed);
+    char tag[160];
+    std::snprintf(tag, sizeof tag, "born-x2 ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d", ny, nx, LB, ng, CX,
+                  RY);
+    float *pc = P.field(1.0f), *pp = P.field(1.0f), *dc = P.field(0.3f), *dp = P.field(0.3f);
+    uint16_t *pcl = P.low(1e-7f), *ppl = P.low(1e-7f), *dcl = P.low(1e-8f), *dpl = P.low(1e-8f);
+    float2 *vm2 = alloc<float2>(P.N, 1);
+    std::uniform_real_distribution<float> v(-2.0f, 2.0f), t(-1e-9f, 1e-9f);
+    for (int64_t c = 0; c < P.N; ++c) vm2[c] = make_float2(v(P.rng), t(P.rng));
+    float *a1 = P.zeros(), *b1 = P.zeros(), *a2 = P.zeros(), *b2 = P.zeros();
+    float *ea1 = P.zeros(), *eb1 = P.zeros(), *ea2 = P.zeros(), *eb2 = P.zeros();
+    uint16_t *a1l = P.zeros_lo(), *b1l = P.zeros_lo(), *a2l = P.zeros_lo(), *b2l = P.zeros_lo();
+    uint16_t *ea1l = P.zeros_lo(), *eb1l = P.zeros_lo(), *ea2l = P.zeros_lo(), *eb2l = P.zeros_lo();
+    float *ra1 = P.traces(), *rb1 = P.traces(), *ra2 = P.traces(), *rb2 = P.traces();
+    const double *sa = P.sincd + P.Bp, *sb = P.sincd + 2 * P.Bp;
+    const unsigned gb = (unsigned)cdiv(P.FE, 256);
+    emu::launch(P.blocks, kBlock, [&] {
+        k_fwd2d<LB, CX, false, false>(pc, pp, a1, pcl, ppl, a1l, P.sp2, nullptr, P.src_cell, sa,
+                                      P.rec_head, P.rec_next, P.row_rec, nullptr, nullptr, P.ny,
+                                      P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+    });
+    emu::launch(gb, 256, [&] {
+        k_born<false>(a1, a1l, pc, pcl, pp, ppl, dc, dcl, dp, dpl, ea1, ea1l, P.sp2, vm2, nullptr,
+                      P.rec_head, P.rec_next, ra1, 1, P.ny, P.nx, 2, P.Bp);
+    });
+    emu::launch(P.blocks, kBlock, [&] {
+        k_fwd2d<LB, CX, false, false>(a1, pc, b1, a1l, pcl, b1l, P.sp2, nullptr, P.src_cell, sb,
+                                      P.rec_head, P.rec_next, P.row_rec, nullptr, nullptr, P.ny,
+                                      P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+    });
+    emu::launch(gb, 256, [&] {
+        k_born<false>(b1, b1l, a1, a1l, pc, pcl, ea1, ea1l, dc, dcl, eb1, eb1l, P.sp2, vm2,
+                      nullptr, P.rec_head, P.rec_next, rb1, 1, P.ny, P.nx, 2, P.Bp);
+    });
+    emu::launch(P.blocks, kBlock, [&] {
+        k_born2d_x2<LB, CX>(pc, pp, a2, b2, pcl, ppl, a2l, b2l, dc, dp, ea2, eb2, dcl, dpl, ea2l,
+                            eb2l, P.sp2, vm2, P.src_cell, sa, sb, P.rec_head, P.rec_next,
+                            P.row_rec, ra2, rb2, P.ny, P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+    });
+    const int64_t nr = (int64_t)P.n_rec * P.Bp;
+    bool ok = same("background n", a1, a2, P.FE, tag) & same("background n low", a1l, a2l, P.FE, tag) &
+              same("background n+1", b1, b2, P.FE, tag) & same("background n+1 low", b1l, b2l, P.FE, tag) &
+              same("Born n", ea1, ea2, P.FE, tag) & same("Born n low", ea1l, ea2l, P.FE, tag) &
+              same("Born n+1", eb1, eb2, P.FE, tag) & same("Born n+1 low", eb1l, eb2l, P.FE, tag) &
+              same("Born traces n", ra1, ra2, nr, tag) & same("Born traces n+1", rb1, rb2, nr, tag);
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+template <int LB, int CX>
+bool born_x2_cases(uint32_t seed) {
+    bool ok = true;
+    const int shapes[][2] = {{29, 37}, {9, 13}, {40, 33}, {3, 5}};
+    for (auto &s : shapes)
+        for (int RY : {4, 8, 64}) ok &= check_born_x2<LB, CX>(s[0], s[1], 2, RY, seed++);
+    ok &= check_born_x2<LB, CX>(29, 37, 1, 4, seed++);
+    return ok;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -783,6 +848,12 @@ int main(int argc, char **argv) {
     }
     if (only.empty() || only == "born") {
         ok &= born_cases(seed + 3000);
+    }
+    if (only.empty() || only == "born-x2") {
+        ok &= born_x2_cases<16, 4>(seed + 5000);
+        ok &= born_x2_cases<32, 4>(seed + 5100);
+        ok &= born_x2_cases<64, 4>(seed + 5200);
+        ok &= born_x2_cases<16, 8>(seed + 5300);
     }
     if (only.empty() || only == "fwd3d") {
         ok &= fwd3d_cases<64, 4, 2>(seed + 1000);
