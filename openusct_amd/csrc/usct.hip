@@ -253,7 +253,7 @@ Tile tile_of(const usct_plan *P, const Tiling &T) {
     } else {
         t.nys = (int)cdiv(P->ny, T.CY);
         t.nyc = (int)cdiv(P->nz, T.RY);
-        t.waves = ng * t.nxs * t.nys * t.nyc;
+        t.waves = ng * t.nxs * round_ys(t.nys) * t.nyc;
     }
     return t;
 }

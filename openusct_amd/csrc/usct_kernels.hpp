@@ -76,6 +76,28 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     return xcd * per + (xcd < rem ? xcd : rem) + slot;
 }
 
+// Strip order of the 3D patch kernels (k_fwd3d, k_adj3d): the wavefronts of a
+// work-group take USCT_GY3D consecutive y strips of one x strip, so the y-halo
+// rows they share are fetched once by one CU instead of by blocks far apart.
+// The grid covers the y strips rounded up to a multiple of USCT_GY3D.
+#ifndef USCT_GY3D
+#define USCT_GY3D 2
+#endif
+__host__ __device__ inline int round_ys(int nys) {
+    return (nys + USCT_GY3D - 1) / USCT_GY3D * USCT_GY3D;
+}
+__device__ __forceinline__ bool strip3d(int t, int nxs, int nys, int nzc, int &xs, int &ys,
+                                        int &zc) {
+    const int q = t % USCT_GY3D;
+    t /= USCT_GY3D;
+    xs = t % nxs;
+    t /= nxs;
+    const int nyg = (nys + USCT_GY3D - 1) / USCT_GY3D;
+    ys = (t % nyg) * USCT_GY3D + q;
+    zc = t / nyg;
+    return ys < nys && zc < nzc;
+}
+
 // Global wavefront index (XCD-remapped block, then wave within the block),
 // marked wave-uniform so everything derived from it lives in scalar registers.
 __device__ __forceinline__ int wave_id() {
@@ -1132,11 +1154,8 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
-    int t = wv / ng;
-    const int xs = t % nxs;
-    t /= nxs;
-    const int ys = t % nys, zc = t / nys;
-    if (zc >= nzc) return;
+    int xs, ys, zc;
+    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -2052,11 +2071,8 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
-    int t = wv / ng;
-    const int xs = t % nxs;
-    t /= nxs;
-    const int ys = t % nys, zc = t / nys;
-    if (zc >= nzc) return;  // uniform over the wavefront
+    int xs, ys, zc;
+    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;

@@ -335,7 +335,7 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     if constexpr (KIND == 0) {
         const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d<LB, CX, CY, true>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head,
                                       rec_next, row_rec, rec, rec_lo, srcd2, nz, ny, nx, Bp, RZ,
@@ -429,7 +429,7 @@ bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     if constexpr (KIND == 0) {
         const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
         emu::launch(blocks, kBlock, [&] {
             k_adj3d<LB, CX, CY>(l1, l2, lk, pa, sp2, src, sinc, rec_head, rec_next, row_rec, res,
                                 srcd2, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
@@ -566,7 +566,7 @@ bool check_fwd3d_x2(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
         constexpr int CXr = 2, CYr = 2;
         const int nxs = (int)cdiv(nx, (int64_t)W * CXr), nys = (int)cdiv(ny, CYr);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * nys * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d<LB, CXr, CYr, true>(A, B, C1, Al, Bl, C1l, sp2, src, sa, rec_head, rec_next,
                                         row_rec, ra1, ral1, da1, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
