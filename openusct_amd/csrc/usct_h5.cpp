@@ -769,8 +769,31 @@ struct usct_h5_file {
         }
     }
 
+    // The chunk list of a dataset, built once; on failure nothing is kept, so a
+    // later call starts afresh.  Every chunk offset must be distinct: a damaged
+    // index could otherwise leave parts of the output unwritten.
     void load_chunks(Desc &d) const {
         if (d.chunks_loaded) return;
+        std::vector<Chunk> list;
+        std::swap(list, d.chunks);
+        try {
+            load_chunk_list(d);
+        } catch (...) {
+            d.chunks.clear();
+            throw;
+        }
+        std::vector<const std::vector<uint64_t> *> offs;
+        for (const Chunk &c : d.chunks) offs.push_back(&c.off);
+        std::sort(offs.begin(), offs.end(), [](auto *a, auto *b) { return *a < *b; });
+        for (size_t i = 1; i < offs.size(); ++i)
+            if (*offs[i] == *offs[i - 1]) {
+                d.chunks.clear();
+                raise(USCT_H5_EFORMAT, "chunk index lists a chunk twice");
+            }
+        d.chunks_loaded = true;
+    }
+
+    void load_chunk_list(Desc &d) const {
         if (d.addr != UNDEF) {
             if (d.index == 0) {
                 uint64_t visits = 0;
@@ -800,7 +823,6 @@ struct usct_h5_file {
                 }
             }
         }
-        d.chunks_loaded = true;
     }
 
     static uint64_t chunk_bytes(const Desc &d) {
