@@ -308,6 +308,10 @@ void fwd3d_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
     if (P->tf.CX == 4 && P->tf.CY == 4) fwd3d_t<LB, 4, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     else if (P->tf.CX == 8 && P->tf.CY == 2) fwd3d_t<LB, 8, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     else if (P->tf.CX == 2 && P->tf.CY == 2) fwd3d_t<LB, 2, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else if (P->tf.CX == 2 && P->tf.CY == 4) fwd3d_t<LB, 2, 4, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else if (P->tf.CX == 2 && P->tf.CY == 1) fwd3d_t<LB, 2, 1, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else if (P->tf.CX == 1 && P->tf.CY == 2) fwd3d_t<LB, 1, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
+    else if (P->tf.CX == 1 && P->tf.CY == 1) fwd3d_t<LB, 1, 1, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
     else fwd3d_t<LB, 4, 2, REC>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
 }
 
@@ -520,6 +524,10 @@ void adj3d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk
     if (P->ta.CX == 4 && P->ta.CY == 4) adj3d_t<LB, 4, 4>(g, s, l1, l2, lk, pa, P, sinc, res, t);
     else if (P->ta.CX == 8 && P->ta.CY == 2) adj3d_t<LB, 8, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
     else if (P->ta.CX == 2 && P->ta.CY == 2) adj3d_t<LB, 2, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else if (P->ta.CX == 2 && P->ta.CY == 4) adj3d_t<LB, 2, 4>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else if (P->ta.CX == 2 && P->ta.CY == 1) adj3d_t<LB, 2, 1>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else if (P->ta.CX == 1 && P->ta.CY == 2) adj3d_t<LB, 1, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
+    else if (P->ta.CX == 1 && P->ta.CY == 1) adj3d_t<LB, 1, 1>(g, s, l1, l2, lk, pa, P, sinc, res, t);
     else adj3d_t<LB, 4, 2>(g, s, l1, l2, lk, pa, P, sinc, res, t);
 }
 
@@ -1316,7 +1324,8 @@ int autotune(usct_plan *P) {
                     for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry, 0});
         } else {
             for (int lb : lbs)
-                for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}, {2, 2}})
+                for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}, {2, 2}, {2, 4}, {2, 1},
+                                {1, 2}, {1, 1}})
                     for (int rz : {4, 8, 16, 32, 64}) {
                         // the adjoint reduces a lane's CX*CY cells over its lane group
                         if (adjoint && pc.first * pc.second > lb) continue;

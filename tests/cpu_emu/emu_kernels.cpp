@@ -861,6 +861,11 @@ int main(int argc, char **argv) {
         ok &= fwd3d_cases<32, 8, 2>(seed + 1200);
         ok &= fwd3d_cases<16, 4, 2>(seed + 1300);
         ok &= fwd3d_cases<16, 2, 2>(seed + 1400);
+        ok &= fwd3d_cases<16, 2, 4>(seed + 1500);
+        ok &= fwd3d_cases<32, 2, 4>(seed + 1600);
+        ok &= fwd3d_cases<16, 2, 1>(seed + 1700);
+        ok &= fwd3d_cases<16, 1, 2>(seed + 1800);
+        ok &= fwd3d_cases<16, 1, 1>(seed + 1900);
     }
     std::printf(ok ? "ALL EQUAL\n" : "DIFFERENCES FOUND\n");
     return ok ? 0 : 1;

@@ -58,5 +58,5 @@ def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     tail = "\n".join((r.stdout + r.stderr).splitlines()[-20:])
     assert r.returncode == 0, tail
     assert "ALL EQUAL" in r.stdout, tail
-    assert r.stdout.count("ok ") == {"fwd": 75, "adj": 65, "fwd3d": 25, "lds": 12,
+    assert r.stdout.count("ok ") == {"fwd": 75, "adj": 65, "fwd3d": 50, "lds": 12,
                                      "born": 5, "x2-3d": 6, "born-x2": 52}[which], tail
