@@ -152,6 +152,18 @@ int usct_misfit_and_gradient_tw(usct_plan *plan, const double *m, const double *
                                 const double *time_weights, int misfit_type, double *J,
                                 double *grad, void *stream);
 
+/* Exact discrete-adjoint gradient for a caller-supplied adjoint source: the
+ * adjoint half of fwi.py:266-332 with `res` given (fwi.py:295-301 computes it
+ * there from the chosen misfit).  adj = dJ/d(traces), device n_tx*nt*n_rec
+ * doubles in the layout of dobs (row 0 of each transmit is never injected, as
+ * the reference's loop stops at k = 1); it is injected as -R^T adj
+ * (fwi.py:316-322, footprints included).  grad: device N doubles, dJ/dm.
+ * With it any misfit of the forward traces (envelope, normalised envelope,
+ * traveltime, fwi.py:134-243) is differentiated through the device adjoint. */
+int usct_gradient_from_source(usct_plan *plan, const double *m, const double *wavelet,
+                              const double *sponge, const double *adj, double *grad,
+                              void *stream);
+
 /* Gauss-Newton Hessian-vector product hv = (J^T J) v summed over the plan's
  * transmits (simulation/ringfwi/fwi.py:410-431, gn_hessian_vector): the exact
  * discrete Born forward of v (fwi.py:350-378) on each transmit's background

@@ -69,7 +69,9 @@ int fail(int code, const char *fmt, ...) {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2, MODE_GNHV = 3 };
+// MODE_ADJSRC: the gradient for a caller-supplied adjoint source (staged into
+// res per pass in place of the residual).
+enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2, MODE_GNHV = 3, MODE_ADJSRC = 4 };
 
 struct Tiling {
     int LB, CX, CY, RY;  // lanes per shot group, cells per lane (x, y), rows per wavefront
@@ -775,7 +777,7 @@ void fp_adjoint_fix(usct_plan *P, hipStream_t s, float *lk, const float *pa, con
 void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool windowed,
                       Profile *pr) {
     const bool born = mode == MODE_GNHV;
-    const bool store = mode == MODE_GRAD || born;
+    const bool store = mode == MODE_GRAD || born || mode == MODE_ADJSRC;
     const int nt = P->nt;
     // rec row 0 = 0 (fwi.py:58: rec starts as zeros, level 0 is never written)
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
@@ -812,7 +814,7 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
         n += adv;
     }
     if (mode == MODE_FWD) return;
-    if (!born) launch_residual(P, s, misfit, store, windowed);
+    if (!born && mode != MODE_ADJSRC) launch_residual(P, s, misfit, store, windowed);
     if (!store) return;
     for (int w = P->NS - 1; w >= 0; --w) {
         const int base = w * P->K;
@@ -920,9 +922,10 @@ int run_batched(usct_plan *P, int mode, int misfit, const double *m, const doubl
                            m, w, P->src_cell, P->sinc, P->sincd, P->nt, P->Bp, P->dt * P->dt);
         const int64_t NJ = (int64_t)P->nt * P->n_rec;
         const dim3 gt((unsigned)cdiv(NJ, kTT), (unsigned)cdiv(P->Bp, kTT));
-        if (dobs)
-            hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT,
-                               P->dobsT_lo, NJ, P->Bp, b0, nb);
+        if (dobs)  // observed data, or (MODE_ADJSRC) the adjoint source straight into res
+            hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs,
+                               mode == MODE_ADJSRC ? P->res : P->dobsT, P->dobsT_lo, NJ, P->Bp,
+                               b0, nb);
         if (tw)
             hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, tw, P->twT, P->twT_lo, NJ,
                                P->Bp, b0, nb);
@@ -961,7 +964,7 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
         HIPX(hipMemsetAsync(P->g_lo, 0, sizeof(float) * N, s));
     }
     const bool born = mode == MODE_GNHV;
-    const bool store = mode == MODE_GRAD || born;
+    const bool store = mode == MODE_GRAD || born || mode == MODE_ADJSRC;
     const int nt = P->nt, nr = P->n_rec;
     const int nd = P->ndim;
     // History slots: level n in Wslot[n] (n >= 1), levels <= 0 are Z.
@@ -1013,7 +1016,10 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
             hipLaunchKernelGGL(k_batched_to_data, gt, dim3(256), 0, s, P->recb, P->recb_lo, data,
                                NJ, 1, b, 1);
         if (mode == MODE_FWD) continue;
-        if (!born) {
+        if (mode == MODE_ADJSRC) {
+            hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->res, P->dobsT_lo,
+                               NJ, 1, b, 1);
+        } else if (!born) {
             // residual of this shot against dobs[b] (FP64 -> FP32 hi + lo)
             hipLaunchKernelGGL(k_dobs_to_batched, gt, dim3(256), 0, s, dobs, P->dobsT,
                                P->dobsT_lo, NJ, 1, b, 1);
@@ -1460,7 +1466,9 @@ int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
                                         : "plan created with USCT_FLAG_SPONGE needs a sponge");
     if ((mode == MODE_MISFIT || mode == MODE_GRAD) && !dobs)
         return fail(USCT_EINVAL, "dobs is required");
-    if ((mode == MODE_GRAD || mode == MODE_GNHV) && (P->flags & USCT_FLAG_FORWARD_ONLY))
+    if (mode == MODE_ADJSRC && !dobs) return fail(USCT_EINVAL, "the adjoint source is required");
+    if ((mode == MODE_GRAD || mode == MODE_GNHV || mode == MODE_ADJSRC) &&
+        (P->flags & USCT_FLAG_FORWARD_ONLY))
         return fail(USCT_ENOTSUP, "plan was created with USCT_FLAG_FORWARD_ONLY");
     if (mode == MODE_GNHV && !v) return fail(USCT_EINVAL, "v is required");
     if (P->n_tx == 0) {
@@ -1806,6 +1814,14 @@ int usct_misfit_and_gradient(usct_plan *P, const double *m, const double *wavele
     if (!J || !grad) return fail(USCT_EINVAL, "J and grad are required");
     return run(P, MODE_GRAD, misfit_type, m, wavelet, sponge, dobs, nullptr, J, grad, stream,
                nullptr);
+}
+
+int usct_gradient_from_source(usct_plan *P, const double *m, const double *wavelet,
+                              const double *sponge, const double *adj, double *grad,
+                              void *stream) {
+    if (!grad) return fail(USCT_EINVAL, "grad is required");
+    return run(P, MODE_ADJSRC, USCT_MISFIT_L2, m, wavelet, sponge, adj, nullptr, nullptr, grad,
+               stream, nullptr);
 }
 
 int usct_misfit_tw(usct_plan *P, const double *m, const double *wavelet, const double *sponge,

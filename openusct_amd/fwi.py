@@ -26,6 +26,8 @@ from collections import OrderedDict
 import numpy as np
 import torch
 
+from . import misfits as trace_misfits
+from .misfits import DEVICE_MISFITS
 from .plan import Plan, linear_indices
 
 # Backend selection for the drop-in functions: "batched" (default) or "plain".
@@ -174,13 +176,26 @@ def _windows_dev(time_weights, plan):
     return _to_dev(np.stack(tw) if tw else np.zeros((0, plan.nt, plan.n_rec)), torch.float64)
 
 
+def _check_misfit(misfit_type):
+    if misfit_type == "gsot":
+        raise NotImplementedError("misfit_type 'gsot' (an optimal assignment per trace, fwi.py:193-217) "
+                                  "is not implemented on the device")
+    if misfit_type not in ("l2", "gcn") + DEVICE_MISFITS:
+        raise ValueError(f"unknown misfit_type {misfit_type!r}")
+
+
 def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, misfit_type="l2",
            footprints=None, time_weights=None):
-    """Misfit only (fwi.py:246-263), optionally with footprints and time windows."""
-    if misfit_type not in ("l2", "gcn"):
-        raise NotImplementedError(f"misfit_type {misfit_type!r} is not implemented on the device")
+    """Misfit only (fwi.py:246-263), optionally with footprints and time windows;
+    envelope / egcn / traveltime are evaluated on the device traces."""
+    _check_misfit(misfit_type)
     plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list,
                               footprints=footprints)
+    if misfit_type in DEVICE_MISFITS:
+        d = plan.forward_fmc(md, wd, sd)
+        J, _ = trace_misfits.misfit_and_source(d, _dobs_dev(dobs, plan), misfit_type,
+                                               _windows_dev(time_weights, plan))
+        return float(J.item())
     J = plan.misfit(md, wd, _dobs_dev(dobs, plan), sd, misfit_type,
                     time_weights=_windows_dev(time_weights, plan))
     return float(J.item())
@@ -189,11 +204,19 @@ def misfit(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, misfit
 def misfit_and_gradient(m, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None,
                         misfit_type="l2", footprints=None, time_weights=None):
     """Misfit and exact discrete-adjoint gradient w.r.t. m (fwi.py:266-332),
-    optionally with finite-aperture footprints and time windows."""
-    if misfit_type not in ("l2", "gcn"):
-        raise NotImplementedError(f"misfit_type {misfit_type!r} is not implemented on the device")
+    optionally with finite-aperture footprints and time windows.  L2 and GCN
+    run fused in the library; envelope / egcn / traveltime take the device
+    traces, form their adjoint source with torch on the device and run the
+    library's adjoint sweep on it (Plan.gradient_from_source)."""
+    _check_misfit(misfit_type)
     plan, md, wd, sd = _setup(m, geom, wavelet, dt, h, nt, sponge, src_list,
                               footprints=footprints)
+    if misfit_type in DEVICE_MISFITS:
+        d = plan.forward_fmc(md, wd, sd)
+        J, adj = trace_misfits.misfit_and_source(d, _dobs_dev(dobs, plan), misfit_type,
+                                                 _windows_dev(time_weights, plan))
+        g = plan.gradient_from_source(md, wd, adj, sd)
+        return float(J.item()), g.cpu().numpy()
     J, g = plan.misfit_and_gradient(md, wd, _dobs_dev(dobs, plan), sd, misfit_type,
                                     time_weights=_windows_dev(time_weights, plan))
     return float(J.item()), g.cpu().numpy()
@@ -340,7 +363,8 @@ def invert(m0, geom, wavelet, dt, h, nt, dobs, sponge=None, src_list=None, n_ite
         mf = lambda mm: misfit(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,  # noqa
                                misfit_type, footprints, time_weights)
     else:
-        be = backend if backend is not None else _THIS
+        # as fwi.py:594-615: a backend serves L2 and GCN; other misfits run here
+        be = backend if backend is not None and misfit_type in ("l2", "gcn") else _THIS
         mg = lambda mm: be.misfit_and_gradient(mm, geom, wavelet, dt, h, nt, dobs, sponge,  # noqa
                                                src_list, misfit_type=misfit_type)
         mf = lambda mm: be.misfit(mm, geom, wavelet, dt, h, nt, dobs, sponge, src_list,  # noqa

@@ -204,6 +204,16 @@ class Plan:
         _lib.check(rc)
         return J, g
 
+    def gradient_from_source(self, m, wavelet, adj, sponge=None, stream=None):
+        """dJ/dm for an adjoint source ``adj`` = dJ/d(traces) of shape
+        (n_tx, nt, n_rec): the device forward (history kept) and the exact
+        adjoint sweep injecting -adj (fwi.py:295-331 with a given residual)."""
+        m, w, sp, a = self._model(m), self._wavelet(wavelet), self._sponge(sponge), self._dobs(adj, "adj")
+        g = torch.empty(self.shape, dtype=torch.float64, device=self.device)
+        _lib.check(_lib.load().usct_gradient_from_source(self._h, _ptr(m), _ptr(w), _ptr(sp), _ptr(a),
+                                                         _ptr(g), _stream_handle(stream)))
+        return g
+
     def gn_hessian_vector(self, m, wavelet, v, sponge=None, out=None, stream=None):
         """Gauss-Newton Hessian-vector product (J^T J) v (fwi.py:410-431), float64
         with the model's shape; the background wavefield is recomputed per call."""

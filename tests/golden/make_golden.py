@@ -23,7 +23,9 @@ outputs of the functions whose behaviour only the reference itself pins:
   them in 2D and 3D (fwi.py:74-89, 316-319), P-window time weights
   (fwi.py:670-699) and windowed misfits (fwi.py:255-262, 298-301), and
   Gauss-Newton products with a sponge and with footprints (fwi.py:410-431),
-  whose sponge part is the reference's own formula.
+  whose sponge part is the reference's own formula;
+* the envelope, normalised-envelope (egcn) and traveltime misfits and their
+  gradients (fwi.py:134-243) on the fmc_2d.npz problem (misfits_2d.npz).
 
 The fixtures are data only (numpy .npz, no pickles).
 """
@@ -316,7 +318,47 @@ def main_apertures(ref_root):
     print("aperture / window / Gauss-Newton fixtures written to", HERE)
 
 
-SECTIONS = {"base": main, "apertures": main_apertures}
+def main_misfits(ref_root):
+    """Envelope, normalised-envelope and traveltime misfits (fwi.py:134-243)
+    through the reference's misfit / misfit_and_gradient, on the problem of
+    fmc_2d.npz (test_gradient.py:20-40), with time windows and with a sponge."""
+    sys.path.insert(0, os.path.join(ref_root, "simulation"))
+    from ringfwi import fwi  # noqa: E402
+    from ringfwi.solver import make_sponge  # noqa: E402
+
+    class Geom:  # the element indices of fmc_2d.npz, as the reference reads a geometry
+        def __init__(self, idx):
+            self.idx = [tuple(int(v) for v in r) for r in idx]
+            self.n_elements = len(self.idx)
+
+        def element_index(self, k):
+            return self.idx[k]
+
+    f = np.load(os.path.join(HERE, "fmc_2d.npz"))
+    geom = Geom(f["idx"])
+    h, dt, nt, w = float(f["h"]), float(f["dt"]), int(f["nt"]), f["wavelet"]
+    m0, src = f["m0"], [int(s) for s in f["src_list"]]
+    dobs = f["nosponge_dobs"]
+    t = np.arange(nt, dtype=np.float64)[:, None]
+    win = np.clip((0.75 * nt - t) / (0.1 * nt), 0.0, 1.0) * np.ones((1, geom.n_elements))
+    tw = [win, win[::-1].copy()]
+    sp = make_sponge(m0.shape, width=8)
+    out = {"dsyn0": fwi.forward_fmc(m0, geom, w, dt, h, nt, None, src),
+           "sponge_dsyn0": fwi.forward_fmc(m0, geom, w, dt, h, nt, sp, src)}
+    for mt in ("envelope", "egcn", "traveltime"):
+        J, g = fwi.misfit_and_gradient(m0, geom, w, dt, h, nt, dobs, None, src, mt)
+        out[f"{mt}_J"], out[f"{mt}_g"] = np.array(J), g
+        out[f"{mt}_misfit"] = np.array(fwi.misfit(m0, geom, w, dt, h, nt, dobs, None, src, mt))
+    J, g = fwi.misfit_and_gradient(m0, geom, w, dt, h, nt, dobs, None, src, "envelope",
+                                   time_weights=tw)
+    out["tw_envelope_J"], out["tw_envelope_g"] = np.array(J), g
+    J, g = fwi.misfit_and_gradient(m0, geom, w, dt, h, nt, f["sponge_dobs"], sp, src, "traveltime")
+    out["sponge_traveltime_J"], out["sponge_traveltime_g"] = np.array(J), g
+    np.savez_compressed(os.path.join(HERE, "misfits_2d.npz"), tw=np.stack(tw), sponge=sp, **out)
+    print("misfit fixtures written to", HERE)
+
+
+SECTIONS = {"base": main, "apertures": main_apertures, "misfits": main_misfits}
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--only=")]

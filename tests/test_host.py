@@ -141,10 +141,12 @@ def test_fwi_rejects_unsupported_options_before_touching_the_device():
     g = IdxGeom([(1, 1)])
     m = np.full((8, 8), 1.0 / 1500.0 ** 2)
     with pytest.raises(NotImplementedError):
-        fwi.misfit(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)), misfit_type="envelope")
+        fwi.misfit(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)), misfit_type="gsot")
     with pytest.raises(NotImplementedError):
         fwi.misfit_and_gradient(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)),
-                                misfit_type="traveltime", time_weights=[np.ones((4, 1))])
+                                misfit_type="gsot", time_weights=[np.ones((4, 1))])
+    with pytest.raises(ValueError):
+        fwi.misfit(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)), misfit_type="l1")
     with pytest.raises(ValueError):  # one footprint per element
         fwi._element_groups(g, (8, 8), None, [])
     with pytest.raises(ValueError):  # a weight per point
@@ -281,3 +283,36 @@ def test_group_plan_rejects_bad_groups_before_any_device_call():
         assert rc == _lib.USCT_EINVAL and msg in err, (name, rc, err)
         rc, err = create(*good, *bad)
         assert rc == _lib.USCT_EINVAL and msg in err, (name, rc, err)
+
+
+@pytest.mark.parametrize("mt", ["envelope", "egcn", "traveltime"])
+def test_device_misfit_values_on_reference_traces(mt):
+    """openusct_amd.misfits on the reference's own traces at m0 gives the
+    reference's misfit (fwi.py:134-263; tests/golden/misfits_2d.npz), with time
+    windows and with a sponge run (torch on the CPU here; the same code runs on
+    the device)."""
+    import torch
+    from openusct_amd import misfits
+    f = golden("misfits_2d.npz")
+    base = golden("fmc_2d.npz")
+    d0, dobs = torch.tensor(f["dsyn0"]), torch.tensor(base["nosponge_dobs"])
+    J, adj = misfits.misfit_and_source(d0, dobs, mt)
+    assert abs(J.item() - float(f[f"{mt}_J"])) <= 1e-10 * abs(float(f[f"{mt}_J"]))
+    assert adj.shape == d0.shape and torch.isfinite(adj).all()
+    if mt == "envelope":
+        J, _ = misfits.misfit_and_source(d0, dobs, mt, torch.tensor(f["tw"]))
+        assert abs(J.item() - float(f["tw_envelope_J"])) <= 1e-10 * float(f["tw_envelope_J"])
+    if mt == "traveltime":
+        J, _ = misfits.misfit_and_source(torch.tensor(f["sponge_dsyn0"]),
+                                         torch.tensor(base["sponge_dobs"]), mt)
+        assert abs(J.item() - float(f["sponge_traveltime_J"])) <= 1e-10 * float(f["sponge_traveltime_J"])
+
+
+def test_unknown_and_host_only_misfits_raise():
+    from openusct_amd import fwi
+    with pytest.raises(NotImplementedError):
+        fwi._check_misfit("gsot")
+    with pytest.raises(ValueError):
+        fwi._check_misfit("l3")
+    for mt in ("l2", "gcn", "envelope", "egcn", "traveltime"):
+        fwi._check_misfit(mt)
