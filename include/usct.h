@@ -159,10 +159,27 @@ int usct_misfit_and_gradient_tw(usct_plan *plan, const double *m, const double *
  * the reference's loop stops at k = 1); it is injected as -R^T adj
  * (fwi.py:316-322, footprints included).  grad: device N doubles, dJ/dm.
  * With it any misfit of the forward traces (envelope, normalised envelope,
- * traveltime, fwi.py:134-243) is differentiated through the device adjoint. */
+ * traveltime, optimal transport; fwi.py:134-243) is differentiated through the
+ * device adjoint. */
 int usct_gradient_from_source(usct_plan *plan, const double *m, const double *wavelet,
                               const double *sponge, const double *adj, double *grad,
                               void *stream);
+
+/* Graph-space optimal transport misfit (fwi.py:193-217) of device traces
+ * dsyn, dobs (n_tx*nt*n_rec doubles, layout of dobs): per trace the optimal
+ * assignment between the samples (tau_i = i/nt, s_i/A) and (tau_j, o_j/A),
+ * A = max|o| + 1e-30, for the cost (s_i - o_j)^2 + eta^2 (tau_i - tau_j)^2
+ * (the reference solves it with scipy's linear_sum_assignment; here a
+ * Hungarian method runs in one work-group per trace).  J_trace: device n_tx*n_rec
+ * doubles, the assignment's cost per trace (NaN for a trace with non-finite
+ * samples); adj: device, layout of dsyn, the adjoint source 2 (s_i -
+ * o_sigma(i)) / A.  workspace: device, usct_gsot_workspace_bytes() bytes,
+ * brought by the caller (the call allocates nothing); status: device int, set
+ * to 0, then 1 if any trace had non-finite samples.  The reference's eta is 1. */
+int64_t usct_gsot_workspace_bytes(int n_tx, int nt, int n_rec);
+int usct_gsot(const double *dsyn, const double *dobs, int n_tx, int nt, int n_rec, double eta,
+              void *workspace, int64_t workspace_bytes, double *J_trace, double *adj,
+              int *status, void *stream);
 
 /* Gauss-Newton Hessian-vector product hv = (J^T J) v summed over the plan's
  * transmits (simulation/ringfwi/fwi.py:410-431, gn_hessian_vector): the exact

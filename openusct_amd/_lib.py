@@ -22,7 +22,8 @@ MISFIT_CODES = {"l2": 0, "gcn": 1}
 EXPORTS = (
     "usct_plan_create", "usct_plan_create_groups", "usct_plan_destroy", "usct_plan_get_info", "usct_forward_fmc",
     "usct_misfit", "usct_misfit_and_gradient", "usct_misfit_tw", "usct_misfit_and_gradient_tw",
-    "usct_gn_hessian_vector", "usct_gradient_from_source", "usct_profile_gradient",
+    "usct_gn_hessian_vector", "usct_gradient_from_source", "usct_gsot_workspace_bytes", "usct_gsot",
+    "usct_profile_gradient",
     "usct_forward_fmc_direct", "usct_misfit_and_gradient_direct", "usct_last_error",
     "usct_version",
 )
@@ -89,6 +90,9 @@ def load(path=None):
     lib.usct_misfit_and_gradient.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp, vp]
     lib.usct_gn_hessian_vector.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     lib.usct_gradient_from_source.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    lib.usct_gsot_workspace_bytes.argtypes = [ip, ip, ip]
+    lib.usct_gsot_workspace_bytes.restype = ctypes.c_int64
+    lib.usct_gsot.argtypes = [vp, vp, ip, ip, ip, dp, vp, ctypes.c_int64, vp, vp, vp, vp]
     lib.usct_misfit_tw.argtypes = [vp, vp, vp, vp, vp, vp, ip, vp, vp]
     lib.usct_misfit_and_gradient_tw.argtypes = [vp, vp, vp, vp, vp, vp, ip, vp, vp, vp]
     lib.usct_profile_gradient.argtypes = [vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_double),

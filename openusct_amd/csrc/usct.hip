@@ -1824,6 +1824,37 @@ int usct_gradient_from_source(usct_plan *P, const double *m, const double *wavel
                stream, nullptr);
 }
 
+int64_t usct_gsot_workspace_bytes(int n_tx, int nt, int n_rec) {
+    if (n_tx < 0 || nt < 1 || n_rec < 0) return -1;
+    const int64_t W = (int64_t)nt + 1, ntr = (int64_t)n_tx * n_rec;
+    return ntr * W * (5 * (int64_t)sizeof(double) + 3 * (int64_t)sizeof(int));
+}
+
+int usct_gsot(const double *dsyn, const double *dobs, int n_tx, int nt, int n_rec, double eta,
+              void *workspace, int64_t workspace_bytes, double *J_trace, double *adj, int *status,
+              void *stream) {
+    if (n_tx < 0 || n_rec < 0 || nt < 1 || nt > (1 << 20))
+        return fail(USCT_EINVAL, "gsot: bad sizes n_tx %d nt %d n_rec %d", n_tx, nt, n_rec);
+    const int64_t ntr = (int64_t)n_tx * n_rec;
+    if (ntr > INT32_MAX) return fail(USCT_EINVAL, "gsot: too many traces");
+    const int64_t need = usct_gsot_workspace_bytes(n_tx, nt, n_rec);
+    if (workspace_bytes < need)
+        return fail(USCT_EINVAL, "gsot: workspace of %lld bytes, %lld needed",
+                    (long long)workspace_bytes, (long long)need);
+    if (!status) return fail(USCT_EINVAL, "gsot: status is required");
+    if (ntr > 0 && (!dsyn || !dobs || !workspace || !J_trace || !adj))
+        return fail(USCT_EINVAL, "gsot: null argument");
+    if (!std::isfinite(eta)) return fail(USCT_EINVAL, "gsot: eta is not finite");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIPX(hipMemsetAsync(status, 0, sizeof(int), s));
+    if (ntr == 0) return 0;
+    double *wsd = static_cast<double *>(workspace);
+    int *wsi = reinterpret_cast<int *>(wsd + ntr * 5 * ((int64_t)nt + 1));
+    hipLaunchKernelGGL(k_gsot, dim3((unsigned)ntr), dim3(256), 0, s, dsyn, dobs, nt, n_rec,
+                       (int)ntr, eta * eta, wsd, wsi, J_trace, adj, status);
+    return check_launch();
+}
+
 int usct_misfit_tw(usct_plan *P, const double *m, const double *wavelet, const double *sponge,
                    const double *dobs, const double *time_weights, int misfit_type, double *J,
                    void *stream) {

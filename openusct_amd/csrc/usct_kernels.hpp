@@ -2672,6 +2672,178 @@ __global__ void k_reduce_partials(const float2 *__restrict__ part, int n, double
     else *J = v;
 }
 
+// ---------------------------------------------------------------------------
+// Graph-space optimal transport misfit (fwi.py:193-217), one work-group per
+// trace (transmit t, receiver r).  Synthetic and observed samples, both divided
+// by A = max|observed| + 1e-30, are points (tau_i, s_i) and (tau_j, o_j) with
+// tau_i = i / nt; the assignment sigma minimising sum_i C(i, sigma(i)),
+//   C(i, j) = (s_i - o_j)^2 + eta^2 (tau_i - tau_j)^2,
+// gives J_trace = sum_i C(i, sigma(i)) and the adjoint source
+// 2 (s_i - o_sigma(i)) / A.  sigma comes from the shortest-augmenting-path
+// Hungarian method: rows enter one at a time, a Dijkstra search over columns
+// on the reduced costs C - u - v finds the augmenting path, and the dual
+// potentials u, v are updated along it.  Costs are evaluated on the fly; the
+// column scan and the potential update run over the work-group's lanes, with
+// ties broken on the smallest column so the result is deterministic.  Each
+// search visits at most nt columns and each augmentation walks at most nt
+// steps; non-finite costs, or a search that finds no column, set *status
+// and end the trace's work (its J is NaN).  Indices below are 1-based with
+// column 0 the search root, as the method is usually written.
+// Workspace per trace: 5 (nt + 1) doubles (u, v, minv, s/A, o/A) and
+// 3 (nt + 1) ints (p: row of each column, way, used).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double gsot_cost(const double *sh, const double *oh, int i, int j, int nt,
+                                            double eta2) {
+    const double ds = sh[i] - oh[j];
+    const double dtau = (double)(i - 1) / nt - (double)(j - 1) / nt;
+    return ds * ds + eta2 * (dtau * dtau);
+}
+
+__global__ void __launch_bounds__(256)
+k_gsot(const double *__restrict__ dsyn, const double *__restrict__ dobs, int nt, int n_rx,
+       int n_traces, double eta2, double *__restrict__ wsd, int *__restrict__ wsi,
+       double *__restrict__ J_trace, double *__restrict__ adj, int *__restrict__ status) {
+    const int tr = blockIdx.x;
+    if (tr >= n_traces) return;  // uniform over the work-group
+    const int t = tr / n_rx, r = tr % n_rx;
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int n = nt;
+    const size_t W = (size_t)n + 1;
+    double *u = wsd + (size_t)tr * 5 * W, *v = u + W, *minv = v + W, *sh = minv + W, *oh = sh + W;
+    int *p = wsi + (size_t)tr * 3 * W, *way = p + W, *used = way + W;
+    const double *s = dsyn + (size_t)t * nt * n_rx + r;
+    const double *o = dobs + (size_t)t * nt * n_rx + r;
+    __shared__ double rv[256];
+    __shared__ int ri[256];
+    __shared__ int s_j0, s_bad;
+    // A = max |o| + 1e-30
+    double a = 0.0;
+    for (int j = tid; j < n; j += nth) a = fmax(a, fabs(o[(size_t)j * n_rx]));
+    rv[tid] = a;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k && tid + k < nth) rv[tid] = fmax(rv[tid], rv[tid + k]);
+        __syncthreads();
+    }
+    const double A = rv[0] + 1e-30;
+    for (int j = tid; j <= n; j += nth) {
+        if (j >= 1) {
+            sh[j] = s[(size_t)(j - 1) * n_rx] / A;
+            oh[j] = o[(size_t)(j - 1) * n_rx] / A;
+            if (!__builtin_isfinite(sh[j]) || !__builtin_isfinite(oh[j])) s_bad = 1;
+        }
+        u[j] = 0.0;
+        v[j] = 0.0;
+        p[j] = 0;
+        way[j] = 0;
+    }
+    __syncthreads();
+    for (int i = 1; i <= n && !s_bad; ++i) {
+        for (int j = tid; j <= n; j += nth) {
+            minv[j] = INFINITY;
+            used[j] = 0;
+        }
+        if (tid == 0) {
+            p[0] = i;
+            s_j0 = 0;
+        }
+        __syncthreads();
+        for (int step = 0; step <= n; ++step) {
+            const int j0 = s_j0;
+            if (tid == 0) used[j0] = 1;
+            __syncthreads();
+            const int i0 = p[j0];
+            double best = INFINITY;
+            int bj = 0;
+            for (int j = 1 + tid; j <= n; j += nth) {
+                if (used[j]) continue;
+                const double cur = gsot_cost(sh, oh, i0, j, n, eta2) - u[i0] - v[j];
+                if (cur < minv[j]) {
+                    minv[j] = cur;
+                    way[j] = j0;
+                }
+                if (minv[j] < best) {
+                    best = minv[j];
+                    bj = j;
+                }
+            }
+            rv[tid] = best;
+            ri[tid] = bj;
+            __syncthreads();
+            for (int k = 128; k > 0; k >>= 1) {
+                if (tid < k && tid + k < nth) {
+                    const double ov = rv[tid + k];
+                    const int oi = ri[tid + k];
+                    if (oi != 0 && (ri[tid] == 0 || ov < rv[tid] || (ov == rv[tid] && oi < ri[tid]))) {
+                        rv[tid] = ov;
+                        ri[tid] = oi;
+                    }
+                }
+                __syncthreads();
+            }
+            const double delta = rv[0];
+            const int j1 = ri[0];
+            if (j1 == 0 || !__builtin_isfinite(delta)) {  // no reachable column: not a valid problem
+                if (tid == 0) s_bad = 1;
+                __syncthreads();
+                break;
+            }
+            for (int j = tid; j <= n; j += nth) {
+                if (used[j]) {
+                    u[p[j]] += delta;
+                    v[j] -= delta;
+                } else {
+                    minv[j] -= delta;
+                }
+            }
+            __syncthreads();
+            if (tid == 0) s_j0 = j1;
+            __syncthreads();
+            if (p[j1] == 0) break;  // free column reached
+            if (step == n) {
+                if (tid == 0) s_bad = 1;
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        if (tid == 0 && !s_bad) {
+            int j0 = s_j0;
+            for (int k = 0; k <= n && j0 != 0; ++k) {
+                const int j1 = way[j0];
+                p[j0] = p[j1];
+                j0 = j1;
+            }
+            if (j0 != 0) s_bad = 1;
+        }
+        __syncthreads();
+    }
+    if (s_bad) {
+        if (tid == 0) {
+            *status = 1;
+            J_trace[tr] = NAN;
+        }
+        for (int i = tid; i < n; i += nth) adj[((size_t)t * nt + i) * n_rx + r] = 0.0;
+        return;
+    }
+    // row -> column (way is free now), the trace's misfit and adjoint source
+    for (int j = 1 + tid; j <= n; j += nth) way[p[j]] = j;
+    __syncthreads();
+    double jt = 0.0;
+    for (int i = 1 + tid; i <= n; i += nth) {
+        const int c = way[i];
+        jt += gsot_cost(sh, oh, i, c, n, eta2);
+        adj[((size_t)t * nt + (i - 1)) * n_rx + r] = 2.0 * (sh[i] - oh[c]) / A;
+    }
+    rv[tid] = jt;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (tid < k && tid + k < nth) rv[tid] += rv[tid + k];
+        __syncthreads();
+    }
+    if (tid == 0) J_trace[tr] = rv[0];
+}
+
 // GCN per-trace statistics over time (fwi.py:160-164): for trace t = (b, j)
 // stats[t] = (|s|^2, |o|^2, <s,o>) in FP64; layout of traces [n][j][b] with
 // stride n_rec*Bp between time samples.

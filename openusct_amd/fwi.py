@@ -177,9 +177,6 @@ def _windows_dev(time_weights, plan):
 
 
 def _check_misfit(misfit_type):
-    if misfit_type == "gsot":
-        raise NotImplementedError("misfit_type 'gsot' (an optimal assignment per trace, fwi.py:193-217) "
-                                  "is not implemented on the device")
     if misfit_type not in ("l2", "gcn") + DEVICE_MISFITS:
         raise ValueError(f"unknown misfit_type {misfit_type!r}")
 

@@ -3,7 +3,8 @@
 ``envelope`` (L2 of Hilbert envelopes), ``egcn`` (global correlation norm of
 the envelopes) and ``traveltime`` (cross-correlation lag per trace) are
 evaluated here on torch tensors of all transmits at once, ``(n_tx, nt,
-n_rx)``, with the reference's per-transmit definitions: every reduction the
+n_rx)``; ``gsot`` (graph-space optimal transport) solves its per-trace
+assignment in the library's kernel; with the reference's per-transmit definitions: every reduction the
 reference makes over one transmit's ``(nt, n_rx)`` block (the envelope
 regulariser's maximum included) is made per transmit.  Each returns the misfit
 J (summed over transmits) and the adjoint source dJ/d(traces) in the
@@ -20,7 +21,8 @@ from __future__ import annotations
 
 import torch
 
-DEVICE_MISFITS = ("envelope", "egcn", "traveltime")
+DEVICE_MISFITS = ("envelope", "egcn", "traveltime", "gsot")
+GSOT_ETA = 1.0  # fwi.py:125
 
 
 def analytic(x):
@@ -115,7 +117,33 @@ def traveltime(dsyn, dobs):
     return J, adj.reshape(n_tx, n_rx, nt).permute(0, 2, 1).contiguous()
 
 
-_FUNCS = {"envelope": envelope, "egcn": egcn, "traveltime": traveltime}
+def gsot(dsyn, dobs, eta=GSOT_ETA):
+    """fwi.py:193-217: graph-space optimal transport per trace, the assignment
+    by the library's Hungarian kernel (usct_gsot), one work-group per trace.
+    Device tensors only."""
+    from . import _lib
+    n_tx, nt, n_rx = dsyn.shape
+    if not dsyn.is_cuda:
+        raise ValueError("the GSOT misfit runs on the device: pass device traces")
+    dsyn = dsyn.to(torch.float64).contiguous()
+    dobs = dobs.to(device=dsyn.device, dtype=torch.float64).contiguous()
+    lib = _lib.load()
+    nbytes = int(lib.usct_gsot_workspace_bytes(n_tx, nt, n_rx))
+    if nbytes < 0:
+        raise ValueError(f"bad trace shape {tuple(dsyn.shape)}")
+    ws = torch.empty(max(nbytes, 8), dtype=torch.uint8, device=dsyn.device)
+    jt = torch.empty(n_tx * n_rx, dtype=torch.float64, device=dsyn.device)
+    adj = torch.empty_like(dsyn)
+    status = torch.zeros(1, dtype=torch.int32, device=dsyn.device)
+    stream = torch.cuda.current_stream(dsyn.device).cuda_stream
+    _lib.check(lib.usct_gsot(dsyn.data_ptr(), dobs.data_ptr(), n_tx, nt, n_rx, float(eta), ws.data_ptr(),
+                             nbytes, jt.data_ptr(), adj.data_ptr(), status.data_ptr(), stream))
+    if int(status.item()) != 0:
+        raise ValueError("GSOT misfit: the traces hold non-finite samples")
+    return jt.sum(), adj
+
+
+_FUNCS = {"envelope": envelope, "egcn": egcn, "traveltime": traveltime, "gsot": gsot}
 
 
 def misfit_and_source(dsyn, dobs, misfit_type, weights=None):

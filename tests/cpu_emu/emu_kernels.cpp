@@ -6,7 +6,10 @@
 // aborts.  Usage: emu_kernels <seed> [case-filter]; exit status 0 = all equal.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <cstdlib>
 #include <memory>
 #include <random>
@@ -820,6 +823,126 @@ bool born_x2_cases(uint32_t seed) {
     return ok;
 }
 
+// GSOT assignment (k_gsot): per trace the returned assignment is a
+// permutation and is optimal, certified by the dual potentials the kernel
+// leaves (reduced costs C - u - v >= 0 everywhere, 0 on the assignment); the
+// trace's misfit and adjoint source equal a direct evaluation; for nt <= 7 the
+// misfit equals the brute-force minimum over all permutations.
+bool check_gsot(int n_tx, int nt, int n_rx, int pattern, uint32_t seed) {
+    std::mt19937 rng(seed);
+    std::normal_distribution<double> g(0.0, 1.0);
+    const int ntr = n_tx * n_rx;
+    const int64_t NE = (int64_t)n_tx * nt * n_rx;
+    double *ds = alloc<double>(NE, 0), *dob = alloc<double>(NE, 0);
+    for (int64_t i = 0; i < NE; ++i) {
+        ds[i] = g(rng);
+        dob[i] = 0.8 * ds[i] + 0.5 * g(rng);
+        if (pattern == 1 && (i / n_rx) % nt < nt / 3) ds[i] = dob[i] = 0.0;  // quiet onsets
+    }
+    if (pattern == 2)  // trace 0: observed all zero; trace 1: both zero
+        for (int k = 0; k < nt; ++k) {
+            dob[(int64_t)k * n_rx] = 0.0;
+            if (n_rx > 1) ds[(int64_t)k * n_rx + 1] = dob[(int64_t)k * n_rx + 1] = 0.0;
+        }
+    const int64_t W = nt + 1;
+    double *wsd = alloc<double>(5 * W * ntr, 0), *Jt = alloc<double>(ntr, 0), *adj = alloc<double>(NE, 0);
+    int *wsi = alloc<int>(3 * W * ntr, 0), *status = alloc<int>(1, 0);
+    emu::launch((unsigned)ntr, 256, [&] {
+        k_gsot(ds, dob, nt, n_rx, ntr, 1.0, wsd, wsi, Jt, adj, status);
+    });
+    char tag[96];
+    std::snprintf(tag, sizeof tag, "gsot n_tx=%d nt=%d n_rx=%d pattern=%d", n_tx, nt, n_rx, pattern);
+    bool ok = *status == 0;
+    if (!ok) std::printf("MISMATCH %s status %d\n", tag, *status);
+    for (int tr = 0; tr < ntr && ok; ++tr) {
+        const int t = tr / n_rx, r = tr % n_rx;
+        const double *u = wsd + (int64_t)tr * 5 * W, *v = u + W, *sh = v + 2 * W, *oh = sh + W;
+        const int *pp = wsi + (int64_t)tr * 3 * W;
+        double A = 0.0;
+        for (int k = 0; k < nt; ++k) A = std::max(A, std::fabs(dob[((int64_t)t * nt + k) * n_rx + r]));
+        A += 1e-30;
+        std::vector<int> col(nt + 1, 0), seen(nt + 1, 0);
+        for (int j = 1; j <= nt; ++j) {
+            if (pp[j] < 1 || pp[j] > nt || seen[pp[j]]++) { ok = false; break; }
+            col[pp[j]] = j;
+        }
+        if (!ok) { std::printf("MISMATCH %s trace %d: not a permutation\n", tag, tr); break; }
+        double cmax = 0.0, jd = 0.0;
+        for (int i = 1; i <= nt; ++i)
+            for (int j = 1; j <= nt; ++j) cmax = std::max(cmax, gsot_cost(sh, oh, i, j, nt, 1.0));
+        const double tol = 1e-9 * (cmax + 1.0);
+        for (int i = 1; i <= nt && ok; ++i) {
+            for (int j = 1; j <= nt; ++j) {
+                const double rc = gsot_cost(sh, oh, i, j, nt, 1.0) - u[i] - v[j];
+                if (rc < -tol || (j == col[i] && std::fabs(rc) > tol)) {
+                    std::printf("MISMATCH %s trace %d: reduced cost %g at (%d,%d)\n", tag, tr, rc, i, j);
+                    ok = false;
+                    break;
+                }
+            }
+            const double c = gsot_cost(sh, oh, i, col[i], nt, 1.0);
+            jd += c;
+            const double s_i = ds[((int64_t)t * nt + i - 1) * n_rx + r] / A;
+            const double o_c = dob[((int64_t)t * nt + col[i] - 1) * n_rx + r] / A;
+            const double want = 2.0 * (s_i - o_c) / A;
+            if (std::memcmp(&want, &adj[((int64_t)t * nt + i - 1) * n_rx + r], 8) != 0) {
+                std::printf("MISMATCH %s trace %d: adjoint source at %d\n", tag, tr, i);
+                ok = false;
+            }
+        }
+        if (ok && std::fabs(jd - Jt[tr]) > 1e-12 * (std::fabs(jd) + 1e-300)) {
+            std::printf("MISMATCH %s trace %d: J %g vs %g\n", tag, tr, Jt[tr], jd);
+            ok = false;
+        }
+        if (ok && nt <= 7) {  // brute force
+            std::vector<int> perm(nt);
+            for (int k = 0; k < nt; ++k) perm[k] = k + 1;
+            double best = INFINITY;
+            do {
+                double c = 0.0;
+                for (int k = 0; k < nt; ++k) c += gsot_cost(sh, oh, k + 1, perm[k], nt, 1.0);
+                best = std::min(best, c);
+            } while (std::next_permutation(perm.begin(), perm.end()));
+            if (std::fabs(best - jd) > 1e-12 * (std::fabs(best) + 1e-300)) {
+                std::printf("MISMATCH %s trace %d: brute force %g vs %g\n", tag, tr, best, jd);
+                ok = false;
+            }
+        }
+    }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+// A trace with a NaN sample ends with the status flag set and a NaN misfit;
+// the other traces are unaffected.
+bool check_gsot_nan() {
+    const int nt = 9, n_rx = 2;
+    double *ds = alloc<double>(nt * n_rx, 0), *dob = alloc<double>(nt * n_rx, 0);
+    for (int i = 0; i < nt * n_rx; ++i) {
+        ds[i] = std::sin(0.7 * i);
+        dob[i] = std::cos(0.3 * i);
+    }
+    ds[4 * n_rx + 1] = NAN;
+    const int64_t W = nt + 1;
+    double *wsd = alloc<double>(5 * W * n_rx, 0), *Jt = alloc<double>(n_rx, 0), *adj = alloc<double>(nt * n_rx, 0);
+    int *wsi = alloc<int>(3 * W * n_rx, 0), *status = alloc<int>(1, 0);
+    emu::launch((unsigned)n_rx, 256, [&] { k_gsot(ds, dob, nt, n_rx, n_rx, 1.0, wsd, wsi, Jt, adj, status); });
+    const bool ok = *status == 1 && std::isnan(Jt[1]) && std::isfinite(Jt[0]) && adj[4 * n_rx + 1] == 0.0;
+    std::printf("%s gsot NaN trace flagged\n", ok ? "ok" : "FAIL");
+    return ok;
+}
+
+bool gsot_cases(uint32_t seed) {
+    bool ok = check_gsot_nan();
+    for (int nt : {1, 2, 5, 7})
+        for (int pattern : {0, 1}) ok &= check_gsot(2, nt, 3, pattern, seed++);
+    ok &= check_gsot(1, 7, 2, 2, seed++);
+    ok &= check_gsot(2, 64, 3, 0, seed++);
+    ok &= check_gsot(1, 300, 2, 1, seed++);
+    ok &= check_gsot(1, 257, 2, 2, seed++);
+    return ok;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -848,6 +971,9 @@ int main(int argc, char **argv) {
     }
     if (only.empty() || only == "born") {
         ok &= born_cases(seed + 3000);
+    }
+    if (only.empty() || only == "gsot") {
+        ok &= gsot_cases(seed + 6000);
     }
     if (only.empty() || only == "born-x2") {
         ok &= born_x2_cases<16, 4>(seed + 5000);

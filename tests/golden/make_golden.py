@@ -24,8 +24,9 @@ outputs of the functions whose behaviour only the reference itself pins:
   (fwi.py:670-699) and windowed misfits (fwi.py:255-262, 298-301), and
   Gauss-Newton products with a sponge and with footprints (fwi.py:410-431),
   whose sponge part is the reference's own formula;
-* the envelope, normalised-envelope (egcn) and traveltime misfits and their
-  gradients (fwi.py:134-243) on the fmc_2d.npz problem (misfits_2d.npz).
+* the envelope, normalised-envelope (egcn), traveltime and graph-space optimal
+  transport (gsot) misfits and their gradients (fwi.py:134-243) on the
+  fmc_2d.npz problem (misfits_2d.npz).
 
 The fixtures are data only (numpy .npz, no pickles).
 """
@@ -345,7 +346,7 @@ def main_misfits(ref_root):
     sp = make_sponge(m0.shape, width=8)
     out = {"dsyn0": fwi.forward_fmc(m0, geom, w, dt, h, nt, None, src),
            "sponge_dsyn0": fwi.forward_fmc(m0, geom, w, dt, h, nt, sp, src)}
-    for mt in ("envelope", "egcn", "traveltime"):
+    for mt in ("envelope", "egcn", "traveltime", "gsot"):
         J, g = fwi.misfit_and_gradient(m0, geom, w, dt, h, nt, dobs, None, src, mt)
         out[f"{mt}_J"], out[f"{mt}_g"] = np.array(J), g
         out[f"{mt}_misfit"] = np.array(fwi.misfit(m0, geom, w, dt, h, nt, dobs, None, src, mt))

@@ -1,14 +1,16 @@
-"""GPU: the envelope, normalised-envelope (egcn) and traveltime misfits
-(simulation/ringfwi/fwi.py:134-243) through the device -- forward traces,
-adjoint source on the device, the library's adjoint sweep on it
+"""GPU: the envelope, normalised-envelope (egcn), traveltime and graph-space
+optimal transport (gsot) misfits (simulation/ringfwi/fwi.py:134-243) through
+the device -- forward traces, adjoint source on the device (gsot: the
+library's assignment kernel), the library's adjoint sweep on it
 (usct_gradient_from_source).
 
 Against the reference's own misfit / misfit_and_gradient on the problem of
 fmc_2d.npz (tests/golden/misfits_2d.npz): relative L2 <= 1e-5 for the gradient
 and misfit, as for L2.  The directional finite-difference checks of
 test_gradient_misfits.py:38-74 with the reference's tolerances (1e-3 for the
-envelope misfits, 0.15 for the stationary-phase traveltime adjoint); the step
-is 1e-4 of m (the reference's 1e-6 assumes FP64 fields).
+envelope misfits, 0.15 for the stationary-phase traveltime adjoint, 1e-2 for
+gsot); the step is 1e-4 of m (the reference's 1e-6 assumes FP64 fields), for
+gsot the reference's 1e-7, which keeps the assignment fixed.
 """
 
 from __future__ import annotations
@@ -39,7 +41,7 @@ def _problem():
 
 
 @pytest.mark.parametrize("method", ["batched", "plain"])
-@pytest.mark.parametrize("mt", ["envelope", "egcn", "traveltime"])
+@pytest.mark.parametrize("mt", ["envelope", "egcn", "traveltime", "gsot"])
 def test_misfit_gradient_vs_reference(mt, method):
     from openusct_amd import fwi
     f, geom, h, dt, nt, w, src = _problem()
@@ -71,8 +73,9 @@ def test_windowed_envelope_and_sponge_traveltime_vs_reference():
     assert max(e1 + e2) < TOL
 
 
-@pytest.mark.parametrize("mt,tol", [("envelope", 1e-3), ("egcn", 1e-3), ("traveltime", 0.15)])
-def test_directional_finite_difference(mt, tol):
+@pytest.mark.parametrize("mt,tol,eps", [("envelope", 1e-3, 1e-4), ("egcn", 1e-3, 1e-4),
+                                        ("traveltime", 0.15, 1e-4), ("gsot", 1e-2, 1e-7)])
+def test_directional_finite_difference(mt, tol, eps):
     """test_gradient_misfits.py:21-74 on the device."""
     from openusct_amd import fwi, phantom
     from openusct_amd.geometry import RingArray
@@ -93,7 +96,6 @@ def test_directional_finite_difference(mt, tol):
     dm[:5, :] = dm[-5:, :] = dm[:, :5] = dm[:, -5:] = 0.0
     dm *= float(np.mean(m0))
     analytic = float(np.sum(g * dm))
-    eps = 1e-4
     Jp = fwi.misfit(m0 + eps * dm, geom, wav, dt, h, nt, dobs, src_list=src, misfit_type=mt)
     Jm = fwi.misfit(m0 - eps * dm, geom, wav, dt, h, nt, dobs, src_list=src, misfit_type=mt)
     fd = (Jp - Jm) / (2 * eps)
@@ -135,3 +137,14 @@ def test_invert_with_envelope_reduces_misfit():
                          misfit_type="envelope")
     print(hist)
     assert hist[-1] < hist[0] and np.all(np.isfinite(m))
+
+
+def test_gsot_flags_non_finite_traces():
+    from openusct_amd import misfits
+    d = torch.randn(2, 40, 3, dtype=torch.float64, device="cuda")
+    o = torch.randn(2, 40, 3, dtype=torch.float64, device="cuda")
+    J, adj = misfits.gsot(d, o)
+    assert torch.isfinite(J) and torch.isfinite(adj).all()
+    d[1, 7, 2] = float("nan")
+    with pytest.raises(ValueError):
+        misfits.gsot(d, o)

@@ -140,11 +140,9 @@ def test_fwi_rejects_unsupported_options_before_touching_the_device():
     from openusct_amd import fwi
     g = IdxGeom([(1, 1)])
     m = np.full((8, 8), 1.0 / 1500.0 ** 2)
-    with pytest.raises(NotImplementedError):
-        fwi.misfit(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)), misfit_type="gsot")
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):
         fwi.misfit_and_gradient(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)),
-                                misfit_type="gsot", time_weights=[np.ones((4, 1))])
+                                misfit_type="wasserstein", time_weights=[np.ones((4, 1))])
     with pytest.raises(ValueError):
         fwi.misfit(m, g, np.zeros(4), 1e-8, 1e-3, 4, np.zeros((1, 4, 1)), misfit_type="l1")
     with pytest.raises(ValueError):  # one footprint per element
@@ -174,8 +172,10 @@ def test_library_loads_and_exports_every_declared_symbol():
     from openusct_amd import _lib
     lib = _lib.load()
     header = open(os.path.join(ROOT, "include", "usct.h")).read()
-    declared = set(re.findall(r"^(?:int|const char \*)\s*(usct_\w+)\(", header, re.M))
+    declared = set(re.findall(r"^(?:int|int64_t|const char \*)\s*(usct_\w+)\(", header, re.M))
     assert declared == set(_lib.EXPORTS)
+    assert lib.usct_gsot_workspace_bytes(2, 10, 3) == 6 * 11 * (5 * 8 + 3 * 4)
+    assert lib.usct_gsot_workspace_bytes(1, 0, 1) == -1
     for name in declared:
         assert hasattr(lib, name), name
     assert lib.usct_version() == 1
@@ -308,11 +308,9 @@ def test_device_misfit_values_on_reference_traces(mt):
         assert abs(J.item() - float(f["sponge_traveltime_J"])) <= 1e-10 * float(f["sponge_traveltime_J"])
 
 
-def test_unknown_and_host_only_misfits_raise():
+def test_unknown_misfits_raise():
     from openusct_amd import fwi
-    with pytest.raises(NotImplementedError):
-        fwi._check_misfit("gsot")
     with pytest.raises(ValueError):
         fwi._check_misfit("l3")
-    for mt in ("l2", "gcn", "envelope", "egcn", "traveltime"):
+    for mt in ("l2", "gcn", "envelope", "egcn", "traveltime", "gsot"):
         fwi._check_misfit(mt)

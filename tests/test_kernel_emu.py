@@ -10,7 +10,9 @@ and 3D forward kernels equal a direct loop over cells of their FP64 update and
 the 3D adjoint (patch and LDS-tile kernels) a direct loop of its FP32 update
 and imaging sums, the Born step a direct loop of its FP64 update, the two-level
 3D LDS forward two single-level launches, the two-level Gauss-Newton launch
-(background and Born levels together) two forward and two Born launches,
+(background and Born levels together) two forward and two Born launches, the
+GSOT assignment kernel against the optimality certificate of its own dual
+potentials and, on short traces, brute force over all permutations,
 over odd grids, sources and receivers on borders
 and strip edges, duplicate receivers and every lane-group / tile shape the
 autotuner may pick, and that no access leaves the guard zones the library
@@ -50,7 +52,7 @@ def emu_binary(tmp_path_factory):
 
 
 @pytest.mark.parametrize("seed", [1, 2])
-@pytest.mark.parametrize("which", ["fwd", "adj", "fwd3d", "lds", "born", "x2-3d", "born-x2"])
+@pytest.mark.parametrize("which", ["fwd", "adj", "fwd3d", "lds", "born", "x2-3d", "born-x2", "gsot"])
 def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
     r = subprocess.run([emu_binary, str(seed), which], capture_output=True, text=True,
@@ -59,4 +61,4 @@ def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     assert r.returncode == 0, tail
     assert "ALL EQUAL" in r.stdout, tail
     assert r.stdout.count("ok ") == {"fwd": 75, "adj": 65, "fwd3d": 50, "lds": 12,
-                                     "born": 5, "x2-3d": 6, "born-x2": 52}[which], tail
+                                     "born": 5, "x2-3d": 6, "born-x2": 52, "gsot": 13}[which], tail
