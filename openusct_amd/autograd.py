@@ -10,14 +10,64 @@ chain rule through m = 1/c^2 (simulation/ringfwi/phantom.py:13-14):
     dJ/dc = g * dm/dc = -2 g / c^3.
 
 The gradient is that of the solver's discrete scheme, so it matches a finite
-difference of the same solver's misfit (not the continuous PDE).
+difference of the same solver's misfit (not the continuous PDE).  Every
+misfit type of fwi.py:134-243 is accepted: L2 and GCN run fused in the
+library; the others form their adjoint source from the device traces
+(openusct_amd.misfits) and run the library's adjoint sweep on it.
+
+``traces(m, plan, wavelet)`` is the lower-level piece: the full-matrix traces
+of the squared-slowness model m as a differentiable tensor whose backward
+applies the exact discrete adjoint (Plan.gradient_from_source), so any misfit
+written in torch on the traces gets its m-gradient from the device solver.
 """
 
 from __future__ import annotations
 
 import torch
 
+from . import misfits as trace_misfits
 from .plan import Plan
+
+
+class Traces(torch.autograd.Function):
+    """m -> traces (n_tx, nt, n_rec); backward = transpose of the discrete
+    forward's Jacobian, applied by the device adjoint sweep."""
+
+    @staticmethod
+    def forward(ctx, m, plan: Plan, wavelet, sponge=None):
+        m64 = m.detach().to(torch.float64)
+        ctx.save_for_backward(m64)
+        ctx.plan, ctx.wavelet, ctx.sponge, ctx.in_dtype = plan, wavelet, sponge, m.dtype
+        d = plan.forward_fmc(m64, wavelet, sponge)
+        return d.to(m.dtype) if m.dtype != torch.float64 else d
+
+    @staticmethod
+    def backward(ctx, grad_d):
+        (m64,) = ctx.saved_tensors
+        g = ctx.plan.gradient_from_source(m64, ctx.wavelet, grad_d.to(torch.float64).contiguous(),
+                                          ctx.sponge)
+        return g.to(ctx.in_dtype), None, None, None
+
+
+def traces(m, plan, wavelet, sponge=None):
+    """Differentiable full-matrix traces of the squared-slowness model ``m``."""
+    return Traces.apply(m, plan, wavelet, sponge)
+
+
+def _misfit_and_gradient(plan, m, wavelet, dobs, sponge, misfit_type):
+    if misfit_type in trace_misfits.DEVICE_MISFITS:
+        d = plan.forward_fmc(m, wavelet, sponge)
+        J, adj = trace_misfits.misfit_and_source(d, dobs.to(d), misfit_type)
+        return J.reshape(1), plan.gradient_from_source(m, wavelet, adj, sponge)
+    return plan.misfit_and_gradient(m, wavelet, dobs, sponge, misfit_type)
+
+
+def _misfit(plan, m, wavelet, dobs, misfit_type, sponge=None):
+    if misfit_type in trace_misfits.DEVICE_MISFITS:
+        d = plan.forward_fmc(m, wavelet, sponge)
+        J, _ = trace_misfits.misfit_and_source(d, dobs.to(d), misfit_type)
+        return J.reshape(1)
+    return plan.misfit(m, wavelet, dobs, sponge, misfit_type=misfit_type)
 
 
 class SoundSpeedMisfit(torch.autograd.Function):
@@ -27,7 +77,7 @@ class SoundSpeedMisfit(torch.autograd.Function):
     def forward(ctx, c, plan: Plan, wavelet, dobs, misfit_type="l2", sponge=None):
         c64 = c.detach().to(torch.float64)
         m = 1.0 / (c64 * c64)
-        J, g = plan.misfit_and_gradient(m, wavelet, dobs, sponge, misfit_type)
+        J, g = _misfit_and_gradient(plan, m, wavelet, dobs, sponge, misfit_type)
         ctx.save_for_backward(c64, g)
         ctx.in_dtype = c.dtype
         return J[0].to(c.dtype) if c.dtype != torch.float64 else J[0]
@@ -72,8 +122,7 @@ def gradient_descent(c0, plan, wavelet, dobs, steps=5, step_frac=0.02, c_bounds=
                 c_try = c - step * d
                 if c_bounds is not None:
                     c_try.clamp_(c_bounds[0], c_bounds[1])
-                J_try = plan.misfit(1.0 / (c_try * c_try), wavelet, dobs,
-                                    misfit_type=misfit_type)
+                J_try = _misfit(plan, 1.0 / (c_try * c_try), wavelet, dobs, misfit_type)
                 if float(J_try.item()) < float(J.item()):
                     accepted = True
                     break
