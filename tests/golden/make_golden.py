@@ -26,7 +26,8 @@ outputs of the functions whose behaviour only the reference itself pins:
   whose sponge part is the reference's own formula;
 * the envelope, normalised-envelope (egcn), traveltime and graph-space optimal
   transport (gsot) misfits and their gradients (fwi.py:134-243) on the
-  fmc_2d.npz problem (misfits_2d.npz).
+  fmc_2d.npz problem, and the envelope misfit with the footprints of
+  aperture_2d.npz (misfits_2d.npz).
 
 The fixtures are data only (numpy .npz, no pickles).
 """
@@ -355,6 +356,17 @@ def main_misfits(ref_root):
     out["tw_envelope_J"], out["tw_envelope_g"] = np.array(J), g
     J, g = fwi.misfit_and_gradient(m0, geom, w, dt, h, nt, f["sponge_dobs"], sp, src, "traveltime")
     out["sponge_traveltime_J"], out["sponge_traveltime_g"] = np.array(J), g
+    # finite-aperture elements (footprints of aperture_2d.npz) with the envelope misfit
+    a = np.load(os.path.join(HERE, "aperture_2d.npz"))
+    ends = np.cumsum(a["fp_counts"])
+    fps = [([tuple(int(v) for v in c) for c in a["fp_cells"][e - k:e]], a["fp_w"][e - k:e])
+           for k, e in zip(a["fp_counts"], ends)]
+    ag = Geom(a["idx"])
+    asrc = [int(s) for s in a["src_list"]]
+    J, g = fwi.misfit_and_gradient(a["m0"], ag, a["wavelet"], float(a["dt"]), float(a["h"]),
+                                   int(a["nt"]), a["nosponge_dobs"], None, asrc, "envelope",
+                                   footprints=fps)
+    out["fp_envelope_J"], out["fp_envelope_g"] = np.array(J), g
     np.savez_compressed(os.path.join(HERE, "misfits_2d.npz"), tw=np.stack(tw), sponge=sp, **out)
     print("misfit fixtures written to", HERE)
 

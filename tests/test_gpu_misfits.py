@@ -184,3 +184,23 @@ def test_traces_autograd_gives_the_device_adjoint():
         _, hist = A.gradient_descent(c0, p, wd, dobs, steps=2, step_frac=0.005, misfit_type="gsot")
         print("gsot descent", hist)
         assert hist[-1] <= hist[0]
+
+
+@pytest.mark.parametrize("method", ["batched", "plain"])
+def test_envelope_with_footprints_vs_reference(method):
+    """Finite-aperture elements through the adjoint-source path (the source
+    spread over each receiver's footprint, fwi.py:316-319)."""
+    from helpers import footprints_from
+    from openusct_amd import fwi
+    a = golden("aperture_2d.npz")
+    g_ref = golden("misfits_2d.npz")
+    fps = footprints_from(a["fp_cells"], a["fp_counts"], a["fp_w"])
+    fwi.METHOD = method
+    print(f"[case] envelope + footprints {method}", flush=True)
+    J, g = fwi.misfit_and_gradient(a["m0"], IdxGeom(a["idx"]), a["wavelet"], float(a["dt"]),
+                                   float(a["h"]), int(a["nt"]), a["nosponge_dobs"], None,
+                                   [int(s) for s in a["src_list"]], "envelope", footprints=fps)
+    e = (abs(J - float(g_ref["fp_envelope_J"])) / float(g_ref["fp_envelope_J"]),
+         rel_l2(g, g_ref["fp_envelope_g"]))
+    print(f"J {e[0]:.2e} gradient {e[1]:.2e}")
+    assert max(e) < TOL
