@@ -94,6 +94,62 @@ class CylinderArray:
         return self.xyz - self.domain_m / 2.0
 
 
+def fibonacci_directions(n, hemisphere=False):
+    """n near-uniform unit vectors on the sphere, or on the upper hemisphere
+    (geometry.py:140-146): a Fibonacci lattice, cos(polar angle) evenly spaced at
+    the midpoints of n intervals and azimuths advancing by the golden angle."""
+    k = np.arange(n) + 0.5
+    cz = (1.0 - k / n) if hemisphere else (1.0 - 2.0 * k / n)
+    rho = np.sqrt(np.clip(1.0 - cz * cz, 0.0, 1.0))
+    az = np.pi * (1.0 + 5.0 ** 0.5) * k
+    return np.column_stack([rho * np.cos(az), rho * np.sin(az), cz])
+
+
+@dataclass
+class ShellArray:
+    """Elements on a spherical shell of radius ``radius_m`` about the centre of a
+    cubic domain (geometry.py:149-186), over the whole sphere or its upper half
+    (z above the centre); indices (iz, iy, ix), rounded half to even."""
+
+    n_elements: int
+    radius_m: float
+    domain_m: float
+    h: float
+    hemisphere: bool = False
+
+    def __post_init__(self):
+        self.n = int(round(self.domain_m / self.h)) + 1
+        c = self.domain_m / 2.0
+        self.xyz = c + self.radius_m * fibonacci_directions(self.n_elements, self.hemisphere)
+        self.idx = [(int(round(z / self.h)), int(round(y / self.h)), int(round(x / self.h)))
+                    for x, y, z in self.xyz]
+
+    @property
+    def grid_shape(self):
+        return (self.n, self.n, self.n)
+
+    def element_index(self, k):
+        return self.idx[k]
+
+    @property
+    def element_positions(self):
+        return self.xyz - self.domain_m / 2.0
+
+    @property
+    def array_type(self):
+        return "hemisphere" if self.hemisphere else "sphere"
+
+
+def HemisphereArray(n_elements, radius_m, domain_m, h):
+    """Shell array over the upper hemisphere (geometry.py:189-191)."""
+    return ShellArray(n_elements, radius_m, domain_m, h, hemisphere=True)
+
+
+def SphereArray(n_elements, radius_m, domain_m, h):
+    """Shell array over the whole sphere (geometry.py:194-196)."""
+    return ShellArray(n_elements, radius_m, domain_m, h, hemisphere=False)
+
+
 @dataclass
 class GridArray:
     """Array given directly by grid index tuples (geometry.py:281-301)."""

@@ -27,7 +27,9 @@ outputs of the functions whose behaviour only the reference itself pins:
 * the envelope, normalised-envelope (egcn), traveltime and graph-space optimal
   transport (gsot) misfits and their gradients (fwi.py:134-243) on the
   fmc_2d.npz problem, and the envelope misfit with the footprints of
-  aperture_2d.npz (misfits_2d.npz).
+  aperture_2d.npz (misfits_2d.npz);
+* the other wavelets, phantoms and array geometries of the reference
+  (builders.npz).
 
 The fixtures are data only (numpy .npz, no pickles).
 """
@@ -371,7 +373,48 @@ def main_misfits(ref_root):
     print("misfit fixtures written to", HERE)
 
 
-SECTIONS = {"base": main, "apertures": main_apertures, "misfits": main_misfits}
+def main_builders(ref_root):
+    """The other input builders: Gabor and Hann tone-burst wavelets
+    (sources.py:29-62), Gaussian smoothing, the sphere background and the 2D /
+    3D Voronoi polycrystals from seeded generators (phantom.py:51-188), shell,
+    hemisphere and sphere arrays (geometry.py:140-196) and footprints on them."""
+    sys.path.insert(0, os.path.join(ref_root, "simulation"))
+    from ringfwi import phantom  # noqa: E402
+    from ringfwi.geometry import HemisphereArray, ShellArray, SphereArray, build_footprints  # noqa: E402
+    from ringfwi.sources import gabor, hann_tone_burst  # noqa: E402
+
+    out = {}
+    out["gabor0"] = gabor(400, 5e-8, 5e5)
+    out["gabor1"] = gabor(333, 2.5e-8, 1.2e6, frac_bw=0.9, t0=2e-6)
+    out["hann0"] = hann_tone_burst(300, 1e-7, 4e5)
+    out["hann1"] = hann_tone_burst(500, 5e-8, 7.5e5, n_cycles=3, t0=1.3e-6)
+    c = phantom.coupling_background((41, 41), 3000.0, 1480.0, 0.015, 1e-3)
+    out["smooth_in"] = c
+    out["smooth_out"] = phantom.smooth(c, 2.0)
+    out["sphere_bg"] = phantom.sphere_background((17, 19, 21), 3000.0, 1480.0, 0.006, 1e-3)
+    lab, ang, th = phantom.voronoi_polycrystal((61, 61), 12, 0.02, 1e-3, rng=np.random.default_rng(7),
+                                                relax=2)
+    out.update(vor_labels=lab, vor_angles=ang, vor_theta=th)
+    lab, axes, col = phantom.voronoi_polycrystal_3d((15, 21, 21), 9, 0.008, 1e-3,
+                                                     rng=np.random.default_rng(8), relax=1)
+    out.update(vor3_labels=lab, vor3_axes=axes, vor3_colat=col)
+    arrays = [SphereArray(40, 0.012, 0.03, 1e-3), HemisphereArray(33, 0.011, 0.03, 7.5e-4),
+              ShellArray(64, 0.02, 0.0512, 2e-4)]
+    for i, a in enumerate(arrays):
+        out[f"shell{i}_idx"] = np.array(a.idx, dtype=np.int64)
+        out[f"shell{i}_pos"] = a.element_positions
+        out[f"shell{i}_n"] = np.array(a.n)
+        out[f"shell{i}_type"] = np.array(a.array_type)
+    fps = build_footprints(arrays[0], 0.004, "disc", height_m=0.003, max_pts=7)
+    out["shell_fp_cells"] = np.array([ix for idxs, _ in fps for ix in idxs], dtype=np.int64)
+    out["shell_fp_counts"] = np.array([len(idxs) for idxs, _ in fps], dtype=np.int64)
+    out["shell_fp_w"] = np.concatenate([np.asarray(w, dtype=np.float64) for _, w in fps])
+    np.savez_compressed(os.path.join(HERE, "builders.npz"), **out)
+    print("builder fixtures written to", HERE)
+
+
+SECTIONS = {"base": main, "apertures": main_apertures, "misfits": main_misfits,
+            "builders": main_builders}
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--only=")]

@@ -346,3 +346,30 @@ def test_c4_reduced_vs_oracle():
     from openusct_amd import configs
     e = _oracle_vs_device(configs.c4(n=64, nt_scale=0.15, shots=4), ckpt_every=50)
     assert e[0] < TOL and e[1] < TOL and e[2] < TOL
+
+
+@pytest.mark.parametrize("hemisphere", [False, True])
+def test_shell_array_3d_vs_oracle(hemisphere):
+    """Sphere / hemisphere arrays (geometry.py:140-196) on a sphere phantom
+    with an inclusion: traces and gradient against the FP64 oracle."""
+    from oracle import oracle as O
+    from openusct_amd import fwi, phantom
+    from openusct_amd.geometry import HemisphereArray, SphereArray
+    from openusct_amd.sources import hann_tone_burst
+    h = 1e-3
+    geom = (HemisphereArray if hemisphere else SphereArray)(26, 0.009, 0.024, h)
+    n = geom.n
+    dt, nt = 1.2e-7, 130
+    w = hann_tone_burst(nt, dt, 0.4e6, n_cycles=2)
+    c_true = phantom.add_sphere(phantom.sphere_background((n, n, n), 2900.0, 1480.0, 0.007, h),
+                                (0.55, 0.5, 0.45), 0.003, 2500.0, h)
+    m0 = phantom.velocity_to_m(phantom.sphere_background((n, n, n), 2900.0, 1480.0, 0.007, h))
+    src = [0, 9, 17, 25]
+    print(f"[case] shell array 3D hemisphere={hemisphere} n={n}", flush=True)
+    dobs = O.forward_fmc(phantom.velocity_to_m(c_true), geom, w, dt, h, nt, None, src)
+    d = fwi.forward_fmc(phantom.velocity_to_m(c_true), geom, w, dt, h, nt, None, src)
+    J_ref, g_ref = O.misfit_and_gradient(m0, geom, w, dt, h, nt, dobs, None, src)
+    J, g = fwi.misfit_and_gradient(m0, geom, w, dt, h, nt, dobs, None, src)
+    e = (rel_l2(d, dobs), abs(J - J_ref) / J_ref, rel_l2(g, g_ref))
+    print(f"traces {e[0]:.2e} J {e[1]:.2e} gradient {e[2]:.2e}")
+    assert max(e) < 1e-5

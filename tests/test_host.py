@@ -314,3 +314,41 @@ def test_unknown_misfits_raise():
         fwi._check_misfit("l3")
     for mt in ("l2", "gcn", "envelope", "egcn", "traveltime", "gsot"):
         fwi._check_misfit(mt)
+
+
+def test_other_wavelets_phantoms_and_shell_arrays_match_reference():
+    """sources.gabor / hann_tone_burst, phantom.smooth / sphere_background /
+    voronoi_polycrystal(_3d) from seeded generators, Shell/Hemisphere/Sphere
+    arrays and footprints on them: bit-exact against the reference
+    (tests/golden/builders.npz)."""
+    from openusct_amd import phantom
+    from openusct_amd.geometry import HemisphereArray, ShellArray, SphereArray, build_footprints
+    from openusct_amd.sources import gabor, hann_tone_burst
+    f = golden("builders.npz")
+    assert np.array_equal(gabor(400, 5e-8, 5e5), f["gabor0"])
+    assert np.array_equal(gabor(333, 2.5e-8, 1.2e6, frac_bw=0.9, t0=2e-6), f["gabor1"])
+    assert np.array_equal(hann_tone_burst(300, 1e-7, 4e5), f["hann0"])
+    assert np.array_equal(hann_tone_burst(500, 5e-8, 7.5e5, n_cycles=3, t0=1.3e-6), f["hann1"])
+    assert np.array_equal(phantom.smooth(f["smooth_in"], 2.0), f["smooth_out"])
+    assert np.array_equal(phantom.sphere_background((17, 19, 21), 3000.0, 1480.0, 0.006, 1e-3),
+                          f["sphere_bg"])
+    lab, ang, th = phantom.voronoi_polycrystal((61, 61), 12, 0.02, 1e-3, rng=np.random.default_rng(7),
+                                               relax=2)
+    assert np.array_equal(lab, f["vor_labels"]) and np.array_equal(ang, f["vor_angles"])
+    assert np.array_equal(th, f["vor_theta"])
+    lab, axes, col = phantom.voronoi_polycrystal_3d((15, 21, 21), 9, 0.008, 1e-3,
+                                                    rng=np.random.default_rng(8), relax=1)
+    assert np.array_equal(lab, f["vor3_labels"]) and np.array_equal(axes, f["vor3_axes"])
+    assert np.array_equal(col, f["vor3_colat"])
+    with pytest.raises(ValueError):
+        phantom.voronoi_polycrystal((5, 5, 5), 2, 0.001, 1e-3)
+    arrays = [SphereArray(40, 0.012, 0.03, 1e-3), HemisphereArray(33, 0.011, 0.03, 7.5e-4),
+              ShellArray(64, 0.02, 0.0512, 2e-4)]
+    for i, a in enumerate(arrays):
+        assert np.array_equal(np.array(a.idx), f[f"shell{i}_idx"]), i
+        assert np.array_equal(a.element_positions, f[f"shell{i}_pos"]), i
+        assert a.n == int(f[f"shell{i}_n"]) and a.array_type == str(f[f"shell{i}_type"])
+    fps = build_footprints(arrays[0], 0.004, "disc", height_m=0.003, max_pts=7)
+    assert np.array_equal(np.array([ix for idxs, _ in fps for ix in idxs]), f["shell_fp_cells"])
+    assert np.array_equal(np.array([len(idxs) for idxs, _ in fps]), f["shell_fp_counts"])
+    assert np.array_equal(np.concatenate([w for _, w in fps]), f["shell_fp_w"])
