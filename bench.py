@@ -42,7 +42,8 @@ WORKLOADS = {
     "C3": "2D ring FMC 1024x1024, dx=0.1mm, dt=25ns, nt=4000, 256-element ring R=44mm, "
           "256 shots, Ricker 500kHz, FP32 fields, checkpointed; one step = one misfit+gradient",
     "C4": "3D cylinder 256^3, dx=0.25mm, dt=30ns, nt=2000, 8 rings x 64 elements R=28mm, "
-          "64 shots, Ricker 500kHz, FP32 fields, checkpoint every 50 steps; one step = one "
+          "64 shots, Ricker 500kHz, FP32 fields, no stored forward levels (the adjoint sweep "
+          "runs the forward backwards; --reverse off: checkpoint every 50 steps); one step = one "
           "misfit+gradient",
     "C5": "serving mix: Poisson-arriving FWI-gradient and forward requests (50/50), grids 2D "
           "{256^2, 512^2, 1024^2} and 3D {128^3, 192^3, 256^3}, 16-256 shots, seeded phantoms, "
@@ -208,6 +209,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ckpt-every", type=int, default=0)
+    ap.add_argument("--reverse", default="auto", choices=["auto", "on", "off"],
+                    help="3D gradients: run the forward backwards alongside the adjoint (on), "
+                         "keep levels (off), or let the plan choose (auto)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the per-launch event timing pass (for counter runs)")
@@ -266,12 +270,15 @@ def main():
     wd = torch.tensor(prob.wavelet, device=dev)
     mt = torch.tensor(prob.m_true, device=dev)
     md = torch.tensor(prob.m0, device=dev)
-    ckpt = args.ckpt_every or prob.ckpt_every
+    reverse = {"auto": None, "on": True, "off": False}[args.reverse]
+    # a 3D plan left to itself chooses between checkpoints and running the
+    # forward backwards; an explicit --ckpt-every or --reverse off keeps levels
+    ckpt = args.ckpt_every or (0 if reverse is not False and len(shape) == 3 else prob.ckpt_every)
 
     print(f"[bench] {args.workload}: building batched plan ({len(tx)} shots, {shape}, nt={nt})",
           file=sys.stderr, flush=True)
     plan = Plan(shape, h, dt, nt, tx, rec, method="batched", ckpt_every=ckpt,
-                autotune=not args.no_autotune)
+                autotune=not args.no_autotune, reverse=reverse if len(shape) == 3 else None)
     print(f"[bench] plan {plan.info}", file=sys.stderr, flush=True)
     info = plan.info
     dobs = plan.forward_fmc(mt, wd)
@@ -345,10 +352,13 @@ def main():
           f"GB/s alg), adj {prof['adj_ms']:.3f} ms ({adj_gbs:.0f} GB/s alg)", file=sys.stderr,
           flush=True)
     suffix = "_x2" if lv == 2 else ""
+    rev = bool(info.get("reverse"))
     # minimum bytes of the adjoint kernel per launch: per shot-cell it reads
-    # lam^{k+1}, lam^{k+2} and one stored level per time level, writes lam per level
-    kmin_bytes = (4.0 * (2 + lv) + 4.0 * lv) * N * per_pass
-    adj_kernel = ("k_adj2d" if len(shape) == 2 else "k_adj3d") + suffix
+    # lam^{k+1}, lam^{k+2} and one stored level per time level, writes lam per level;
+    # the reversing kernel instead reads forward levels k-1 and k (FP32 hi + bf16 lo)
+    # and writes level k-2: 12 + 18 bytes
+    kmin_bytes = (30.0 if rev else 4.0 * (2 + lv) + 4.0 * lv) * N * per_pass
+    adj_kernel = ("k_adj2d" if len(shape) == 2 else ("k_radj3d" if rev else "k_adj3d")) + suffix
     fwd_kernel = ("k_fwd2d" if len(shape) == 2 else "k_fwd3d") + ("_x2" if lvf == 2 else "")
     if len(shape) == 3 and info.get("fwd_tile_kind") == 2:
         fwd_kernel = "k_fwd3d_x2_lds"
@@ -401,8 +411,10 @@ def main():
         cpu = cpu_baseline()
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "pmc_traffic_ring512_128shots.json")
-    if args.workload == "C2" and len(tx) == 128 and os.path.exists(tfile):
+    tfile = os.path.join(ROOT, "profiles", {"C2": "pmc_traffic_ring512_128shots.json",
+                                            "C4": "pmc_traffic_cylinder256_64shots.json"}.get(
+                                                args.workload, "none"))
+    if len(tx) == {"C2": 128, "C4": 64}.get(args.workload) and os.path.exists(tfile):
         # HBM bytes per launch of the adjoint kernel from the committed rocprofv3 --pmc
         # FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy of known size)
         with open(tfile) as fh:
@@ -435,8 +447,8 @@ def main():
         "misfit": Jv,
         "roofline": {
             "bound": "hbm",
-            "kernel": adj_kernel + f" (adjoint + imaging, {lv} time level(s) per launch, all "
-                                   "shots of a pass)",
+            "kernel": adj_kernel + f" (adjoint + imaging{', forward level re-created' if rev else ''}"
+                                   f", {lv} time level(s) per launch, all shots of a pass)",
             "achieved": round(adj_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -60,7 +60,11 @@ enum {
     USCT_FLAG_AUTOTUNE = 2,  /* time candidate tile shapes at creation, keep the best */
     USCT_FLAG_NO_GRAPH = 4,  /* batched: launch eagerly instead of replaying a hipGraph */
     USCT_FLAG_FORWARD_ONLY = 8, /* no gradient workspace (forward_fmc / misfit only) */
-    USCT_FLAG_NO_FUSE = 16      /* batched 2D: one time level per launch instead of two */
+    USCT_FLAG_NO_FUSE = 16,     /* batched 2D: one time level per launch instead of two */
+    USCT_FLAG_NO_REVERSE = 32,  /* batched 3D gradients: keep forward levels (whole history or
+                                   checkpoints) instead of running the forward backwards */
+    USCT_FLAG_REVERSE = 64      /* batched 3D gradients: run the forward backwards alongside the
+                                   adjoint even where the whole history would fit */
 };
 
 /* Misfit codes (software/python/uap.py:41). */
@@ -87,6 +91,8 @@ typedef struct {
                                  1 = LDS-staged work-group tile of tile_x x tile_y cells */
     int fwd_tile_kind;        /* the same for the forward kernel; 2 = LDS-staged tiles
                                  advancing two time levels per launch */
+    int reverse;              /* 1: the gradient keeps no forward levels; the adjoint sweep
+                                 runs the forward backwards (3D, no sponge, point elements) */
 } usct_plan_info;
 
 /*
@@ -94,7 +100,8 @@ typedef struct {
  *   dims      ndim entries, row-major (2D: ny, nx; 3D: nz, ny, nx)
  *   tx_lin    n_tx linear source cells (one shot each; uap.py:25-30)
  *   rec_lin   n_rec linear receiver cells (duplicates allowed and accumulate)
- *   ckpt_every  0 = choose from the memory budget
+ *   ckpt_every  0 = choose from the memory budget (3D: whole history, checkpoints or
+ *               running the forward backwards, whichever streams the fewest bytes)
  *   mem_budget  bytes of HBM the plan may hold; 0 = 85% of free memory
  */
 int usct_plan_create(usct_plan **plan, int ndim, const int *dims, double h, double dt,

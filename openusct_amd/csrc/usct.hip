@@ -114,6 +114,9 @@ struct usct_plan {
     int Bp = 1;        // padded shots per pass
     int passes = 1;
     int K = 0, NS = 1;  // checkpoint interval and number of windows
+    // 3D gradients without stored levels: the adjoint sweep runs the forward
+    // backwards alongside it (k_radj3d); Wslot then holds three adjoint levels
+    bool rev = false;
     int64_t budget = 0;
     // device memory
     std::vector<void *> allocs;
@@ -570,6 +573,46 @@ void launch_adj_batched(const usct_plan *P, hipStream_t s, const float *l1, cons
     }
 }
 
+// Adjoint level k with the forward run backwards (3D plans that keep no
+// levels): lam^{k+1}, lam^{k+2} -> lam^k; forward levels k-1 (stencil) and k
+// -> k-2, written when k >= 3.  Lane patches 2x2, 2x1, 1x2 or 1x1: the FP64
+// halo of the forward level makes larger patches spill.
+template <int LB, int CX, int CY>
+void radj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk, Lvl pc,
+              Lvl pk, Lvl po, int wr, const usct_plan *P, int k, const Tile &t) {
+    const double *sd = P->sincd + (int64_t)k * P->Bp;
+    const float *res = P->res + (int64_t)k * P->n_rec * P->Bp;
+    hipLaunchKernelGGL((k_radj3d<LB, CX, CY>), g, dim3(kBlock), 0, s, l1, l2, lk, pc.hi, pc.lo,
+                       pk.hi, pk.lo, po.hi, po.lo, wr, P->sp2, P->src_cell, sd, P->rec_head,
+                       P->rec_next, P->row_rec, res, P->g_hi, P->g_lo, P->nz, P->ny, P->nx, P->Bp,
+                       P->ta.RY, t.nxs, t.nys, t.nyc, P->ng_adj());
+}
+
+template <int LB>
+void radj3d_lb(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk, Lvl pc,
+               Lvl pk, Lvl po, int wr, const usct_plan *P, int k, const Tile &t) {
+    if (P->ta.CX == 2 && P->ta.CY == 2) radj3d_t<LB, 2, 2>(g, s, l1, l2, lk, pc, pk, po, wr, P, k, t);
+    else if (P->ta.CX == 2 && P->ta.CY == 1) radj3d_t<LB, 2, 1>(g, s, l1, l2, lk, pc, pk, po, wr, P, k, t);
+    else if (P->ta.CX == 1 && P->ta.CY == 2) radj3d_t<LB, 1, 2>(g, s, l1, l2, lk, pc, pk, po, wr, P, k, t);
+    else radj3d_t<LB, 1, 1>(g, s, l1, l2, lk, pc, pk, po, wr, P, k, t);
+}
+
+bool radj_patch(int cx, int cy) {
+    return (cx == 2 && cy == 2) || (cx == 2 && cy == 1) || (cx == 1 && cy == 2) ||
+           (cx == 1 && cy == 1);
+}
+
+void launch_radj_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
+                         float *lk, Lvl pc, Lvl pk, Lvl po, bool wr, int k) {
+    const Tile t = tile_of(P, P->ta);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+    switch (P->ta.LB) {
+        case 16: radj3d_lb<16>(g, s, l1, l2, lk, pc, pk, po, wr ? 1 : 0, P, k, t); break;
+        case 32: radj3d_lb<32>(g, s, l1, l2, lk, pc, pk, po, wr ? 1 : 0, P, k, t); break;
+        default: radj3d_lb<64>(g, s, l1, l2, lk, pc, pk, po, wr ? 1 : 0, P, k, t); break;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Level pointers
 // ---------------------------------------------------------------------------
@@ -589,7 +632,7 @@ int ck_slot(const usct_plan *P, int n) {
 // levels, whose low parts are kept.
 Lvl fwd_level(const usct_plan *P, int n, bool store) {
     if (n <= 0) return {P->Z, P->Zlo};
-    if (store) {
+    if (store && !P->rev) {
         const int c = ck_slot(P, n);
         if (c >= 0) return {P->CK[c], P->CKlo[c]};
         const int base_last = (P->NS - 1) * P->K;
@@ -609,6 +652,9 @@ Lvl win_level(const usct_plan *P, int s, int level) {
 }
 
 float *lam_level(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->F[k % 4]; }
+// Adjoint levels of a reversing plan: three slots of their own (the forward
+// rotation F holds the levels being run backwards).
+float *lam_rev(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->Wslot[k % 3]; }
 
 void ev_start(Profile *pr, std::vector<hipEvent_t> &v, hipStream_t s) {
     if (!pr) return;
@@ -816,6 +862,21 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
     if (mode == MODE_FWD) return;
     if (!born && mode != MODE_ADJSRC) launch_residual(P, s, misfit, store, windowed);
     if (!store) return;
+    if (P->rev) {
+        // the forward's last two levels are in the rotation; each launch takes
+        // them one level back while the adjoint advances
+        for (int k = nt - 1; k >= 1; --k) {
+            if (pr) ev_start(pr, pr->adj_ev, s);
+            launch_radj_batched(P, s, lam_rev(P, k + 1), lam_rev(P, k + 2), lam_rev(P, k),
+                                fwd_level(P, k - 1, false), fwd_level(P, k, false),
+                                fwd_level(P, k - 2, false), k >= 3, k);
+            if (pr) {
+                ev_stop(pr, pr->adj_ev, s);
+                pr->adj_lv.push_back(1);
+            }
+        }
+        return;
+    }
     for (int w = P->NS - 1; w >= 0; --w) {
         const int base = w * P->K;
         const int khi = std::min(base + P->K, nt - 1);
@@ -1088,15 +1149,29 @@ int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
     return b;
 }
 
+// A 3D gradient plan may run the forward backwards instead of keeping levels
+// (no sponge in 3D, point elements only: footprint plans keep levels).
+bool reversible(const usct_plan *P) {
+    return P->ndim == 3 && !P->sponge && P->method == USCT_METHOD_BATCHED &&
+           !(P->flags & (USCT_FLAG_FORWARD_ONLY | USCT_FLAG_NO_REVERSE));
+}
+
 int choose_batched(usct_plan *P, int ckpt_every) {
     const int B = P->n_tx;
     const int nt = P->nt;
     // HBM bytes streamed per shot-cell and time level (forward, adjoint,
     // re-computed forward): the two-level 2D kernels move about 13-14 bytes per
-    // level, the single-level ones 18-20.
+    // level, the single-level ones 18-20; the reversing adjoint takes 1.7x the
+    // forward's time (7.4 against 4.3 ms per level on 256^3 with 64 shots).
     const bool x2 = P->ndim == 2 && !P->sponge && !(P->flags & USCT_FLAG_NO_FUSE);
-    const double b_fwd = x2 ? 13.0 : 19.0, b_adj = x2 ? 14.0 : 17.0;
+    const double b_fwd = x2 ? 13.0 : 19.0, b_adj = x2 ? 14.0 : 17.0, b_radj = 33.0;
+    const bool force_rev = (P->flags & USCT_FLAG_REVERSE) != 0;
+    if (force_rev && !reversible(P))
+        return fail(USCT_ENOTSUP, "USCT_FLAG_REVERSE needs a batched 3D gradient plan without "
+                                  "footprints or USCT_FLAG_NO_REVERSE");
+    const bool try_rev = reversible(P) && (force_rev || ckpt_every <= 0);
     MemPlan best{0, 0, 0, 0, 0, 1e300};
+    bool best_rev = false;
     int best_lb = 64;
     // Every lane-group width, every number of groups per pass (padded shots
     // count in the cost), whole history or checkpoints: the cheapest plan that
@@ -1115,21 +1190,35 @@ int choose_batched(usct_plan *P, int ckpt_every) {
                 int kmin = std::max(2, (int)std::lround(std::sqrt(2.0 * std::max(nt - 1, 1))));
                 Ks.push_back(kmin);
             }
-            for (int K : Ks) {
-                const int NS = (int)std::max<int64_t>(1, cdiv(nt - 1, K));
-                const int64_t bytes =
-                    batched_bytes(P, Bp, K, NS, !(P->flags & USCT_FLAG_FORWARD_ONLY));
-                if (bytes > P->budget) continue;
-                const double per = b_fwd + b_adj + (NS > 1 ? b_fwd * (1.0 - 1.0 / NS) : 0.0);
-                // narrower lane groups stream a little less efficiently
-                const double eff = lb == 64 ? 1.0 : (lb == 32 ? 0.95 : 0.88);
-                // launch overhead of ~3 us per step launch, in streamed-byte units at 5 TB/s
-                const double launches = (double)passes * nt * (NS > 1 ? 3.0 : 2.0);
-                const double cost = (double)passes * Bp * per * (double)P->N * nt / eff +
-                                    launches * 15e6;
-                if (cost < best.cost) {
-                    best = MemPlan{Bp, passes, K, NS, bytes, cost};
-                    best_lb = lb;
+            // narrower lane groups stream a little less efficiently
+            const double eff = lb == 64 ? 1.0 : (lb == 32 ? 0.95 : 0.88);
+            if (!force_rev)
+                for (int K : Ks) {
+                    const int NS = (int)std::max<int64_t>(1, cdiv(nt - 1, K));
+                    const int64_t bytes =
+                        batched_bytes(P, Bp, K, NS, !(P->flags & USCT_FLAG_FORWARD_ONLY));
+                    if (bytes > P->budget) continue;
+                    const double per = b_fwd + b_adj + (NS > 1 ? b_fwd * (1.0 - 1.0 / NS) : 0.0);
+                    // launch overhead of ~3 us per step launch, in streamed-byte units at 5 TB/s
+                    const double launches = (double)passes * nt * (NS > 1 ? 3.0 : 2.0);
+                    const double cost = (double)passes * Bp * per * (double)P->N * nt / eff +
+                                        launches * 15e6;
+                    if (cost < best.cost) {
+                        best = MemPlan{Bp, passes, K, NS, bytes, cost};
+                        best_rev = false;
+                        best_lb = lb;
+                    }
+                }
+            if (try_rev) {  // three forward and three adjoint levels, nothing kept
+                const int64_t bytes = batched_bytes(P, Bp, 1, 1, true);
+                if (bytes <= P->budget) {
+                    const double cost = (double)passes * Bp * (b_fwd + b_radj) * (double)P->N * nt /
+                                            eff + (double)passes * nt * 2.0 * 15e6;
+                    if (cost < best.cost) {
+                        best = MemPlan{Bp, passes, 1, 1, bytes, cost};
+                        best_rev = true;
+                        best_lb = lb;
+                    }
                 }
             }
         }
@@ -1142,6 +1231,9 @@ int choose_batched(usct_plan *P, int ckpt_every) {
     P->passes = best.passes;
     P->K = best.K;
     P->NS = best.NS;
+    P->rev = best_rev;
+    // the reversing adjoint has its own (smaller) set of lane patches
+    if (P->rev && !radj_patch(P->ta.CX, P->ta.CY)) P->ta.CX = P->ta.CY = 2;
     return 0;
 }
 
@@ -1262,6 +1354,9 @@ double time_tile(usct_plan *P, bool adjoint) {
         if (!adjoint) {
             if (x2) launch_fwd2_batched(P, P->stream, L[1], L[0], L[2], L[3], 1, false);
             else launch_fwd_batched(P, P->stream, L[1], L[0], L[2], 1, false);
+        } else if (P->rev) {
+            launch_radj_batched(P, P->stream, P->Wslot[r % 3], P->Wslot[(r + 1) % 3],
+                                P->Wslot[(r + 2) % 3], L[0], L[1], L[2], true, 1);
         } else if (x2) {
             launch_adj2_batched(P, P->stream, L[0].hi, L[1].hi, L[2].hi, L[3].hi, P->Wslot[0],
                                 P->Wslot[0], 2);
@@ -1288,16 +1383,17 @@ double time_tile(usct_plan *P, bool adjoint) {
 // separately.  Results are kept per (ndim, dims, padded shots, sponge) for the
 // life of the process.
 struct TuneKey {
-    int ndim, nz, ny, nx, Bp, sponge;
+    int ndim, nz, ny, nx, Bp, sponge, rev;
     bool operator<(const TuneKey &o) const {
-        return std::tie(ndim, nz, ny, nx, Bp, sponge) < std::tie(o.ndim, o.nz, o.ny, o.nx, o.Bp, o.sponge);
+        return std::tie(ndim, nz, ny, nx, Bp, sponge, rev) <
+               std::tie(o.ndim, o.nz, o.ny, o.nx, o.Bp, o.sponge, o.rev);
     }
 };
 struct TuneVal { Tiling tf, ta; };
 std::map<TuneKey, TuneVal> g_tuned;
 
 int autotune(usct_plan *P) {
-    const TuneKey key{P->ndim, P->nz, P->ny, P->nx, P->Bp, P->sponge ? 1 : 0};
+    const TuneKey key{P->ndim, P->nz, P->ny, P->nx, P->Bp, P->sponge ? 1 : 0, P->rev ? 1 : 0};
     auto hit = g_tuned.find(key);
     if (hit != g_tuned.end()) {
         P->tf = hit->second.tf;
@@ -1309,7 +1405,8 @@ int autotune(usct_plan *P) {
         HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
         HIPX(hipMemset(P->Lo[i], 0, sizeof(uint16_t) * P->field_elems()));
     }
-    if (!P->Wslot.empty()) HIPX(hipMemset(P->Wslot[0], 0, sizeof(float) * P->field_elems()));
+    for (size_t i = 0; i < std::min<size_t>(P->Wslot.size(), 3); ++i)
+        HIPX(hipMemset(P->Wslot[i], 0, sizeof(float) * P->field_elems()));
     HIPX(hipMemset(P->sinc, 0, sizeof(float) * P->nt * P->Bp));
     HIPX(hipMemset(P->sincd, 0, sizeof(double) * P->nt * P->Bp));
     HIPX(hipMemset(P->sp2, 0, sizeof(float2) * P->N));
@@ -1328,6 +1425,10 @@ int autotune(usct_plan *P) {
             for (int lb : lbs)
                 for (int cx : {4, 8})
                     for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry, 0});
+        } else if (adjoint && P->rev) {
+            for (int lb : lbs)
+                for (auto pc : {std::pair<int, int>{2, 2}, {2, 1}, {1, 2}, {1, 1}})
+                    for (int rz : {4, 8, 16, 32, 64}) cands.push_back({lb, pc.first, pc.second, rz, 0});
         } else {
             for (int lb : lbs)
                 for (auto pc : {std::pair<int, int>{4, 4}, {4, 2}, {8, 2}, {2, 2}, {2, 4}, {2, 1},
@@ -1749,9 +1850,12 @@ int usct_plan_create_groups(usct_plan **out, int ndim, const int *dims, double h
     std::vector<int64_t> tx_first(std::max(n_tx, 1)), rec_first(n_rec);
     for (int t = 0; t < n_tx; ++t) tx_first[t] = tx_cells[tx_off[t]];
     for (int j = 0; j < n_rec; ++j) rec_first[j] = rec_cells[rec_off[j]];
+    if (flags & USCT_FLAG_REVERSE)
+        return fail(USCT_ENOTSUP, "footprint plans keep forward levels (USCT_FLAG_REVERSE)");
     usct_plan *P = nullptr;
+    // the footprint corrections read stored forward levels
     TRY(usct_plan_create(&P, ndim, dims, h, dt, nt, tx_first.data(), n_tx, rec_first.data(), n_rec,
-                         method, ckpt_every, mem_budget, flags));
+                         method, ckpt_every, mem_budget, flags | USCT_FLAG_NO_REVERSE));
     const int rc = attach_groups(P, tx_off, tx_cells, tx_w, rec_off, rec_cells, rec_w);
     if (rc) {
         usct_plan_destroy(P);
@@ -1788,9 +1892,10 @@ int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
     info->shots_padded = P->method == USCT_METHOD_PLAIN ? P->n_tx : P->Bp * P->passes;
     info->shot_passes = P->passes;
     info->shots_per_pass = P->Bp;
-    info->ckpt_every = P->K;
-    info->n_windows = P->NS;
+    info->ckpt_every = P->rev ? 0 : P->K;
+    info->n_windows = P->rev ? 0 : P->NS;
     info->workspace_bytes = P->bytes;
+    info->reverse = P->rev ? 1 : 0;
     return 0;
 }
 

@@ -2238,6 +2238,225 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 }
 
 // ---------------------------------------------------------------------------
+// Batched 3D adjoint level k with the forward run backwards alongside it.
+// Without a sponge the leapfrog recursion is its own inverse:
+//   p^{k-2} = 2 p^{k-1} - p^k + inc,  inc = S'(L p^{k-1}) [+ S_src w[k-1]],
+// the forward's FP64 update with the roles of p^n and p^{n-2} exchanged, and
+// inc is exactly the second difference p^k - 2p^{k-1} + p^{k-2} the imaging
+// condition multiplies by lam^k (fwi.py:323-326).  One launch therefore
+// advances lam (the k_adj3d update, bit for bit), re-creates the forward level
+// k-2 (hi + lo, FP64 from the full state, as k_fwd3d) and images with inc, so
+// the gradient needs neither stored levels nor a re-computed forward sweep.
+//   pc/pcl  level k-1 (stencil), pk/pkl level k (own cells),
+//   po/pol  level k-2 (written when wr != 0; the caller passes wr = 0 for
+//           k <= 2, whose levels 0 and -1 are zero by definition)
+//   sincd   row k of the FP64 source increments.  Addressing as k_fwd3d.
+// ---------------------------------------------------------------------------
+template <int LB, int CX, int CY>
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3D
+k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
+         const float *__restrict__ pc, const uint16_t *__restrict__ pcl,
+         const float *__restrict__ pk, const uint16_t *__restrict__ pkl, float *__restrict__ po,
+         uint16_t *__restrict__ pol, int wr, const float2 *__restrict__ sp2,
+         const int *__restrict__ src_cell, const double *__restrict__ sincd,
+         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+         const int *__restrict__ row_rec, const float *__restrict__ res, float *__restrict__ g_hi,
+         float *__restrict__ g_lo, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
+         int nzc, int ng) {
+    constexpr int W = kWave / LB;
+    constexpr int CC = CX * CY;
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    int xs, ys, zc;
+    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;  // uniform over the wavefront
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = ys * CY;
+    const int z0 = zc * RZ;
+    const int z1 = min(z0 + RZ, nz);
+    const size_t rowE = (size_t)nx * Bp;
+    const size_t planeE = rowE * ny;
+    const size_t planeC = (size_t)nx * ny;
+    const size_t N = planeC * nz;
+    const int my_src = src_cell[b];
+    const double my_inc = sincd[b];
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    auto row = [&](int j) -> size_t { return (size_t)min(max(y0 + j, 0), ny - 1); };
+    auto F = [&](const float *plane, int j, int i) -> float {
+        return ldb(plane + row(j) * rowE + (ptrdiff_t)i * Bp, o4);
+    };
+    auto Q = [&](const float *plane, const float2 *splane, int j, int i) -> float {
+        return smul(ldb(splane + row(j) * nx + i, oc), F(plane, j, i));
+    };
+    // full state hi + lo of level k-1 at (row j, column i) of a plane, in FP64
+    auto P = [&](size_t plane, int j, int i) -> double {
+        const size_t e = plane + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+        return (double)ldb(pc + e, o4) + (double)bf2f(ldb(pcl + e, o2));
+    };
+    float qm[CY][CX], qc[CY + 2][CX + 2], qp[CY][CX], lc[CY][CX], lpn[CY][CX];
+    double zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
+    auto load_halo = [&](const float *lz, const float2 *sz, size_t pl) {
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            qc[0][i + 1] = Q(lz, sz, -1, i);
+            qc[CY + 1][i + 1] = Q(lz, sz, CY, i);
+            zcu[0][i + 1] = P(pl, -1, i);
+            zcu[CY + 1][i + 1] = P(pl, CY, i);
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            qc[j + 1][0] = Q(lz, sz, j, -1);
+            qc[j + 1][CX + 1] = Q(lz, sz, j, CX);
+            zcu[j + 1][0] = P(pl, j, -1);
+            zcu[j + 1][CX + 1] = P(pl, j, CX);
+        }
+    };
+    // raw loads issued one plane ahead: own cells of lam^{k+1} and of level
+    // k-1 on plane z+1 (an, nh/nl), of lam^{k+2} and of level k on plane z
+    // (l2n, kh/kl)
+    float an[CY][CX], nh[CY][CX], l2n[CY][CX], kh[CY][CX];
+    uint16_t nl[CY][CX], kl[CY][CX];
+    auto load_raw = [&](size_t pl, size_t pl2) {
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const size_t e = pl + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+                const size_t e2 = pl2 + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+                an[j][i] = ldb(l1 + e, o4);
+                nh[j][i] = ldb(pc + e, o4);
+                nl[j][i] = ldb(pcl + e, o2);
+                l2n[j][i] = ldn<USCT_NT3D>(l2 + e2, o4);
+                kh[j][i] = ldn<USCT_NT3D>(pk + e2, o4);
+                kl[j][i] = ldn<USCT_NT3D>(pkl + e2, o2);
+            }
+    };
+    {
+        const int zmi = max(z0 - 1, 0);
+        const float *lm = l1 + (size_t)zmi * planeE, *l0 = l1 + (size_t)z0 * planeE;
+        const float2 *sm = sp2 + (size_t)zmi * planeC, *s0 = sp2 + (size_t)z0 * planeC;
+        const size_t pm = (size_t)zmi * planeE, p0 = (size_t)z0 * planeE;
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                qm[j][i] = Q(lm, sm, j, i);
+                zm[j][i] = P(pm, j, i);
+                const float v = F(l0, j, i);
+                lc[j][i] = v;
+                qc[j + 1][i + 1] = smul(ldb(s0 + row(j) * nx + i, oc), v);
+                zcu[j + 1][i + 1] = P(p0, j, i);
+            }
+        load_halo(l0, s0, p0);
+        load_raw((size_t)min(z0 + 1, nz - 1) * planeE, p0);
+    }
+    // the lane's reduced cell (the same on every plane) and its gradient
+    // accumulators, loaded one plane ahead so the update does not wait on them
+    const bool gown = (sb & (LB / CC - 1)) == 0;
+    const int gci = sum_cell<LB, CC>(lane);
+    const int gx = x0 + gci % CX, gy = y0 + gci / CX;
+    const bool gval = gown && gx < nx && gy < ny;
+    const size_t gbase = (size_t)sg * N + (size_t)gy * nx + gx;
+    float ghn = 0.0f, gln = 0.0f;
+    if (gval) {
+        ghn = g_hi[gbase + (size_t)z0 * planeC];
+        gln = g_lo[gbase + (size_t)z0 * planeC];
+    }
+    for (int z = z0; z < z1; ++z) {
+        const int zpi = min(z + 1, nz - 1);
+        const float *lnz = l1 + (size_t)zpi * planeE;
+        const float2 *snz = sp2 + (size_t)zpi * planeC;
+        const size_t pnz = (size_t)zpi * planeE;
+        float l2c[CY][CX];
+        double q[CY][CX];
+        const float gh0 = ghn, gl0 = gln;
+        if (gval && z + 1 < z1) {
+            ghn = g_hi[gbase + (size_t)(z + 1) * planeC];
+            gln = g_lo[gbase + (size_t)(z + 1) * planeC];
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const float v = an[j][i];
+                lpn[j][i] = v;
+                qp[j][i] = smul(ldb(snz + row(j) * nx + i, oc), v);
+                zp[j][i] = hl(nh[j][i], nl[j][i]);
+                l2c[j][i] = l2n[j][i];
+                q[j][i] = hl(kh[j][i], kl[j][i]);
+            }
+        if (z + 1 < z1) load_raw((size_t)min(z + 2, nz - 1) * planeE, pnz);
+        const bool zint = (z >= 1) && (z <= nz - 2);
+        float tv[CC];
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            const int y = y0 + j;
+            const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const bool has_rec = y < ny && row_rec[z * ny + y] > 0;
+            const size_t ro = (size_t)z * planeE + (size_t)y * rowE;
+            const float2 *spr = sp2 + (size_t)z * planeC + (size_t)y * nx;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                tv[j * CX + i] = 0.0f;
+                if (x < nx && y < ny) {
+                    const bool in = yint && x >= 1 && x <= nx - 2;
+                    const float lq = in ? lap7(qc[j + 1][i + 1], qp[j][i], qm[j][i], qc[j + 2][i + 1],
+                                               qc[j][i + 1], qc[j + 1][i + 2], qc[j + 1][i])
+                                        : 0.0f;
+                    const size_t ei = ro + (size_t)i * Bp;
+                    float v = leap(lc[j][i], l2c[j][i], lq);
+                    const int c = (z * ny + y) * nx + x;
+                    if (has_rec) {
+                        for (int r = rec_head[c]; r >= 0; r = rec_next[r])
+                            v = v - res[(size_t)r * Bp + b];
+                    }
+                    stn<USCT_NT3D>(lk + ei, o4, v);
+                    double lu = 0.0;
+                    if (in)
+                        lu = lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
+                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
+                    double inc = s2d(ldb(spr + i, oc)) * lu;
+                    if (c == my_src) inc = inc + my_inc;
+                    if (wr) {
+                        const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
+                        uint16_t lb;
+                        const float pv = split_hl(nv, lb);
+                        stn<USCT_NT3D>(po + ei, o4, pv);
+                        stn<USCT_NT3D>(pol + ei, o2, lb);
+                    }
+                    tv[j * CX + i] = v * (float)inc;
+                }
+            }
+        }
+        const float s = reduce_scatter<LB, CC>(tv, lane);
+        if (gval) {
+            float hi = gh0, lo = gl0;
+            neumaier(hi, lo, s);
+            g_hi[gbase + (size_t)z * planeC] = hi;
+            g_lo[gbase + (size_t)z * planeC] = lo;
+        }
+        if (z + 1 < z1) {
+#pragma unroll
+            for (int j = 0; j < CY; ++j)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    qm[j][i] = qc[j + 1][i + 1];
+                    qc[j + 1][i + 1] = qp[j][i];
+                    lc[j][i] = lpn[j][i];
+                    zm[j][i] = zcu[j + 1][i + 1];
+                    zcu[j + 1][i + 1] = zp[j][i];
+                }
+            load_halo(lnz, snz, pnz);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Plain (one shot) kernels: one thread per cell, dimension-general (2D: nz = 1).
 // ---------------------------------------------------------------------------
 template <bool SPONGE, bool REC>

@@ -487,6 +487,128 @@ bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     return ok;
 }
 
+// 3D adjoint with the forward run backwards (k_radj3d) against direct loops:
+// lam^k bit for bit (the k_adj3d update), level k-2 hi + lo bit for bit (the
+// FP64 leapfrog with p^k in the place of p^{n-2}), the imaging sums of
+// lam^k * (float)inc against a direct FP64 sum; with wr = 0 level k-2 is left
+// as it was.
+template <int LB, int CX, int CY>
+bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed) {
+    const int Bp = LB * ng, W = 64 / LB;
+    const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
+    std::mt19937 rng(seed);
+    char tag[160];
+    std::snprintf(tag, sizeof tag, "radj3d nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d", nz,
+                  ny, nx, LB, ng, CX, CY, RZ, wr);
+    std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
+        dl(-1e-7f, 1e-7f);
+    float2 *sp2 = alloc<float2>(N, 1);
+    for (int64_t c = 0; c < N; ++c) sp2[c] = make_float2(u(rng), tt(rng));
+    float *l1 = alloc<float>(FE, Bp), *l2 = alloc<float>(FE, Bp), *lk = alloc<float>(FE, Bp);
+    float *pc = alloc<float>(FE, Bp), *pk = alloc<float>(FE, Bp), *po = alloc<float>(FE, Bp);
+    uint16_t *pcl = alloc<uint16_t>(FE, Bp), *pkl = alloc<uint16_t>(FE, Bp),
+             *pol = alloc<uint16_t>(FE, Bp);
+    for (int64_t i = 0; i < FE; ++i) {
+        l1[i] = d(rng);
+        l2[i] = d(rng);
+        pc[i] = d(rng);
+        pk[i] = d(rng);
+        pcl[i] = f2bf(dl(rng));
+        pkl[i] = f2bf(dl(rng));
+        po[i] = 7.0f;
+        pol[i] = 0x1234;
+    }
+    int *src = alloc<int>(Bp, 0);
+    std::uniform_int_distribution<int> cz(0, nz - 1), cy(0, ny - 1), cx(0, nx - 1);
+    for (int b = 0; b < Bp; ++b) src[b] = (cz(rng) * ny + cy(rng)) * nx + (b % 3 == 0 ? 0 : cx(rng));
+    src[Bp - 1] = -1;
+    double *sincd = alloc<double>(Bp, 0);
+    for (int b = 0; b < Bp; ++b) sincd[b] = (double)d(rng) * 1.2345678901;
+    std::vector<int> rc = {0, (int)N - 1, (int)(N / 2), (int)(N / 3), (int)(N / 3)};
+    const int n_rec = (int)rc.size();
+    int *rec_head = alloc<int>(N, 0), *rec_next = alloc<int>(n_rec, 0);
+    int *row_rec = alloc<int>((int64_t)nz * ny, 0);
+    for (int64_t c = 0; c < N; ++c) rec_head[c] = -1;
+    for (int j = n_rec - 1; j >= 0; --j) {
+        rec_next[j] = rec_head[rc[j]];
+        rec_head[rc[j]] = j;
+    }
+    for (int j = 0; j < n_rec; ++j) row_rec[rc[j] / nx]++;
+    float *res = alloc<float>((int64_t)n_rec * Bp, 0);
+    for (int64_t i = 0; i < (int64_t)n_rec * Bp; ++i) res[i] = d(rng);
+    float *gh = alloc<float>(ng * N, 0), *gl = alloc<float>(ng * N, 0);
+    const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
+    const int nzc = (int)cdiv(nz, RZ);
+    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+    emu::launch(blocks, kBlock, [&] {
+        k_radj3d<LB, CX, CY>(l1, l2, lk, pc, pcl, pk, pkl, po, pol, wr, sp2, src, sincd, rec_head,
+                             rec_next, row_rec, res, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
+    });
+    bool ok = true;
+    const int64_t sz = (int64_t)nx * ny;
+    std::vector<double> gref(ng * N, 0.0);
+    auto Pv = [&](int64_t c, int b) { return (double)pc[c * Bp + b] + (double)bf2f(pcl[c * Bp + b]); };
+    for (int64_t c = 0; c < N && ok; ++c) {
+        const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / sz);
+        const bool in = z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2;
+        for (int b = 0; b < Bp; ++b) {
+            auto q = [&](int64_t e) { return smul(sp2[e], l1[e * Bp + b]); };
+            const float lq = in ? lap7(q(c), q(c + sz), q(c - sz), q(c + nx), q(c - nx), q(c + 1),
+                                       q(c - 1))
+                                : 0.0f;
+            float v = leap(l1[c * Bp + b], l2[c * Bp + b], lq);
+            for (int r = rec_head[c]; r >= 0; r = rec_next[r]) v = v - res[(int64_t)r * Bp + b];
+            if (std::memcmp(&v, &lk[c * Bp + b], 4) != 0) {
+                std::printf("MISMATCH %s lam cell %lld shot %d\n", tag, (long long)c, b);
+                ok = false;
+                break;
+            }
+            double lu = 0.0;
+            if (in)
+                lu = lap7d(Pv(c, b), Pv(c + sz, b), Pv(c - sz, b), Pv(c + nx, b), Pv(c - nx, b),
+                           Pv(c + 1, b), Pv(c - 1, b));
+            double inc = ((double)sp2[c].x + (double)sp2[c].y) * lu;
+            if (c == src[b]) inc = inc + sincd[b];
+            const double qk = (double)pk[c * Bp + b] + (double)bf2f(pkl[c * Bp + b]);
+            const double nv = (2.0 * Pv(c, b) - qk) + inc;
+            float pv = 7.0f;
+            uint16_t lb = 0x1234;
+            if (wr) {
+                pv = (float)nv;
+                lb = f2bf((float)(nv - (double)pv));
+            }
+            if (std::memcmp(&pv, &po[c * Bp + b], 4) != 0 || lb != pol[c * Bp + b]) {
+                std::printf("MISMATCH %s level k-2 cell %lld shot %d\n", tag, (long long)c, b);
+                ok = false;
+                break;
+            }
+            gref[(b / LB) * N + c] += (double)(v * (float)inc);
+        }
+    }
+    double num = 0, den = 0;
+    for (int64_t i = 0; i < ng * N; ++i) {
+        const double e = ((double)gh[i] + (double)gl[i]) - gref[i];
+        num += e * e;
+        den += gref[i] * gref[i];
+    }
+    if (ok && !(num <= 1e-12 * den)) {
+        std::printf("MISMATCH %s imaging sums rel err %.3e\n", tag, std::sqrt(num / den));
+        ok = false;
+    }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+template <int LB, int CX, int CY>
+bool radj3d_cases(uint32_t seed) {
+    bool ok = true;
+    ok &= check_radj3d<LB, CX, CY>(9, 7, 11, 2, 4, 1, seed++);
+    ok &= check_radj3d<LB, CX, CY>(5, 6, 13, 1, 64, 1, seed++);
+    ok &= check_radj3d<LB, CX, CY>(12, 9, 5, 2, 8, 0, seed++);
+    ok &= check_radj3d<LB, CX, CY>(3, 3, 3, 1, 2, 1, seed++);
+    return ok;
+}
+
 template <int LB, int CX, int CY>
 bool fwd3d_cases(uint32_t seed) {
     bool ok = true;
@@ -992,6 +1114,15 @@ int main(int argc, char **argv) {
         ok &= fwd3d_cases<16, 2, 1>(seed + 1700);
         ok &= fwd3d_cases<16, 1, 2>(seed + 1800);
         ok &= fwd3d_cases<16, 1, 1>(seed + 1900);
+    }
+    if (only.empty() || only == "radj3d") {
+        ok &= radj3d_cases<16, 2, 2>(seed + 7000);
+        ok &= radj3d_cases<16, 2, 1>(seed + 7100);
+        ok &= radj3d_cases<16, 1, 2>(seed + 7200);
+        ok &= radj3d_cases<16, 1, 1>(seed + 7300);
+        ok &= radj3d_cases<32, 2, 2>(seed + 7400);
+        ok &= radj3d_cases<64, 2, 2>(seed + 7500);
+        ok &= radj3d_cases<64, 1, 1>(seed + 7600);
     }
     std::printf(ok ? "ALL EQUAL\n" : "DIFFERENCES FOUND\n");
     return ok ? 0 : 1;
