@@ -1,0 +1,45 @@
+"""Figure helpers shared by the demonstrations (matplotlib optional)."""
+
+from __future__ import annotations
+
+import os
+
+
+def pyplot():
+    """matplotlib.pyplot with the Agg backend, or None when it is absent."""
+    try:
+        import matplotlib
+    except ImportError:
+        return None
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    return plt
+
+
+def save(fig, out_dir, name):
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, name)
+    fig.savefig(path, dpi=130)
+    return path
+
+
+def velocity_panels(plt, panels, extent, vmin, vmax, history, marks=None, title_hist="Misfit convergence"):
+    """One row of sound-speed images and the normalised misfit curve."""
+    import numpy as np
+    fig, ax = plt.subplots(1, len(panels) + 1, figsize=(4.2 * (len(panels) + 1), 4.2))
+    for a, (field, title) in zip(ax, panels):
+        im = a.imshow(field, origin="lower", extent=extent, cmap="viridis", vmin=vmin, vmax=vmax)
+        if marks is not None:
+            a.plot(marks[:, 0] * 1e3, marks[:, 1] * 1e3, "w.", ms=5)
+        a.set_title(title)
+        a.set_xlabel("x (mm)")
+        a.set_ylabel("y (mm)")
+        fig.colorbar(im, ax=a, fraction=0.046, label="c (m/s)")
+    h = np.asarray(history, dtype=float)
+    ax[-1].semilogy(h / h[0], "o-")
+    ax[-1].set_title(title_hist)
+    ax[-1].set_xlabel("iteration")
+    ax[-1].set_ylabel("normalised misfit")
+    ax[-1].grid(True, which="both", alpha=0.3)
+    fig.tight_layout()
+    return fig
