@@ -29,7 +29,11 @@ outputs of the functions whose behaviour only the reference itself pins:
   fmc_2d.npz problem, and the envelope misfit with the footprints of
   aperture_2d.npz (misfits_2d.npz);
 * the other wavelets, phantoms and array geometries of the reference
-  (builders.npz).
+  (builders.npz);
+* short optimiser trajectories of invert (fwi.py:535-667): steepest descent
+  with the masked, Gaussian-smoothed (scipy) gradient, bounds and the halving
+  line search, and truncated Gauss-Newton, in 2D (with and without a sponge)
+  and 3D (invert.npz).
 
 The fixtures are data only (numpy .npz, no pickles).
 """
@@ -413,8 +417,65 @@ def main_builders(ref_root):
     print("builder fixtures written to", HERE)
 
 
+def main_invert(ref_root):
+    """Trajectories of the reference's invert (fwi.py:535-667) on the problems
+    of fmc_2d.npz and fmc_3d.npz: iterates and misfit histories."""
+    sys.path.insert(0, os.path.join(ref_root, "simulation"))
+    from ringfwi import fwi, phantom  # noqa: E402
+    from ringfwi.solver import make_sponge  # noqa: E402
+
+    class Geom:  # element indices of a fixture, as the reference reads a geometry
+        def __init__(self, idx):
+            self.idx = [tuple(int(v) for v in r) for r in idx]
+            self.n_elements = len(self.idx)
+
+        def element_index(self, k):
+            return self.idx[k]
+
+    out = {}
+    f = np.load(os.path.join(HERE, "fmc_2d.npz"))
+    geom = Geom(f["idx"])
+    h, dt, nt, w = float(f["h"]), float(f["dt"]), int(f["nt"]), f["wavelet"]
+    m0, src = f["m0"], [int(s) for s in f["src_list"]]
+    n = m0.shape[0]
+    yy, xx = np.mgrid[0:n, 0:n]
+    mask = (np.hypot(xx - (n - 1) / 2.0, yy - (n - 1) / 2.0) <= 0.4 * (n - 1)).astype(float)
+    bounds = (phantom.velocity_to_m(3600.0), phantom.velocity_to_m(2500.0))
+    sp = make_sponge(m0.shape, width=8)
+    cases = {
+        "gd": dict(dobs=f["nosponge_dobs"], sponge=None, kw=dict(n_iter=3, step_frac=0.02)),
+        "gd_mask": dict(dobs=f["nosponge_dobs"], sponge=None,
+                        kw=dict(n_iter=3, step_frac=0.03, update_mask=mask, m_bounds=bounds)),
+        "gd_gcn": dict(dobs=f["nosponge_dobs"], sponge=None,
+                       kw=dict(n_iter=3, step_frac=0.02, misfit_type="gcn")),
+        "gn": dict(dobs=f["nosponge_dobs"], sponge=None,
+                   kw=dict(n_iter=2, optimizer="gn", n_cg=4, update_mask=mask, m_bounds=bounds)),
+    }
+    for name, c in cases.items():
+        m, hist = fwi.invert(m0, geom, w, dt, h, nt, c["dobs"], c["sponge"], src, **c["kw"])
+        out[f"{name}_m"], out[f"{name}_history"] = m, np.array(hist)
+        print(name, len(hist), hist[-1] / hist[0])
+    out["mask"] = mask
+    # the sponge as a border layer (sponge_2d.npz): the reference's inexact
+    # sponge gradient still descends there
+    s2 = np.load(os.path.join(HERE, "sponge_2d.npz"))
+    m, hist = fwi.invert(s2["m0"], Geom(s2["idx"]), s2["wavelet"], float(s2["dt"]), float(s2["h"]),
+                         int(s2["nt"]), s2["dobs"], s2["sponge"], [int(s) for s in s2["src_list"]],
+                         n_iter=3, step_frac=0.02)
+    out["gd_sponge_m"], out["gd_sponge_history"] = m, np.array(hist)
+    print("gd_sponge", len(hist), hist[-1] / hist[0])
+    g3 = np.load(os.path.join(HERE, "fmc_3d.npz"))
+    m, hist = fwi.invert(g3["m0"], Geom(g3["idx"]), g3["wavelet"], float(g3["dt"]), float(g3["h"]),
+                         int(g3["nt"]), g3["dobs"], None, [int(s) for s in g3["src_list"]],
+                         n_iter=3, step_frac=0.005)
+    out["gd3d_m"], out["gd3d_history"] = m, np.array(hist)
+    print("gd3d", len(hist), hist[-1] / hist[0])
+    np.savez_compressed(os.path.join(HERE, "invert.npz"), **out)
+    print("invert fixtures written to", HERE)
+
+
 SECTIONS = {"base": main, "apertures": main_apertures, "misfits": main_misfits,
-            "builders": main_builders}
+            "builders": main_builders, "invert": main_invert}
 
 if __name__ == "__main__":
     args = [a for a in sys.argv[1:] if not a.startswith("--only=")]
