@@ -104,7 +104,10 @@ def clear_plans():
 def _to_dev(a, dtype):
     if isinstance(a, torch.Tensor):
         return a.to(device=_device(), dtype=dtype).contiguous()
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=_device())
+    a = np.ascontiguousarray(a)
+    if not a.flags.writeable:  # read-only arrays (np.load, broadcasts): torch wants its own copy
+        a = a.copy()
+    return torch.as_tensor(a, dtype=dtype, device=_device())
 
 
 def _shape_of(m):
