@@ -1393,6 +1393,10 @@ struct TuneVal { Tiling tf, ta; };
 std::map<TuneKey, TuneVal> g_tuned;
 
 int autotune(usct_plan *P) {
+    // The timing launches advance level 1 (two-level candidates also level 2) and
+    // read that row of the source and residual arrays; with nt = 1 no level is
+    // ever advanced, so there is nothing to time and row 1 does not exist.
+    if (P->nt < 2) return 0;
     const TuneKey key{P->ndim, P->nz, P->ny, P->nx, P->Bp, P->sponge ? 1 : 0, P->rev ? 1 : 0};
     auto hit = g_tuned.find(key);
     if (hit != g_tuned.end()) {

@@ -173,6 +173,33 @@ def _plan_problem(shape_n=61, n_src=5, nt=350):
     return f, h, dt, nt, tx, rec, m0, mt, wd
 
 
+@pytest.mark.parametrize("name", ["fmc_2d.npz", "fmc_3d.npz"])
+@pytest.mark.parametrize("method", ["batched", "plain"])
+def test_single_time_level(method, name):
+    """nt = 1: no level is advanced (fwi.py:64 loops n = 1..nt-1), so the traces
+    are the zero row 0, J = 0.5 sum(dobs^2) and the gradient is zero (the
+    adjoint loop k = nt-1..1 is empty); an autotuned plan is created without
+    timing anything (its timing launches would need level 1)."""
+    from openusct_amd.plan import Plan, linear_indices
+    f = golden(name)
+    shape = f["m0"].shape
+    rec = linear_indices(IdxGeom(f["idx"]).idx, shape)
+    tx = rec[[int(s) for s in f["src_list"]]]
+    dev = torch.device("cuda")
+    m0 = torch.tensor(f["m0"], device=dev)
+    w = torch.tensor(f["wavelet"][:1], device=dev)
+    dobs = torch.randn(len(tx), 1, len(rec), dtype=torch.float64, device=dev,
+                       generator=torch.Generator(device=dev).manual_seed(4)) * 1e-6
+    with Plan(shape, float(f["h"]), float(f["dt"]), 1, tx, rec, method=method) as p:
+        print(f"[case] nt = 1 {name} {method} {p.info}", flush=True)
+        d = p.forward_fmc(m0, w)
+        J, g = p.misfit_and_gradient(m0, w, dobs)
+    J_ref = 0.5 * float((dobs ** 2).sum())
+    assert d.shape == (len(tx), 1, len(rec)) and not d.any()
+    assert abs(J.item() - J_ref) <= 1e-6 * J_ref
+    assert not g.any()
+
+
 def test_checkpointed_gradient_equals_whole_history():
     from openusct_amd.plan import Plan
     f, h, dt, nt, tx, rec, m0, mt, wd = _plan_problem()
