@@ -138,6 +138,29 @@ def test_reversing_c4_reduced_vs_oracle():
     assert max(e) < TOL
 
 
+def test_reversing_long_time_axis():
+    """C4's spacing, time step and full axis (2000 levels) on a 64^3 grid with
+    its ellipsoid phantom, 8 shots: the levels re-created over 2000 steps still
+    give the gradient of the plan that keeps checkpoints, to <= 1e-6."""
+    from openusct_amd import configs
+    from openusct_amd.plan import Plan, linear_indices
+    prob = configs.c4(n=64, nt_scale=1.0, shots=8)
+    rec = linear_indices(prob.geom.idx, prob.shape)
+    tx = rec[prob.src_list]
+    dev = torch.device("cuda")
+    m0, mt = torch.tensor(prob.m0, device=dev), torch.tensor(prob.m_true, device=dev)
+    wd = torch.tensor(prob.wavelet, device=dev)
+    with Plan(prob.shape, prob.h, prob.dt, prob.nt, tx, rec, reverse=True) as rp:
+        print(f"[case] reversing over {prob.nt} levels {rp.info}", flush=True)
+        dobs = rp.forward_fmc(mt, wd)
+        J, g = rp.misfit_and_gradient(m0, wd, dobs)
+    with Plan(prob.shape, prob.h, prob.dt, prob.nt, tx, rec, reverse=False, ckpt_every=50) as kp:
+        Jk, gk = kp.misfit_and_gradient(m0, wd, dobs)
+    e = (abs(J.item() - Jk.item()) / Jk.item(), rel_l2(g.cpu().numpy(), gk.cpu().numpy()))
+    print(f"J {e[0]:.2e} gradient {e[1]:.2e}")
+    assert e[0] < 1e-9 and e[1] < 1e-6
+
+
 def test_reversing_gauss_newton_and_adjoint_source():
     """The Hessian-vector product (background levels re-created for the
     adjoint's imaging) against the oracle, and the caller-supplied adjoint
