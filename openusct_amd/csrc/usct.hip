@@ -790,6 +790,44 @@ void launch_born2_batched(const usct_plan *P, hipStream_t s, int n, Lvl u2, Lvl 
     }
 }
 
+// Background and Born level n in one launch (3D, no footprints): k_born3d on
+// the forward's lane groups and planes per wavefront, USCT_GN3CX x USCT_GN3CY
+// cells per lane (two FP64 halos per lane: smaller patches than the forward's;
+// on the 256^3, 64-shot product 2x1 takes 33.5 s, 1x2 33.9, 1x1 34.6, 2x2 36.7,
+// against 39.5 for separate k_fwd3d and k_born launches).
+#ifndef USCT_GN3CX
+#define USCT_GN3CX 2
+#endif
+#ifndef USCT_GN3CY
+#define USCT_GN3CY 1
+#endif
+template <int LB>
+void born3_lb(hipStream_t s, const usct_plan *P, int n, Lvl u2, Lvl u1, Lvl u0) {
+    Tiling tg = P->tf;
+    tg.CX = USCT_GN3CX;
+    tg.CY = USCT_GN3CY;
+    tg.lds = 0;
+    const Tile t = tile_of(P, tg);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+    const double *sd = P->sincd + (int64_t)n * P->Bp;
+    float *d2 = P->srcd2 + (int64_t)n * P->Bp;
+    float *res = P->res + (int64_t)n * P->n_rec * P->Bp;
+    const int a = born_slot(n), b1 = born_slot(n - 1), c2 = born_slot(n - 2);
+    hipLaunchKernelGGL((k_born3d<LB, USCT_GN3CX, USCT_GN3CY>), g, dim3(kBlock), 0, s, u1.hi, u2.hi,
+                       u0.hi, u1.lo, u2.lo, u0.lo, P->Dh[b1], P->Dh[c2], P->Dh[a], P->Dl[b1],
+                       P->Dl[c2], P->Dl[a], P->sp2, P->vm2, P->src_cell, sd, P->rec_head,
+                       P->rec_next, P->row_rec, res, d2, P->nz, P->ny, P->nx, P->Bp, tg.RY, t.nxs,
+                       t.nys, t.nyc, P->Bp / tg.LB);
+}
+
+void launch_born3_batched(const usct_plan *P, hipStream_t s, int n, Lvl u2, Lvl u1, Lvl u0) {
+    switch (P->tf.LB) {
+        case 16: born3_lb<16>(s, P, n, u2, u1, u0); break;
+        case 32: born3_lb<32>(s, P, n, u2, u1, u0); break;
+        default: born3_lb<64>(s, P, n, u2, u1, u0); break;
+    }
+}
+
 // Footprint plans: receiver groups of Born level n into row n of the adjoint source.
 void fp_born_record(usct_plan *P, hipStream_t s, int n) {
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
@@ -844,6 +882,18 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
                 pr->fwd_lv.push_back(2);
             }
             n += 2;
+            continue;
+        }
+        if (born && P->ndim == 3 && fusable(P)) {
+            // background and Born level n together
+            if (pr) ev_start(pr, pr->fwd_ev, s);
+            launch_born3_batched(P, s, n, fwd_level(P, n - 2, true), fwd_level(P, n - 1, true),
+                                 fwd_level(P, n, true));
+            if (pr) {
+                ev_stop(pr, pr->fwd_ev, s);
+                pr->fwd_lv.push_back(1);
+            }
+            n += 1;
             continue;
         }
         const int adv = fwd_step(P, s, fwd_level(P, n - 2, store), fwd_level(P, n - 1, store),

@@ -1282,6 +1282,194 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 }
 
 // ---------------------------------------------------------------------------
+// Gauss-Newton, 3D without sponge or footprints: background level n (the
+// k_fwd3d update, without trace recording) and the Born level n that uses it
+// (the k_born update) in one launch, on k_fwd3d's patch march with a second
+// field.  The Born source reads the background's second difference
+// U^n - 2U^{n-1} + U^{n-2} from the registers that hold the stored levels, so
+// both levels, srcd2 and the Born traces (row n of the adjoint source res) are
+// bit-identical to k_fwd3d<REC=false> followed by k_born<false>.  Per
+// shot-cell it reads 24 bytes and writes 12 instead of reading 18 + 30 and
+// writing 6 + 6.
+//   pc/pp/pn, dc/dp/dn: levels n-1, n-2 (read) and n (written) of the
+//   background and the Born field; sincd/srcd2: row n.
+// ---------------------------------------------------------------------------
+template <int LB, int CX, int CY>
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3D
+k_born3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
+         const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
+         uint16_t *__restrict__ pnl, const float *__restrict__ dc, const float *__restrict__ dp,
+         float *__restrict__ dn, const uint16_t *__restrict__ dcl, const uint16_t *__restrict__ dpl,
+         uint16_t *__restrict__ dnl, const float2 *__restrict__ sp2, const float2 *__restrict__ vm2,
+         const int *__restrict__ src_cell, const double *__restrict__ sincd,
+         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+         const int *__restrict__ row_rec, float *__restrict__ res, float *__restrict__ srcd2,
+         int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+    constexpr int W = kWave / LB;
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    int xs, ys, zc;
+    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = ys * CY;
+    const int z0 = zc * RZ;
+    const int z1 = min(z0 + RZ, nz);
+    if (x0 >= nx) return;
+    const int my_src = src_cell[b];
+    const double my_inc = sincd[b];
+    const size_t rowE = (size_t)nx * Bp;
+    const size_t planeE = rowE * ny;
+    const size_t planeC = (size_t)nx * ny;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    auto row = [&](int j) -> size_t { return (size_t)min(max(y0 + j, 0), ny - 1); };
+    auto at = [&](size_t plane, int j, int i) -> size_t {
+        return plane + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+    };
+    // full state hi + lo of the background (P) and the Born field (D), in FP64
+    auto P = [&](size_t plane, int j, int i) -> double {
+        const size_t e = at(plane, j, i);
+        return (double)ldb(pc + e, o4) + (double)bf2f(ldb(pcl + e, o2));
+    };
+    auto D = [&](size_t plane, int j, int i) -> double {
+        const size_t e = at(plane, j, i);
+        return (double)ldb(dc + e, o4) + (double)bf2f(ldb(dcl + e, o2));
+    };
+    double zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
+    double em[CY][CX], ecu[CY + 2][CX + 2], ep[CY][CX];
+    auto load_halo = [&](size_t pl) {
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            zcu[0][i + 1] = P(pl, -1, i);
+            zcu[CY + 1][i + 1] = P(pl, CY, i);
+            ecu[0][i + 1] = D(pl, -1, i);
+            ecu[CY + 1][i + 1] = D(pl, CY, i);
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            zcu[j + 1][0] = P(pl, j, -1);
+            zcu[j + 1][CX + 1] = P(pl, j, CX);
+            ecu[j + 1][0] = D(pl, j, -1);
+            ecu[j + 1][CX + 1] = D(pl, j, CX);
+        }
+    };
+    // raw loads issued one plane ahead: own cells of level n-1 on plane z+1 and
+    // of level n-2 on plane z, for both fields
+    float nh[CY][CX], qh[CY][CX], mh[CY][CX], rh[CY][CX];
+    uint16_t nl[CY][CX], ql[CY][CX], ml[CY][CX], rl[CY][CX];
+    auto load_raw = [&](size_t pl, size_t pl2) {
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const size_t e = at(pl, j, i), e2 = at(pl2, j, i);
+                nh[j][i] = ldb(pc + e, o4);
+                nl[j][i] = ldb(pcl + e, o2);
+                mh[j][i] = ldb(dc + e, o4);
+                ml[j][i] = ldb(dcl + e, o2);
+                qh[j][i] = ldn<USCT_NT3D>(pp + e2, o4);
+                ql[j][i] = ldn<USCT_NT3D>(ppl + e2, o2);
+                rh[j][i] = ldn<USCT_NT3D>(dp + e2, o4);
+                rl[j][i] = ldn<USCT_NT3D>(dpl + e2, o2);
+            }
+    };
+    {
+        const size_t pm = (size_t)max(z0 - 1, 0) * planeE;
+        const size_t p0 = (size_t)z0 * planeE;
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                zm[j][i] = P(pm, j, i);
+                zcu[j + 1][i + 1] = P(p0, j, i);
+                em[j][i] = D(pm, j, i);
+                ecu[j + 1][i + 1] = D(p0, j, i);
+            }
+        load_halo(p0);
+        load_raw((size_t)min(z0 + 1, nz - 1) * planeE, p0);
+    }
+    for (int z = z0; z < z1; ++z) {
+        const size_t pnz = (size_t)min(z + 1, nz - 1) * planeE;
+        double q[CY][CX], r[CY][CX];
+#pragma unroll
+        for (int j = 0; j < CY; ++j)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                zp[j][i] = hl(nh[j][i], nl[j][i]);
+                ep[j][i] = hl(mh[j][i], ml[j][i]);
+                q[j][i] = hl(qh[j][i], ql[j][i]);
+                r[j][i] = hl(rh[j][i], rl[j][i]);
+            }
+        if (z + 1 < z1) load_raw((size_t)min(z + 2, nz - 1) * planeE, pnz);
+        const bool zint = (z >= 1) && (z <= nz - 2);
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            const int y = y0 + j;
+            const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const bool has_rec = y < ny && row_rec[z * ny + y] > 0;
+            const size_t ro = (size_t)z * planeE + (size_t)y * rowE;
+            const float2 *spr = sp2 + (size_t)z * planeC + (size_t)y * nx;
+            const float2 *vmr = vm2 + (size_t)z * planeC + (size_t)y * nx;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                if (x < nx && y < ny) {
+                    const bool in = yint && x >= 1 && x <= nx - 2;
+                    const double s = s2d(ldb(spr + i, oc));
+                    // background level n (k_fwd3d)
+                    double lu = 0.0;
+                    if (in)
+                        lu = lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
+                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
+                    double inc = s * lu;
+                    const int c = (z * ny + y) * nx + x;
+                    if (c == my_src) inc = inc + my_inc;
+                    const size_t ei = ro + (size_t)i * Bp;
+                    const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
+                    uint16_t lb;
+                    const float v = split_hl(nv, lb);
+                    stn<USCT_NT3D>(pn + ei, o4, v);
+                    stn<USCT_NT3D>(pnl + ei, o2, lb);
+                    if (c == my_src) srcd2[b] = (float)inc;
+                    // Born level n (k_born): the stored background levels' second difference
+                    double ld = 0.0;
+                    if (in)
+                        ld = lap7d(ecu[j + 1][i + 1], ep[j][i], em[j][i], ecu[j + 2][i + 1],
+                                   ecu[j][i + 1], ecu[j + 1][i + 2], ecu[j + 1][i]);
+                    const double d2u = (hl(v, lb) - 2.0 * zcu[j + 1][i + 1]) + q[j][i];
+                    const double bv = (2.0 * ecu[j + 1][i + 1] - r[j][i]) + s * ld -
+                                      s2d(ldb(vmr + i, oc)) * d2u;
+                    uint16_t bl;
+                    const float w = split_hl(bv, bl);
+                    stn<USCT_NT3D>(dn + ei, o4, w);
+                    stn<USCT_NT3D>(dnl + ei, o2, bl);
+                    if (has_rec) {
+                        for (int rj = rec_head[c]; rj >= 0; rj = rec_next[rj])
+                            res[(size_t)rj * Bp + b] = w;
+                    }
+                }
+            }
+        }
+        if (z + 1 < z1) {
+#pragma unroll
+            for (int j = 0; j < CY; ++j)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    zm[j][i] = zcu[j + 1][i + 1];
+                    zcu[j + 1][i + 1] = zp[j][i];
+                    em[j][i] = ecu[j + 1][i + 1];
+                    ecu[j + 1][i + 1] = ep[j][i];
+                }
+            load_halo(pnz);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // LDS-tiled 3D kernels.  A work-group of 256 lanes owns a TX x TY tile of cells
 // (for LB shots) and marches RZ planes along z; lane = (shot, patch), each lane
 // a PX x PY patch of the tile.  The centre plane of the tile, with a one-cell

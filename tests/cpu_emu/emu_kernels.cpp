@@ -599,6 +599,95 @@ bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed)
     return ok;
 }
 
+// Fused 3D Gauss-Newton level (k_born3d) against k_fwd3d<REC=false> followed by
+// k_born<false> on the same inputs: background and Born levels (hi and lo),
+// srcd2 and the Born traces, bit for bit.
+template <int LB, int CX, int CY>
+bool check_born3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
+    const int Bp = LB * ng, W = 64 / LB;
+    const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
+    std::mt19937 rng(seed);
+    char tag[160];
+    std::snprintf(tag, sizeof tag, "born3d nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d", nz, ny, nx,
+                  LB, ng, CX, CY, RZ);
+    std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
+        dl(-1e-7f, 1e-7f);
+    float2 *sp2 = alloc<float2>(N, 1), *vm2 = alloc<float2>(N, 1);
+    for (int64_t c = 0; c < N; ++c) {
+        sp2[c] = make_float2(u(rng), tt(rng));
+        vm2[c] = make_float2(d(rng), tt(rng));
+    }
+    auto fl = [&](bool rnd) {
+        float *p = alloc<float>(FE, Bp);
+        if (rnd) for (int64_t i = 0; i < FE; ++i) p[i] = d(rng);
+        return p;
+    };
+    auto lo = [&](bool rnd) {
+        uint16_t *p = alloc<uint16_t>(FE, Bp);
+        if (rnd) for (int64_t i = 0; i < FE; ++i) p[i] = f2bf(dl(rng));
+        return p;
+    };
+    float *pc = fl(true), *pp = fl(true), *dc = fl(true), *dp = fl(true);
+    uint16_t *pcl = lo(true), *ppl = lo(true), *dcl = lo(true), *dpl = lo(true);
+    float *pn_a = fl(false), *pn_b = fl(false), *dn_a = fl(false), *dn_b = fl(false);
+    uint16_t *pnl_a = lo(false), *pnl_b = lo(false), *dnl_a = lo(false), *dnl_b = lo(false);
+    int *src = alloc<int>(Bp, 0);
+    std::uniform_int_distribution<int> cz(0, nz - 1), cy(0, ny - 1), cx(0, nx - 1);
+    for (int b = 0; b < Bp; ++b) src[b] = (cz(rng) * ny + cy(rng)) * nx + (b % 3 == 0 ? 0 : cx(rng));
+    src[Bp - 1] = -1;
+    double *sincd = alloc<double>(Bp, 0);
+    for (int b = 0; b < Bp; ++b) sincd[b] = (double)d(rng) * 1.2345678901;
+    std::vector<int> rc = {0, (int)N - 1, (int)(N / 2), (int)(N / 3), (int)(N / 3)};
+    const int n_rec = (int)rc.size();
+    int *rec_head = alloc<int>(N, 0), *rec_next = alloc<int>(n_rec, 0);
+    int *row_rec = alloc<int>((int64_t)nz * ny, 0);
+    for (int64_t c = 0; c < N; ++c) rec_head[c] = -1;
+    for (int j = n_rec - 1; j >= 0; --j) {
+        rec_next[j] = rec_head[rc[j]];
+        rec_head[rc[j]] = j;
+    }
+    for (int j = 0; j < n_rec; ++j) row_rec[rc[j] / nx]++;
+    float *res_a = alloc<float>((int64_t)n_rec * Bp, 0), *res_b = alloc<float>((int64_t)n_rec * Bp, 0);
+    float *s2_a = alloc<float>(Bp, 0), *s2_b = alloc<float>(Bp, 0);
+    const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
+    const int nzc = (int)cdiv(nz, RZ);
+    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+    emu::launch(blocks, kBlock, [&] {
+        k_born3d<LB, CX, CY>(pc, pp, pn_a, pcl, ppl, pnl_a, dc, dp, dn_a, dcl, dpl, dnl_a, sp2, vm2,
+                             src, sincd, rec_head, rec_next, row_rec, res_a, s2_a, nz, ny, nx, Bp,
+                             RZ, nxs, nys, nzc, ng);
+    });
+    emu::launch(blocks, kBlock, [&] {
+        k_fwd3d<LB, CX, CY, false>(pc, pp, pn_b, pcl, ppl, pnl_b, sp2, src, sincd, rec_head, rec_next,
+                                   row_rec, nullptr, nullptr, s2_b, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
+                                   ng);
+    });
+    emu::launch((unsigned)cdiv(FE, 256), 256, [&] {
+        k_born<false>(pn_b, pnl_b, pc, pcl, pp, ppl, dc, dcl, dp, dpl, dn_b, dnl_b, sp2, vm2, nullptr,
+                      rec_head, rec_next, res_b, nz, ny, nx, 3, Bp);
+    });
+    bool ok = same("background hi", pn_a, pn_b, FE, tag) && same("background lo", pnl_a, pnl_b, FE, tag) &&
+              same("Born hi", dn_a, dn_b, FE, tag) && same("Born lo", dnl_a, dnl_b, FE, tag) &&
+              same("Born traces", res_a, res_b, (int64_t)n_rec * Bp, tag);
+    for (int b = 0; b < Bp && ok; ++b)
+        if (src[b] >= 0 && std::memcmp(&s2_a[b], &s2_b[b], 4) != 0) {
+            std::printf("MISMATCH %s srcd2 shot %d\n", tag, b);
+            ok = false;
+        }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+template <int LB, int CX, int CY>
+bool born3d_cases(uint32_t seed) {
+    bool ok = true;
+    ok &= check_born3d<LB, CX, CY>(9, 7, 11, 2, 4, seed++);
+    ok &= check_born3d<LB, CX, CY>(5, 6, 13, 1, 64, seed++);
+    ok &= check_born3d<LB, CX, CY>(12, 9, 5, 2, 8, seed++);
+    ok &= check_born3d<LB, CX, CY>(3, 3, 3, 1, 2, seed++);
+    return ok;
+}
+
 template <int LB, int CX, int CY>
 bool radj3d_cases(uint32_t seed) {
     bool ok = true;
@@ -1114,6 +1203,13 @@ int main(int argc, char **argv) {
         ok &= fwd3d_cases<16, 2, 1>(seed + 1700);
         ok &= fwd3d_cases<16, 1, 2>(seed + 1800);
         ok &= fwd3d_cases<16, 1, 1>(seed + 1900);
+    }
+    if (only.empty() || only == "born3d") {
+        ok &= born3d_cases<16, 1, 1>(seed + 8000);
+        ok &= born3d_cases<16, 2, 1>(seed + 8100);
+        ok &= born3d_cases<16, 1, 2>(seed + 8200);
+        ok &= born3d_cases<64, 1, 1>(seed + 8300);
+        ok &= born3d_cases<32, 2, 2>(seed + 8400);
     }
     if (only.empty() || only == "radj3d") {
         ok &= radj3d_cases<16, 2, 2>(seed + 7000);

@@ -121,6 +121,33 @@ def test_hessian_vector_plans_bit_identical():
     assert rel_l2(hv.cpu().numpy(), ref.cpu().numpy()) < 1e-6
 
 
+def test_hessian_vector_3d_fused_bit_identical():
+    """3D: background and Born levels in one launch (k_born3d) against separate
+    k_fwd3d and k_born launches (fuse=False), bit for bit -- keeping the whole
+    history, with checkpoints (whose re-computed background levels come from
+    k_fwd3d) and with the forward run backwards."""
+    from openusct_amd.plan import Plan, linear_indices
+    f = golden("fmc_3d_fine.npz")
+    shape = f["m0"].shape
+    rec = linear_indices(IdxGeom(f["idx"]).idx, shape)
+    tx = rec[[int(s) for s in f["src_list"]]]
+    h, dt, nt = float(f["h"]), float(f["dt"]), int(f["nt"])
+    dev = torch.device("cuda")
+    md, wd = torch.tensor(f["m0"], device=dev), torch.tensor(f["wavelet"], device=dev)
+    vd = torch.tensor(_v(shape, f["m0"], seed=5), device=dev)
+    for rev in (False, True):
+        with Plan(shape, h, dt, nt, tx, rec, autotune=False, fuse=False, reverse=rev) as p:
+            ref = p.gn_hessian_vector(md, wd, vd)
+        cases = [dict(reverse=rev)] + ([dict(reverse=False, ckpt_every=20)] if not rev else [])
+        for kw in cases:
+            with Plan(shape, h, dt, nt, tx, rec, autotune=False, **kw) as p:
+                print(f"[case] GN Hv 3D fused {kw} {p.info}", flush=True)
+                for rep in range(2):
+                    hv = p.gn_hessian_vector(md, wd, vd)
+                    torch.cuda.synchronize()
+                    assert torch.equal(hv, ref), (kw, rep)
+
+
 def test_hessian_is_symmetric_positive():
     """<u, Hv> = <Hu, v> and <v, Hv> > 0 on the device."""
     from openusct_amd import fwi
