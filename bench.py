@@ -191,9 +191,40 @@ def cpu_baseline(budget_s=25.0):
     grad(prob.m0, els, d)
     t = time.perf_counter() - t0
     upd = 2.0 * n * n * (nt - 1) * shots
-    return {"value": upd / t / 1e9, "unit": "Gpt-updates/s", "cores": cores, "kind": kind,
-            "sample": f"512x512 grid, nt=1000, first {shots} of the 128 shots of the benchmark "
-                      f"workload, FP64 misfit+gradient, {t:.1f} s"}
+    out = {"value": upd / t / 1e9, "unit": "Gpt-updates/s", "cores": cores, "kind": kind,
+           "sample": f"512x512 grid, nt=1000, first {shots} of the 128 shots of the benchmark "
+                     f"workload, FP64 misfit+gradient, {t:.1f} s"}
+    if kind == "reference":
+        # the same core on one thread: a child process, since this process's OpenMP
+        # runtime already runs with every core
+        try:
+            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-one-thread"],
+                               env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True,
+                               text=True, timeout=300)
+            out["one_thread"] = json.loads(r.stdout.strip().splitlines()[-1])
+        except Exception as e:  # the main line must still print
+            out["one_thread"] = {"error": repr(e)}
+    return out
+
+
+def cpu_one_thread():
+    """One shot of the reference core's gradient on the 512^2 / 128-element problem
+    (run in a child process with OMP_NUM_THREADS=1)."""
+    from openusct_amd import configs
+    from openusct_amd.plan import linear_indices
+    sys.path.insert(0, os.path.join(ROOT, "oracle", "_ref"))
+    import _uap
+
+    prob = configs.c2()
+    h, dt, nt, w, n = prob.h, prob.dt, prob.nt, prob.wavelet, prob.shape[0]
+    rec = linear_indices(prob.geom.idx, prob.shape).astype(np.int32)
+    tx = rec[[prob.src_list[0]]].copy()
+    d = _uap.forward_fmc(prob.m_true, h, dt, nt, tx, rec, w)
+    t0 = time.perf_counter()
+    _uap.misfit_and_gradient(prob.m0, h, dt, nt, tx, rec, w, d, 0)
+    t = time.perf_counter() - t0
+    return {"value": 2.0 * n * n * (nt - 1) / t / 1e9, "unit": "Gpt-updates/s", "cores": 1,
+            "sample": f"shot 0 of the benchmark workload, FP64 misfit+gradient, {t:.1f} s"}
 
 
 def main():
@@ -230,7 +261,11 @@ def main():
     ap.add_argument("--gn-products", type=int, default=-1,
                     help="Gauss-Newton Hessian-vector products timed on the same plan "
                          "(default: 3 for C2, else 0)")
+    ap.add_argument("--cpu-one-thread", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_one_thread:
+        print(json.dumps(cpu_one_thread()), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(shots would be sharded across GPUs with one all-reduce of g; not built).",
