@@ -138,6 +138,31 @@ def test_reversing_c4_reduced_vs_oracle():
     assert max(e) < TOL
 
 
+@pytest.mark.parametrize("misfit", ["l2", "gcn"])
+def test_reversing_windowed_and_gcn(misfit):
+    """Time windows (fwi.py:298-301) and the GCN misfit only change the adjoint
+    source: with the forward run backwards they give the gradients of the plan
+    that keeps levels."""
+    from openusct_amd.plan import Plan
+    f, shape, h, dt, nt, tx, rec, m0, mt, wd = _golden_plan_inputs("fmc_3d_fine.npz")
+    dobs = torch.tensor(f["dobs"], device=m0.device)
+    t = torch.arange(nt, dtype=torch.float64, device=m0.device)[:, None]
+    win = torch.clamp((0.7 * nt - t) / (0.1 * nt), 0.0, 1.0).expand(nt, len(rec))
+    tw = torch.stack([win, win.flip(0)]).contiguous()
+    out = {}
+    for rev in (True, False):
+        with Plan(shape, h, dt, nt, tx, rec, reverse=rev) as p:
+            print(f"[case] reversing={rev} {misfit} windowed", flush=True)
+            out[rev] = (p.misfit_and_gradient(m0, wd, dobs, misfit_type=misfit),
+                        p.misfit_and_gradient(m0, wd, dobs, misfit_type=misfit, time_weights=tw))
+    for k in range(2):
+        (J1, g1), (J2, g2) = out[True][k], out[False][k]
+        e = rel_l2(g1.cpu().numpy(), g2.cpu().numpy())
+        print(f"{'windowed' if k else 'plain'}: J {abs(J1.item() - J2.item()) / abs(J2.item()):.2e} "
+              f"gradient {e:.2e}")
+        assert torch.equal(J1, J2) and e < 1e-6
+
+
 def test_reversing_long_time_axis():
     """C4's spacing, time step and full axis (2000 levels) on a 64^3 grid with
     its ellipsoid phantom, 8 shots: the levels re-created over 2000 steps still
