@@ -197,13 +197,23 @@ __device__ __forceinline__ void stb(T *base, unsigned boff, T v) {
 // reuse): non-temporal loads (NT bit 1) and stores (NT bit 2) bypass the L2's
 // normal retention, leaving it to the rows other wavefronts re-read.  Measured
 // on the 512^2, 128-shot gradient the 2D two-level kernels gain 9 % with both;
-// the 3D adjoint loses 6 % with both (the non-temporal loads give up the
-// scalar-base addressing form).
+// the 3D adjoint that reads stored levels (k_adj3d, 16-lane tiles) loses 6 % with
+// both (the non-temporal loads give up the scalar-base addressing form).
 #ifndef USCT_NT3D
 #define USCT_NT3D 0
 #endif
 #ifndef USCT_NT2D
 #define USCT_NT2D 3
+#endif
+// The reversing 3D adjoint (k_radj3d) gains with both: on the 256^3, 64-shot
+// gradient 6.80 ms per level against 7.42 (loads only 7.33, stores only 7.04).
+#ifndef USCT_NTR3D
+#define USCT_NTR3D 3
+#endif
+// The 3D forward kernels (k_fwd3d, k_born3d) on the same gradient, whose tuned
+// tiles are 64-lane groups: 4.00 ms per level with both against 4.31.
+#ifndef USCT_NTF3D
+#define USCT_NTF3D 3
 #endif
 template <int NT, class T>
 __device__ __forceinline__ T ldn(const T *base, unsigned boff) {
@@ -1205,8 +1215,8 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                 h[j][i] = ldb(pc + e, o4);
                 l[j][i] = ldb(pcl + e, o2);
                 const size_t eq = ppl_ + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
-                qh_[j][i] = ldn<USCT_NT3D>(pp + eq, o4);
-                ql_[j][i] = ldn<USCT_NT3D>(ppl + eq, o2);
+                qh_[j][i] = ldn<USCT_NTF3D>(pp + eq, o4);
+                ql_[j][i] = ldn<USCT_NTF3D>(ppl + eq, o2);
             }
     };
     {
@@ -1256,8 +1266,8 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
                     uint16_t lb;
                     const float v = split_hl(nv, lb);
-                    stn<USCT_NT3D>(pn + ei, o4, v);
-                    stn<USCT_NT3D>(pnl + ei, o2, lb);
+                    stn<USCT_NTF3D>(pn + ei, o4, v);
+                    stn<USCT_NTF3D>(pnl + ei, o2, lb);
                     if (c == my_src) srcd2[b] = (float)inc;
                     if (has_rec) {
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
@@ -1371,10 +1381,10 @@ k_born3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__re
                 nl[j][i] = ldb(pcl + e, o2);
                 mh[j][i] = ldb(dc + e, o4);
                 ml[j][i] = ldb(dcl + e, o2);
-                qh[j][i] = ldn<USCT_NT3D>(pp + e2, o4);
-                ql[j][i] = ldn<USCT_NT3D>(ppl + e2, o2);
-                rh[j][i] = ldn<USCT_NT3D>(dp + e2, o4);
-                rl[j][i] = ldn<USCT_NT3D>(dpl + e2, o2);
+                qh[j][i] = ldn<USCT_NTF3D>(pp + e2, o4);
+                ql[j][i] = ldn<USCT_NTF3D>(ppl + e2, o2);
+                rh[j][i] = ldn<USCT_NTF3D>(dp + e2, o4);
+                rl[j][i] = ldn<USCT_NTF3D>(dpl + e2, o2);
             }
     };
     {
@@ -1432,8 +1442,8 @@ k_born3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__re
                     const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
                     uint16_t lb;
                     const float v = split_hl(nv, lb);
-                    stn<USCT_NT3D>(pn + ei, o4, v);
-                    stn<USCT_NT3D>(pnl + ei, o2, lb);
+                    stn<USCT_NTF3D>(pn + ei, o4, v);
+                    stn<USCT_NTF3D>(pnl + ei, o2, lb);
                     if (c == my_src) srcd2[b] = (float)inc;
                     // Born level n (k_born): the stored background levels' second difference
                     double ld = 0.0;
@@ -1445,8 +1455,8 @@ k_born3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__re
                                       s2d(ldb(vmr + i, oc)) * d2u;
                     uint16_t bl;
                     const float w = split_hl(bv, bl);
-                    stn<USCT_NT3D>(dn + ei, o4, w);
-                    stn<USCT_NT3D>(dnl + ei, o2, bl);
+                    stn<USCT_NTF3D>(dn + ei, o4, w);
+                    stn<USCT_NTF3D>(dnl + ei, o2, bl);
                     if (has_rec) {
                         for (int rj = rec_head[c]; rj >= 0; rj = rec_next[rj])
                             res[(size_t)rj * Bp + b] = w;
@@ -2518,9 +2528,9 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
                 an[j][i] = ldb(l1 + e, o4);
                 nh[j][i] = ldb(pc + e, o4);
                 nl[j][i] = ldb(pcl + e, o2);
-                l2n[j][i] = ldn<USCT_NT3D>(l2 + e2, o4);
-                kh[j][i] = ldn<USCT_NT3D>(pk + e2, o4);
-                kl[j][i] = ldn<USCT_NT3D>(pkl + e2, o2);
+                l2n[j][i] = ldn<USCT_NTR3D>(l2 + e2, o4);
+                kh[j][i] = ldn<USCT_NTR3D>(pk + e2, o4);
+                kl[j][i] = ldn<USCT_NTR3D>(pkl + e2, o2);
             }
     };
     {
@@ -2603,7 +2613,7 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r])
                             v = v - res[(size_t)r * Bp + b];
                     }
-                    stn<USCT_NT3D>(lk + ei, o4, v);
+                    stn<USCT_NTR3D>(lk + ei, o4, v);
                     double lu = 0.0;
                     if (in)
                         lu = lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
@@ -2614,8 +2624,8 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
                         const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
                         uint16_t lb;
                         const float pv = split_hl(nv, lb);
-                        stn<USCT_NT3D>(po + ei, o4, pv);
-                        stn<USCT_NT3D>(pol + ei, o2, lb);
+                        stn<USCT_NTR3D>(po + ei, o4, pv);
+                        stn<USCT_NTR3D>(pol + ei, o2, lb);
                     }
                     tv[j * CX + i] = v * (float)inc;
                 }
