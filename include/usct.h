@@ -23,6 +23,8 @@
  *                                   which the reference runs in Python only
  *                                   (fwi.py:594-615)
  *   usct_gn_hessian_vector          simulation/ringfwi/fwi.py:410-431
+ *   usct_uap_forward_fmc            software/core/libuap.hpp:109-116, same argument types
+ *   usct_uap_misfit_and_gradient    software/core/libuap.hpp:169-224, same argument types
  *   usct_plan_* / usct_forward_fmc / usct_misfit / usct_misfit_and_gradient
  *                                   the same three operations (plus misfit-only,
  *                                   software/python/uap.py:56-67) split into a
@@ -150,7 +152,9 @@ int usct_misfit_and_gradient(usct_plan *plan, const double *m, const double *wav
 /* Windowed misfit (fwi.py:246-263 with time_weights, fwi.py:255-262): the misfit
  * of W*dsyn against W*dobs per transmit, and with it the gradient, whose adjoint
  * source is multiplied by W again (fwi.py:298-301).  time_weights: device
- * n_tx*nt*n_rec doubles in the layout of dobs, or NULL (no window). */
+ * n_tx*nt*n_rec doubles in the layout of dobs, or NULL (no window).  The first
+ * windowed call allocates the plan's window traces (8 bytes per trace sample of
+ * a pass) and fails with USCT_ENOMEM when they would exceed its mem_budget. */
 int usct_misfit_tw(usct_plan *plan, const double *m, const double *wavelet,
                    const double *sponge, const double *dobs, const double *time_weights,
                    int misfit_type, double *J, void *stream);
@@ -195,7 +199,9 @@ int usct_gsot(const double *dsyn, const double *dobs, int n_tx, int nt, int n_re
  * divided by m.  v, hv: device N doubles.  The background wavefield is
  * recomputed each call (checkpointed, as for the gradient); the plan needs
  * gradient workspace (no USCT_FLAG_FORWARD_ONLY) and allocates the Born state
- * (four levels of the plan's field size, 6 bytes per element) on first use.
+ * (four levels of the plan's field size, 6 bytes per element) on first use;
+ * that first call fails with USCT_ENOMEM when the Born state would take the
+ * plan past its mem_budget.
  * Plans that advance two levels per launch (2D, no sponge, no footprints)
  * advance the background and the Born field together. */
 int usct_gn_hessian_vector(usct_plan *plan, const double *m, const double *wavelet,
@@ -225,6 +231,24 @@ int usct_misfit_and_gradient_direct(const double *m, const int *dims, int ndim, 
                                     const int64_t *rec_lin, int n_rec, const double *wavelet,
                                     const double *dobs, double *grad, double *J_host,
                                     int misfit_type, int method, void *stream);
+
+/* The libuap interface itself, host memory in and out: the exact argument types
+ * of uap::forward_fmc (software/core/libuap.hpp:109-112) and
+ * uap::misfit_and_gradient (libuap.hpp:169-173, returning J), so a C or MEX
+ * caller written against libuap binds unchanged.  m, wavelet, dobs, data and
+ * grad are host arrays; tx_lin / rec_lin are int linear cells.  Each call
+ * stages its arrays into HBM, runs a batched plan (created and destroyed
+ * inside the call) on the null stream and copies the results back before it
+ * returns.  usct_uap_forward_fmc returns a usct_status;
+ * usct_uap_misfit_and_gradient returns J, or NaN on failure with the reason in
+ * usct_last_error(). */
+int usct_uap_forward_fmc(const double *m, const int *dims, int ndim, double h, double dt, int nt,
+                         const int *tx_lin, int n_tx, const int *rec_lin, int n_rec,
+                         const double *wavelet, double *data);
+double usct_uap_misfit_and_gradient(const double *m, const int *dims, int ndim, double h,
+                                    double dt, int nt, const int *tx_lin, int n_tx,
+                                    const int *rec_lin, int n_rec, const double *wavelet,
+                                    const double *dobs, double *grad, int misfit_type);
 
 const char *usct_last_error(void);
 int usct_version(void);

@@ -1591,9 +1591,23 @@ int leave(usct_plan *P, void *stream) {
 // Born state and v/m buffers, allocated once per plan (Gauss-Newton only).
 // The Born fields and v/m carry the guard zones of the forward fields: the
 // two-level Born kernel reads halo columns past the first and last cell.
+// Both are held against the plan's memory budget: a call whose buffers do not
+// fit fails with USCT_ENOMEM before anything is allocated.
+int fits_budget(const usct_plan *P, int64_t need, const char *what) {
+    if (P->bytes + need > P->budget)
+        return fail(USCT_ENOMEM,
+                    "%s need %lld bytes; the plan holds %lld of its %lld-byte memory budget",
+                    what, (long long)need, (long long)P->bytes, (long long)P->budget);
+    return 0;
+}
+
 int ensure_gn(usct_plan *P) {
     if (P->vm2) return 0;
     const int64_t FE = P->field_elems();
+    const int64_t g = kGuardCells * std::max<int64_t>(P->Bp, 1);
+    const int64_t need = 4 * ((FE + 2 * g) * 4 + (2 * cdiv(FE, 2) + 2 * g) * 2) +
+                         (P->N + 2 * kGuardCells) * 8;
+    TRY(fits_budget(P, need, "the Gauss-Newton Born state and v/m"));
     for (int i = 0; i < 4; ++i) {
         TRY(dalloc_guarded(P, &P->Dh[i], FE, P->Bp));
         TRY(dalloc_guarded(P, &P->Dl[i], 2 * cdiv(FE, 2), P->Bp));
@@ -1605,6 +1619,7 @@ int ensure_gn(usct_plan *P) {
 int ensure_windows(usct_plan *P) {
     if (P->twT) return 0;
     const int64_t count = (int64_t)P->nt * P->n_rec * P->Bp;
+    TRY(fits_budget(P, 2 * count * (int64_t)sizeof(float), "the time-window traces"));
     TRY(dalloc_t(P, &P->twT_lo, count));
     return dalloc_t(P, &P->twT, count);
 }
@@ -1805,6 +1820,44 @@ int attach_groups(usct_plan *P, const int64_t *tx_off, const int64_t *tx_cells,
     P->fp_nse = (int64_t)sb.size();
     P->fp = true;
     return 0;
+}
+
+}  // namespace
+
+namespace {
+
+// Host arrays of the libuap interface (int cells, host memory) staged into HBM
+// for a plan: the device buffers of one call, freed when it returns.
+struct HostStage {
+    std::vector<void *> bufs;
+    ~HostStage() {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+    template <class T>
+    int alloc(T **p, int64_t count) {
+        void *q = nullptr;
+        if (hipMalloc(&q, sizeof(T) * (size_t)std::max<int64_t>(count, 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(USCT_ENOMEM, "staging buffer of %lld bytes",
+                        (long long)(sizeof(T) * count));
+        }
+        bufs.push_back(q);
+        *p = (T *)q;
+        return 0;
+    }
+};
+
+int widen_cells(const int *a, int n, const char *what, std::vector<int64_t> &out) {
+    if (n > 0 && !a) return fail(USCT_EINVAL, "%s is null", what);
+    out.assign(std::max(n, 0), 0);
+    for (int i = 0; i < n; ++i) out[i] = a[i];
+    return 0;
+}
+
+int64_t cells_of(const int *dims, int ndim) {
+    int64_t n = 1;
+    for (int d = 0; d < ndim; ++d) n *= std::max(dims[d], 0);
+    return n;
 }
 
 }  // namespace
@@ -2120,6 +2173,84 @@ int usct_misfit_and_gradient_direct(const double *m, const int *dims, int ndim, 
     (void)hipFree(Jd);
     usct_plan_destroy(P);
     return rc;
+}
+
+int usct_uap_forward_fmc(const double *m, const int *dims, int ndim, double h, double dt, int nt,
+                         const int *tx_lin, int n_tx, const int *rec_lin, int n_rec,
+                         const double *wavelet, double *data) {
+    if (!dims || !m || !wavelet || (n_tx > 0 && !data)) return fail(USCT_EINVAL, "null argument");
+    if (ndim != 2 && ndim != 3) return fail(USCT_EINVAL, "ndim must be 2 or 3 (got %d)", ndim);
+    std::vector<int64_t> tx, rec;
+    TRY(widen_cells(tx_lin, n_tx, "tx_lin", tx));
+    TRY(widen_cells(rec_lin, n_rec, "rec_lin", rec));
+    const int64_t N = cells_of(dims, ndim);
+    const int64_t nd = (int64_t)std::max(n_tx, 0) * std::max(nt, 0) * std::max(n_rec, 0);
+    usct_plan *P = nullptr;
+    TRY(usct_plan_create(&P, ndim, dims, h, dt, nt, tx.data(), n_tx, rec.data(), n_rec,
+                         USCT_METHOD_BATCHED, 0, 0, USCT_FLAG_NO_GRAPH | USCT_FLAG_FORWARD_ONLY));
+    HostStage st;
+    double *md = nullptr, *wd = nullptr, *dd = nullptr;
+    int rc = st.alloc(&md, N);
+    if (!rc) rc = st.alloc(&wd, nt);
+    if (!rc) rc = st.alloc(&dd, nd);
+    if (!rc && hipMemcpy(md, m, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(USCT_EHIP, "copy of m failed");
+    if (!rc && hipMemcpy(wd, wavelet, sizeof(double) * nt, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(USCT_EHIP, "copy of the wavelet failed");
+    if (!rc) rc = usct_forward_fmc(P, md, wd, nullptr, dd, nullptr);
+    if (!rc && nd && hipMemcpy(data, dd, sizeof(double) * nd, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(USCT_EHIP, "copy of the traces failed");
+    usct_plan_destroy(P);
+    return rc;
+}
+
+double usct_uap_misfit_and_gradient(const double *m, const int *dims, int ndim, double h,
+                                    double dt, int nt, const int *tx_lin, int n_tx,
+                                    const int *rec_lin, int n_rec, const double *wavelet,
+                                    const double *dobs, double *grad, int misfit_type) {
+    const double bad = std::nan("");
+    if (!dims || !m || !wavelet || !grad || (n_tx > 0 && !dobs)) {
+        fail(USCT_EINVAL, "null argument");
+        return bad;
+    }
+    if (ndim != 2 && ndim != 3) {
+        fail(USCT_EINVAL, "ndim must be 2 or 3 (got %d)", ndim);
+        return bad;
+    }
+    std::vector<int64_t> tx, rec;
+    if (widen_cells(tx_lin, n_tx, "tx_lin", tx) || widen_cells(rec_lin, n_rec, "rec_lin", rec))
+        return bad;
+    const int64_t N = cells_of(dims, ndim);
+    const int64_t nd = (int64_t)std::max(n_tx, 0) * std::max(nt, 0) * std::max(n_rec, 0);
+    usct_plan *P = nullptr;
+    if (usct_plan_create(&P, ndim, dims, h, dt, nt, tx.data(), n_tx, rec.data(), n_rec,
+                         USCT_METHOD_BATCHED, 0, 0, USCT_FLAG_NO_GRAPH))
+        return bad;
+    double J = bad;
+    {
+        HostStage st;
+        double *md = nullptr, *wd = nullptr, *od = nullptr, *gd = nullptr, *Jd = nullptr;
+        int rc = st.alloc(&md, N);
+        if (!rc) rc = st.alloc(&wd, nt);
+        if (!rc) rc = st.alloc(&od, nd);
+        if (!rc) rc = st.alloc(&gd, N);
+        if (!rc) rc = st.alloc(&Jd, 1);
+        if (!rc && hipMemcpy(md, m, sizeof(double) * N, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(USCT_EHIP, "copy of m failed");
+        if (!rc && hipMemcpy(wd, wavelet, sizeof(double) * nt, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(USCT_EHIP, "copy of the wavelet failed");
+        if (!rc && nd && hipMemcpy(od, dobs, sizeof(double) * nd, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(USCT_EHIP, "copy of dobs failed");
+        if (!rc) rc = usct_misfit_and_gradient(P, md, wd, nullptr, od, misfit_type, Jd, gd, nullptr);
+        double Jh = bad;
+        if (!rc && hipMemcpy(grad, gd, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(USCT_EHIP, "copy of the gradient failed");
+        if (!rc && hipMemcpy(&Jh, Jd, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(USCT_EHIP, "copy of J failed");
+        if (!rc) J = Jh;
+    }
+    usct_plan_destroy(P);
+    return J;
 }
 
 const char *usct_last_error(void) { return g_err.c_str(); }

@@ -43,3 +43,24 @@ def velocity_panels(plt, panels, extent, vmin, vmax, history, marks=None, title_
     ax[-1].grid(True, which="both", alpha=0.3)
     fig.tight_layout()
     return fig
+
+
+def snapshots(plt, frames, extent, marks, dt):
+    """Pressure frames side by side on one shared, symmetric colour scale
+    (each frame divided by the largest |p| over all frames), the elements as
+    open circles and one colour bar for the row.  frames: {step: 2D array}."""
+    import numpy as np
+    steps = sorted(frames)
+    peak = max(float(np.abs(frames[s]).max()) for s in steps) or 1.0
+    fig, ax = plt.subplots(1, len(steps), figsize=(4.0 * len(steps) + 1.0, 4.0), squeeze=False)
+    im = None
+    for a, s in zip(ax[0], steps):
+        im = a.imshow(frames[s] / peak, origin="lower", extent=extent, cmap="RdBu_r", vmin=-1.0,
+                      vmax=1.0)
+        a.scatter(marks[:, 0] * 1e3, marks[:, 1] * 1e3, s=14, facecolors="none", edgecolors="k",
+                  linewidths=0.6)
+        a.set_title(f"step {s}  ({s * dt * 1e6:.2f} us)")
+        a.set_xlabel("x (mm)")
+    ax[0, 0].set_ylabel("y (mm)")
+    fig.colorbar(im, ax=list(ax[0]), shrink=0.85, label="p / max |p|")
+    return fig

@@ -33,7 +33,8 @@ def main(out_dir="figures", n_iter=8, verbose=True):
     zz, yy, xx = np.mgrid[0:n, 0:n, 0:n].astype(float) * h
     cxy = (n - 1) * h / 2.0
     mask = (np.hypot(xx - cxy, yy - cxy) <= 0.95 * 0.012).astype(float)
-    bounds = (phantom.velocity_to_m(3600.0), phantom.velocity_to_m(2500.0))
+    bounds_c = (2500.0, 3600.0)
+    bounds = (phantom.velocity_to_m(bounds_c[1]), phantom.velocity_to_m(bounds_c[0]))
     src = list(range(0, geom.n_elements, 3))
     if verbose:
         print(f"grid {n}^3, {geom.n_elements} elements, {len(src)} transmitters, nt={nt}")
@@ -54,22 +55,12 @@ def main(out_dir="figures", n_iter=8, verbose=True):
               f"{history[-1] / history[0] * 100:.1f} % of initial, flaw {out['c_flaw']:.0f} m/s")
     plt = _plot.pyplot()
     if plt is not None and out_dir:
-        ext = [0, domain * 1e3, 0, domain * 1e3]
-        fig, ax = plt.subplots(2, 4, figsize=(17, 8.5))
-        cuts = (("xy (axial)", lambda v: v[iz]), ("xz", lambda v: v[:, iy, :]),
-                ("yz", lambda v: v[:, :, ix]))
-        for col, (name, cut) in enumerate(cuts):
-            for row, (vol, tag) in enumerate(((c_true, "true"), (c_rec, "FWI"))):
-                im = ax[row, col].imshow(cut(vol), origin="lower", extent=ext, cmap="viridis",
-                                         vmin=2600, vmax=3100)
-                ax[row, col].set_title(f"{tag}: {name} slice")
-                fig.colorbar(im, ax=ax[row, col], fraction=0.046, label="c (m/s)")
-        hist = np.asarray(history, dtype=float)
-        ax[0, 3].semilogy(hist / hist[0], "o-")
-        ax[0, 3].set_title("Misfit convergence")
-        ax[0, 3].grid(True, which="both", alpha=0.3)
-        ax[1, 3].axis("off")
-        fig.tight_layout()
+        # one row: the true and reconstructed axial plane through the flaw, the
+        # reconstruction's two vertical planes through it, and the misfit curve
+        panels = [(c_true[iz], f"true, z index {iz}"), (c_rec[iz], f"FWI, z index {iz}"),
+                  (c_rec[:, iy, :], f"FWI, y index {iy}"), (c_rec[:, :, ix], f"FWI, x index {ix}")]
+        fig = _plot.velocity_panels(plt, panels, [0, domain * 1e3, 0, domain * 1e3],
+                                    bounds_c[0], bounds_c[1], history)
         out["figure"] = _plot.save(fig, out_dir, "reconstruction_3d.png")
         plt.close(fig)
     return out

@@ -32,16 +32,7 @@ def main(out_dir="figures", verbose=True):
         print("peak |p| at " + ", ".join(f"step {fr}: {np.abs(hist[fr]).max():.3e}" for fr in FRAMES))
     plt = _plot.pyplot()
     if plt is not None and out_dir:
-        fig, ax = plt.subplots(1, len(FRAMES), figsize=(13, 4.2))
-        ext = [0, domain * 1e3, 0, domain * 1e3]
-        for a, fr in zip(ax, FRAMES):
-            amp = float(np.abs(hist[fr]).max()) + 1e-30
-            a.imshow(hist[fr], origin="lower", extent=ext, cmap="seismic", vmin=-amp, vmax=amp)
-            a.plot(geom.xy[:, 0] * 1e3, geom.xy[:, 1] * 1e3, "k.", ms=4)
-            a.set_title(f"t = {fr * dt * 1e6:.1f} us")
-            a.set_xlabel("x (mm)")
-            a.set_ylabel("y (mm)")
-        fig.tight_layout()
+        fig = _plot.snapshots(plt, out["frames"], [0, domain * 1e3, 0, domain * 1e3], geom.xy, dt)
         out["figure"] = _plot.save(fig, out_dir, "wavefield.png")
         plt.close(fig)
     return out

@@ -11,7 +11,9 @@
  *   - the observed data equal to the starting model's own traces give J = 0
  *     and a zero gradient, exactly;
  *   - the stateless call and the plan give the same traces, bit for bit;
- *   - an element outside the grid is refused with USCT_EINVAL and a message.
+ *   - an element outside the grid is refused with USCT_EINVAL and a message;
+ *   - the libuap-typed entry points (host arrays, int cells) give the same
+ *     traces bit for bit and the same misfit and gradient.
  * With an argument it also writes, as float64 in this order, the wavelet (nt),
  * the flaw model m (41 x 41), the element indices (12), the traces
  * (12 x nt x 12), the gradient (41 x 41) and J to that file, for a comparison
@@ -151,6 +153,36 @@ int main(int argc, char **argv) {
     }
     printf("misfit J = %.6e, sum|grad| = %.6e\n", J, gs);
     check(J > 0.0 && gs > 0.0 && finite, "misfit and gradient of the flaw data are positive and finite");
+
+    /* the libuap interface itself: host arrays and int cells, exactly the
+     * argument types of uap::forward_fmc / uap::misfit_and_gradient */
+    {
+        int elem32[N_EL];
+        for (int k = 0; k < N_EL; ++k) elem32[k] = (int)elem[k];
+        double *data3 = malloc(sizeof(double) * ND), *grad3 = malloc(sizeof(double) * N);
+        if (!data3 || !grad3) return 2;
+        USCT(usct_uap_forward_fmc(m, dims, 2, h, dt, nt, elem32, N_EL, elem32, N_EL, wavelet, data3));
+        check(memcmp(data, data3, sizeof(double) * ND) == 0,
+              "usct_uap_forward_fmc (host arrays, int cells) traces bit for bit");
+        const double J3 = usct_uap_misfit_and_gradient(m0, dims, 2, h, dt, nt, elem32, N_EL, elem32,
+                                                       N_EL, wavelet, data, grad3, 0);
+        double dg = 0.0, ng = 0.0;
+        for (size_t i = 0; i < N; ++i) {
+            dg += (grad3[i] - grad[i]) * (grad3[i] - grad[i]);
+            ng += grad[i] * grad[i];
+        }
+        printf("usct_uap_misfit_and_gradient: J = %.6e, |J - J_plan| / J = %.2e, gradient rel L2 "
+               "%.2e\n", J3, fabs(J3 - J) / J, sqrt(dg / ng));
+        check(fabs(J3 - J) <= 1e-9 * J && sqrt(dg) <= 1e-6 * sqrt(ng),
+              "usct_uap_misfit_and_gradient equals the plan's misfit and gradient");
+        int badc[2] = {elem32[0], (int)N};
+        const double Jbad = usct_uap_misfit_and_gradient(m0, dims, 2, h, dt, nt, badc, 2, badc, 2,
+                                                         wavelet, data, grad3, 0);
+        check(isnan(Jbad) && strlen(usct_last_error()) > 0,
+              "usct_uap_misfit_and_gradient returns NaN and a message for a cell outside the grid");
+        free(data3);
+        free(grad3);
+    }
 
     /* reciprocity of the traces */
     double rmax = 0.0;

@@ -172,7 +172,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     from openusct_amd import _lib
     lib = _lib.load()
     header = open(os.path.join(ROOT, "include", "usct.h")).read()
-    declared = set(re.findall(r"^(?:int|int64_t|const char \*)\s*(usct_\w+)\(", header, re.M))
+    declared = set(re.findall(r"^(?:int|int64_t|double|const char \*)\s*(usct_\w+)\(", header, re.M))
     assert declared == set(_lib.EXPORTS)
     assert lib.usct_gsot_workspace_bytes(2, 10, 3) == 6 * 11 * (5 * 8 + 3 * 4)
     assert lib.usct_gsot_workspace_bytes(1, 0, 1) == -1
