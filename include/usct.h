@@ -250,6 +250,24 @@ double usct_uap_misfit_and_gradient(const double *m, const int *dims, int ndim, 
                                     const int *rec_lin, int n_rec, const double *wavelet,
                                     const double *dobs, double *grad, int misfit_type);
 
+/* usct_forward_fmc_direct with method USCT_METHOD_PLAIN, allocating no device
+ * memory.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture elsewhere in their program, tight
+ * serving loops, harnesses that forbid hipMalloc).  Arguments and results as
+ * usct_forward_fmc_direct (m, wavelet, data: device; tx_lin, rec_lin: host);
+ * the traces equal its plain-method traces bit for bit.  Every buffer the
+ * computation needs lives in one static device array of 64 MiB that each
+ * device the library runs on holds until the process ends; a problem needs
+ * about 36 bytes per grid cell plus 8 per trace sample (nt * n_rec), and
+ * larger ones fail with USCT_ENOMEM before any work is queued.  Calls on one
+ * device share that array: calls that overlap in time on one device (other
+ * streams or host threads) race; calls ordered on one stream do not.  The call
+ * queues its work on `stream` and returns; host index arrays may be reused
+ * when it returns. */
+int usct_forward_fmc_noalloc(const double *m, const int *dims, int ndim, double h, double dt,
+                             int nt, const int64_t *tx_lin, int n_tx, const int64_t *rec_lin,
+                             int n_rec, const double *wavelet, double *data, void *stream);
+
 const char *usct_last_error(void);
 int usct_version(void);
 

@@ -26,7 +26,7 @@ EXPORTS = (
     "usct_gn_hessian_vector", "usct_gradient_from_source", "usct_gsot_workspace_bytes", "usct_gsot",
     "usct_profile_gradient",
     "usct_forward_fmc_direct", "usct_misfit_and_gradient_direct", "usct_uap_forward_fmc",
-    "usct_uap_misfit_and_gradient", "usct_last_error",
+    "usct_uap_misfit_and_gradient", "usct_forward_fmc_noalloc", "usct_last_error",
     "usct_version",
 )
 
@@ -109,6 +109,8 @@ def load(path=None):
     lib.usct_uap_misfit_and_gradient.argtypes = [vp, intp, ip, dp, dp, ip, intp, ip, intp, ip,
                                                  vp, vp, vp, ip]
     lib.usct_uap_misfit_and_gradient.restype = ctypes.c_double
+    lib.usct_forward_fmc_noalloc.argtypes = [vp, intp, ip, dp, dp, ip, i64p, ip, i64p, ip, vp,
+                                             vp, vp]
     lib.usct_last_error.restype = ctypes.c_char_p
     lib.usct_version.restype = ctypes.c_int
     _lib = lib

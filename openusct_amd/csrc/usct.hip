@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1824,7 +1825,93 @@ int attach_groups(usct_plan *P, const int64_t *tx_off, const int64_t *tx_cells,
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Forward without device allocation (usct_forward_fmc_noalloc)
+//
+// The plain one-shot-at-a-time forward with every buffer it needs (S' pairs,
+// the zero level and three rotating levels with their low parts, receiver
+// chains, source terms, one shot's traces) carved from one file-scope device
+// array.  Each device the library runs on holds its own copy of the array
+// until the process ends; host code reaches a device's copy only through the
+// address hipGetSymbolAddress returns for it on that device, kept per device.
+// Calls on one device share the array: calls that overlap in time on one
+// device (different streams or host threads) race; calls ordered on one
+// stream do not.  Everything a call reads before writing (the zero level,
+// receiver chains, trace row 0) is cleared by kernels at the start of the call.
+// ---------------------------------------------------------------------------
 namespace {
+constexpr int64_t kScratchBytes = (int64_t)64 << 20;
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's copy of `scratch` (looked up once per
+// device, with that device current).
+int scratch_ptr(void **out) {
+    int dev = 0;
+    HIPX(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(USCT_ENOTSUP, "device %d: the static workspace supports devices 0..%d", dev,
+                    kMaxDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPX(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(USCT_EHIP, "hipGetSymbolAddress returned a null workspace address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// Consecutive 256-byte aligned pieces of a workspace (base nullptr: sizing only).
+struct Carve {
+    char *base;
+    int64_t off = 0;
+    template <class T>
+    T *take(int64_t count) {
+        T *p = reinterpret_cast<T *>(base + off);
+        off += cdiv(std::max<int64_t>(count, 1) * (int64_t)sizeof(T), 256) * 256;
+        return p;
+    }
+};
+
+struct NoallocBufs {
+    float2 *sp2;
+    float *Z, *F[3];
+    uint16_t *Zlo, *Lo[3];
+    int *head, *next, *tx32;
+    int64_t *rec64, *tx64;
+    float *sinc, *srcd2, *recb, *recb_lo;
+    double *sincd;
+};
+
+int64_t carve_noalloc(char *base, int64_t N, int nt, int n_tx, int n_rec, NoallocBufs &B) {
+    Carve c{base};
+    B.sp2 = c.take<float2>(N);
+    B.Z = c.take<float>(N);
+    for (int i = 0; i < 3; ++i) B.F[i] = c.take<float>(N);
+    // low parts padded to whole 4-byte words (cleared as floats)
+    B.Zlo = c.take<uint16_t>(2 * cdiv(N, 2));
+    for (int i = 0; i < 3; ++i) B.Lo[i] = c.take<uint16_t>(N);
+    B.head = c.take<int>(N);
+    B.next = c.take<int>(n_rec);
+    B.rec64 = c.take<int64_t>(n_rec);
+    B.tx64 = c.take<int64_t>(n_tx);
+    B.tx32 = c.take<int>(n_tx);
+    B.sinc = c.take<float>(nt);
+    B.sincd = c.take<double>(nt);
+    B.srcd2 = c.take<float>(nt);
+    B.recb = c.take<float>((int64_t)nt * n_rec);
+    B.recb_lo = c.take<float>((int64_t)nt * n_rec);
+    return c.off;
+}
 
 // Host arrays of the libuap interface (int cells, host memory) staged into HBM
 // for a plan: the device buffers of one call, freed when it returns.
@@ -2173,6 +2260,63 @@ int usct_misfit_and_gradient_direct(const double *m, const int *dims, int ndim, 
     (void)hipFree(Jd);
     usct_plan_destroy(P);
     return rc;
+}
+
+int usct_forward_fmc_noalloc(const double *m, const int *dims, int ndim, double h, double dt,
+                             int nt, const int64_t *tx_lin, int n_tx, const int64_t *rec_lin,
+                             int n_rec, const double *wavelet, double *data, void *stream) {
+    if (!dims) return fail(USCT_EINVAL, "null argument");
+    int64_t N = 0;
+    TRY(validate(ndim, dims, h, dt, nt, tx_lin, n_tx, rec_lin, n_rec, &N));
+    if (!m || !wavelet || (n_tx > 0 && !data))
+        return fail(USCT_EINVAL, "m, wavelet and data are required");
+    NoallocBufs B;
+    const int64_t need = carve_noalloc(nullptr, N, nt, n_tx, n_rec, B);
+    if (need > kScratchBytes)
+        return fail(USCT_ENOMEM,
+                    "the forward of %lld cells, %d levels, %d transmits and %d receivers needs "
+                    "%lld bytes of workspace; the static workspace holds %lld "
+                    "(use usct_forward_fmc_direct or a plan)",
+                    (long long)N, nt, n_tx, n_rec, (long long)need, (long long)kScratchBytes);
+    if (n_tx == 0) return 0;
+    void *base = nullptr;
+    TRY(scratch_ptr(&base));
+    carve_noalloc((char *)base, N, nt, n_tx, n_rec, B);
+    hipStream_t s = (hipStream_t)stream;
+    const int nz = ndim == 3 ? dims[0] : 1, ny = dims[ndim - 2], nx = dims[ndim - 1];
+    const unsigned gN = grid_for(N);
+    // levels <= 0 are the zero level; level n >= 1 rotates over three buffers
+    auto lvl = [&](int n) { return n <= 0 ? B.Z : B.F[n % 3]; };
+    auto llo = [&](int n) { return n <= 0 ? B.Zlo : B.Lo[n % 3]; };
+    hipLaunchKernelGGL(k_fill, dim3(gN), dim3(256), 0, s, B.Z, N, 0.0f);
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(cdiv(N, 2))), dim3(256), 0, s, (float *)B.Zlo,
+                       cdiv(N, 2), 0.0f);
+    hipLaunchKernelGGL(k_fill_i32, dim3(gN), dim3(256), 0, s, B.head, N, -1);
+    HIPX(hipMemcpyAsync(B.rec64, rec_lin, sizeof(int64_t) * n_rec, hipMemcpyHostToDevice, s));
+    HIPX(hipMemcpyAsync(B.tx64, tx_lin, sizeof(int64_t) * n_tx, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_rec_chains, dim3(1), dim3(64), 0, s, B.rec64, n_rec, B.head, B.next,
+                       B.tx64, n_tx, B.tx32);
+    hipLaunchKernelGGL(k_setup_s, dim3(gN), dim3(256), 0, s, m, B.sp2, N, dt * dt, 1.0 / (h * h));
+    const int64_t NJ = (int64_t)nt * n_rec;
+    const dim3 gt((unsigned)cdiv(NJ, kTT), 1);
+    for (int b = 0; b < n_tx; ++b) {
+        hipLaunchKernelGGL(k_setup_sinc, dim3(grid_for(nt)), dim3(256), 0, s, m, wavelet,
+                           B.tx32 + b, B.sinc, B.sincd, nt, 1, dt * dt);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n_rec)), dim3(256), 0, s, B.recb,
+                           (int64_t)n_rec, 0.0f);
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n_rec)), dim3(256), 0, s, B.recb_lo,
+                           (int64_t)n_rec, 0.0f);
+        for (int n = 1; n < nt; ++n)
+            hipLaunchKernelGGL((k_fwd_plain<false, true>), dim3(gN), dim3(256), 0, s, lvl(n - 1),
+                               lvl(n - 2), lvl(n), llo(n - 1), llo(n - 2), llo(n), B.sp2,
+                               (const float *)nullptr, (int)tx_lin[b], B.sincd + n, B.head, B.next,
+                               B.recb + (int64_t)n * n_rec, B.recb_lo + (int64_t)n * n_rec,
+                               B.srcd2 + n, nz, ny, nx, ndim);
+        hipLaunchKernelGGL(k_batched_to_data, gt, dim3(256), 0, s, B.recb, B.recb_lo, data, NJ,
+                           1, b, 1);
+        TRY(check_launch());
+    }
+    return check_launch();
 }
 
 int usct_uap_forward_fmc(const double *m, const int *dims, int ndim, double h, double dt, int nt,

@@ -2984,6 +2984,29 @@ __global__ void k_fill(float *__restrict__ p, int64_t n, float v) {
         p[i] = v;
 }
 
+__global__ void k_fill_i32(int *__restrict__ p, int64_t n, int v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Receiver chains on the device (one thread, as the host builds them for a
+// plan): head[c] = first receiver on cell c, next[j] = the following one, in
+// ascending receiver order, so duplicate receivers accumulate in the order
+// fwi.py:321-322 subtracts them.  head must hold -1 everywhere on entry.
+// Also narrows the transmit cells to int (the source kernels' type).
+__global__ void k_rec_chains(const int64_t *__restrict__ rec, int n_rec, int *__restrict__ head,
+                             int *__restrict__ next, const int64_t *__restrict__ tx, int n_tx,
+                             int *__restrict__ tx32) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int j = n_rec - 1; j >= 0; --j) {
+        const int64_t c = rec[j];
+        next[j] = head[c];
+        head[c] = j;
+    }
+    for (int i = 0; i < n_tx; ++i) tx32[i] = (int)tx[i];
+}
+
 // Traces move between the caller's layout data[b][n][j] (FP64) and the
 // batched layout [n][j][b] (FP32 hi + lo) through 32 x 32 LDS tiles, so both
 // the reads and the writes are coalesced.  Grid: (ceil(nt*n_rec / 32),
