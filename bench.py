@@ -1,24 +1,37 @@
 """Benchmark: FWI gradient throughput (Gpt-updates/s, forward + adjoint).
 
-Workload (one MI355X): 2D ring full-matrix capture on a 512 x 512 grid,
-dx = 0.2 mm, dt = 50 ns, nt = 1000, 128-element ring of radius 22 mm, all 128
-elements transmit, Ricker 500 kHz, true model = 1500 m/s water with three
-seeded inclusions (c in [1400, 1800] m/s), initial model 1500 m/s.  Observed
-data is synthesised on the device.  A step is one misfit-and-gradient
-evaluation over all 128 shots: 2 * N * (nt - 1) * 128 = 6.70e10 grid-point
-updates.
+Headline workload (one MI355X): the 256-shot 2D ring full-matrix capture at
+512 x 512 -- dx = 0.2 mm, dt = 50 ns, nt = 1000, a 256-element ring of radius
+22 mm, every element transmits, Ricker 500 kHz, true model 1500 m/s water with
+three seeded inclusions (c in [1400, 1800] m/s), starting model 1500 m/s.
+Observed data is synthesised on the device.  A step is one misfit-and-gradient
+over all 256 shots: 2 * N * (nt - 1) * 256 = 1.34e11 grid-point updates.
 
-Prints one JSON line with the batched throughput as ``value``, the plain
-per-shot path (one launch per shot per time step) beside it, the roofline of
-the dominant kernel (per-launch HIP-event timing) and the reference CPU core
-timed on a bounded sample of the same kind of workload.
+The JSON line carries, besides the headline value:
+  plain_value      the plain per-shot path (one launch per shot per time level)
+                   on the same inputs;
+  roofline         the dominant kernel of the gradient against the HBM peak,
+                   on the bytes that kernel cannot avoid per launch (its
+                   operands once), timed per launch with HIP events on the
+                   plan's stream; HBM traffic per launch from committed
+                   rocprofv3 --pmc runs; the survey's per-update byte basis
+                   beside it under its own name;
+  c4               the 3D cylinder at 256^3 with 64 shots (nt = 2000): the
+                   same figures for one gradient;
+  stream_ceiling   read, write and copy rates of streaming kernels
+                   (libusct_bw.so) on the same device;
+  cpu_baseline     the reference's own C++ core (FP64, OpenMP) on the 256^2,
+                   8-shot problem, median of 3 after one warm-up, plus one thread;
+  serving, gauss_newton, dataset_io   the other legs, on samples.
 """
 
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
+import statistics
 import subprocess
 import sys
 import time
@@ -29,16 +42,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_FWD = 16.0   # algorithmic bytes of one forward update (FP32)
+BYTES_FWD = 16.0   # the survey's algorithmic bytes of one forward update (FP32)
 BYTES_ADJ = 28.0   # adjoint + imaging update
 BYTES_PER_UPDATE = 22.0
-
+METRIC = "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency"
 
 WORKLOADS = {
+    "C2-256": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 256-element ring R=22mm, "
+              "256 shots, Ricker 500kHz, FP32 fields; one step = one misfit+gradient over all "
+              "256 shots",
     "C2": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 128-element ring R=22mm, "
           "128 shots, Ricker 500kHz, FP32 fields; one step = one misfit+gradient over all shots",
-    "C2-256": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 256-element ring R=22mm, "
-              "256 shots, Ricker 500kHz, FP32 fields; one step = one misfit+gradient",
     "C3": "2D ring FMC 1024x1024, dx=0.1mm, dt=25ns, nt=4000, 256-element ring R=44mm, "
           "256 shots, Ricker 500kHz, FP32 fields, checkpointed; one step = one misfit+gradient",
     "C4": "3D cylinder 256^3, dx=0.25mm, dt=30ns, nt=2000, 8 rings x 64 elements R=28mm, "
@@ -49,6 +63,12 @@ WORKLOADS = {
           "{256^2, 512^2, 1024^2} and 3D {128^3, 192^3, 256^3}, 16-256 shots, seeded phantoms, "
           "admitted by HBM footprint; value = aggregate Gpt-updates/s over the run",
 }
+
+# rocprofv3 --pmc summaries (HBM bytes per launch) committed under profiles/
+TRAFFIC_FILES = {"C2-256": "pmc_traffic_ring512_256shots.json",
+                 "C2": "pmc_traffic_ring512_128shots.json",
+                 "C4": "pmc_traffic_cylinder256_64shots.json"}
+FULL_SHOTS = {"C2-256": 256, "C2": 128, "C3": 256, "C4": 64}
 
 
 def serving(n_requests, nt_scale, rate=150.0, seed=5, with_3d=True):
@@ -141,95 +161,341 @@ def dataset_io(traces, prob):
     return out
 
 
-def workload(name="C2"):
+def workload(name):
     from openusct_amd import configs
     return configs.BUILDERS[name]()
 
 
-def cpu_baseline(budget_s=25.0):
-    """Time the reference's own C++ core (oracle/_ref, built from its sources) on a
-    bounded sample of the benchmark workload: the 512^2 / 128-element ring problem
-    (nt = 1000) with its first k shots, k sized so the timed gradient takes about
-    budget_s.  FP64 with OpenMP over all cores of this process's affinity mask;
-    falls back to the oracle restatement when the reference core is not built."""
-    from openusct_amd import configs
-    from openusct_amd.plan import linear_indices
+# ---------------------------------------------------------------------------
+# streaming ceiling (libusct_bw.so)
+# ---------------------------------------------------------------------------
+def _bw_lib():
+    lib = ctypes.CDLL(os.path.join(ROOT, "openusct_amd", "libusct_bw.so"))
+    lib.usct_bw_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]
+    lib.usct_bw_device_cus.restype = ctypes.c_int
+    return lib
 
-    cores = len(os.sched_getaffinity(0))
-    prob = configs.c2()
-    h, dt, nt, w = prob.h, prob.dt, prob.nt, prob.wavelet
-    n = prob.shape[0]
-    rec = linear_indices(prob.geom.idx, prob.shape).astype(np.int32)
-    kind = "reference"
-    try:
-        sys.path.insert(0, os.path.join(ROOT, "oracle", "_ref"))
-        import _uap  # the reference's pybind11 module, compiled from its own sources
 
-        def fwd(m, els):
-            return _uap.forward_fmc(m, h, dt, nt, rec[els].copy(), rec, w)
-
-        def grad(m, els, d):
-            return _uap.misfit_and_gradient(m, h, dt, nt, rec[els].copy(), rec, w, d, 0)
-    except ImportError:
-        from oracle import oracle as O
-        kind = "port"
-
-        def fwd(m, els):
-            return O.forward_fmc(m, prob.geom, w, dt, h, nt, None, els)
-
-        def grad(m, els, d):
-            return O.misfit_and_gradient(m, prob.geom, w, dt, h, nt, d, None, els)
-    # one shot first (also the warm-up) to size the sample to the time budget
-    d1 = fwd(prob.m_true, [0])
-    t0 = time.perf_counter()
-    grad(prob.m0, [0], d1)
-    t1 = time.perf_counter() - t0
-    shots = int(max(1, min(len(prob.src_list), budget_s // max(t1, 1e-3))))
-    els = list(prob.src_list[:shots])
-    d = fwd(prob.m_true, els)
-    t0 = time.perf_counter()
-    grad(prob.m0, els, d)
-    t = time.perf_counter() - t0
-    upd = 2.0 * n * n * (nt - 1) * shots
-    out = {"value": upd / t / 1e9, "unit": "Gpt-updates/s", "cores": cores, "kind": kind,
-           "sample": f"512x512 grid, nt=1000, first {shots} of the 128 shots of the benchmark "
-                     f"workload, FP64 misfit+gradient, {t:.1f} s"}
-    if kind == "reference":
-        # the same core on one thread: a child process, since this process's OpenMP
-        # runtime already runs with every core
-        try:
-            r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-one-thread"],
-                               env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True,
-                               text=True, timeout=300)
-            out["one_thread"] = json.loads(r.stdout.strip().splitlines()[-1])
-        except Exception as e:  # the main line must still print
-            out["one_thread"] = {"error": repr(e)}
+def stream_ceiling(gib=4, reps=5):
+    """Best read / write / copy rates of the streaming kernels over `gib` GiB
+    buffers (far beyond L2 and Infinity Cache), over work-groups per CU {2, 4, 8,
+    16} and ordinary or non-temporal accesses.  Copy counts read + write bytes."""
+    import torch
+    lib = _bw_lib()
+    nbytes = gib << 30
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    cus = lib.usct_bw_device_cus()
+    part = torch.empty(cus * 64, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    out = {"buffer_bytes": nbytes, "cus": cus, "reps": reps}
+    for kind, name, mult in ((0, "read", 1), (1, "write", 1), (2, "copy", 2)):
+        best = None
+        for bpc in (2, 4, 8, 16):
+            for nt in (0, 1):
+                ms = ctypes.c_float()
+                rc = lib.usct_bw_run(kind, a.data_ptr(), b.data_ptr(), part.data_ptr(), nbytes, bpc,
+                                     nt, reps, ctypes.byref(ms), None)
+                if rc:
+                    raise RuntimeError(f"usct_bw_run({name}) failed: {rc}")
+                gbs = mult * nbytes / (ms.value / 1e3) / 1e9
+                if best is None or gbs > best[0]:
+                    best = (gbs, bpc, nt, ms.value)
+        out[name] = {"GBps": round(best[0], 1), "blocks_per_cu": best[1],
+                     "nontemporal": bool(best[2]), "ms": round(best[3], 4)}
+    del a, b, part
+    torch.cuda.empty_cache()
     return out
 
 
-def cpu_one_thread():
-    """One shot of the reference core's gradient on the 512^2 / 128-element problem
-    on one OpenMP thread (run in a child process)."""
+def calib_copy(mib):
+    """Counter calibration: three launches of the streaming copy kernel over `mib`
+    MiB (each reads and writes exactly mib MiB)."""
+    import torch
+    lib = _bw_lib()
+    nbytes = mib << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    ms = ctypes.c_float()
+    rc = lib.usct_bw_run(2, a.data_ptr(), b.data_ptr(), None, nbytes, 8, 0, 2, ctypes.byref(ms),
+                         None)
+    torch.cuda.synchronize()
+    if rc:
+        raise RuntimeError("calibration copy failed")
+    del a, b
+
+
+# ---------------------------------------------------------------------------
+# kernel byte model
+# ---------------------------------------------------------------------------
+def kernel_min_bytes(kernel, N, shots, groups):
+    """(read, write) bytes a launch of `kernel` cannot avoid: every operand once.
+    Per shot-cell: fields (FP32 4 B; forward state FP32 hi + bf16 lo = 6 B);
+    per cell: the S' pair (8 B) and, for adjoints, the gradient accumulators of
+    each lane group (hi + lo, read and written: 8 + 8 B per group)."""
+    per_sc = {
+        # read p^{n-1}, p^{n-2}; write p^n, p^{n+1}
+        "k_fwd2d_x2": (12.0, 12.0),
+        # read p^{n-1}, p^{n-2}; write p^n
+        "k_fwd2d": (12.0, 6.0), "k_fwd3d": (12.0, 6.0),
+        # read lam^{k+1}, lam^{k+2}, p^{k-1}, p^{k-2}; write lam^k, lam^{k-1}
+        "k_adj2d_x2": (16.0, 8.0),
+        # read lam^{k+1}, lam^{k+2}, one stored level; write lam^k
+        "k_adj2d": (12.0, 4.0), "k_adj3d": (12.0, 4.0),
+        # read lam^{k+1}, lam^{k+2}, forward levels k-1 and k (hi + lo); write lam^k, level k-2
+        "k_radj3d": (20.0, 10.0),
+    }[kernel]
+    adj = "adj" in kernel
+    rd = per_sc[0] * N * shots + 8.0 * N + (8.0 * N * groups if adj else 0.0)
+    wr = per_sc[1] * N * shots + (8.0 * N * groups if adj else 0.0)
+    return rd, wr
+
+
+def load_traffic(name, kernel, shots):
+    f = TRAFFIC_FILES.get(name)
+    path = os.path.join(ROOT, "profiles", f) if f else None
+    if not path or shots != FULL_SHOTS.get(name) or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        k = json.load(fh)["kernels"].get(kernel)
+    return None if k is None else (k["read_bytes"], k["write_bytes"])
+
+
+# ---------------------------------------------------------------------------
+# one gradient workload
+# ---------------------------------------------------------------------------
+def gradient_leg(name, steps, warmup, args, keep=False):
+    """Batched plan of workload `name`: observed data, `warmup` untimed and `steps`
+    timed gradients, then an eager pass with per-launch HIP events.  Returns the
+    figures, and the plan, model tensors and data when ``keep``."""
+    import torch
+    from openusct_amd.plan import Plan, linear_indices
+
+    dev = torch.device("cuda", 0)
+    prob = workload(name)
+    shape, h, dt, nt = prob.shape, prob.h, prob.dt, prob.nt
+    N = int(np.prod(shape))
+    rec = linear_indices(prob.geom.idx, shape)
+    src = prob.src_list[: args.shots] if args.shots and name == args.workload else prob.src_list
+    tx = rec[src].copy()
+    wd = torch.tensor(prob.wavelet, device=dev)
+    mt = torch.tensor(prob.m_true, device=dev)
+    md = torch.tensor(prob.m0, device=dev)
+    reverse = {"auto": None, "on": True, "off": False}[args.reverse]
+    ckpt = args.ckpt_every or (0 if reverse is not False and len(shape) == 3 else prob.ckpt_every)
+    torch.cuda.reset_peak_memory_stats()
+    print(f"[bench] {name}: building batched plan ({len(tx)} shots, {shape}, nt={nt})",
+          file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    plan = Plan(shape, h, dt, nt, tx, rec, method="batched", ckpt_every=ckpt,
+                autotune=not args.no_autotune, reverse=reverse if len(shape) == 3 else None)
+    t_plan = time.perf_counter() - t0
+    info = plan.info
+    print(f"[bench] {name}: plan in {t_plan:.1f} s: {info}", file=sys.stderr, flush=True)
+    dobs = plan.forward_fmc(mt, wd)
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        J, g = plan.misfit_and_gradient(md, wd, dobs)
+    torch.cuda.synchronize()
+    print(f"[bench] {name}: warmup done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        J, g = plan.misfit_and_gradient(md, wd, dobs)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    upd = 2.0 * N * (nt - 1) * len(tx)
+    value = upd / (ms / 1e3) / 1e9
+    print(f"[bench] {name}: batched {ms:.1f} ms/gradient, {value:.1f} Gpt-updates/s",
+          file=sys.stderr, flush=True)
+    peak = torch.cuda.max_memory_allocated() + info["workspace_bytes"]
+    out = {"name": name, "workload": WORKLOADS[name] if len(tx) == len(prob.src_list) else
+           WORKLOADS[name] + f" (first {len(tx)} shots)", "grid": list(shape), "nt": nt,
+           "shots": len(tx), "receivers": len(rec), "plan": info, "plan_seconds": round(t_plan, 2),
+           "ms_per_gradient": round(ms, 3), "value": value, "misfit": float(J.item()),
+           "peak_hbm_bytes": int(peak), "updates": upd, "steps": steps, "warmup": warmup}
+    if not args.no_profile:
+        prof = plan.profile_gradient(md, wd, dobs)
+        out["roofline"] = roofline(name, prof, info, N, len(tx), shape, value)
+        out["recompute_level_fraction"] = round(
+            max(prof["n_fwd"] - (nt - 1) * info["shot_passes"], 0) /
+            max(prof["n_fwd"] + prof["n_adj"], 1), 4)
+        r = out["roofline"]
+        print(f"[bench] {name}: {r['kernel']} {r['per_launch_ms']:.3f} ms per launch, "
+              f"{r['achieved']:.0f} GB/s on its minimum bytes (frac {r['frac']}); "
+              f"{r['other_kernel']['name']} {r['other_kernel']['per_launch_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
+    if keep:
+        return out, (prob, plan, mt, md, wd, dobs, g, rec, tx)
+    plan.close()
+    del dobs, g
+    torch.cuda.empty_cache()
+    return out, None
+
+
+def roofline(name, prof, info, N, shots, shape, value):
+    """Roofline of the gradient's dominant kernel (the one with the larger share of
+    the launch time), on the bytes it cannot avoid per launch."""
+    rev = bool(info.get("reverse"))
+    lva = max(int(prof.get("adj_levels", 1)), 1)
+    lvf = max(int(prof.get("fwd_levels", 1)), 1)
+    d2 = len(shape) == 2
+    adj_k = ("k_adj2d" if d2 else ("k_radj3d" if rev else "k_adj3d")) + ("_x2" if lva == 2 else "")
+    fwd_k = ("k_fwd2d" if d2 else "k_fwd3d") + ("_x2" if lvf == 2 else "")
+    passes = max(info["shot_passes"], 1)
+    per_pass = shots / passes
+    groups = max(info["shots_per_pass"] // max(info["lanes_per_group"], 1), 1)
+    t_adj = prof["adj_ms"] * prof["n_adj"] / lva
+    t_fwd = prof["fwd_ms"] * prof["n_fwd"] / lvf
+
+    def one(kernel, ms, lv, survey_bytes):
+        rd, wr = kernel_min_bytes(kernel, N, per_pass, groups)
+        ach = (rd + wr) / (ms / 1e3) / 1e9
+        tr = load_traffic(name, kernel, shots)
+        d = {"name": kernel, "per_launch_ms": round(ms, 4), "levels_per_launch": lv,
+             "min_read_bytes": rd, "min_write_bytes": wr, "achieved": round(ach, 1),
+             "frac": round(ach / HBM_PEAK_GBS, 4),
+             "traffic": None if tr is None else round(tr[0] + tr[1]),
+             "traffic_GBps": None if tr is None else round((tr[0] + tr[1]) / (ms / 1e3) / 1e9, 1),
+             # the survey's per-update basis (28 B per adjoint update, 16 B per forward
+             # update) counts every level's operands once per level; a two-level launch
+             # reuses them from registers, so this figure may exceed the peak
+             "survey_basis": {"bytes": survey_bytes * N * per_pass * lv,
+                              "GBps": round(survey_bytes * N * per_pass * lv / (ms / 1e3) / 1e9, 1)}}
+        return d
+
+    a = one(adj_k, prof["adj_ms"], lva, BYTES_ADJ)
+    f = one(fwd_k, prof["fwd_ms"], lvf, BYTES_FWD)
+    dom, other = (a, f) if t_adj >= t_fwd else (f, a)
+    what = {"k_adj2d_x2": "adjoint + imaging, two time levels per launch",
+            "k_adj2d": "adjoint + imaging, one time level per launch",
+            "k_adj3d": "3D adjoint + imaging from stored levels, one level per launch",
+            "k_radj3d": "3D adjoint + imaging with the forward level re-created, one level per launch",
+            "k_fwd2d_x2": "forward, two time levels per launch",
+            "k_fwd2d": "forward, one level per launch", "k_fwd3d": "3D forward, one level per launch"}
+    return {"bound": "hbm", "kernel": dom["name"], "what": what.get(dom["name"], ""),
+            "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": dom["frac"], "traffic": dom["traffic"], "traffic_GBps": dom["traffic_GBps"],
+            "per_launch_ms": dom["per_launch_ms"], "levels_per_launch": dom["levels_per_launch"],
+            "shots_per_launch": per_pass, "min_read_bytes": dom["min_read_bytes"],
+            "min_write_bytes": dom["min_write_bytes"], "survey_basis": dom["survey_basis"],
+            "time_share": round(max(t_adj, t_fwd) / max(t_adj + t_fwd, 1e-30), 3),
+            "other_kernel": other,
+            "gradient_survey_basis": {"bytes_per_update": BYTES_PER_UPDATE,
+                                      "GBps": round(value * BYTES_PER_UPDATE, 1),
+                                      "frac": round(value * BYTES_PER_UPDATE / HBM_PEAK_GBS, 4)}}
+
+
+def plain_leg(prob, tx, rec, shots):
+    """The plain per-shot path (one launch per shot per level, full history) on the
+    first `shots` shots: one warm-up and one timed gradient."""
+    import torch
+    from openusct_amd.plan import Plan
+    dev = torch.device("cuda", 0)
+    shape, h, dt, nt = prob.shape, prob.h, prob.dt, prob.nt
+    N = int(np.prod(shape))
+    wd = torch.tensor(prob.wavelet, device=dev)
+    mt = torch.tensor(prob.m_true, device=dev)
+    md = torch.tensor(prob.m0, device=dev)
+    ps = min(shots, len(tx))
+    with Plan(shape, h, dt, nt, tx[:ps], rec, method="plain") as pplan:
+        pd = pplan.forward_fmc(mt, wd)
+        pplan.misfit_and_gradient(md, wd, pd)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pplan.misfit_and_gradient(md, wd, pd)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+    del pd
+    torch.cuda.empty_cache()
+    v = 2.0 * N * (nt - 1) * ps / tp / 1e9
+    print(f"[bench] plain {tp * 1e3:.1f} ms/gradient over {ps} shots, {v:.1f} Gpt-updates/s",
+          file=sys.stderr, flush=True)
+    return v, ps
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the reference's own core (oracle/_ref), in child processes
+# ---------------------------------------------------------------------------
+def cpu_child(which, reps, threads):
+    """Time the reference's C++ core (FP64) in this process, which has not touched
+    the device and whose OpenMP runtime starts with the environment it was given.
+    which = "c1": the 256^2, 8-shot problem, one warm-up then `reps` timed
+    gradients; "c2": the first shots of the 512^2, 128-element problem."""
     from openusct_amd import configs
     from openusct_amd.plan import linear_indices
     sys.path.insert(0, os.path.join(ROOT, "oracle", "_ref"))
-    import _uap
+    import _uap  # the reference's pybind11 module, compiled from its own sources
 
-    prob = configs.c2()
-    h, dt, nt, w, n = prob.h, prob.dt, prob.nt, prob.wavelet, prob.shape[0]
+    prob = configs.c1() if which == "c1" else configs.c2()
+    h, dt, nt, w = prob.h, prob.dt, prob.nt, prob.wavelet
     rec = linear_indices(prob.geom.idx, prob.shape).astype(np.int32)
-    tx = rec[[prob.src_list[0]]].copy()
+    els = list(prob.src_list) if which == "c1" else list(prob.src_list[:reps])
+    tx = rec[els].copy()
     d = _uap.forward_fmc(prob.m_true, h, dt, nt, tx, rec, w)
+    times = []
+    n_runs = reps + 1 if which == "c1" else 1
     from threadpoolctl import threadpool_info, threadpool_limits
-    # one OpenMP thread set through the runtime (a host may fix OMP_NUM_THREADS)
-    with threadpool_limits(limits=1, user_api="openmp"):
+    # the thread count is also set through the OpenMP runtime: a host may fix
+    # OMP_NUM_THREADS for every process it starts
+    with threadpool_limits(limits=threads, user_api="openmp"):
         omp = [i["num_threads"] for i in threadpool_info() if i.get("internal_api") == "openmp"]
-        t0 = time.perf_counter()
-        _uap.misfit_and_gradient(prob.m0, h, dt, nt, tx, rec, w, d, 0)
-        t = time.perf_counter() - t0
-    return {"value": 2.0 * n * n * (nt - 1) / t / 1e9, "unit": "Gpt-updates/s", "cores": 1,
-            "omp_threads": omp, "sample": f"shot 0 of the benchmark workload, FP64 "
-                                          f"misfit+gradient, {t:.2f} s"}
+        for _ in range(n_runs):
+            t0 = time.perf_counter()
+            _uap.misfit_and_gradient(prob.m0, h, dt, nt, tx, rec, w, d, 0)
+            times.append(time.perf_counter() - t0)
+    if which == "c1":
+        times = times[1:]  # the first run is the warm-up
+    upd = 2.0 * int(np.prod(prob.shape)) * (nt - 1) * len(els)
+    return {"times_s": [round(t, 4) for t in times], "median_s": statistics.median(times),
+            "updates": upd, "shots": len(els), "omp_threads": omp,
+            "env": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND")}}
+
+
+def _run_child(which, reps, threads):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               HIP_VISIBLE_DEVICES="")
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", which,
+                        "--cpu-reps", str(reps), "--cpu-threads", str(threads)], env=env,
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline():
+    """The reference's own C++ core (oracle/_ref, built from its sources) on the
+    256^2 / 64-element / 8-shot problem (nt = 1000, 1.048e9 updates): OpenMP on every
+    core of this process's affinity mask with OMP_PROC_BIND=close, one warm-up and the
+    median of 3 gradients; the same on one thread; and, as a second sample, the first
+    shots of the 512^2 / 128-element problem."""
+    if not os.path.isdir(os.path.join(ROOT, "oracle", "_ref")):
+        return {"error": "oracle/_ref (the reference's core) is not built"}
+    cores = len(os.sched_getaffinity(0))
+    c1 = _run_child("c1", 3, cores)
+    v = c1["updates"] / c1["median_s"] / 1e9
+    print(f"[bench] CPU reference core, {cores} threads: {c1['median_s']:.3f} s per C1 gradient "
+          f"({v:.3f} Gpt-updates/s)", file=sys.stderr, flush=True)
+    out = {"value": round(v, 4), "unit": "Gpt-updates/s", "cores": cores, "kind": "reference",
+           "sample": f"256x256 grid, 64-element ring, 8 shots, nt=1000 (1.048e9 updates), FP64 "
+                     f"misfit+gradient by the reference's C++ core, OMP_PROC_BIND=close, one "
+                     f"warm-up then the median of 3: {c1['times_s']} s",
+           "c1": c1}
+    try:
+        one = _run_child("c1", 1, 1)
+        out["one_thread"] = {"value": round(one["updates"] / one["median_s"] / 1e9, 4),
+                             "cores": 1, "seconds": one["times_s"], "omp_threads": one["omp_threads"]}
+    except Exception as e:  # the main line must still print
+        out["one_thread"] = {"error": repr(e)}
+    try:
+        c2 = _run_child("c2", 4, cores)
+        out["c2_sample"] = {"value": round(c2["updates"] / c2["median_s"] / 1e9, 4),
+                            "sample": f"first {c2['shots']} shots of the 512^2, 128-element problem",
+                            "seconds": c2["times_s"]}
+    except Exception as e:
+        out["c2_sample"] = {"error": repr(e)}
+    return out
 
 
 def main():
@@ -237,13 +503,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="C2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="C2-256", choices=sorted(WORKLOADS))
     ap.add_argument("--shots", type=int, default=0, help="use only the first N shots")
     ap.add_argument("--plain-shots", type=int, default=0,
                     help="shots of the plain per-shot path timed beside the batched one "
-                         "(default: 128 in 2D, 2 for C4, whose plain plan keeps 134 GB of history)")
+                         "(default: every shot in 2D, 2 in 3D, whose plain plan keeps a 134 GB history)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 3D (C4) leg of the 2D headline")
+    ap.add_argument("--c4-steps", type=int, default=1)
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-kernel ceiling")
     ap.add_argument("--ckpt-every", type=int, default=0)
     ap.add_argument("--reverse", default="auto", choices=["auto", "on", "off"],
                     help="3D gradients: run the forward backwards alongside the adjoint (on), "
@@ -252,7 +521,7 @@ def main():
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the per-launch event timing pass (for counter runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="first run an elementwise copy of this many MiB (counter calibration)")
+                    help="first run the streaming copy kernel over this many MiB (counter calibration)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the serving sample reported beside the main line "
                          "(C5: of the serving run itself); 0 = skip")
@@ -262,14 +531,16 @@ def main():
     ap.add_argument("--serve-nt-scale", type=float, default=0.05,
                     help="time-axis scale of the serving requests")
     ap.add_argument("--no-io", action="store_true",
-                    help="skip the acquisition-file leg (C2 only: write and read its traces)")
+                    help="skip the acquisition-file leg (write and read the workload's traces)")
     ap.add_argument("--gn-products", type=int, default=-1,
                     help="Gauss-Newton Hessian-vector products timed on the same plan "
-                         "(default: 3 for C2, else 0)")
-    ap.add_argument("--cpu-one-thread", action="store_true", help=argparse.SUPPRESS)
+                         "(default: 3 for the 512^2 workloads, else 0)")
+    ap.add_argument("--cpu-child", default="", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-reps", type=int, default=3, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.cpu_one_thread:
-        print(json.dumps(cpu_one_thread()), flush=True)
+    if args.cpu_child:
+        print(json.dumps(cpu_child(args.cpu_child, args.cpu_reps, args.cpu_threads)), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -277,66 +548,23 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
-    from openusct_amd.plan import Plan, linear_indices
-
-    dev = torch.device("cuda", 0)
     if args.calib_copy_mb:
-        # y = x * 1: reads and writes exactly calib_copy_mb MiB each
-        x = torch.ones(args.calib_copy_mb << 18, dtype=torch.float32, device=dev)
-        for _ in range(3):
-            y = x * 1.0
-        torch.cuda.synchronize()
-        del x, y
+        calib_copy(args.calib_copy_mb)
     if args.workload == "C5":
         sv = serving(args.serve_requests or 100, args.serve_nt_scale)
         print(json.dumps({
-            "metric": "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency",
-            "value": sv["gpt_updates_per_s"], "unit": "Gpt-updates/s", "n_gpus": 1, "steps": 1,
-            "warmup": 0, "ms_per_step": round(sv["makespan_s"] * 1e3, 3),
+            "metric": METRIC, "value": sv["gpt_updates_per_s"], "unit": "Gpt-updates/s",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(sv["makespan_s"] * 1e3, 3),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
             "data": "synthetic (seeded phantoms, observed data simulated on the device)",
             "config": {"workload": WORKLOADS["C5"]}, "serving": sv,
             "roofline": None, "cpu_baseline": None}), flush=True)
         return
-    prob = workload(args.workload)
-    shape, h, dt, nt = prob.shape, prob.h, prob.dt, prob.nt
-    N = int(np.prod(shape))
-    rec = linear_indices(prob.geom.idx, shape)
-    src = prob.src_list[: args.shots] if args.shots else prob.src_list
-    tx = rec[src].copy()
-    wd = torch.tensor(prob.wavelet, device=dev)
-    mt = torch.tensor(prob.m_true, device=dev)
-    md = torch.tensor(prob.m0, device=dev)
-    reverse = {"auto": None, "on": True, "off": False}[args.reverse]
-    # a 3D plan left to itself chooses between checkpoints and running the
-    # forward backwards; an explicit --ckpt-every or --reverse off keeps levels
-    ckpt = args.ckpt_every or (0 if reverse is not False and len(shape) == 3 else prob.ckpt_every)
 
-    print(f"[bench] {args.workload}: building batched plan ({len(tx)} shots, {shape}, nt={nt})",
-          file=sys.stderr, flush=True)
-    plan = Plan(shape, h, dt, nt, tx, rec, method="batched", ckpt_every=ckpt,
-                autotune=not args.no_autotune, reverse=reverse if len(shape) == 3 else None)
-    print(f"[bench] plan {plan.info}", file=sys.stderr, flush=True)
-    info = plan.info
-    dobs = plan.forward_fmc(mt, wd)
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        J, g = plan.misfit_and_gradient(md, wd, dobs)
-    torch.cuda.synchronize()
-    print("[bench] warmup done", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        J, g = plan.misfit_and_gradient(md, wd, dobs)
-    torch.cuda.synchronize()
-    dt_run = time.perf_counter() - t0
-    ms = dt_run / args.steps * 1e3
-    upd = 2.0 * N * (nt - 1) * len(tx)
-    value = upd / (ms / 1e3) / 1e9
-    print(f"[bench] batched {ms:.1f} ms/gradient, {value:.1f} Gpt-updates/s", file=sys.stderr,
-          flush=True)
+    head, kept = gradient_leg(args.workload, args.steps, args.warmup, args, keep=True)
+    prob, plan, mt, md, wd, dobs, g, rec, tx = kept
 
     gd = None
     gd_steps = args.gd_steps if args.gd_steps >= 0 else (5 if args.workload == "C3" else 0)
@@ -355,10 +583,12 @@ def main():
               file=sys.stderr, flush=True)
 
     gn = None
-    gn_k = args.gn_products if args.gn_products >= 0 else (3 if args.workload == "C2" else 0)
+    gn_k = args.gn_products if args.gn_products >= 0 else \
+        (3 if args.workload in ("C2", "C2-256") else 0)
     if gn_k:
         # Gauss-Newton products (J^T J) v: background forward, Born forward and
         # adjoint sweep per call; v a smooth direction of model size
+        N = int(np.prod(prob.shape))
         v = (g / g.abs().max()) * md.mean()
         plan.gn_hessian_vector(md, wd, v)  # first call allocates the Born state
         torch.cuda.synchronize()
@@ -367,64 +597,39 @@ def main():
             hv = plan.gn_hessian_vector(md, wd, v)
         torch.cuda.synchronize()
         ms_hv = (time.perf_counter() - t0) / gn_k * 1e3
-        upd_hv = 3.0 * N * (nt - 1) * len(tx)
+        upd_hv = 3.0 * N * (prob.nt - 1) * len(tx)
         gn = {"products": gn_k, "ms_per_product": round(ms_hv, 3),
-              "vs_gradient": round(ms_hv / ms, 3),
+              "vs_gradient": round(ms_hv / head["ms_per_gradient"], 3),
               "gpt_updates_per_s": round(upd_hv / (ms_hv / 1e3) / 1e9, 3),
               "counted": "3 sweeps of N (nt-1) shots: background forward, Born forward, adjoint",
               "finite": bool(torch.isfinite(hv).all().item())}
         print(f"[bench] Gauss-Newton product {ms_hv:.1f} ms ({gn['vs_gradient']}x a gradient), "
               f"{gn['gpt_updates_per_s']} Gpt-updates/s", file=sys.stderr, flush=True)
         del hv, v
-
-    if args.no_profile:
-        prof = {"fwd_ms": float("nan"), "adj_ms": float("nan"), "n_fwd": 0, "n_adj": 0}
-    else:
-        prof = plan.profile_gradient(md, wd, dobs)
-    per_pass = len(tx) / max(info["shot_passes"], 1)
-    lv = max(int(prof.get("adj_levels", 1)), 1)    # time levels per adjoint launch
-    lvf = max(int(prof.get("fwd_levels", 1)), 1)   # and per forward launch
-    adj_bytes = BYTES_ADJ * N * per_pass * lv
-    fwd_bytes = BYTES_FWD * N * per_pass * lvf
-    adj_gbs = adj_bytes / (prof["adj_ms"] / 1e3) / 1e9
-    fwd_gbs = fwd_bytes / (prof["fwd_ms"] / 1e3) / 1e9
-    print(f"[bench] per launch ({lvf}/{lv} level(s)): fwd {prof['fwd_ms']:.3f} ms ({fwd_gbs:.0f} "
-          f"GB/s alg), adj {prof['adj_ms']:.3f} ms ({adj_gbs:.0f} GB/s alg)", file=sys.stderr,
-          flush=True)
-    suffix = "_x2" if lv == 2 else ""
-    rev = bool(info.get("reverse"))
-    # minimum bytes of the adjoint kernel per launch: per shot-cell it reads
-    # lam^{k+1}, lam^{k+2} and one stored level per time level, writes lam per level;
-    # the reversing kernel instead reads forward levels k-1 and k (FP32 hi + bf16 lo)
-    # and writes level k-2: 12 + 18 bytes
-    kmin_bytes = (30.0 if rev else 4.0 * (2 + lv) + 4.0 * lv) * N * per_pass
-    adj_kernel = ("k_adj2d" if len(shape) == 2 else ("k_radj3d" if rev else "k_adj3d")) + suffix
-    fwd_kernel = ("k_fwd2d" if len(shape) == 2 else "k_fwd3d") + ("_x2" if lvf == 2 else "")
-    if len(shape) == 3 and info.get("fwd_tile_kind") == 2:
-        fwd_kernel = "k_fwd3d_x2_lds"
-    Jv = float(J.item())
     traces = None
-    if args.workload == "C2" and not args.no_io:
+    if args.workload in ("C2", "C2-256") and not args.no_io:
         traces = dobs.cpu().numpy()
     plan.close()
-    del dobs
+    del dobs, g, kept
     torch.cuda.empty_cache()
 
-    plain = None
+    plain, plain_shots = None, 0
     if not args.no_plain and args.workload in ("C2", "C2-256", "C4"):
-        ps = min(args.plain_shots or (2 if len(shape) == 3 else 128), len(tx))
-        pplan = Plan(shape, h, dt, nt, tx[:ps], rec, method="plain")
-        pd = pplan.forward_fmc(mt, wd)
-        pplan.misfit_and_gradient(md, wd, pd)  # warm-up
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        pplan.misfit_and_gradient(md, wd, pd)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - t0
-        plain = 2.0 * N * (nt - 1) * ps / tp / 1e9
-        print(f"[bench] plain {tp * 1e3:.1f} ms/gradient over {ps} shots, {plain:.1f} "
-              f"Gpt-updates/s", file=sys.stderr, flush=True)
-        pplan.close()
+        plain, plain_shots = plain_leg(prob, tx, rec, args.plain_shots or
+                                       (2 if len(prob.shape) == 3 else len(tx)))
+
+    c4 = None
+    if not args.no_c4 and args.workload in ("C2", "C2-256"):
+        c4, _ = gradient_leg("C4", args.c4_steps, 1, args)
+        c4["value"] = round(c4["value"], 3)
+        if not args.no_plain:
+            cp = workload("C4")
+            from openusct_amd.plan import linear_indices
+            rec4 = linear_indices(cp.geom.idx, cp.shape)
+            pv, ps = plain_leg(cp, rec4[cp.src_list].copy(), rec4, 2)
+            c4["plain_value"] = round(pv, 3)
+            c4["plain_shots"] = ps
+            c4["batched_vs_plain"] = round(c4["value"] / pv, 3)
 
     io = None
     if traces is not None:
@@ -445,67 +650,48 @@ def main():
         print(f"[bench] serving p50 {sv['p50_s']:.3f} s, p99 {sv['p99_s']:.3f} s, "
               f"{sv['gpt_updates_per_s']:.1f} Gpt-updates/s", file=sys.stderr, flush=True)
 
+    ceiling = None
+    if not args.no_ceiling:
+        ceiling = stream_ceiling()
+        print(f"[bench] streaming ceiling: read {ceiling['read']['GBps']} GB/s, write "
+              f"{ceiling['write']['GBps']} GB/s, copy {ceiling['copy']['GBps']} GB/s",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         print("[bench] timing the reference CPU core", file=sys.stderr, flush=True)
-        cpu = cpu_baseline()
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # the main line must still print
+            cpu = {"error": repr(e)}
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", {"C2": "pmc_traffic_ring512_128shots.json",
-                                            "C4": "pmc_traffic_cylinder256_64shots.json"}.get(
-                                                args.workload, "none"))
-    if len(tx) == {"C2": 128, "C4": 64}.get(args.workload) and os.path.exists(tfile):
-        # HBM bytes per launch of the adjoint kernel from the committed rocprofv3 --pmc
-        # FETCH_SIZE / WRITE_SIZE runs (calibrated against a copy of known size)
-        with open(tfile) as fh:
-            k = json.load(fh)["kernels"].get(adj_kernel)
-        if k is not None:
-            traffic = round(k["read_bytes"] + k["write_bytes"])
-
+    value = head["value"]
     line = {
-        "metric": "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency",
+        "metric": METRIC,
         "value": round(value, 3),
         "unit": "Gpt-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": head["ms_per_gradient"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded inclusion phantom, observed data simulated on the device)",
         "config": {
-            "workload": WORKLOADS[args.workload] if not args.shots else
-            WORKLOADS[args.workload] + f" (first {len(tx)} shots)",
-            "grid": list(shape), "nt": nt, "shots": len(tx), "receivers": len(rec),
-            "method": "batched", "plan": info, "peak_hbm_bytes": torch.cuda.max_memory_allocated() + info["workspace_bytes"],
-            "recompute_level_fraction": round(max(prof["n_fwd"] - (nt - 1) * info["shot_passes"], 0) / max(prof["n_fwd"] + prof["n_adj"], 1), 4),
+            "workload": head["workload"], "grid": head["grid"], "nt": head["nt"],
+            "shots": head["shots"], "receivers": head["receivers"], "method": "batched",
+            "plan": head["plan"], "peak_hbm_bytes": head["peak_hbm_bytes"],
+            "recompute_level_fraction": head.get("recompute_level_fraction"),
         },
         "plain_value": None if plain is None else round(plain, 3),
+        "plain_shots": plain_shots or None,
         "batched_vs_plain": None if plain is None else round(value / plain, 3),
-        "misfit": Jv,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": adj_kernel + f" (adjoint + imaging{', forward level re-created' if rev else ''}"
-                                   f", {lv} time level(s) per launch, all shots of a pass)",
-            "achieved": round(adj_gbs, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(adj_gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "per_launch_ms": round(prof["adj_ms"], 4),
-            "alg_bytes_per_launch": adj_bytes,
-            "levels_per_launch": lv,
-            # HBM bytes the kernel itself cannot avoid per launch (its operands once):
-            # two-level kernels read two levels and write two; against the time above
-            "kernel_min_bytes_per_launch": kmin_bytes,
-            "kernel_min_frac": round(kmin_bytes / (prof["adj_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "fwd_kernel": {"name": fwd_kernel, "achieved": round(fwd_gbs, 1),
-                           "per_launch_ms": round(prof["fwd_ms"], 4),
-                           "alg_bytes_per_launch": fwd_bytes, "levels_per_launch": lvf},
-            "gradient_level": round(value * BYTES_PER_UPDATE, 1),
-        },
+        "misfit": head["misfit"],
+        "roofline": head.get("roofline"),
+        "c4": c4,
+        "stream_ceiling": ceiling,
         "cpu_baseline": cpu,
         "serving": sv,
         "gradient_descent": gd,

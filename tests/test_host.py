@@ -188,6 +188,36 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert b"dims" in lib.usct_last_error()
 
 
+def test_bandwidth_library_exports_every_declared_symbol():
+    """libusct_bw.so (the HBM streaming kernels bench.py measures the device's
+    ceiling with) loads and exports what include/usct_bw.h declares; a bad
+    argument is refused before any HIP call."""
+    lib = ctypes.CDLL(os.path.join(ROOT, "openusct_amd", "libusct_bw.so"))
+    header = open(os.path.join(ROOT, "include", "usct_bw.h")).read()
+    declared = set(re.findall(r"^int\s+(usct_\w+)\(", header, re.M))
+    assert declared == {"usct_bw_run", "usct_bw_device_cus"}
+    for name in declared:
+        assert hasattr(lib, name), name
+    ms = ctypes.c_float()
+    lib.usct_bw_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]
+    assert lib.usct_bw_run(2, None, None, None, 1000, 8, 0, 1, ctypes.byref(ms), None) == 1  # % 16
+    assert lib.usct_bw_run(7, None, None, None, 1024, 8, 0, 1, ctypes.byref(ms), None) == 1
+
+
+def test_kernel_min_bytes_model():
+    """bench.py's per-launch minimum bytes: per shot-cell 24 B for both two-level 2D
+    kernels, 18 B for the 3D forward, 30 B for the reversing 3D adjoint; per cell
+    the S' pair and, for adjoints, the gradient accumulators of each lane group."""
+    import bench
+    N, B = 1000, 64
+    assert bench.kernel_min_bytes("k_adj2d_x2", N, B, 1) == (16 * N * B + 8 * N + 8 * N, 8 * N * B + 8 * N)
+    assert bench.kernel_min_bytes("k_fwd2d_x2", N, B, 1) == (12 * N * B + 8 * N, 12 * N * B)
+    assert sum(bench.kernel_min_bytes("k_fwd3d", N, B, 1)) == 18 * N * B + 8 * N
+    assert sum(bench.kernel_min_bytes("k_radj3d", N, B, 2)) == 30 * N * B + 8 * N + 32 * N
+
+
 def test_footprint_model_orders():
     from openusct_amd.plan import footprint_bytes
     small = footprint_bytes((256, 256), 1000, 8, 64)
