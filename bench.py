@@ -48,6 +48,9 @@ BYTES_PER_UPDATE = 22.0
 METRIC = "FWI gradient Gpt-updates/s (fwd+adj) batched vs plain; p99 solve latency"
 
 WORKLOADS = {
+    "C1-16": "2D ring FMC 256x256, dx=0.2mm, dt=50ns, nt=1000, 64-element ring R=22mm, 16 shots "
+             "(every 4th element), Ricker 500kHz, FP32 fields; one step = one misfit+gradient "
+             "(a small request: launch overhead at its largest)",
     "C2-256": "2D ring FMC 512x512, dx=0.2mm, dt=50ns, nt=1000, 256-element ring R=22mm, "
               "256 shots, Ricker 500kHz, FP32 fields; one step = one misfit+gradient over all "
               "256 shots",
@@ -68,7 +71,7 @@ WORKLOADS = {
 TRAFFIC_FILES = {"C2-256": "pmc_traffic_ring512_256shots.json",
                  "C2": "pmc_traffic_ring512_128shots.json",
                  "C4": "pmc_traffic_cylinder256_64shots.json"}
-FULL_SHOTS = {"C2-256": 256, "C2": 128, "C3": 256, "C4": 64}
+FULL_SHOTS = {"C1-16": 16, "C2-256": 256, "C2": 128, "C3": 256, "C4": 64}
 
 
 def serving(n_requests, nt_scale, rate=150.0, seed=5, with_3d=True):

@@ -2,6 +2,7 @@
 
 C1  2D ring 256^2, dx 0.2 mm, dt 50 ns, nt 1000, 64 elements R = 22 mm, shots
     every 8th element (8), 3 seeded inclusions (default_rng(1)).
+C1-16  the same with 16 shots (every 4th element).
 C2  2D ring 512^2, dx 0.2 mm, dt 50 ns, nt 1000, 128 elements, 128 shots, seed 2.
     (dx = 0.1 mm with dt = 50 ns violates the 2D CFL limit, so the grid keeps
     C1's spacing and doubles the domain.)
@@ -70,6 +71,12 @@ def c1(nt_scale=1.0, shots=None):
     return _ring_problem("C1", 64, 0.022, 0.051, 2e-4, 5e-8, 1000, 1, 8, 16.0, nt_scale, shots)
 
 
+def c1_16(nt_scale=1.0, shots=None):
+    """The 256^2 problem with 16 shots (every 4th element): a small request, where
+    the per-launch cost of a gradient's ~1000 launches is largest."""
+    return _ring_problem("C1-16", 64, 0.022, 0.051, 2e-4, 5e-8, 1000, 1, 4, 16.0, nt_scale, shots)
+
+
 def c2(nt_scale=1.0, shots=None):
     return _ring_problem("C2", 128, 0.022, 0.1022, 2e-4, 5e-8, 1000, 2, 1, 16.0, nt_scale, shots)
 
@@ -104,4 +111,4 @@ def c4(nt_scale=1.0, shots=None, n=256):
                    np.full((geom.n,) * 3, 1.0 / 1500.0 ** 2), src, ckpt_every=50)
 
 
-BUILDERS = {"C1": c1, "C2": c2, "C2-256": c2_256, "C3": c3, "C4": c4}
+BUILDERS = {"C1": c1, "C1-16": c1_16, "C2": c2, "C2-256": c2_256, "C3": c3, "C4": c4}

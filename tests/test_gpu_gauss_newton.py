@@ -197,3 +197,39 @@ def test_gauss_newton_beats_gradient_descent():
     print(f"[case] 4 iterations: GD misfit -> {red_gd * 100:.2f}% | GN -> {red_gn * 100:.3f}%")
     assert red_gn < red_gd
     assert red_gn < 0.05
+
+
+def test_lazy_buffers_are_held_to_the_plan_budget():
+    """A plan whose budget fits the gradient but not the Born state (or the time
+    windows' traces) refuses the Gauss-Newton product (and the windowed call) with
+    MemoryError before allocating; gradients on it keep working, and a plan with
+    room runs the same product.  The 256^2 problem, 8 shots, whole history kept:
+    the Born state needs 24 B per shot-cell (25 MB), the windows 8 B per trace
+    sample (8 MB), and the tight budget leaves 4 MB beyond the plan."""
+    from openusct_amd import configs
+    from openusct_amd.plan import Plan, linear_indices
+    prob = configs.c1()
+    shape, h, dt, nt = prob.shape, prob.h, prob.dt, prob.nt
+    rec = linear_indices(prob.geom.idx, shape)
+    tx = rec[prob.src_list]
+    dev = torch.device("cuda")
+    md, wd = torch.tensor(prob.m0, device=dev), torch.tensor(prob.wavelet, device=dev)
+    mt = torch.tensor(prob.m_true, device=dev)
+    kw = dict(autotune=False, ckpt_every=nt - 1)
+    with Plan(shape, h, dt, nt, tx, rec, **kw) as free:
+        held = free.info["workspace_bytes"]
+        dobs = free.forward_fmc(mt, wd)
+        v = torch.ones_like(md) * md.mean()
+        hv = free.gn_hessian_vector(md, wd, v)
+    budget = held + (4 << 20)
+    print(f"[case] 256^2, 8 shots: plan of {held} bytes, budget {budget}", flush=True)
+    with Plan(shape, h, dt, nt, tx, rec, mem_budget=budget, **kw) as tight:
+        assert tight.info["workspace_bytes"] == held
+        with pytest.raises(MemoryError, match="Gauss-Newton"):
+            tight.gn_hessian_vector(md, wd, v)
+        with pytest.raises(MemoryError, match="time-window"):
+            tight.misfit(md, wd, dobs, time_weights=torch.ones_like(dobs))
+        assert tight.info["workspace_bytes"] == held
+        J, g = tight.misfit_and_gradient(md, wd, dobs)
+        assert J.item() > 0.0 and torch.isfinite(g).all()
+    assert torch.isfinite(hv).all() and hv.abs().max().item() > 0.0

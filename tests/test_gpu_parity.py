@@ -400,3 +400,53 @@ def test_shell_array_3d_vs_oracle(hemisphere):
     e = (rel_l2(d, dobs), abs(J - J_ref) / J_ref, rel_l2(g, g_ref))
     print(f"traces {e[0]:.2e} J {e[1]:.2e} gradient {e[2]:.2e}")
     assert max(e) < 1e-5
+
+
+def _cylinder21(nt):
+    """smoke()'s 3D problem: a 21^3 cylinder (2 rings x 6 elements, h 1.5 mm,
+    300 kHz Ricker, 2 shots) whose time axis ends soon after the scattered
+    arrivals reach the elements."""
+    from openusct_amd import phantom
+    from openusct_amd.geometry import CylinderArray
+    from openusct_amd.sources import ricker
+    h, dt = 1.5e-3, 1.5e-7
+    cyl = CylinderArray(n_rings=2, per_ring=6, radius_m=0.012, height_m=0.009, domain_m=0.03, h=h)
+    n = cyl.n
+    w = ricker(nt, dt, 0.3e6)
+    c = phantom.cylinder_background((n,) * 3, 3000.0, 1480.0, 0.009, h)
+    mt = phantom.velocity_to_m(phantom.add_sphere(c, (0.5, 0.5, 0.5), 0.004, 2700.0, h))
+    return cyl, h, dt, w, mt, phantom.velocity_to_m(c), [0, 7]
+
+
+def _cylinder21_error(nt, reverse):
+    from oracle import oracle as O
+    from openusct_amd.plan import Plan, linear_indices
+    cyl, h, dt, w, mt, m0, src = _cylinder21(nt)
+    d = O.forward_fmc(mt, cyl, w, dt, h, nt, None, src)
+    J_ref, g_ref = O.misfit_and_gradient(m0, cyl, w, dt, h, nt, d, None, src)
+    rec = linear_indices(cyl.idx, m0.shape)
+    dev = torch.device("cuda")
+    with Plan(m0.shape, h, dt, nt, rec[src], rec, reverse=reverse) as p:
+        print(f"[case] 21^3 cylinder nt={nt} reverse={reverse} {p.info}", flush=True)
+        J, g = p.misfit_and_gradient(torch.tensor(m0, device=dev), torch.tensor(w, device=dev),
+                                     torch.tensor(d, device=dev))
+    e = (abs(J.item() - J_ref) / J_ref, rel_l2(g.cpu().numpy(), g_ref))
+    print(f"21^3 cylinder nt={nt} reverse={reverse}: J {e[0]:.2e}  gradient {e[1]:.2e}")
+    return e
+
+
+@pytest.mark.parametrize("reverse", [True, False])
+def test_short_axis_cylinder_200_levels(reverse):
+    """Regression bound of the FP32 hi + bf16 lo forward state on a short time
+    axis (2.2e-6 when measured), with and without the forward run backwards."""
+    e = _cylinder21_error(200, reverse)
+    assert e[0] < TOL and e[1] < TOL
+
+
+@pytest.mark.skip(reason="known limit of the FP32 hi + bf16 lo forward state: the gradient is "
+                         "1.9e-5 from the FP64 oracle on this 120-level axis (the traces' 4e-10 "
+                         "rounding is 3e-7 of the small residual, amplified ~60x by the adjoint)")
+@pytest.mark.parametrize("reverse", [True, False])
+def test_short_axis_cylinder_120_levels(reverse):
+    e = _cylinder21_error(120, reverse)
+    assert e[0] < TOL and e[1] < TOL
