@@ -95,6 +95,9 @@ typedef struct {
                                  advancing two time levels per launch */
     int reverse;              /* 1: the gradient keeps no forward levels; the adjoint sweep
                                  runs the forward backwards (3D, no sponge, point elements) */
+    int sched;                /* 2D two-level adjoint kernel: row-chunk order and march
+                                 (bit 0: odd chunks march upwards; bit 1: chunk index fastest) */
+    int fwd_sched;            /* the same for the two-level forward kernel */
 } usct_plan_info;
 
 /*

@@ -51,6 +51,8 @@ class PlanInfo(ctypes.Structure):
         ("tile_kind", ctypes.c_int),
         ("fwd_tile_kind", ctypes.c_int),
         ("reverse", ctypes.c_int),
+        ("sched", ctypes.c_int),
+        ("fwd_sched", ctypes.c_int),
     ]
 
     def as_dict(self):

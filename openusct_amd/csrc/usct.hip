@@ -79,6 +79,8 @@ struct Tiling {
     int lds = 0;         // 3D: 1 = LDS-staged work-group tiles of CX x CY cells (patch per
                          // lane from lds_patch), rows = planes per work-group; 2 = the same
                          // tiles advancing two forward levels per launch
+    int sched = 0;       // 2D two-level kernels: bit 0 = odd row chunks march upwards,
+                         // bit 1 = row chunk fastest in the wavefront order
 };
 
 // Lane patch of an LDS tile: 256 lanes = LB shots x (TX/PX) x (TY/PY) patches.
@@ -394,7 +396,7 @@ void fwd2d_x2_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usc
     hipLaunchKernelGGL((k_fwd2d_x2<LB, CX, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pa.hi,
                        pb.hi, pc.lo, pp.lo, pa.lo, pb.lo, P->sp2, P->src_cell, sa, sa + P->Bp,
                        P->rec_head, P->rec_next, P->row_rec, ra, rla, ra + tr, rla + tr, P->ny,
-                       P->nx, P->Bp, P->tf.RY, t.nxs, t.nyc, P->Bp / P->tf.LB);
+                       P->nx, P->Bp, P->tf.RY, t.nxs, t.nyc, P->Bp / P->tf.LB, P->tf.sched);
 }
 
 template <int LB, bool REC>
@@ -495,7 +497,7 @@ void adj2d_x2_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *
     hipLaunchKernelGGL((k_adj2d_x2<LB, CX>), g, dim3(kBlock), 0, s, l1, l2, lu, lv, pa1, pa2,
                        P->sp2, P->src_cell, su, su - P->Bp, P->rec_head, P->rec_next, P->row_rec,
                        ru, ru - tr, P->g_hi, P->g_lo, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nyc,
-                       P->ng_adj());
+                       P->ng_adj(), P->ta.sched);
 }
 
 void launch_adj2_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
@@ -1472,14 +1474,36 @@ int autotune(usct_plan *P) {
     std::vector<int> lbs;
     for (int c : {64, 32, 16})
         if (P->Bp % c == 0) lbs.push_back(c);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
     const char *log = std::getenv("USCT_TUNE_LOG");  // print every candidate's time
     for (int adjoint = 0; adjoint < 2; ++adjoint) {
         if (adjoint && P->Wslot.empty()) break;  // forward-only plan
         std::vector<Tiling> cands;
         if (P->ndim == 2) {
+            // the two-level kernels also try the chunk orders and march directions
+            const bool x2 = adjoint ? fused_adj(P) : fused_fwd(P);
             for (int lb : lbs)
-                for (int cx : {4, 8})
-                    for (int ry : {4, 8, 16, 32, 64}) cands.push_back({lb, cx, 1, ry, 0});
+                for (int cx : {4, 8}) {
+                    // rows per wavefront: powers of two, and the counts that make the
+                    // grid one, two or three whole rounds of resident wavefronts
+                    // (3 per SIMD), so that no round runs part-empty
+                    std::vector<int> rys{4, 8, 16, 32, 64};
+                    const int64_t per_chunk =
+                        (P->Bp / lb) * cdiv(P->nx, (int64_t)(kWave / lb) * cx);
+                    for (int k = 1; k <= 3; ++k) {
+                        const int64_t nyc =
+                            std::max<int64_t>(1, cdiv((int64_t)k * cus * 4 * 3, per_chunk));
+                        const int ry = (int)cdiv(P->ny, nyc);
+                        if (ry >= 2 && std::find(rys.begin(), rys.end(), ry) == rys.end())
+                            rys.push_back(ry);
+                    }
+                    for (int ry : rys)
+                        for (int sc = 0; sc < (x2 ? 4 : 1); ++sc)
+                            cands.push_back({lb, cx, 1, ry, 0, sc});
+                }
         } else if (adjoint && P->rev) {
             for (int lb : lbs)
                 for (auto pc : {std::pair<int, int>{2, 2}, {2, 1}, {1, 2}, {1, 1}})
@@ -1513,10 +1537,10 @@ int autotune(usct_plan *P) {
             T = c;
             const double t = time_tile(P, adjoint != 0);
             if (log && *log == '1')
-                std::fprintf(stderr, "[usct tune] %s lanes %d %s %dx%d rows %d: %.3f ms\n",
+                std::fprintf(stderr, "[usct tune] %s lanes %d %s %dx%d rows %d sched %d: %.3f ms\n",
                              adjoint ? "adjoint" : "forward", c.LB,
                              c.lds == 2 ? "lds-tile-x2" : (c.lds ? "lds-tile" : "patch"),
-                             c.CX, c.CY, c.RY, t);
+                             c.CX, c.CY, c.RY, c.sched, t);
             if (t < best) { best = t; bt = c; }
         }
         T = bt;
@@ -2090,6 +2114,8 @@ int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
     info->n_windows = P->rev ? 0 : P->NS;
     info->workspace_bytes = P->bytes;
     info->reverse = P->rev ? 1 : 0;
+    info->sched = P->ta.sched;
+    info->fwd_sched = P->tf.sched;
     return 0;
 }
 

@@ -399,9 +399,38 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 //   rec_a / rec_b: trace rows of levels n and n+1.
 // Halo columns read one or two cells beyond the strip; the guard zones of the
 // operands cover the reads before the first and after the last row.
+//
+// Row chunks and their order (sched, wave-uniform):
+//   bit 0  odd row chunks march upwards (from their last row to their first),
+//          so that two neighbouring chunks reach the rows they both read (the
+//          halo rows of their common boundary) at the same time, while those
+//          rows are still in L2: downward chunks meet at their ends, upward
+//          ones at their starts;
+//   bit 1  consecutive wavefronts take consecutive row chunks of one strip
+//          (chunk index fastest) instead of consecutive strips of one chunk,
+//          so that vertically neighbouring chunks run on the same XCD.
+// Only the order in which rows are visited changes: every value is formed
+// from the same operands in the same order, so the results do not depend on
+// sched.  Rows are named relative to the front row yf and the march
+// direction dir: row(r) = yf + r * dir; "ahead" rows are r > 0.
 // ---------------------------------------------------------------------------
 // Row index clamped to the grid (halo rows of a wavefront read the edge row).
 __device__ __forceinline__ size_t clamp_row(int y, int ny) { return (size_t)min(max(y, 0), ny - 1); }
+
+// Strip and row chunk of a 2D wavefront task t (shot groups excluded), with the
+// march direction; false for the tasks that round the grid up to whole blocks.
+__device__ __forceinline__ bool chunk2d(int t, int nxs, int nyc, int sched, int &xs, int &yc,
+                                        int &dir) {
+    if (sched & 2) {
+        yc = t % nyc;
+        xs = t / nyc;
+    } else {
+        xs = t % nxs;
+        yc = t / nxs;
+    }
+    dir = ((sched & 1) && (yc & 1)) ? -1 : 1;
+    return xs < nxs && yc < nyc;
+}
 
 template <int LB, int CX, bool REC>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR2D
@@ -414,15 +443,14 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
            const int *__restrict__ rec_next, const int *__restrict__ row_rec,
            float *__restrict__ rec_a, float *__restrict__ rec_lo_a, float *__restrict__ rec_b,
            float *__restrict__ rec_lo_b, int ny, int nx, int Bp, int RY, int nxs, int nyc,
-           int ng) {
+           int ng, int sched) {
     constexpr int W = kWave / LB;
     constexpr int CA = CX + 2;  // A columns x0-1 .. x0+CX
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
-    const int t = wv / ng;
-    const int xs = t % nxs, yc = t / nxs;
-    if (yc >= nyc) return;
+    int xs, yc, dir;
+    if (!chunk2d(wv / ng, nxs, nyc, sched, xs, yc, dir)) return;
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -435,18 +463,21 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
     const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
     const unsigned oc = (unsigned)x0 * 8u;
+    // first front row: one halo row outside the chunk, on the side the march
+    // starts from; rows y0-1 .. y1 are visited (RY + 2 fronts)
+    const int yfs = dir > 0 ? y0 - 1 : y1;
+    auto rowE_at = [&](int yf, int r) { return clamp_row(yf + r * dir, ny) * rowE; };
     // Full state (FP64) around the front row yf (index j <-> column x0-2+j for
-    // uc/un, x0-1+j for um): um = pc row yf-1, uc = yf, un = yf+1.  un and q
-    // (level n-2 on row yf, columns x0-1 ..) come from raw loads issued one
+    // uc/un, x0-1+j for um): um = pc row(-1), uc = row(0), un = row(+1).  un and
+    // q (level n-2 on row(0), columns x0-1 ..) come from raw loads issued one
     // iteration ahead (nh/nl, qh/ql).
     double um[CA], uc[CA + 2], un[CA + 2];
     float nh[CA + 2], qh[CA];
     uint16_t nl[CA + 2], ql[CA];
-    // A rows yf-2, yf-1, yf (index j <-> column x0-1+j), as stored
+    // A rows(-2), (-1), (0) (index j <-> column x0-1+j), as stored
     double am[CA], ac[CA], an[CA];
     {
-        const size_t r0 = clamp_row(y0 - 2, ny) * rowE, r1 = clamp_row(y0 - 1, ny) * rowE;
-        const size_t r2 = clamp_row(y0, ny) * rowE;
+        const size_t r0 = rowE_at(yfs, -1), r1 = rowE_at(yfs, 0), r2 = rowE_at(yfs, 1);
 #pragma unroll
         for (int j = 0; j < CA; ++j) {
             const ptrdiff_t c = (ptrdiff_t)(j - 1) * Bp;
@@ -464,17 +495,18 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
 #pragma unroll
         for (int j = 0; j < CA; ++j) { am[j] = 0.0; ac[j] = 0.0; an[j] = 0.0; }
     }
-    for (int yf = y0 - 1; yf <= y1; ++yf) {
+    const int n_front = y1 - y0 + 2;
+    for (int it = 0, yf = yfs; it < n_front; ++it, yf += dir) {
         const int yr = (int)clamp_row(yf, ny);  // row actually read for halo rows
 #pragma unroll
         for (int j = 0; j < CA + 2; ++j) un[j] = hl(nh[j], nl[j]);
         double q[CA];
 #pragma unroll
         for (int j = 0; j < CA; ++j) q[j] = hl(qh[j], ql[j]);
-        // ---- loads for the next iteration: pc row yf+2, pp row yf+1
+        // ---- loads for the next iteration: pc row(+2), pp row(+1)
         {
-            const size_t rnn = clamp_row(yf + 2, ny) * rowE;
-            const size_t en = clamp_row(yf + 1, ny) * rowE;
+            const size_t rnn = rowE_at(yf, 2);
+            const size_t en = rowE_at(yf, 1);
 #pragma unroll
             for (int j = 0; j < CA + 2; ++j) {
                 const ptrdiff_t c = (ptrdiff_t)(j - 2) * Bp;
@@ -498,7 +530,12 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
         for (int j = 0; j < CA; ++j) {
             const int x = x0 - 1 + j;
             double lu = 0.0;
-            if (yint && x >= 1 && x <= nx - 2) lu = lap5d(uc[j + 1], un[j + 1], um[j], uc[j + 2], uc[j]);
+            if (yint && x >= 1 && x <= nx - 2) {
+                // rows y+1 and y-1 of level n-1
+                const double yp = dir > 0 ? un[j + 1] : um[j];
+                const double ym = dir > 0 ? um[j] : un[j + 1];
+                lu = lap5d(uc[j + 1], yp, ym, uc[j + 2], uc[j]);
+            }
             double inc = s2d(ldb(spr + (j - 1), oc)) * lu;
             if (yf * nx + x == my_src) inc = inc + inc_a;
             const double nv = (2.0 * uc[j + 1] - q[j]) + inc;
@@ -519,8 +556,8 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
                 }
             }
         }
-        // ---- level n+1 on row y = yf-1, own columns
-        const int y = yf - 1;
+        // ---- level n+1 on row y = row(-1), own columns
+        const int y = yf - dir;
         if (y >= y0 && y < y1) {
             const bool yint_b = y >= 1 && y <= ny - 2;
             const size_t rob = (size_t)y * rowE;
@@ -531,8 +568,11 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
                 const int x = x0 + i;
                 if (x < nx) {
                     double lu = 0.0;
-                    if (yint_b && x >= 1 && x <= nx - 2)
-                        lu = lap5d(ac[i + 1], an[i + 1], am[i + 1], ac[i + 2], ac[i]);
+                    if (yint_b && x >= 1 && x <= nx - 2) {
+                        const double yp = dir > 0 ? an[i + 1] : am[i + 1];
+                        const double ym = dir > 0 ? am[i + 1] : an[i + 1];
+                        lu = lap5d(ac[i + 1], yp, ym, ac[i + 2], ac[i]);
+                    }
                     double inc = s2d(ldb(spb + i, oc)) * lu;
                     if (y * nx + x == my_src) inc = inc + inc_b;
                     const double nv = (2.0 * ac[i + 1] - um[i + 1]) + inc;
@@ -923,6 +963,8 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 // so fields and gradient are bit-identical to those launches.  Per shot-cell
 // the pair reads 16 bytes and writes 8 instead of reading 24 and writing 8.
 //   sinc_u/sinc_v: S_src w[k-1], S_src w[k-2]; res_u/res_v: residual rows k, k-1.
+// Row chunks, their order and march direction: sched, as for k_fwd2d_x2 (V
+// trails U by one row in the direction of the march).
 // ---------------------------------------------------------------------------
 template <int LB, int CX>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR2D
@@ -933,15 +975,14 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
            const int *__restrict__ rec_head, const int *__restrict__ rec_next,
            const int *__restrict__ row_rec, const float *__restrict__ res_u,
            const float *__restrict__ res_v, float *__restrict__ g_hi, float *__restrict__ g_lo,
-           int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng) {
+           int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng, int sched) {
     constexpr int W = kWave / LB;
     constexpr int CA = CX + 2;  // U columns x0-1 .. x0+CX
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
-    const int t = wv / ng;
-    const int xs = t % nxs, yc = t / nxs;
-    if (yc >= nyc) return;  // uniform over the wavefront
+    int xs, yc, dir;
+    if (!chunk2d(wv / ng, nxs, nyc, sched, xs, yc, dir)) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -953,25 +994,28 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
     const float inc_u = sinc_u[b], inc_v = sinc_v[b];
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
     const unsigned oc = (unsigned)x0 * 8u;
-    // Registers at the top of the iteration for front row yf (index j <-> column
-    // x0-2+j for the CA+2-wide rows, x0-1+j for CA-wide, x0+j for CX-wide):
-    //   qm, qc       S' lam^{k+1} rows yf-1 (CA), yf (CA+2)
-    //   lc           lam^{k+1} row yf (CA)
-    //   lr           lam^{k+1} row yf+1 (CA+2), loaded one iteration ahead
-    //   l2c          lam^{k+2} row yf (CA), loaded one iteration ahead
-    //   fm, fc, fr   p^{k-1} rows yf-1 (CX), yf (CX+2), yf+1 (CX+2, ahead)
-    //   hm, hc, hr   p^{k-2} rows yf-2 (CX), yf-1 (CX+2), yf (CX+2, ahead)
-    //   wm, wc, wn   S' U rows yf-2, yf-1, yf-... (CA; shifted as U is made)
-    //   l1p, up      row yf-1, own columns: lam^{k+1} and U
+    const int yfs = dir > 0 ? y0 - 1 : y1;  // first front row (as k_fwd2d_x2)
+    auto row_at = [&](int yf, int r) { return clamp_row(yf + r * dir, ny); };
+    // Registers at the top of the iteration for front row yf, rows named
+    // relative to it (row(r) = yf + r * dir; index j <-> column x0-2+j for the
+    // CA+2-wide rows, x0-1+j for CA-wide, x0+j for CX-wide):
+    //   qm, qc       S' lam^{k+1} rows (-1) (CA), (0) (CA+2)
+    //   lc           lam^{k+1} row (0) (CA)
+    //   lr           lam^{k+1} row (+1) (CA+2), loaded one iteration ahead
+    //   l2c          lam^{k+2} row (0) (CA), loaded one iteration ahead
+    //   fm, fc, fr   p^{k-1} rows (-1) (CX), (0) (CX+2), (+1) (CX+2, ahead)
+    //   hm, hc, hr   p^{k-2} rows (-2) (CX), (-1) (CX+2), (0) (CX+2, ahead)
+    //   wm, wc, wn   S' U rows (-2), (-1), (0) (CA; shifted as U is made)
+    //   l1p, up      row (-1), own columns: lam^{k+1} and U
     float qm[CA], qc[CA + 2], lc[CA], lr[CA + 2], l2c[CA];
     float fm[CX], fc[CX + 2], fr[CX + 2];
     float hm[CX], hc[CX + 2], hr[CX + 2];
     float wm[CA], wc[CA], wn[CA];
     float l1p[CX], up[CX];
-    float s_prev = 0.0f;  // U imaging sum of row yf-1 (this lane's cell)
+    float s_prev = 0.0f;  // U imaging sum of row (-1) (this lane's cell)
     {
-        const size_t r0 = clamp_row(y0 - 2, ny), r1 = clamp_row(y0 - 1, ny);
-        const size_t r2 = clamp_row(y0, ny);
+        const size_t r0 = row_at(yfs, -1), r1 = row_at(yfs, 0);
+        const size_t r2 = row_at(yfs, 1);
         const float *a0 = l1 + r0 * rowE, *a1 = l1 + r1 * rowE, *a2 = l1 + r2 * rowE;
         const float2 *s0 = sp2 + r0 * nx, *s1 = sp2 + r1 * nx;
 #pragma unroll
@@ -986,7 +1030,7 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
             if (j >= 1 && j <= CA) lc[j - 1] = v;
             lr[j] = ldb(a2 + (ptrdiff_t)(j - 2) * Bp, o4);
         }
-        const float *h0 = pa2 + clamp_row(y0 - 3, ny) * rowE;
+        const float *h0 = pa2 + row_at(yfs, -2) * rowE;
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
             fm[i] = ldb(pa1 + r0 * rowE + (ptrdiff_t)i * Bp, o4);
@@ -1010,19 +1054,22 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
     const bool gval = (sb & (LB / CX - 1)) == 0 && gx < nx;
     float *const ghb = g_hi + (size_t)sg * N + gx;
     float *const glb = g_lo + (size_t)sg * N + gx;
-    for (int yf = y0 - 1; yf <= y1; ++yf) {
-        const size_t yr = clamp_row(yf, ny), yn = clamp_row(yf + 1, ny);
+    const int n_front = y1 - y0 + 2;
+    for (int it = 0, yf = yfs; it < n_front; ++it, yf += dir) {
+        const size_t yr = clamp_row(yf, ny), yn = row_at(yf, 1);
+        const int y = yf - dir;  // row (-1): the V level's row
         float gh0 = 0.0f, gl0 = 0.0f;
-        if (gval && yf - 1 >= y0 && yf - 1 < y1) {
-            gh0 = ghb[(size_t)(yf - 1) * nx];
-            gl0 = glb[(size_t)(yf - 1) * nx];
+        if (gval && y >= y0 && y < y1) {
+            gh0 = ghb[(size_t)y * nx];
+            gl0 = glb[(size_t)y * nx];
         }
         // ---- loads for the next iteration
         float ln2[CA + 2], l2n[CA], fn2[CX + 2], hn[CX + 2];
         {
-            const float *a = l1 + clamp_row(yf + 2, ny) * rowE;
+            const size_t r2 = row_at(yf, 2);
+            const float *a = l1 + r2 * rowE;
             const float *c = l2 + yn * rowE;
-            const float *f = pa1 + clamp_row(yf + 2, ny) * rowE;
+            const float *f = pa1 + r2 * rowE;
             const float *h = pa2 + yn * rowE;
 #pragma unroll
             for (int j = 0; j < CA + 2; ++j) ln2[j] = ldb(a + (ptrdiff_t)(j - 2) * Bp, o4);
@@ -1034,7 +1081,7 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
                 hn[i] = ldb(h + (ptrdiff_t)(i - 1) * Bp, o4);
             }
         }
-        // ---- S' lam^{k+1} on row yf+1
+        // ---- S' lam^{k+1} on row (+1)
         float qn[CA + 2];
         {
             const float2 *snr = sp2 + yn * nx;
@@ -1051,7 +1098,10 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
         for (int j = 0; j < CA; ++j) {
             const int x = x0 - 1 + j;
             const bool in = yint && x >= 1 && x <= nx - 2;
-            const float lq = in ? lap5(qc[j + 1], qn[j + 1], qm[j], qc[j + 2], qc[j]) : 0.0f;
+            // rows y+1 and y-1 of S' lam^{k+1}
+            const float yp = dir > 0 ? qn[j + 1] : qm[j];
+            const float ym = dir > 0 ? qm[j] : qn[j + 1];
+            const float lq = in ? lap5(qc[j + 1], yp, ym, qc[j + 2], qc[j]) : 0.0f;
             float v = leap(lc[j], l2c[j], lq);
             if (rec_u && x >= 0 && x < nx) {
                 for (int r = rec_head[(int)yr * nx + x]; r >= 0; r = rec_next[r])
@@ -1073,7 +1123,9 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
                 tv[i] = 0.0f;
                 if (x < nx) {
                     const bool in = yint && x >= 1 && x <= nx - 2;
-                    const float lp = in ? lap5(fc[i + 1], fr[i + 1], fm[i], fc[i + 2], fc[i]) : 0.0f;
+                    const float yp = dir > 0 ? fr[i + 1] : fm[i];
+                    const float ym = dir > 0 ? fm[i] : fr[i + 1];
+                    const float lp = in ? lap5(fc[i + 1], yp, ym, fc[i + 2], fc[i]) : 0.0f;
                     float d2 = smul(ldb(spr + i, oc), lp);
                     if ((int)yr * nx + x == my_src) d2 = d2 + inc_u;
                     tv[i] = uo[i] * d2;
@@ -1081,8 +1133,7 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
             }
             s_u = reduce_scatter<LB, CX>(tv, lane);
         }
-        // ---- V = lam^{k-1} on row y = yf-1, own columns
-        const int y = yf - 1;
+        // ---- V = lam^{k-1} on row y = row(-1), own columns
         if (y >= y0 && y < y1) {
             const bool yint_v = y >= 1 && y <= ny - 2;
             const size_t rov = (size_t)y * rowE;
@@ -1094,14 +1145,18 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
                 tv[i] = 0.0f;
                 if (x < nx) {
                     const bool in = yint_v && x >= 1 && x <= nx - 2;
-                    const float lw = in ? lap5(wc[i + 1], wn[i + 1], wm[i + 1], wc[i + 2], wc[i]) : 0.0f;
+                    const float wp = dir > 0 ? wn[i + 1] : wm[i + 1];
+                    const float wq = dir > 0 ? wm[i + 1] : wn[i + 1];
+                    const float lw = in ? lap5(wc[i + 1], wp, wq, wc[i + 2], wc[i]) : 0.0f;
                     float v = leap(up[i], l1p[i], lw);
                     if (rec_v) {
                         for (int r = rec_head[y * nx + x]; r >= 0; r = rec_next[r])
                             v = v - res_v[(size_t)r * Bp + b];
                     }
                     stn<USCT_NT2D>(lv + rov + (size_t)i * Bp, o4, v);
-                    const float lp = in ? lap5(hc[i + 1], hr[i + 1], hm[i], hc[i + 2], hc[i]) : 0.0f;
+                    const float hp = dir > 0 ? hr[i + 1] : hm[i];
+                    const float hq = dir > 0 ? hm[i] : hr[i + 1];
+                    const float lp = in ? lap5(hc[i + 1], hp, hq, hc[i + 2], hc[i]) : 0.0f;
                     float d2 = smul(ldb(spv + i, oc), lp);
                     if (y * nx + x == my_src) d2 = d2 + inc_v;
                     tv[i] = v * d2;

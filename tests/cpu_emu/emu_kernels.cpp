@@ -188,11 +188,11 @@ bool check_fwd_direct(int ny, int nx, int ng, int RY, uint32_t seed) {
 }
 
 template <int LB, int CX, bool REC>
-bool check_fwd(int ny, int nx, int ng, int RY, uint32_t seed) {
+bool check_fwd(int ny, int nx, int ng, int RY, uint32_t seed, int sched) {
     Problem P(ny, nx, LB, ng, CX, RY, seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, "fwd ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d rec=%d", ny, nx, LB, ng,
-                  CX, RY, (int)REC);
+    std::snprintf(tag, sizeof tag, "fwd ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d rec=%d sched=%d", ny, nx,
+                  LB, ng, CX, RY, (int)REC, sched);
     float *pc = P.field(1.0f), *pp = P.field(1.0f);
     uint16_t *pcl = P.low(1e-7f), *ppl = P.low(1e-7f);
     float *a1 = P.zeros(), *b1 = P.zeros(), *a2 = P.zeros(), *b2 = P.zeros();
@@ -213,7 +213,7 @@ bool check_fwd(int ny, int nx, int ng, int RY, uint32_t seed) {
     emu::launch(P.blocks, kBlock, [&] {
         k_fwd2d_x2<LB, CX, REC>(pc, pp, a2, b2, pcl, ppl, a2l, b2l, P.sp2, P.src_cell, sa, sb,
                                 P.rec_head, P.rec_next, P.row_rec, ra2, ral2, rb2, rbl2, P.ny,
-                                P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+                                P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng, sched);
     });
     const int64_t nr = (int64_t)P.n_rec * P.Bp;
     bool ok = same("level n", a1, a2, P.FE, tag) & same("level n low", a1l, a2l, P.FE, tag) &
@@ -229,22 +229,23 @@ template <int LB, int CX>
 bool fwd_cases(uint32_t seed) {
     bool ok = true;
     const int shapes[][2] = {{29, 37}, {9, 13}, {40, 33}, {3, 5}};
+    // every row-chunk order and march direction (sched 0..3)
     for (auto &s : shapes)
-        for (int RY : {4, 8, 64}) {
-            ok &= check_fwd<LB, CX, true>(s[0], s[1], 2, RY, seed++);
-        }
-    ok &= check_fwd<LB, CX, false>(29, 37, 1, 4, seed++);
+        for (int RY : {4, 8, 64})
+            for (int sched = 0; sched < 4; ++sched)
+                ok &= check_fwd<LB, CX, true>(s[0], s[1], 2, RY, seed++, sched);
+    ok &= check_fwd<LB, CX, false>(29, 37, 1, 4, seed++, 3);
     ok &= check_fwd_direct<LB, CX, false>(29, 37, 2, 8, seed++);
     ok &= check_fwd_direct<LB, CX, true>(9, 13, 1, 4, seed++);
     return ok;
 }
 
 template <int LB, int CX>
-bool check_adj(int ny, int nx, int ng, int RY, uint32_t seed) {
+bool check_adj(int ny, int nx, int ng, int RY, uint32_t seed, int sched) {
     Problem P(ny, nx, LB, ng, CX, RY, seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, "adj ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d", ny, nx, LB, ng, CX,
-                  RY);
+    std::snprintf(tag, sizeof tag, "adj ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d sched=%d", ny, nx, LB,
+                  ng, CX, RY, sched);
     float *l1 = P.field(1.0f), *l2 = P.field(1.0f), *pa1 = P.field(1.0f), *pa2 = P.field(1.0f);
     float *u1 = P.zeros(), *v1 = P.zeros(), *u2 = P.zeros(), *v2 = P.zeros();
     const int64_t nr = (int64_t)P.n_rec * P.Bp;
@@ -274,7 +275,7 @@ bool check_adj(int ny, int nx, int ng, int RY, uint32_t seed) {
     emu::launch(P.blocks, kBlock, [&] {
         k_adj2d_x2<LB, CX>(l1, l2, u2, v2, pa1, pa2, P.sp2, P.src_cell, su, sv, P.rec_head,
                            P.rec_next, P.row_rec, ru, rv, gh2, gl2, P.ny, P.nx, P.Bp, P.RY, P.nxs,
-                           P.nyc, P.ng);
+                           P.nyc, P.ng, sched);
     });
     const bool ok = same("lam^k", u1, u2, P.FE, tag) & same("lam^{k-1}", v1, v2, P.FE, tag) &
                     same("g_hi", gh1, gh2, gn, tag) & same("g_lo", gl1, gl2, gn, tag);
@@ -287,8 +288,10 @@ bool adj_cases(uint32_t seed) {
     bool ok = true;
     const int shapes[][2] = {{29, 37}, {9, 13}, {40, 33}, {3, 5}};
     for (auto &s : shapes)
-        for (int RY : {4, 8, 64}) ok &= check_adj<LB, CX>(s[0], s[1], 2, RY, seed++);
-    ok &= check_adj<LB, CX>(29, 37, 1, 4, seed++);
+        for (int RY : {4, 8, 64})
+            for (int sched = 0; sched < 4; ++sched)
+                ok &= check_adj<LB, CX>(s[0], s[1], 2, RY, seed++, sched);
+    ok &= check_adj<LB, CX>(29, 37, 1, 4, seed++, 3);
     return ok;
 }
 

@@ -28,16 +28,32 @@ def _cdiv(a, b):
     return -(-a // b)
 
 
-def _extent_2d(nx, ny, Bp, LB, CX, RY, adjoint):
+def _chunk2d(t, nxs, nyc, sched):
+    """(xs, yc, dir) of a 2D wavefront task, as chunk2d in usct_kernels.hpp."""
+    if sched & 2:
+        yc, xs = t % nyc, t // nyc
+    else:
+        xs, yc = t % nxs, t // nxs
+    d = -1 if (sched & 1) and (yc & 1) else 1
+    return xs, yc, d
+
+
+def _extent_2d(nx, ny, Bp, LB, CX, RY, adjoint, sched=0):
     W = 64 // LB
     ng, nxs, nyc = Bp // LB, _cdiv(nx, W * CX), _cdiv(ny, RY)
     lo = hi = 0
-    for wv in range(ng * nxs * nyc):
+    n_waves = _cdiv(ng * nxs * nyc, 4) * 4  # whole blocks of 4 wavefronts
+    for wv in range(n_waves):
         sg, t = wv % ng, wv // ng
-        xs, yc = t % nxs, t // nxs
+        xs, yc, d = _chunk2d(t, nxs, nyc, sched)
+        if xs >= nxs or yc >= nyc:
+            continue  # the kernels return for these wavefronts
         y0 = yc * RY
         y1 = min(y0 + RY, ny)
-        rows = {max(y0 - 1, 0), y0} | {min(y + 1, ny - 1) for y in range(y0, y1)}
+        # every row the march touches: fronts y0-1 .. y1 and up to two rows ahead
+        # and behind them, clamped to the grid (two-level kernels)
+        fronts = range(y0 - 1, y1 + 1)
+        rows = {min(max(f + r * d, 0), ny - 1) for f in fronts for r in (-2, -1, 0, 1, 2)}
         for lane in range(64):
             sb, st = lane % LB, lane // LB
             b, x0 = sg * LB + sb, (xs * W + st) * CX
@@ -50,24 +66,47 @@ def _extent_2d(nx, ny, Bp, LB, CX, RY, adjoint):
     return lo, hi - nx * ny * Bp
 
 
-def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint):
+GY3D = 2  # USCT_GY3D: y strips per x strip taken by consecutive wavefronts
+
+
+def _strip3d(t, nxs, nys, nzc):
+    """(xs, ys, zc, valid) of a 3D wavefront task, as strip3d in usct_kernels.hpp."""
+    q = t % GY3D
+    t //= GY3D
+    xs = t % nxs
+    t //= nxs
+    nyg = _cdiv(nys, GY3D)
+    ys = (t % nyg) * GY3D + q
+    zc = t // nyg
+    return xs, ys, zc, ys < nys and zc < nzc
+
+
+def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint, RZ=4):
     W = 64 // LB
-    ng, nxs, nys = Bp // LB, _cdiv(nx, W * CX), _cdiv(ny, CY)
+    ng, nxs, nys, nzc = Bp // LB, _cdiv(nx, W * CX), _cdiv(ny, CY), _cdiv(nz, RZ)
     lo = hi = 0
-    for xs in range(nxs):
-        for lane in range(64):
-            sb, st = lane % LB, lane // LB
-            for sg in range(ng):
-                b, x0 = sg * LB + sb, (xs * W + st) * CX
-                if not adjoint and x0 >= nx:
-                    continue
-                for z in (0, nz - 1):
-                    for ys in (0, nys - 1):
-                        for j in (-1, CY):
-                            r = min(max(ys * CY + j, 0), ny - 1)
-                            for i in (-1, CX):
-                                e = ((z * ny + r) * nx + x0 + i) * Bp + b
-                                lo, hi = min(lo, e), max(hi, e)
+    n_tasks = nxs * _cdiv(nys, GY3D) * GY3D * nzc
+    n_waves = _cdiv(ng * n_tasks, 4) * 4
+    for wv in range(n_waves):
+        sg, t = wv % ng, wv // ng
+        xs, ys, zc, ok = _strip3d(t, nxs, nys, nzc)
+        if not ok:
+            continue  # the kernels return for these wavefronts
+        z0 = zc * RZ
+        planes = {min(max(z, 0), nz - 1) for z in range(z0 - 2, min(z0 + RZ, nz) + 2)}
+        zs = (min(planes), max(planes))
+        for st in range(W):
+            x0 = (xs * W + st) * CX
+            if not adjoint and x0 >= nx:
+                continue
+            for sb in (0, LB - 1):
+                b = sg * LB + sb
+                for z in zs:
+                    for j in (-1, CY):
+                        r = min(max(ys * CY + j, 0), ny - 1)
+                        for i in (-1, CX):
+                            e = ((z * ny + r) * nx + x0 + i) * Bp + b
+                            lo, hi = min(lo, e), max(hi, e)
     return lo, hi - nx * ny * nz * Bp
 
 
@@ -77,10 +116,12 @@ def test_2d_halo_reads_stay_in_guard(nx, ny):
     for LB in (16, 32, 64):
         for Bp in (LB, 2 * LB):
             for CX in (4, 8):
-                for RY in (4, 64):
+                for RY in (4, 43, 64):
                     for adj in (False, True):
-                        lo, over = _extent_2d(nx, ny, Bp, LB, CX, RY, adj)
-                        assert lo >= -g * Bp and over < g * Bp, (nx, ny, Bp, LB, CX, lo, over)
+                        for sched in range(4):
+                            lo, over = _extent_2d(nx, ny, Bp, LB, CX, RY, adj, sched)
+                            assert lo >= -g * Bp and over < g * Bp, \
+                                (nx, ny, Bp, LB, CX, sched, lo, over)
 
 
 @pytest.mark.parametrize("nx,ny,nz", [(21, 21, 21), (5, 6, 7), (33, 17, 9), (64, 64, 8)])
@@ -88,9 +129,10 @@ def test_3d_halo_reads_stay_in_guard(nx, ny, nz):
     g = _guard_cells()
     for LB in (16, 32, 64):
         for Bp in (LB, 2 * LB):
-            for CX, CY in ((4, 4), (4, 2), (8, 2)):
-                if CX * CY > LB:
-                    continue
+            # every lane patch the tuner may pick (the reversing adjoint's are a subset)
+            for CX, CY in ((4, 4), (4, 2), (8, 2), (2, 2), (2, 4), (2, 1), (1, 2), (1, 1)):
                 for adj in (False, True):
+                    if adj and CX * CY > LB:
+                        continue  # the adjoint reduces a lane's cells over its lane group
                     lo, over = _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adj)
                     assert lo >= -g * Bp and over < g * Bp, (nx, ny, nz, Bp, LB, CX, CY, lo, over)
