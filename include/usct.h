@@ -92,7 +92,9 @@ typedef struct {
     int tile_kind;            /* adjoint, 3D: 0 = cells per lane (tile_x x tile_y),
                                  1 = LDS-staged work-group tile of tile_x x tile_y cells */
     int fwd_tile_kind;        /* the same for the forward kernel; 2 = LDS-staged tiles
-                                 advancing two time levels per launch */
+                                 advancing two time levels per launch; 3 = cells per lane
+                                 advancing two time levels per launch (z marched in
+                                 registers) */
     int reverse;              /* 1: the gradient keeps no forward levels; the adjoint sweep
                                  runs the forward backwards (3D, no sponge, point elements) */
     int sched;                /* 2D two-level adjoint kernel: row-chunk order and march

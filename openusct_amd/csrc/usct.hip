@@ -78,7 +78,8 @@ struct Tiling {
     int LB, CX, CY, RY;  // lanes per shot group, cells per lane (x, y), rows per wavefront
     int lds = 0;         // 3D: 1 = LDS-staged work-group tiles of CX x CY cells (patch per
                          // lane from lds_patch), rows = planes per work-group; 2 = the same
-                         // tiles advancing two forward levels per launch
+                         // tiles advancing two forward levels per launch; 3 = CX x CY lane
+                         // patches advancing two forward levels per launch (k_fwd3d_x2)
     int sched = 0;       // 2D two-level kernels: bit 0 = odd row chunks march upwards,
                          // bit 1 = row chunk fastest in the wavefront order
 };
@@ -376,7 +377,7 @@ void launch_fwd_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl p
             else fwd2d_sel<false, false>(g, s, pc, pp, pn, P, sinc, rec, rl, t);
         }
     } else {
-        if (P->tf.lds) {
+        if (P->tf.lds == 1 || P->tf.lds == 2) {
             if (record) fwd3d_lds_sel<true>(s, pc, pp, pn, P, sinc, rec, rl);
             else fwd3d_lds_sel<false>(s, pc, pp, pn, P, sinc, rec, rl);
         } else {
@@ -425,8 +426,45 @@ void fwd3d_x2_lds_t(hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_pl
                        ng);
 }
 
+// Two 3D forward levels per launch on CX x CY lane patches (register z-march).
+template <int LB, int CX, int CY, bool REC>
+void fwd3d_x2_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_plan *P, int n,
+                const Tile &t) {
+    const double *sa = P->sincd + (int64_t)n * P->Bp;
+    float *da = P->srcd2 + (int64_t)n * P->Bp;
+    const int64_t tr = (int64_t)P->n_rec * P->Bp;
+    float *ra = P->recb + (int64_t)n * tr, *rla = P->recb_lo + (int64_t)n * tr;
+    hipLaunchKernelGGL((k_fwd3d_x2<LB, CX, CY, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pa.hi,
+                       pb.hi, pc.lo, pp.lo, pa.lo, pb.lo, P->sp2, P->src_cell, sa, sa + P->Bp,
+                       P->rec_head, P->rec_next, P->row_rec, ra, rla, ra + tr, rla + tr, da,
+                       da + P->Bp, P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys, t.nyc,
+                       P->Bp / P->tf.LB);
+}
+
+template <int LB, bool REC>
+void fwd3d_x2_lb(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usct_plan *P, int n,
+                 const Tile &t) {
+    if (P->tf.CX == 2 && P->tf.CY == 1) fwd3d_x2_t<LB, 2, 1, REC>(g, s, pc, pp, pa, pb, P, n, t);
+    else if (P->tf.CX == 1 && P->tf.CY == 2) fwd3d_x2_t<LB, 1, 2, REC>(g, s, pc, pp, pa, pb, P, n, t);
+    else fwd3d_x2_t<LB, 1, 1, REC>(g, s, pc, pp, pa, pb, P, n, t);
+}
+
 void launch_fwd2_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb,
                          int n, bool record) {
+    if (P->ndim == 3 && P->tf.lds == 3) {
+        const Tile t = tile_of(P, P->tf);
+        const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+#define USCT_X23(LB_)                                                         \
+    if (record) fwd3d_x2_lb<LB_, true>(g, s, pc, pp, pa, pb, P, n, t);        \
+    else fwd3d_x2_lb<LB_, false>(g, s, pc, pp, pa, pb, P, n, t);
+        switch (P->tf.LB) {
+            case 16: USCT_X23(16) break;
+            case 32: USCT_X23(32) break;
+            default: USCT_X23(64) break;
+        }
+#undef USCT_X23
+        return;
+    }
     if (P->ndim == 3) {
         if (record) fwd3d_x2_lds_t<true>(s, pc, pp, pa, pb, P, n);
         else fwd3d_x2_lds_t<false>(s, pc, pp, pa, pb, P, n);
@@ -453,7 +491,7 @@ bool fusable(const usct_plan *P) {
            !(P->flags & USCT_FLAG_NO_FUSE);
 }
 bool fused_fwd(const usct_plan *P) {
-    return fusable(P) && (P->ndim == 2 || P->tf.lds == 2);
+    return fusable(P) && (P->ndim == 2 || P->tf.lds == 2 || P->tf.lds == 3);
 }
 bool fused_adj(const usct_plan *P) { return fusable(P) && P->ndim == 2; }
 
@@ -1529,6 +1567,12 @@ int autotune(usct_plan *P) {
             // two forward levels per launch on 8 x 8 tiles of 16 shots
             if (!adjoint && P->Bp % 16 == 0 && fusable(P))
                 for (int rz : {8, 16, 32, 64}) cands.push_back({16, 8, 8, rz, 2});
+            // two forward levels per launch on lane patches, marching z in registers
+            if (!adjoint && fusable(P))
+                for (int lb : lbs)
+                    for (auto pc : {std::pair<int, int>{1, 1}, {2, 1}, {1, 2}})
+                        for (int rz : {4, 8, 16, 32, 64})
+                            cands.push_back({lb, pc.first, pc.second, rz, 3});
         }
         Tiling &T = adjoint ? P->ta : P->tf;
         double best = 1e300;
@@ -1539,7 +1583,8 @@ int autotune(usct_plan *P) {
             if (log && *log == '1')
                 std::fprintf(stderr, "[usct tune] %s lanes %d %s %dx%d rows %d sched %d: %.3f ms\n",
                              adjoint ? "adjoint" : "forward", c.LB,
-                             c.lds == 2 ? "lds-tile-x2" : (c.lds ? "lds-tile" : "patch"),
+                             c.lds == 3 ? "patch-x2"
+                                        : (c.lds == 2 ? "lds-tile-x2" : (c.lds ? "lds-tile" : "patch")),
                              c.CX, c.CY, c.RY, c.sched, t);
             if (t < best) { best = t; bt = c; }
         }

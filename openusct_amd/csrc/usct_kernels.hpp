@@ -1347,6 +1347,239 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 }
 
 // ---------------------------------------------------------------------------
+// Batched 3D forward, two levels per launch (temporal blocking along z, no
+// sponge): level n (A) from levels n-1 (P, pc) and n-2 (pp), then level n+1
+// (B) from A and P, both with k_fwd3d's FP64 update.  Each lane owns a CX x CY
+// patch of cells of LB shots and marches planes along z with B one plane
+// behind A.  A is formed on the patch and the cells next to it in x and y (the
+// "A region": what B's stencil needs on B's own plane), P is held on the A
+// region and, on the front plane, on the cells next to the A region (the "P
+// region"); B's planes above and below need A at the patch only.  Halo cells
+// of A are computed with the same operations on the same operands as their
+// owners (source term included), and B reads A as its stored hi + lo, so both
+// levels, their traces and the recorded source increments are bit-identical to
+// two k_fwd3d launches.  Per shot-cell the pair reads 12 bytes and writes 12
+// instead of reading 24 and writing 12; the x and y halo reads of P and of
+// level n-2 come from neighbouring lanes' cells, through L1 and L2.
+// Everything stays in registers: no LDS, no barriers.  The next plane's P
+// region and level n-2 on the A region are loaded one plane ahead.
+//   sincd_a / sincd_b: S_src w[n-1], S_src w[n]; rec_a / rec_b: trace rows of
+//   levels n and n+1; srcd2_a / srcd2_b: increments at each shot's source cell.
+// Rows are clamped to the grid, columns read up to two cells beyond the strip
+// (the guard zones cover them), planes are clamped.
+// ---------------------------------------------------------------------------
+template <int CX, int CY>
+struct Reg3 {
+    // cells relative to the patch origin: the patch (indices 0 .. NO-1), then
+    // the rest of the A region (plus-shaped ring), then the rest of the P region
+    // (corner cells of the ring and the cells two steps out along x and y)
+    static constexpr int NO = CX * CY;
+    static constexpr int NA = NO + 2 * (CX + CY);
+    static constexpr int NP = NA + 4 + 2 * (CX + CY);
+    int dx[NP], dy[NP];
+    int nb[NA][4];   // P-region indices of an A cell's neighbours: y+1, y-1, x+1, x-1
+    int anb[NO][4];  // A-region indices of a patch cell's neighbours, the same order
+    constexpr Reg3() : dx(), dy(), nb(), anb() {
+        int k = 0;
+        for (int j = 0; j < CY; ++j)
+            for (int i = 0; i < CX; ++i) { dx[k] = i; dy[k] = j; ++k; }
+        for (int i = 0; i < CX; ++i) { dx[k] = i; dy[k] = -1; ++k; dx[k] = i; dy[k] = CY; ++k; }
+        for (int j = 0; j < CY; ++j) { dx[k] = -1; dy[k] = j; ++k; dx[k] = CX; dy[k] = j; ++k; }
+        dx[k] = -1; dy[k] = -1; ++k;
+        dx[k] = CX; dy[k] = -1; ++k;
+        dx[k] = -1; dy[k] = CY; ++k;
+        dx[k] = CX; dy[k] = CY; ++k;
+        for (int i = 0; i < CX; ++i) { dx[k] = i; dy[k] = -2; ++k; dx[k] = i; dy[k] = CY + 1; ++k; }
+        for (int j = 0; j < CY; ++j) { dx[k] = -2; dy[k] = j; ++k; dx[k] = CX + 1; dy[k] = j; ++k; }
+        for (int a = 0; a < NA; ++a) {
+            const int ox[4] = {0, 0, 1, -1}, oy[4] = {1, -1, 0, 0};
+            for (int d = 0; d < 4; ++d) {
+                nb[a][d] = -1;
+                for (int q = 0; q < NP; ++q)
+                    if (dx[q] == dx[a] + ox[d] && dy[q] == dy[a] + oy[d]) nb[a][d] = q;
+                if (a < NO) anb[a][d] = nb[a][d];
+            }
+        }
+    }
+};
+
+template <int LB, int CX, int CY, bool REC>
+__global__ void __launch_bounds__(kBlock) USCT_ATTR3D
+k_fwd3d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
+           float *__restrict__ pb, const uint16_t *__restrict__ pcl,
+           const uint16_t *__restrict__ ppl, uint16_t *__restrict__ pal,
+           uint16_t *__restrict__ pbl, const float2 *__restrict__ sp2,
+           const int *__restrict__ src_cell, const double *__restrict__ sincd_a,
+           const double *__restrict__ sincd_b, const int *__restrict__ rec_head,
+           const int *__restrict__ rec_next, const int *__restrict__ row_rec,
+           float *__restrict__ rec_a, float *__restrict__ rec_lo_a, float *__restrict__ rec_b,
+           float *__restrict__ rec_lo_b, float *__restrict__ srcd2_a, float *__restrict__ srcd2_b,
+           int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+    using R = Reg3<CX, CY>;
+    constexpr R T{};
+    constexpr int NO = R::NO, NA = R::NA, NP = R::NP;
+    constexpr int W = kWave / LB;
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    const int sg = wv % ng;
+    int xs, ys, zc;
+    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = ys * CY;
+    const int z0 = zc * RZ;
+    const int z1 = min(z0 + RZ, nz);
+    if (x0 >= nx) return;
+    const int my_src = src_cell[b];
+    const double inc_a = sincd_a[b], inc_b = sincd_b[b];
+    const size_t rowE = (size_t)nx * Bp;
+    const size_t planeE = rowE * ny;
+    const size_t planeC = (size_t)nx * ny;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    auto rowc = [&](int q) -> size_t { return (size_t)min(max(y0 + T.dy[q], 0), ny - 1); };
+    auto plane = [&](int z) -> size_t { return (size_t)min(max(z, 0), nz - 1) * planeE; };
+    auto el = [&](size_t pl, int q) -> size_t {
+        return pl + rowc(q) * rowE + (size_t)((ptrdiff_t)T.dx[q] * Bp);
+    };
+    // P (level n-1) as FP64: um = front plane - 1 (A region), uc = front plane
+    // (P region), un = front plane + 1 (A region).  Raw loads run ahead: the A
+    // region of plane front+1 (nh/nl, converted into un at the front) and the
+    // rest of the P region of plane front (rh/rl, completing uc), both issued
+    // one front earlier; level n-2 on the A region of the front plane (qh/ql).
+    constexpr int NR = NP - NA;
+    double um[NA], uc[NP], un[NA];
+    float nh[NA], rh[NR], qh[NA];
+    uint16_t nl[NA], rl[NR], ql[NA];
+    // A (level n) as stored: planes front-2 (patch), front-1 and front (A region)
+    double am[NO], ac[NA], an[NA];
+    {
+        const size_t pm = plane(z0 - 2), p0 = plane(z0 - 1), p1 = plane(z0);
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            um[q] = hl(ldb(pc + el(pm, q), o4), ldb(pcl + el(pm, q), o2));
+            nh[q] = ldb(pc + el(p1, q), o4);
+            nl[q] = ldb(pcl + el(p1, q), o2);
+            qh[q] = ldn<USCT_NTF3D>(pp + el(p0, q), o4);
+            ql[q] = ldn<USCT_NTF3D>(ppl + el(p0, q), o2);
+        }
+#pragma unroll
+        for (int q = 0; q < NA; ++q) uc[q] = hl(ldb(pc + el(p0, q), o4), ldb(pcl + el(p0, q), o2));
+#pragma unroll
+        for (int q = NA; q < NP; ++q) {
+            rh[q - NA] = ldb(pc + el(p0, q), o4);
+            rl[q - NA] = ldb(pcl + el(p0, q), o2);
+        }
+#pragma unroll
+        for (int q = 0; q < NA; ++q) { ac[q] = 0.0; an[q] = 0.0; }
+#pragma unroll
+        for (int q = 0; q < NO; ++q) am[q] = 0.0;
+    }
+    for (int zf = z0 - 1; zf <= z1; ++zf) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) un[q] = hl(nh[q], nl[q]);
+#pragma unroll
+        for (int q = NA; q < NP; ++q) uc[q] = hl(rh[q - NA], rl[q - NA]);
+        double qq[NA];
+#pragma unroll
+        for (int q = 0; q < NA; ++q) qq[q] = hl(qh[q], ql[q]);
+        // ---- loads for the next front: the A region of plane zf+2, the rest of
+        // plane zf+1's P region, level n-2 on plane zf+1
+        if (zf < z1) {
+            const size_t pn2 = plane(zf + 2), pn1 = plane(zf + 1);
+#pragma unroll
+            for (int q = 0; q < NA; ++q) {
+                nh[q] = ldb(pc + el(pn2, q), o4);
+                nl[q] = ldb(pcl + el(pn2, q), o2);
+                qh[q] = ldn<USCT_NTF3D>(pp + el(pn1, q), o4);
+                ql[q] = ldn<USCT_NTF3D>(ppl + el(pn1, q), o2);
+            }
+#pragma unroll
+            for (int q = NA; q < NP; ++q) {
+                rh[q - NA] = ldb(pc + el(pn1, q), o4);
+                rl[q - NA] = ldb(pcl + el(pn1, q), o2);
+            }
+        }
+        // ---- level n on plane zf, A region
+        const bool own_a = zf >= z0 && zf < z1;
+        const bool zint = zf >= 1 && zf <= nz - 2;
+        const int zr = min(max(zf, 0), nz - 1);
+        double anew[NA];
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            const int x = x0 + T.dx[q], y = y0 + T.dy[q];
+            const int yr = min(max(y, 0), ny - 1);
+            double lu = 0.0;
+            if (zint && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+                lu = lap7d(uc[q], un[q], um[q], uc[T.nb[q][0]], uc[T.nb[q][1]], uc[T.nb[q][2]],
+                           uc[T.nb[q][3]]);
+            double inc = s2d(ldb(sp2 + (size_t)zr * planeC + (size_t)yr * nx + T.dx[q], oc)) * lu;
+            const int c = (zf * ny + y) * nx + x;
+            if (c == my_src) inc = inc + inc_a;
+            const double nv = (2.0 * uc[q] - qq[q]) + inc;
+            uint16_t lb;
+            const float v = split_hl(nv, lb);
+            anew[q] = hl(v, lb);
+            if (q < NO && own_a && x < nx && y < ny) {
+                const size_t e = (size_t)zf * planeE + (size_t)y * rowE + (size_t)T.dx[q] * Bp;
+                stn<USCT_NTF3D>(pa + e, o4, v);
+                stn<USCT_NTF3D>(pal + e, o2, lb);
+                if (c == my_src) srcd2_a[b] = (float)inc;
+                if (REC && row_rec[zf * ny + y] > 0) {
+                    for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
+                        rec_a[(size_t)r * Bp + b] = v;
+                        rec_lo_a[(size_t)r * Bp + b] = bf2f(lb);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NO; ++q) am[q] = ac[q];
+#pragma unroll
+        for (int q = 0; q < NA; ++q) { ac[q] = an[q]; an[q] = anew[q]; }
+        // ---- level n+1 on plane z = zf-1, the patch
+        const int z = zf - 1;
+        if (z >= z0 && z < z1) {
+            const bool zint_b = z >= 1 && z <= nz - 2;
+#pragma unroll
+            for (int q = 0; q < NO; ++q) {
+                const int x = x0 + T.dx[q], y = y0 + T.dy[q];
+                if (x < nx && y < ny) {
+                    double lu = 0.0;
+                    if (zint_b && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2)
+                        lu = lap7d(ac[q], an[q], am[q], ac[T.anb[q][0]], ac[T.anb[q][1]],
+                                   ac[T.anb[q][2]], ac[T.anb[q][3]]);
+                    double inc = s2d(ldb(sp2 + (size_t)z * planeC + (size_t)y * nx + T.dx[q], oc)) * lu;
+                    const int c = (z * ny + y) * nx + x;
+                    if (c == my_src) inc = inc + inc_b;
+                    const double nv = (2.0 * ac[q] - um[q]) + inc;
+                    uint16_t lb;
+                    const float v = split_hl(nv, lb);
+                    const size_t e = (size_t)z * planeE + (size_t)y * rowE + (size_t)T.dx[q] * Bp;
+                    stn<USCT_NTF3D>(pb + e, o4, v);
+                    stn<USCT_NTF3D>(pbl + e, o2, lb);
+                    if (c == my_src) srcd2_b[b] = (float)inc;
+                    if (REC && row_rec[z * ny + y] > 0) {
+                        for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
+                            rec_b[(size_t)r * Bp + b] = v;
+                            rec_lo_b[(size_t)r * Bp + b] = bf2f(lb);
+                        }
+                    }
+                }
+            }
+        }
+        // ---- advance P (the rest of the new front plane's P region arrives in rh/rl)
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+            um[q] = uc[q];
+            uc[q] = un[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Gauss-Newton, 3D without sponge or footprints: background level n (the
 // k_fwd3d update, without trace recording) and the Born level n that uses it
 // (the k_born update) in one launch, on k_fwd3d's patch march with a second

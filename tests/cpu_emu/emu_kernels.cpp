@@ -836,6 +836,146 @@ bool check_fwd3d_x2(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     return ok;
 }
 
+// Two-level 3D forward on lane patches (k_fwd3d_x2, z marched in registers) against
+// two single-level k_fwd3d launches with 2 x 2 patches:
+// levels n and n+1 (hi and lo), both levels' traces and recorded source
+// increments, bit for bit.
+template <int LB, int CX, int CY>
+bool check_fwd3d_x2p(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
+    const int Bp = LB * ng, W = 64 / LB;
+    const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
+    std::mt19937 rng(seed);
+    char tag[160];
+    std::snprintf(tag, sizeof tag, "fwd3d-x2-patch nz=%d ny=%d nx=%d LB=%d ng=%d P=%dx%d RZ=%d", nz,
+                  ny, nx, LB, ng, CX, CY, RZ);
+    constexpr int TX = 2 * CX, TY = 2 * CY;  // source and receiver cells on patch edges
+    std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
+        dl(-1e-7f, 1e-7f);
+    float2 *sp2 = alloc<float2>(N, 1);
+    for (int64_t c = 0; c < N; ++c) sp2[c] = make_float2(u(rng), tt(rng));
+    auto field = [&](float *&h, uint16_t *&l, bool fill) {
+        h = alloc<float>(FE, Bp);
+        l = alloc<uint16_t>(FE, Bp);
+        if (fill)
+            for (int64_t i = 0; i < FE; ++i) {
+                h[i] = d(rng);
+                l[i] = f2bf(dl(rng));
+            }
+    };
+    float *A, *B, *C1, *D1, *C2, *D2;
+    uint16_t *Al, *Bl, *C1l, *D1l, *C2l, *D2l;
+    field(A, Al, true);
+    field(B, Bl, true);
+    field(C1, C1l, false);
+    field(D1, D1l, false);
+    field(C2, C2l, false);
+    field(D2, D2l, false);
+    int *src = alloc<int>(Bp, 0);
+    std::uniform_int_distribution<int> cz(0, nz - 1), cy(0, ny - 1), cx(0, nx - 1);
+    for (int b = 0; b < Bp; ++b) {
+        int x = cx(rng), y = cy(rng);
+        if (b % 5 == 0) x = 0;
+        if (b % 5 == 1) x = std::min(TX, nx - 1);       // a tile edge
+        if (b % 5 == 2) y = std::min(TY - 1, ny - 1);
+        src[b] = (cz(rng) * ny + y) * nx + x;
+    }
+    src[Bp - 1] = -1;
+    double *sa = alloc<double>(Bp, 0), *sb = alloc<double>(Bp, 0);
+    for (int b = 0; b < Bp; ++b) {
+        sa[b] = (double)d(rng) * 1.2345678901;
+        sb[b] = (double)d(rng) * 0.9876543211;
+    }
+    float *da1 = alloc<float>(Bp, 0), *db1 = alloc<float>(Bp, 0), *da2 = alloc<float>(Bp, 0),
+          *db2 = alloc<float>(Bp, 0);
+    std::vector<int> rc = {0, (int)N - 1, (int)(N / 2), (int)(N / 3), (int)(N / 3),
+                           (int)std::min<int64_t>(N - 1, (int64_t)TX)};
+    const int n_rec = (int)rc.size();
+    int *rec_head = alloc<int>(N, 0), *rec_next = alloc<int>(n_rec, 0);
+    int *row_rec = alloc<int>((int64_t)nz * ny, 0);
+    for (int64_t c = 0; c < N; ++c) rec_head[c] = -1;
+    for (int j = n_rec - 1; j >= 0; --j) {
+        rec_next[j] = rec_head[rc[j]];
+        rec_head[rc[j]] = j;
+    }
+    for (int j = 0; j < n_rec; ++j) row_rec[rc[j] / nx]++;
+    const int64_t NR = (int64_t)n_rec * Bp;
+    float *ra1 = alloc<float>(NR, 0), *ral1 = alloc<float>(NR, 0), *rb1 = alloc<float>(NR, 0),
+          *rbl1 = alloc<float>(NR, 0), *ra2 = alloc<float>(NR, 0), *ral2 = alloc<float>(NR, 0),
+          *rb2 = alloc<float>(NR, 0), *rbl2 = alloc<float>(NR, 0);
+    // reference: two single-level launches (2 x 2 patches)
+    {
+        constexpr int CXr = 2, CYr = 2;
+        const int nxs = (int)cdiv(nx, (int64_t)W * CXr), nys = (int)cdiv(ny, CYr);
+        const int nzc = (int)cdiv(nz, RZ);
+        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+        emu::launch(blocks, kBlock, [&] {
+            k_fwd3d<LB, CXr, CYr, true>(A, B, C1, Al, Bl, C1l, sp2, src, sa, rec_head, rec_next,
+                                        row_rec, ra1, ral1, da1, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
+                                        ng);
+        });
+        emu::launch(blocks, kBlock, [&] {
+            k_fwd3d<LB, CXr, CYr, true>(C1, A, D1, C1l, Al, D1l, sp2, src, sb, rec_head, rec_next,
+                                        row_rec, rb1, rbl1, db1, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
+                                        ng);
+        });
+    }
+    {
+        const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
+        const int nzc = (int)cdiv(nz, RZ);
+        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+        emu::launch(blocks, kBlock, [&] {
+            k_fwd3d_x2<LB, CX, CY, true>(A, B, C2, D2, Al, Bl, C2l, D2l, sp2, src, sa, sb, rec_head,
+                                         rec_next, row_rec, ra2, ral2, rb2, rbl2, da2, db2, nz, ny,
+                                         nx, Bp, RZ, nxs, nys, nzc, ng);
+        });
+    }
+    bool ok = std::memcmp(C1, C2, sizeof(float) * FE) == 0 &&
+              std::memcmp(C1l, C2l, sizeof(uint16_t) * FE) == 0;
+    if (!ok) {
+        for (int64_t i = 0; i < FE; ++i)
+            if (std::memcmp(&C1[i], &C2[i], 4) || C1l[i] != C2l[i]) {
+                std::printf("MISMATCH %s level n at cell %lld shot %lld\n", tag, (long long)(i / Bp),
+                            (long long)(i % Bp));
+                break;
+            }
+    }
+    if (ok && (std::memcmp(D1, D2, sizeof(float) * FE) || std::memcmp(D1l, D2l, sizeof(uint16_t) * FE))) {
+        ok = false;
+        for (int64_t i = 0; i < FE; ++i)
+            if (std::memcmp(&D1[i], &D2[i], 4) || D1l[i] != D2l[i]) {
+                std::printf("MISMATCH %s level n+1 at cell %lld shot %lld\n", tag,
+                            (long long)(i / Bp), (long long)(i % Bp));
+                break;
+            }
+    }
+    if (ok && (std::memcmp(ra1, ra2, 4 * NR) || std::memcmp(ral1, ral2, 4 * NR) ||
+               std::memcmp(rb1, rb2, 4 * NR) || std::memcmp(rbl1, rbl2, 4 * NR))) {
+        std::printf("MISMATCH %s traces\n", tag);
+        ok = false;
+    }
+    if (ok && (std::memcmp(da1, da2, 4 * Bp) || std::memcmp(db1, db2, 4 * Bp))) {
+        std::printf("MISMATCH %s recorded source increments\n", tag);
+        ok = false;
+    }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+bool x2p_3d_cases(uint32_t seed) {
+    bool ok = true;
+    ok &= check_fwd3d_x2p<64, 1, 1>(9, 7, 11, 2, 4, seed++);
+    ok &= check_fwd3d_x2p<64, 1, 1>(12, 17, 19, 1, 64, seed++);
+    ok &= check_fwd3d_x2p<64, 2, 1>(7, 9, 21, 2, 3, seed++);
+    ok &= check_fwd3d_x2p<64, 1, 2>(5, 6, 13, 1, 2, seed++);
+    ok &= check_fwd3d_x2p<32, 1, 1>(3, 3, 3, 1, 1, seed++);
+    ok &= check_fwd3d_x2p<32, 2, 1>(10, 16, 8, 1, 5, seed++);
+    ok &= check_fwd3d_x2p<16, 1, 1>(8, 5, 6, 2, 4, seed++);
+    ok &= check_fwd3d_x2p<16, 1, 2>(6, 9, 7, 1, 8, seed++);
+    ok &= check_fwd3d_x2p<16, 2, 1>(11, 4, 9, 2, 16, seed++);
+    ok &= check_fwd3d_x2p<64, 2, 1>(4, 4, 4, 1, 1, seed++);
+    return ok;
+}
+
 bool x2_3d_cases(uint32_t seed) {
     bool ok = true;
     ok &= check_fwd3d_x2<16, 8, 8, 2, 2>(9, 7, 11, 2, 4, seed++);
@@ -1179,6 +1319,9 @@ int main(int argc, char **argv) {
     }
     if (only.empty() || only == "lds") {
         ok &= lds_cases(seed + 2000);
+    }
+    if (only.empty() || only == "x2p-3d") {
+        ok &= x2p_3d_cases(seed + 9000);
     }
     if (only.empty() || only == "x2-3d") {
         ok &= x2_3d_cases(seed + 4000);

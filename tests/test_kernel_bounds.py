@@ -81,7 +81,7 @@ def _strip3d(t, nxs, nys, nzc):
     return xs, ys, zc, ys < nys and zc < nzc
 
 
-def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint, RZ=4):
+def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint, RZ=4, halo=1):
     W = 64 // LB
     ng, nxs, nys, nzc = Bp // LB, _cdiv(nx, W * CX), _cdiv(ny, CY), _cdiv(nz, RZ)
     lo = hi = 0
@@ -102,9 +102,9 @@ def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint, RZ=4):
             for sb in (0, LB - 1):
                 b = sg * LB + sb
                 for z in zs:
-                    for j in (-1, CY):
+                    for j in (-halo, CY - 1 + halo):
                         r = min(max(ys * CY + j, 0), ny - 1)
-                        for i in (-1, CX):
+                        for i in (-halo, CX - 1 + halo):
                             e = ((z * ny + r) * nx + x0 + i) * Bp + b
                             lo, hi = min(lo, e), max(hi, e)
     return lo, hi - nx * ny * nz * Bp
@@ -136,3 +136,7 @@ def test_3d_halo_reads_stay_in_guard(nx, ny, nz):
                         continue  # the adjoint reduces a lane's cells over its lane group
                     lo, over = _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adj)
                     assert lo >= -g * Bp and over < g * Bp, (nx, ny, nz, Bp, LB, CX, CY, lo, over)
+            # the two-level forward on lane patches reads two cells out in x and y
+            for CX, CY in ((1, 1), (2, 1), (1, 2)):
+                lo, over = _extent_3d(nx, ny, nz, Bp, LB, CX, CY, False, halo=2)
+                assert lo >= -g * Bp and over < g * Bp, (nx, ny, nz, Bp, LB, CX, CY, "x2", lo, over)
