@@ -516,6 +516,8 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the 3D (C4) leg of the 2D headline")
     ap.add_argument("--c4-steps", type=int, default=1)
     ap.add_argument("--no-ceiling", action="store_true", help="skip the streaming-kernel ceiling")
+    ap.add_argument("--only-ceiling", action="store_true",
+                    help="measure only the streaming-kernel ceiling and print it as JSON")
     ap.add_argument("--ckpt-every", type=int, default=0)
     ap.add_argument("--reverse", default="auto", choices=["auto", "on", "off"],
                     help="3D gradients: run the forward backwards alongside the adjoint (on), "
@@ -553,6 +555,9 @@ def main():
 
     import torch
 
+    if args.only_ceiling:
+        print(json.dumps(stream_ceiling()), flush=True)
+        return
     if args.calib_copy_mb:
         calib_copy(args.calib_copy_mb)
     if args.workload == "C5":

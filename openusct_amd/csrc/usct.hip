@@ -1568,11 +1568,12 @@ int autotune(usct_plan *P) {
             if (!adjoint && P->Bp % 16 == 0 && fusable(P))
                 for (int rz : {8, 16, 32, 64}) cands.push_back({16, 8, 8, rz, 2});
             // two forward levels per launch on lane patches, marching z in registers
+            // (long z chunks: the planes above and below a chunk cost it the most;
+            // 1 x 2 measured level with 2 x 1)
             if (!adjoint && fusable(P))
                 for (int lb : lbs)
-                    for (auto pc : {std::pair<int, int>{1, 1}, {2, 1}, {1, 2}})
-                        for (int rz : {4, 8, 16, 32, 64})
-                            cands.push_back({lb, pc.first, pc.second, rz, 3});
+                    for (auto pc : {std::pair<int, int>{1, 1}, {2, 1}})
+                        for (int rz : {32, 64}) cands.push_back({lb, pc.first, pc.second, rz, 3});
         }
         Tiling &T = adjoint ? P->ta : P->tf;
         double best = 1e300;
