@@ -65,8 +65,13 @@ enum {
     USCT_FLAG_NO_FUSE = 16,     /* batched 2D: one time level per launch instead of two */
     USCT_FLAG_NO_REVERSE = 32,  /* batched 3D gradients: keep forward levels (whole history or
                                    checkpoints) instead of running the forward backwards */
-    USCT_FLAG_REVERSE = 64      /* batched 3D gradients: run the forward backwards alongside the
+    USCT_FLAG_REVERSE = 64,     /* batched 3D gradients: run the forward backwards alongside the
                                    adjoint even where the whole history would fit */
+    USCT_FLAG_FP32_LO = 128     /* carry the forward state as FP32 hi + FP32 lo (8 bytes per
+                                   shot-cell instead of 6; about 48 significant bits instead of 32):
+                                   for time axes that end soon after the scattered arrivals, where
+                                   the traces' rounding is a visible share of a small residual;
+                                   point-element plans; Gauss-Newton products return USCT_ENOTSUP */
 };
 
 /* Misfit codes (software/python/uap.py:41). */

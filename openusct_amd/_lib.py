@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libusct.so")
 USCT_OK, USCT_EINVAL, USCT_ENOMEM, USCT_EHIP, USCT_ENOTSUP = range(5)
 METHOD_AUTO, METHOD_PLAIN, METHOD_BATCHED = 0, 1, 2
 FLAG_SPONGE, FLAG_AUTOTUNE, FLAG_NO_GRAPH, FLAG_FORWARD_ONLY, FLAG_NO_FUSE = 1, 2, 4, 8, 16
-FLAG_NO_REVERSE, FLAG_REVERSE = 32, 64
+FLAG_NO_REVERSE, FLAG_REVERSE, FLAG_FP32_LO = 32, 64, 128
 MISFIT_CODES = {"l2": 0, "gcn": 1}
 
 # Every symbol declared in include/usct.h.

@@ -33,6 +33,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/usct.h"
@@ -141,6 +142,10 @@ struct usct_plan {
     uint16_t *Lo[4] = {nullptr, nullptr, nullptr, nullptr};
     uint16_t *Zlo = nullptr;
     std::vector<uint16_t *> CKlo;
+    // USCT_FLAG_FP32_LO: the low parts are FP32 (two uint16_t slots per element
+    // in the buffers above, passed to the kernels as float)
+    bool lo32 = false;
+    int64_t lo_units() const { return lo32 ? 2 : 1; }
     float *recb = nullptr, *recb_lo = nullptr, *res = nullptr;
     float *dobsT = nullptr, *dobsT_lo = nullptr;  // observed traces, FP32 hi + lo
     float2 *part = nullptr;
@@ -277,10 +282,15 @@ template <int LB, int CX, bool SP, bool REC>
 void fwd2d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
              const float *sinc, float *rec, float *rec_lo, const Tile &t) {
     const double *sd = P->sincd + (sinc - P->sinc);
-    hipLaunchKernelGGL((k_fwd2d<LB, CX, SP, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pn.hi,
-                       pc.lo, pp.lo, pn.lo, P->sp2, P->spg, P->src_cell, sd, P->rec_head,
-                       P->rec_next, P->row_rec, rec, rec_lo, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs,
-                       t.nyc, P->Bp / P->tf.LB);
+    auto go = [&](auto lo_type) {
+        using L = decltype(lo_type);
+        hipLaunchKernelGGL((k_fwd2d<LB, CX, SP, REC, L>), g, dim3(kBlock), 0, s, pc.hi, pp.hi,
+                           pn.hi, (const L *)pc.lo, (const L *)pp.lo, (L *)pn.lo, P->sp2, P->spg,
+                           P->src_cell, sd, P->rec_head, P->rec_next, P->row_rec, rec, rec_lo,
+                           P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nyc, P->Bp / P->tf.LB);
+    };
+    if (P->lo32) go(float{});
+    else go(uint16_t{});
 }
 
 template <int LB, bool SP, bool REC>
@@ -305,10 +315,16 @@ void fwd3d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
              const float *sinc, float *rec, float *rl, const Tile &t) {
     float *d2 = P->srcd2 + (sinc - P->sinc);
     const double *sd = P->sincd + (sinc - P->sinc);
-    hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pn.hi,
-                       pc.lo, pp.lo, pn.lo, P->sp2, P->src_cell, sd, P->rec_head, P->rec_next,
-                       P->row_rec, rec, rl, d2, P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys,
-                       t.nyc, P->Bp / P->tf.LB);
+    auto go = [&](auto lo_type) {
+        using L = decltype(lo_type);
+        hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC, L>), g, dim3(kBlock), 0, s, pc.hi, pp.hi,
+                           pn.hi, (const L *)pc.lo, (const L *)pp.lo, (L *)pn.lo, P->sp2,
+                           P->src_cell, sd, P->rec_head, P->rec_next, P->row_rec, rec, rl, d2,
+                           P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys, t.nyc,
+                           P->Bp / P->tf.LB);
+    };
+    if (P->lo32) go(float{});
+    else go(uint16_t{});
 }
 
 template <int LB, bool REC>
@@ -394,10 +410,16 @@ void fwd2d_x2_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usc
     const double *sa = P->sincd + (int64_t)n * P->Bp;
     const int64_t tr = (int64_t)P->n_rec * P->Bp;
     float *ra = P->recb + (int64_t)n * tr, *rla = P->recb_lo + (int64_t)n * tr;
-    hipLaunchKernelGGL((k_fwd2d_x2<LB, CX, REC>), g, dim3(kBlock), 0, s, pc.hi, pp.hi, pa.hi,
-                       pb.hi, pc.lo, pp.lo, pa.lo, pb.lo, P->sp2, P->src_cell, sa, sa + P->Bp,
-                       P->rec_head, P->rec_next, P->row_rec, ra, rla, ra + tr, rla + tr, P->ny,
-                       P->nx, P->Bp, P->tf.RY, t.nxs, t.nyc, P->Bp / P->tf.LB, P->tf.sched);
+    auto go = [&](auto lo_type) {
+        using L = decltype(lo_type);
+        hipLaunchKernelGGL((k_fwd2d_x2<LB, CX, REC, L>), g, dim3(kBlock), 0, s, pc.hi, pp.hi,
+                           pa.hi, pb.hi, (const L *)pc.lo, (const L *)pp.lo, (L *)pa.lo,
+                           (L *)pb.lo, P->sp2, P->src_cell, sa, sa + P->Bp, P->rec_head,
+                           P->rec_next, P->row_rec, ra, rla, ra + tr, rla + tr, P->ny, P->nx,
+                           P->Bp, P->tf.RY, t.nxs, t.nyc, P->Bp / P->tf.LB, P->tf.sched);
+    };
+    if (P->lo32) go(float{});
+    else go(uint16_t{});
 }
 
 template <int LB, bool REC>
@@ -623,10 +645,16 @@ void radj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk
               Lvl pk, Lvl po, int wr, const usct_plan *P, int k, const Tile &t) {
     const double *sd = P->sincd + (int64_t)k * P->Bp;
     const float *res = P->res + (int64_t)k * P->n_rec * P->Bp;
-    hipLaunchKernelGGL((k_radj3d<LB, CX, CY>), g, dim3(kBlock), 0, s, l1, l2, lk, pc.hi, pc.lo,
-                       pk.hi, pk.lo, po.hi, po.lo, wr, P->sp2, P->src_cell, sd, P->rec_head,
-                       P->rec_next, P->row_rec, res, P->g_hi, P->g_lo, P->nz, P->ny, P->nx, P->Bp,
-                       P->ta.RY, t.nxs, t.nys, t.nyc, P->ng_adj());
+    auto go = [&](auto lo_type) {
+        using L = decltype(lo_type);
+        hipLaunchKernelGGL((k_radj3d<LB, CX, CY, L>), g, dim3(kBlock), 0, s, l1, l2, lk, pc.hi,
+                           (const L *)pc.lo, pk.hi, (const L *)pk.lo, po.hi, (L *)po.lo, wr, P->sp2,
+                           P->src_cell, sd, P->rec_head, P->rec_next, P->row_rec, res, P->g_hi,
+                           P->g_lo, P->nz, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nys, t.nyc,
+                           P->ng_adj());
+    };
+    if (P->lo32) go(float{});
+    else go(uint16_t{});
 }
 
 template <int LB>
@@ -1139,18 +1167,23 @@ int run_plain(usct_plan *P, int mode, int misfit, const double *m, const double 
         if (born) born_clear(P, s);
         for (int n = 1; n < nt; ++n) {
             if (pr) ev_start(pr, pr->fwd_ev, s);
-            if (P->sponge)
-                hipLaunchKernelGGL((k_fwd_plain<true, true>), dim3(gN), dim3(256), 0, s, lvl(n - 1),
-                                   lvl(n - 2), lvl(n), llo(n - 1), llo(n - 2), llo(n), P->sp2,
-                                   P->spg, src, P->sincd + n, P->rec_head, P->rec_next,
+            auto go = [&](auto lo_type, auto sponge_tag) {
+                using L = decltype(lo_type);
+                constexpr bool SP = decltype(sponge_tag)::value;
+                hipLaunchKernelGGL((k_fwd_plain<SP, true, L>), dim3(gN), dim3(256), 0, s,
+                                   lvl(n - 1), lvl(n - 2), lvl(n), (const L *)llo(n - 1),
+                                   (const L *)llo(n - 2), (L *)llo(n), P->sp2, P->spg, src,
+                                   P->sincd + n, P->rec_head, P->rec_next,
                                    P->recb + (int64_t)n * nr, P->recb_lo + (int64_t)n * nr,
                                    P->srcd2 + n, P->nz, P->ny, P->nx, nd);
-            else
-                hipLaunchKernelGGL((k_fwd_plain<false, true>), dim3(gN), dim3(256), 0, s,
-                                   lvl(n - 1), lvl(n - 2), lvl(n), llo(n - 1), llo(n - 2), llo(n),
-                                   P->sp2, P->spg, src, P->sincd + n, P->rec_head, P->rec_next,
-                                   P->recb + (int64_t)n * nr, P->recb_lo + (int64_t)n * nr,
-                                   P->srcd2 + n, P->nz, P->ny, P->nx, nd);
+            };
+            if (P->lo32) {
+                if (P->sponge) go(float{}, std::true_type{});
+                else go(float{}, std::false_type{});
+            } else {
+                if (P->sponge) go(uint16_t{}, std::true_type{});
+                else go(uint16_t{}, std::false_type{});
+            }
             if (pr) {
                 ev_stop(pr, pr->fwd_ev, s);
                 pr->fwd_lv.push_back(1);
@@ -1233,8 +1266,9 @@ int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
         fields += (K + 2) + 2 * std::max(NS - 2, 0);
         halves += 2 * std::max(NS - 2, 0);
     }
-    int64_t b = fields * field + halves * (field / 2) + 5 * traces;  // rec hi/lo, dobs hi/lo, res
-    b += (fields * 4 + halves * 2) * 2 * kGuardCells * Bp;
+    const int64_t lo_b = P->lo32 ? 4 : 2;  // bytes of a low part
+    int64_t b = fields * field + halves * (field / 4 * lo_b) + 5 * traces;  // rec hi/lo, dobs hi/lo, res
+    b += (fields * 4 + halves * lo_b) * 2 * kGuardCells * Bp;
     b += P->N * 8 /*sp2*/ + P->N * 4 /*sponge*/ + 2 * P->N * 4 * (Bp / 16) /*g*/;
     b += (int64_t)P->nt * Bp * 4 + (int64_t)P->n_rec * Bp * 24 + (int64_t)P->N * 4 + (1 << 20);
     return b;
@@ -1348,9 +1382,10 @@ int allocate_batched(usct_plan *P) {
     TRY(dalloc_guarded(P, &P->Z, FE, P->Bp));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * FE));
     for (int i = 0; i < 4; ++i) TRY(dalloc_guarded(P, &P->F[i], FE, P->Bp));
-    for (int i = 0; i < 4; ++i) TRY(dalloc_guarded(P, &P->Lo[i], FE, P->Bp));
-    TRY(dalloc_guarded(P, &P->Zlo, FE, P->Bp));
-    HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * FE));
+    const int64_t u = P->lo_units();
+    for (int i = 0; i < 4; ++i) TRY(dalloc_guarded(P, &P->Lo[i], FE * u, P->Bp * u));
+    TRY(dalloc_guarded(P, &P->Zlo, FE * u, P->Bp * u));
+    HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * FE * u));
     const bool fwd_only = (P->flags & USCT_FLAG_FORWARD_ONLY) != 0;
     const int nW = fwd_only ? 0 : P->K + 2;
     P->Wslot.resize(nW);
@@ -1359,7 +1394,7 @@ int allocate_batched(usct_plan *P) {
     P->CK.resize(nC);
     for (int i = 0; i < nC; ++i) TRY(dalloc_guarded(P, &P->CK[i], FE, P->Bp));
     P->CKlo.resize(nC);
-    for (int i = 0; i < nC; ++i) TRY(dalloc_guarded(P, &P->CKlo[i], FE, P->Bp));
+    for (int i = 0; i < nC; ++i) TRY(dalloc_guarded(P, &P->CKlo[i], FE * u, P->Bp * u));
     const int64_t ntr = (int64_t)P->n_rec * P->Bp;
     TRY(dalloc_t(P, &P->recb, (int64_t)P->nt * ntr));
     TRY(dalloc_t(P, &P->recb_lo, (int64_t)P->nt * ntr));
@@ -1395,9 +1430,9 @@ int allocate_plain(usct_plan *P) {
     TRY(dalloc_t(P, &P->Z, N));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * N));
     for (int i = 0; i < 4; ++i) TRY(dalloc_t(P, &P->F[i], N));
-    for (int i = 0; i < 4; ++i) TRY(dalloc_t(P, &P->Lo[i], N));
-    TRY(dalloc_t(P, &P->Zlo, N));
-    HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * N));
+    for (int i = 0; i < 4; ++i) TRY(dalloc_t(P, &P->Lo[i], N * P->lo_units()));
+    TRY(dalloc_t(P, &P->Zlo, N * P->lo_units()));
+    HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * N * P->lo_units()));
     // full history, levels 1..nt-1 (slot 0 unused)
     const int64_t hist = (int64_t)P->nt * N * 4;
     if (hist + P->bytes > P->budget)
@@ -1473,11 +1508,14 @@ double time_tile(usct_plan *P, bool adjoint) {
 // cells per lane and rows per wavefront, for the forward and the adjoint kernel
 // separately.  Results are kept per (ndim, dims, padded shots, sponge) for the
 // life of the process.
+// Plans share tuned tiles when the candidates and kernels they time are the same:
+// grid, shots per pass, sponge, reversal, fusion (whether two-level kernels are
+// candidates) and the low part's type (which kernel kinds can take it).
 struct TuneKey {
-    int ndim, nz, ny, nx, Bp, sponge, rev;
+    int ndim, nz, ny, nx, Bp, sponge, rev, fuse, lo32;
     bool operator<(const TuneKey &o) const {
-        return std::tie(ndim, nz, ny, nx, Bp, sponge, rev) <
-               std::tie(o.ndim, o.nz, o.ny, o.nx, o.Bp, o.sponge, o.rev);
+        return std::tie(ndim, nz, ny, nx, Bp, sponge, rev, fuse, lo32) <
+               std::tie(o.ndim, o.nz, o.ny, o.nx, o.Bp, o.sponge, o.rev, o.fuse, o.lo32);
     }
 };
 struct TuneVal { Tiling tf, ta; };
@@ -1488,7 +1526,8 @@ int autotune(usct_plan *P) {
     // read that row of the source and residual arrays; with nt = 1 no level is
     // ever advanced, so there is nothing to time and row 1 does not exist.
     if (P->nt < 2) return 0;
-    const TuneKey key{P->ndim, P->nz, P->ny, P->nx, P->Bp, P->sponge ? 1 : 0, P->rev ? 1 : 0};
+    const TuneKey key{P->ndim, P->nz, P->ny, P->nx, P->Bp, P->sponge ? 1 : 0, P->rev ? 1 : 0,
+                      fusable(P) ? 1 : 0, P->lo32 ? 1 : 0};
     auto hit = g_tuned.find(key);
     if (hit != g_tuned.end()) {
         P->tf = hit->second.tf;
@@ -1498,7 +1537,7 @@ int autotune(usct_plan *P) {
     // zero the fields used by the timing launches
     for (int i = 0; i < 4; ++i) {
         HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
-        HIPX(hipMemset(P->Lo[i], 0, sizeof(uint16_t) * P->field_elems()));
+        HIPX(hipMemset(P->Lo[i], 0, sizeof(uint16_t) * P->field_elems() * P->lo_units()));
     }
     for (size_t i = 0; i < std::min<size_t>(P->Wslot.size(), 3); ++i)
         HIPX(hipMemset(P->Wslot[i], 0, sizeof(float) * P->field_elems()));
@@ -1575,6 +1614,10 @@ int autotune(usct_plan *P) {
                     for (auto pc : {std::pair<int, int>{1, 1}, {2, 1}})
                         for (int rz : {32, 64}) cands.push_back({lb, pc.first, pc.second, rz, 3});
         }
+        if (P->lo32 && !adjoint)  // the LDS and two-level 3D forwards take bfloat16 low parts
+            cands.erase(std::remove_if(cands.begin(), cands.end(),
+                                       [](const Tiling &c) { return c.lds != 0; }),
+                        cands.end());
         Tiling &T = adjoint ? P->ta : P->tf;
         double best = 1e300;
         Tiling bt = T;
@@ -1712,6 +1755,9 @@ int run(usct_plan *P, int mode, int misfit, const double *m, const double *w,
         (P->flags & USCT_FLAG_FORWARD_ONLY))
         return fail(USCT_ENOTSUP, "plan was created with USCT_FLAG_FORWARD_ONLY");
     if (mode == MODE_GNHV && !v) return fail(USCT_EINVAL, "v is required");
+    if (mode == MODE_GNHV && P->lo32)
+        return fail(USCT_ENOTSUP, "Gauss-Newton products are not implemented for plans with "
+                                  "USCT_FLAG_FP32_LO");
     if (P->n_tx == 0) {
         TRY(enter(P, stream));
         if (J) HIPX(hipMemsetAsync(J, 0, sizeof(double), P->stream));
@@ -2048,6 +2094,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     P->method = method == USCT_METHOD_PLAIN ? USCT_METHOD_PLAIN : USCT_METHOD_BATCHED;
     P->flags = flags;
     P->sponge = (flags & USCT_FLAG_SPONGE) != 0;
+    P->lo32 = (flags & USCT_FLAG_FP32_LO) != 0;
     if (ndim == 3) P->tf = P->ta = Tiling{64, 2, 2, 32, 0};
     int rc = 0;
     auto bail = [&](int code) {
@@ -2116,6 +2163,8 @@ int usct_plan_create_groups(usct_plan **out, int ndim, const int *dims, double h
     for (int j = 0; j < n_rec; ++j) rec_first[j] = rec_cells[rec_off[j]];
     if (flags & USCT_FLAG_REVERSE)
         return fail(USCT_ENOTSUP, "footprint plans keep forward levels (USCT_FLAG_REVERSE)");
+    if (flags & USCT_FLAG_FP32_LO)
+        return fail(USCT_ENOTSUP, "USCT_FLAG_FP32_LO is not implemented for footprint plans");
     usct_plan *P = nullptr;
     // the footprint corrections read stored forward levels
     TRY(usct_plan_create(&P, ndim, dims, h, dt, nt, tx_first.data(), n_tx, rec_first.data(), n_rec,

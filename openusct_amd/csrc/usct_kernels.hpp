@@ -291,6 +291,18 @@ __device__ __forceinline__ float split_hl(double v, uint16_t &lo) {
     lo = f2bf((float)(v - (double)h));
     return h;
 }
+// The same with an FP32 low part (plans created with USCT_FLAG_FP32_LO): the
+// state then carries about 48 significant bits instead of 32.  The kernels
+// that read or write the forward state take the low part's type as a
+// template parameter L (uint16_t for bfloat16, float).
+__device__ __forceinline__ double hl(float h, float l) { return (double)h + (double)l; }
+__device__ __forceinline__ float split_hl(double v, float &lo) {
+    const float h = (float)v;
+    lo = (float)(v - (double)h);
+    return h;
+}
+__device__ __forceinline__ float lo_f(uint16_t b) { return bf2f(b); }
+__device__ __forceinline__ float lo_f(float f) { return f; }
 
 // ---------------------------------------------------------------------------
 // Batched 2D forward step: level n from levels n-1 (pc) and n-2 (pp) into pn.
@@ -305,11 +317,11 @@ __device__ __forceinline__ float split_hl(double v, uint16_t &lo) {
 // guard zone every field carries before and after it), values that only feed
 // border cells, whose Laplacian is zero.
 // ---------------------------------------------------------------------------
-template <int LB, int CX, bool SPONGE, bool REC>
+template <int LB, int CX, bool SPONGE, bool REC, class L = uint16_t>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR2D
 k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
-        const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
-        uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+        const L *__restrict__ pcl, const L *__restrict__ ppl,
+        L *__restrict__ pnl, const float2 *__restrict__ sp2,
         const float *__restrict__ sponge, const int *__restrict__ src_cell,
         const double *__restrict__ sincd, const int *__restrict__ rec_head,
         const int *__restrict__ rec_next, const int *__restrict__ row_rec,
@@ -332,7 +344,7 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     const double my_inc = sincd[b];
     const size_t rowE = (size_t)nx * Bp;
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;  // lane byte offset, float fields
-    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;  // bf16 fields
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * (unsigned)sizeof(L);  // bf16 fields
     const unsigned oc = (unsigned)x0 * 8u;             // S' (float2 per cell)
     auto P = [&](size_t e) -> double { return hl(ldb(pc + e, o4), ldb(pcl + e, o2)); };
     double rm[CX], rc[CX + 2], rp[CX];
@@ -362,14 +374,14 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                 const size_t ei = ro + (size_t)i * Bp;
                 double nv = (2.0 * rc[i + 1] - hl(ldb(pp + ei, o4), ldb(ppl + ei, o2))) + inc;
                 if (SPONGE) nv = nv * (double)sponge[(size_t)y * nx + x];
-                uint16_t lb;
+                L lb;
                 const float v = split_hl(nv, lb);
                 stb(pn + ei, o4, v);
                 stb(pnl + ei, o2, lb);
                 if (has_rec) {
                     for (int j = rec_head[y * nx + x]; j >= 0; j = rec_next[j]) {
                         rec[(size_t)j * Bp + b] = v;
-                        rec_lo[(size_t)j * Bp + b] = bf2f(lb);
+                        rec_lo[(size_t)j * Bp + b] = lo_f(lb);
                     }
                 }
             }
@@ -432,12 +444,12 @@ __device__ __forceinline__ bool chunk2d(int t, int nxs, int nyc, int sched, int 
     return xs < nxs && yc < nyc;
 }
 
-template <int LB, int CX, bool REC>
+template <int LB, int CX, bool REC, class L = uint16_t>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR2D
 k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
-           float *__restrict__ pb, const uint16_t *__restrict__ pcl,
-           const uint16_t *__restrict__ ppl, uint16_t *__restrict__ pal,
-           uint16_t *__restrict__ pbl, const float2 *__restrict__ sp2,
+           float *__restrict__ pb, const L *__restrict__ pcl,
+           const L *__restrict__ ppl, L *__restrict__ pal,
+           L *__restrict__ pbl, const float2 *__restrict__ sp2,
            const int *__restrict__ src_cell, const double *__restrict__ sincd_a,
            const double *__restrict__ sincd_b, const int *__restrict__ rec_head,
            const int *__restrict__ rec_next, const int *__restrict__ row_rec,
@@ -461,7 +473,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
     const double inc_a = sincd_a[b], inc_b = sincd_b[b];
     const size_t rowE = (size_t)nx * Bp;
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
-    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * (unsigned)sizeof(L);
     const unsigned oc = (unsigned)x0 * 8u;
     // first front row: one halo row outside the chunk, on the side the march
     // starts from; rows y0-1 .. y1 are visited (RY + 2 fronts)
@@ -473,7 +485,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
     // iteration ahead (nh/nl, qh/ql).
     double um[CA], uc[CA + 2], un[CA + 2];
     float nh[CA + 2], qh[CA];
-    uint16_t nl[CA + 2], ql[CA];
+    L nl[CA + 2], ql[CA];
     // A rows(-2), (-1), (0) (index j <-> column x0-1+j), as stored
     double am[CA], ac[CA], an[CA];
     {
@@ -539,7 +551,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
             double inc = s2d(ldb(spr + (j - 1), oc)) * lu;
             if (yf * nx + x == my_src) inc = inc + inc_a;
             const double nv = (2.0 * uc[j + 1] - q[j]) + inc;
-            uint16_t lb;
+            L lb;
             const float v = split_hl(nv, lb);
             am[j] = ac[j];
             ac[j] = an[j];
@@ -551,7 +563,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
                 if (rec_a_row) {
                     for (int r = rec_head[yr * nx + x]; r >= 0; r = rec_next[r]) {
                         rec_a[(size_t)r * Bp + b] = v;
-                        rec_lo_a[(size_t)r * Bp + b] = bf2f(lb);
+                        rec_lo_a[(size_t)r * Bp + b] = lo_f(lb);
                     }
                 }
             }
@@ -576,7 +588,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
                     double inc = s2d(ldb(spb + i, oc)) * lu;
                     if (y * nx + x == my_src) inc = inc + inc_b;
                     const double nv = (2.0 * ac[i + 1] - um[i + 1]) + inc;
-                    uint16_t lb;
+                    L lb;
                     const float v = split_hl(nv, lb);
                     const size_t e = rob + (size_t)i * Bp;
                     stn<USCT_NT2D>(pb + e, o4, v);
@@ -584,7 +596,7 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
                     if (rec_b_row) {
                         for (int r = rec_head[y * nx + x]; r >= 0; r = rec_next[r]) {
                             rec_b[(size_t)r * Bp + b] = v;
-                            rec_lo_b[(size_t)r * Bp + b] = bf2f(lb);
+                            rec_lo_b[(size_t)r * Bp + b] = lo_f(lb);
                         }
                     }
                 }
@@ -1205,11 +1217,11 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
 // adjoint's imaging.
 // ---------------------------------------------------------------------------
 
-template <int LB, int CX, int CY, bool REC>
+template <int LB, int CX, int CY, bool REC, class L = uint16_t>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
-        const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
-        uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+        const L *__restrict__ pcl, const L *__restrict__ ppl,
+        L *__restrict__ pnl, const float2 *__restrict__ sp2,
         const int *__restrict__ src_cell, const double *__restrict__ sincd,
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
         const int *__restrict__ row_rec, float *__restrict__ rec, float *__restrict__ rec_lo,
@@ -1234,20 +1246,20 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     const size_t planeE = rowE * ny;
     const size_t planeC = (size_t)nx * ny;
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
-    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * (unsigned)sizeof(L);
     const unsigned oc = (unsigned)x0 * 8u;
     // row j of the patch (j = -1..CY), clamped into the grid; wave-uniform
     auto row = [&](int j) -> size_t { return (size_t)min(max(y0 + j, 0), ny - 1); };
     // full state hi + lo of cell (row j, column i) of a plane, in FP64
     auto P = [&](size_t plane, int j, int i) -> double {
         const size_t e = plane + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
-        return (double)ldb(pc + e, o4) + (double)bf2f(ldb(pcl + e, o2));
+        return (double)ldb(pc + e, o4) + (double)lo_f(ldb(pcl + e, o2));
     };
     double zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
     // raw loads issued one plane ahead: own cells of plane z+1 (nh/nl) and of
     // level n-2 on plane z (qh/ql)
     float nh[CY][CX], qh[CY][CX];
-    uint16_t nl[CY][CX], ql[CY][CX];
+    L nl[CY][CX], ql[CY][CX];
     auto load_halo = [&](size_t pl) {
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
@@ -1260,8 +1272,8 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             zcu[j + 1][CX + 1] = P(pl, j, CX);
         }
     };
-    auto load_raw = [&](size_t pl, size_t ppl_, float (&h)[CY][CX], uint16_t (&l)[CY][CX],
-                        float (&qh_)[CY][CX], uint16_t (&ql_)[CY][CX]) {
+    auto load_raw = [&](size_t pl, size_t ppl_, float (&h)[CY][CX], L (&l)[CY][CX],
+                        float (&qh_)[CY][CX], L (&ql_)[CY][CX]) {
 #pragma unroll
         for (int j = 0; j < CY; ++j)
 #pragma unroll
@@ -1319,7 +1331,7 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     if (c == my_src) inc = inc + my_inc;
                     const size_t ei = ro + (size_t)i * Bp;
                     const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
-                    uint16_t lb;
+                    L lb;
                     const float v = split_hl(nv, lb);
                     stn<USCT_NTF3D>(pn + ei, o4, v);
                     stn<USCT_NTF3D>(pnl + ei, o2, lb);
@@ -1327,7 +1339,7 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     if (has_rec) {
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
                             rec[(size_t)r * Bp + b] = v;
-                            rec_lo[(size_t)r * Bp + b] = bf2f(lb);
+                            rec_lo[(size_t)r * Bp + b] = lo_f(lb);
                         }
                     }
                 }
@@ -2738,12 +2750,12 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 //           k <= 2, whose levels 0 and -1 are zero by definition)
 //   sincd   row k of the FP64 source increments.  Addressing as k_fwd3d.
 // ---------------------------------------------------------------------------
-template <int LB, int CX, int CY>
+template <int LB, int CX, int CY, class L = uint16_t>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
-         const float *__restrict__ pc, const uint16_t *__restrict__ pcl,
-         const float *__restrict__ pk, const uint16_t *__restrict__ pkl, float *__restrict__ po,
-         uint16_t *__restrict__ pol, int wr, const float2 *__restrict__ sp2,
+         const float *__restrict__ pc, const L *__restrict__ pcl,
+         const float *__restrict__ pk, const L *__restrict__ pkl, float *__restrict__ po,
+         L *__restrict__ pol, int wr, const float2 *__restrict__ sp2,
          const int *__restrict__ src_cell, const double *__restrict__ sincd,
          const int *__restrict__ rec_head, const int *__restrict__ rec_next,
          const int *__restrict__ row_rec, const float *__restrict__ res, float *__restrict__ g_hi,
@@ -2769,7 +2781,7 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
     const int my_src = src_cell[b];
     const double my_inc = sincd[b];
     const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
-    const unsigned o2 = (unsigned)(x0 * Bp + b) * 2u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * (unsigned)sizeof(L);
     const unsigned oc = (unsigned)x0 * 8u;
     auto row = [&](int j) -> size_t { return (size_t)min(max(y0 + j, 0), ny - 1); };
     auto F = [&](const float *plane, int j, int i) -> float {
@@ -2781,7 +2793,7 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
     // full state hi + lo of level k-1 at (row j, column i) of a plane, in FP64
     auto P = [&](size_t plane, int j, int i) -> double {
         const size_t e = plane + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
-        return (double)ldb(pc + e, o4) + (double)bf2f(ldb(pcl + e, o2));
+        return (double)ldb(pc + e, o4) + (double)lo_f(ldb(pcl + e, o2));
     };
     float qm[CY][CX], qc[CY + 2][CX + 2], qp[CY][CX], lc[CY][CX], lpn[CY][CX];
     double zm[CY][CX], zcu[CY + 2][CX + 2], zp[CY][CX];
@@ -2805,7 +2817,7 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
     // k-1 on plane z+1 (an, nh/nl), of lam^{k+2} and of level k on plane z
     // (l2n, kh/kl)
     float an[CY][CX], nh[CY][CX], l2n[CY][CX], kh[CY][CX];
-    uint16_t nl[CY][CX], kl[CY][CX];
+    L nl[CY][CX], kl[CY][CX];
     auto load_raw = [&](size_t pl, size_t pl2) {
 #pragma unroll
         for (int j = 0; j < CY; ++j)
@@ -2910,7 +2922,7 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
                     if (c == my_src) inc = inc + my_inc;
                     if (wr) {
                         const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
-                        uint16_t lb;
+                        L lb;
                         const float pv = split_hl(nv, lb);
                         stn<USCT_NTR3D>(po + ei, o4, pv);
                         stn<USCT_NTR3D>(pol + ei, o2, lb);
@@ -2945,11 +2957,11 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
 // ---------------------------------------------------------------------------
 // Plain (one shot) kernels: one thread per cell, dimension-general (2D: nz = 1).
 // ---------------------------------------------------------------------------
-template <bool SPONGE, bool REC>
+template <bool SPONGE, bool REC, class L = uint16_t>
 __global__ void __launch_bounds__(kBlock)
 k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
-            const uint16_t *__restrict__ pcl, const uint16_t *__restrict__ ppl,
-            uint16_t *__restrict__ pnl, const float2 *__restrict__ sp2,
+            const L *__restrict__ pcl, const L *__restrict__ ppl,
+            L *__restrict__ pnl, const float2 *__restrict__ sp2,
             const float *__restrict__ sponge, int src, const double *__restrict__ sincd,
             const int *__restrict__ rec_head, const int *__restrict__ rec_next,
             float *__restrict__ rec, float *__restrict__ rec_lo, float *__restrict__ srcd2, int nz,
@@ -2977,14 +2989,14 @@ k_fwd_plain(const float *__restrict__ pc, const float *__restrict__ pp, float *_
     double nv = (2.0 * P(c) - hl(pp[c], ppl[c])) + inc;
     if (SPONGE) nv = nv * (double)sponge[c];
     if (c == src) *srcd2 = (float)inc;
-    uint16_t lb;
+    L lb;
     const float v = split_hl(nv, lb);
     pn[c] = v;
     pnl[c] = lb;
     if (REC) {
         for (int j = rec_head[c]; j >= 0; j = rec_next[j]) {
             rec[j] = v;
-            rec_lo[j] = bf2f(lb);
+            rec_lo[j] = lo_f(lb);
         }
     }
 }

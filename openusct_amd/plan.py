@@ -46,13 +46,16 @@ class Plan:
 
     def __init__(self, shape, h, dt, nt, tx_lin, rec_lin, method="batched", ckpt_every=0,
                  mem_budget=0, sponge=False, autotune=True, graph=True, forward_only=False,
-                 fuse=True, tx_groups=None, rec_groups=None, reverse=None):
+                 fuse=True, tx_groups=None, rec_groups=None, reverse=None, fp32_lo=False):
         """``tx_lin`` / ``rec_lin``: one linear cell per transmit / receiver.  With
         ``tx_groups`` and ``rec_groups`` (lists of (linear cells, weights) per
         element, as finite-aperture footprints) those replace the point cells and
         ``tx_lin`` / ``rec_lin`` are ignored.  ``reverse`` (3D gradients): None lets
         the plan choose between keeping forward levels and running the forward
-        backwards alongside the adjoint; True / False force one."""
+        backwards alongside the adjoint; True / False force one.  ``fp32_lo`` carries
+        the forward state as FP32 hi + FP32 lo instead of FP32 hi + bfloat16 lo (8
+        bytes per shot-cell instead of 6), for time axes so short that the traces'
+        rounding is a visible share of the residual; no Gauss-Newton products."""
         lib = _lib.load()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -75,7 +78,8 @@ class Plan:
         flags = (_lib.FLAG_SPONGE if sponge else 0) | (_lib.FLAG_AUTOTUNE if autotune else 0) | \
             (0 if graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_FORWARD_ONLY if forward_only else 0) | \
             (0 if fuse else _lib.FLAG_NO_FUSE) | \
-            {None: 0, True: _lib.FLAG_REVERSE, False: _lib.FLAG_NO_REVERSE}[reverse]
+            {None: 0, True: _lib.FLAG_REVERSE, False: _lib.FLAG_NO_REVERSE}[reverse] | \
+            (_lib.FLAG_FP32_LO if fp32_lo else 0)
         dims = (ctypes.c_int * self.ndim)(*self.shape)
         handle = ctypes.c_void_p()
         torch.cuda.init()

@@ -39,6 +39,14 @@ T *alloc(int64_t count, int64_t guard_stride) {
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// A low part from its value: bfloat16 bits (round to nearest even) or FP32.
+template <class L>
+L mk_lo(float r);
+template <>
+uint16_t mk_lo<uint16_t>(float r) { return f2bf(r); }
+template <>
+float mk_lo<float>(float r) { return r; }
+
 struct Problem {
     int ny, nx, Bp, LB, CX, RY, ng, n_rec;
     int64_t N, FE;
@@ -122,6 +130,16 @@ struct Problem {
     }
     float *zeros() { return alloc<float>(FE, Bp); }
     uint16_t *zeros_lo() { return alloc<uint16_t>(FE, Bp); }
+    // low parts of either type (bfloat16 bits or FP32)
+    template <class L>
+    L *lowT(float scale) {
+        L *p = alloc<L>(FE, Bp);
+        std::uniform_real_distribution<float> d(-scale, scale);
+        for (int64_t i = 0; i < FE; ++i) p[i] = mk_lo<L>(d(rng));
+        return p;
+    }
+    template <class L>
+    L *zeros_loT() { return alloc<L>(FE, Bp); }
     float *traces() { return alloc<float>((int64_t)n_rec * Bp, 0); }
 };
 
@@ -138,9 +156,10 @@ bool same(const char *what, const T *a, const T *b, int64_t n, const char *tag) 
 
 // One 2D forward level by a direct loop over cells: the FP64 update the
 // kernels implement (full hi + lo state, FP64 S', source and sponge).
-bool direct_fwd2d(const Problem &P, const float *pc, const uint16_t *pcl, const float *pp,
-                  const uint16_t *ppl, const double *sincd, const float *sponge, const float *pn,
-                  const uint16_t *pnl, const char *tag) {
+template <class L>
+bool direct_fwd2d(const Problem &P, const float *pc, const L *pcl, const float *pp,
+                  const L *ppl, const double *sincd, const float *sponge, const float *pn,
+                  const L *pnl, const char *tag) {
     auto Pv = [&](int64_t c, int b) { return hl(pc[c * P.Bp + b], pcl[c * P.Bp + b]); };
     for (int y = 0; y < P.ny; ++y)
         for (int x = 0; x < P.nx; ++x)
@@ -153,7 +172,7 @@ bool direct_fwd2d(const Problem &P, const float *pc, const uint16_t *pcl, const 
                 if (c == P.src_cell[b]) inc = inc + sincd[b];
                 double nv = (2.0 * Pv(c, b) - hl(pp[c * P.Bp + b], ppl[c * P.Bp + b])) + inc;
                 if (sponge) nv = nv * (double)sponge[c];
-                uint16_t lb;
+                L lb;
                 const float v = split_hl(nv, lb);
                 if (std::memcmp(&v, &pn[c * P.Bp + b], 4) != 0 || lb != pnl[c * P.Bp + b]) {
                     std::printf("MISMATCH %s direct FP64 update at cell %lld shot %d\n", tag,
@@ -164,21 +183,21 @@ bool direct_fwd2d(const Problem &P, const float *pc, const uint16_t *pcl, const 
     return true;
 }
 
-template <int LB, int CX, bool SPONGE>
+template <int LB, int CX, bool SPONGE, class L = uint16_t>
 bool check_fwd_direct(int ny, int nx, int ng, int RY, uint32_t seed) {
     Problem P(ny, nx, LB, ng, CX, RY, seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, "fwd-direct ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d sponge=%d", ny,
-                  nx, LB, ng, CX, RY, (int)SPONGE);
+    std::snprintf(tag, sizeof tag, "fwd-direct ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d sponge=%d lo=%d", ny,
+                  nx, LB, ng, CX, RY, (int)SPONGE, (int)sizeof(L));
     float *pc = P.field(1.0f), *pp = P.field(1.0f), *pn = P.zeros();
-    uint16_t *pcl = P.low(1e-7f), *ppl = P.low(1e-7f), *pnl = P.zeros_lo();
+    L *pcl = P.template lowT<L>(1e-7f), *ppl = P.template lowT<L>(1e-7f), *pnl = P.template zeros_loT<L>();
     float *sp = alloc<float>(P.N, 0);
     std::uniform_real_distribution<float> d(0.5f, 1.0f);
     for (int64_t c = 0; c < P.N; ++c) sp[c] = d(P.rng);
     float *ra = P.traces(), *rl = P.traces();
     const double *sa = P.sincd + P.Bp;
     emu::launch(P.blocks, kBlock, [&] {
-        k_fwd2d<LB, CX, SPONGE, true>(pc, pp, pn, pcl, ppl, pnl, P.sp2, sp, P.src_cell, sa,
+        k_fwd2d<LB, CX, SPONGE, true, L>(pc, pp, pn, pcl, ppl, pnl, P.sp2, sp, P.src_cell, sa,
                                       P.rec_head, P.rec_next, P.row_rec, ra, rl, P.ny, P.nx, P.Bp,
                                       P.RY, P.nxs, P.nyc, P.ng);
     });
@@ -187,31 +206,31 @@ bool check_fwd_direct(int ny, int nx, int ng, int RY, uint32_t seed) {
     return ok;
 }
 
-template <int LB, int CX, bool REC>
+template <int LB, int CX, bool REC, class L = uint16_t>
 bool check_fwd(int ny, int nx, int ng, int RY, uint32_t seed, int sched) {
     Problem P(ny, nx, LB, ng, CX, RY, seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, "fwd ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d rec=%d sched=%d", ny, nx,
-                  LB, ng, CX, RY, (int)REC, sched);
+    std::snprintf(tag, sizeof tag, "fwd ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d rec=%d sched=%d lo=%d", ny,
+                  nx, LB, ng, CX, RY, (int)REC, sched, (int)sizeof(L));
     float *pc = P.field(1.0f), *pp = P.field(1.0f);
-    uint16_t *pcl = P.low(1e-7f), *ppl = P.low(1e-7f);
+    L *pcl = P.template lowT<L>(1e-7f), *ppl = P.template lowT<L>(1e-7f);
     float *a1 = P.zeros(), *b1 = P.zeros(), *a2 = P.zeros(), *b2 = P.zeros();
-    uint16_t *a1l = P.zeros_lo(), *b1l = P.zeros_lo(), *a2l = P.zeros_lo(), *b2l = P.zeros_lo();
+    L *a1l = P.template zeros_loT<L>(), *b1l = P.template zeros_loT<L>(), *a2l = P.template zeros_loT<L>(), *b2l = P.template zeros_loT<L>();
     float *ra1 = P.traces(), *ral1 = P.traces(), *rb1 = P.traces(), *rbl1 = P.traces();
     float *ra2 = P.traces(), *ral2 = P.traces(), *rb2 = P.traces(), *rbl2 = P.traces();
     const double *sa = P.sincd + P.Bp, *sb = P.sincd + 2 * P.Bp;
     emu::launch(P.blocks, kBlock, [&] {
-        k_fwd2d<LB, CX, false, REC>(pc, pp, a1, pcl, ppl, a1l, P.sp2, nullptr, P.src_cell, sa,
+        k_fwd2d<LB, CX, false, REC, L>(pc, pp, a1, pcl, ppl, a1l, P.sp2, nullptr, P.src_cell, sa,
                                     P.rec_head, P.rec_next, P.row_rec, ra1, ral1, P.ny, P.nx, P.Bp,
                                     P.RY, P.nxs, P.nyc, P.ng);
     });
     emu::launch(P.blocks, kBlock, [&] {
-        k_fwd2d<LB, CX, false, REC>(a1, pc, b1, a1l, pcl, b1l, P.sp2, nullptr, P.src_cell, sb,
+        k_fwd2d<LB, CX, false, REC, L>(a1, pc, b1, a1l, pcl, b1l, P.sp2, nullptr, P.src_cell, sb,
                                     P.rec_head, P.rec_next, P.row_rec, rb1, rbl1, P.ny, P.nx, P.Bp,
                                     P.RY, P.nxs, P.nyc, P.ng);
     });
     emu::launch(P.blocks, kBlock, [&] {
-        k_fwd2d_x2<LB, CX, REC>(pc, pp, a2, b2, pcl, ppl, a2l, b2l, P.sp2, P.src_cell, sa, sb,
+        k_fwd2d_x2<LB, CX, REC, L>(pc, pp, a2, b2, pcl, ppl, a2l, b2l, P.sp2, P.src_cell, sa, sb,
                                 P.rec_head, P.rec_next, P.row_rec, ra2, ral2, rb2, rbl2, P.ny,
                                 P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng, sched);
     });
@@ -299,26 +318,26 @@ bool adj_cases(uint32_t seed) {
 // (full hi + lo state, FP64 S' and source term, split into FP32 hi + bf16 lo).
 // KIND 0: k_fwd3d with CX x CY patches per lane; KIND 1: k_fwd3d_lds with a
 // TX x TY tile per work-group (CX, CY = TX, TY) of PX x PY patches per lane.
-template <int LB, int CX, int CY, int KIND = 0, int PX = 1, int PY = 1>
+template <int LB, int CX, int CY, int KIND = 0, int PX = 1, int PY = 1, class L = uint16_t>
 bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     const int Bp = LB * ng, W = 64 / LB;
     const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
     std::mt19937 rng(seed);
     char tag[160];
     std::snprintf(tag, sizeof tag, "%s nz=%d ny=%d nx=%d LB=%d ng=%d C/T=%dx%d P=%dx%d RZ=%d",
-                  KIND ? "fwd3d-lds" : "fwd3d", nz, ny, nx, LB, ng, CX, CY, PX, PY, RZ);
+                  KIND ? "fwd3d-lds" : (sizeof(L) == 4 ? "fwd3d-lo32" : "fwd3d"), nz, ny, nx, LB, ng, CX, CY, PX, PY, RZ);
     std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
         dl(-1e-7f, 1e-7f);
     float2 *sp2 = alloc<float2>(N, 1);
     for (int64_t c = 0; c < N; ++c) sp2[c] = make_float2(u(rng), tt(rng));
     float *pc = alloc<float>(FE, Bp), *pp = alloc<float>(FE, Bp), *pn = alloc<float>(FE, Bp);
-    uint16_t *pcl = alloc<uint16_t>(FE, Bp), *ppl = alloc<uint16_t>(FE, Bp),
-             *pnl = alloc<uint16_t>(FE, Bp);
+    L *pcl = alloc<L>(FE, Bp), *ppl = alloc<L>(FE, Bp),
+             *pnl = alloc<L>(FE, Bp);
     for (int64_t i = 0; i < FE; ++i) {
         pc[i] = d(rng);
         pp[i] = d(rng);
-        pcl[i] = f2bf(dl(rng));
-        ppl[i] = f2bf(dl(rng));
+        pcl[i] = mk_lo<L>(dl(rng));
+        ppl[i] = mk_lo<L>(dl(rng));
     }
     int *src = alloc<int>(Bp, 0);
     std::uniform_int_distribution<int> cz(0, nz - 1), cy(0, ny - 1), cx(0, nx - 1);
@@ -343,7 +362,7 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
         const int nzc = (int)cdiv(nz, RZ);
         const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
         emu::launch(blocks, kBlock, [&] {
-            k_fwd3d<LB, CX, CY, true>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head,
+            k_fwd3d<LB, CX, CY, true, L>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head,
                                       rec_next, row_rec, rec, rec_lo, srcd2, nz, ny, nx, Bp, RZ,
                                       nxs, nys, nzc, ng);
         });
@@ -357,7 +376,7 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
         });
     }
     bool ok = true;
-    auto Pv = [&](int64_t c, int b) { return (double)pc[c * Bp + b] + (double)bf2f(pcl[c * Bp + b]); };
+    auto Pv = [&](int64_t c, int b) { return (double)pc[c * Bp + b] + (double)lo_f(pcl[c * Bp + b]); };
     for (int z = 0; z < nz && ok; ++z)
         for (int y = 0; y < ny && ok; ++y)
             for (int x = 0; x < nx && ok; ++x)
@@ -369,10 +388,10 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
                                    Pv(c + nx, b), Pv(c - nx, b), Pv(c + 1, b), Pv(c - 1, b));
                     double inc = ((double)sp2[c].x + (double)sp2[c].y) * lu;
                     if (c == src[b]) inc = inc + sincd[b];
-                    const double q = (double)pp[c * Bp + b] + (double)bf2f(ppl[c * Bp + b]);
+                    const double q = (double)pp[c * Bp + b] + (double)lo_f(ppl[c * Bp + b]);
                     const double nv = (2.0 * Pv(c, b) - q) + inc;
                     const float v = (float)nv;
-                    const uint16_t lb = f2bf((float)(nv - (double)v));
+                    const L lb = mk_lo<L>((float)(nv - (double)v));
                     if (std::memcmp(&v, &pn[c * Bp + b], 4) != 0 || lb != pnl[c * Bp + b]) {
                         std::printf("MISMATCH %s cell %lld shot %d\n", tag, (long long)c, b);
                         ok = false;
@@ -383,7 +402,7 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
                         ok = false;
                     }
                     for (int r = rec_head[c]; r >= 0; r = rec_next[r])
-                        if (rec[(int64_t)r * Bp + b] != v || rec_lo[(int64_t)r * Bp + b] != bf2f(lb)) {
+                        if (rec[(int64_t)r * Bp + b] != v || rec_lo[(int64_t)r * Bp + b] != lo_f(lb)) {
                             std::printf("MISMATCH %s trace %d shot %d\n", tag, r, b);
                             ok = false;
                         }
@@ -495,13 +514,13 @@ bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
 // FP64 leapfrog with p^k in the place of p^{n-2}), the imaging sums of
 // lam^k * (float)inc against a direct FP64 sum; with wr = 0 level k-2 is left
 // as it was.
-template <int LB, int CX, int CY>
+template <int LB, int CX, int CY, class L = uint16_t>
 bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed) {
     const int Bp = LB * ng, W = 64 / LB;
     const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
     std::mt19937 rng(seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, "radj3d nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d", nz,
+    std::snprintf(tag, sizeof tag, sizeof(L) == 4 ? "radj3d-lo32 nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d" : "radj3d nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d", nz,
                   ny, nx, LB, ng, CX, CY, RZ, wr);
     std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
         dl(-1e-7f, 1e-7f);
@@ -509,15 +528,15 @@ bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed)
     for (int64_t c = 0; c < N; ++c) sp2[c] = make_float2(u(rng), tt(rng));
     float *l1 = alloc<float>(FE, Bp), *l2 = alloc<float>(FE, Bp), *lk = alloc<float>(FE, Bp);
     float *pc = alloc<float>(FE, Bp), *pk = alloc<float>(FE, Bp), *po = alloc<float>(FE, Bp);
-    uint16_t *pcl = alloc<uint16_t>(FE, Bp), *pkl = alloc<uint16_t>(FE, Bp),
-             *pol = alloc<uint16_t>(FE, Bp);
+    L *pcl = alloc<L>(FE, Bp), *pkl = alloc<L>(FE, Bp),
+             *pol = alloc<L>(FE, Bp);
     for (int64_t i = 0; i < FE; ++i) {
         l1[i] = d(rng);
         l2[i] = d(rng);
         pc[i] = d(rng);
         pk[i] = d(rng);
-        pcl[i] = f2bf(dl(rng));
-        pkl[i] = f2bf(dl(rng));
+        pcl[i] = mk_lo<L>(dl(rng));
+        pkl[i] = mk_lo<L>(dl(rng));
         po[i] = 7.0f;
         pol[i] = 0x1234;
     }
@@ -544,13 +563,13 @@ bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed)
     const int nzc = (int)cdiv(nz, RZ);
     const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
     emu::launch(blocks, kBlock, [&] {
-        k_radj3d<LB, CX, CY>(l1, l2, lk, pc, pcl, pk, pkl, po, pol, wr, sp2, src, sincd, rec_head,
+        k_radj3d<LB, CX, CY, L>(l1, l2, lk, pc, pcl, pk, pkl, po, pol, wr, sp2, src, sincd, rec_head,
                              rec_next, row_rec, res, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
     });
     bool ok = true;
     const int64_t sz = (int64_t)nx * ny;
     std::vector<double> gref(ng * N, 0.0);
-    auto Pv = [&](int64_t c, int b) { return (double)pc[c * Bp + b] + (double)bf2f(pcl[c * Bp + b]); };
+    auto Pv = [&](int64_t c, int b) { return (double)pc[c * Bp + b] + (double)lo_f(pcl[c * Bp + b]); };
     for (int64_t c = 0; c < N && ok; ++c) {
         const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / sz);
         const bool in = z >= 1 && z <= nz - 2 && y >= 1 && y <= ny - 2 && x >= 1 && x <= nx - 2;
@@ -572,13 +591,13 @@ bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed)
                            Pv(c + 1, b), Pv(c - 1, b));
             double inc = ((double)sp2[c].x + (double)sp2[c].y) * lu;
             if (c == src[b]) inc = inc + sincd[b];
-            const double qk = (double)pk[c * Bp + b] + (double)bf2f(pkl[c * Bp + b]);
+            const double qk = (double)pk[c * Bp + b] + (double)lo_f(pkl[c * Bp + b]);
             const double nv = (2.0 * Pv(c, b) - qk) + inc;
             float pv = 7.0f;
-            uint16_t lb = 0x1234;
+            L lb = 0x1234;
             if (wr) {
                 pv = (float)nv;
-                lb = f2bf((float)(nv - (double)pv));
+                lb = mk_lo<L>((float)(nv - (double)pv));
             }
             if (std::memcmp(&pv, &po[c * Bp + b], 4) != 0 || lb != pol[c * Bp + b]) {
                 std::printf("MISMATCH %s level k-2 cell %lld shot %d\n", tag, (long long)c, b);
@@ -961,6 +980,26 @@ bool check_fwd3d_x2p(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     return ok;
 }
 
+// FP32 low parts (USCT_FLAG_FP32_LO): the 2D forward against its direct FP64 loop,
+// the two-level 2D forward against two single-level launches, the 3D forward and
+// the reversing 3D adjoint against their direct loops.
+bool lo32_cases(uint32_t seed) {
+    bool ok = true;
+    ok &= check_fwd_direct<64, 4, false, float>(29, 37, 2, 8, seed++);
+    ok &= check_fwd_direct<16, 8, true, float>(9, 13, 1, 4, seed++);
+    for (int sched = 0; sched < 4; ++sched) {
+        ok &= check_fwd<64, 4, true, float>(29, 37, 2, 8, seed++, sched);
+        ok &= check_fwd<16, 8, true, float>(40, 33, 2, 4, seed++, sched);
+    }
+    ok &= check_fwd<32, 4, true, float>(3, 5, 2, 64, seed++, 0);
+    ok &= check_fwd3d<64, 2, 2, 0, 1, 1, float>(9, 7, 11, 2, 4, seed++);
+    ok &= check_fwd3d<16, 1, 2, 0, 1, 1, float>(5, 6, 13, 1, 64, seed++);
+    ok &= check_radj3d<64, 2, 2, float>(9, 7, 11, 2, 4, 1, seed++);
+    ok &= check_radj3d<32, 1, 1, float>(12, 9, 5, 2, 8, 0, seed++);
+    ok &= check_radj3d<16, 2, 1, float>(5, 6, 13, 1, 64, 1, seed++);
+    return ok;
+}
+
 bool x2p_3d_cases(uint32_t seed) {
     bool ok = true;
     ok &= check_fwd3d_x2p<64, 1, 1>(9, 7, 11, 2, 4, seed++);
@@ -1319,6 +1358,9 @@ int main(int argc, char **argv) {
     }
     if (only.empty() || only == "lds") {
         ok &= lds_cases(seed + 2000);
+    }
+    if (only.empty() || only == "lo32") {
+        ok &= lo32_cases(seed + 9500);
     }
     if (only.empty() || only == "x2p-3d") {
         ok &= x2p_3d_cases(seed + 9000);

@@ -418,7 +418,7 @@ def _cylinder21(nt):
     return cyl, h, dt, w, mt, phantom.velocity_to_m(c), [0, 7]
 
 
-def _cylinder21_error(nt, reverse):
+def _cylinder21_error(nt, reverse, fp32_lo=False):
     from oracle import oracle as O
     from openusct_amd.plan import Plan, linear_indices
     cyl, h, dt, w, mt, m0, src = _cylinder21(nt)
@@ -426,12 +426,15 @@ def _cylinder21_error(nt, reverse):
     J_ref, g_ref = O.misfit_and_gradient(m0, cyl, w, dt, h, nt, d, None, src)
     rec = linear_indices(cyl.idx, m0.shape)
     dev = torch.device("cuda")
-    with Plan(m0.shape, h, dt, nt, rec[src], rec, reverse=reverse) as p:
-        print(f"[case] 21^3 cylinder nt={nt} reverse={reverse} {p.info}", flush=True)
+    with Plan(m0.shape, h, dt, nt, rec[src], rec, reverse=reverse, fp32_lo=fp32_lo) as p:
+        print(f"[case] 21^3 cylinder nt={nt} reverse={reverse} fp32_lo={fp32_lo} {p.info}",
+              flush=True)
+        dd = p.forward_fmc(torch.tensor(mt, device=dev), torch.tensor(w, device=dev)).cpu().numpy()
         J, g = p.misfit_and_gradient(torch.tensor(m0, device=dev), torch.tensor(w, device=dev),
                                      torch.tensor(d, device=dev))
     e = (abs(J.item() - J_ref) / J_ref, rel_l2(g.cpu().numpy(), g_ref))
-    print(f"21^3 cylinder nt={nt} reverse={reverse}: J {e[0]:.2e}  gradient {e[1]:.2e}")
+    print(f"21^3 cylinder nt={nt} reverse={reverse} fp32_lo={fp32_lo}: traces {rel_l2(dd, d):.2e}  "
+          f"J {e[0]:.2e}  gradient {e[1]:.2e}")
     return e
 
 
@@ -450,3 +453,47 @@ def test_short_axis_cylinder_200_levels(reverse):
 def test_short_axis_cylinder_120_levels(reverse):
     e = _cylinder21_error(120, reverse)
     assert e[0] < TOL and e[1] < TOL
+
+
+@pytest.mark.parametrize("reverse", [True, False])
+def test_short_axis_cylinder_120_levels_fp32_lo(reverse):
+    """The case above with the forward state carried as FP32 hi + FP32 lo
+    (USCT_FLAG_FP32_LO): the traces' rounding drops by 2^-16 and the gradient
+    meets the bar."""
+    e = _cylinder21_error(120, reverse, fp32_lo=True)
+    assert e[0] < TOL and e[1] < TOL
+
+
+def test_fp32_lo_plans_agree_and_refuse_gauss_newton():
+    """With FP32 low parts: the 256^2 problem's traces are closer to the FP64
+    oracle than with bfloat16 ones; plain and batched traces are bit-identical,
+    and so are the gradients of one and two levels per launch; a Gauss-Newton
+    product is refused."""
+    from oracle import oracle as O
+    from openusct_amd import configs
+    from openusct_amd.plan import Plan, linear_indices
+    prob = configs.c1(nt_scale=0.5, shots=4)
+    rec = linear_indices(prob.geom.idx, prob.shape)
+    tx = rec[prob.src_list]
+    dev = torch.device("cuda")
+    mt, m0 = torch.tensor(prob.m_true, device=dev), torch.tensor(prob.m0, device=dev)
+    w = torch.tensor(prob.wavelet, device=dev)
+    dref = O.forward_fmc(prob.m_true, prob.geom, prob.wavelet, prob.dt, prob.h, prob.nt, None,
+                         prob.src_list)
+    out = {}
+    for name, kw in (("bf16", {}), ("fp32", {"fp32_lo": True, "autotune": False}),
+                     ("fp32 plain", {"fp32_lo": True, "method": "plain"}),
+                     ("fp32 one level", {"fp32_lo": True, "fuse": False, "autotune": False})):
+        with Plan(prob.shape, prob.h, prob.dt, prob.nt, tx, rec, **kw) as p:
+            d = p.forward_fmc(mt, w)
+            J, g = p.misfit_and_gradient(m0, w, torch.tensor(dref, device=dev))
+            out[name] = (d.cpu().numpy(), J.item(), g.cpu().numpy())
+            if name == "fp32":
+                with pytest.raises(NotImplementedError, match="Gauss-Newton"):
+                    p.gn_hessian_vector(m0, w, m0.clone())
+    e16, e32 = rel_l2(out["bf16"][0], dref), rel_l2(out["fp32"][0], dref)
+    print(f"[case] 256^2, 4 shots, nt {prob.nt}: traces vs oracle bf16 lo {e16:.2e}, fp32 lo {e32:.2e}")
+    assert e32 < 0.1 * e16
+    assert np.array_equal(out["fp32"][0], out["fp32 plain"][0])
+    assert out["fp32"][1] == out["fp32 one level"][1]
+    assert np.array_equal(out["fp32"][2], out["fp32 one level"][2])
