@@ -103,8 +103,12 @@ typedef struct {
     int reverse;              /* 1: the gradient keeps no forward levels; the adjoint sweep
                                  runs the forward backwards (3D, no sponge, point elements) */
     int sched;                /* 2D two-level adjoint kernel: row-chunk order and march
-                                 (bit 0: odd chunks march upwards; bit 1: chunk index fastest) */
+                                 (bit 0: odd chunks march upwards; bit 1: chunk index fastest;
+                                 bit 2: shot group slowest) */
     int fwd_sched;            /* the same for the two-level forward kernel */
+    int strip_block;          /* 3D patch adjoint kernel: wavefront order over blocks of
+                                 (strip_block >> 8) x (strip_block & 255) x-by-y strips */
+    int fwd_strip_block;      /* the same for the forward kernel */
 } usct_plan_info;
 
 /*

@@ -53,6 +53,8 @@ class PlanInfo(ctypes.Structure):
         ("reverse", ctypes.c_int),
         ("sched", ctypes.c_int),
         ("fwd_sched", ctypes.c_int),
+        ("strip_block", ctypes.c_int),
+        ("fwd_strip_block", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -82,7 +82,9 @@ struct Tiling {
                          // tiles advancing two forward levels per launch; 3 = CX x CY lane
                          // patches advancing two forward levels per launch (k_fwd3d_x2)
     int sched = 0;       // 2D two-level kernels: bit 0 = odd row chunks march upwards,
-                         // bit 1 = row chunk fastest in the wavefront order
+                         // bit 1 = row chunk fastest in the wavefront order, bit 2 = shot
+                         // group slowest
+    int blk = kBlk3dDefault;  // 3D patch kernels: strip blocks of (blk >> 8) x (blk & 255)
 };
 
 // Lane patch of an LDS tile: 256 lanes = LB shots x (TX/PX) x (TY/PY) patches.
@@ -267,7 +269,7 @@ Tile tile_of(const usct_plan *P, const Tiling &T) {
     } else {
         t.nys = (int)cdiv(P->ny, T.CY);
         t.nyc = (int)cdiv(P->nz, T.RY);
-        t.waves = ng * t.nxs * round_ys(t.nys) * t.nyc;
+        t.waves = ng * tasks3d(t.nxs, t.nys, t.nyc, T.blk);
     }
     return t;
 }
@@ -321,7 +323,7 @@ void fwd3d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
                            pn.hi, (const L *)pc.lo, (const L *)pp.lo, (L *)pn.lo, P->sp2,
                            P->src_cell, sd, P->rec_head, P->rec_next, P->row_rec, rec, rl, d2,
                            P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys, t.nyc,
-                           P->Bp / P->tf.LB);
+                           P->Bp / P->tf.LB, P->tf.blk);
     };
     if (P->lo32) go(float{});
     else go(uint16_t{});
@@ -460,7 +462,7 @@ void fwd3d_x2_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pa, Lvl pb, const usc
                        pb.hi, pc.lo, pp.lo, pa.lo, pb.lo, P->sp2, P->src_cell, sa, sa + P->Bp,
                        P->rec_head, P->rec_next, P->row_rec, ra, rla, ra + tr, rla + tr, da,
                        da + P->Bp, P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys, t.nyc,
-                       P->Bp / P->tf.LB);
+                       P->Bp / P->tf.LB, P->tf.blk);
 }
 
 template <int LB, bool REC>
@@ -582,7 +584,8 @@ void adj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk,
     const float *d2 = P->srcd2 + (sinc - P->sinc);
     hipLaunchKernelGGL((k_adj3d<LB, CX, CY>), g, dim3(kBlock), 0, s, l1, l2, lk, pa, P->sp2,
                        P->src_cell, sinc, P->rec_head, P->rec_next, P->row_rec, res, d2, P->g_hi,
-                       P->g_lo, P->nz, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nys, t.nyc, P->ng_adj());
+                       P->g_lo, P->nz, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nys, t.nyc, P->ng_adj(),
+                       P->ta.blk);
 }
 
 template <int LB>
@@ -651,7 +654,7 @@ void radj3d_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *lk
                            (const L *)pc.lo, pk.hi, (const L *)pk.lo, po.hi, (L *)po.lo, wr, P->sp2,
                            P->src_cell, sd, P->rec_head, P->rec_next, P->row_rec, res, P->g_hi,
                            P->g_lo, P->nz, P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nys, t.nyc,
-                           P->ng_adj());
+                           P->ng_adj(), P->ta.blk);
     };
     if (P->lo32) go(float{});
     else go(uint16_t{});
@@ -886,7 +889,7 @@ void born3_lb(hipStream_t s, const usct_plan *P, int n, Lvl u2, Lvl u1, Lvl u0) 
                        u0.hi, u1.lo, u2.lo, u0.lo, P->Dh[b1], P->Dh[c2], P->Dh[a], P->Dl[b1],
                        P->Dl[c2], P->Dl[a], P->sp2, P->vm2, P->src_cell, sd, P->rec_head,
                        P->rec_next, P->row_rec, res, d2, P->nz, P->ny, P->nx, P->Bp, tg.RY, t.nxs,
-                       t.nys, t.nyc, P->Bp / tg.LB);
+                       t.nys, t.nyc, P->Bp / tg.LB, tg.blk);
 }
 
 void launch_born3_batched(const usct_plan *P, hipStream_t s, int n, Lvl u2, Lvl u1, Lvl u0) {
@@ -1578,7 +1581,7 @@ int autotune(usct_plan *P) {
                             rys.push_back(ry);
                     }
                     for (int ry : rys)
-                        for (int sc = 0; sc < (x2 ? 4 : 1); ++sc)
+                        for (int sc = 0; sc < (x2 ? 8 : 1); ++sc)
                             cands.push_back({lb, cx, 1, ry, 0, sc});
                 }
         } else if (adjoint && P->rev) {
@@ -1633,6 +1636,19 @@ int autotune(usct_plan *P) {
             if (t < best) { best = t; bt = c; }
         }
         T = bt;
+        // 3D patch kernels: the order of the strips (blocks of bx x by strips) for
+        // the chosen tile, so that strips sharing halo cells run together
+        if (P->ndim == 3 && (T.lds == 0 || T.lds == 3)) {
+            for (int blk : {(2 << 8) | 2, (4 << 8) | 4}) {
+                T.blk = blk;
+                const double t = time_tile(P, adjoint != 0);
+                if (log && *log == '1')
+                    std::fprintf(stderr, "[usct tune] %s strip blocks %dx%d: %.3f ms\n",
+                                 adjoint ? "adjoint" : "forward", blk >> 8, blk & 255, t);
+                if (t < best) { best = t; bt = T; }
+            }
+            T = bt;
+        }
     }
     g_tuned[key] = TuneVal{P->tf, P->ta};
     return 0;
@@ -2211,6 +2227,8 @@ int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
     info->reverse = P->rev ? 1 : 0;
     info->sched = P->ta.sched;
     info->fwd_sched = P->tf.sched;
+    info->strip_block = P->ta.blk;
+    info->fwd_strip_block = P->tf.blk;
     return 0;
 }
 

@@ -76,26 +76,29 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     return xcd * per + (xcd < rem ? xcd : rem) + slot;
 }
 
-// Strip order of the 3D patch kernels (k_fwd3d, k_adj3d): the wavefronts of a
-// work-group take USCT_GY3D consecutive y strips of one x strip, so the y-halo
-// rows they share are fetched once by one CU instead of by blocks far apart.
-// The grid covers the y strips rounded up to a multiple of USCT_GY3D.
-#ifndef USCT_GY3D
-#define USCT_GY3D 2
-#endif
-__host__ __device__ inline int round_ys(int nys) {
-    return (nys + USCT_GY3D - 1) / USCT_GY3D * USCT_GY3D;
+// Strip order of the 3D patch kernels (k_fwd3d, k_adj3d, k_radj3d, k_born3d,
+// k_fwd3d_x2): consecutive wavefronts take the strips of a block of bx x by
+// strips (y fastest), blocks follow along x, then along y, then z chunks, so
+// strips whose halo cells neighbour each other run at about the same time on
+// the same XCD and the halo rows and columns they share are fetched from L2.
+// blk = (bx << 8) | by, both in [1, 255]; bx = 1, by = 2 pairs two y strips of
+// one x strip.  The grid covers the strips rounded up to whole blocks.
+constexpr int kBlk3dDefault = (1 << 8) | 2;
+__host__ __device__ inline int64_t tasks3d(int nxs, int nys, int nzc, int blk) {
+    const int by = blk & 255, bx = blk >> 8;
+    return (int64_t)((nxs + bx - 1) / bx * bx) * ((nys + by - 1) / by * by) * nzc;
 }
-__device__ __forceinline__ bool strip3d(int t, int nxs, int nys, int nzc, int &xs, int &ys,
-                                        int &zc) {
-    const int q = t % USCT_GY3D;
-    t /= USCT_GY3D;
-    xs = t % nxs;
-    t /= nxs;
-    const int nyg = (nys + USCT_GY3D - 1) / USCT_GY3D;
-    ys = (t % nyg) * USCT_GY3D + q;
-    zc = t / nyg;
-    return ys < nys && zc < nzc;
+__device__ __forceinline__ bool strip3d(int t, int nxs, int nys, int nzc, int blk, int &xs,
+                                        int &ys, int &zc) {
+    const int by = blk & 255, bx = blk >> 8;
+    const int q = t % (bx * by);
+    t /= bx * by;
+    const int nbx = (nxs + bx - 1) / bx, nby = (nys + by - 1) / by;
+    xs = (t % nbx) * bx + q / by;
+    t /= nbx;
+    ys = (t % nby) * by + q % by;
+    zc = t / nby;
+    return xs < nxs && ys < nys && zc < nzc;
 }
 
 // Global wavefront index (XCD-remapped block, then wave within the block),
@@ -420,7 +423,10 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 //          ones at their starts;
 //   bit 1  consecutive wavefronts take consecutive row chunks of one strip
 //          (chunk index fastest) instead of consecutive strips of one chunk,
-//          so that vertically neighbouring chunks run on the same XCD.
+//          so that vertically neighbouring chunks run on the same XCD;
+//   bit 2  shot groups slowest instead of fastest: the wavefronts of a
+//          work-group take neighbouring strips (or chunks) of one shot group
+//          and read each other's halo columns on one CU.
 // Only the order in which rows are visited changes: every value is formed
 // from the same operands in the same order, so the results do not depend on
 // sched.  Rows are named relative to the front row yf and the march
@@ -429,10 +435,20 @@ k_fwd2d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
 // Row index clamped to the grid (halo rows of a wavefront read the edge row).
 __device__ __forceinline__ size_t clamp_row(int y, int ny) { return (size_t)min(max(y, 0), ny - 1); }
 
-// Strip and row chunk of a 2D wavefront task t (shot groups excluded), with the
-// march direction; false for the tasks that round the grid up to whole blocks.
-__device__ __forceinline__ bool chunk2d(int t, int nxs, int nyc, int sched, int &xs, int &yc,
-                                        int &dir) {
+// Shot group, strip and row chunk of 2D wavefront wv, with the march direction;
+// false for the wavefronts that round the grid up to whole blocks.  Shot groups
+// are fastest (the wavefronts of a work-group share strips) unless sched bit 2
+// is set (a work-group holds neighbouring strips of one shot group).
+__device__ __forceinline__ bool chunk2d(int wv, int ng, int nxs, int nyc, int sched, int &sg,
+                                        int &xs, int &yc, int &dir) {
+    int t;
+    if (sched & 4) {
+        sg = wv / (nxs * nyc);
+        t = wv % (nxs * nyc);
+    } else {
+        sg = wv % ng;
+        t = wv / ng;
+    }
     if (sched & 2) {
         yc = t % nyc;
         xs = t / nyc;
@@ -441,7 +457,7 @@ __device__ __forceinline__ bool chunk2d(int t, int nxs, int nyc, int sched, int 
         yc = t / nxs;
     }
     dir = ((sched & 1) && (yc & 1)) ? -1 : 1;
-    return xs < nxs && yc < nyc;
+    return sg < ng && xs < nxs && yc < nyc;
 }
 
 template <int LB, int CX, bool REC, class L = uint16_t>
@@ -460,9 +476,8 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
     constexpr int CA = CX + 2;  // A columns x0-1 .. x0+CX
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
-    const int sg = wv % ng;
-    int xs, yc, dir;
-    if (!chunk2d(wv / ng, nxs, nyc, sched, xs, yc, dir)) return;
+    int sg, xs, yc, dir;
+    if (!chunk2d(wv, ng, nxs, nyc, sched, sg, xs, yc, dir)) return;
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -992,9 +1007,8 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
     constexpr int CA = CX + 2;  // U columns x0-1 .. x0+CX
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
-    const int sg = wv % ng;
-    int xs, yc, dir;
-    if (!chunk2d(wv / ng, nxs, nyc, sched, xs, yc, dir)) return;  // uniform over the wavefront
+    int sg, xs, yc, dir;
+    if (!chunk2d(wv, ng, nxs, nyc, sched, sg, xs, yc, dir)) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -1226,13 +1240,13 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
         const int *__restrict__ row_rec, float *__restrict__ rec, float *__restrict__ rec_lo,
         float *__restrict__ srcd2, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
-        int nzc, int ng) {
+        int nzc, int ng, int blk) {
     constexpr int W = kWave / LB;
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     int xs, ys, zc;
-    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;
+    if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -1426,7 +1440,7 @@ k_fwd3d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
            const int *__restrict__ rec_next, const int *__restrict__ row_rec,
            float *__restrict__ rec_a, float *__restrict__ rec_lo_a, float *__restrict__ rec_b,
            float *__restrict__ rec_lo_b, float *__restrict__ srcd2_a, float *__restrict__ srcd2_b,
-           int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+           int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng, int blk) {
     using R = Reg3<CX, CY>;
     constexpr R T{};
     constexpr int NO = R::NO, NA = R::NA, NP = R::NP;
@@ -1435,7 +1449,7 @@ k_fwd3d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     int xs, ys, zc;
-    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;
+    if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -1614,13 +1628,13 @@ k_born3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__re
          const int *__restrict__ src_cell, const double *__restrict__ sincd,
          const int *__restrict__ rec_head, const int *__restrict__ rec_next,
          const int *__restrict__ row_rec, float *__restrict__ res, float *__restrict__ srcd2,
-         int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+         int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng, int blk) {
     constexpr int W = kWave / LB;
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     int xs, ys, zc;
-    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;
+    if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -2563,14 +2577,14 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
         const int *__restrict__ rec_head, const int *__restrict__ rec_next,
         const int *__restrict__ row_rec, const float *__restrict__ res,
         const float *__restrict__ srcd2, float *__restrict__ g_hi, float *__restrict__ g_lo,
-        int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng) {
+        int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys, int nzc, int ng, int blk) {
     constexpr int W = kWave / LB;
     constexpr int CC = CX * CY;
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     int xs, ys, zc;
-    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;  // uniform over the wavefront
+    if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -2760,14 +2774,14 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
          const int *__restrict__ rec_head, const int *__restrict__ rec_next,
          const int *__restrict__ row_rec, const float *__restrict__ res, float *__restrict__ g_hi,
          float *__restrict__ g_lo, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
-         int nzc, int ng) {
+         int nzc, int ng, int blk) {
     constexpr int W = kWave / LB;
     constexpr int CC = CX * CY;
     const int wv = wave_id();
     const int lane = threadIdx.x & 63;
     const int sg = wv % ng;
     int xs, ys, zc;
-    if (!strip3d(wv / ng, nxs, nys, nzc, xs, ys, zc)) return;  // uniform over the wavefront
+    if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;

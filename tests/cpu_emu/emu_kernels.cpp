@@ -39,6 +39,9 @@ T *alloc(int64_t count, int64_t guard_stride) {
 
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// strip order of the 3D patch kernels in the current check (tasks3d / strip3d)
+int g_blk = kBlk3dDefault;
+
 // A low part from its value: bfloat16 bits (round to nearest even) or FP32.
 template <class L>
 L mk_lo(float r);
@@ -248,10 +251,10 @@ template <int LB, int CX>
 bool fwd_cases(uint32_t seed) {
     bool ok = true;
     const int shapes[][2] = {{29, 37}, {9, 13}, {40, 33}, {3, 5}};
-    // every row-chunk order and march direction (sched 0..3)
+    // every wavefront order and march direction (sched 0..7)
     for (auto &s : shapes)
         for (int RY : {4, 8, 64})
-            for (int sched = 0; sched < 4; ++sched)
+            for (int sched = 0; sched < 8; ++sched)
                 ok &= check_fwd<LB, CX, true>(s[0], s[1], 2, RY, seed++, sched);
     ok &= check_fwd<LB, CX, false>(29, 37, 1, 4, seed++, 3);
     ok &= check_fwd_direct<LB, CX, false>(29, 37, 2, 8, seed++);
@@ -308,7 +311,7 @@ bool adj_cases(uint32_t seed) {
     const int shapes[][2] = {{29, 37}, {9, 13}, {40, 33}, {3, 5}};
     for (auto &s : shapes)
         for (int RY : {4, 8, 64})
-            for (int sched = 0; sched < 4; ++sched)
+            for (int sched = 0; sched < 8; ++sched)
                 ok &= check_adj<LB, CX>(s[0], s[1], 2, RY, seed++, sched);
     ok &= check_adj<LB, CX>(29, 37, 1, 4, seed++, 3);
     return ok;
@@ -360,11 +363,11 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     if constexpr (KIND == 0) {
         const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d<LB, CX, CY, true, L>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head,
                                       rec_next, row_rec, rec, rec_lo, srcd2, nz, ny, nx, Bp, RZ,
-                                      nxs, nys, nzc, ng);
+                                      nxs, nys, nzc, ng, g_blk);
         });
     } else {
         const int ntx = (int)cdiv(nx, CX), nty = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
@@ -454,10 +457,10 @@ bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     if constexpr (KIND == 0) {
         const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
         emu::launch(blocks, kBlock, [&] {
             k_adj3d<LB, CX, CY>(l1, l2, lk, pa, sp2, src, sinc, rec_head, rec_next, row_rec, res,
-                                srcd2, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
+                                srcd2, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng, g_blk);
         });
     } else {
         const int ntx = (int)cdiv(nx, CX), nty = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
@@ -561,10 +564,11 @@ bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed)
     float *gh = alloc<float>(ng * N, 0), *gl = alloc<float>(ng * N, 0);
     const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
     const int nzc = (int)cdiv(nz, RZ);
-    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+    const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
     emu::launch(blocks, kBlock, [&] {
         k_radj3d<LB, CX, CY, L>(l1, l2, lk, pc, pcl, pk, pkl, po, pol, wr, sp2, src, sincd, rec_head,
-                             rec_next, row_rec, res, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng);
+                                rec_next, row_rec, res, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
+                                ng, g_blk);
     });
     bool ok = true;
     const int64_t sz = (int64_t)nx * ny;
@@ -673,16 +677,16 @@ bool check_born3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     float *s2_a = alloc<float>(Bp, 0), *s2_b = alloc<float>(Bp, 0);
     const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
     const int nzc = (int)cdiv(nz, RZ);
-    const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+    const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
     emu::launch(blocks, kBlock, [&] {
         k_born3d<LB, CX, CY>(pc, pp, pn_a, pcl, ppl, pnl_a, dc, dp, dn_a, dcl, dpl, dnl_a, sp2, vm2,
                              src, sincd, rec_head, rec_next, row_rec, res_a, s2_a, nz, ny, nx, Bp,
-                             RZ, nxs, nys, nzc, ng);
+                             RZ, nxs, nys, nzc, ng, g_blk);
     });
     emu::launch(blocks, kBlock, [&] {
         k_fwd3d<LB, CX, CY, false>(pc, pp, pn_b, pcl, ppl, pnl_b, sp2, src, sincd, rec_head, rec_next,
                                    row_rec, nullptr, nullptr, s2_b, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
-                                   ng);
+                                   ng, g_blk);
     });
     emu::launch((unsigned)cdiv(FE, 256), 256, [&] {
         k_born<false>(pn_b, pnl_b, pc, pcl, pp, ppl, dc, dcl, dp, dpl, dn_b, dnl_b, sp2, vm2, nullptr,
@@ -802,16 +806,16 @@ bool check_fwd3d_x2(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
         constexpr int CXr = 2, CYr = 2;
         const int nxs = (int)cdiv(nx, (int64_t)W * CXr), nys = (int)cdiv(ny, CYr);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d<LB, CXr, CYr, true>(A, B, C1, Al, Bl, C1l, sp2, src, sa, rec_head, rec_next,
                                         row_rec, ra1, ral1, da1, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
-                                        ng);
+                                        ng, g_blk);
         });
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d<LB, CXr, CYr, true>(C1, A, D1, C1l, Al, D1l, sp2, src, sb, rec_head, rec_next,
                                         row_rec, rb1, rbl1, db1, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
-                                        ng);
+                                        ng, g_blk);
         });
     }
     {
@@ -926,26 +930,26 @@ bool check_fwd3d_x2p(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
         constexpr int CXr = 2, CYr = 2;
         const int nxs = (int)cdiv(nx, (int64_t)W * CXr), nys = (int)cdiv(ny, CYr);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d<LB, CXr, CYr, true>(A, B, C1, Al, Bl, C1l, sp2, src, sa, rec_head, rec_next,
                                         row_rec, ra1, ral1, da1, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
-                                        ng);
+                                        ng, g_blk);
         });
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d<LB, CXr, CYr, true>(C1, A, D1, C1l, Al, D1l, sp2, src, sb, rec_head, rec_next,
                                         row_rec, rb1, rbl1, db1, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
-                                        ng);
+                                        ng, g_blk);
         });
     }
     {
         const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
         const int nzc = (int)cdiv(nz, RZ);
-        const unsigned blocks = (unsigned)cdiv((int64_t)ng * nxs * round_ys(nys) * nzc, 4);
+        const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
         emu::launch(blocks, kBlock, [&] {
             k_fwd3d_x2<LB, CX, CY, true>(A, B, C2, D2, Al, Bl, C2l, D2l, sp2, src, sa, sb, rec_head,
                                          rec_next, row_rec, ra2, ral2, rb2, rbl2, da2, db2, nz, ny,
-                                         nx, Bp, RZ, nxs, nys, nzc, ng);
+                                         nx, Bp, RZ, nxs, nys, nzc, ng, g_blk);
         });
     }
     bool ok = std::memcmp(C1, C2, sizeof(float) * FE) == 0 &&
@@ -987,7 +991,7 @@ bool lo32_cases(uint32_t seed) {
     bool ok = true;
     ok &= check_fwd_direct<64, 4, false, float>(29, 37, 2, 8, seed++);
     ok &= check_fwd_direct<16, 8, true, float>(9, 13, 1, 4, seed++);
-    for (int sched = 0; sched < 4; ++sched) {
+    for (int sched = 0; sched < 8; ++sched) {
         ok &= check_fwd<64, 4, true, float>(29, 37, 2, 8, seed++, sched);
         ok &= check_fwd<16, 8, true, float>(40, 33, 2, 4, seed++, sched);
     }
@@ -1358,6 +1362,20 @@ int main(int argc, char **argv) {
     }
     if (only.empty() || only == "lds") {
         ok &= lds_cases(seed + 2000);
+    }
+    if (only.empty() || only == "blk3d") {
+        // the 3D patch kernels with strip blocks other than the default pairs
+        for (int blk : {(2 << 8) | 2, (4 << 8) | 4, (8 << 8) | 8, (16 << 8) | 8, (4 << 8) | 16}) {
+            g_blk = blk;
+            ok &= fwd3d_cases<64, 2, 2>(seed + 9700 + blk);
+            ok &= fwd3d_cases<16, 4, 2>(seed + 9800 + blk);
+            ok &= radj3d_cases<64, 2, 2>(seed + 9900 + blk);
+            ok &= radj3d_cases<32, 1, 1>(seed + 10000 + blk);
+            ok &= born3d_cases<64, 2, 1>(seed + 10100 + blk);
+            ok &= check_fwd3d_x2p<64, 1, 1>(9, 7, 11, 2, 4, seed + 10200 + blk);
+            ok &= check_fwd3d_x2p<16, 2, 1>(11, 4, 9, 2, 16, seed + 10300 + blk);
+        }
+        g_blk = kBlk3dDefault;
     }
     if (only.empty() || only == "lo32") {
         ok &= lo32_cases(seed + 9500);

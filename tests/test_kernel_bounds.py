@@ -28,14 +28,18 @@ def _cdiv(a, b):
     return -(-a // b)
 
 
-def _chunk2d(t, nxs, nyc, sched):
-    """(xs, yc, dir) of a 2D wavefront task, as chunk2d in usct_kernels.hpp."""
+def _chunk2d(wv, ng, nxs, nyc, sched):
+    """(sg, xs, yc, dir) of 2D wavefront wv, as chunk2d in usct_kernels.hpp."""
+    if sched & 4:
+        sg, t = wv // (nxs * nyc), wv % (nxs * nyc)
+    else:
+        sg, t = wv % ng, wv // ng
     if sched & 2:
         yc, xs = t % nyc, t // nyc
     else:
         xs, yc = t % nxs, t // nxs
     d = -1 if (sched & 1) and (yc & 1) else 1
-    return xs, yc, d
+    return sg, xs, yc, d
 
 
 def _extent_2d(nx, ny, Bp, LB, CX, RY, adjoint, sched=0):
@@ -44,9 +48,8 @@ def _extent_2d(nx, ny, Bp, LB, CX, RY, adjoint, sched=0):
     lo = hi = 0
     n_waves = _cdiv(ng * nxs * nyc, 4) * 4  # whole blocks of 4 wavefronts
     for wv in range(n_waves):
-        sg, t = wv % ng, wv // ng
-        xs, yc, d = _chunk2d(t, nxs, nyc, sched)
-        if xs >= nxs or yc >= nyc:
+        sg, xs, yc, d = _chunk2d(wv, ng, nxs, nyc, sched)
+        if sg >= ng or xs >= nxs or yc >= nyc:
             continue  # the kernels return for these wavefronts
         y0 = yc * RY
         y1 = min(y0 + RY, ny)
@@ -66,30 +69,42 @@ def _extent_2d(nx, ny, Bp, LB, CX, RY, adjoint, sched=0):
     return lo, hi - nx * ny * Bp
 
 
-GY3D = 2  # USCT_GY3D: y strips per x strip taken by consecutive wavefronts
+BLOCKS3D = ((1, 2), (2, 2), (4, 4), (8, 4), (8, 8), (16, 8), (4, 16))  # strip blocks (bx, by)
 
 
-def _strip3d(t, nxs, nys, nzc):
+def _tasks3d(nxs, nys, nzc, bx, by):
+    return _cdiv(nxs, bx) * bx * _cdiv(nys, by) * by * nzc
+
+
+def _strip3d(t, nxs, nys, nzc, bx=1, by=2):
     """(xs, ys, zc, valid) of a 3D wavefront task, as strip3d in usct_kernels.hpp."""
-    q = t % GY3D
-    t //= GY3D
-    xs = t % nxs
-    t //= nxs
-    nyg = _cdiv(nys, GY3D)
-    ys = (t % nyg) * GY3D + q
-    zc = t // nyg
-    return xs, ys, zc, ys < nys and zc < nzc
+    q = t % (bx * by)
+    t //= bx * by
+    nbx, nby = _cdiv(nxs, bx), _cdiv(nys, by)
+    xs = (t % nbx) * bx + q // by
+    t //= nbx
+    ys = (t % nby) * by + q % by
+    zc = t // nby
+    return xs, ys, zc, xs < nxs and ys < nys and zc < nzc
 
 
-def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint, RZ=4, halo=1):
+def test_strip_blocks_visit_every_strip_once():
+    for nxs, nys, nzc in ((7, 5, 3), (16, 16, 2), (1, 9, 4), (33, 2, 1)):
+        for bx, by in BLOCKS3D:
+            seen = [_strip3d(t, nxs, nys, nzc, bx, by) for t in range(_tasks3d(nxs, nys, nzc, bx, by))]
+            valid = [(x, y, z) for x, y, z, ok in seen if ok]
+            assert sorted(valid) == [(x, y, z) for x in range(nxs) for y in range(nys)
+                                     for z in range(nzc)], (nxs, nys, nzc, bx, by)
+
+
+def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint, RZ=4, halo=1, blk=(1, 2)):
     W = 64 // LB
     ng, nxs, nys, nzc = Bp // LB, _cdiv(nx, W * CX), _cdiv(ny, CY), _cdiv(nz, RZ)
     lo = hi = 0
-    n_tasks = nxs * _cdiv(nys, GY3D) * GY3D * nzc
-    n_waves = _cdiv(ng * n_tasks, 4) * 4
+    n_waves = _cdiv(ng * _tasks3d(nxs, nys, nzc, *blk), 4) * 4
     for wv in range(n_waves):
         sg, t = wv % ng, wv // ng
-        xs, ys, zc, ok = _strip3d(t, nxs, nys, nzc)
+        xs, ys, zc, ok = _strip3d(t, nxs, nys, nzc, *blk)
         if not ok:
             continue  # the kernels return for these wavefronts
         z0 = zc * RZ
@@ -118,7 +133,7 @@ def test_2d_halo_reads_stay_in_guard(nx, ny):
             for CX in (4, 8):
                 for RY in (4, 43, 64):
                     for adj in (False, True):
-                        for sched in range(4):
+                        for sched in range(8):
                             lo, over = _extent_2d(nx, ny, Bp, LB, CX, RY, adj, sched)
                             assert lo >= -g * Bp and over < g * Bp, \
                                 (nx, ny, Bp, LB, CX, sched, lo, over)
@@ -140,3 +155,7 @@ def test_3d_halo_reads_stay_in_guard(nx, ny, nz):
             for CX, CY in ((1, 1), (2, 1), (1, 2)):
                 lo, over = _extent_3d(nx, ny, nz, Bp, LB, CX, CY, False, halo=2)
                 assert lo >= -g * Bp and over < g * Bp, (nx, ny, nz, Bp, LB, CX, CY, "x2", lo, over)
+        # strip blocks change only the order of the strips, not their set
+        for blk in BLOCKS3D[1:]:
+            lo, over = _extent_3d(nx, ny, nz, 2 * LB, LB, 2, 2, True, blk=blk)
+            assert lo >= -g * 2 * LB and over < g * 2 * LB, (nx, ny, nz, LB, blk, lo, over)
