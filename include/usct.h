@@ -67,11 +67,17 @@ enum {
                                    checkpoints) instead of running the forward backwards */
     USCT_FLAG_REVERSE = 64,     /* batched 3D gradients: run the forward backwards alongside the
                                    adjoint even where the whole history would fit */
-    USCT_FLAG_FP32_LO = 128     /* carry the forward state as FP32 hi + FP32 lo (8 bytes per
+    USCT_FLAG_FP32_LO = 128,    /* carry the forward state as FP32 hi + FP32 lo (8 bytes per
                                    shot-cell instead of 6; about 48 significant bits instead of 32):
                                    for time axes that end soon after the scattered arrivals, where
                                    the traces' rounding is a visible share of a small residual;
                                    point-element plans; Gauss-Newton products return USCT_ENOTSUP */
+    USCT_FLAG_LEVELS_2 = 256,   /* batched 2D without sponge: fused launches advance two time
+                                   levels (the default) */
+    USCT_FLAG_LEVELS_3 = 512    /* batched 2D without sponge: fused launches advance three time
+                                   levels (fewer bytes per level, more registers and arithmetic:
+                                   slower than two on MI355X at 512^2 x 128 shots); one more
+                                   rotation buffer; USCT_ENOTSUP where launches cannot be fused */
 };
 
 /* Misfit codes (software/python/uap.py:41). */
@@ -90,7 +96,8 @@ typedef struct {
     int shots_per_pass;    /* batched: padded shots per pass */
     int ckpt_every;        /* checkpoint interval K (>= nt-1: whole history stored) */
     int n_windows;         /* recompute windows per pass */
-    int64_t workspace_bytes;  /* device memory held by the plan */
+    int64_t workspace_bytes;  /* device memory held by the plan (with usct_device_cache: the
+                                 blocks it holds, which may exceed what it asked for) */
     int fwd_lanes_per_group;  /* the same four for the forward kernel */
     int fwd_tile_x, fwd_tile_y;
     int fwd_rows_per_wave;
@@ -109,6 +116,8 @@ typedef struct {
     int strip_block;          /* 3D patch adjoint kernel: wavefront order over blocks of
                                  (strip_block >> 8) x (strip_block & 255) x-by-y strips */
     int fwd_strip_block;      /* the same for the forward kernel */
+    int levels_per_launch;    /* time levels one adjoint launch advances (1, 2 or 3) */
+    int fwd_levels_per_launch;  /* the same for the forward */
 } usct_plan_info;
 
 /*
@@ -140,6 +149,20 @@ int usct_plan_create_groups(usct_plan **plan, int ndim, const int *dims, double 
                             const int64_t *rec_cells, const double *rec_w, int method,
                             int ckpt_every, int64_t mem_budget, int flags);
 int usct_plan_destroy(usct_plan *plan);
+
+/*
+ * Device memory kept from destroyed plans for later plans, per process: at most
+ * max_bytes (0, the default: nothing is kept, and what is kept is freed).  While a
+ * cap is set, a plan's buffers are 2 MiB multiples taken from a kept block of its
+ * device within 1.25x of their size where one exists (hipMalloc otherwise), and
+ * usct_plan_destroy keeps them instead of calling hipFree, which waits for the
+ * whole device.  A hipMalloc that fails frees the device's kept blocks and is tried
+ * once more; kept memory counts as free for a plan's default budget.  *held (may
+ * be NULL): bytes kept after the call.  No counterpart in the reference, whose
+ * calls hold no device memory between them; for loops that create a plan per
+ * request (openusct_amd.serve).
+ */
+int usct_device_cache(int64_t max_bytes, int64_t *held);
 int usct_plan_get_info(const usct_plan *plan, usct_plan_info *info);
 
 /*

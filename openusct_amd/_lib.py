@@ -17,11 +17,13 @@ USCT_OK, USCT_EINVAL, USCT_ENOMEM, USCT_EHIP, USCT_ENOTSUP = range(5)
 METHOD_AUTO, METHOD_PLAIN, METHOD_BATCHED = 0, 1, 2
 FLAG_SPONGE, FLAG_AUTOTUNE, FLAG_NO_GRAPH, FLAG_FORWARD_ONLY, FLAG_NO_FUSE = 1, 2, 4, 8, 16
 FLAG_NO_REVERSE, FLAG_REVERSE, FLAG_FP32_LO = 32, 64, 128
+FLAG_LEVELS_2, FLAG_LEVELS_3 = 256, 512
 MISFIT_CODES = {"l2": 0, "gcn": 1}
 
 # Every symbol declared in include/usct.h.
 EXPORTS = (
-    "usct_plan_create", "usct_plan_create_groups", "usct_plan_destroy", "usct_plan_get_info", "usct_forward_fmc",
+    "usct_plan_create", "usct_plan_create_groups", "usct_plan_destroy", "usct_device_cache",
+    "usct_plan_get_info", "usct_forward_fmc",
     "usct_misfit", "usct_misfit_and_gradient", "usct_misfit_tw", "usct_misfit_and_gradient_tw",
     "usct_gn_hessian_vector", "usct_gradient_from_source", "usct_gsot_workspace_bytes", "usct_gsot",
     "usct_profile_gradient",
@@ -55,6 +57,8 @@ class PlanInfo(ctypes.Structure):
         ("fwd_sched", ctypes.c_int),
         ("strip_block", ctypes.c_int),
         ("fwd_strip_block", ctypes.c_int),
+        ("levels_per_launch", ctypes.c_int),
+        ("fwd_levels_per_launch", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -91,6 +95,7 @@ def load(path=None):
                                             ip, i64p, i64p, vp, ip, i64p, i64p, vp,
                                             ip, ip, ctypes.c_int64, ip]
     lib.usct_plan_destroy.argtypes = [vp]
+    lib.usct_device_cache.argtypes = [ctypes.c_int64, i64p]
     lib.usct_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
     lib.usct_forward_fmc.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.usct_misfit.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]

@@ -75,6 +75,11 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 // res per pass in place of the residual).
 enum Mode { MODE_FWD = 0, MODE_MISFIT = 1, MODE_GRAD = 2, MODE_GNHV = 3, MODE_ADJSRC = 4 };
 
+// Time levels per launch of the fused 2D kernels: two, or three for plans created
+// with USCT_FLAG_LEVELS_3 (which need kLvMax + 2 rotation buffers).
+constexpr int kLvMax = 3;
+constexpr int kMaxRot = kLvMax + 2;
+
 struct Tiling {
     int LB, CX, CY, RY;  // lanes per shot group, cells per lane (x, y), rows per wavefront
     int lds = 0;         // 3D: 1 = LDS-staged work-group tiles of CX x CY cells (patch per
@@ -85,6 +90,8 @@ struct Tiling {
                          // bit 1 = row chunk fastest in the wavefront order, bit 2 = shot
                          // group slowest
     int blk = kBlk3dDefault;  // 3D patch kernels: strip blocks of (blk >> 8) x (blk & 255)
+    int lv = 2;               // 2D fused kernels: time levels per launch (2: k_fwd2d_x2 /
+                              // k_adj2d_x2, 3: k_fwd2d_xn / k_adj2d_xn)
 };
 
 // Lane patch of an LDS tile: 256 lanes = LB shots x (TX/PX) x (TY/PY) patches.
@@ -125,9 +132,10 @@ struct usct_plan {
     // backwards alongside it (k_radj3d); Wslot then holds three adjoint levels
     bool rev = false;
     int64_t budget = 0;
-    // device memory
-    std::vector<void *> allocs;
-    int64_t bytes = 0;
+    // device memory: blocks (pointer, bytes held) and the bytes the plan asked for
+    std::vector<std::pair<void *, int64_t>> allocs;
+    int64_t bytes = 0, held = 0;
+    int dev = 0;  // the device current at creation: the plan's memory and stream live there
     float2 *sp2 = nullptr;
     float *spg = nullptr;  // sponge (FP32)
     int *rec_head = nullptr, *rec_next = nullptr;
@@ -138,10 +146,13 @@ struct usct_plan {
     double *sincd = nullptr; // [nt][Bp] the same in FP64 (3D kernels)
     float *srcd2 = nullptr;  // [nt][Bp] recorded increment at each shot's source cell
     float *Z = nullptr;      // zero field
-    float *F[4] = {nullptr, nullptr, nullptr, nullptr};  // rotation (4: fused two-level kernels)
+    // rotation of levels that are not kept: a launch reads levels n-1, n-2 and writes
+    // up to n+LV-1, so nrot >= LV + 2 buffers (4, or 5 where three-level launches run)
+    int nrot = 4;
+    float *F[kMaxRot] = {};
     std::vector<float *> Wslot, CK;
     // bfloat16 low parts of the forward state: rotation, zero level, checkpoints
-    uint16_t *Lo[4] = {nullptr, nullptr, nullptr, nullptr};
+    uint16_t *Lo[kMaxRot] = {};
     uint16_t *Zlo = nullptr;
     std::vector<uint16_t *> CKlo;
     // USCT_FLAG_FP32_LO: the low parts are FP32 (two uint16_t slots per element
@@ -193,16 +204,104 @@ struct usct_plan {
 
 namespace {
 
+// Device memory of destroyed plans kept for later plans (usct_device_cache): a
+// serving loop creates a plan per request, and hipFree waits for the whole
+// device while a first hipMalloc of many GB maps its pages.  Kept blocks are
+// 2 MiB multiples; a request takes the smallest kept block of its device that
+// is at least its size and at most 1.25 times it.
+constexpr int64_t kCacheGrain = (int64_t)2 << 20;
+struct KeptBlock {
+    void *p;
+    int64_t bytes;
+    int dev;
+};
+std::mutex g_cache_mu;
+std::vector<KeptBlock> g_kept;
+int64_t g_cache_cap = 0, g_kept_bytes = 0;
+
+void free_block(const KeptBlock &b) {
+    int cur = 0;
+    const bool other = hipGetDevice(&cur) == hipSuccess && cur != b.dev;
+    if (other) (void)hipSetDevice(b.dev);
+    (void)hipFree(b.p);
+    if (other) (void)hipSetDevice(cur);
+}
+
+// Free kept blocks (of device dev, or of every device when dev < 0) until at most
+// `keep` bytes are kept; returns the bytes freed.  Caller holds g_cache_mu.
+int64_t release_kept(int dev, int64_t keep) {
+    int64_t freed = 0;
+    for (size_t i = g_kept.size(); i-- > 0 && g_kept_bytes > keep;) {
+        if (dev >= 0 && g_kept[i].dev != dev) continue;
+        free_block(g_kept[i]);
+        g_kept_bytes -= g_kept[i].bytes;
+        freed += g_kept[i].bytes;
+        g_kept.erase(g_kept.begin() + (ptrdiff_t)i);
+    }
+    return freed;
+}
+
+int64_t kept_on(int dev) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    int64_t b = 0;
+    for (const KeptBlock &k : g_kept) b += k.dev == dev ? k.bytes : 0;
+    return b;
+}
+
 int dalloc(usct_plan *P, void **p, int64_t bytes) {
     if (bytes <= 0) bytes = 16;
-    hipError_t e = hipMalloc(p, (size_t)bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(USCT_ENOMEM, "hipMalloc(%lld bytes) failed: %s", (long long)bytes,
-                    hipGetErrorString(e));
+    const int dev = P->dev;
+    int cur = dev;
+    (void)hipGetDevice(&cur);
+    if (cur != dev) (void)hipSetDevice(dev);
+    struct Restore {
+        int cur, dev;
+        ~Restore() {
+            if (cur != dev) (void)hipSetDevice(cur);
+        }
+    } restore{cur, dev};
+    int64_t size = bytes;
+    *p = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        if (g_cache_cap > 0) {
+            size = cdiv(bytes, kCacheGrain) * kCacheGrain;
+            int best = -1;
+            for (size_t i = 0; i < g_kept.size(); ++i) {
+                const KeptBlock &k = g_kept[i];
+                if (k.dev == dev && k.bytes >= size && k.bytes <= size + size / 4 &&
+                    (best < 0 || k.bytes < g_kept[best].bytes))
+                    best = (int)i;
+            }
+            if (best >= 0) {
+                *p = g_kept[best].p;
+                size = g_kept[best].bytes;
+                g_kept_bytes -= size;
+                g_kept.erase(g_kept.begin() + best);
+            }
+        }
     }
-    P->allocs.push_back(*p);
+    if (!*p) {
+        hipError_t e = hipMalloc(p, (size_t)size);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            int64_t freed = 0;
+            {
+                std::lock_guard<std::mutex> lk(g_cache_mu);
+                freed = release_kept(dev, 0);
+            }
+            if (freed > 0) e = hipMalloc(p, (size_t)size);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *p = nullptr;
+            return fail(USCT_ENOMEM, "hipMalloc(%lld bytes) failed: %s", (long long)size,
+                        hipGetErrorString(e));
+        }
+    }
+    P->allocs.emplace_back(*p, size);
     P->bytes += bytes;
+    P->held += size;
     return 0;
 }
 
@@ -231,8 +330,21 @@ int dalloc_guarded(usct_plan *P, T **p, int64_t count, int64_t stride) {
 void free_plan_memory(usct_plan *P) {
     for (auto &kv : P->graphs) (void)hipGraphExecDestroy(kv.second);
     P->graphs.clear();
-    for (void *p : P->allocs) (void)hipFree(p);
+    // blocks go back to the cache only once no work of this plan can touch them
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (auto &a : P->allocs) {
+            if (g_cache_cap > 0 && g_kept_bytes + a.second <= g_cache_cap) {
+                g_kept.push_back({a.first, a.second, P->dev});
+                g_kept_bytes += a.second;
+            } else {
+                free_block({a.first, a.second, P->dev});
+            }
+        }
+    }
     P->allocs.clear();
+    P->held = 0;
     P->Wslot.clear();
     P->CK.clear();
     P->CKlo.clear();
@@ -507,6 +619,48 @@ void launch_fwd2_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, Lvl 
 #undef USCT_X2
 }
 
+// LV forward levels per launch (2D, no sponge): levels n .. n+LV-1 go to
+// out[0 .. LV-1]; a null hi or lo is not stored.
+template <int LB, int CX, int LV, bool REC>
+void fwd2d_xn_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, const Lvl *out, const usct_plan *P, int n,
+                const Tile &t) {
+    const int64_t tr = (int64_t)P->n_rec * P->Bp;
+    auto go = [&](auto lo_type) {
+        using L = decltype(lo_type);
+        FwdLevels<LV, L> o{};
+        for (int i = 0; i < LV; ++i) {
+            o.hi[i] = out[i].hi;
+            o.lo[i] = (L *)out[i].lo;
+            o.sincd[i] = P->sincd + (int64_t)(n + i) * P->Bp;
+            o.rec[i] = P->recb + (int64_t)(n + i) * tr;
+            o.rec_lo[i] = P->recb_lo + (int64_t)(n + i) * tr;
+        }
+        hipLaunchKernelGGL((k_fwd2d_xn<LB, CX, LV, REC, L>), g, dim3(kBlock), 0, s, pc.hi, pp.hi,
+                           (const L *)pc.lo, (const L *)pp.lo, o, P->sp2, P->src_cell,
+                           P->rec_head, P->rec_next, P->row_rec, P->ny, P->nx, P->Bp, P->tf.RY,
+                           t.nxs, t.nyc, P->Bp / P->tf.LB, P->tf.sched);
+    };
+    if (P->lo32) go(float{});
+    else go(uint16_t{});
+}
+
+// Three forward levels per launch: k_fwd2d_xn with four cells per lane (eight
+// spill at two wavefronts per SIMD).
+void launch_fwd3_batched(const usct_plan *P, hipStream_t s, Lvl pc, Lvl pp, const Lvl *out, int n,
+                         bool record) {
+    const Tile t = tile_of(P, P->tf);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+#define USCT_X3(LB_)                                                           \
+    if (record) fwd2d_xn_t<LB_, 4, 3, true>(g, s, pc, pp, out, P, n, t);       \
+    else fwd2d_xn_t<LB_, 4, 3, false>(g, s, pc, pp, out, P, n, t);
+    switch (P->tf.LB) {
+        case 16: USCT_X3(16) break;
+        case 32: USCT_X3(32) break;
+        default: USCT_X3(64) break;
+    }
+#undef USCT_X3
+}
+
 // Whether the plan's forward / adjoint launches advance two time levels: 2D both
 // (no sponge), 3D the forward when its tile is the two-level LDS kind.  Footprint
 // plans correct every level after its launch, so one level per launch.
@@ -575,6 +729,36 @@ void launch_adj2_batched(const usct_plan *P, hipStream_t s, const float *l1, con
         default: USCT_A2(64) break;
     }
 #undef USCT_A2
+}
+
+// Three adjoint levels per launch (2D, no sponge): lam^k, lam^{k-1}, lam^{k-2}
+// to out[0 .. 2] (null: not stored), imaged against p^{k-1}, p^{k-2}, p^{k-3} (pf).
+template <int LB>
+void adj2d_x3_t(dim3 g, hipStream_t s, const float *l1, const float *l2, float *const *out,
+                const float *const *pf, const usct_plan *P, int k, const Tile &t) {
+    constexpr int LV = 3;
+    const int64_t tr = (int64_t)P->n_rec * P->Bp;
+    AdjLevels<LV> o{};
+    for (int i = 0; i < LV; ++i) {
+        o.lam[i] = out[i];
+        o.pf[i] = pf[i];
+        o.sinc[i] = P->sinc + (int64_t)(k - i) * P->Bp;
+        o.res[i] = P->res + (int64_t)(k - i) * tr;
+    }
+    hipLaunchKernelGGL((k_adj2d_xn<LB, 4, LV>), g, dim3(kBlock), 0, s, l1, l2, o, P->sp2,
+                       P->src_cell, P->rec_head, P->rec_next, P->row_rec, P->g_hi, P->g_lo,
+                       P->ny, P->nx, P->Bp, P->ta.RY, t.nxs, t.nyc, P->ng_adj(), P->ta.sched);
+}
+
+void launch_adj3_batched(const usct_plan *P, hipStream_t s, const float *l1, const float *l2,
+                         float *const *out, const float *const *pf, int k) {
+    const Tile t = tile_of(P, P->ta);
+    const dim3 g((unsigned)cdiv(t.waves, kBlock / kWave));
+    switch (P->ta.LB) {
+        case 16: adj2d_x3_t<16>(g, s, l1, l2, out, pf, P, k, t); break;
+        case 32: adj2d_x3_t<32>(g, s, l1, l2, out, pf, P, k, t); break;
+        default: adj2d_x3_t<64>(g, s, l1, l2, out, pf, P, k, t); break;
+    }
 }
 
 template <int LB, int CX, int CY>
@@ -708,9 +892,9 @@ Lvl fwd_level(const usct_plan *P, int n, bool store) {
         const int c = ck_slot(P, n);
         if (c >= 0) return {P->CK[c], P->CKlo[c]};
         const int base_last = (P->NS - 1) * P->K;
-        if (n >= base_last - 1) return {P->Wslot[n - base_last + 1], P->Lo[n % 4]};
+        if (n >= base_last - 1) return {P->Wslot[n - base_last + 1], P->Lo[n % P->nrot]};
     }
-    return {P->F[n % 4], P->Lo[n % 4]};
+    return {P->F[n % P->nrot], P->Lo[n % P->nrot]};
 }
 
 // Where level `level` lives while window s is re-computed / consumed.
@@ -718,15 +902,30 @@ Lvl win_level(const usct_plan *P, int s, int level) {
     if (level <= 0) return {P->Z, P->Zlo};
     const int base = s * P->K;
     const int i = level - base + 1;
-    if (s == P->NS - 1 || i >= 2) return {P->Wslot[i], P->Lo[level % 4]};
+    if (s == P->NS - 1 || i >= 2) return {P->Wslot[i], P->Lo[level % P->nrot]};
     const int c = 2 * (s - 1) + i;
     return {P->CK[c], P->CKlo[c]};
 }
 
-float *lam_level(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->F[k % 4]; }
+float *lam_level(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->F[k % P->nrot]; }
 // Adjoint levels of a reversing plan: three slots of their own (the forward
 // rotation F holds the levels being run backwards).
 float *lam_rev(const usct_plan *P, int k) { return k >= P->nt ? P->Z : P->Wslot[k % 3]; }
+
+// Forward levels a gradient keeps beyond the launch after next: checkpoint
+// pairs (hi and lo) and the last window's levels (hi).  The others live in the
+// rotation, where only the last two levels of a launch are read again.
+bool level_kept(const usct_plan *P, int n, bool store) {
+    if (!store || P->rev || n <= 0) return false;
+    return ck_slot(P, n) >= 0 || n >= (P->NS - 1) * P->K - 1;
+}
+
+// Time levels per launch of a plan's fused forward / adjoint kernels (1: none).
+int fwd_levels(const usct_plan *P) {
+    if (!fused_fwd(P)) return 1;
+    return P->ndim == 2 ? P->tf.lv : 2;
+}
+int adj_levels(const usct_plan *P) { return fused_adj(P) ? P->ta.lv : 1; }
 
 void ev_start(Profile *pr, std::vector<hipEvent_t> &v, hipStream_t s) {
     if (!pr) return;
@@ -941,9 +1140,26 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
     hipLaunchKernelGGL(k_fill, dim3(grid_for((int64_t)P->n_rec * P->Bp)), dim3(256), 0, s,
                        P->recb_lo, (int64_t)P->n_rec * P->Bp, 0.0f);
     const bool x2 = fused_fwd(P), x2a = fused_adj(P);
+    const int lvf = fwd_levels(P), lva = adj_levels(P);
     if (born) born_clear(P, s);
     for (int n = 1; n < nt;) {
         const bool two = x2 && n + 1 < nt;
+        if (!born && lvf >= 3 && n + 2 < nt) {
+            // three levels: the first is stored only where the gradient keeps it
+            Lvl out[3];
+            for (int i = 0; i < 3; ++i) out[i] = fwd_level(P, n + i, store);
+            if (!level_kept(P, n, store)) out[0].hi = nullptr;
+            if (ck_slot(P, n) < 0 || !level_kept(P, n, store)) out[0].lo = nullptr;
+            if (pr) ev_start(pr, pr->fwd_ev, s);
+            launch_fwd3_batched(P, s, fwd_level(P, n - 1, store), fwd_level(P, n - 2, store), out,
+                                n, true);
+            if (pr) {
+                ev_stop(pr, pr->fwd_ev, s);
+                pr->fwd_lv.push_back(3);
+            }
+            n += 3;
+            continue;
+        }
         if (born && two && P->ndim == 2) {
             // background and Born levels n, n+1 together (fusable: no sponge, no footprints)
             if (pr) ev_start(pr, pr->fwd_ev, s);
@@ -1004,6 +1220,20 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
         const int khi = std::min(base + P->K, nt - 1);
         if (w < P->NS - 1) {
             for (int n = base + 1; n <= khi;) {
+                if (lvf >= 3 && n + 2 <= khi) {
+                    Lvl out[3];
+                    for (int i = 0; i < 3; ++i) out[i] = win_level(P, w, n + i);
+                    out[0].lo = nullptr;  // levels of a window keep their hi only
+                    if (pr) ev_start(pr, pr->fwd_ev, s);
+                    launch_fwd3_batched(P, s, win_level(P, w, n - 1), win_level(P, w, n - 2), out,
+                                        n, false);
+                    if (pr) {
+                        ev_stop(pr, pr->fwd_ev, s);
+                        pr->fwd_lv.push_back(3);
+                    }
+                    n += 3;
+                    continue;
+                }
                 const bool two = x2 && n + 1 <= khi;
                 const int adv = fwd_step(P, s, win_level(P, w, n - 2), win_level(P, w, n - 1),
                                          win_level(P, w, n), two ? win_level(P, w, n + 1) : Lvl{},
@@ -1014,6 +1244,20 @@ void batched_sequence(usct_plan *P, hipStream_t s, int mode, int misfit, bool wi
             }
         }
         for (int k = khi; k >= base + 1;) {
+            if (lva >= 3 && k - 2 >= base + 1) {
+                // lam^k is read by no later launch: not stored
+                float *out[3] = {nullptr, lam_level(P, k - 1), lam_level(P, k - 2)};
+                const float *pf[3] = {win_level(P, w, k - 1).hi, win_level(P, w, k - 2).hi,
+                                      win_level(P, w, k - 3).hi};
+                if (pr) ev_start(pr, pr->adj_ev, s);
+                launch_adj3_batched(P, s, lam_level(P, k + 1), lam_level(P, k + 2), out, pf, k);
+                if (pr) {
+                    ev_stop(pr, pr->adj_ev, s);
+                    pr->adj_lv.push_back(3);
+                }
+                k -= 3;
+                continue;
+            }
             const bool two = x2a && k - 1 >= base + 1;
             if (pr) ev_start(pr, pr->adj_ev, s);
             if (two) {
@@ -1263,8 +1507,8 @@ struct MemPlan {
 int64_t batched_bytes(const usct_plan *P, int Bp, int K, int NS, bool grad) {
     const int64_t field = P->N * (int64_t)Bp * 4;
     const int64_t traces = (int64_t)P->nt * P->n_rec * Bp * 4;
-    int64_t fields = 5;  // Z + F[4]
-    int64_t halves = 5;  // Zlo + Lo[4]
+    int64_t fields = 1 + P->nrot;  // Z + F
+    int64_t halves = 1 + P->nrot;  // Zlo + Lo
     if (grad) {
         fields += (K + 2) + 2 * std::max(NS - 2, 0);
         halves += 2 * std::max(NS - 2, 0);
@@ -1293,6 +1537,8 @@ int choose_batched(usct_plan *P, int ckpt_every) {
     // forward's time (7.4 against 4.3 ms per level on 256^3 with 64 shots).
     const bool x2 = P->ndim == 2 && !P->sponge && !(P->flags & USCT_FLAG_NO_FUSE);
     const double b_fwd = x2 ? 13.0 : 19.0, b_adj = x2 ? 14.0 : 17.0, b_radj = 33.0;
+    // plans that launch three levels at a time rotate over five buffers
+    P->nrot = x2 && (P->flags & USCT_FLAG_LEVELS_3) ? kMaxRot : 4;
     const bool force_rev = (P->flags & USCT_FLAG_REVERSE) != 0;
     if (force_rev && !reversible(P))
         return fail(USCT_ENOTSUP, "USCT_FLAG_REVERSE needs a batched 3D gradient plan without "
@@ -1384,9 +1630,9 @@ int allocate_batched(usct_plan *P) {
     const int64_t FE = P->field_elems();
     TRY(dalloc_guarded(P, &P->Z, FE, P->Bp));
     HIPX(hipMemset(P->Z, 0, sizeof(float) * FE));
-    for (int i = 0; i < 4; ++i) TRY(dalloc_guarded(P, &P->F[i], FE, P->Bp));
+    for (int i = 0; i < P->nrot; ++i) TRY(dalloc_guarded(P, &P->F[i], FE, P->Bp));
     const int64_t u = P->lo_units();
-    for (int i = 0; i < 4; ++i) TRY(dalloc_guarded(P, &P->Lo[i], FE * u, P->Bp * u));
+    for (int i = 0; i < P->nrot; ++i) TRY(dalloc_guarded(P, &P->Lo[i], FE * u, P->Bp * u));
     TRY(dalloc_guarded(P, &P->Zlo, FE * u, P->Bp * u));
     HIPX(hipMemset(P->Zlo, 0, sizeof(uint16_t) * FE * u));
     const bool fwd_only = (P->flags & USCT_FLAG_FORWARD_ONLY) != 0;
@@ -1476,11 +1722,20 @@ double time_tile(usct_plan *P, bool adjoint) {
     (void)hipEventCreate(&a);
     (void)hipEventCreate(&b);
     const int reps = 4;
-    const bool x2 = (adjoint ? fused_adj(P) : fused_fwd(P)) && P->nt >= 3;  // levels 1, 2 exist
+    const int lv = adjoint ? adj_levels(P) : fwd_levels(P);
+    // the launches write time levels 1 .. lv (traces, source rows): they must exist
+    if (lv >= 3 && (P->nt < 4 || P->nrot < 5)) return 1e30;
+    const bool x2 = lv == 2 && P->nt >= 3;
     auto step = [&](int r) {
-        Lvl L[4];
-        for (int i = 0; i < 4; ++i) L[i] = {P->F[(r + i) % 4], P->Lo[(r + i) % 4]};
-        if (!adjoint) {
+        Lvl L[kMaxRot];
+        for (int i = 0; i < P->nrot; ++i) L[i] = {P->F[(r + i) % P->nrot], P->Lo[(r + i) % P->nrot]};
+        if (!adjoint && lv >= 3) {
+            launch_fwd3_batched(P, P->stream, L[1], L[0], &L[2], 1, false);
+        } else if (adjoint && lv >= 3) {
+            float *out[3] = {L[2].hi, L[3].hi, L[4].hi};
+            const float *pf[3] = {P->Wslot[0], P->Wslot[0], P->Wslot[0]};
+            launch_adj3_batched(P, P->stream, L[0].hi, L[1].hi, out, pf, 3);
+        } else if (!adjoint) {
             if (x2) launch_fwd2_batched(P, P->stream, L[1], L[0], L[2], L[3], 1, false);
             else launch_fwd_batched(P, P->stream, L[1], L[0], L[2], 1, false);
         } else if (P->rev) {
@@ -1504,7 +1759,8 @@ double time_tile(usct_plan *P, bool adjoint) {
     else (void)hipEventElapsedTime(&ms, a, b);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    return x2 ? ms / 2.0 : ms;  // per time level: one- and two-level candidates compare
+    // per time level: candidates advancing different numbers of levels compare
+    return lv >= 3 ? ms / lv : (x2 ? ms / 2.0 : ms);
 }
 
 // Autotuner: lane-group width (among those that divide the padded shot count),
@@ -1515,10 +1771,10 @@ double time_tile(usct_plan *P, bool adjoint) {
 // grid, shots per pass, sponge, reversal, fusion (whether two-level kernels are
 // candidates) and the low part's type (which kernel kinds can take it).
 struct TuneKey {
-    int ndim, nz, ny, nx, Bp, sponge, rev, fuse, lo32;
+    int ndim, nz, ny, nx, Bp, sponge, rev, fuse, lo32, levels;
     bool operator<(const TuneKey &o) const {
-        return std::tie(ndim, nz, ny, nx, Bp, sponge, rev, fuse, lo32) <
-               std::tie(o.ndim, o.nz, o.ny, o.nx, o.Bp, o.sponge, o.rev, o.fuse, o.lo32);
+        return std::tie(ndim, nz, ny, nx, Bp, sponge, rev, fuse, lo32, levels) <
+               std::tie(o.ndim, o.nz, o.ny, o.nx, o.Bp, o.sponge, o.rev, o.fuse, o.lo32, o.levels);
     }
 };
 struct TuneVal { Tiling tf, ta; };
@@ -1530,7 +1786,8 @@ int autotune(usct_plan *P) {
     // ever advanced, so there is nothing to time and row 1 does not exist.
     if (P->nt < 2) return 0;
     const TuneKey key{P->ndim, P->nz, P->ny, P->nx, P->Bp, P->sponge ? 1 : 0, P->rev ? 1 : 0,
-                      fusable(P) ? 1 : 0, P->lo32 ? 1 : 0};
+                      fusable(P) ? 1 : 0, P->lo32 ? 1 : 0,
+                      P->flags & (USCT_FLAG_LEVELS_2 | USCT_FLAG_LEVELS_3)};
     auto hit = g_tuned.find(key);
     if (hit != g_tuned.end()) {
         P->tf = hit->second.tf;
@@ -1538,7 +1795,7 @@ int autotune(usct_plan *P) {
         return 0;
     }
     // zero the fields used by the timing launches
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < P->nrot; ++i) {
         HIPX(hipMemset(P->F[i], 0, sizeof(float) * P->field_elems()));
         HIPX(hipMemset(P->Lo[i], 0, sizeof(uint16_t) * P->field_elems() * P->lo_units()));
     }
@@ -1565,8 +1822,10 @@ int autotune(usct_plan *P) {
         if (P->ndim == 2) {
             // the two-level kernels also try the chunk orders and march directions
             const bool x2 = adjoint ? fused_adj(P) : fused_fwd(P);
+            const bool only3 = x2 && P->nrot >= kMaxRot;
             for (int lb : lbs)
                 for (int cx : {4, 8}) {
+                    if (only3) break;
                     // rows per wavefront: powers of two, and the counts that make the
                     // grid one, two or three whole rounds of resident wavefronts
                     // (3 per SIMD), so that no round runs part-empty
@@ -1583,6 +1842,22 @@ int autotune(usct_plan *P) {
                     for (int ry : rys)
                         for (int sc = 0; sc < (x2 ? 8 : 1); ++sc)
                             cands.push_back({lb, cx, 1, ry, 0, sc});
+                }
+            // three levels per launch (four cells per lane, two wavefronts per SIMD)
+            if (only3 && P->nt >= 4)
+                for (int lb : lbs) {
+                    std::vector<int> rys{8, 16, 32, 64};
+                    const int64_t per_chunk = (P->Bp / lb) * cdiv(P->nx, (int64_t)(kWave / lb) * 4);
+                    for (int k = 1; k <= 3; ++k) {
+                        const int64_t nyc =
+                            std::max<int64_t>(1, cdiv((int64_t)k * cus * 4 * 2, per_chunk));
+                        const int ry = (int)cdiv(P->ny, nyc);
+                        if (ry >= 3 && std::find(rys.begin(), rys.end(), ry) == rys.end())
+                            rys.push_back(ry);
+                    }
+                    for (int ry : rys)
+                        for (int sc = 0; sc < 8; ++sc)
+                            cands.push_back({lb, 4, 1, ry, 0, sc, kBlk3dDefault, 3});
                 }
         } else if (adjoint && P->rev) {
             for (int lb : lbs)
@@ -1628,11 +1903,12 @@ int autotune(usct_plan *P) {
             T = c;
             const double t = time_tile(P, adjoint != 0);
             if (log && *log == '1')
-                std::fprintf(stderr, "[usct tune] %s lanes %d %s %dx%d rows %d sched %d: %.3f ms\n",
+                std::fprintf(stderr,
+                             "[usct tune] %s lanes %d %s %dx%d rows %d sched %d levels %d: %.3f ms\n",
                              adjoint ? "adjoint" : "forward", c.LB,
                              c.lds == 3 ? "patch-x2"
                                         : (c.lds == 2 ? "lds-tile-x2" : (c.lds ? "lds-tile" : "patch")),
-                             c.CX, c.CY, c.RY, c.sched, t);
+                             c.CX, c.CY, c.RY, c.sched, c.lv, t);
             if (t < best) { best = t; bt = c; }
         }
         T = bt;
@@ -2094,6 +2370,12 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
         return fail(USCT_ENOTSUP, "the sponge is defined for 2D grids only (solver.py:30)");
     if (method != USCT_METHOD_AUTO && method != USCT_METHOD_PLAIN && method != USCT_METHOD_BATCHED)
         return fail(USCT_EINVAL, "unknown method %d", method);
+    if ((flags & USCT_FLAG_LEVELS_2) && (flags & USCT_FLAG_LEVELS_3))
+        return fail(USCT_EINVAL, "USCT_FLAG_LEVELS_2 and USCT_FLAG_LEVELS_3 exclude each other");
+    if ((flags & USCT_FLAG_LEVELS_3) &&
+        (ndim != 2 || method == USCT_METHOD_PLAIN || (flags & (USCT_FLAG_SPONGE | USCT_FLAG_NO_FUSE))))
+        return fail(USCT_ENOTSUP, "three-level launches need a batched 2D plan without the sponge "
+                                  "or USCT_FLAG_NO_FUSE");
     usct_plan *P = new usct_plan();
     P->ndim = ndim;
     P->nz = ndim == 3 ? dims[0] : 1;
@@ -2121,6 +2403,7 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
         delete P;
         return code;
     };
+    if (hipGetDevice(&P->dev) != hipSuccess) return bail(fail(USCT_EHIP, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&P->ev_in, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&P->ev_out, hipEventDisableTiming) != hipSuccess)
@@ -2128,7 +2411,9 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess)
         return bail(fail(USCT_EHIP, "hipMemGetInfo failed"));
-    P->budget = mem_budget > 0 ? mem_budget : (int64_t)(0.85 * (double)free_b);
+    // memory kept from destroyed plans (usct_device_cache) is free for this one
+    P->budget = mem_budget > 0 ? mem_budget
+                               : (int64_t)(0.85 * (double)((int64_t)free_b + kept_on(P->dev)));
     if ((rc = build_receivers(P))) return bail(rc);
     if ((rc = dalloc_guarded(P, &P->sp2, N, 1))) return bail(rc);
     if (P->sponge && (rc = dalloc_t(P, &P->spg, N))) return bail(rc);
@@ -2149,6 +2434,10 @@ int usct_plan_create(usct_plan **out, int ndim, const int *dims, double h, doubl
             if ((rc = choose_batched(P, ckpt_every))) return bail(rc);
             P->budget += P->bytes;
             if ((rc = allocate_batched(P))) return bail(rc);
+            if (P->nrot >= kMaxRot) {  // three time levels per fused 2D launch
+                P->tf.lv = P->ta.lv = 3;
+                P->tf.CX = P->ta.CX = 4;
+            }
             default_rows(P, P->tf);
             default_rows(P, P->ta);
             if ((flags & USCT_FLAG_AUTOTUNE) && (rc = autotune(P))) return bail(rc);
@@ -2194,6 +2483,15 @@ int usct_plan_create_groups(usct_plan **out, int ndim, const int *dims, double h
     return 0;
 }
 
+int usct_device_cache(int64_t max_bytes, int64_t *held) {
+    if (max_bytes < 0) return fail(USCT_EINVAL, "max_bytes must be >= 0");
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache_cap = max_bytes;
+    release_kept(-1, max_bytes);
+    if (held) *held = g_kept_bytes;
+    return 0;
+}
+
 int usct_plan_destroy(usct_plan *P) {
     if (!P) return 0;
     if (P->stream) (void)hipStreamSynchronize(P->stream);
@@ -2223,12 +2521,14 @@ int usct_plan_get_info(const usct_plan *P, usct_plan_info *info) {
     info->shots_per_pass = P->Bp;
     info->ckpt_every = P->rev ? 0 : P->K;
     info->n_windows = P->rev ? 0 : P->NS;
-    info->workspace_bytes = P->bytes;
+    info->workspace_bytes = P->held;
     info->reverse = P->rev ? 1 : 0;
     info->sched = P->ta.sched;
     info->fwd_sched = P->tf.sched;
     info->strip_block = P->ta.blk;
     info->fwd_strip_block = P->tf.blk;
+    info->levels_per_launch = adj_levels(P);
+    info->fwd_levels_per_launch = fwd_levels(P);
     return 0;
 }
 

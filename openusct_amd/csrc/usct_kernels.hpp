@@ -66,6 +66,10 @@ constexpr int kBlock = 256;  // 4 wavefronts
 #define USCT_OCC3DX2 2
 #endif
 #define USCT_ATTR3DX2 __attribute__((amdgpu_waves_per_eu(USCT_OCC3DX2)))
+// Wavefronts per SIMD of the 2D kernels that advance LV levels per launch.
+#ifndef USCT_OCCXN
+#define USCT_OCCXN(LV) ((LV) <= 2 ? 3 : 2)
+#endif
 
 // Blocks are dispatched round-robin over the 8 XCDs (block i -> XCD i % 8).
 // Give each XCD a contiguous range of logical blocks so neighbouring strips,
@@ -622,6 +626,176 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
         for (int j = 0; j < CA; ++j) um[j] = uc[j + 1];
 #pragma unroll
         for (int j = 0; j < CA + 2; ++j) uc[j] = un[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched 2D forward, LV time levels per launch (temporal blocking along the
+// row march, no sponge): levels n .. n+LV-1 from levels n-1 (pc) and n-2 (pp).
+// Level l of the launch (1-based) trails level 1 by l-1 rows in the direction
+// of the march; it is computed on LV-l halo columns either side of the lane's
+// CX columns and on LV-l halo rows either side of the row chunk, so what it
+// hands to level l+1 equals the owners' values.  Every level is the FP64
+// update of k_fwd2d with the same operand order, so levels and traces are
+// bit-identical to LV single-level launches.  Per shot-cell a launch reads
+// levels n-1 and n-2 (hi + lo) once and writes the parts whose pointers `out`
+// sets: the next launch needs hi + lo of the last two levels and a gradient
+// keeps the hi of every level, so a gradient's forward moves 12 bytes per
+// shot-cell and level at LV = 2, 28/3 at LV = 3 and 8 at LV = 4.
+// Row chunks, their order and march direction: sched, as k_fwd2d_x2.
+// ---------------------------------------------------------------------------
+template <int LV, class L>
+struct FwdLevels {
+    float *hi[LV];                // level n+i (index i), FP32 part; nullptr: not stored
+    L *lo[LV];                    // its low part; nullptr: not stored
+    const double *sincd[LV];      // S_src w[n+i-1] per shot (FP64)
+    float *rec[LV], *rec_lo[LV];  // trace rows of level n+i (REC)
+};
+
+template <int LB, int CX, int LV, bool REC, class L = uint16_t>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(USCT_OCCXN(LV))))
+k_fwd2d_xn(const float *__restrict__ pc, const float *__restrict__ pp, const L *__restrict__ pcl,
+           const L *__restrict__ ppl, const FwdLevels<LV, L> out, const float2 *__restrict__ sp2,
+           const int *__restrict__ src_cell, const int *__restrict__ rec_head,
+           const int *__restrict__ rec_next, const int *__restrict__ row_rec, int ny, int nx,
+           int Bp, int RY, int nxs, int nyc, int ng, int sched) {
+    static_assert(LV >= 2 && LV <= 4, "levels per launch");
+    constexpr int W = kWave / LB;
+    constexpr int NC = CX + 2 * LV;  // columns x0-LV .. x0+CX+LV-1 (index g)
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    int sg, xs, yc, dir;
+    if (!chunk2d(wv, ng, nxs, nyc, sched, sg, xs, yc, dir)) return;
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    if (x0 >= nx) return;
+    const int y0 = yc * RY;
+    const int y1 = min(y0 + RY, ny);
+    const int my_src = src_cell[b];
+    double inc[LV];
+#pragma unroll
+    for (int i = 0; i < LV; ++i) inc[i] = out.sincd[i][b];
+    const size_t rowE = (size_t)nx * Bp;
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned o2 = (unsigned)(x0 * Bp + b) * (unsigned)sizeof(L);
+    const unsigned oc = (unsigned)x0 * 8u;
+    // level 1's first row: LV-1 halo rows outside the chunk on the side the
+    // march starts from; RY + 2(LV-1) fronts
+    const int yfs = dir > 0 ? y0 - (LV - 1) : y1 + (LV - 2);
+    const int n_front = y1 - y0 + 2 * (LV - 1);
+    auto rowE_at = [&](int yf, int r) { return clamp_row(yf + r * dir, ny) * rowE; };
+    // S[l][k][g]: level l of the launch (l = 0: level n-1 as loaded) in FP64.
+    // With front row yf (the row level 1 computes at front t), level l's row
+    // is yf - (l-1)*dir (level 0's: yf + dir), and the row a level takes at
+    // front t sits in slot k = t % 3: with the front loop unrolled by three the
+    // slots of the newest, middle and oldest rows are constants and no row is
+    // ever copied.  q: level n-2 on row yf.  nh/nl and qh/ql: raw loads issued
+    // one front ahead (level n-1 on row yf+2dir, level n-2 on yf+dir).
+    double S[LV][3][NC];
+    double q[NC];
+    float nh[NC], qh[NC];
+    L nl[NC], ql[NC];
+    {
+        // level 0's rows of fronts -1 (slot 2) and -2 (slot 1)
+        const size_t r0 = rowE_at(yfs, -1), r1 = rowE_at(yfs, 0), r2 = rowE_at(yfs, 1);
+#pragma unroll
+        for (int g = 0; g < NC; ++g) {
+            const ptrdiff_t c = (ptrdiff_t)(g - LV) * Bp;
+            S[0][2][g] = hl(ldb(pc + r1 + c, o4), ldb(pcl + r1 + c, o2));
+            nh[g] = ldb(pc + r2 + c, o4);
+            nl[g] = ldb(pcl + r2 + c, o2);
+            if (g >= 1 && g < NC - 1) {
+                S[0][1][g] = hl(ldb(pc + r0 + c, o4), ldb(pcl + r0 + c, o2));
+                qh[g] = ldn<USCT_NT2D>(pp + r1 + c, o4);
+                ql[g] = ldn<USCT_NT2D>(ppl + r1 + c, o2);
+            } else {
+                S[0][1][g] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int l = 1; l < LV; ++l)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int g = 0; g < NC; ++g) S[l][k][g] = 0.0;
+    }
+    for (int t0 = 0; t0 < n_front; t0 += 3) {
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph) {
+            const int t = t0 + ph;
+            if (t >= n_front) break;  // uniform over the wavefront
+            const int yf = yfs + t * dir;
+            const int kn = ph, km = (ph + 2) % 3, ko = (ph + 1) % 3;  // newest, middle, oldest
+#pragma unroll
+            for (int g = 0; g < NC; ++g) S[0][kn][g] = hl(nh[g], nl[g]);
+#pragma unroll
+            for (int g = 1; g < NC - 1; ++g) q[g] = hl(qh[g], ql[g]);
+            // ---- loads for the next front (none after the last)
+            if (t + 1 < n_front) {
+                const size_t rnn = rowE_at(yf, 2), en = rowE_at(yf, 1);
+#pragma unroll
+                for (int g = 0; g < NC; ++g) {
+                    const ptrdiff_t c = (ptrdiff_t)(g - LV) * Bp;
+                    nh[g] = ldb(pc + rnn + c, o4);
+                    nl[g] = ldb(pcl + rnn + c, o2);
+                }
+#pragma unroll
+                for (int g = 1; g < NC - 1; ++g) {
+                    const ptrdiff_t c = (ptrdiff_t)(g - LV) * Bp;
+                    qh[g] = ldn<USCT_NT2D>(pp + en + c, o4);
+                    ql[g] = ldn<USCT_NT2D>(ppl + en + c, o2);
+                }
+            }
+            // ---- level l on row yf - (l-1)*dir, columns x0-LV+l .. x0+CX+LV-l-1
+#pragma unroll
+            for (int l = 1; l <= LV; ++l) {
+                const int y = yf - (l - 1) * dir;
+                const bool act = y >= y0 - (LV - l) && y < y1 + (LV - l);
+                const bool own = y >= y0 && y < y1;
+                const bool yint = y >= 1 && y <= ny - 2;
+                const int yr = (int)clamp_row(y, ny);
+                const size_t ro = (size_t)yr * rowE;
+                const float2 *spr = sp2 + (size_t)yr * nx;
+                const bool rec_row = REC && own && row_rec[yr] > 0;
+                float *const oh = out.hi[l - 1];
+                L *const ol = out.lo[l - 1];
+#pragma unroll
+                for (int g = l; g < NC - l; ++g) {
+                    const int x = x0 - LV + g;
+                    double nw = 0.0;
+                    if (act) {
+                        // level l-1 on rows y+dir (newest), y, y-dir; level l-2 on row y
+                        const double cur = S[l - 1][km][g];
+                        const double prev = l == 1 ? q[g] : S[l - 2][ko][g];
+                        double lu = 0.0;
+                        if (yint && x >= 1 && x <= nx - 2) {
+                            const double yp = dir > 0 ? S[l - 1][kn][g] : S[l - 1][ko][g];
+                            const double ym = dir > 0 ? S[l - 1][ko][g] : S[l - 1][kn][g];
+                            lu = lap5d(cur, yp, ym, S[l - 1][km][g + 1], S[l - 1][km][g - 1]);
+                        }
+                        double incr = s2d(ldb(spr + (g - LV), oc)) * lu;
+                        if (y * nx + x == my_src) incr = incr + inc[l - 1];
+                        const double nv = (2.0 * cur - prev) + incr;
+                        L lb;
+                        const float v = split_hl(nv, lb);
+                        nw = hl(v, lb);
+                        if (own && g >= LV && g < LV + CX && x < nx) {
+                            const ptrdiff_t e = (ptrdiff_t)ro + (ptrdiff_t)(g - LV) * Bp;
+                            if (oh) stn<USCT_NT2D>(oh + e, o4, v);
+                            if (ol) stn<USCT_NT2D>(ol + e, o2, lb);
+                            if (rec_row) {
+                                for (int r = rec_head[yr * nx + x]; r >= 0; r = rec_next[r]) {
+                                    out.rec[l - 1][(size_t)r * Bp + b] = v;
+                                    out.rec_lo[l - 1][(size_t)r * Bp + b] = lo_f(lb);
+                                }
+                            }
+                        }
+                    }
+                    if (l < LV) S[l][kn][g] = nw;
+                }
+            }
+        }
     }
 }
 
@@ -1217,6 +1391,249 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
             fr[i] = fn2[i];
             hc[i] = hr[i];
             hr[i] = hn[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched 2D adjoint with the imaging condition, LV levels per launch (no
+// sponge): lam^k .. lam^{k-LV+1} from lam^{k+1} (l1) and lam^{k+2} (l2).
+// Level l of the launch (1-based, lam^{k-l+1}) trails level 1 by l-1 rows,
+// on LV-l halo columns and rows (with the receiver residuals of those cells),
+// as k_fwd2d_xn; it is imaged against its own stored forward level:
+//   g += lam^{k-l+1} d2p^{k-l+1},  d2p^{k-l+1} = S' lap_u(p^{k-l}) + S_src w[k-l].
+// A lane keeps each level's reduced imaging sum until the last level reaches
+// the same row, and each accumulator cell takes its LV Neumaier updates in
+// the order LV single-level launches apply them: fields and gradient are
+// bit-identical to those launches.  Per shot-cell a launch reads lam^{k+1},
+// lam^{k+2} and LV forward levels and writes the levels `out` asks for (the
+// next launch reads the last two): 24 bytes per two levels, 28 per three, 32
+// per four.
+// ---------------------------------------------------------------------------
+template <int LV>
+struct AdjLevels {
+    float *lam[LV];         // lam^{k-i} (index i); nullptr: not stored
+    const float *pf[LV];    // p^{k-i-1}, FP32 part (the level d2p^{k-i} differentiates)
+    const float *sinc[LV];  // S_src w[k-i-1] per shot
+    const float *res[LV];   // residual row k-i
+};
+
+template <int LB, int CX, int LV>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(USCT_OCCXN(LV))))
+k_adj2d_xn(const float *__restrict__ l1, const float *__restrict__ l2, const AdjLevels<LV> lv,
+           const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
+           const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+           const int *__restrict__ row_rec, float *__restrict__ g_hi, float *__restrict__ g_lo,
+           int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng, int sched) {
+    static_assert(LV >= 2 && LV <= 4, "levels per launch");
+    constexpr int W = kWave / LB;
+    constexpr int NC = CX + 2 * LV;  // columns x0-LV .. x0+CX+LV-1 (index g)
+    const int wv = wave_id();
+    const int lane = threadIdx.x & 63;
+    int sg, xs, yc, dir;
+    if (!chunk2d(wv, ng, nxs, nyc, sched, sg, xs, yc, dir)) return;  // uniform over the wavefront
+    const int sb = lane % LB, st = lane / LB;
+    const int b = sg * LB + sb;
+    const int x0 = (xs * W + st) * CX;
+    const int y0 = yc * RY;
+    const int y1 = min(y0 + RY, ny);
+    const size_t rowE = (size_t)nx * Bp;
+    const size_t N = (size_t)ny * nx;
+    const int my_src = src_cell[b];
+    float inc[LV];
+#pragma unroll
+    for (int i = 0; i < LV; ++i) inc[i] = lv.sinc[i][b];
+    const unsigned o4 = (unsigned)(x0 * Bp + b) * 4u;
+    const unsigned oc = (unsigned)x0 * 8u;
+    const int yfs = dir > 0 ? y0 - (LV - 1) : y1 + (LV - 2);
+    const int n_front = y1 - y0 + 2 * (LV - 1);
+    auto row_at = [&](int yf, int r) { return clamp_row(yf + r * dir, ny); };
+    // Q[l][k][g] = S' lam and R[l][k][g] = lam of level l of the launch (l = 0:
+    // lam^{k+1} as loaded).  With front row yf (the row level 1 computes at
+    // front t), level l's row is yf - (l-1)*dir (level 0's: yf + dir), and the
+    // row a level takes at front t sits in slot k = t % 3: with the front loop
+    // unrolled by three no row is ever copied.  ln: lam^{k+1} on row yf+2dir,
+    // c2 / c2n: lam^{k+2} on rows yf / yf+dir (raw loads one front ahead).
+    // F[l][k][i]: forward level p^{k-l} (index l-1) around level l's row y,
+    // columns x0-1 .. x0+CX (index i): row y+dir of front t in slot t % 3; the
+    // row for the next front is loaded into the oldest slot once this front's
+    // imaging has read it.
+    float Q[LV][3][NC], R[LV][3][NC];
+    float ln[NC], c2[NC], c2n[NC];
+    float F[LV][3][CX + 2];
+    float hist[LV][LV];  // imaging sums of levels 1 .. LV-1, newest first
+    {
+        // level 0's rows of fronts -1 (slot 2) and -2 (slot 1)
+        const size_t r0 = row_at(yfs, -1), r1 = row_at(yfs, 0), r2 = row_at(yfs, 1);
+        const float *a0 = l1 + r0 * rowE, *a1 = l1 + r1 * rowE, *a2 = l1 + r2 * rowE;
+        const float2 *s0 = sp2 + r0 * nx, *s1 = sp2 + r1 * nx;
+#pragma unroll
+        for (int g = 0; g < NC; ++g) {
+            const ptrdiff_t c = (ptrdiff_t)(g - LV) * Bp;
+            ln[g] = ldb(a2 + c, o4);
+            R[0][2][g] = ldb(a1 + c, o4);
+            Q[0][2][g] = smul(ldb(s1 + (g - LV), oc), R[0][2][g]);
+            if (g >= 1 && g < NC - 1) {
+                R[0][1][g] = ldb(a0 + c, o4);
+                Q[0][1][g] = smul(ldb(s0 + (g - LV), oc), R[0][1][g]);
+                c2n[g] = ldn<USCT_NT2D>(l2 + r1 * rowE + c, o4);
+            } else {
+                R[0][1][g] = 0.0f;
+                Q[0][1][g] = 0.0f;
+            }
+        }
+#pragma unroll
+        for (int l = 1; l < LV; ++l)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int g = 0; g < NC; ++g) { Q[l][k][g] = 0.0f; R[l][k][g] = 0.0f; }
+#pragma unroll
+        for (int l = 1; l <= LV; ++l) {
+            // rows y+dir, y, y-dir of front 0 in slots 0, 2, 1
+            const int y = yfs - (l - 1) * dir;
+            const float *f = lv.pf[l - 1];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const float *fr = f + row_at(y, 1 - r) * rowE;
+#pragma unroll
+                for (int i = 0; i < CX + 2; ++i)
+                    F[l - 1][(3 - r) % 3][i] = ldb(fr + (ptrdiff_t)(i - 1) * Bp, o4);
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < LV; ++l)
+#pragma unroll
+            for (int d = 0; d < LV; ++d) hist[l][d] = 0.0f;
+    }
+    // the lane's reduced cell (the same on every row)
+    const int gx = x0 + sum_cell<LB, CX>(lane);
+    const bool gval = (sb & (LB / CX - 1)) == 0 && gx < nx;
+    float *const ghb = g_hi + (size_t)sg * N + gx;
+    float *const glb = g_lo + (size_t)sg * N + gx;
+    for (int t0 = 0; t0 < n_front; t0 += 3) {
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph) {
+            const int t = t0 + ph;
+            if (t >= n_front) break;  // uniform over the wavefront
+            const int yf = yfs + t * dir;
+            const int kn = ph, km = (ph + 2) % 3, ko = (ph + 1) % 3;  // newest, middle, oldest
+            const int yg = yf - (LV - 1) * dir;  // the last level's row: accumulators updated there
+            const bool gown = yg >= y0 && yg < y1;
+            float gh0 = 0.0f, gl0 = 0.0f;
+            if (gval && gown) {
+                gh0 = ghb[(size_t)yg * nx];
+                gl0 = glb[(size_t)yg * nx];
+            }
+            // ---- level 0's newest row (S' lam^{k+1} on row yf+dir), lam^{k+2} on row yf
+            {
+                const float2 *snr = sp2 + row_at(yf, 1) * nx;
+#pragma unroll
+                for (int g = 0; g < NC; ++g) {
+                    R[0][kn][g] = ln[g];
+                    Q[0][kn][g] = smul(ldb(snr + (g - LV), oc), ln[g]);
+                }
+#pragma unroll
+                for (int g = 1; g < NC - 1; ++g) c2[g] = c2n[g];
+            }
+            // ---- loads for the next front: lam^{k+1} row yf+2dir, lam^{k+2} row yf+dir
+            const bool more = t + 1 < n_front;
+            if (more) {
+                const float *a = l1 + row_at(yf, 2) * rowE;
+                const float *c = l2 + row_at(yf, 1) * rowE;
+#pragma unroll
+                for (int g = 0; g < NC; ++g) ln[g] = ldb(a + (ptrdiff_t)(g - LV) * Bp, o4);
+#pragma unroll
+                for (int g = 1; g < NC - 1; ++g)
+                    c2n[g] = ldn<USCT_NT2D>(c + (ptrdiff_t)(g - LV) * Bp, o4);
+            }
+            float s_cur[LV];
+#pragma unroll
+            for (int l = 1; l <= LV; ++l) {
+                const int y = yf - (l - 1) * dir;
+                const bool act = y >= y0 - (LV - l) && y < y1 + (LV - l);
+                const bool own = y >= y0 && y < y1;
+                const bool yint = y >= 1 && y <= ny - 2;
+                const size_t yr = clamp_row(y, ny);
+                const bool rec = act && y >= 0 && y < ny && row_rec[yr] > 0;
+                const float2 *spr = sp2 + yr * nx;
+                float *const out = lv.lam[l - 1];
+                const float *const res = lv.res[l - 1];
+                float uo[CX];
+#pragma unroll
+                for (int g = l; g < NC - l; ++g) {
+                    const int x = x0 - LV + g;
+                    float v = 0.0f;
+                    if (act) {
+                        // level l-1 on rows y+dir (newest), y, y-dir; level l-2 on row y
+                        const float cur = R[l - 1][km][g];
+                        const float prev = l == 1 ? c2[g] : R[l - 2][ko][g];
+                        const bool in = yint && x >= 1 && x <= nx - 2;
+                        const float yp = dir > 0 ? Q[l - 1][kn][g] : Q[l - 1][ko][g];
+                        const float ym = dir > 0 ? Q[l - 1][ko][g] : Q[l - 1][kn][g];
+                        const float lq =
+                            in ? lap5(Q[l - 1][km][g], yp, ym, Q[l - 1][km][g + 1], Q[l - 1][km][g - 1])
+                               : 0.0f;
+                        v = leap(cur, prev, lq);
+                        if (rec && x >= 0 && x < nx) {
+                            for (int r = rec_head[(int)yr * nx + x]; r >= 0; r = rec_next[r])
+                                v = v - res[(size_t)r * Bp + b];
+                        }
+                        if (own && out && g >= LV && g < LV + CX && x < nx)
+                            stn<USCT_NT2D>(out + yr * rowE + (size_t)(g - LV) * Bp, o4, v);
+                    }
+                    if (g >= LV && g < LV + CX) uo[g - LV] = v;
+                    if (l < LV) {
+                        R[l][kn][g] = v;
+                        Q[l][kn][g] = act ? smul(ldb(spr + (g - LV), oc), v) : 0.0f;
+                    }
+                }
+                // imaging of level l on its row (own rows only)
+                s_cur[l - 1] = 0.0f;
+                if (own) {
+                    float tv[CX];
+#pragma unroll
+                    for (int i = 0; i < CX; ++i) {
+                        const int x = x0 + i;
+                        tv[i] = 0.0f;
+                        if (x < nx) {
+                            const bool in = yint && x >= 1 && x <= nx - 2;
+                            const float fp = dir > 0 ? F[l - 1][kn][i + 1] : F[l - 1][ko][i + 1];
+                            const float fq = dir > 0 ? F[l - 1][ko][i + 1] : F[l - 1][kn][i + 1];
+                            const float lp = in ? lap5(F[l - 1][km][i + 1], fp, fq, F[l - 1][km][i + 2],
+                                                       F[l - 1][km][i])
+                                                : 0.0f;
+                            float d2 = smul(ldb(spr + i, oc), lp);
+                            if ((int)yr * nx + x == my_src) d2 = d2 + inc[l - 1];
+                            tv[i] = uo[i] * d2;
+                        }
+                    }
+                    s_cur[l - 1] = reduce_scatter<LB, CX>(tv, lane);
+                }
+                // the forward level's row for the next front, while it is within one
+                // row of the chunk (the oldest slot has been read)
+                const int yn = y + 2 * dir;
+                if (more && yn >= y0 - 1 && yn <= y1) {
+                    const float *f = lv.pf[l - 1] + clamp_row(yn, ny) * rowE;
+#pragma unroll
+                    for (int i = 0; i < CX + 2; ++i) F[l - 1][ko][i] = ldb(f + (ptrdiff_t)(i - 1) * Bp, o4);
+                }
+            }
+            // ---- accumulators of row yg: levels 1 .. LV in order (as LV single-level launches)
+            if (gval && gown) {
+                float hi = gh0, lo = gl0;
+#pragma unroll
+                for (int l = 1; l < LV; ++l) neumaier(hi, lo, hist[l - 1][LV - l - 1]);
+                neumaier(hi, lo, s_cur[LV - 1]);
+                ghb[(size_t)yg * nx] = hi;
+                glb[(size_t)yg * nx] = lo;
+            }
+#pragma unroll
+            for (int l = 1; l < LV; ++l) {
+#pragma unroll
+                for (int d = LV - l - 1; d >= 1; --d) hist[l - 1][d] = hist[l - 1][d - 1];
+                hist[l - 1][0] = s_cur[l - 1];
+            }
         }
     }
 }

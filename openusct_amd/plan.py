@@ -46,7 +46,8 @@ class Plan:
 
     def __init__(self, shape, h, dt, nt, tx_lin, rec_lin, method="batched", ckpt_every=0,
                  mem_budget=0, sponge=False, autotune=True, graph=True, forward_only=False,
-                 fuse=True, tx_groups=None, rec_groups=None, reverse=None, fp32_lo=False):
+                 fuse=True, tx_groups=None, rec_groups=None, reverse=None, fp32_lo=False,
+                 levels=None):
         """``tx_lin`` / ``rec_lin``: one linear cell per transmit / receiver.  With
         ``tx_groups`` and ``rec_groups`` (lists of (linear cells, weights) per
         element, as finite-aperture footprints) those replace the point cells and
@@ -55,7 +56,10 @@ class Plan:
         backwards alongside the adjoint; True / False force one.  ``fp32_lo`` carries
         the forward state as FP32 hi + FP32 lo instead of FP32 hi + bfloat16 lo (8
         bytes per shot-cell instead of 6), for time axes so short that the traces'
-        rounding is a visible share of the residual; no Gauss-Newton products."""
+        rounding is a visible share of the residual; no Gauss-Newton products.
+        ``levels`` (batched 2D without sponge): time levels per fused launch, 2 (the
+        default) or 3 (fewer bytes per level, more registers and arithmetic: slower
+        on MI355X at 512^2 x 128 shots).  The results do not depend on it."""
         lib = _lib.load()
         self.shape = tuple(int(s) for s in shape)
         self.ndim = len(self.shape)
@@ -79,7 +83,8 @@ class Plan:
             (0 if graph else _lib.FLAG_NO_GRAPH) | (_lib.FLAG_FORWARD_ONLY if forward_only else 0) | \
             (0 if fuse else _lib.FLAG_NO_FUSE) | \
             {None: 0, True: _lib.FLAG_REVERSE, False: _lib.FLAG_NO_REVERSE}[reverse] | \
-            (_lib.FLAG_FP32_LO if fp32_lo else 0)
+            (_lib.FLAG_FP32_LO if fp32_lo else 0) | \
+            {None: 0, 2: _lib.FLAG_LEVELS_2, 3: _lib.FLAG_LEVELS_3}[levels]
         dims = (ctypes.c_int * self.ndim)(*self.shape)
         handle = ctypes.c_void_p()
         torch.cuda.init()
@@ -262,6 +267,16 @@ def _csr(groups):
         off.append(len(cells))
     return (np.asarray(off, dtype=np.int64), np.asarray(cells or [0], dtype=np.int64),
             np.asarray(w or [0.0], dtype=np.float64))
+
+
+def device_cache(max_bytes):
+    """Keep up to ``max_bytes`` of device memory from closed plans for later plans
+    (0 frees what is kept and keeps nothing, the default); returns the bytes kept.
+    Closing a plan then costs no hipFree (which waits for the whole device) and a
+    new plan of a similar size takes kept blocks instead of mapping fresh memory."""
+    held = ctypes.c_int64(0)
+    _lib.check(_lib.load().usct_device_cache(int(max_bytes), ctypes.byref(held)))
+    return int(held.value)
 
 
 def footprint_bytes(shape, nt, n_tx, n_rec, ckpt_every=0, gradient=True):

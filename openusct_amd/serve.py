@@ -14,9 +14,11 @@ Results do not depend on what else is in flight: a request's arithmetic is
 fixed by its own inputs (tile shapes are chosen by rule, not by timing), and
 its reductions run in a fixed order inside its own buffers.
 
-Plans of finished requests are released lazily (``hipFree`` waits for the
-whole device): only when a waiting request needs their memory, or when the
-device is idle.
+The device memory of a finished request's plan is kept for the next plans
+(``device_cache``, capped at the loop's budget) instead of being freed: hipFree
+waits for the whole device, and a first hipMalloc of many GB maps fresh pages,
+each of which took longer than a small request's solve.  What is kept is freed
+when the loop ends.
 """
 
 from __future__ import annotations
@@ -28,7 +30,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .plan import Plan, footprint_bytes
+from .plan import Plan, device_cache, footprint_bytes
 
 
 @dataclass
@@ -103,25 +105,13 @@ class Server:
         self.use_graph = use_graph
         self.device = torch.device("cuda", torch.cuda.current_device())
         self.cap = self.budget // 2  # largest reservation of one request
-        self._held = 0            # bytes of plans in flight or awaiting release
-        self._retired = []        # finished plans not yet freed
+        self._held = 0            # bytes of plans in flight
 
     # -- memory -----------------------------------------------------------------
-    def _release(self):
-        if not self._retired:
-            return
-        for p, nbytes in self._retired:
-            p.close()
-            self._held -= nbytes
-        self._retired.clear()
-
     def _fits(self, req, in_flight):
-        need = req.footprint(self.cap) + (64 << 20)
+        # kept blocks are 2 MiB multiples, up to 1.25x the buffer they serve
+        need = int(req.footprint(self.cap) * 1.25) + (64 << 20)
         if self._held + need <= self.budget:
-            return True
-        retired = sum(b for _, b in self._retired)
-        if retired and self._held - retired + need <= self.budget:
-            self._release()
             return True
         if need > self.budget:
             raise MemoryError(f"request {req.rid} needs {need} bytes, budget {self.budget}")
@@ -135,7 +125,7 @@ class Server:
         plan = Plan(req.shape, req.h, req.dt, req.nt, req.tx, req.rec, method="batched",
                     mem_budget=int(est * 1.05) + (64 << 20), autotune=False,
                     graph=self.use_graph, forward_only=req.kind == "forward")
-        nbytes = max(int(plan.info["workspace_bytes"]), est)
+        nbytes = max(int(plan.info["workspace_bytes"]), int(est * 1.25))
         self._held += nbytes
         fl = _Flight(req, plan, stream, torch.cuda.Event(), now)
         fl.tensors.append(nbytes)
@@ -156,6 +146,13 @@ class Server:
 
     def run(self, requests, on_result=None, progress=None):
         """Serve ``requests`` (arrival times relative to now); returns results by rid."""
+        device_cache(self.budget)
+        try:
+            return self._run(requests, on_result, progress)
+        finally:
+            device_cache(0)
+
+    def _run(self, requests, on_result, progress):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue, flights, results = [], [], {}
         t0 = time.perf_counter()
@@ -186,7 +183,8 @@ class Server:
                                  fl.grad, fl.data, r.arrival, fl.admitted, done, r.updates,
                                  fl.tensors[0])
                     results[r.rid] = res
-                    self._retired.append((fl.plan, fl.tensors[0]))
+                    fl.plan.close()  # its blocks are kept for the next plans
+                    self._held -= fl.tensors[0]
                     if on_result:
                         on_result(res)
                     if progress:
@@ -195,12 +193,10 @@ class Server:
                     still.append(fl)
             flights = still
             if not flights and not queue:
-                self._release()
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
             elif not any(f.event.query() for f in flights):
                 time.sleep(2e-4)
-        self._release()
         return results
 
 

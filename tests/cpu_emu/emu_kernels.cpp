@@ -57,8 +57,8 @@ struct Problem {
     unsigned blocks;
     float2 *sp2;
     int *src_cell, *rec_head, *rec_next, *row_rec;
-    float *sinc;    // [3][Bp]
-    double *sincd;  // [3][Bp], the same values in FP64
+    float *sinc;    // [5][Bp]
+    double *sincd;  // [5][Bp], the same values in FP64
     std::mt19937 rng;
 
     Problem(int ny_, int nx_, int LB_, int ng_, int CX_, int RY_, uint32_t seed)
@@ -90,10 +90,10 @@ struct Problem {
             src_cell[b] = y * nx + x;
         }
         src_cell[Bp - 1] = -1;
-        sinc = alloc<float>(3 * (int64_t)Bp, 0);
-        sincd = alloc<double>(3 * (int64_t)Bp, 0);
+        sinc = alloc<float>(5 * (int64_t)Bp, 0);
+        sincd = alloc<double>(5 * (int64_t)Bp, 0);
         std::uniform_real_distribution<float> w(-2.0f, 2.0f);
-        for (int i = 0; i < 3 * Bp; ++i) {
+        for (int i = 0; i < 5 * Bp; ++i) {
             sincd[i] = (double)w(rng) * 1.0000001234567;
             sinc[i] = (float)sincd[i];
         }
@@ -314,6 +314,165 @@ bool adj_cases(uint32_t seed) {
             for (int sched = 0; sched < 8; ++sched)
                 ok &= check_adj<LB, CX>(s[0], s[1], 2, RY, seed++, sched);
     ok &= check_adj<LB, CX>(29, 37, 1, 4, seed++, 3);
+    return ok;
+}
+
+// LV-level forward launch (k_fwd2d_xn) against LV single-level launches of
+// k_fwd2d.  sparse: the launch stores only what a gradient's forward asks of
+// it (hi of every level but the second, lo of the last two), the rest must be
+// left untouched.
+template <int LB, int CX, int LV, bool REC, class L = uint16_t>
+bool check_fwdn(int ny, int nx, int ng, int RY, uint32_t seed, int sched, bool sparse) {
+    Problem P(ny, nx, LB, ng, CX, RY, seed);
+    char tag[200];
+    std::snprintf(tag, sizeof tag,
+                  "fwd-x%d ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d rec=%d sched=%d lo=%d sparse=%d", LV,
+                  ny, nx, LB, ng, CX, RY, (int)REC, sched, (int)sizeof(L), (int)sparse);
+    float *pc = P.field(1.0f), *pp = P.field(1.0f);
+    L *pcl = P.template lowT<L>(1e-7f), *ppl = P.template lowT<L>(1e-7f);
+    float *ref[LV], *out[LV], *rr[LV], *rrl[LV], *ro[LV], *rol[LV];
+    L *refl[LV], *outl[LV];
+    const float *c = pc, *p = pp;
+    const L *cl = pcl, *pl = ppl;
+    for (int i = 0; i < LV; ++i) {
+        ref[i] = P.zeros();
+        refl[i] = P.template zeros_loT<L>();
+        out[i] = P.zeros();
+        outl[i] = P.template zeros_loT<L>();
+        rr[i] = P.traces();
+        rrl[i] = P.traces();
+        ro[i] = P.traces();
+        rol[i] = P.traces();
+        const double *sd = P.sincd + (int64_t)(i + 1) * P.Bp;
+        float *a = ref[i];
+        L *al = refl[i];
+        emu::launch(P.blocks, kBlock, [&] {
+            k_fwd2d<LB, CX, false, REC, L>(c, p, a, cl, pl, al, P.sp2, nullptr, P.src_cell, sd,
+                                           P.rec_head, P.rec_next, P.row_rec, rr[i], rrl[i], P.ny,
+                                           P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+        });
+        p = c;
+        pl = cl;
+        c = ref[i];
+        cl = refl[i];
+    }
+    FwdLevels<LV, L> o{};
+    for (int i = 0; i < LV; ++i) {
+        const bool last2 = i >= LV - 2;
+        o.hi[i] = sparse && !last2 && i == 1 ? nullptr : out[i];
+        o.lo[i] = sparse && !last2 ? nullptr : outl[i];
+        o.sincd[i] = P.sincd + (int64_t)(i + 1) * P.Bp;
+        o.rec[i] = ro[i];
+        o.rec_lo[i] = rol[i];
+    }
+    emu::launch(P.blocks, kBlock, [&] {
+        k_fwd2d_xn<LB, CX, LV, REC, L>(pc, pp, pcl, ppl, o, P.sp2, P.src_cell, P.rec_head,
+                                       P.rec_next, P.row_rec, P.ny, P.nx, P.Bp, P.RY, P.nxs, P.nyc,
+                                       P.ng, sched);
+    });
+    const int64_t nr = (int64_t)P.n_rec * P.Bp;
+    float *zf = P.zeros();
+    L *zl = P.template zeros_loT<L>();
+    bool ok = true;
+    for (int i = 0; i < LV; ++i) {
+        char what[64];
+        std::snprintf(what, sizeof what, "level n+%d", i);
+        ok &= o.hi[i] ? same(what, ref[i], out[i], P.FE, tag) : same(what, zf, out[i], P.FE, tag);
+        std::snprintf(what, sizeof what, "level n+%d low", i);
+        ok &= o.lo[i] ? same(what, refl[i], outl[i], P.FE, tag) : same(what, zl, outl[i], P.FE, tag);
+        if (REC) {
+            std::snprintf(what, sizeof what, "traces n+%d", i);
+            ok &= same(what, rr[i], ro[i], nr, tag);
+            std::snprintf(what, sizeof what, "traces n+%d low", i);
+            ok &= same(what, rrl[i], rol[i], nr, tag);
+        }
+    }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+// LV-level adjoint launch (k_adj2d_xn) against LV single-level launches of
+// k_adj2d: adjoint levels and the Neumaier gradient accumulators.
+template <int LB, int CX, int LV>
+bool check_adjn(int ny, int nx, int ng, int RY, uint32_t seed, int sched, bool sparse) {
+    Problem P(ny, nx, LB, ng, CX, RY, seed);
+    char tag[200];
+    std::snprintf(tag, sizeof tag, "adj-x%d ny=%d nx=%d LB=%d ng=%d CX=%d RY=%d sched=%d sparse=%d",
+                  LV, ny, nx, LB, ng, CX, RY, sched, (int)sparse);
+    float *l1 = P.field(1.0f), *l2 = P.field(1.0f);
+    float *pf[LV], *ref[LV], *out[LV];
+    for (int i = 0; i < LV; ++i) {
+        pf[i] = P.field(1.0f);
+        ref[i] = P.zeros();
+        out[i] = P.zeros();
+    }
+    const int64_t nr = (int64_t)P.n_rec * P.Bp;
+    float *res = alloc<float>(LV * nr, 0);
+    std::uniform_real_distribution<float> d(-3.0f, 3.0f);
+    for (int64_t i = 0; i < LV * nr; ++i) res[i] = d(P.rng);
+    const int64_t gn = (int64_t)P.ng * P.N;
+    float *gh1 = alloc<float>(gn, 0), *gl1 = alloc<float>(gn, 0);
+    float *gh2 = alloc<float>(gn, 0), *gl2 = alloc<float>(gn, 0);
+    std::uniform_real_distribution<float> gd(-10.0f, 10.0f), ld(-1e-6f, 1e-6f);
+    for (int64_t i = 0; i < gn; ++i) {
+        gh1[i] = gh2[i] = gd(P.rng);
+        gl1[i] = gl2[i] = ld(P.rng);
+    }
+    const float *a = l1, *b = l2;
+    for (int i = 0; i < LV; ++i) {
+        float *u = ref[i];
+        const float *si = P.sinc + (int64_t)(i + 1) * P.Bp, *ri = res + i * nr;
+        emu::launch(P.blocks, kBlock, [&] {
+            k_adj2d<LB, CX, false>(a, b, u, pf[i], nullptr, nullptr, P.sp2, nullptr, P.src_cell,
+                                   si, P.rec_head, P.rec_next, P.row_rec, ri, nullptr, gh1, gl1,
+                                   P.ny, P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng);
+        });
+        b = a;
+        a = ref[i];
+    }
+    AdjLevels<LV> o{};
+    for (int i = 0; i < LV; ++i) {
+        o.lam[i] = sparse && i < LV - 2 ? nullptr : out[i];
+        o.pf[i] = pf[i];
+        o.sinc[i] = P.sinc + (int64_t)(i + 1) * P.Bp;
+        o.res[i] = res + i * nr;
+    }
+    emu::launch(P.blocks, kBlock, [&] {
+        k_adj2d_xn<LB, CX, LV>(l1, l2, o, P.sp2, P.src_cell, P.rec_head, P.rec_next, P.row_rec,
+                               gh2, gl2, P.ny, P.nx, P.Bp, P.RY, P.nxs, P.nyc, P.ng, sched);
+    });
+    float *zf = P.zeros();
+    bool ok = same("g_hi", gh1, gh2, gn, tag) & same("g_lo", gl1, gl2, gn, tag);
+    for (int i = 0; i < LV; ++i) {
+        char what[64];
+        std::snprintf(what, sizeof what, "lam^{k-%d}", i);
+        ok &= o.lam[i] ? same(what, ref[i], out[i], P.FE, tag) : same(what, zf, out[i], P.FE, tag);
+    }
+    std::printf("%s %s\n", ok ? "ok" : "FAIL", tag);
+    return ok;
+}
+
+// all: every wavefront order and march direction (sched 0..7) over five grids and
+// four row counts; else sched 0, 5, 6 over three grids and two row counts
+template <int LB, int CX, int LV>
+bool xn_cases(uint32_t seed, bool all) {
+    bool ok = true;
+    const int shapes[][2] = {{29, 37}, {3, 5}, {7, 4}, {9, 13}, {40, 33}};
+    for (int si = 0; si < (all ? 5 : 3); ++si)
+        for (int RY : {1, 5, 4, 64}) {
+            if (!all && RY != 1 && RY != 5) continue;
+            const int *s = shapes[si];
+            for (int sched = 0; sched < 8; ++sched) {
+                if (!all && sched != 0 && sched != 5 && sched != 6) continue;
+                const bool sparse = (sched & 1) != 0;
+                ok &= check_fwdn<LB, CX, LV, true>(s[0], s[1], 2, RY, seed++, sched, sparse);
+                ok &= check_adjn<LB, CX, LV>(s[0], s[1], 2, RY, seed++, sched, sparse);
+            }
+        }
+    ok &= check_fwdn<LB, CX, LV, false>(29, 37, 1, 4, seed++, 3, true);
+    ok &= check_fwdn<LB, CX, LV, true, float>(29, 37, 2, 8, seed++, 1, false);
+    ok &= check_fwdn<LB, CX, LV, true, float>(9, 13, 1, 3, seed++, 6, true);
+    ok &= check_adjn<LB, CX, LV>(29, 37, 1, 4, seed++, 3, false);
     return ok;
 }
 
@@ -1359,6 +1518,14 @@ int main(int argc, char **argv) {
         ok &= adj_cases<32, 4>(seed + 700);
         ok &= adj_cases<64, 4>(seed + 800);
         ok &= adj_cases<64, 8>(seed + 900);
+    }
+    if (only.empty() || only == "xn") {
+        ok &= xn_cases<64, 4, 3>(seed + 11500, true);
+        ok &= xn_cases<64, 4, 2>(seed + 11000, false);
+        ok &= xn_cases<16, 4, 3>(seed + 12000, false);
+        ok &= xn_cases<32, 4, 3>(seed + 12500, false);
+        ok &= xn_cases<16, 8, 3>(seed + 13000, false);
+        ok &= xn_cases<64, 4, 4>(seed + 13500, false);
     }
     if (only.empty() || only == "lds") {
         ok &= lds_cases(seed + 2000);

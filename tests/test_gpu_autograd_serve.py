@@ -140,3 +140,36 @@ def test_server_admission_under_tight_budget():
             assert a.J == b.J and torch.equal(a.grad, b.grad)
         else:
             assert torch.equal(a.data, b.data)
+
+
+def test_device_cache_keeps_and_reuses_plan_memory():
+    """usct_device_cache: a destroyed plan's blocks are kept (up to the cap), the
+    next plan of the same shape takes them instead of new memory, results do not
+    change, and a cap of zero frees what is kept."""
+    from openusct_amd.plan import Plan, device_cache
+    reqs = _small_mix(n=3, seed=13)
+    r = next(q for q in reqs if q.kind == "gradient")
+    dev = torch.device("cuda")
+    args = (r.shape, r.h, r.dt, r.nt, r.tx, r.rec)
+    m, w, d = (torch.as_tensor(a, device=dev) for a in (r.m, r.wavelet, r.dobs))
+    assert device_cache(0) == 0
+    with Plan(*args, autotune=False) as p:
+        J0, g0 = p.misfit_and_gradient(m, w, d)
+        plain_bytes = p.info["workspace_bytes"]
+    try:
+        assert device_cache(64 << 30) == 0
+        with Plan(*args, autotune=False) as p:
+            held = p.info["workspace_bytes"]
+            assert plain_bytes <= held <= plain_bytes * 1.25 + 64 * (2 << 20)
+            J1, g1 = p.misfit_and_gradient(m, w, d)
+        kept = device_cache(64 << 30)
+        print(f"[case] device cache: plan asked {plain_bytes}, held {held}, kept {kept}", flush=True)
+        assert kept == held
+        with Plan(*args, autotune=False) as p:
+            assert device_cache(64 << 30) == 0  # every block taken again
+            J2, g2 = p.misfit_and_gradient(m, w, d)
+        assert device_cache(held - 1) < held  # a lower cap frees blocks
+    finally:
+        assert device_cache(0) == 0
+    for J, g in ((J1, g1), (J2, g2)):
+        assert torch.equal(J, J0) and torch.equal(g, g0)

@@ -229,8 +229,12 @@ def test_two_level_launches_bit_identical(nt, K):
         print(f"[case] two-level vs one-level nt={nt} K={K} {info}", flush=True)
         dobs = one.forward_fmc(mt, wd)
         J1, g1 = one.misfit_and_gradient(m0, wd, dobs)
-    with Plan(m0.shape, h, dt, nt, tx, rec, fuse=True, **kw) as two:
-        assert two.info == info
+    with Plan(m0.shape, h, dt, nt, tx, rec, fuse=True, levels=2, **kw) as two:
+        # the same tiles; fused plans hold one more rotation buffer
+        skip = ("workspace_bytes", "levels_per_launch", "fwd_levels_per_launch")
+        assert {k: v for k, v in two.info.items() if k not in skip} == \
+            {k: v for k, v in info.items() if k not in skip}
+        assert two.info["levels_per_launch"] == min(2, nt - 1) or nt < 3
         d2 = two.forward_fmc(mt, wd)
         J2, g2 = two.misfit_and_gradient(m0, wd, dobs)
     assert torch.equal(d2, dobs)

@@ -14,7 +14,9 @@ direct loop of both updates and its imaging sums, the fused 3D Gauss-Newton
 level (k_born3d) k_fwd3d followed by k_born, the Born step a direct loop of its FP64 update, the two-level
 3D forwards (LDS tiles, and lane patches marching z in registers) two single-level launches, the
 forward kernels and the reversing 3D adjoint with FP32 low parts (USCT_FLAG_FP32_LO) against their direct
-loops and two single-level launches, the 3D patch kernels with every strip-block order the tuner may pick, the two-level Gauss-Newton launch
+loops and two single-level launches, the 3D patch kernels with every strip-block order the tuner may pick, the 2D kernels that advance
+two, three or four levels per launch (k_fwd2d_xn, k_adj2d_xn: with bfloat16 and FP32 low parts,
+storing every level or only what a gradient keeps) LV single-level launches, the two-level Gauss-Newton launch
 (background and Born levels together) two forward and two Born launches, the
 GSOT assignment kernel against the optimality certificate of its own dual
 potentials and, on short traces, brute force over all permutations,
@@ -58,7 +60,7 @@ def emu_binary(tmp_path_factory):
 
 @pytest.mark.parametrize("seed", [1, 2])
 @pytest.mark.parametrize("which", ["fwd", "adj", "fwd3d", "lds", "born", "x2-3d", "x2p-3d", "born-x2",
-                                   "gsot", "radj3d", "born3d", "lo32", "blk3d"])
+                                   "gsot", "radj3d", "born3d", "lo32", "blk3d", "xn"])
 def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
     r = subprocess.run([emu_binary, str(seed), which], capture_output=True, text=True,
@@ -69,4 +71,4 @@ def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     assert r.stdout.count("ok ") == {"fwd": 495, "adj": 485, "fwd3d": 50, "lds": 12,
                                      "born": 5, "x2-3d": 6, "x2p-3d": 10, "born-x2": 52, "gsot": 13,
                                      "radj3d": 28, "born3d": 20, "lo32": 24,
-                                     "blk3d": 120}[which], tail
+                                     "blk3d": 120, "xn": 524}[which], tail
