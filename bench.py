@@ -177,37 +177,49 @@ def _bw_lib():
     lib.usct_bw_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]
+    lib.usct_bw_run_pass.argtypes = lib.usct_bw_run.argtypes
+    lib.usct_bw_pass_groups.argtypes = [ctypes.c_int64, ctypes.c_int]
+    lib.usct_bw_pass_groups.restype = ctypes.c_int64
     lib.usct_bw_device_cus.restype = ctypes.c_int
     return lib
 
 
 def stream_ceiling(gib=4, reps=5):
     """Best read / write / copy rates of the streaming kernels over `gib` GiB
-    buffers (far beyond L2 and Infinity Cache), over work-groups per CU {2, 4, 8,
-    16} and ordinary or non-temporal accesses.  Copy counts read + write bytes."""
+    buffers (far beyond L2 and Infinity Cache): grid-stride loops over {2, 4, 8,
+    16} work-groups per CU and single-pass grids with {1, 2, 4, 8} accesses per
+    lane, each with ordinary or non-temporal accesses.  Copy counts read + write
+    bytes.  Every configuration is kept in `all` (GB/s)."""
     import torch
     lib = _bw_lib()
     nbytes = gib << 30
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     b = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
     cus = lib.usct_bw_device_cus()
-    part = torch.empty(cus * 64, dtype=torch.float32, device="cuda")
+    groups = max(cus * 64, lib.usct_bw_pass_groups(nbytes, 1))
+    part = torch.empty(groups, dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
     out = {"buffer_bytes": nbytes, "cus": cus, "reps": reps}
     for kind, name, mult in ((0, "read", 1), (1, "write", 1), (2, "copy", 2)):
-        best = None
-        for bpc in (2, 4, 8, 16):
-            for nt in (0, 1):
-                ms = ctypes.c_float()
-                rc = lib.usct_bw_run(kind, a.data_ptr(), b.data_ptr(), part.data_ptr(), nbytes, bpc,
-                                     nt, reps, ctypes.byref(ms), None)
-                if rc:
-                    raise RuntimeError(f"usct_bw_run({name}) failed: {rc}")
-                gbs = mult * nbytes / (ms.value / 1e3) / 1e9
-                if best is None or gbs > best[0]:
-                    best = (gbs, bpc, nt, ms.value)
-        out[name] = {"GBps": round(best[0], 1), "blocks_per_cu": best[1],
-                     "nontemporal": bool(best[2]), "ms": round(best[3], 4)}
+        best, every = None, {}
+        for layout, params in (("grid-stride", (2, 4, 8, 16)), ("single-pass", (1, 2, 4, 8))):
+            fn = lib.usct_bw_run if layout == "grid-stride" else lib.usct_bw_run_pass
+            for prm in params:
+                for nt in (0, 1):
+                    ms = ctypes.c_float()
+                    rc = fn(kind, a.data_ptr(), b.data_ptr(), part.data_ptr(), nbytes, prm, nt, reps,
+                            ctypes.byref(ms), None)
+                    if rc:
+                        raise RuntimeError(f"usct_bw ({layout}, {name}) failed: {rc}")
+                    gbs = mult * nbytes / (ms.value / 1e3) / 1e9
+                    key = (f"{layout} {'blocks_per_cu' if layout == 'grid-stride' else 'per_lane'} "
+                           f"{prm}{' nt' if nt else ''}")
+                    every[key] = round(gbs, 1)
+                    if best is None or gbs > best[0]:
+                        best = (gbs, layout, prm, nt, ms.value)
+        out[name] = {"GBps": round(best[0], 1), "layout": best[1],
+                     ("blocks_per_cu" if best[1] == "grid-stride" else "per_lane"): best[2],
+                     "nontemporal": bool(best[3]), "ms": round(best[4], 4), "all": every}
     del a, b, part
     torch.cuda.empty_cache()
     return out

@@ -28,6 +28,18 @@ enum { USCT_BW_READ = 0, USCT_BW_WRITE = 1, USCT_BW_COPY = 2 };
 int usct_bw_run(int kind, const void *src, void *dst, float *partials, int64_t bytes,
                 int blocks_per_cu, int nontemporal, int reps, float *ms_out, void *stream);
 
+/*
+ * The same in the single-pass layout: a grid of usct_bw_pass_groups(bytes,
+ * per_lane) work-groups of 256 lanes covers the buffer once, each lane making
+ * per_lane (1, 2, 4 or 8) 16-byte accesses 4 KiB apart and retiring.
+ * partials: that many floats for READ.  Returns as usct_bw_run.
+ */
+int usct_bw_run_pass(int kind, const void *src, void *dst, float *partials, int64_t bytes,
+                     int per_lane, int nontemporal, int reps, float *ms_out, void *stream);
+
+/* Work-groups of the single-pass grid (READ's partials), or -1. */
+int64_t usct_bw_pass_groups(int64_t bytes, int per_lane);
+
 /* Compute units of the current device, or -1. */
 int usct_bw_device_cus(void);
 

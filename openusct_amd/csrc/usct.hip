@@ -1726,18 +1726,22 @@ double time_tile(usct_plan *P, bool adjoint) {
     // the launches write time levels 1 .. lv (traces, source rows): they must exist
     if (lv >= 3 && (P->nt < 4 || P->nrot < 5)) return 1e30;
     const bool x2 = lv == 2 && P->nt >= 3;
+    // The forward pass of a gradient records traces as it goes: time the same
+    // kernel instances (they differ in registers and schedule from the ones that
+    // do not record); the launches write trace row 1, which every forward pass
+    // writes again.
     auto step = [&](int r) {
         Lvl L[kMaxRot];
         for (int i = 0; i < P->nrot; ++i) L[i] = {P->F[(r + i) % P->nrot], P->Lo[(r + i) % P->nrot]};
         if (!adjoint && lv >= 3) {
-            launch_fwd3_batched(P, P->stream, L[1], L[0], &L[2], 1, false);
+            launch_fwd3_batched(P, P->stream, L[1], L[0], &L[2], 1, true);
         } else if (adjoint && lv >= 3) {
             float *out[3] = {L[2].hi, L[3].hi, L[4].hi};
             const float *pf[3] = {P->Wslot[0], P->Wslot[0], P->Wslot[0]};
             launch_adj3_batched(P, P->stream, L[0].hi, L[1].hi, out, pf, 3);
         } else if (!adjoint) {
-            if (x2) launch_fwd2_batched(P, P->stream, L[1], L[0], L[2], L[3], 1, false);
-            else launch_fwd_batched(P, P->stream, L[1], L[0], L[2], 1, false);
+            if (x2) launch_fwd2_batched(P, P->stream, L[1], L[0], L[2], L[3], 1, true);
+            else launch_fwd_batched(P, P->stream, L[1], L[0], L[2], 1, true);
         } else if (P->rev) {
             launch_radj_batched(P, P->stream, P->Wslot[r % 3], P->Wslot[(r + 1) % 3],
                                 P->Wslot[(r + 2) % 3], L[0], L[1], L[2], true, 1);

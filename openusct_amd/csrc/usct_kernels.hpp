@@ -464,24 +464,22 @@ __device__ __forceinline__ bool chunk2d(int wv, int ng, int nxs, int nyc, int sc
     return sg < ng && xs < nxs && yc < nyc;
 }
 
-template <int LB, int CX, bool REC, class L = uint16_t>
-__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
-k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
-           float *__restrict__ pb, const L *__restrict__ pcl,
-           const L *__restrict__ ppl, L *__restrict__ pal,
-           L *__restrict__ pbl, const float2 *__restrict__ sp2,
-           const int *__restrict__ src_cell, const double *__restrict__ sincd_a,
-           const double *__restrict__ sincd_b, const int *__restrict__ rec_head,
-           const int *__restrict__ rec_next, const int *__restrict__ row_rec,
-           float *__restrict__ rec_a, float *__restrict__ rec_lo_a, float *__restrict__ rec_b,
-           float *__restrict__ rec_lo_b, int ny, int nx, int Bp, int RY, int nxs, int nyc,
-           int ng, int sched) {
+// The march direction is a template parameter of the body (the kernel picks
+// the instance per wavefront), so row selections by direction cost nothing.
+template <int LB, int CX, bool REC, class L, int DIR>
+__device__ __forceinline__ void fwd2d_x2_body(
+    const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
+    float *__restrict__ pb, const L *__restrict__ pcl, const L *__restrict__ ppl,
+    L *__restrict__ pal, L *__restrict__ pbl, const float2 *__restrict__ sp2,
+    const int *__restrict__ src_cell, const double *__restrict__ sincd_a,
+    const double *__restrict__ sincd_b, const int *__restrict__ rec_head,
+    const int *__restrict__ rec_next, const int *__restrict__ row_rec, float *__restrict__ rec_a,
+    float *__restrict__ rec_lo_a, float *__restrict__ rec_b, float *__restrict__ rec_lo_b, int ny,
+    int nx, int Bp, int RY, int sg, int xs, int yc) {
     constexpr int W = kWave / LB;
     constexpr int CA = CX + 2;  // A columns x0-1 .. x0+CX
-    const int wv = wave_id();
+    constexpr int dir = DIR;
     const int lane = threadIdx.x & 63;
-    int sg, xs, yc, dir;
-    if (!chunk2d(wv, ng, nxs, nyc, sched, sg, xs, yc, dir)) return;
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -627,6 +625,30 @@ k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__
 #pragma unroll
         for (int j = 0; j < CA + 2; ++j) uc[j] = un[j];
     }
+}
+
+template <int LB, int CX, bool REC, class L = uint16_t>
+__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
+k_fwd2d_x2(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pa,
+           float *__restrict__ pb, const L *__restrict__ pcl,
+           const L *__restrict__ ppl, L *__restrict__ pal,
+           L *__restrict__ pbl, const float2 *__restrict__ sp2,
+           const int *__restrict__ src_cell, const double *__restrict__ sincd_a,
+           const double *__restrict__ sincd_b, const int *__restrict__ rec_head,
+           const int *__restrict__ rec_next, const int *__restrict__ row_rec,
+           float *__restrict__ rec_a, float *__restrict__ rec_lo_a, float *__restrict__ rec_b,
+           float *__restrict__ rec_lo_b, int ny, int nx, int Bp, int RY, int nxs, int nyc,
+           int ng, int sched) {
+    int sg, xs, yc, dir;
+    if (!chunk2d(wave_id(), ng, nxs, nyc, sched, sg, xs, yc, dir)) return;  // wave-uniform
+    if (dir > 0)
+        fwd2d_x2_body<LB, CX, REC, L, 1>(pc, pp, pa, pb, pcl, ppl, pal, pbl, sp2, src_cell,
+                                         sincd_a, sincd_b, rec_head, rec_next, row_rec, rec_a,
+                                         rec_lo_a, rec_b, rec_lo_b, ny, nx, Bp, RY, sg, xs, yc);
+    else
+        fwd2d_x2_body<LB, CX, REC, L, -1>(pc, pp, pa, pb, pcl, ppl, pal, pbl, sp2, src_cell,
+                                          sincd_a, sincd_b, rec_head, rec_next, row_rec, rec_a,
+                                          rec_lo_a, rec_b, rec_lo_b, ny, nx, Bp, RY, sg, xs, yc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1167,22 +1189,20 @@ k_adj2d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 // Row chunks, their order and march direction: sched, as for k_fwd2d_x2 (V
 // trails U by one row in the direction of the march).
 // ---------------------------------------------------------------------------
-template <int LB, int CX>
-__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
-k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lu,
-           float *__restrict__ lv, const float *__restrict__ pa1, const float *__restrict__ pa2,
-           const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
-           const float *__restrict__ sinc_u, const float *__restrict__ sinc_v,
-           const int *__restrict__ rec_head, const int *__restrict__ rec_next,
-           const int *__restrict__ row_rec, const float *__restrict__ res_u,
-           const float *__restrict__ res_v, float *__restrict__ g_hi, float *__restrict__ g_lo,
-           int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng, int sched) {
+template <int LB, int CX, int DIR>
+__device__ __forceinline__ void adj2d_x2_body(
+    const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lu,
+    float *__restrict__ lv, const float *__restrict__ pa1, const float *__restrict__ pa2,
+    const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
+    const float *__restrict__ sinc_u, const float *__restrict__ sinc_v,
+    const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+    const int *__restrict__ row_rec, const float *__restrict__ res_u,
+    const float *__restrict__ res_v, float *__restrict__ g_hi, float *__restrict__ g_lo, int ny,
+    int nx, int Bp, int RY, int sg, int xs, int yc) {
     constexpr int W = kWave / LB;
     constexpr int CA = CX + 2;  // U columns x0-1 .. x0+CX
-    const int wv = wave_id();
+    constexpr int dir = DIR;
     const int lane = threadIdx.x & 63;
-    int sg, xs, yc, dir;
-    if (!chunk2d(wv, ng, nxs, nyc, sched, sg, xs, yc, dir)) return;  // uniform over the wavefront
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -1393,6 +1413,28 @@ k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__
             hr[i] = hn[i];
         }
     }
+}
+
+template <int LB, int CX>
+__global__ void __launch_bounds__(kBlock) USCT_ATTR2D
+k_adj2d_x2(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lu,
+           float *__restrict__ lv, const float *__restrict__ pa1, const float *__restrict__ pa2,
+           const float2 *__restrict__ sp2, const int *__restrict__ src_cell,
+           const float *__restrict__ sinc_u, const float *__restrict__ sinc_v,
+           const int *__restrict__ rec_head, const int *__restrict__ rec_next,
+           const int *__restrict__ row_rec, const float *__restrict__ res_u,
+           const float *__restrict__ res_v, float *__restrict__ g_hi, float *__restrict__ g_lo,
+           int ny, int nx, int Bp, int RY, int nxs, int nyc, int ng, int sched) {
+    int sg, xs, yc, dir;
+    if (!chunk2d(wave_id(), ng, nxs, nyc, sched, sg, xs, yc, dir)) return;  // wave-uniform
+    if (dir > 0)
+        adj2d_x2_body<LB, CX, 1>(l1, l2, lu, lv, pa1, pa2, sp2, src_cell, sinc_u, sinc_v,
+                                 rec_head, rec_next, row_rec, res_u, res_v, g_hi, g_lo, ny, nx,
+                                 Bp, RY, sg, xs, yc);
+    else
+        adj2d_x2_body<LB, CX, -1>(l1, l2, lu, lv, pa1, pa2, sp2, src_cell, sinc_u, sinc_v,
+                                  rec_head, rec_next, row_rec, res_u, res_v, g_hi, g_lo, ny, nx,
+                                  Bp, RY, sg, xs, yc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1703,6 +1745,42 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             zcu[j + 1][CX + 1] = P(pl, j, CX);
         }
     };
+    // The halo of the next plane is also loaded a whole plane ahead, as raw hi /
+    // lo (the cells next to the patch, which neighbouring wavefronts own and
+    // often no longer hold in L2): hh/hq[k] for k = the CX cells of row -1, the
+    // CX of row CY, then rows 0..CY-1 at columns -1 and CX.
+    constexpr int NH = 2 * (CX + CY);
+    float hh[NH];
+    L hq[NH];
+    auto load_halo_raw = [&](size_t pl) {
+        auto one = [&](int k, int j, int i) {
+            const size_t e = pl + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
+            hh[k] = ldb(pc + e, o4);
+            hq[k] = ldb(pcl + e, o2);
+        };
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            one(i, -1, i);
+            one(CX + i, CY, i);
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            one(2 * CX + 2 * j, j, -1);
+            one(2 * CX + 2 * j + 1, j, CX);
+        }
+    };
+    auto halo_from_raw = [&]() {
+#pragma unroll
+        for (int i = 0; i < CX; ++i) {
+            zcu[0][i + 1] = hl(hh[i], hq[i]);
+            zcu[CY + 1][i + 1] = hl(hh[CX + i], hq[CX + i]);
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            zcu[j + 1][0] = hl(hh[2 * CX + 2 * j], hq[2 * CX + 2 * j]);
+            zcu[j + 1][CX + 1] = hl(hh[2 * CX + 2 * j + 1], hq[2 * CX + 2 * j + 1]);
+        }
+    };
     auto load_raw = [&](size_t pl, size_t ppl_, float (&h)[CY][CX], L (&l)[CY][CX],
                         float (&qh_)[CY][CX], L (&ql_)[CY][CX]) {
 #pragma unroll
@@ -1729,6 +1807,7 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             }
         load_halo(p0);
         load_raw((size_t)min(z0 + 1, nz - 1) * planeE, p0, nh, nl, qh, ql);
+        if (z0 + 1 < z1) load_halo_raw((size_t)(z0 + 1) * planeE);
     }
     for (int z = z0; z < z1; ++z) {
         const size_t pnz = (size_t)min(z + 1, nz - 1) * planeE;
@@ -1784,7 +1863,8 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                     zm[j][i] = zcu[j + 1][i + 1];
                     zcu[j + 1][i + 1] = zp[j][i];
                 }
-            load_halo(pnz);
+            halo_from_raw();  // plane z+1, loaded one iteration ago
+            if (z + 2 < z1) load_halo_raw((size_t)(z + 2) * planeE);
         }
     }
 }

@@ -194,16 +194,26 @@ def test_bandwidth_library_exports_every_declared_symbol():
     argument is refused before any HIP call."""
     lib = ctypes.CDLL(os.path.join(ROOT, "openusct_amd", "libusct_bw.so"))
     header = open(os.path.join(ROOT, "include", "usct_bw.h")).read()
-    declared = set(re.findall(r"^int\s+(usct_\w+)\(", header, re.M))
-    assert declared == {"usct_bw_run", "usct_bw_device_cus"}
+    declared = set(re.findall(r"^(?:int|int64_t)\s+(usct_\w+)\(", header, re.M))
+    assert declared == {"usct_bw_run", "usct_bw_run_pass", "usct_bw_pass_groups",
+                        "usct_bw_device_cus"}
     for name in declared:
         assert hasattr(lib, name), name
     ms = ctypes.c_float()
-    lib.usct_bw_run.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]
+    args = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+            ctypes.c_void_p]
+    lib.usct_bw_run.argtypes = args
+    lib.usct_bw_run_pass.argtypes = args
     assert lib.usct_bw_run(2, None, None, None, 1000, 8, 0, 1, ctypes.byref(ms), None) == 1  # % 16
     assert lib.usct_bw_run(7, None, None, None, 1024, 8, 0, 1, ctypes.byref(ms), None) == 1
+    assert lib.usct_bw_run_pass(2, None, None, None, 1000, 1, 0, 1, ctypes.byref(ms), None) == 1
+    assert lib.usct_bw_run_pass(2, None, None, None, 1024, 3, 0, 1, ctypes.byref(ms), None) == 1
+    lib.usct_bw_pass_groups.restype = ctypes.c_int64
+    lib.usct_bw_pass_groups.argtypes = [ctypes.c_int64, ctypes.c_int]
+    # 4 GiB in 16-byte elements, 256 lanes per work-group, one access per lane
+    assert lib.usct_bw_pass_groups(4 << 30, 1) == (4 << 30) // 16 // 256
+    assert lib.usct_bw_pass_groups(4096 * 2 + 16, 2) == 2
 
 
 def test_kernel_min_bytes_model():
