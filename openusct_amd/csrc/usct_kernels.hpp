@@ -158,27 +158,56 @@ __device__ __forceinline__ void neumaier(float &s, float &c, float x) {
     s = t;
 }
 
+// The value of lane (lane ^ OFF) of the wavefront.  On the device: quad
+// permutations of DPP for OFF = 1, 2 (no LDS trip), the bit-mask swizzle for
+// OFF = 4, 8, 16 (within each half of the wavefront) and a permute through the
+// LDS crossbar for OFF = 32; none needs per-call index arithmetic.
+template <int OFF>
+__device__ __forceinline__ float lane_xor(float v, int lane) {
+    static_assert(OFF >= 1 && OFF <= 32 && (OFF & (OFF - 1)) == 0, "pow2 lane offset");
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int x = __float_as_int(v);
+    if constexpr (OFF == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true));
+    else if constexpr (OFF == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true));
+    else if constexpr (OFF < 32) return __int_as_float(__builtin_amdgcn_ds_swizzle(x, (OFF << 10) | 0x1F));
+    else return __int_as_float(__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, x));
+#else
+    (void)lane;
+    return __shfl_xor(v, OFF, kWave);
+#endif
+}
+
 // Reduce CX per-lane values over each aligned group of LB lanes and scatter the
 // CX sums: afterwards lane l holds the sum for cell index sum_cell<LB,CX>(l)
 // (recursive halving: CX/2 + CX/4 + ... + log2(LB/CX) shuffles instead of
-// CX*log2(LB)).  Order is fixed, so results do not depend on scheduling.
+// CX*log2(LB)).  Order is fixed, so results do not depend on scheduling.  The
+// two candidates of each exchange are read as values before the lane picks one
+// (a pick between array elements would index the array at run time).
+template <int LB, int CX, int OFF = LB / 2>
+__device__ __forceinline__ void halve_stages(float (&v)[CX], int lane) {
+    if constexpr (OFF >= LB / CX) {
+        constexpr int half = CX * OFF / LB;  // CX/2 at OFF = LB/2, then halving
+        const bool up = (lane & OFF) != 0;
+#pragma unroll
+        for (int i = 0; i < half; ++i) {
+            const float lo_v = v[i], hi_v = v[i + half];
+            const float send = up ? lo_v : hi_v;
+            const float keep = up ? hi_v : lo_v;
+            v[i] = keep + lane_xor<OFF>(send, lane);
+        }
+        halve_stages<LB, CX, OFF / 2>(v, lane);
+    }
+}
+template <int LB, int OFF>
+__device__ __forceinline__ float sum_stages(float s, int lane) {
+    if constexpr (OFF >= 1) return sum_stages<LB, OFF / 2>(s + lane_xor<OFF>(s, lane), lane);
+    else return s;
+}
 template <int LB, int CX>
 __device__ __forceinline__ float reduce_scatter(float (&v)[CX], int lane) {
     static_assert((LB & (LB - 1)) == 0 && (CX & (CX - 1)) == 0 && CX <= LB, "pow2");
-#pragma unroll
-    for (int half = CX / 2, off = LB / 2; half >= 1; half >>= 1, off >>= 1) {
-        const bool up = (lane & off) != 0;
-#pragma unroll
-        for (int i = 0; i < half; ++i) {
-            const float send = up ? v[i] : v[i + half];
-            const float keep = up ? v[i + half] : v[i];
-            v[i] = keep + __shfl_xor(send, off, kWave);
-        }
-    }
-    float s = v[0];
-#pragma unroll
-    for (int off = LB / (2 * CX); off >= 1; off >>= 1) s = s + __shfl_xor(s, off, kWave);
-    return s;
+    if constexpr (CX > 1) halve_stages<LB, CX>(v, lane);
+    return sum_stages<LB, LB / (2 * CX)>(v[0], lane);
 }
 
 template <int LB, int CX>
