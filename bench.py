@@ -278,6 +278,67 @@ def load_traffic(name, kernel, shots):
     return None if k is None else (k["read_bytes"], k["write_bytes"])
 
 
+def reduce_pmc(fetch_csv, write_csv, out_json, workload_text, calib_bytes):
+    """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE, KiB per dispatch) of the same command started with
+    --calib-copy-mb: each counter is divided by what it reads for the streaming
+    copy kernel (k_bw_copy) against the bytes that kernel moves, and each kernel
+    keeps the median over its dispatches of the most frequent grid size (the
+    tuned tiling; the autotuner's timing launches run on other data)."""
+    import collections
+    import csv
+
+    def per_kernel(path, counter):
+        d = collections.defaultdict(list)
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter:
+                    continue
+                name = r["Kernel_Name"].replace("(anonymous namespace)::", "")
+                name = name.split("(")[0].split("<")[0].replace("void ", "").strip()
+                name = name.split("::")[-1]
+                d[name].append((int(r["Grid_Size"]), float(r["Counter_Value"]) * 1024.0))
+        return d
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    calib = fetch.get("k_bw_copy"), write.get("k_bw_copy")
+    if not calib[0] or not calib[1]:
+        raise SystemExit("no k_bw_copy dispatches: start the profiled command with --calib-copy-mb")
+    moved_bytes = float(calib_bytes)  # the copy reads and writes exactly this much
+    f_scale = float(np.median([v for _, v in calib[0]])) / moved_bytes
+    w_scale = float(np.median([v for _, v in calib[1]])) / moved_bytes
+    out = {"workload": workload_text,
+           "calibration": f"rocprofv3 --pmc on the streaming copy kernel (k_bw_copy, libusct_bw.so) of "
+                          f"{moved_bytes / 2**20:.0f} MiB: FETCH_SIZE reads "
+                          f"{f_scale * moved_bytes / 1024:.1f} KiB and WRITE_SIZE "
+                          f"{w_scale * moved_bytes / 1024:.1f} KiB per dispatch against "
+                          f"{moved_bytes / 1024:.0f} KiB moved each way",
+           "fetch_scale": round(f_scale, 6), "write_scale": round(w_scale, 6), "kernels": {},
+           "note": "bytes per dispatch: the median over the dispatches with the most frequent grid size "
+                   "of each kernel (the tuned tiling; the median leaves out the autotuner's timing "
+                   "launches, which run on other data), each counter divided by its calibration scale; "
+                   "FETCH_SIZE counts the L2's requests to the fabric, so lines served by the Infinity "
+                   "Cache count as fetched"}
+    # The two runs tune their tiles separately and may pick different ones: each
+    # counter takes its own run's most frequent grid for the kernel.
+    for name in sorted(set(fetch) & set(write)):
+        if name == "k_bw_copy" or not name.startswith("k_"):  # runtime copies and fills
+            continue
+        gf = collections.Counter(g for g, _ in fetch[name]).most_common(1)[0][0]
+        gw = collections.Counter(g for g, _ in write[name]).most_common(1)[0][0]
+        fv = [v for g, v in fetch[name] if g == gf]
+        wv = [v for g, v in write[name] if g == gw]
+        out["kernels"][name] = {"grid_size": gf, "dispatches": len(fetch[name]),
+                                "read_bytes": int(round(float(np.median(fv)) / f_scale)),
+                                "write_bytes": int(round(float(np.median(wv)) / w_scale))}
+        if gw != gf:
+            out["kernels"][name]["grid_size_write_run"] = gw
+    with open(out_json, "w") as fh:
+        json.dump(out, fh, indent=1)
+        fh.write("\n")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # one gradient workload
 # ---------------------------------------------------------------------------
@@ -552,12 +613,22 @@ def main():
     ap.add_argument("--gn-products", type=int, default=-1,
                     help="Gauss-Newton Hessian-vector products timed on the same plan "
                          "(default: 3 for the 512^2 workloads, else 0)")
+    ap.add_argument("--reduce-pmc", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "OUT_JSON"),
+                    help="no GPU: reduce two rocprofv3 --pmc counter files (FETCH_SIZE, WRITE_SIZE) "
+                         "of a run started with --calib-copy-mb to per-kernel HBM bytes per launch")
+    ap.add_argument("--pmc-workload", default="", help="workload text stored with --reduce-pmc")
     ap.add_argument("--cpu-child", default="", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-reps", type=int, default=3, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_child:
         print(json.dumps(cpu_child(args.cpu_child, args.cpu_reps, args.cpu_threads)), flush=True)
+        return
+    if args.reduce_pmc:
+        if not args.calib_copy_mb:
+            raise SystemExit("--reduce-pmc needs --calib-copy-mb (the calibration copy's MiB)")
+        out = reduce_pmc(*args.reduce_pmc, args.pmc_workload, args.calib_copy_mb << 20)
+        print(json.dumps({k: v for k, v in out["kernels"].items()}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
