@@ -3429,45 +3429,60 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
             }
         if (z + 1 < z1) load_raw((size_t)min(z + 2, nz - 1) * planeE, pnz);
         const bool zint = (z >= 1) && (z <= nz - 2);
-        float tv[CC];
+        // The arithmetic of every cell of the patch first, with no branch between
+        // cells, so their FP64 chains interleave; then receivers, stores and the
+        // imaging products of the cells inside the grid.  Cells outside the grid
+        // (strips rounded up) compute on clamped rows and are not stored.
+        float vv[CY][CX], pvv[CY][CX];
+        double incv[CY][CX];
+        L lbv[CY][CX];
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
             const int y = y0 + j;
             const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const float2 *spr = sp2 + (size_t)z * planeC + row(j) * nx;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                const bool in = yint && x >= 1 && x <= nx - 2;
+                const float lq = in ? lap7(qc[j + 1][i + 1], qp[j][i], qm[j][i], qc[j + 2][i + 1],
+                                           qc[j][i + 1], qc[j + 1][i + 2], qc[j + 1][i])
+                                    : 0.0f;
+                vv[j][i] = leap(lc[j][i], l2c[j][i], lq);
+                const double lu = in ? lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
+                                             zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i])
+                                     : 0.0;
+                double inc = s2d(ldb(spr + i, oc)) * lu;
+                if ((z * ny + y) * nx + x == my_src) inc = inc + my_inc;
+                incv[j][i] = inc;
+                const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
+                pvv[j][i] = split_hl(nv, lbv[j][i]);
+            }
+        }
+        float tv[CC];
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            const int y = y0 + j;
             const bool has_rec = y < ny && row_rec[z * ny + y] > 0;
             const size_t ro = (size_t)z * planeE + (size_t)y * rowE;
-            const float2 *spr = sp2 + (size_t)z * planeC + (size_t)y * nx;
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
                 const int x = x0 + i;
                 tv[j * CX + i] = 0.0f;
                 if (x < nx && y < ny) {
-                    const bool in = yint && x >= 1 && x <= nx - 2;
-                    const float lq = in ? lap7(qc[j + 1][i + 1], qp[j][i], qm[j][i], qc[j + 2][i + 1],
-                                               qc[j][i + 1], qc[j + 1][i + 2], qc[j + 1][i])
-                                        : 0.0f;
                     const size_t ei = ro + (size_t)i * Bp;
-                    float v = leap(lc[j][i], l2c[j][i], lq);
-                    const int c = (z * ny + y) * nx + x;
+                    float v = vv[j][i];
                     if (has_rec) {
+                        const int c = (z * ny + y) * nx + x;
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r])
                             v = v - res[(size_t)r * Bp + b];
                     }
                     stn<USCT_NTR3D>(lk + ei, o4, v);
-                    double lu = 0.0;
-                    if (in)
-                        lu = lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
-                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
-                    double inc = s2d(ldb(spr + i, oc)) * lu;
-                    if (c == my_src) inc = inc + my_inc;
                     if (wr) {
-                        const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
-                        L lb;
-                        const float pv = split_hl(nv, lb);
-                        stn<USCT_NTR3D>(po + ei, o4, pv);
-                        stn<USCT_NTR3D>(pol + ei, o2, lb);
+                        stn<USCT_NTR3D>(po + ei, o4, pvv[j][i]);
+                        stn<USCT_NTR3D>(pol + ei, o2, lbv[j][i]);
                     }
-                    tv[j * CX + i] = v * (float)inc;
+                    tv[j * CX + i] = v * (float)incv[j][i];
                 }
             }
         }
