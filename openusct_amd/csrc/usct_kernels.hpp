@@ -1850,35 +1850,51 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             }
         if (z + 1 < z1) load_raw((size_t)min(z + 2, nz - 1) * planeE, pnz, nh, nl, qh, ql);
         const bool zint = (z >= 1) && (z <= nz - 2);
+        // The arithmetic of every cell of the patch first, with no branch between
+        // cells, so their FP64 chains interleave; then the stores, source increment
+        // and traces of the cells inside the grid.  Cells outside the grid (strips
+        // rounded up) compute on clamped rows and are not stored.
+        float vv[CY][CX];
+        double incv[CY][CX];
+        L lbv[CY][CX];
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
             const int y = y0 + j;
             const bool yint = zint && (y >= 1) && (y <= ny - 2);
+            const float2 *spr = sp2 + (size_t)z * planeC + row(j) * nx;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                const int x = x0 + i;
+                const double lu = (yint && x >= 1 && x <= nx - 2)
+                                      ? lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
+                                              zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i])
+                                      : 0.0;
+                double inc = s2d(ldb(spr + i, oc)) * lu;
+                if ((z * ny + y) * nx + x == my_src) inc = inc + my_inc;
+                incv[j][i] = inc;
+                const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
+                vv[j][i] = split_hl(nv, lbv[j][i]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < CY; ++j) {
+            const int y = y0 + j;
             const bool has_rec = REC && y < ny && row_rec[z * ny + y] > 0;
             const size_t ro = (size_t)z * planeE + (size_t)y * rowE;
-            const float2 *spr = sp2 + (size_t)z * planeC + (size_t)y * nx;
 #pragma unroll
             for (int i = 0; i < CX; ++i) {
                 const int x = x0 + i;
                 if (x < nx && y < ny) {
-                    double lu = 0.0;
-                    if (yint && x >= 1 && x <= nx - 2)
-                        lu = lap7d(zcu[j + 1][i + 1], zp[j][i], zm[j][i], zcu[j + 2][i + 1],
-                                   zcu[j][i + 1], zcu[j + 1][i + 2], zcu[j + 1][i]);
-                    double inc = s2d(ldb(spr + i, oc)) * lu;
                     const int c = (z * ny + y) * nx + x;
-                    if (c == my_src) inc = inc + my_inc;
                     const size_t ei = ro + (size_t)i * Bp;
-                    const double nv = (2.0 * zcu[j + 1][i + 1] - q[j][i]) + inc;
-                    L lb;
-                    const float v = split_hl(nv, lb);
+                    const float v = vv[j][i];
                     stn<USCT_NTF3D>(pn + ei, o4, v);
-                    stn<USCT_NTF3D>(pnl + ei, o2, lb);
-                    if (c == my_src) srcd2[b] = (float)inc;
+                    stn<USCT_NTF3D>(pnl + ei, o2, lbv[j][i]);
+                    if (c == my_src) srcd2[b] = (float)incv[j][i];
                     if (has_rec) {
                         for (int r = rec_head[c]; r >= 0; r = rec_next[r]) {
                             rec[(size_t)r * Bp + b] = v;
-                            rec_lo[(size_t)r * Bp + b] = lo_f(lb);
+                            rec_lo[(size_t)r * Bp + b] = lo_f(lbv[j][i]);
                         }
                     }
                 }
