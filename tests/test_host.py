@@ -4,6 +4,7 @@ vectors, index handling, argument checks, and the C-ABI library's exports."""
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 import re
 
@@ -392,3 +393,39 @@ def test_other_wavelets_phantoms_and_shell_arrays_match_reference():
     assert np.array_equal(np.array([ix for idxs, _ in fps for ix in idxs]), f["shell_fp_cells"])
     assert np.array_equal(np.array([len(idxs) for idxs, _ in fps]), f["shell_fp_counts"])
     assert np.array_equal(np.concatenate([w for _, w in fps]), f["shell_fp_w"])
+
+
+def test_reduce_pmc_scales_counters_and_keeps_each_runs_tuned_grid(tmp_path):
+    """bench.py --reduce-pmc: each counter is divided by what it reads for the
+    calibration copy (FETCH_SIZE 0.5, WRITE_SIZE 1.0 of the bytes moved here), each
+    kernel keeps the median over its run's most frequent grid (the two runs may tune
+    to different grids), and runtime copies and fills are left out."""
+    import csv
+    import bench
+    cols = ["Dispatch_Id", "Grid_Size", "Kernel_Name", "Counter_Name", "Counter_Value"]
+
+    def write(path, counter, rows):
+        with open(path, "w", newline="") as fh:
+            w = csv.writer(fh)
+            w.writerow(cols)
+            for i, (grid, name, kib) in enumerate(rows):
+                w.writerow([i, grid, name, counter, kib])
+
+    mib = 1 << 20
+    fetch, wr = tmp_path / "f.csv", tmp_path / "w.csv"
+    write(fetch, "FETCH_SIZE", [(4096, "k_bw_copy", 0.5 * 1024 * 1024),
+                                (10, "k_adj2d_x2", 300.0), (10, "k_adj2d_x2", 310.0),
+                                (20, "k_adj2d_x2", 999.0), (10, "k_adj2d_x2", 305.0),
+                                (64, "__amd_rocclr_copyBuffer", 8.0)])
+    write(wr, "WRITE_SIZE", [(4096, "k_bw_copy", 1.0 * 1024 * 1024),
+                             (20, "k_adj2d_x2", 100.0), (20, "k_adj2d_x2", 110.0),
+                             (20, "k_adj2d_x2", 90.0), (10, "k_adj2d_x2", 999.0),
+                             (64, "__amd_rocclr_copyBuffer", 8.0)])
+    out = bench.reduce_pmc(str(fetch), str(wr), str(tmp_path / "o.json"), "test", 1024 * mib)
+    assert out["fetch_scale"] == 0.5 and out["write_scale"] == 1.0
+    k = out["kernels"]
+    assert set(k) == {"k_adj2d_x2"}
+    assert k["k_adj2d_x2"]["grid_size"] == 10 and k["k_adj2d_x2"]["grid_size_write_run"] == 20
+    assert k["k_adj2d_x2"]["read_bytes"] == 305 * 1024 * 2
+    assert k["k_adj2d_x2"]["write_bytes"] == 100 * 1024
+    assert json.load(open(tmp_path / "o.json"))["kernels"] == k
