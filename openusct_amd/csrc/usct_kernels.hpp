@@ -1774,10 +1774,12 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             zcu[j + 1][CX + 1] = P(pl, j, CX);
         }
     };
-    // The halo of the next plane is also loaded a whole plane ahead, as raw hi /
-    // lo (the cells next to the patch, which neighbouring wavefronts own and
-    // often no longer hold in L2): hh/hq[k] for k = the CX cells of row -1, the
-    // CX of row CY, then rows 0..CY-1 at columns -1 and CX.
+    // The halo of the next plane (the cells next to the patch, which neighbouring
+    // wavefronts own) is also loaded a whole plane ahead, as raw hi / lo: a
+    // plane of work covers its latency, and its lines are requested while the
+    // owners still hold them in L2 (on the 256^3, 64-shot gradient the reads
+    // fell from 19.0 to 15.9 bytes per shot-cell).  hh/hq[k] for k = the CX
+    // cells of row -1, the CX of row CY, then rows 0..CY-1 at columns -1 and CX.
     constexpr int NH = 2 * (CX + CY);
     float hh[NH];
     L hq[NH];
