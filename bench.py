@@ -297,6 +297,7 @@ def touch_device_memory(gib=140):
     torch.cuda.empty_cache()
     print(f"[bench] device memory touched: {n / 2**30:.0f} GiB in {time.perf_counter() - t0:.2f} s",
           flush=True)
+    return n
 
 
 def reduce_pmc(fetch_csv, write_csv, out_json, workload_text, calib_bytes):
@@ -677,8 +678,7 @@ def main():
             "roofline": None, "cpu_baseline": None}), flush=True)
         return
 
-    if not args.no_touch:
-        touch_device_memory()
+    touched = 0 if args.no_touch else touch_device_memory()
     head, kept = gradient_leg(args.workload, args.steps, args.warmup, args, keep=True)
     prob, plan, mt, md, wd, dobs, g, rec, tx = kept
 
@@ -808,6 +808,7 @@ def main():
         "roofline": head.get("roofline"),
         "c4": c4,
         "stream_ceiling": ceiling,
+        "device_memory_touched_bytes": touched,
         "cpu_baseline": cpu,
         "serving": sv,
         "gradient_descent": gd,
