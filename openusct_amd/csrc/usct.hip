@@ -1621,7 +1621,9 @@ void default_rows(const usct_plan *P, Tiling &T) {
     const int64_t cols = P->ndim == 2 ? ng * nxs : ng * nxs * cdiv(P->ny, T.CY);
     const int64_t depth = P->ndim == 2 ? P->ny : P->nz;
     const int64_t target = 256 * 4 * 8;
-    int r = 64;
+    // 3D: at most 16 planes per wavefront (the autotuner's picks on 256^3 are 8
+    // and 16; 64-plane chunks measured 9-13 % slower launches)
+    int r = P->ndim == 3 ? 16 : 64;
     while (r > 4 && cols * cdiv(depth, r) < target) r >>= 1;
     T.RY = r;
 }
