@@ -102,7 +102,10 @@ typedef struct {
     int fwd_tile_x, fwd_tile_y;
     int fwd_rows_per_wave;
     int tile_kind;            /* adjoint, 3D: 0 = cells per lane (tile_x x tile_y),
-                                 1 = LDS-staged work-group tile of tile_x x tile_y cells */
+                                 1 = LDS-staged work-group tile of tile_x x tile_y cells,
+                                 4 = cells per lane in work-groups of four strips stacked
+                                 along y that pass their edge rows through LDS (plans with
+                                 reverse = 1) */
     int fwd_tile_kind;        /* the same for the forward kernel; 2 = LDS-staged tiles
                                  advancing two time levels per launch; 3 = cells per lane
                                  advancing two time levels per launch (z marched in

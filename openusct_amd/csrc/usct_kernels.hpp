@@ -3307,8 +3307,18 @@ k_adj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__res
 //   po/pol  level k-2 (written when wr != 0; the caller passes wr = 0 for
 //           k <= 2, whose levels 0 and -1 are zero by definition)
 //   sincd   row k of the FP64 source increments.  Addressing as k_fwd3d.
+// YSH: the kYsh wavefronts of a work-group take strips stacked along y (one x
+// strip, z chunk and shot group: work-group t -> shot group t % ng, then x
+// strip, then block of kYsh y strips, then z chunk) and march together; at each
+// plane they hand the edge rows of their centre plane (S' lam^{k+1} and level
+// k-1 in FP64) to each other through LDS, one barrier per plane, so only the
+// group's outer y halo rows are loaded from memory.  The values are the ones
+// the loads would give (the same cells, the same operations): bit-identical to
+// YSH = false.  Strips past the grid's last y strip compute on clamped rows and
+// store nothing, so that every wavefront of the group reaches every barrier.
 // ---------------------------------------------------------------------------
-template <int LB, int CX, int CY, class L = uint16_t>
+constexpr int kYsh = kBlock / kWave;
+template <int LB, int CX, int CY, class L = uint16_t, bool YSH = false>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__restrict__ lk,
          const float *__restrict__ pc, const L *__restrict__ pcl,
@@ -3321,11 +3331,24 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
          int nzc, int ng, int blk) {
     constexpr int W = kWave / LB;
     constexpr int CC = CX * CY;
-    const int wv = wave_id();
     const int lane = threadIdx.x & 63;
-    const int sg = wv % ng;
-    int xs, ys, zc;
-    if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;  // uniform over the wavefront
+    int sg, xs, ys, zc;
+    const int wy = YSH ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+    if constexpr (YSH) {
+        int t = xcd_remap(blockIdx.x, gridDim.x);
+        sg = t % ng;
+        t /= ng;
+        xs = t % nxs;
+        t /= nxs;
+        const int nyb = (nys + kYsh - 1) / kYsh;
+        ys = (t % nyb) * kYsh + wy;
+        zc = t / nyb;
+        if (zc >= nzc) return;  // uniform over the work-group
+    } else {
+        const int wv = wave_id();
+        sg = wv % ng;
+        if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;  // uniform over the wavefront
+    }
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -3371,6 +3394,58 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
             zcu[j + 1][CX + 1] = P(pl, j, CX);
         }
     };
+    // YSH: edge rows [plane parity][wavefront][top, bottom][cell][lane]
+    constexpr int kEdge = YSH ? 2 * kYsh * 2 * CX * 64 : 1;
+    __shared__ float eq[kEdge];
+    __shared__ double ez[kEdge];
+    int par = 0;
+    auto halo = [&](const float *lz, const float2 *sz, size_t pl) {
+        if constexpr (!YSH) {
+            load_halo(lz, sz, pl);
+        } else {
+#pragma unroll
+            for (int j = 0; j < CY; ++j) {
+                qc[j + 1][0] = Q(lz, sz, j, -1);
+                qc[j + 1][CX + 1] = Q(lz, sz, j, CX);
+                zcu[j + 1][0] = P(pl, j, -1);
+                zcu[j + 1][CX + 1] = P(pl, j, CX);
+            }
+            if (wy == 0)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    qc[0][i + 1] = Q(lz, sz, -1, i);
+                    zcu[0][i + 1] = P(pl, -1, i);
+                }
+            if (wy == kYsh - 1)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    qc[CY + 1][i + 1] = Q(lz, sz, CY, i);
+                    zcu[CY + 1][i + 1] = P(pl, CY, i);
+                }
+            const int e0 = par * (kYsh * 2 * CX * 64) + lane;
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                eq[e0 + ((wy * 2 + 0) * CX + i) * 64] = qc[1][i + 1];
+                eq[e0 + ((wy * 2 + 1) * CX + i) * 64] = qc[CY][i + 1];
+                ez[e0 + ((wy * 2 + 0) * CX + i) * 64] = zcu[1][i + 1];
+                ez[e0 + ((wy * 2 + 1) * CX + i) * 64] = zcu[CY][i + 1];
+            }
+            __syncthreads();
+            if (wy > 0)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    qc[0][i + 1] = eq[e0 + (((wy - 1) * 2 + 1) * CX + i) * 64];
+                    zcu[0][i + 1] = ez[e0 + (((wy - 1) * 2 + 1) * CX + i) * 64];
+                }
+            if (wy < kYsh - 1)
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    qc[CY + 1][i + 1] = eq[e0 + (((wy + 1) * 2 + 0) * CX + i) * 64];
+                    zcu[CY + 1][i + 1] = ez[e0 + (((wy + 1) * 2 + 0) * CX + i) * 64];
+                }
+            par ^= 1;
+        }
+    };
     // raw loads issued one plane ahead: own cells of lam^{k+1} and of level
     // k-1 on plane z+1 (an, nh/nl), of lam^{k+2} and of level k on plane z
     // (l2n, kh/kl)
@@ -3407,7 +3482,7 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
                 qc[j + 1][i + 1] = smul(ldb(s0 + row(j) * nx + i, oc), v);
                 zcu[j + 1][i + 1] = P(p0, j, i);
             }
-        load_halo(l0, s0, p0);
+        halo(l0, s0, p0);
         load_raw((size_t)min(z0 + 1, nz - 1) * planeE, p0);
     }
     // the lane's reduced cell (the same on every plane) and its gradient
@@ -3522,7 +3597,7 @@ k_radj3d(const float *__restrict__ l1, const float *__restrict__ l2, float *__re
                     zm[j][i] = zcu[j + 1][i + 1];
                     zcu[j + 1][i + 1] = zp[j][i];
                 }
-            load_halo(lnz, snz, pnz);
+            halo(lnz, snz, pnz);
         }
     }
 }

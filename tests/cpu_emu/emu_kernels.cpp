@@ -676,14 +676,14 @@ bool check_adj3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
 // FP64 leapfrog with p^k in the place of p^{n-2}), the imaging sums of
 // lam^k * (float)inc against a direct FP64 sum; with wr = 0 level k-2 is left
 // as it was.
-template <int LB, int CX, int CY, class L = uint16_t>
+template <int LB, int CX, int CY, class L = uint16_t, bool YSH = false>
 bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed) {
     const int Bp = LB * ng, W = 64 / LB;
     const int64_t N = (int64_t)nz * ny * nx, FE = N * Bp;
     std::mt19937 rng(seed);
     char tag[160];
-    std::snprintf(tag, sizeof tag, sizeof(L) == 4 ? "radj3d-lo32 nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d" : "radj3d nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d", nz,
-                  ny, nx, LB, ng, CX, CY, RZ, wr);
+    std::snprintf(tag, sizeof tag, sizeof(L) == 4 ? "radj3d-lo32 nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d ysh=%d" : "radj3d nz=%d ny=%d nx=%d LB=%d ng=%d C=%dx%d RZ=%d wr=%d ysh=%d", nz,
+                  ny, nx, LB, ng, CX, CY, RZ, wr, (int)YSH);
     std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
         dl(-1e-7f, 1e-7f);
     float2 *sp2 = alloc<float2>(N, 1);
@@ -723,11 +723,13 @@ bool check_radj3d(int nz, int ny, int nx, int ng, int RZ, int wr, uint32_t seed)
     float *gh = alloc<float>(ng * N, 0), *gl = alloc<float>(ng * N, 0);
     const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
     const int nzc = (int)cdiv(nz, RZ);
-    const unsigned blocks = (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
+    // YSH: one work-group per shot group, x strip, block of kYsh y strips and z chunk
+    const unsigned blocks = YSH ? (unsigned)(ng * nxs * cdiv(nys, kYsh) * nzc)
+                                : (unsigned)cdiv(ng * tasks3d(nxs, nys, nzc, g_blk), 4);
     emu::launch(blocks, kBlock, [&] {
-        k_radj3d<LB, CX, CY, L>(l1, l2, lk, pc, pcl, pk, pkl, po, pol, wr, sp2, src, sincd, rec_head,
-                                rec_next, row_rec, res, gh, gl, nz, ny, nx, Bp, RZ, nxs, nys, nzc,
-                                ng, g_blk);
+        k_radj3d<LB, CX, CY, L, YSH>(l1, l2, lk, pc, pcl, pk, pkl, po, pol, wr, sp2, src, sincd,
+                                     rec_head, rec_next, row_rec, res, gh, gl, nz, ny, nx, Bp, RZ,
+                                     nxs, nys, nzc, ng, g_blk);
     });
     bool ok = true;
     const int64_t sz = (int64_t)nx * ny;
@@ -873,13 +875,20 @@ bool born3d_cases(uint32_t seed) {
     return ok;
 }
 
-template <int LB, int CX, int CY>
+template <int LB, int CX, int CY, bool YSH = false>
 bool radj3d_cases(uint32_t seed) {
     bool ok = true;
-    ok &= check_radj3d<LB, CX, CY>(9, 7, 11, 2, 4, 1, seed++);
-    ok &= check_radj3d<LB, CX, CY>(5, 6, 13, 1, 64, 1, seed++);
-    ok &= check_radj3d<LB, CX, CY>(12, 9, 5, 2, 8, 0, seed++);
-    ok &= check_radj3d<LB, CX, CY>(3, 3, 3, 1, 2, 1, seed++);
+    ok &= check_radj3d<LB, CX, CY, uint16_t, YSH>(9, 7, 11, 2, 4, 1, seed++);
+    ok &= check_radj3d<LB, CX, CY, uint16_t, YSH>(5, 6, 13, 1, 64, 1, seed++);
+    ok &= check_radj3d<LB, CX, CY, uint16_t, YSH>(12, 9, 5, 2, 8, 0, seed++);
+    ok &= check_radj3d<LB, CX, CY, uint16_t, YSH>(3, 3, 3, 1, 2, 1, seed++);
+    if (YSH) {
+        // y extents that leave the last block of y strips part-empty, and more
+        // than one block of strips along y
+        ok &= check_radj3d<LB, CX, CY, uint16_t, YSH>(6, 21, 7, 1, 3, 1, seed++);
+        ok &= check_radj3d<LB, CX, CY, uint16_t, YSH>(7, 18, 9, 2, 5, 1, seed++);
+        ok &= check_radj3d<LB, CX, CY, float, YSH>(8, 17, 6, 1, 4, 1, seed++);
+    }
     return ok;
 }
 
@@ -1583,6 +1592,14 @@ int main(int argc, char **argv) {
         ok &= born3d_cases<16, 1, 2>(seed + 8200);
         ok &= born3d_cases<64, 1, 1>(seed + 8300);
         ok &= born3d_cases<32, 2, 2>(seed + 8400);
+    }
+    if (only.empty() || only == "radj3d_ysh") {
+        ok &= radj3d_cases<64, 2, 2, true>(seed + 7700);
+        ok &= radj3d_cases<64, 2, 1, true>(seed + 7800);
+        ok &= radj3d_cases<32, 2, 2, true>(seed + 7900);
+        ok &= radj3d_cases<32, 2, 1, true>(seed + 8000);
+        ok &= radj3d_cases<16, 2, 2, true>(seed + 8100);
+        ok &= radj3d_cases<16, 2, 1, true>(seed + 8200);
     }
     if (only.empty() || only == "radj3d") {
         ok &= radj3d_cases<16, 2, 2>(seed + 7000);

@@ -115,6 +115,34 @@ def test_reversing_passes_graph_and_eager():
             assert torch.equal(Je, Jg) and torch.equal(ge, gg), rep
 
 
+@pytest.mark.parametrize("name", ["fmc_3d.npz", "fmc_3d_fine.npz"])
+def test_reversing_work_groups_sharing_edge_rows(name, monkeypatch):
+    """Tile kind 4: the reversing adjoint in work-groups of four strips stacked
+    along y that pass their edge rows through LDS (forced through the autotuner's
+    USCT_TUNE_KIND).  Against the golden vectors, the default reversing plan
+    (other tiles: the gradient's shot sums run in another order, so <= 1e-6) and,
+    bit for bit, a second call of the same plan."""
+    from openusct_amd.plan import Plan
+    f, shape, h, dt, nt, tx, rec, m0, mt, wd = _golden_plan_inputs(name)
+    dobs = torch.tensor(f["dobs"], device=m0.device)
+    with Plan(shape, h, dt, nt, tx, rec, reverse=True, autotune=False) as dp:
+        Jd, gd = dp.misfit_and_gradient(m0, wd, dobs)
+    monkeypatch.setenv("USCT_TUNE_KIND", "4")
+    with Plan(shape, h, dt, nt, tx, rec, reverse=True, autotune=True) as yp:
+        info = yp.info
+        print(f"[case] reversing, edge rows through LDS, {name} {info}", flush=True)
+        assert info["tile_kind"] == 4 and info["reverse"] == 1
+        J, g = yp.misfit_and_gradient(m0, wd, dobs)
+        Jr, gr = yp.misfit_and_gradient(m0, wd, dobs)
+    monkeypatch.delenv("USCT_TUNE_KIND")
+    g_np = g.cpu().numpy()
+    e = (abs(J.item() - float(f["J"])) / float(f["J"]), rel_l2(g_np, f["g"]),
+         rel_l2(g_np, gd.cpu().numpy()))
+    print(f"J {e[0]:.2e} gradient vs golden {e[1]:.2e} vs default tiles {e[2]:.2e}")
+    assert torch.equal(J, Jd) and torch.equal(J, Jr) and torch.equal(g, gr)
+    assert e[0] < TOL and e[1] < TOL and e[2] < 1e-6
+
+
 def test_reversing_c4_reduced_vs_oracle():
     """C4 spacing, time step and array on a 64^3 grid, 300 steps, 4 shots."""
     from oracle import oracle as O
