@@ -97,6 +97,31 @@ def test_strip_blocks_visit_every_strip_once():
                                      for z in range(nzc)], (nxs, nys, nzc, bx, by)
 
 
+def _ysh_strips(ng, nxs, nys, nzc, waves=4):
+    """(sg, xs, ys, zc) of every wavefront of the reversing adjoint's tile kind 4,
+    as k_radj3d<..., YSH = true> maps work-group t and wavefront w."""
+    nyb = _cdiv(nys, waves)
+    out = []
+    for t in range(ng * nxs * nyb * nzc):
+        sg, t2 = t % ng, t // ng
+        xs, t2 = t2 % nxs, t2 // nxs
+        for w in range(waves):
+            out.append((sg, xs, (t2 % nyb) * waves + w, t2 // nyb))
+    return out
+
+
+def test_y_stacked_work_groups_visit_every_strip_once():
+    """Tile kind 4: every strip of every shot group and z chunk once; the strips
+    past the last y strip (a part-empty last block) read clamped rows only."""
+    for ng, nxs, nys, nzc in ((1, 7, 5, 3), (2, 3, 8, 2), (3, 1, 9, 4), (1, 128, 128, 32)):
+        got = _ysh_strips(ng, nxs, nys, nzc)
+        valid = sorted(v for v in got if v[2] < nys)
+        assert valid == [(g, x, y, z) for g in range(ng) for x in range(nxs) for y in range(nys)
+                         for z in range(nzc)], (ng, nxs, nys, nzc)
+        assert all(v[3] < nzc for v in got)
+        assert len(got) - len(valid) == ng * nxs * nzc * (_cdiv(nys, 4) * 4 - nys)
+
+
 def _extent_3d(nx, ny, nz, Bp, LB, CX, CY, adjoint, RZ=4, halo=1, blk=(1, 2)):
     W = 64 // LB
     ng, nxs, nys, nzc = Bp // LB, _cdiv(nx, W * CX), _cdiv(ny, CY), _cdiv(nz, RZ)
