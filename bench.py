@@ -268,13 +268,16 @@ def kernel_min_bytes(kernel, N, shots, groups):
     return rd, wr
 
 
-def load_traffic(name, kernel, shots):
+def load_traffic(name, kernel, shots, tile_kind=0):
+    """Counted (read, write) bytes per launch of `kernel` from the workload's traffic
+    file: the entry for the plan's tile kind where it is not 0 (none: no figure)."""
     f = TRAFFIC_FILES.get(name)
     path = os.path.join(ROOT, "profiles", f) if f else None
     if not path or shots != FULL_SHOTS.get(name) or not os.path.exists(path):
         return None
     with open(path) as fh:
-        k = json.load(fh)["kernels"].get(kernel)
+        d = json.load(fh)
+    k = (d.get("kernels_tile_kind", {}).get(str(tile_kind), {}) if tile_kind else d["kernels"]).get(kernel)
     return None if k is None else (k["read_bytes"], k["write_bytes"])
 
 
@@ -300,13 +303,17 @@ def touch_device_memory(gib=140):
     return n
 
 
-def reduce_pmc(fetch_csv, write_csv, out_json, workload_text, calib_bytes):
+def reduce_pmc(fetch_csv, write_csv, out_json, workload_text, calib_bytes, tile_kind=0,
+               only=None):
     """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, then
     WRITE_SIZE, KiB per dispatch) of the same command started with
     --calib-copy-mb: each counter is divided by what it reads for the streaming
     copy kernel (k_bw_copy) against the bytes that kernel moves, and each kernel
     keeps the median over its dispatches of the most frequent grid size (the
-    tuned tiling; the autotuner's timing launches run on other data)."""
+    tuned tiling; the autotuner's timing launches run on other data).  With
+    tile_kind != 0 (runs whose plan was restricted to that tile kind) the kernels
+    named in `only` go into out_json's "kernels_tile_kind" entry for that kind, and
+    the rest of out_json is kept."""
     import collections
     import csv
 
@@ -355,6 +362,14 @@ def reduce_pmc(fetch_csv, write_csv, out_json, workload_text, calib_bytes):
                                 "write_bytes": int(round(float(np.median(wv)) / w_scale))}
         if gw != gf:
             out["kernels"][name]["grid_size_write_run"] = gw
+    if tile_kind:
+        with open(out_json) as fh:
+            base = json.load(fh)
+        base.setdefault("kernels_tile_kind", {})[str(tile_kind)] = {
+            k: v for k, v in out["kernels"].items() if only is None or k in only}
+        base["note_tile_kind"] = ("kernels_tile_kind: the same counters for plans whose tuner was "
+                                  "restricted to that tile kind (USCT_TUNE_KIND), same calibration")
+        out = base
     with open(out_json, "w") as fh:
         json.dump(out, fh, indent=1)
         fh.write("\n")
@@ -447,10 +462,10 @@ def roofline(name, prof, info, N, shots, shape, value):
     t_adj = prof["adj_ms"] * prof["n_adj"] / lva
     t_fwd = prof["fwd_ms"] * prof["n_fwd"] / lvf
 
-    def one(kernel, ms, lv, survey_bytes):
+    def one(kernel, ms, lv, survey_bytes, tile_kind):
         rd, wr = kernel_min_bytes(kernel, N, per_pass, groups)
         ach = (rd + wr) / (ms / 1e3) / 1e9
-        tr = load_traffic(name, kernel, shots)
+        tr = load_traffic(name, kernel, shots, tile_kind)
         d = {"name": kernel, "per_launch_ms": round(ms, 4), "levels_per_launch": lv,
              "min_read_bytes": rd, "min_write_bytes": wr, "achieved": round(ach, 1),
              "frac": round(ach / HBM_PEAK_GBS, 4),
@@ -463,8 +478,8 @@ def roofline(name, prof, info, N, shots, shape, value):
                               "GBps": round(survey_bytes * N * per_pass * lv / (ms / 1e3) / 1e9, 1)}}
         return d
 
-    a = one(adj_k, prof["adj_ms"], lva, BYTES_ADJ)
-    f = one(fwd_k, prof["fwd_ms"], lvf, BYTES_FWD)
+    a = one(adj_k, prof["adj_ms"], lva, BYTES_ADJ, int(info.get("tile_kind", 0)))
+    f = one(fwd_k, prof["fwd_ms"], lvf, BYTES_FWD, int(info.get("fwd_tile_kind", 0)))
     dom, other = (a, f) if t_adj >= t_fwd else (f, a)
     what = {"k_adj2d_x2": "adjoint + imaging, two time levels per launch",
             "k_adj2d": "adjoint + imaging, one time level per launch",
@@ -641,6 +656,10 @@ def main():
                     help="no GPU: reduce two rocprofv3 --pmc counter files (FETCH_SIZE, WRITE_SIZE) "
                          "of a run started with --calib-copy-mb to per-kernel HBM bytes per launch")
     ap.add_argument("--pmc-workload", default="", help="workload text stored with --reduce-pmc")
+    ap.add_argument("--pmc-tile-kind", type=int, default=0,
+                    help="--reduce-pmc of runs restricted to this tile kind: add the kernels named "
+                         "by --pmc-kernels to OUT_JSON's entry for the kind")
+    ap.add_argument("--pmc-kernels", default="", help="comma-separated kernel names")
     ap.add_argument("--cpu-child", default="", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-reps", type=int, default=3, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-threads", type=int, default=1, help=argparse.SUPPRESS)
@@ -651,7 +670,8 @@ def main():
     if args.reduce_pmc:
         if not args.calib_copy_mb:
             raise SystemExit("--reduce-pmc needs --calib-copy-mb (the calibration copy's MiB)")
-        out = reduce_pmc(*args.reduce_pmc, args.pmc_workload, args.calib_copy_mb << 20)
+        out = reduce_pmc(*args.reduce_pmc, args.pmc_workload, args.calib_copy_mb << 20,
+                         args.pmc_tile_kind, args.pmc_kernels.split(",") if args.pmc_kernels else None)
         print(json.dumps({k: v for k, v in out["kernels"].items()}), flush=True)
         return
     if args.gpus != 1:

@@ -429,3 +429,9 @@ def test_reduce_pmc_scales_counters_and_keeps_each_runs_tuned_grid(tmp_path):
     assert k["k_adj2d_x2"]["read_bytes"] == 305 * 1024 * 2
     assert k["k_adj2d_x2"]["write_bytes"] == 100 * 1024
     assert json.load(open(tmp_path / "o.json"))["kernels"] == k
+    # runs restricted to one tile kind add that kind's entry and keep the rest
+    out4 = bench.reduce_pmc(str(fetch), str(wr), str(tmp_path / "o.json"), "test", 1024 * mib,
+                            tile_kind=4, only=["k_adj2d_x2", "k_other"])
+    saved = json.load(open(tmp_path / "o.json"))
+    assert saved["kernels"] == k and out4["kernels"] == k
+    assert saved["kernels_tile_kind"]["4"] == {"k_adj2d_x2": k["k_adj2d_x2"]}
