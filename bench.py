@@ -87,11 +87,11 @@ def serving(n_requests, nt_scale, rate=150.0, seed=5, with_3d=True):
     fill_observed(reqs)
     print(f"[bench] serving: {n_requests} requests, observed data ready in "
           f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    step = max(1, n_requests // 10)
+    t_serve = time.perf_counter()
 
     def progress(k, n):
-        if k % step == 0 or k == n:
-            print(f"[bench] serving: {k}/{n} done", file=sys.stderr, flush=True)
+        print(f"[bench] serving: {k}/{n} done at {time.perf_counter() - t_serve:.1f} s",
+              file=sys.stderr, flush=True)
 
     res = Server().run(reqs, progress=progress)
     s = summarize(res)

@@ -1803,6 +1803,7 @@ struct TuneKey {
 };
 struct TuneVal { Tiling tf, ta; };
 std::map<TuneKey, TuneVal> g_tuned;
+std::mutex g_tuned_mu;  // plans may be created on several host threads
 
 int autotune(usct_plan *P) {
     // The timing launches advance level 1 (two-level candidates also level 2) and
@@ -1817,11 +1818,14 @@ int autotune(usct_plan *P) {
     // choice is neither taken from nor kept in the cache
     const char *only_kind = std::getenv("USCT_TUNE_KIND");
     const bool restricted = only_kind && *only_kind;
-    auto hit = g_tuned.find(key);
-    if (!restricted && hit != g_tuned.end()) {
-        P->tf = hit->second.tf;
-        P->ta = hit->second.ta;
-        return 0;
+    if (!restricted) {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        auto hit = g_tuned.find(key);
+        if (hit != g_tuned.end()) {
+            P->tf = hit->second.tf;
+            P->ta = hit->second.ta;
+            return 0;
+        }
     }
     // zero the fields used by the timing launches
     for (int i = 0; i < P->nrot; ++i) {
@@ -1991,7 +1995,10 @@ int autotune(usct_plan *P) {
             T = bt;
         }
     }
-    if (!restricted) g_tuned[key] = TuneVal{P->tf, P->ta};
+    if (!restricted) {
+        std::lock_guard<std::mutex> lk(g_tuned_mu);
+        g_tuned[key] = TuneVal{P->tf, P->ta};
+    }
     return 0;
 }
 

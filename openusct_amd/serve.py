@@ -8,7 +8,11 @@ starvation guard that stops overtaking once the oldest request has waited
 Every admitted request gets its own plan (device-resident model, wavefields,
 checkpoints and traces) and its own HIP stream, so the device overlaps the
 kernels of all requests in flight.  Each request is batched across its own
-shots.
+shots.  A request's plan set-up, input copies and kernel launches run on a
+worker thread of its own: a long request enqueues thousands of launches, and
+the HIP runtime blocks the enqueuing thread while the device's queues are
+full, which in the loop's own thread held back admission and completion of
+every other request.
 
 Results do not depend on what else is in flight: a request's arithmetic is
 fixed by its own inputs (tile shapes are chosen by rule, not by timing), and
@@ -24,6 +28,7 @@ when the loop ends.
 from __future__ import annotations
 
 import collections
+import concurrent.futures
 import time
 from dataclasses import dataclass, field
 
@@ -74,6 +79,7 @@ class Result:
     done: float
     updates: int
     workspace_bytes: int = 0
+    enqueue_s: float = 0.0    # host seconds of plan set-up, copies and launches
 
     @property
     def latency(self):
@@ -83,14 +89,16 @@ class Result:
 @dataclass
 class _Flight:
     req: Request
-    plan: Plan
-    stream: torch.cuda.Stream
-    event: torch.cuda.Event
+    held: int                 # bytes reserved at admission
     admitted: float
+    future: concurrent.futures.Future | None = None
+    plan: Plan | None = None
+    event: torch.cuda.Event | None = None
     J: torch.Tensor | None = None
     grad: torch.Tensor | None = None
     data: torch.Tensor | None = None
     tensors: list = field(default_factory=list)
+    enqueue_s: float = 0.0
 
 
 class Server:
@@ -108,9 +116,12 @@ class Server:
         self._held = 0            # bytes of plans in flight
 
     # -- memory -----------------------------------------------------------------
-    def _fits(self, req, in_flight):
+    def _need(self, req):
         # kept blocks are 2 MiB multiples, up to 1.25x the buffer they serve
-        need = int(req.footprint(self.cap) * 1.25) + (64 << 20)
+        return int(req.footprint(self.cap) * 1.25) + (64 << 20)
+
+    def _fits(self, req, in_flight):
+        need = self._need(req)
         if self._held + need <= self.budget:
             return True
         if need > self.budget:
@@ -118,85 +129,119 @@ class Server:
         return False
 
     # -- execution ----------------------------------------------------------------
-    def _launch(self, req, now):
+    def _launch(self, req, now, pool):
+        """Reserve the request's bytes and hand its set-up and launches to a worker."""
+        if req.kind not in ("gradient", "forward"):
+            raise ValueError(f"unknown request kind {req.kind!r}")
+        fl = _Flight(req, self._need(req), now)
+        self._held += fl.held
+        fl.future = pool.submit(self._enqueue, fl)
+        return fl
+
+    def _enqueue(self, fl):
+        """Worker thread: plan, input copies and the request's launches on its own
+        stream, then an event that completes with them."""
+        t = time.perf_counter()
+        req = fl.req
+        torch.cuda.set_device(self.device)
         stream = torch.cuda.Stream(device=self.device)
         est = req.footprint(self.cap)
-        # the plan's own planner must stay within what admission accounted for
-        plan = Plan(req.shape, req.h, req.dt, req.nt, req.tx, req.rec, method="batched",
-                    mem_budget=int(est * 1.05) + (64 << 20), autotune=False,
-                    graph=self.use_graph, forward_only=req.kind == "forward")
-        nbytes = max(int(plan.info["workspace_bytes"]), int(est * 1.25))
-        self._held += nbytes
-        fl = _Flight(req, plan, stream, torch.cuda.Event(), now)
-        fl.tensors.append(nbytes)
+        # the plan's own planner stays within what admission reserved
+        fl.plan = Plan(req.shape, req.h, req.dt, req.nt, req.tx, req.rec, method="batched",
+                       mem_budget=int(est * 1.05) + (64 << 20), autotune=False,
+                       graph=self.use_graph, forward_only=req.kind == "forward")
         with torch.cuda.stream(stream):
             m = torch.as_tensor(req.m, dtype=torch.float64).to(self.device, non_blocking=False)
             w = torch.as_tensor(req.wavelet, dtype=torch.float64).to(self.device)
             if req.kind == "gradient":
                 d = torch.as_tensor(req.dobs, dtype=torch.float64).to(self.device)
-                fl.J, fl.grad = plan.misfit_and_gradient(m, w, d, stream=stream)
+                fl.J, fl.grad = fl.plan.misfit_and_gradient(m, w, d, stream=stream)
                 fl.tensors += [m, w, d]
-            elif req.kind == "forward":
-                fl.data = plan.forward_fmc(m, w, stream=stream)
-                fl.tensors += [m, w]
             else:
-                raise ValueError(f"unknown request kind {req.kind!r}")
-            fl.event.record(stream)
-        return fl
+                fl.data = fl.plan.forward_fmc(m, w, stream=stream)
+                fl.tensors += [m, w]
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        fl.tensors.append(stream)
+        fl.event = ev
+        fl.enqueue_s = time.perf_counter() - t
 
     def run(self, requests, on_result=None, progress=None):
         """Serve ``requests`` (arrival times relative to now); returns results by rid."""
         device_cache(self.budget)
         try:
-            return self._run(requests, on_result, progress)
+            with concurrent.futures.ThreadPoolExecutor(max_workers=self.max_inflight) as pool:
+                return self._run(requests, on_result, progress, pool)
         finally:
             device_cache(0)
 
-    def _run(self, requests, on_result, progress):
+    @staticmethod
+    def _finished(fl):
+        """True once the worker has enqueued everything and the device has run it;
+        a worker's exception is raised here."""
+        if not fl.future.done():
+            return False
+        fl.future.result()
+        return fl.event.query()
+
+    def _run(self, requests, on_result, progress, pool):
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         queue, flights, results = [], [], {}
         t0 = time.perf_counter()
         n_total = len(pending)
-        while pending or queue or flights:
-            now = time.perf_counter() - t0
-            while pending and pending[0].arrival <= now:
-                queue.append(pending.popleft())
-            # admission: first fit in arrival order; no overtaking a request that has
-            # waited longer than max_wait_s
-            i = 0
-            while i < len(queue) and len(flights) < self.max_inflight:
-                r = queue[i]
-                if self._fits(r, flights):
-                    queue.pop(i)
-                    flights.append(self._launch(r, time.perf_counter() - t0))
-                    continue
-                if now - queue[0].arrival > self.max_wait_s:
-                    break
-                i += 1
-            # completion
-            still = []
+        try:
+            while pending or queue or flights:
+                now = time.perf_counter() - t0
+                while pending and pending[0].arrival <= now:
+                    queue.append(pending.popleft())
+                # admission: first fit in arrival order; no overtaking a request that
+                # has waited longer than max_wait_s
+                i = 0
+                while i < len(queue) and len(flights) < self.max_inflight:
+                    r = queue[i]
+                    if self._fits(r, flights):
+                        queue.pop(i)
+                        flights.append(self._launch(r, time.perf_counter() - t0, pool))
+                        continue
+                    if now - queue[0].arrival > self.max_wait_s:
+                        break
+                    i += 1
+                # completion
+                still = []
+                for fl in flights:
+                    if self._finished(fl):
+                        done = time.perf_counter() - t0
+                        r = fl.req
+                        res = Result(r.rid, r.kind, None if fl.J is None else float(fl.J.item()),
+                                     fl.grad, fl.data, r.arrival, fl.admitted, done, r.updates,
+                                     int(fl.plan.info["workspace_bytes"]), fl.enqueue_s)
+                        results[r.rid] = res
+                        fl.plan.close()  # its blocks are kept for the next plans
+                        self._held -= fl.held
+                        if on_result:
+                            on_result(res)
+                        if progress:
+                            progress(len(results), n_total)
+                    else:
+                        still.append(fl)
+                flights = still
+                if not flights and not queue:
+                    if pending:
+                        time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0),
+                                                0.01)))
+                elif not any(f.future.done() and f.event.query() for f in flights):
+                    time.sleep(2e-4)
+        finally:
+            # a failed request leaves the others' workers to finish before the pool
+            # closes; their plans are closed here
             for fl in flights:
-                if fl.event.query():
-                    done = time.perf_counter() - t0
-                    r = fl.req
-                    res = Result(r.rid, r.kind, None if fl.J is None else float(fl.J.item()),
-                                 fl.grad, fl.data, r.arrival, fl.admitted, done, r.updates,
-                                 fl.tensors[0])
-                    results[r.rid] = res
-                    fl.plan.close()  # its blocks are kept for the next plans
-                    self._held -= fl.tensors[0]
-                    if on_result:
-                        on_result(res)
-                    if progress:
-                        progress(len(results), n_total)
-                else:
-                    still.append(fl)
-            flights = still
-            if not flights and not queue:
-                if pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - (time.perf_counter() - t0), 0.01)))
-            elif not any(f.event.query() for f in flights):
-                time.sleep(2e-4)
+                try:
+                    fl.future.result()
+                except Exception:
+                    pass
+                if fl.plan is not None:
+                    torch.cuda.synchronize(self.device)
+                    fl.plan.close()
         return results
 
 
@@ -207,10 +252,11 @@ def summarize(results):
     t_first = min(r.arrival for r in res)
     t_last = max(r.done for r in res)
     upd = sum(r.updates for r in res)
+    enq = np.array([r.enqueue_s for r in res])
     return {"requests": len(res), "p50_s": float(np.percentile(lat, 50)),
             "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()),
             "makespan_s": t_last - t_first, "gpt_updates_per_s": upd / (t_last - t_first) / 1e9,
-            "updates": upd}
+            "updates": upd, "enqueue_s_mean": float(enq.mean()), "enqueue_s_max": float(enq.max())}
 
 
 def make_mix(n_requests=500, seed=5, nt_scale=1.0, shots=(16, 256), grids=None, rate=None,
