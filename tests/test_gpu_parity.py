@@ -1,7 +1,7 @@
 """GPU parity: the HIP path against the reference's golden vectors and the FP64 oracle.
 
-Tolerance (the north-star bar for FP32 fields vs the FP64 reference): relative
-L2 <= 1e-5 for traces, misfits and gradients.  Between the device's own
+Tolerance (the bar for FP32 fields against the FP64 reference): relative L2
+<= 1e-5 for traces, misfits and gradients.  Between the device's own
 methods (plain / batched, graph / eager, checkpointed / whole history) forward
 traces must agree bit for bit.
 """
