@@ -142,6 +142,31 @@ def test_server_admission_under_tight_budget():
             assert torch.equal(a.data, b.data)
 
 
+def test_server_request_that_fails_raises_and_the_loop_recovers():
+    """A request whose plan cannot be built (a source cell outside the grid) fails
+    on its worker thread: run() raises the plan's ValueError after the other
+    requests in flight have finished and their plans are closed, and a new loop
+    then serves the remaining requests with the results they have alone."""
+    import copy
+    from openusct_amd.plan import device_cache
+    from openusct_amd.serve import Server
+    reqs = _small_mix(n=4, seed=14)
+    bad = copy.copy(reqs[1])
+    bad.rid = 99
+    bad.tx = np.array([int(np.prod(bad.shape)) + 5], np.int64)
+    print("[case] serving: one request with a source outside its grid", flush=True)
+    with pytest.raises(ValueError):
+        Server(max_inflight=4).run([reqs[0], bad, reqs[2]])
+    assert device_cache(0) == 0
+    again = Server(max_inflight=4).run(reqs)
+    alone = Server(max_inflight=1).run([reqs[0]])
+    a, b = again[reqs[0].rid], alone[reqs[0].rid]
+    if reqs[0].kind == "gradient":
+        assert a.J == b.J and torch.equal(a.grad, b.grad)
+    else:
+        assert torch.equal(a.data, b.data)
+
+
 def test_device_cache_keeps_and_reuses_plan_memory():
     """usct_device_cache: a destroyed plan's blocks are kept (up to the cap), the
     next plan of the same shape takes them instead of new memory, results do not
