@@ -109,7 +109,7 @@ typedef struct {
     int fwd_tile_kind;        /* the same for the forward kernel; 2 = LDS-staged tiles
                                  advancing two time levels per launch; 3 = cells per lane
                                  advancing two time levels per launch (z marched in
-                                 registers) */
+                                 registers); 4 = as for the adjoint (64-lane groups) */
     int reverse;              /* 1: the gradient keeps no forward levels; the adjoint sweep
                                  runs the forward backwards (3D, no sponge, point elements) */
     int sched;                /* 2D two-level adjoint kernel: row-chunk order and march

@@ -86,8 +86,9 @@ struct Tiling {
                          // lane from lds_patch), rows = planes per work-group; 2 = the same
                          // tiles advancing two forward levels per launch; 3 = CX x CY lane
                          // patches advancing two forward levels per launch (k_fwd3d_x2);
-                         // 4 (reversing adjoint) = lane patches in work-groups of kYsh
-                         // strips stacked along y that share their edge rows (blk = 1 x kYsh)
+                         // 4 (reversing adjoint, 3D forward) = lane patches in work-groups
+                         // of kYsh strips stacked along y that share their edge rows
+                         // (blk = 1 x kYsh)
     int sched = 0;       // 2D two-level kernels: bit 0 = odd row chunks march upwards,
                          // bit 1 = row chunk fastest in the wavefront order, bit 2 = shot
                          // group slowest
@@ -431,16 +432,25 @@ void fwd3d_t(dim3 g, hipStream_t s, Lvl pc, Lvl pp, Lvl pn, const usct_plan *P,
              const float *sinc, float *rec, float *rl, const Tile &t) {
     float *d2 = P->srcd2 + (sinc - P->sinc);
     const double *sd = P->sincd + (sinc - P->sinc);
-    auto go = [&](auto lo_type) {
+    auto go = [&](auto lo_type, auto ysh) {
         using L = decltype(lo_type);
-        hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC, L>), g, dim3(kBlock), 0, s, pc.hi, pp.hi,
-                           pn.hi, (const L *)pc.lo, (const L *)pp.lo, (L *)pn.lo, P->sp2,
-                           P->src_cell, sd, P->rec_head, P->rec_next, P->row_rec, rec, rl, d2,
-                           P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys, t.nyc,
+        hipLaunchKernelGGL((k_fwd3d<LB, CX, CY, REC, L, decltype(ysh)::value>), g, dim3(kBlock), 0,
+                           s, pc.hi, pp.hi, pn.hi, (const L *)pc.lo, (const L *)pp.lo, (L *)pn.lo,
+                           P->sp2, P->src_cell, sd, P->rec_head, P->rec_next, P->row_rec, rec, rl,
+                           d2, P->nz, P->ny, P->nx, P->Bp, P->tf.RY, t.nxs, t.nys, t.nyc,
                            P->Bp / P->tf.LB, P->tf.blk);
     };
-    if (P->lo32) go(float{});
-    else go(uint16_t{});
+    // tile kind 4: work-groups of y-stacked strips sharing their edge rows (built
+    // for 64-lane groups with 2 x 2 patches)
+    if constexpr (LB == 64 && CX == 2 && CY == 2) {
+        if (P->tf.lds == 4) {
+            if (P->lo32) go(float{}, std::true_type{});
+            else go(uint16_t{}, std::true_type{});
+            return;
+        }
+    }
+    if (P->lo32) go(float{}, std::false_type{});
+    else go(uint16_t{}, std::false_type{});
 }
 
 template <int LB, bool REC>
@@ -1920,6 +1930,10 @@ int autotune(usct_plan *P) {
                 for (auto tl : tiles)
                     for (int rz : {8, 16, 32, 64}) cands.push_back({lb, tl.first, tl.second, rz, 1});
             }
+            // forward: work-groups of four y-stacked strips sharing their edge rows
+            // (tile kind 4, 64-lane groups, 2 x 2 patches)
+            if (!adjoint && std::find(lbs.begin(), lbs.end(), 64) != lbs.end())
+                for (int rz : {8, 16, 32}) cands.push_back({64, 2, 2, rz, 4, 0, (1 << 8) | kYshF});
             // two forward levels per launch on 8 x 8 tiles of 16 shots
             if (!adjoint && P->Bp % 16 == 0 && fusable(P))
                 for (int rz : {8, 16, 32, 64}) cands.push_back({16, 8, 8, rz, 2});
@@ -1933,7 +1947,7 @@ int autotune(usct_plan *P) {
         }
         if (P->lo32 && !adjoint)  // the LDS and two-level 3D forwards take bfloat16 low parts
             cands.erase(std::remove_if(cands.begin(), cands.end(),
-                                       [](const Tiling &c) { return c.lds != 0; }),
+                                       [](const Tiling &c) { return c.lds != 0 && c.lds != 4; }),
                         cands.end());
         if (restricted) {
             const int k = std::atoi(only_kind);

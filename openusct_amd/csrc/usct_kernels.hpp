@@ -1717,9 +1717,15 @@ k_adj2d_xn(const float *__restrict__ l1, const float *__restrict__ l2, const Adj
 // hi + lo state); the own cells of the next plane are loaded one plane ahead.
 // srcd2[b] records the increment applied at shot b's source cell for the
 // adjoint's imaging.
+// YSH (64-lane groups): the work-group's kYsh wavefronts take strips stacked
+// along y and hand each other the edge rows of the next plane through LDS at
+// each plane (one barrier per plane), as k_radj3d's YSH; only the group's outer
+// y halo rows and the x halo are loaded a plane ahead.  Bit-identical to YSH =
+// false (the same cells, the same operations).
 // ---------------------------------------------------------------------------
 
-template <int LB, int CX, int CY, bool REC, class L = uint16_t>
+constexpr int kYshF = kBlock / kWave;
+template <int LB, int CX, int CY, bool REC, class L = uint16_t, bool YSH = false>
 __global__ void __launch_bounds__(kBlock) USCT_ATTR3D
 k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__restrict__ pn,
         const L *__restrict__ pcl, const L *__restrict__ ppl,
@@ -1730,11 +1736,25 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
         float *__restrict__ srcd2, int nz, int ny, int nx, int Bp, int RZ, int nxs, int nys,
         int nzc, int ng, int blk) {
     constexpr int W = kWave / LB;
-    const int wv = wave_id();
+    static_assert(!YSH || LB == kWave, "YSH: one strip per wavefront");
     const int lane = threadIdx.x & 63;
-    const int sg = wv % ng;
-    int xs, ys, zc;
-    if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;
+    int sg, xs, ys, zc;
+    const int wy = YSH ? __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) : 0;
+    if constexpr (YSH) {
+        int t = xcd_remap(blockIdx.x, gridDim.x);
+        sg = t % ng;
+        t /= ng;
+        xs = t % nxs;
+        t /= nxs;
+        const int nyb = (nys + kYshF - 1) / kYshF;
+        ys = (t % nyb) * kYshF + wy;
+        zc = t / nyb;
+        if (zc >= nzc) return;  // uniform over the work-group
+    } else {
+        const int wv = wave_id();
+        sg = wv % ng;
+        if (!strip3d(wv / ng, nxs, nys, nzc, blk, xs, ys, zc)) return;
+    }
     const int sb = lane % LB, st = lane / LB;
     const int b = sg * LB + sb;
     const int x0 = (xs * W + st) * CX;
@@ -1783,16 +1803,32 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
     constexpr int NH = 2 * (CX + CY);
     float hh[NH];
     L hq[NH];
+    float yoh[YSH ? CX : 1];
+    L yoq[YSH ? CX : 1];
     auto load_halo_raw = [&](size_t pl) {
         auto one = [&](int k, int j, int i) {
             const size_t e = pl + row(j) * rowE + (size_t)((ptrdiff_t)i * Bp);
             hh[k] = ldb(pc + e, o4);
             hq[k] = ldb(pcl + e, o2);
         };
+        if constexpr (YSH) {
+            // the group's outer y halo row (row -1 above the first wavefront, row
+            // CY below the last); selected by value, so nothing is indexed by wy
+            if (wy == 0 || wy == kYshF - 1) {
+                const int jo = wy == 0 ? -1 : CY;
 #pragma unroll
-        for (int i = 0; i < CX; ++i) {
-            one(i, -1, i);
-            one(CX + i, CY, i);
+                for (int i = 0; i < CX; ++i) {
+                    const size_t e = pl + row(jo) * rowE + (size_t)((ptrdiff_t)i * Bp);
+                    yoh[i] = ldb(pc + e, o4);
+                    yoq[i] = ldb(pcl + e, o2);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                one(i, -1, i);
+                one(CX + i, CY, i);
+            }
         }
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
@@ -1800,17 +1836,50 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
             one(2 * CX + 2 * j + 1, j, CX);
         }
     };
-    auto halo_from_raw = [&]() {
+    // YSH: edge rows [plane parity][wavefront][top, bottom][cell][lane] of the
+    // plane the patch moves to
+    constexpr int kEdge = YSH ? 2 * kYshF * 2 * CX * 64 : 1;
+    __shared__ double ez[kEdge];
+    int par = 0;
+    auto share_rows = [&]() {
+        const int e0 = par * (kYshF * 2 * CX * 64) + lane;
 #pragma unroll
         for (int i = 0; i < CX; ++i) {
-            zcu[0][i + 1] = hl(hh[i], hq[i]);
-            zcu[CY + 1][i + 1] = hl(hh[CX + i], hq[CX + i]);
+            ez[e0 + ((wy * 2 + 0) * CX + i) * 64] = zcu[1][i + 1];
+            ez[e0 + ((wy * 2 + 1) * CX + i) * 64] = zcu[CY][i + 1];
+        }
+        __syncthreads();
+        if (wy > 0)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) zcu[0][i + 1] = ez[e0 + (((wy - 1) * 2 + 1) * CX + i) * 64];
+        if (wy < kYshF - 1)
+#pragma unroll
+            for (int i = 0; i < CX; ++i) zcu[CY + 1][i + 1] = ez[e0 + (((wy + 1) * 2 + 0) * CX + i) * 64];
+        par ^= 1;
+    };
+    auto halo_from_raw = [&]() {
+        if constexpr (YSH) {
+            if (wy == 0 || wy == kYshF - 1) {
+#pragma unroll
+                for (int i = 0; i < CX; ++i) {
+                    const double v = hl(yoh[i], yoq[i]);
+                    zcu[0][i + 1] = wy == 0 ? v : zcu[0][i + 1];
+                    zcu[CY + 1][i + 1] = wy == 0 ? zcu[CY + 1][i + 1] : v;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < CX; ++i) {
+                zcu[0][i + 1] = hl(hh[i], hq[i]);
+                zcu[CY + 1][i + 1] = hl(hh[CX + i], hq[CX + i]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < CY; ++j) {
             zcu[j + 1][0] = hl(hh[2 * CX + 2 * j], hq[2 * CX + 2 * j]);
             zcu[j + 1][CX + 1] = hl(hh[2 * CX + 2 * j + 1], hq[2 * CX + 2 * j + 1]);
         }
+        if constexpr (YSH) share_rows();
     };
     auto load_raw = [&](size_t pl, size_t ppl_, float (&h)[CY][CX], L (&l)[CY][CX],
                         float (&qh_)[CY][CX], L (&ql_)[CY][CX]) {
@@ -1837,6 +1906,7 @@ k_fwd3d(const float *__restrict__ pc, const float *__restrict__ pp, float *__res
                 zcu[j + 1][i + 1] = P(p0, j, i);
             }
         load_halo(p0);
+        if constexpr (YSH) share_rows();  // the loaded inner y halo rows are overwritten
         load_raw((size_t)min(z0 + 1, nz - 1) * planeE, p0, nh, nl, qh, ql);
         if (z0 + 1 < z1) load_halo_raw((size_t)(z0 + 1) * planeE);
     }

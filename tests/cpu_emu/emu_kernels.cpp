@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -479,7 +480,8 @@ bool xn_cases(uint32_t seed, bool all) {
 // 3D forward: k_fwd3d against a direct loop over cells of the same FP64 update
 // (full hi + lo state, FP64 S' and source term, split into FP32 hi + bf16 lo).
 // KIND 0: k_fwd3d with CX x CY patches per lane; KIND 1: k_fwd3d_lds with a
-// TX x TY tile per work-group (CX, CY = TX, TY) of PX x PY patches per lane.
+// TX x TY tile per work-group (CX, CY = TX, TY) of PX x PY patches per lane;
+// KIND 2: k_fwd3d in work-groups of y-stacked strips sharing edge rows (YSH).
 template <int LB, int CX, int CY, int KIND = 0, int PX = 1, int PY = 1, class L = uint16_t>
 bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     const int Bp = LB * ng, W = 64 / LB;
@@ -487,7 +489,9 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
     std::mt19937 rng(seed);
     char tag[160];
     std::snprintf(tag, sizeof tag, "%s nz=%d ny=%d nx=%d LB=%d ng=%d C/T=%dx%d P=%dx%d RZ=%d",
-                  KIND ? "fwd3d-lds" : (sizeof(L) == 4 ? "fwd3d-lo32" : "fwd3d"), nz, ny, nx, LB, ng, CX, CY, PX, PY, RZ);
+                  KIND == 2 ? (sizeof(L) == 4 ? "fwd3d-ysh-lo32" : "fwd3d-ysh")
+                            : (KIND ? "fwd3d-lds" : (sizeof(L) == 4 ? "fwd3d-lo32" : "fwd3d")),
+                  nz, ny, nx, LB, ng, CX, CY, PX, PY, RZ);
     std::uniform_real_distribution<float> u(0.02f, 0.1f), tt(-1e-9f, 1e-9f), d(-1.0f, 1.0f),
         dl(-1e-7f, 1e-7f);
     float2 *sp2 = alloc<float2>(N, 1);
@@ -527,6 +531,15 @@ bool check_fwd3d(int nz, int ny, int nx, int ng, int RZ, uint32_t seed) {
             k_fwd3d<LB, CX, CY, true, L>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd, rec_head,
                                       rec_next, row_rec, rec, rec_lo, srcd2, nz, ny, nx, Bp, RZ,
                                       nxs, nys, nzc, ng, g_blk);
+        });
+    } else if constexpr (KIND == 2) {
+        const int nxs = (int)cdiv(nx, (int64_t)W * CX), nys = (int)cdiv(ny, CY);
+        const int nzc = (int)cdiv(nz, RZ);
+        const unsigned blocks = (unsigned)(ng * nxs * cdiv(nys, kYshF) * nzc);
+        emu::launch(blocks, kBlock, [&] {
+            k_fwd3d<LB, CX, CY, true, L, true>(pc, pp, pn, pcl, ppl, pnl, sp2, src, sincd,
+                                               rec_head, rec_next, row_rec, rec, rec_lo, srcd2,
+                                               nz, ny, nx, Bp, RZ, nxs, nys, nzc, ng, g_blk);
         });
     } else {
         const int ntx = (int)cdiv(nx, CX), nty = (int)cdiv(ny, CY), nzc = (int)cdiv(nz, RZ);
@@ -1592,6 +1605,15 @@ int main(int argc, char **argv) {
         ok &= born3d_cases<16, 1, 2>(seed + 8200);
         ok &= born3d_cases<64, 1, 1>(seed + 8300);
         ok &= born3d_cases<32, 2, 2>(seed + 8400);
+    }
+    if (only.empty() || only == "fwd3d_ysh") {
+        uint32_t sd = seed + 8300;
+        for (auto c : {std::array<int, 5>{9, 7, 11, 2, 4}, {5, 6, 13, 1, 64}, {12, 9, 5, 2, 8},
+                       {3, 3, 3, 1, 2}, {6, 21, 7, 1, 3}, {7, 18, 9, 2, 5}}) {
+            ok &= check_fwd3d<64, 2, 2, 2>(c[0], c[1], c[2], c[3], c[4], sd++);
+            ok &= check_fwd3d<64, 2, 1, 2>(c[0], c[1], c[2], c[3], c[4], sd++);
+        }
+        ok &= check_fwd3d<64, 2, 2, 2, 1, 1, float>(8, 17, 6, 1, 4, sd++);
     }
     if (only.empty() || only == "radj3d_ysh") {
         ok &= radj3d_cases<64, 2, 2, true>(seed + 7700);

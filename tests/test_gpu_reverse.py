@@ -143,6 +143,30 @@ def test_reversing_work_groups_sharing_edge_rows(name, monkeypatch):
     assert e[0] < TOL and e[1] < TOL and e[2] < 1e-6
 
 
+def test_forward_work_groups_sharing_edge_rows(monkeypatch):
+    """Tile kind 4 for the 3D forward (64-lane groups): 64 transmitters on the
+    golden 3D problem's grid, traces bit for bit against the default tiles, the
+    reversing gradient (both kernels of kind 4) within 1e-6 of the default plan's."""
+    from openusct_amd.plan import Plan
+    f, shape, h, dt, nt, _, rec, m0, mt, wd = _golden_plan_inputs("fmc_3d_fine.npz")
+    tx = np.array([rec[(7 * i) % len(rec)] for i in range(64)], np.int64)
+    with Plan(shape, h, dt, nt, tx, rec, reverse=True, autotune=False) as dp:
+        d0 = dp.forward_fmc(mt, wd)
+        J0, g0 = dp.misfit_and_gradient(m0, wd, d0 * 1.01)
+    monkeypatch.setenv("USCT_TUNE_KIND", "4")
+    with Plan(shape, h, dt, nt, tx, rec, reverse=True, autotune=True) as yp:
+        info = yp.info
+        print(f"[case] forward and reversing adjoint, edge rows through LDS {info}", flush=True)
+        assert info["fwd_tile_kind"] == 4 and info["tile_kind"] == 4
+        d4 = yp.forward_fmc(mt, wd)
+        J4, g4 = yp.misfit_and_gradient(m0, wd, d0 * 1.01)
+    monkeypatch.delenv("USCT_TUNE_KIND")
+    e = rel_l2(g4.cpu().numpy(), g0.cpu().numpy())
+    print(f"gradient vs default tiles {e:.2e}")
+    assert torch.equal(d4, d0) and torch.equal(J4, J0)
+    assert e < 1e-6
+
+
 def test_reversing_c4_reduced_vs_oracle():
     """C4 spacing, time step and array on a 64^3 grid, 300 steps, 4 shots."""
     from oracle import oracle as O

@@ -60,7 +60,8 @@ def emu_binary(tmp_path_factory):
 
 @pytest.mark.parametrize("seed", [1, 2])
 @pytest.mark.parametrize("which", ["fwd", "adj", "fwd3d", "lds", "born", "x2-3d", "x2p-3d", "born-x2",
-                                   "gsot", "radj3d", "radj3d_ysh", "born3d", "lo32", "blk3d", "xn"])
+                                   "gsot", "radj3d", "radj3d_ysh", "fwd3d_ysh", "born3d", "lo32", "blk3d",
+                                   "xn"])
 def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1")
     r = subprocess.run([emu_binary, str(seed), which], capture_output=True, text=True,
@@ -70,5 +71,5 @@ def test_two_level_kernels_equal_single_level(emu_binary, which, seed):
     assert "ALL EQUAL" in r.stdout, tail
     assert r.stdout.count("ok ") == {"fwd": 495, "adj": 485, "fwd3d": 50, "lds": 12,
                                      "born": 5, "x2-3d": 6, "x2p-3d": 10, "born-x2": 52, "gsot": 13,
-                                     "radj3d": 28, "radj3d_ysh": 42, "born3d": 20, "lo32": 24,
+                                     "radj3d": 28, "radj3d_ysh": 42, "fwd3d_ysh": 13, "born3d": 20, "lo32": 24,
                                      "blk3d": 120, "xn": 524}[which], tail
