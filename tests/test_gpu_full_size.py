@@ -65,6 +65,39 @@ def test_c4_full_reversing_vs_checkpointed():
     assert e[0] < 1e-9 and e[1] < 1e-6
 
 
+def test_c4_full_edge_rows_through_lds(monkeypatch):
+    """C4 at full size with both 3D kernels of tile kind 4 (work-groups of four
+    y-stacked strips passing edge rows through LDS) against the default tiles:
+    traces and misfit bit for bit, gradient within 1e-6."""
+    from openusct_amd import configs
+    from openusct_amd.plan import Plan, linear_indices
+    prob = configs.c4()
+    rec = linear_indices(prob.geom.idx, prob.shape)
+    tx = rec[prob.src_list]
+    mt, m0, wd = _tensors(prob)
+    t0 = time.time()
+    with Plan(prob.shape, prob.h, prob.dt, prob.nt, tx, rec, reverse=True, autotune=False) as dp:
+        print(f"[case] C4 default tiles {dp.info}", flush=True)
+        dobs = dp.forward_fmc(mt, wd)
+        J0, g0 = dp.misfit_and_gradient(m0, wd, dobs * 1.01)
+        g0 = g0.cpu().numpy()
+    torch.cuda.empty_cache()
+    monkeypatch.setenv("USCT_TUNE_KIND", "4")
+    with Plan(prob.shape, prob.h, prob.dt, prob.nt, tx, rec, reverse=True, autotune=True) as yp:
+        info = yp.info
+        print(f"[case] C4 tile kind 4 {info}", flush=True)
+        assert info["tile_kind"] == 4 and info["fwd_tile_kind"] == 4
+        d4 = yp.forward_fmc(mt, wd)
+        J4, g4 = yp.misfit_and_gradient(m0, wd, dobs * 1.01)
+        g4 = g4.cpu().numpy()
+    monkeypatch.delenv("USCT_TUNE_KIND")
+    e = rel_l2(g4, g0)
+    print(f"C4 kind 4: traces equal {torch.equal(d4, dobs)}, gradient {e:.2e}, "
+          f"bitwise {np.array_equal(g4, g0)} ({time.time() - t0:.0f} s)")
+    assert torch.equal(d4, dobs) and torch.equal(J4, J0)
+    assert e < 1e-6
+
+
 def test_c3_full_one_pass_vs_multi_pass():
     from openusct_amd import configs
     from openusct_amd.plan import Plan, linear_indices
